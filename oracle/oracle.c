@@ -1,0 +1,820 @@
+/* TEST INFRASTRUCTURE — the CPU oracle for the plain flat TurboQuant encode +
+ * search path. It is the checker, never the thing measured or shipped: only
+ * tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load it.
+ *
+ * A plain-C restatement of the reference's algorithm. Each function cites the
+ * reference file:line it follows (paths relative to the reference root).
+ * Built with -ffp-contract=off: every fused multiply-add that the reference's
+ * compiled code performs (it builds with -mfma -ffp-contract=fast) is written
+ * here as an explicit fma()/fmaf() call, so the semantics are fixed in the
+ * source, not by a compiler's contraction choices.
+ *
+ * Pinning (what checks each part, see tests/test_oracle.py):
+ *   - everything is compared bit-for-bit with the reference's own compiled
+ *     functions (oracle/_ref, built from the reference sources) on seeded
+ *     inputs and against the committed golden vectors in tests/golden/;
+ *   - independent of the reference: ChaCha20 block function vs RFC 7539 §2.3.2
+ *     (the 8-round variant shares the code path), Hadamard involution, norm
+ *     preservation, codebook symmetry/monotonicity, brute-force order
+ *     statistics, brute-force LUT sums and top-k by full sort.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---- constants (reference include/vectorcache/constants.hpp:7-28) ------ */
+#define OC_BLOCK 192
+#define OC_CHUNK 32
+#define OC_FLUSH_EVERY 256
+#define OC_MAX_DIM 16384
+static const float kMinInputNorm = 1e-10f;
+static const double kDegenerateInnerEps = 1e-12;
+
+void oc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+/* ---- ChaCha8 keystream (reference src/chacha8.cpp:12-85) ----------------
+ * IETF layout: 4 constant words, 8 key words (little-endian seed), a 64-bit
+ * block counter in words 12..13, zero nonce in 14..15; 8 rounds = 4 double
+ * rounds; output = permuted + input; counter increments per block. */
+typedef struct {
+  uint32_t in[16];
+  uint32_t out[16];
+  int pos;
+} oc_chacha;
+
+static uint32_t rotl(uint32_t v, int s) { return (v << s) | (v >> (32 - s)); }
+
+#define QR(a, b, c, d)   \
+  do {                   \
+    a += b;              \
+    d = rotl(d ^ a, 16); \
+    c += d;              \
+    b = rotl(b ^ c, 12); \
+    a += b;              \
+    d = rotl(d ^ a, 8);  \
+    c += d;              \
+    b = rotl(b ^ c, 7);  \
+  } while (0)
+
+/* Exposed for the RFC 7539 known-answer test (rounds = 20) and ChaCha8 use. */
+void oc_chacha_block(const uint32_t in[16], uint32_t out[16], int rounds) {
+  uint32_t x[16];
+  memcpy(x, in, sizeof(x));
+  for (int r = 0; r < rounds; r += 2) {
+    QR(x[0], x[4], x[8], x[12]);
+    QR(x[1], x[5], x[9], x[13]);
+    QR(x[2], x[6], x[10], x[14]);
+    QR(x[3], x[7], x[11], x[15]);
+    QR(x[0], x[5], x[10], x[15]);
+    QR(x[1], x[6], x[11], x[12]);
+    QR(x[2], x[7], x[8], x[13]);
+    QR(x[3], x[4], x[9], x[14]);
+  }
+  for (int i = 0; i < 16; ++i) out[i] = x[i] + in[i];
+}
+
+static void chacha_init(oc_chacha* g, const uint8_t seed[32]) {
+  g->in[0] = 0x61707865u;
+  g->in[1] = 0x3320646eu;
+  g->in[2] = 0x79622d32u;
+  g->in[3] = 0x6b206574u;
+  for (int i = 0; i < 8; ++i) {
+    const uint8_t* p = seed + 4 * i;
+    g->in[4 + i] = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
+                   ((uint32_t)p[3] << 24);
+  }
+  g->in[12] = g->in[13] = g->in[14] = g->in[15] = 0;
+  g->pos = 16;
+}
+
+static uint32_t chacha_u32(oc_chacha* g) {
+  if (g->pos >= 16) {
+    oc_chacha_block(g->in, g->out, 8);
+    if (++g->in[12] == 0) ++g->in[13];
+    g->pos = 0;
+  }
+  return g->out[g->pos++];
+}
+
+static uint64_t chacha_u64(oc_chacha* g) {
+  const uint64_t lo = chacha_u32(g);
+  const uint64_t hi = chacha_u32(g);
+  return lo | (hi << 32);
+}
+
+/* ---- Rotation tables (reference src/transform/rotation.cpp:18-35,139-160) */
+/* Fixed 32-byte rotation seed: a format constant of the reference
+ * (rotation.cpp:18-20); it must be identical for codes to be compatible. */
+static const uint8_t kRotationSeed[32] = {164, 143, 161, 123, 88,  50,  61,  10,
+                                          234, 184, 161, 204, 105, 1,   20,  184,
+                                          43,  140, 200, 117, 24,  180, 247, 84,
+                                          141, 68,  110, 161, 228, 223, 32,  242};
+
+int oc_rotation_tables(size_t dim, uint32_t* perm, float* signs, float* inv_sqrt_block,
+                       size_t* block) {
+  if (dim == 0 || dim % 8 != 0 || dim > OC_MAX_DIM) return -1;
+  oc_chacha g;
+  chacha_init(&g, kRotationSeed);
+  /* signs are drawn first, one u32 each, low bit set -> -1 */
+  for (size_t i = 0; i < dim; ++i) signs[i] = (chacha_u32(&g) & 1u) ? -1.f : 1.f;
+  /* then Fisher-Yates from the top, j = u64 mod (i+1) */
+  for (size_t i = 0; i < dim; ++i) perm[i] = (uint32_t)i;
+  for (size_t i = dim - 1; i >= 1; --i) {
+    const size_t j = (size_t)(chacha_u64(&g) % ((uint64_t)i + 1));
+    const uint32_t t = perm[i];
+    perm[i] = perm[j];
+    perm[j] = t;
+  }
+  const size_t b = dim & (~dim + 1); /* largest power of two dividing dim */
+  *block = b;
+  *inv_sqrt_block = 1.f / sqrtf((float)b);
+  return 0;
+}
+
+/* ---- norm (reference src/encode/encode.cpp:16-26,290) ------------------
+ * 8 interleaved chains c[j % 8] += x_j^2 (fused), pairwise tree, sqrtf. */
+float oc_row_norm(const float* row, size_t dim) {
+  float c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t j = 0; j < dim; ++j) c[j % 8] = fmaf(row[j], row[j], c[j % 8]);
+  const float s = ((c[0] + c[1]) + (c[2] + c[3])) + ((c[4] + c[5]) + (c[6] + c[7]));
+  return sqrtf(s);
+}
+
+/* ---- rotation apply (reference src/transform/rotation.cpp:38-121,185-205)
+ * y[i] = (x[perm[i]] * inv) * sign[i]; per block of B: butterflies with
+ * len = 1, 2, ..., B/2 (a+b, a-b); then y *= 1/sqrt(B). One round. */
+static void rotate_with(size_t dim, const uint32_t* perm, const float* signs, size_t block,
+                        float isb, const float* src, float inv, float* dst) {
+  for (size_t i = 0; i < dim; ++i) dst[i] = (src[perm[i]] * inv) * signs[i];
+  for (size_t off = 0; off < dim; off += block) {
+    float* y = dst + off;
+    for (size_t len = 1; len < block; len <<= 1) {
+      for (size_t i = 0; i < block; i += 2 * len) {
+        for (size_t j = i; j < i + len; ++j) {
+          const float a = y[j];
+          const float b = y[j + len];
+          y[j] = a + b;
+          y[j + len] = a - b;
+        }
+      }
+    }
+    for (size_t i = 0; i < block; ++i) y[i] *= isb;
+  }
+}
+
+typedef struct {
+  size_t dim, block;
+  float isb;
+  uint32_t* perm;
+  float* signs;
+} oc_rot;
+
+static int rot_make(oc_rot* r, size_t dim) {
+  r->dim = dim;
+  r->perm = (uint32_t*)malloc(dim * sizeof(uint32_t));
+  r->signs = (float*)malloc(dim * sizeof(float));
+  if (!r->perm || !r->signs) return -1;
+  return oc_rotation_tables(dim, r->perm, r->signs, &r->isb, &r->block);
+}
+
+static void rot_free(oc_rot* r) {
+  free(r->perm);
+  free(r->signs);
+}
+
+int oc_rotate_scaled(size_t dim, const float* src, float inv, float* dst) {
+  oc_rot r;
+  if (rot_make(&r, dim) != 0) return -1;
+  rotate_with(dim, r.perm, r.signs, r.block, r.isb, src, inv, dst);
+  rot_free(&r);
+  return 0;
+}
+
+/* rotate_batch_into (reference src/encode/encode.cpp:159-183) */
+static void rotate_rows(const oc_rot* r, const float* x, size_t n, float* rotated, float* norms) {
+  const size_t dim = r->dim;
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)n; ++i) {
+    const float* src = x + (size_t)i * dim;
+    const float nv = oc_row_norm(src, dim);
+    norms[i] = nv;
+    const float inv = (nv > kMinInputNorm) ? (1.f / nv) : 0.f;
+    rotate_with(dim, r->perm, r->signs, r->block, r->isb, src, inv, rotated + (size_t)i * dim);
+  }
+}
+
+/* ---- Lloyd-Max Beta((d-1)/2,(d-1)/2) codebook ---------------------------
+ * (reference src/quantize/codebook.cpp:12-226). Log-gamma by the Lanczos
+ * approximation with the published g = 7, n = 9 coefficients; regularized
+ * incomplete beta by the modified-Lentz continued fraction (Numerical
+ * Recipes betacf); conditional means by adaptive Simpson; 200 iterations,
+ * tolerance 1e-12; f32 centroids, boundaries = f32 midpoints. */
+static const double kLanczos[9] = {0.99999999999980993,  676.5203681218851,
+                                   -1259.1392167224028,  771.32342877765313,
+                                   -176.61502916214059,  12.507343278686905,
+                                   -0.13857109526572012, 9.984369654078861e-6,
+                                   1.5056327351493116e-7};
+static const double kPi = 3.14159265358979323846;
+
+static double lgamma_lanczos(double z) {
+  if (z < 0.5) return log(kPi) - log(sin(kPi * z)) - lgamma_lanczos(1.0 - z);
+  z -= 1.0;
+  double acc = kLanczos[0];
+  for (int i = 1; i < 9; ++i) acc += kLanczos[i] / (z + (double)i);
+  const double t = z + 7.5;
+  return 0.5 * log(2.0 * kPi) + (z + 0.5) * log(t) - t + log(acc);
+}
+
+static double clampmin(double v) { return fabs(v) < 1e-30 ? 1e-30 : v; }
+
+static double beta_contfrac(double a, double b, double x) {
+  const double qab = a + b, qap = a + 1.0, qam = a - 1.0;
+  double c = 1.0;
+  double d = 1.0 / clampmin(1.0 - qab * x / qap);
+  double h = d;
+  for (int m = 1; m <= 200; ++m) {
+    const int m2 = 2 * m;
+    double aa = (double)m * (b - (double)m) * x / ((qam + m2) * (a + m2));
+    d = 1.0 / clampmin(1.0 + aa * d);
+    c = clampmin(1.0 + aa / c);
+    h *= d * c;
+    aa = -(a + (double)m) * (qab + (double)m) * x / ((a + m2) * (qap + m2));
+    d = 1.0 / clampmin(1.0 + aa * d);
+    c = clampmin(1.0 + aa / c);
+    const double del = d * c;
+    h *= del;
+    if (fabs(del - 1.0) < 3e-14) break;
+  }
+  return h;
+}
+
+static double inc_beta(double a, double b, double x) {
+  if (x <= 0.0) return 0.0;
+  if (x >= 1.0) return 1.0;
+  const double lbeta = lgamma_lanczos(a) + lgamma_lanczos(b) - lgamma_lanczos(a + b);
+  const double front = exp(log(x) * a + log(1.0 - x) * b - lbeta);
+  if (x < (a + 1.0) / (a + b + 2.0)) return front / a * beta_contfrac(a, b, x);
+  return 1.0 - front / b * beta_contfrac(b, a, 1.0 - x);
+}
+
+double oc_beta_cdf_aa(double a, double t) { return inc_beta(a, a, t); }
+
+typedef struct {
+  double a, lbeta;
+} beta_pdf_t;
+
+static double beta_pdf(const beta_pdf_t* p, double t) {
+  if (t <= 0.0 || t >= 1.0) return 0.0;
+  return exp((p->a - 1.0) * log(t) + (p->a - 1.0) * log(1.0 - t) - p->lbeta);
+}
+
+/* integrand x * pdf((x+1)/2) / 2 on [-1, 1] */
+static double cond_mean_f(const beta_pdf_t* p, double x) {
+  return x * beta_pdf(p, (x + 1.0) / 2.0) / 2.0;
+}
+
+static double simpson_rec(const beta_pdf_t* p, double a, double b, double fa, double fb,
+                          double fm, double whole, double tol, int depth) {
+  const double mid = (a + b) / 2.0;
+  const double lm = (a + mid) / 2.0, rm = (mid + b) / 2.0;
+  const double flm = cond_mean_f(p, lm), frm = cond_mean_f(p, rm);
+  const double left = (mid - a) / 6.0 * (fa + 4.0 * flm + fm);
+  const double right = (b - mid) / 6.0 * (fm + 4.0 * frm + fb);
+  const double both = left + right;
+  if (depth == 0 || fabs(both - whole) < 15.0 * tol) return both + (both - whole) / 15.0;
+  return simpson_rec(p, a, mid, fa, fm, flm, left, tol / 2.0, depth - 1) +
+         simpson_rec(p, mid, b, fm, fb, frm, right, tol / 2.0, depth - 1);
+}
+
+static double simpson(const beta_pdf_t* p, double a, double b, double tol, int depth) {
+  const double mid = (a + b) / 2.0;
+  const double fa = cond_mean_f(p, a), fb = cond_mean_f(p, b), fm = cond_mean_f(p, mid);
+  const double whole = (b - a) / 6.0 * (fa + 4.0 * fm + fb);
+  return simpson_rec(p, a, b, fa, fb, fm, whole, tol, depth);
+}
+
+int oc_codebook(size_t bits, size_t dim, float* boundaries, float* centroids) {
+  if (bits < 2 || bits > 4 || dim < 2) return -1;
+  const double a = ((double)dim - 1.0) / 2.0;
+  beta_pdf_t p;
+  p.a = a;
+  p.lbeta = 2.0 * lgamma_lanczos(a) - lgamma_lanczos(2.0 * a);
+  const size_t L = (size_t)1 << bits;
+  const double sd = sqrt(2.0 * a / ((2.0 * a + 1.0) * 4.0 * a));
+  const double spread = 3.0 * sd;
+  double cen[16], nxt[16], edge[17];
+  for (size_t i = 0; i < L; ++i)
+    cen[i] = -spread + 2.0 * spread * (double)i / (double)(L - 1);
+  for (int it = 0; it < 200; ++it) {
+    edge[0] = -1.0;
+    for (size_t i = 0; i + 1 < L; ++i) edge[i + 1] = (cen[i] + cen[i + 1]) / 2.0;
+    edge[L] = 1.0;
+    for (size_t i = 0; i < L; ++i) {
+      const double lo = edge[i], hi = edge[i + 1];
+      const double prob = inc_beta(a, a, (hi + 1.0) / 2.0) - inc_beta(a, a, (lo + 1.0) / 2.0);
+      nxt[i] = (prob < 1e-15) ? cen[i] : simpson(&p, lo, hi, 1e-14, 50) / prob;
+    }
+    double worst = 0.0;
+    for (size_t i = 0; i < L; ++i) {
+      const double dlt = fabs(cen[i] - nxt[i]);
+      if (dlt > worst) worst = dlt;
+      cen[i] = nxt[i];
+    }
+    if (worst < 1e-12) break;
+  }
+  for (size_t i = 0; i < L; ++i) centroids[i] = (float)cen[i];
+  for (size_t i = 0; i + 1 < L; ++i) boundaries[i] = (centroids[i] + centroids[i + 1]) * 0.5f;
+  return 0;
+}
+
+/* ---- quantize + alpha + bit-planes (reference src/encode/encode.cpp:112-231)
+ * code = #{i : y > b_i} (strict); alpha = norm / (float)inner with inner a
+ * 4-chain double sum of y * reconstruction, 0 when inner <= 1e-12. Bit-plane
+ * p of 8 codes is one byte, code k at bit (7 - k), plane-major per row.
+ * With TQ+: compare (y + shift) * scale, reconstruct c * (1/scale) - shift. */
+static float quantize_row(const float* y, size_t dim, size_t bits, const float* bnd,
+                          const float* cen, const float* shift, const float* scale,
+                          const float* inv_scale, int identity, float norm, uint8_t* packed) {
+  double ch[4] = {0, 0, 0, 0};
+  const size_t nb = ((size_t)1 << bits) - 1;
+  const size_t bpp = dim / 8;
+  for (size_t c = 0; c < dim / 8; ++c) {
+    uint8_t codes[8];
+    for (size_t k = 0; k < 8; ++k) {
+      const size_t j = c * 8 + k;
+      unsigned v = 0;
+      if (identity) {
+        for (size_t bi = 0; bi < nb; ++bi) v += (y[j] > bnd[bi]) ? 1u : 0u;
+        /* the product of two floats is exact in double; fused or not agrees */
+        ch[j % 4] = fma((double)y[j], (double)cen[v], ch[j % 4]);
+      } else {
+        const float cal = (y[j] + shift[j]) * scale[j];
+        for (size_t bi = 0; bi < nb; ++bi) v += (cal > bnd[bi]) ? 1u : 0u;
+        const double recon = fma((double)cen[v], (double)inv_scale[j], -(double)shift[j]);
+        ch[j % 4] = fma((double)y[j], recon, ch[j % 4]);
+      }
+      codes[k] = (uint8_t)v;
+    }
+    for (size_t p = 0; p < bits; ++p) {
+      unsigned byte = 0;
+      for (size_t k = 0; k < 8; ++k) byte |= ((codes[k] >> p) & 1u) << (7 - k);
+      packed[p * bpp + c] = (uint8_t)byte;
+    }
+  }
+  const double inner = (ch[0] + ch[1]) + (ch[2] + ch[3]);
+  return (inner > kDegenerateInnerEps) ? norm / (float)inner : 0.f;
+}
+
+int oc_encode(const float* x, size_t n, size_t dim, size_t bits, const float* shift,
+              const float* scale, uint8_t* packed_out, float* alpha_out, float* rotated_out) {
+  if (dim == 0 || dim % 8 != 0 || bits < 2 || bits > 4) return -1;
+  oc_rot r;
+  if (rot_make(&r, dim) != 0) return -1;
+  float bnd[15], cen[16];
+  oc_codebook(bits, dim, bnd, cen);
+  float* rot = rotated_out ? rotated_out : (float*)malloc(n * dim * sizeof(float) + 1);
+  float* norms = (float*)malloc(n * sizeof(float) + 1);
+  float* inv_s = (float*)malloc(dim * sizeof(float));
+  rotate_rows(&r, x, n, rot, norms);
+  /* identity also when every shift is 0 and every scale 1 (encode.cpp:204-210) */
+  int identity = 1;
+  for (size_t d = 0; shift != NULL && scale != NULL && d < dim; ++d) {
+    inv_s[d] = 1.f / scale[d];
+    if (shift[d] != 0.f || scale[d] != 1.f) identity = 0;
+  }
+  const size_t bpr = bits * (dim / 8);
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)n; ++i) {
+    alpha_out[i] = quantize_row(rot + (size_t)i * dim, dim, bits, bnd, cen, shift, scale, inv_s,
+                                identity, norms[i], packed_out + (size_t)i * bpr);
+  }
+  if (!rotated_out) free(rot);
+  free(norms);
+  free(inv_s);
+  rot_free(&r);
+  return 0;
+}
+
+/* ---- TQ+ calibration (reference src/encode/encode.cpp:142-150,233-286,327-335)
+ * Anchors p_hi = I_{(c_outer+1)/2}(a, a), p_lo = 1 - p_hi; per dimension the
+ * lo_idx-th and hi_idx-th order statistics of the rotated sample, compared as
+ * order-preserving u32 keys of the f32 bits. */
+static uint32_t f32_key(float x) {
+  uint32_t b;
+  memcpy(&b, &x, 4);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+static float f32_unkey(uint32_t k) {
+  const uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  float x;
+  memcpy(&x, &b, 4);
+  return x;
+}
+
+static int cmp_u32(const void* pa, const void* pb) {
+  const uint32_t a = *(const uint32_t*)pa, b = *(const uint32_t*)pb;
+  return (a > b) - (a < b);
+}
+
+int oc_fit_calibration(const float* x, size_t n, size_t dim, size_t bits, float* shift_out,
+                       float* scale_out) {
+  if (n < 2 || dim == 0 || dim % 8 != 0 || bits < 2 || bits > 4) return -1;
+  oc_rot r;
+  if (rot_make(&r, dim) != 0) return -1;
+  float bnd[15], cen[16];
+  oc_codebook(bits, dim, bnd, cen);
+  float* rot = (float*)malloc(n * dim * sizeof(float));
+  float* norms = (float*)malloc(n * sizeof(float));
+  rotate_rows(&r, x, n, rot, norms);
+  const double a = ((double)dim - 1.0) / 2.0;
+  float c_outer = 0.f;
+  for (size_t i = 0; i < ((size_t)1 << bits); ++i)
+    if (fabsf(cen[i]) > c_outer) c_outer = fabsf(cen[i]);
+  const double p_hi = inc_beta(a, a, ((double)c_outer + 1.0) / 2.0);
+  const double p_lo = 1.0 - p_hi;
+  const float qc_lo = -c_outer, qc_hi = c_outer;
+  const float qc_span = qc_hi - qc_lo;
+  const size_t lo_idx = (size_t)((double)n * p_lo);
+  size_t hi_idx = (size_t)((double)n * p_hi);
+  if (hi_idx > n - 1) hi_idx = n - 1;
+  if (hi_idx < lo_idx + 1) hi_idx = lo_idx + 1;
+#pragma omp parallel
+  {
+    uint32_t* col = (uint32_t*)malloc(n * sizeof(uint32_t));
+#pragma omp for schedule(static)
+    for (long long d = 0; d < (long long)dim; ++d) {
+      for (size_t i = 0; i < n; ++i) col[i] = f32_key(rot[i * dim + (size_t)d]);
+      qsort(col, n, sizeof(uint32_t), cmp_u32);
+      const float q_lo = f32_unkey(col[lo_idx]);
+      const float q_hi = f32_unkey(col[hi_idx]);
+      const float span = q_hi - q_lo;
+      shift_out[d] = 0.f;
+      scale_out[d] = 1.f;
+      if (!(span > 1e-12f) || !(qc_span > 0.f)) continue;
+      const float s = qc_span / span;
+      scale_out[d] = s;
+      shift_out[d] = qc_lo / s - q_lo;
+    }
+    free(col);
+  }
+  free(rot);
+  free(norms);
+  rot_free(&r);
+  return 0;
+}
+
+/* ---- bit-planes -> row-major byte groups (reference src/pack/pack.cpp:70-131)
+ * b=4 and b=3: byte g = code[2g] << 4 | code[2g+1]  (3-bit codes in nibbles);
+ * b=2: byte g = c[4g] << 6 | c[4g+1] << 4 | c[4g+2] << 2 | c[4g+3]. */
+static size_t groups_of(size_t bits, size_t dim) { return dim / (8 / bits); }
+
+int oc_codes_flat(const uint8_t* packed, size_t n, size_t bits, size_t dim, uint8_t* out) {
+  if (dim % 8 != 0 || bits < 2 || bits > 4) return -1;
+  const size_t bpp = dim / 8, bpr = bits * bpp, G = groups_of(bits, dim);
+  const size_t cpb = 8 / bits;            /* codes per byte: 2, 2, 4 */
+  const size_t field = (bits == 3) ? 4 : bits;
+#pragma omp parallel for schedule(static)
+  for (long long v = 0; v < (long long)n; ++v) {
+    const uint8_t* row = packed + (size_t)v * bpr;
+    uint8_t* o = out + (size_t)v * G;
+    memset(o, 0, G);
+    for (size_t j = 0; j < dim; ++j) {
+      unsigned code = 0;
+      for (size_t p = 0; p < bits; ++p) code |= ((row[p * bpp + j / 8] >> (7 - j % 8)) & 1u) << p;
+      const size_t g = j / cpb;
+      const size_t shift = (cpb - 1 - j % cpb) * field;
+      o[g] = (uint8_t)(o[g] | (code << shift));
+    }
+  }
+  return 0;
+}
+
+/* ---- FastScan 192x1 PERM0 layout (reference src/pack/pack.cpp:157-186,
+ * include/vectorcache/pack/pack.hpp:14) ------------------------------------
+ * For block bk, group g, 32-lane chunk ch, j < 16 with va = 192 bk + 32 ch +
+ * P[j], vb = va + 16: byte j = hi(va) | hi(vb) << 4, byte 16 + j = lo(va) |
+ * lo(vb) << 4. Vectors >= n read as zero. */
+static const unsigned kPerm0[16] = {0, 8, 1, 9, 2, 10, 3, 11, 4, 12, 5, 13, 6, 14, 7, 15};
+
+int oc_pack_perm0(const uint8_t* flat, size_t n, size_t G, uint8_t* out) {
+  const size_t nb = (n + OC_BLOCK - 1) / OC_BLOCK;
+#pragma omp parallel for schedule(static)
+  for (long long bk = 0; bk < (long long)nb; ++bk) {
+    for (size_t g = 0; g < G; ++g) {
+      uint8_t* o = out + ((size_t)bk * G + g) * OC_BLOCK;
+      for (size_t ch = 0; ch < OC_BLOCK / OC_CHUNK; ++ch) {
+        for (size_t j = 0; j < 16; ++j) {
+          const size_t va = (size_t)bk * OC_BLOCK + ch * OC_CHUNK + kPerm0[j];
+          const size_t vb = va + 16;
+          const unsigned a = va < n ? flat[va * G + g] : 0u;
+          const unsigned b = vb < n ? flat[vb * G + g] : 0u;
+          o[ch * OC_CHUNK + j] = (uint8_t)((a >> 4) | ((b >> 4) << 4));
+          o[ch * OC_CHUNK + 16 + j] = (uint8_t)((a & 15u) | ((b & 15u) << 4));
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+/* inverse of oc_pack_perm0 (reference read_code, src/pack/pack.cpp:189-203,403-420) */
+int oc_unpack_perm0(const uint8_t* blocked, size_t n, size_t G, uint8_t* flat_out) {
+  static const unsigned kInv[16] = {0, 2, 4, 6, 8, 10, 12, 14, 1, 3, 5, 7, 9, 11, 13, 15};
+#pragma omp parallel for schedule(static)
+  for (long long v = 0; v < (long long)n; ++v) {
+    const size_t bk = (size_t)v / OC_BLOCK, lane = (size_t)v % OC_BLOCK;
+    const size_t ch = lane / OC_CHUNK, l = lane % OC_CHUNK;
+    const size_t j = kInv[l & 15];
+    for (size_t g = 0; g < G; ++g) {
+      const uint8_t* t = blocked + (bk * G + g) * OC_BLOCK + ch * OC_CHUNK;
+      const unsigned hp = t[j], lp = t[16 + j];
+      const unsigned hi = (l < 16) ? (hp & 15u) : (hp >> 4);
+      const unsigned lo = (l < 16) ? (lp & 15u) : (lp >> 4);
+      flat_out[(size_t)v * G + g] = (uint8_t)((hi << 4) | lo);
+    }
+  }
+  return 0;
+}
+
+/* sequential 192x1 layout (reference src/pack/pack.cpp:133-154) */
+int oc_pack_seq(const uint8_t* flat, size_t n, size_t G, uint8_t* out) {
+  const size_t nb = (n + OC_BLOCK - 1) / OC_BLOCK;
+#pragma omp parallel for schedule(static)
+  for (long long bk = 0; bk < (long long)nb; ++bk) {
+    for (size_t g = 0; g < G; ++g) {
+      for (size_t lane = 0; lane < OC_BLOCK; ++lane) {
+        const size_t v = (size_t)bk * OC_BLOCK + lane;
+        out[((size_t)bk * G + g) * OC_BLOCK + lane] = v < n ? flat[v * G + g] : 0;
+      }
+    }
+  }
+  return 0;
+}
+
+/* ---- per-query LUT (reference src/search/search.cpp:149-219) -------------
+ * For group g the first 16 entries are indexed by the byte's high nibble (the
+ * group's first dims), the second 16 by the low nibble. Entry = sum over the
+ * nibble's codes of q_d * centroid (sequential from 0.f; for b=2 the second
+ * product is fused into the running sum). bias = sequential float sum of
+ * (min_hi + min_lo); scale = max span / 127; u8 = clamp(round((v - min) /
+ * scale), 0, 127) computed as (v - min) * (1/scale). */
+static void build_lut(const float* q, size_t dim, size_t bits, const float* cen, uint8_t* lut,
+                      float* scale_out, float* bias_out, float* fv) {
+  const size_t cpb = 8 / bits, cpn = cpb / 2, G = dim / cpb;
+  const unsigned mask = (1u << bits) - 1u;
+  float span = 0.f, bias = 0.f;
+  for (size_t g = 0; g < G; ++g) {
+    float mn[2], mx[2];
+    for (int half = 0; half < 2; ++half) {
+      mn[half] = FLT_MAX;
+      mx[half] = -FLT_MAX;
+      const float* qq = q + g * cpb + (size_t)half * cpn;
+      for (unsigned nib = 0; nib < 16; ++nib) {
+        float s = 0.f;
+        for (size_t c = 0; c < cpn; ++c) {
+          const unsigned code = (nib >> ((cpn - 1 - c) * bits)) & mask;
+          s = fmaf(qq[c], cen[code], s);
+        }
+        fv[g * 32 + (size_t)half * 16 + nib] = s;
+        if (s < mn[half]) mn[half] = s;
+        if (s > mx[half]) mx[half] = s;
+      }
+    }
+    bias += mn[0] + mn[1];
+    if (mx[0] - mn[0] > span) span = mx[0] - mn[0];
+    if (mx[1] - mn[1] > span) span = mx[1] - mn[1];
+    fv[G * 32 + 2 * g] = mn[0];
+    fv[G * 32 + 2 * g + 1] = mn[1];
+  }
+  float scale = (span > 0.f) ? span / 127.f : 1.f;
+  float inv = 1.f;
+  if (scale >= FLT_MIN)
+    inv = 1.f / scale;
+  else
+    scale = 1.f;
+  for (size_t g = 0; g < G; ++g) {
+    for (int half = 0; half < 2; ++half) {
+      const float mnv = fv[G * 32 + 2 * g + (size_t)half];
+      for (unsigned nib = 0; nib < 16; ++nib) {
+        float r = roundf((fv[g * 32 + (size_t)half * 16 + nib] - mnv) * inv);
+        if (r < 0.f) r = 0.f;
+        if (r > 127.f) r = 127.f;
+        lut[g * 32 + (size_t)half * 16 + nib] = (uint8_t)r;
+      }
+    }
+  }
+  *scale_out = scale;
+  *bias_out = bias;
+}
+
+/* prepare_queries (reference src/search/search.cpp:762-860, AVX2 backend):
+ * norm + rotation as for encode; with TQ+ the LUT is built from q / scale and
+ * bias_corr = (float) -sum_d (double) q_d * shift_d. */
+int oc_prepare_queries(const float* q, size_t nq, size_t dim, size_t bits, const float* shift,
+                       const float* scale, float* q_rot_out, float* bias_corr_out,
+                       uint8_t* luts_out, float* lut_scale_out, float* lut_bias_out) {
+  if (dim == 0 || dim % 8 != 0 || bits < 2 || bits > 4) return -1;
+  oc_rot r;
+  if (rot_make(&r, dim) != 0) return -1;
+  float bnd[15], cen[16];
+  oc_codebook(bits, dim, bnd, cen);
+  const size_t G = groups_of(bits, dim);
+  const int tq = (shift != NULL && scale != NULL);
+#pragma omp parallel
+  {
+    float* row = (float*)malloc(dim * sizeof(float));
+    float* fv = (float*)malloc((G * 34) * sizeof(float));
+#pragma omp for schedule(static)
+    for (long long i = 0; i < (long long)nq; ++i) {
+      const float* src = q + (size_t)i * dim;
+      const float nv = oc_row_norm(src, dim);
+      const float inv = (nv > kMinInputNorm) ? (1.f / nv) : 0.f;
+      rotate_with(dim, r.perm, r.signs, r.block, r.isb, src, inv, row);
+      float* dst = q_rot_out + (size_t)i * dim;
+      float bc = 0.f;
+      if (tq) {
+        double acc = 0.0;
+        for (size_t d = 0; d < dim; ++d) {
+          dst[d] = row[d] / scale[d];
+          acc = fma(-(double)row[d], (double)shift[d], acc);
+        }
+        bc = (float)acc;
+      } else {
+        memcpy(dst, row, dim * sizeof(float));
+      }
+      bias_corr_out[i] = bc;
+      build_lut(dst, dim, bits, cen, luts_out + (size_t)i * G * 32, &lut_scale_out[i],
+                &lut_bias_out[i], fv);
+    }
+    free(row);
+    free(fv);
+  }
+  rot_free(&r);
+  return 0;
+}
+
+/* ---- the scan (reference src/search/search_avx2.cpp:95-222) --------------
+ * For vector v and 256-group window w the exact integer sum
+ *   A_w = sum_g lut[g][hi(byte)] + lut[g][16 + lo(byte)]   (<= 65024)
+ * then fa = bias + bias_corr; fa = fmaf(scale, (float) A_w, fa) per window in
+ * order; score = fa * alpha_v. */
+int oc_window_sums(const uint8_t* lut, const uint8_t* flat, size_t n, size_t G,
+                   uint32_t* sums_out) {
+  const size_t W = (G + OC_FLUSH_EVERY - 1) / OC_FLUSH_EVERY;
+  for (size_t v = 0; v < n; ++v) {
+    for (size_t w = 0; w < W; ++w) {
+      uint32_t acc = 0;
+      const size_t g1 = (w + 1) * OC_FLUSH_EVERY < G ? (w + 1) * OC_FLUSH_EVERY : G;
+      for (size_t g = w * OC_FLUSH_EVERY; g < g1; ++g) {
+        const unsigned byte = flat[v * G + g];
+        acc += lut[g * 32 + (byte >> 4)] + lut[g * 32 + 16 + (byte & 15u)];
+      }
+      sums_out[v * W + w] = acc;
+    }
+  }
+  return 0;
+}
+
+typedef struct {
+  float s;
+  uint64_t id;
+} oc_cand;
+
+/* (score desc, id asc): the reference's merge/sort order
+ * (src/search/search.cpp:739-752, include/vectorcache/search/topk_heap.hpp:19-22) */
+static int cand_cmp(const void* pa, const void* pb) {
+  const oc_cand* a = (const oc_cand*)pa;
+  const oc_cand* b = (const oc_cand*)pb;
+  if (a->s != b->s) return (a->s > b->s) ? -1 : 1;
+  return (a->id < b->id) ? -1 : (a->id > b->id);
+}
+
+/* Top-k per query over all n vectors. A score is admitted only if it is
+ * > -inf (the reference's heaps start at -inf with strict '>' admission, so
+ * NaN and -inf never enter). k_eff = min(k, n); rows with fewer admitted
+ * candidates are padded with (0.f, 0). -0.f scores are reported as +0.f. */
+int oc_scan_topk(const uint8_t* luts, const float* lut_scale, const float* lut_bias_total,
+                 size_t nq, const uint8_t* flat, size_t n, size_t G, const float* alphas,
+                 const uint64_t* id_map, size_t k, float* scores_out, uint64_t* ids_out,
+                 size_t* k_out) {
+  const size_t keff = k < n ? k : n;
+  *k_out = (n == 0) ? 0 : keff;
+  if (n == 0 || nq == 0 || keff == 0) return 0;
+  const size_t W = (G + OC_FLUSH_EVERY - 1) / OC_FLUSH_EVERY;
+#pragma omp parallel
+  {
+    oc_cand* c = (oc_cand*)malloc(n * sizeof(oc_cand));
+#pragma omp for schedule(dynamic)
+    for (long long qi = 0; qi < (long long)nq; ++qi) {
+      const uint8_t* lut = luts + (size_t)qi * G * 32;
+      size_t cnt = 0;
+      for (size_t v = 0; v < n; ++v) {
+        float fa = lut_bias_total[qi];
+        const uint8_t* row = flat + v * G;
+        for (size_t w = 0; w < W; ++w) {
+          uint32_t acc = 0;
+          const size_t g1 = (w + 1) * OC_FLUSH_EVERY < G ? (w + 1) * OC_FLUSH_EVERY : G;
+          for (size_t g = w * OC_FLUSH_EVERY; g < g1; ++g) {
+            const unsigned byte = row[g];
+            acc += lut[g * 32 + (byte >> 4)] + lut[g * 32 + 16 + (byte & 15u)];
+          }
+          fa = fmaf(lut_scale[qi], (float)acc, fa);
+        }
+        const float s = fa * alphas[v] + 0.f;
+        if (s > -INFINITY) {
+          c[cnt].s = s;
+          c[cnt].id = id_map ? id_map[v] : (uint64_t)v;
+          ++cnt;
+        }
+      }
+      qsort(c, cnt, sizeof(oc_cand), cand_cmp);
+      for (size_t j = 0; j < keff; ++j) {
+        scores_out[(size_t)qi * keff + j] = j < cnt ? c[j].s : 0.f;
+        ids_out[(size_t)qi * keff + j] = j < cnt ? c[j].id : 0;
+      }
+    }
+    free(c);
+  }
+  return 0;
+}
+
+/* search_flat (reference src/search/search.cpp:1293-1302) over PERM0 codes. */
+int oc_search_flat(const float* q, size_t nq, size_t dim, size_t k, size_t bits,
+                   const uint8_t* blocked_perm0, const float* alphas, size_t n,
+                   const float* shift, const float* scale, float* scores_out, uint64_t* ids_out,
+                   size_t* k_out) {
+  if (dim == 0 || dim % 8 != 0 || bits < 2 || bits > 4) return -1;
+  const size_t G = groups_of(bits, dim);
+  float* q_rot = (float*)malloc(nq * dim * sizeof(float) + 1);
+  float* bc = (float*)malloc(nq * sizeof(float) + 1);
+  float* ls = (float*)malloc(nq * sizeof(float) + 1);
+  float* lb = (float*)malloc(nq * sizeof(float) + 1);
+  uint8_t* luts = (uint8_t*)malloc(nq * G * 32 + 1);
+  uint8_t* flat = (uint8_t*)malloc(n * G + 1);
+  int rc = oc_prepare_queries(q, nq, dim, bits, shift, scale, q_rot, bc, luts, ls, lb);
+  if (rc == 0) {
+    for (size_t i = 0; i < nq; ++i) lb[i] = lb[i] + bc[i];
+    oc_unpack_perm0(blocked_perm0, n, G, flat);
+    rc = oc_scan_topk(luts, ls, lb, nq, flat, n, G, alphas, NULL, k, scores_out, ids_out, k_out);
+  }
+  free(q_rot);
+  free(bc);
+  free(ls);
+  free(lb);
+  free(luts);
+  free(flat);
+  return rc;
+}
+
+/* merge_search_results (reference src/search/search.cpp:1551-1600): union of
+ * both rows, duplicates by id keep the higher score, top k_dst by (score desc,
+ * id asc), short rows padded with (0.f, 0). */
+static int cand_id_cmp(const void* pa, const void* pb) {
+  const oc_cand* a = (const oc_cand*)pa;
+  const oc_cand* b = (const oc_cand*)pb;
+  if (a->id != b->id) return (a->id < b->id) ? -1 : 1;
+  if (a->s != b->s) return (a->s > b->s) ? -1 : 1;
+  return 0;
+}
+
+int oc_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t* dst_ids, size_t k_src,
+                     const float* src_scores, const uint64_t* src_ids) {
+  if (k_src == 0 || k_dst == 0) return 0;
+  oc_cand* c = (oc_cand*)malloc((k_dst + k_src) * sizeof(oc_cand));
+  for (size_t qi = 0; qi < nq; ++qi) {
+    size_t m = 0;
+    for (size_t j = 0; j < k_dst; ++j) {
+      c[m].s = dst_scores[qi * k_dst + j];
+      c[m++].id = dst_ids[qi * k_dst + j];
+    }
+    for (size_t j = 0; j < k_src; ++j) {
+      c[m].s = src_scores[qi * k_src + j];
+      c[m++].id = src_ids[qi * k_src + j];
+    }
+    qsort(c, m, sizeof(oc_cand), cand_id_cmp);
+    size_t u = 0;
+    for (size_t j = 0; j < m; ++j)
+      if (u == 0 || c[u - 1].id != c[j].id) c[u++] = c[j];
+    qsort(c, u, sizeof(oc_cand), cand_cmp);
+    for (size_t j = 0; j < k_dst; ++j) {
+      dst_scores[qi * k_dst + j] = j < u ? c[j].s : 0.f;
+      dst_ids[qi * k_dst + j] = j < u ? c[j].id : 0;
+    }
+  }
+  free(c);
+  return 0;
+}

@@ -1,0 +1,52 @@
+/* Test infrastructure only — the CPU oracle for the plain flat TurboQuant
+ * encode + search path. Only tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg may load it, and only as the checker. The product path
+ * (libvectorcache_hip.so) never links or calls it.
+ *
+ * All entry points return 0 on success, a negative value on bad arguments. */
+#ifndef VC_ORACLE_H
+#define VC_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int oc_rotation_tables(size_t dim, uint32_t* perm, float* signs, float* inv_sqrt_block,
+                       size_t* block);
+int oc_codebook(size_t bits, size_t dim, float* boundaries, float* centroids);
+double oc_beta_cdf_aa(double a, double t);
+float oc_row_norm(const float* row, size_t dim);
+int oc_rotate_scaled(size_t dim, const float* src, float inv, float* dst);
+int oc_encode(const float* x, size_t n, size_t dim, size_t bits, const float* shift,
+              const float* scale, uint8_t* packed_out, float* alpha_out, float* rotated_out);
+int oc_fit_calibration(const float* x, size_t n, size_t dim, size_t bits, float* shift_out,
+                       float* scale_out);
+int oc_codes_flat(const uint8_t* packed, size_t n, size_t bits, size_t dim, uint8_t* out);
+int oc_pack_perm0(const uint8_t* flat, size_t n, size_t n_groups, uint8_t* out);
+int oc_pack_seq(const uint8_t* flat, size_t n, size_t n_groups, uint8_t* out);
+int oc_unpack_perm0(const uint8_t* blocked, size_t n, size_t n_groups, uint8_t* flat_out);
+int oc_prepare_queries(const float* q, size_t nq, size_t dim, size_t bits, const float* shift,
+                       const float* scale, float* q_rot_out, float* bias_corr_out,
+                       uint8_t* luts_out, float* lut_scale_out, float* lut_bias_out);
+int oc_window_sums(const uint8_t* lut, const uint8_t* flat, size_t n, size_t n_groups,
+                   uint32_t* sums_out);
+int oc_scan_topk(const uint8_t* luts, const float* lut_scale, const float* lut_bias_total,
+                 size_t nq, const uint8_t* flat, size_t n, size_t n_groups, const float* alphas,
+                 const uint64_t* id_map, size_t k, float* scores_out, uint64_t* ids_out,
+                 size_t* k_out);
+int oc_search_flat(const float* q, size_t nq, size_t dim, size_t k, size_t bits,
+                   const uint8_t* blocked_perm0, const float* alphas, size_t n,
+                   const float* shift, const float* scale, float* scores_out, uint64_t* ids_out,
+                   size_t* k_out);
+int oc_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t* dst_ids, size_t k_src,
+                     const float* src_scores, const uint64_t* src_ids);
+void oc_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

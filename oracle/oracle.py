@@ -1,0 +1,245 @@
+"""Test infrastructure only: ctypes bindings for the two CPU checkers.
+
+* ``Oracle("port")``      -> oracle/liboracle.so, this repository's C restatement
+  of the reference's plain flat TurboQuant path (oracle/oracle.c);
+* ``Oracle("reference")`` -> oracle/_ref/libvcref.so, the reference's own
+  sources compiled in place (oracle/Makefile, oracle/ref_harness.cpp).
+
+Both expose the same numpy-level functions. Only tests/, __graft_entry__.smoke()
+and bench.py's CPU-baseline leg import this module, as the checker; the product
+path (vectorcache_amd) never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_P = ctypes.c_void_p
+_S = ctypes.c_size_t
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+class Oracle:
+    def __init__(self, kind: str = "port"):
+        self.kind = kind
+        if kind == "port":
+            path, self._pre = HERE / "liboracle.so", "oc_"
+        elif kind == "reference":
+            path, self._pre = HERE / "_ref" / "libvcref.so", "vcref_"
+        else:
+            raise ValueError(kind)
+        if not path.exists():
+            raise FileNotFoundError(f"{path} is not built (make -C oracle{' ref' if kind == 'reference' else ''})")
+        if kind == "reference":
+            os.environ["TURBOVEC_NO_VNNI"] = "1"
+        self.lib = ctypes.CDLL(str(path))
+        self.lib[self._pre + "row_norm"].restype = ctypes.c_float
+        self.lib[self._pre + "beta_cdf_aa"].restype = ctypes.c_double
+
+    def _f(self, name):
+        return self.lib[self._pre + name]
+
+    def _check(self, rc, name):
+        if rc != 0:
+            msg = ""
+            if self.kind == "reference":
+                f = self.lib.vcref_last_error
+                f.restype = ctypes.c_char_p
+                msg = f().decode()
+            raise RuntimeError(f"{self.kind}:{name} failed ({rc}) {msg}")
+
+    def set_threads(self, n: int):
+        self._f("set_threads")(ctypes.c_int(int(n)))
+
+    # ---- host precompute -------------------------------------------------
+    def rotation_tables(self, dim):
+        perm = np.zeros(dim, np.uint32)
+        signs = np.zeros(dim, np.float32)
+        isb, blk = ctypes.c_float(), ctypes.c_size_t()
+        self._check(self._f("rotation_tables")(_S(dim), _ptr(perm), _ptr(signs), ctypes.byref(isb),
+                                               ctypes.byref(blk)), "rotation_tables")
+        return perm, signs, np.float32(isb.value), int(blk.value)
+
+    def codebook(self, bits, dim):
+        b = np.zeros(15, np.float32)
+        c = np.zeros(16, np.float32)
+        self._check(self._f("codebook")(_S(bits), _S(dim), _ptr(b), _ptr(c)), "codebook")
+        L = 1 << bits
+        return b[: L - 1].copy(), c[:L].copy()
+
+    def beta_cdf_aa(self, a, t):
+        return float(self._f("beta_cdf_aa")(ctypes.c_double(a), ctypes.c_double(t)))
+
+    def row_norm(self, row):
+        row = np.ascontiguousarray(row, np.float32)
+        return np.float32(self._f("row_norm")(_ptr(row), _S(row.size)))
+
+    def rotate_scaled(self, src, inv):
+        src = np.ascontiguousarray(src, np.float32)
+        dst = np.zeros_like(src)
+        self._check(self._f("rotate_scaled")(_S(src.size), _ptr(src), ctypes.c_float(inv), _ptr(dst)),
+                    "rotate_scaled")
+        return dst
+
+    # ---- encode -----------------------------------------------------------
+    def encode(self, x, bits, shift=None, scale=None, want_rotated=False):
+        x = np.ascontiguousarray(x, np.float32)
+        n, dim = x.shape
+        packed = np.zeros((n, bits * dim // 8), np.uint8)
+        alpha = np.zeros(n, np.float32)
+        rot = np.zeros((n, dim), np.float32) if want_rotated else None
+        sh = None if shift is None else np.ascontiguousarray(shift, np.float32)
+        sc = None if scale is None else np.ascontiguousarray(scale, np.float32)
+        self._check(self._f("encode")(_ptr(x), _S(n), _S(dim), _S(bits), _ptr(sh), _ptr(sc),
+                                      _ptr(packed), _ptr(alpha), _ptr(rot)), "encode")
+        return (packed, alpha, rot) if want_rotated else (packed, alpha)
+
+    def fit_calibration(self, x, bits):
+        x = np.ascontiguousarray(x, np.float32)
+        n, dim = x.shape
+        shift = np.zeros(dim, np.float32)
+        scale = np.zeros(dim, np.float32)
+        self._check(self._f("fit_calibration")(_ptr(x), _S(n), _S(dim), _S(bits), _ptr(shift),
+                                               _ptr(scale)), "fit_calibration")
+        return shift, scale
+
+    # ---- pack -------------------------------------------------------------
+    def codes_flat(self, packed, bits, dim):
+        packed = np.ascontiguousarray(packed, np.uint8)
+        n = packed.shape[0]
+        G = dim // (8 // bits)
+        out = np.zeros((n, G), np.uint8)
+        self._check(self._f("codes_flat")(_ptr(packed), _S(n), _S(bits), _S(dim), _ptr(out)),
+                    "codes_flat")
+        return out
+
+    def repack_perm0(self, packed, bits, dim):
+        """bit-planes -> PERM0 192x1 blocked bytes."""
+        packed = np.ascontiguousarray(packed, np.uint8)
+        n = packed.shape[0]
+        G = dim // (8 // bits)
+        nb = (n + 191) // 192
+        out = np.zeros(nb * G * 192, np.uint8)
+        if self.kind == "reference":
+            nbo = ctypes.c_size_t()
+            self._check(self.lib.vcref_repack_perm0(_ptr(packed), _S(n), _S(bits), _S(dim), _ptr(out),
+                                                    ctypes.byref(nbo)), "repack_perm0")
+        else:
+            flat = self.codes_flat(packed, bits, dim)
+            self._check(self.lib.oc_pack_perm0(_ptr(flat), _S(n), _S(G), _ptr(out)), "pack_perm0")
+        return out
+
+    def repack_seq(self, packed, bits, dim):
+        packed = np.ascontiguousarray(packed, np.uint8)
+        n = packed.shape[0]
+        G = dim // (8 // bits)
+        nb = (n + 191) // 192
+        out = np.zeros(nb * G * 192, np.uint8)
+        if self.kind == "reference":
+            self._check(self.lib.vcref_repack_seq(_ptr(packed), _S(n), _S(bits), _S(dim), _ptr(out)),
+                        "repack_seq")
+        else:
+            flat = self.codes_flat(packed, bits, dim)
+            self._check(self.lib.oc_pack_seq(_ptr(flat), _S(n), _S(G), _ptr(out)), "pack_seq")
+        return out
+
+    def unpack_perm0(self, blocked, n, G):
+        """PERM0 blocked -> row-major flat byte groups (port only)."""
+        blocked = np.ascontiguousarray(blocked, np.uint8)
+        out = np.zeros((n, G), np.uint8)
+        self._check(self.lib.oc_unpack_perm0(_ptr(blocked), _S(n), _S(G), _ptr(out)), "unpack_perm0")
+        return out
+
+    # ---- search -------------------------------------------------------------
+    def prepare_queries(self, q, bits, shift=None, scale=None):
+        q = np.ascontiguousarray(q, np.float32)
+        nq, dim = q.shape
+        G = dim // (8 // bits)
+        q_rot = np.zeros((nq, dim), np.float32)
+        bc = np.zeros(nq, np.float32)
+        luts = np.zeros((nq, G * 32), np.uint8)
+        ls = np.zeros(nq, np.float32)
+        lb = np.zeros(nq, np.float32)
+        sh = None if shift is None else np.ascontiguousarray(shift, np.float32)
+        sc = None if scale is None else np.ascontiguousarray(scale, np.float32)
+        self._check(self._f("prepare_queries")(_ptr(q), _S(nq), _S(dim), _S(bits), _ptr(sh), _ptr(sc),
+                                               _ptr(q_rot), _ptr(bc), _ptr(luts), _ptr(ls), _ptr(lb)),
+                    "prepare_queries")
+        return dict(q_rot=q_rot, bias_corr=bc, luts=luts, lut_scale=ls, lut_bias=lb)
+
+    def search_flat(self, q, k, bits, blocked_perm0, alphas, shift=None, scale=None):
+        q = np.ascontiguousarray(q, np.float32)
+        nq, dim = q.shape
+        n = alphas.size
+        alphas = np.ascontiguousarray(alphas, np.float32)
+        blocked_perm0 = np.ascontiguousarray(blocked_perm0, np.uint8)
+        kk = max(min(k, n), 0)
+        scores = np.zeros(max(nq * kk, 1), np.float32)
+        ids = np.zeros(max(nq * kk, 1), np.uint64)
+        kout = ctypes.c_size_t()
+        sh = None if shift is None else np.ascontiguousarray(shift, np.float32)
+        sc = None if scale is None else np.ascontiguousarray(scale, np.float32)
+        if self.kind == "reference":
+            nb = (n + 191) // 192
+            rc = self.lib.vcref_search_flat(_ptr(q), _S(nq), _S(dim), _S(k), _S(bits), _ptr(blocked_perm0),
+                                            _S(nb), _ptr(alphas), _S(n), _ptr(sh), _ptr(sc), _ptr(scores),
+                                            _ptr(ids), ctypes.byref(kout))
+        else:
+            rc = self.lib.oc_search_flat(_ptr(q), _S(nq), _S(dim), _S(k), _S(bits), _ptr(blocked_perm0),
+                                         _ptr(alphas), _S(n), _ptr(sh), _ptr(sc), _ptr(scores), _ptr(ids),
+                                         ctypes.byref(kout))
+        self._check(rc, "search_flat")
+        ko = int(kout.value)
+        return scores[: nq * ko].reshape(nq, ko), ids[: nq * ko].reshape(nq, ko)
+
+    def scan_topk(self, luts, lut_scale, lut_bias_total, flat, alphas, k, id_map=None):
+        """Scan + top-k from prepared LUTs and row-major flat codes (port only)."""
+        luts = np.ascontiguousarray(luts, np.uint8)
+        flat = np.ascontiguousarray(flat, np.uint8)
+        nq = luts.shape[0]
+        n, G = flat.shape
+        kk = max(min(k, n), 0)
+        scores = np.zeros(max(nq * kk, 1), np.float32)
+        ids = np.zeros(max(nq * kk, 1), np.uint64)
+        kout = ctypes.c_size_t()
+        im = None if id_map is None else np.ascontiguousarray(id_map, np.uint64)
+        self._check(self.lib.oc_scan_topk(_ptr(luts), _ptr(np.ascontiguousarray(lut_scale, np.float32)),
+                                          _ptr(np.ascontiguousarray(lut_bias_total, np.float32)), _S(nq),
+                                          _ptr(flat), _S(n), _S(G), _ptr(np.ascontiguousarray(alphas, np.float32)),
+                                          _ptr(im), _S(k), _ptr(scores), _ptr(ids), ctypes.byref(kout)),
+                    "scan_topk")
+        ko = int(kout.value)
+        return scores[: nq * ko].reshape(nq, ko), ids[: nq * ko].reshape(nq, ko)
+
+    def window_sums(self, lut, flat):
+        lut = np.ascontiguousarray(lut, np.uint8)
+        flat = np.ascontiguousarray(flat, np.uint8)
+        n, G = flat.shape
+        W = (G + 255) // 256
+        out = np.zeros((n, W), np.uint32)
+        self._check(self.lib.oc_window_sums(_ptr(lut), _ptr(flat), _S(n), _S(G), _ptr(out)), "window_sums")
+        return out
+
+    def merge_results(self, dst_scores, dst_ids, src_scores, src_ids):
+        ds = np.ascontiguousarray(dst_scores, np.float32).copy()
+        di = np.ascontiguousarray(dst_ids, np.uint64).copy()
+        ss = np.ascontiguousarray(src_scores, np.float32)
+        si = np.ascontiguousarray(src_ids, np.uint64)
+        nq, kd = ds.shape
+        ks = ss.shape[1]
+        self._check(self._f("merge_results")(_S(nq), _S(kd), _ptr(ds), _ptr(di), _S(ks), _ptr(ss), _ptr(si)),
+                    "merge_results")
+        return ds, di
+
+
+def available(kind: str) -> bool:
+    if kind == "port":
+        return (HERE / "liboracle.so").exists()
+    return (HERE / "_ref" / "libvcref.so").exists()
