@@ -1,0 +1,167 @@
+/* vectorcache_hip.h — C ABI of libvectorcache_hip.so, the MI355X (gfx950)
+ * implementation of VectorCache's plain flat TurboQuant encode + search path.
+ *
+ * Plain C: pointers and sizes only. Every call returns a vc_status; on failure
+ * vc_last_error() holds a thread-local message. The C++20 TurboQuantIndex
+ * wrapper (vectorcache_amd/csrc/include/vectorcache/turbo_quant_index.hpp)
+ * maps statuses back onto the reference's exception types:
+ *   VC_ERR_INVALID_ARGUMENT -> std::invalid_argument
+ *   VC_ERR_STATE            -> std::runtime_error   (e.g. search before prepare)
+ *   VC_ERR_OUT_OF_RANGE     -> std::out_of_range
+ *   VC_ERR_DEVICE / OOM     -> std::runtime_error
+ *
+ * Numerics: codes, PERM0 blocks, alpha, per-query LUTs, LUT-scan integer sums,
+ * scores and top-k ids are bit-identical to the reference CPU path
+ * (AVX2 PERM0 uint8-LUT scan) on the same inputs. Result rows are ordered by
+ * (score desc, id asc); a -0.0f score is reported as +0.0f.
+ *
+ * "host" pointers are ordinary CPU memory; "device" pointers are HIP device
+ * memory on the index's device; `stream` is a hipStream_t (NULL = the index's
+ * own stream).
+ */
+#ifndef VECTORCACHE_HIP_H
+#define VECTORCACHE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  VC_OK = 0,
+  VC_ERR_INVALID_ARGUMENT = 1,
+  VC_ERR_STATE = 2,
+  VC_ERR_OUT_OF_RANGE = 3,
+  VC_ERR_DEVICE = 4,
+  VC_ERR_OOM = 5
+} vc_status;
+
+typedef struct vc_index vc_index;
+
+/* Thread-local message of the last failing call on this thread. */
+const char* vc_last_error(void);
+
+/* Library version string. */
+const char* vc_version(void);
+
+/* ---- TurboQuantIndex ------------------------------------------------------
+ * Replaces the reference's index class shape (include/vectorcache/index.hpp:18-78:
+ * ctor(dim, bits), add, prepare, search, dim/size/bit_width/codes_bytes/
+ * index_bytes) for the plain flat TurboQuant path, plus calibrate()
+ * (README.md:108; semantics of fit_calibration, include/vectorcache/encode/
+ * encode.hpp:33-35).
+ * dim: positive multiple of 8, <= 16384 (src/index.cpp:129-131);
+ * bits in {2, 3, 4} (src/index.cpp:132-137). */
+vc_status vc_index_create(size_t dim, size_t bits, vc_index** out);
+vc_status vc_index_destroy(vc_index* index);
+
+/* TQ+ calibration from n sample rows (host, row-major n x dim). Must precede
+ * the first prepare() that encodes vectors. n >= 2 (src/encode/encode.cpp:327-335). */
+vc_status vc_index_calibrate(vc_index* index, const float* sample, size_t n);
+/* Same, sample already in device memory. */
+vc_status vc_index_calibrate_device(vc_index* index, const float* sample_dev, size_t n);
+
+/* Append n row-major vectors (host) — replaces add(std::span<const float>)
+ * (src/index.cpp:151-213). Vectors are staged on the device; prepare() encodes
+ * them. add() after prepare() is allowed and marks the index unprepared. */
+vc_status vc_index_add(vc_index* index, const float* vectors, size_t n);
+vc_status vc_index_add_device(vc_index* index, const float* vectors_dev, size_t n);
+
+/* Encode every staged vector on the device (replaces prepare(),
+ * src/index.cpp:215-265). */
+vc_status vc_index_prepare(vc_index* index);
+
+/* Top-k search — replaces search(queries, k) (src/index.cpp:267-289) and
+ * search_flat (src/search/search.cpp:1293-1302). queries: nq x dim host rows.
+ * k is clipped to size(); *k_out receives the clipped k. scores / ids are
+ * nq x (*k_out) row-major, caller-allocated with room for nq * min(k, size()).
+ * Search before prepare -> VC_ERR_STATE. */
+vc_status vc_index_search(vc_index* index, const float* queries, size_t nq, size_t k,
+                          float* scores, uint64_t* ids, size_t* k_out);
+/* Same with device-resident queries and outputs, ordered on `stream`
+ * (NULL: the index's stream). The call returns once the work is enqueued. */
+vc_status vc_index_search_device(vc_index* index, const float* queries_dev, size_t nq, size_t k,
+                                 float* scores_dev, uint64_t* ids_dev, size_t* k_out,
+                                 void* stream);
+
+/* Accessors (index.hpp:23-40). */
+size_t vc_index_dim(const vc_index* index);
+size_t vc_index_size(const vc_index* index);
+size_t vc_index_bits(const vc_index* index);
+int vc_index_is_prepared(const vc_index* index);
+int vc_index_is_calibrated(const vc_index* index);
+size_t vc_index_codes_bytes(const vc_index* index);
+size_t vc_index_index_bytes(const vc_index* index);
+
+/* ---- exports in the reference's formats (parity and interop) --------------
+ * packed: bit-plane codes exactly as encode() writes them (n x bits*dim/8,
+ *         src/encode/encode.cpp:112-157);
+ * flat:   row-major byte groups (n x G, codes_flat_from_packed, src/pack/pack.cpp:98-131);
+ * perm0:  192x1 PERM0 blocked bytes (ceil(n/192) x G x 192, repack_perm0,
+ *         src/pack/pack.cpp:157-186,388-401);
+ * alpha:  per-vector scale (n floats);
+ * calibration: TQ+ shift/scale (dim floats each; identity when uncalibrated). */
+vc_status vc_index_export_packed(vc_index* index, uint8_t* packed_host);
+vc_status vc_index_export_flat(vc_index* index, uint8_t* flat_host);
+vc_status vc_index_export_perm0(vc_index* index, uint8_t* perm0_host);
+vc_status vc_index_export_alpha(vc_index* index, float* alpha_host);
+vc_status vc_index_get_calibration(const vc_index* index, float* shift_host, float* scale_host);
+
+/* Per-query state as prepare_queries builds it (src/search/search.cpp:762-860):
+ * luts nq x G*32 u8, lut_scale, lut_bias (without bias_corr), bias_corr. */
+vc_status vc_index_prepare_queries(vc_index* index, const float* queries, size_t nq,
+                                   uint8_t* luts, float* lut_scale, float* lut_bias,
+                                   float* bias_corr);
+
+/* ---- stateless entry points (host buffers) ---------------------------------
+ * Host precompute, bit-identical to the reference:
+ * Rotation tables (src/transform/rotation.cpp:139-160), codebook
+ * (src/quantize/codebook.cpp:203-226). */
+vc_status vc_rotation_tables(size_t dim, uint32_t* perm, float* signs, float* inv_sqrt_block,
+                             size_t* block);
+vc_status vc_codebook(size_t bits, size_t dim, float* boundaries, float* centroids);
+
+/* encode() on the GPU (src/encode/encode.cpp:294-307): n x dim host rows ->
+ * bit-plane codes and alpha. shift/scale: TQ+ calibration or both NULL. */
+vc_status vc_encode(const float* vectors, size_t n, size_t dim, size_t bits, const float* shift,
+                    const float* scale, uint8_t* packed_out, float* alpha_out);
+
+/* fit_calibration() on the GPU (src/encode/encode.cpp:327-335). */
+vc_status vc_fit_calibration(const float* sample, size_t n, size_t dim, size_t bits,
+                             float* shift_out, float* scale_out);
+
+/* score_prepared with an id map over PERM0 blocked codes (the multi-bucket
+ * plumbing, src/search/search.cpp:1072-1154): queries (host) scored against n
+ * vectors given as PERM0 blocks + alpha; ids reported through id_map (may be
+ * NULL = identity). Output nq x min(k, n). */
+vc_status vc_search_perm0(const float* queries, size_t nq, size_t dim, size_t bits, size_t k,
+                          const uint8_t* perm0, const float* alpha, size_t n,
+                          const uint64_t* id_map, const float* shift, const float* scale,
+                          float* scores, uint64_t* ids, size_t* k_out);
+
+/* merge_search_results (src/search/search.cpp:1551-1600) on the GPU:
+ * dst (nq x k_dst) <- top k_dst of dst U src by (score desc, id asc), a
+ * duplicated id keeping its higher score, short rows padded with (0, 0). */
+vc_status vc_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t* dst_ids,
+                           size_t k_src, const float* src_scores, const uint64_t* src_ids);
+
+/* ---- tuning / introspection ------------------------------------------------
+ * Query-tile size T the scan kernel uses for (dim, bits, nq) — 0 = automatic.
+ * vc_set_scan_tile forces T in {1, 4, 8} for later searches (0 restores auto). */
+vc_status vc_set_scan_tile(size_t tile);
+size_t vc_scan_tile_for(size_t dim, size_t bits, size_t nq);
+
+/* Instrumentation (no reference counterpart): when profiling is on, every
+ * search brackets its scan kernel with HIP events on the launch stream.
+ * vc_last_scan_stats waits for the last such kernel and reports its duration
+ * in ms, its query tile and the number of vector ranges (work split) it used. */
+vc_status vc_set_profiling(int on);
+vc_status vc_last_scan_stats(double* ms, size_t* tile, size_t* n_ranges);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VECTORCACHE_HIP_H */

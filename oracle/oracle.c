@@ -235,7 +235,7 @@ static double lgamma_lanczos(double z) {
   double acc = kLanczos[0];
   for (int i = 1; i < 9; ++i) acc += kLanczos[i] / (z + (double)i);
   const double t = z + 7.5;
-  return 0.5 * log(2.0 * kPi) + (z + 0.5) * log(t) - t + log(acc);
+  return fma(z + 0.5, log(t), 0.5 * log(2.0 * kPi)) - t + log(acc);
 }
 
 static double clampmin(double v) { return fabs(v) < 1e-30 ? 1e-30 : v; }
@@ -248,11 +248,11 @@ static double beta_contfrac(double a, double b, double x) {
   for (int m = 1; m <= 200; ++m) {
     const int m2 = 2 * m;
     double aa = (double)m * (b - (double)m) * x / ((qam + m2) * (a + m2));
-    d = 1.0 / clampmin(1.0 + aa * d);
+    d = 1.0 / clampmin(fma(aa, d, 1.0));
     c = clampmin(1.0 + aa / c);
     h *= d * c;
     aa = -(a + (double)m) * (qab + (double)m) * x / ((a + m2) * (qap + m2));
-    d = 1.0 / clampmin(1.0 + aa * d);
+    d = 1.0 / clampmin(fma(aa, d, 1.0));
     c = clampmin(1.0 + aa / c);
     const double del = d * c;
     h *= del;
@@ -265,9 +265,9 @@ static double inc_beta(double a, double b, double x) {
   if (x <= 0.0) return 0.0;
   if (x >= 1.0) return 1.0;
   const double lbeta = lgamma_lanczos(a) + lgamma_lanczos(b) - lgamma_lanczos(a + b);
-  const double front = exp(log(x) * a + log(1.0 - x) * b - lbeta);
+  const double front = exp(fma(log(x), a, log(1.0 - x) * b) - lbeta);
   if (x < (a + 1.0) / (a + b + 2.0)) return front / a * beta_contfrac(a, b, x);
-  return 1.0 - front / b * beta_contfrac(b, a, 1.0 - x);
+  return fma(-(front / b), beta_contfrac(b, a, 1.0 - x), 1.0);
 }
 
 double oc_beta_cdf_aa(double a, double t) { return inc_beta(a, a, t); }
@@ -278,7 +278,7 @@ typedef struct {
 
 static double beta_pdf(const beta_pdf_t* p, double t) {
   if (t <= 0.0 || t >= 1.0) return 0.0;
-  return exp((p->a - 1.0) * log(t) + (p->a - 1.0) * log(1.0 - t) - p->lbeta);
+  return exp(fma(p->a - 1.0, log(t), (p->a - 1.0) * log(1.0 - t)) - p->lbeta);
 }
 
 /* integrand x * pdf((x+1)/2) / 2 on [-1, 1] */
@@ -291,8 +291,8 @@ static double simpson_rec(const beta_pdf_t* p, double a, double b, double fa, do
   const double mid = (a + b) / 2.0;
   const double lm = (a + mid) / 2.0, rm = (mid + b) / 2.0;
   const double flm = cond_mean_f(p, lm), frm = cond_mean_f(p, rm);
-  const double left = (mid - a) / 6.0 * (fa + 4.0 * flm + fm);
-  const double right = (b - mid) / 6.0 * (fm + 4.0 * frm + fb);
+  const double left = (mid - a) / 6.0 * (fma(4.0, flm, fa) + fm);
+  const double right = (b - mid) / 6.0 * (fma(4.0, frm, fm) + fb);
   const double both = left + right;
   if (depth == 0 || fabs(both - whole) < 15.0 * tol) return both + (both - whole) / 15.0;
   return simpson_rec(p, a, mid, fa, fm, flm, left, tol / 2.0, depth - 1) +
@@ -302,7 +302,7 @@ static double simpson_rec(const beta_pdf_t* p, double a, double b, double fa, do
 static double simpson(const beta_pdf_t* p, double a, double b, double tol, int depth) {
   const double mid = (a + b) / 2.0;
   const double fa = cond_mean_f(p, a), fb = cond_mean_f(p, b), fm = cond_mean_f(p, mid);
-  const double whole = (b - a) / 6.0 * (fa + 4.0 * fm + fb);
+  const double whole = (b - a) / 6.0 * (fma(4.0, fm, fa) + fb);
   return simpson_rec(p, a, b, fa, fb, fm, whole, tol, depth);
 }
 
@@ -703,7 +703,9 @@ static int cand_cmp(const void* pa, const void* pb) {
   return (a->id < b->id) ? -1 : (a->id > b->id);
 }
 
-/* Top-k per query over all n vectors. A score is admitted only if it is
+/* Top-k per query over all n vectors (reference src/search/search.cpp:
+ * 600-758,1072-1154: with an id map the ranking breaks ties on the position
+ * and maps ids afterwards, remap_ids :324-332). A score is admitted only if it is
  * > -inf (the reference's heaps start at -inf with strict '>' admission, so
  * NaN and -inf never enter). k_eff = min(k, n); rows with fewer admitted
  * candidates are padded with (0.f, 0). -0.f scores are reported as +0.f. */
@@ -737,14 +739,14 @@ int oc_scan_topk(const uint8_t* luts, const float* lut_scale, const float* lut_b
         const float s = fa * alphas[v] + 0.f;
         if (s > -INFINITY) {
           c[cnt].s = s;
-          c[cnt].id = id_map ? id_map[v] : (uint64_t)v;
+          c[cnt].id = (uint64_t)v; /* ties break on the position, mapped after */
           ++cnt;
         }
       }
       qsort(c, cnt, sizeof(oc_cand), cand_cmp);
       for (size_t j = 0; j < keff; ++j) {
         scores_out[(size_t)qi * keff + j] = j < cnt ? c[j].s : 0.f;
-        ids_out[(size_t)qi * keff + j] = j < cnt ? c[j].id : 0;
+        ids_out[(size_t)qi * keff + j] = j < cnt ? (id_map ? id_map[c[j].id] : c[j].id) : 0;
       }
     }
     free(c);

@@ -44,6 +44,8 @@ int oc_search_flat(const float* q, size_t nq, size_t dim, size_t k, size_t bits,
 int oc_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t* dst_ids, size_t k_src,
                      const float* src_scores, const uint64_t* src_ids);
 void oc_set_threads(int n);
+/* the ChaCha block function (8 rounds for the rotation, 20 for RFC 7539 tests) */
+void oc_chacha_block(const uint32_t in[16], uint32_t out[16], int rounds);
 
 #ifdef __cplusplus
 }

@@ -40,11 +40,12 @@ class Oracle:
         if kind == "reference":
             os.environ["TURBOVEC_NO_VNNI"] = "1"
         self.lib = ctypes.CDLL(str(path))
-        self.lib[self._pre + "row_norm"].restype = ctypes.c_float
-        self.lib[self._pre + "beta_cdf_aa"].restype = ctypes.c_double
+        # attribute access caches the function object, so restype sticks
+        self._f("row_norm").restype = ctypes.c_float
+        self._f("beta_cdf_aa").restype = ctypes.c_double
 
     def _f(self, name):
-        return self.lib[self._pre + name]
+        return getattr(self.lib, self._pre + name)
 
     def _check(self, rc, name):
         if rc != 0:

@@ -1,0 +1,25 @@
+"""Seeded inputs of the small golden cases (shared by make_golden.py and the tests)."""
+from __future__ import annotations
+
+import numpy as np
+
+from vectorcache_amd import synthetic
+
+# (bits, dim, n, with TQ+ calibration)
+CASES = [(4, 128, 300, False), (3, 128, 300, False), (2, 128, 300, False), (4, 200, 250, False),
+         (4, 768, 200, True), (2, 768, 200, True), (3, 1536, 150, True), (4, 1536, 150, True),
+         (2, 1536, 150, False), (4, 8, 64, False)]
+KS = [1, 10, 64, 1000]
+
+
+def case_data(bits: int, dim: int, n: int, idx: int):
+    """Database, queries and a calibration sample with the edge cases the tests need."""
+    x = synthetic.mixture(n, dim, 16, 0.3, 7000 + idx, 7100 + idx)
+    x[3] = 0.0                                  # zero vector: alpha = 0
+    x[7] = x[11]                                # exact duplicate: score tie
+    x[9] = x[12] * np.float32(1e-12)            # below the 1e-10 norm floor
+    q = synthetic.mixture(6, dim, 16, 0.3, 7000 + idx, 7200 + idx)
+    q[2] = x[11]                                # query equal to a duplicated vector
+    q[4] = 0.0                                  # zero query
+    sample = synthetic.mixture(400, dim, 16, 0.3, 7000 + idx, 7300 + idx)
+    return x, q, sample

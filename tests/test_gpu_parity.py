@@ -1,0 +1,217 @@
+"""GPU parity: the HIP path (through the C ABI) against the reference's golden
+vectors and against the CPU oracle on the same seeded inputs.
+
+Bar: bit-exact codes, PERM0 blocks, alpha, TQ+ shift/scale, LUT bytes/scale/
+bias, scores and top-k ids (ties by lower id). Scores compare with -0.0 ==
++0.0 (the library reports -0.0 as +0.0), every other bit must match.
+"""
+import numpy as np
+import pytest
+
+from util import bits_equal, scores_equal, sha
+
+pytestmark = pytest.mark.gpu
+
+
+def _index(vc, c, x, g, sample):
+    ix = vc.TurboQuantIndex(c["dim"], c["bits"])
+    if c["tq"]:
+        ix.calibrate(sample)
+    ix.add(x)
+    ix.prepare()
+    return ix
+
+
+def test_version(vc):
+    assert "gfx950" in vc.version()
+
+
+def test_encode_matches_golden(vc, golden_cases):
+    for c, g, x, q, sample in golden_cases:
+        sh, sc = (g["shift"], g["scale"]) if c["tq"] else (None, None)
+        packed, alpha = vc.encode(x, c["bits"], sh, sc)
+        assert np.array_equal(packed, g["packed"]), c
+        assert bits_equal(alpha, g["alpha"]), c
+
+
+def test_fit_calibration_matches_golden(vc, golden_cases):
+    for c, g, x, q, sample in golden_cases:
+        if not c["tq"]:
+            continue
+        sh, sc = vc.fit_calibration(sample, c["bits"])
+        assert bits_equal(sh, g["shift"]), c
+        assert bits_equal(sc, g["scale"]), c
+
+
+def test_index_exports_and_search_match_golden(vc, golden_cases):
+    for c, g, x, q, sample in golden_cases:
+        ix = _index(vc, c, x, g, sample)
+        assert ix.size() == c["n"]
+        if c["tq"]:
+            sh, sc = ix.calibration()
+            assert bits_equal(sh, g["shift"]) and bits_equal(sc, g["scale"]), c
+        assert np.array_equal(ix.export_packed(), g["packed"]), c
+        assert np.array_equal(ix.export_flat(), g["flat"]), c
+        assert sha(ix.export_perm0()) == bytes(g["perm0_sha"]).hex(), c
+        assert bits_equal(ix.export_alpha(), g["alpha"]), c
+        prep = ix.prepare_queries(q)
+        assert np.array_equal(prep["luts"], g["prep_luts"]), c
+        assert bits_equal(prep["lut_scale"], g["prep_lut_scale"]), c
+        assert bits_equal(prep["lut_bias"], g["prep_lut_bias"]), c
+        assert bits_equal(prep["bias_corr"], g["prep_bias_corr"]), c
+        for k in (1, 10, 64, 1000):
+            r = ix.search(q, k)
+            assert np.array_equal(r.ids, g[f"ids_k{k}"]), (c, k)
+            assert scores_equal(r.scores, g[f"scores_k{k}"]), (c, k)
+
+
+@pytest.mark.parametrize("tile", [1, 4, 8])
+def test_scan_tiles_agree_with_golden(vc, golden_cases, tile):
+    vc.set_scan_tile(tile)
+    try:
+        for c, g, x, q, sample in golden_cases:
+            ix = _index(vc, c, x, g, sample)
+            for k in (10, 64):
+                r = ix.search(q, k)
+                assert np.array_equal(r.ids, g[f"ids_k{k}"]), (c, k, tile)
+                assert scores_equal(r.scores, g[f"scores_k{k}"]), (c, k, tile)
+    finally:
+        vc.set_scan_tile(0)
+
+
+def test_idmap_scoring_matches_golden(vc, golden_cases):
+    for c, g, x, q, sample in golden_cases:
+        if "idmap" not in g:
+            continue
+        # the PERM0 bytes the golden run scanned are the GPU's own export
+        ix = _index(vc, c, x, g, sample)
+        perm0 = ix.export_perm0()
+        r = vc.search_perm0(q, 10, c["bits"], perm0, g["alpha"], id_map=g["idmap"])
+        assert np.array_equal(r.ids, g["idmap_ids"]), c
+        assert scores_equal(r.scores, g["idmap_scores"]), c
+
+
+def test_merge_matches_golden(vc):
+    from conftest import load_golden
+
+    m = load_golden("merge.npz")
+    dst = vc.SearchResults(m["ds"], m["di"])
+    src = vc.SearchResults(m["ss"], m["si"])
+    out = vc.merge_search_results(dst, src)
+    assert np.array_equal(out.ids, m["mi"])
+    assert bits_equal(out.scores, m["ms"])
+
+
+def test_flat100k_full_size_digests(vc, manifest, oracle):
+    """The 100k x 128 4-bit workload at full size: every output's SHA-256
+    equals the reference's."""
+    from vectorcache_amd import synthetic
+
+    want = manifest["flat100k"]
+    x = synthetic.flat100k_db()
+    q = synthetic.flat100k_queries()
+    assert sha(x) == want["x"] and sha(q) == want["q"]
+    ix = vc.TurboQuantIndex(128, 4)
+    ix.add(x)
+    ix.prepare()
+    assert sha(ix.export_packed()) == want["packed"]
+    assert sha(ix.export_alpha()) == want["alpha"]
+    assert sha(ix.export_perm0()) == want["perm0"]
+    prep = ix.prepare_queries(q)
+    assert sha(prep["luts"]) == want["luts"]
+    assert sha(prep["lut_scale"]) == want["lut_scale"]
+    assert sha(prep["lut_bias"]) == want["lut_bias"]
+    r = ix.search(q, 10)
+    assert sha(r.ids) == want["ids_k10"]
+    assert sha(r.scores) == want["scores_k10"]
+
+
+def _oracle_search(oracle, x, q, bits, k, shift=None, scale=None):
+    packed, alpha = oracle.encode(x, bits, shift, scale)
+    flat = oracle.codes_flat(packed, bits, x.shape[1])
+    prep = oracle.prepare_queries(q, bits, shift, scale)
+    return oracle.scan_topk(prep["luts"], prep["lut_scale"], prep["lut_bias"] + prep["bias_corr"], flat, alpha, k)
+
+
+@pytest.mark.parametrize("bits,dim,n,nq,k", [(4, 768, 20000, 37, 100), (2, 1536, 7000, 9, 64),
+                                             (3, 384, 5000, 70, 1), (4, 1536, 3000, 5, 2048)])
+def test_random_vs_oracle(vc, oracle, bits, dim, n, nq, k):
+    from vectorcache_amd import synthetic
+
+    x = synthetic.mixture(n, dim, 64, 0.3, 11 + dim, 12 + n)
+    q = synthetic.mixture(nq, dim, 64, 0.3, 11 + dim, 13 + nq)
+    s_o, i_o = _oracle_search(oracle, x, q, bits, k)
+    ix = vc.TurboQuantIndex(dim, bits)
+    ix.add(x)
+    ix.prepare()
+    r = ix.search(q, k)
+    assert np.array_equal(r.ids, i_o)
+    assert scores_equal(r.scores, s_o)
+
+
+def test_ties_resolve_to_lower_id(vc, oracle):
+    r = np.random.default_rng(3)
+    base = r.standard_normal((50, 256)).astype(np.float32)
+    x = base[r.integers(0, 50, 9000)]
+    q = r.standard_normal((6, 256)).astype(np.float32)
+    s_o, i_o = _oracle_search(oracle, x, q, 4, 300)
+    ix = vc.TurboQuantIndex(256, 4)
+    ix.add(x)
+    ix.prepare()
+    res = ix.search(q, 300)
+    assert np.array_equal(res.ids, i_o)
+    assert scores_equal(res.scores, s_o)
+    # within equal scores ids ascend
+    for row_s, row_i in zip(res.scores, res.ids):
+        same = row_s[1:] == row_s[:-1]
+        assert np.all(row_i[1:][same] > row_i[:-1][same])
+
+
+def test_incremental_add_equals_bulk_and_batch_equals_single(vc):
+    from vectorcache_amd import synthetic
+
+    x = synthetic.mixture(3000, 128, 32, 0.3, 5, 6)
+    q = synthetic.mixture(7, 128, 32, 0.3, 5, 7)
+    a = vc.TurboQuantIndex(128, 4)
+    a.add(x)
+    a.prepare()
+    b = vc.TurboQuantIndex(128, 4)
+    b.add(x[:1000])
+    b.prepare()
+    b.add(x[1000:2100])
+    b.add(x[2100:])
+    assert not b.prepared
+    b.prepare()
+    ra, rb = a.search(q, 25), b.search(q, 25)
+    assert np.array_equal(ra.ids, rb.ids) and bits_equal(ra.scores, rb.scores)
+    for i in range(q.shape[0]):
+        ri = a.search(q[i : i + 1], 25)
+        assert np.array_equal(ri.ids[0], ra.ids[i]) and bits_equal(ri.scores[0], ra.scores[i])
+
+
+def test_api_errors_and_edges(vc):
+    from vectorcache_amd._capi import IndexStateError
+
+    with pytest.raises(ValueError):
+        vc.TurboQuantIndex(100, 4)  # not a multiple of 8
+    with pytest.raises(ValueError):
+        vc.TurboQuantIndex(128, 5)
+    with pytest.raises(ValueError):
+        vc.TurboQuantIndex(16392, 4)
+    ix = vc.TurboQuantIndex(64, 4)
+    x = np.random.default_rng(0).standard_normal((10, 64)).astype(np.float32)
+    with pytest.raises(ValueError):
+        ix.add(x.ravel()[:-3])  # size not a multiple of dim
+    ix.add(x)
+    with pytest.raises(IndexStateError):
+        ix.search(x[:2], 3)  # before prepare
+    ix.prepare()
+    r = ix.search(x[:2], 50)  # k clipped to size
+    assert r.k == 10 and r.ids[0, 0] == 0 and r.ids[1, 0] == 1
+    with pytest.raises(IndexStateError):
+        ix.calibrate(x)  # after vectors were encoded
+    empty = vc.TurboQuantIndex(64, 4)
+    empty.prepare()
+    assert empty.search(x[:3], 5).k == 0
+    with pytest.raises(ValueError):
+        vc.TurboQuantIndex(64, 4).calibrate(x[:1])  # needs >= 2 rows

@@ -1,0 +1,100 @@
+"""ctypes bindings of libvectorcache_hip.so (include/vectorcache_hip.h).
+
+Importing this module loads the in-tree HIP library and raises ImportError if
+it is not built: there is no CPU fallback on the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libvectorcache_hip.so"
+
+if not LIB_PATH.exists():
+    raise ImportError(f"{LIB_PATH} is not built; run `python -m vectorcache_amd.build` (HIP, gfx950)")
+
+lib = ctypes.CDLL(str(LIB_PATH))
+
+P = ctypes.c_void_p
+S = ctypes.c_size_t
+F = ctypes.c_float
+I = ctypes.c_int
+
+VC_OK, VC_ERR_INVALID_ARGUMENT, VC_ERR_STATE, VC_ERR_OUT_OF_RANGE, VC_ERR_DEVICE, VC_ERR_OOM = range(6)
+
+
+class VectorCacheError(RuntimeError):
+    """Device or allocation failure (VC_ERR_DEVICE, VC_ERR_OOM)."""
+
+
+class DeviceOutOfMemory(VectorCacheError):
+    pass
+
+
+class IndexStateError(RuntimeError):
+    """The reference's std::runtime_error for call-order errors (search before prepare)."""
+
+
+_SIGS = {
+    "vc_last_error": ([], ctypes.c_char_p),
+    "vc_version": ([], ctypes.c_char_p),
+    "vc_index_create": ([S, S, ctypes.POINTER(P)], I),
+    "vc_index_destroy": ([P], I),
+    "vc_index_calibrate": ([P, P, S], I),
+    "vc_index_calibrate_device": ([P, P, S], I),
+    "vc_index_add": ([P, P, S], I),
+    "vc_index_add_device": ([P, P, S], I),
+    "vc_index_prepare": ([P], I),
+    "vc_index_search": ([P, P, S, S, P, P, ctypes.POINTER(S)], I),
+    "vc_index_search_device": ([P, P, S, S, P, P, ctypes.POINTER(S), P], I),
+    "vc_index_dim": ([P], S),
+    "vc_index_size": ([P], S),
+    "vc_index_bits": ([P], S),
+    "vc_index_is_prepared": ([P], I),
+    "vc_index_is_calibrated": ([P], I),
+    "vc_index_codes_bytes": ([P], S),
+    "vc_index_index_bytes": ([P], S),
+    "vc_index_export_packed": ([P, P], I),
+    "vc_index_export_flat": ([P, P], I),
+    "vc_index_export_perm0": ([P, P], I),
+    "vc_index_export_alpha": ([P, P], I),
+    "vc_index_get_calibration": ([P, P, P], I),
+    "vc_index_prepare_queries": ([P, P, S, P, P, P, P], I),
+    "vc_rotation_tables": ([S, P, P, P, ctypes.POINTER(S)], I),
+    "vc_codebook": ([S, S, P, P], I),
+    "vc_encode": ([P, S, S, S, P, P, P, P], I),
+    "vc_fit_calibration": ([P, S, S, S, P, P], I),
+    "vc_search_perm0": ([P, S, S, S, S, P, P, S, P, P, P, P, P, ctypes.POINTER(S)], I),
+    "vc_merge_results": ([S, S, P, P, S, P, P], I),
+    "vc_set_scan_tile": ([S], I),
+    "vc_scan_tile_for": ([S, S, S], S),
+    "vc_set_profiling": ([I], I),
+    "vc_last_scan_stats": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(S), ctypes.POINTER(S)], I),
+}
+
+for _name, (_args, _res) in _SIGS.items():
+    _f = getattr(lib, _name)
+    _f.argtypes = _args
+    _f.restype = _res
+
+EXPORTED = tuple(_SIGS)
+
+
+def check(status: int) -> None:
+    """Raise the Python counterpart of the reference's exception for a status."""
+    if status == VC_OK:
+        return
+    msg = (lib.vc_last_error() or b"").decode()
+    if status == VC_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if status == VC_ERR_STATE:
+        raise IndexStateError(msg)
+    if status == VC_ERR_OUT_OF_RANGE:
+        raise IndexError(msg)
+    if status == VC_ERR_OOM:
+        raise DeviceOutOfMemory(msg)
+    raise VectorCacheError(msg)
+
+
+def version() -> str:
+    return lib.vc_version().decode()

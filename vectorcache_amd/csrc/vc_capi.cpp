@@ -1,0 +1,742 @@
+// C ABI of libvectorcache_hip.so (include/vectorcache_hip.h).
+//
+// Owns device memory and streams; the arithmetic lives in vc_kernels.hip and
+// the host precompute in vc_host.cpp. There is no CPU fallback: every entry
+// that computes runs on the GPU and reports VC_ERR_DEVICE when it cannot.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "vc_host.hpp"
+#include "vc_internal.hpp"
+#include "vectorcache_hip.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+std::atomic<int> g_forced_tile{0};
+
+// scan-kernel instrumentation (vc_set_profiling / vc_last_scan_stats)
+struct ScanProbe {
+  std::mutex mu;
+  bool on = false;
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool recorded = false;
+  int tile = 0, ranges = 0;
+} g_probe;
+
+struct VcError {
+  vc_status status;
+  std::string msg;
+};
+
+[[noreturn]] void fail(vc_status st, const std::string& msg) { throw VcError{st, msg}; }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  const vc_status st = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? VC_ERR_OOM : VC_ERR_DEVICE;
+  fail(st, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+vc_status guarded(F&& f) {
+  try {
+    f();
+    return VC_OK;
+  } catch (const VcError& e) {
+    g_last_error = e.msg;
+    return e.status;
+  } catch (const std::invalid_argument& e) {
+    g_last_error = e.what();
+    return VC_ERR_INVALID_ARGUMENT;
+  } catch (const std::out_of_range& e) {
+    g_last_error = e.what();
+    return VC_ERR_OUT_OF_RANGE;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return VC_ERR_OOM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return VC_ERR_DEVICE;
+  }
+}
+
+void check_dim_bits(size_t dim, size_t bits) {
+  if (dim == 0 || dim % 8 != 0) fail(VC_ERR_INVALID_ARGUMENT, "dim must be a positive multiple of 8");
+  if (dim > vchip::kMaxDim) fail(VC_ERR_INVALID_ARGUMENT, "dim exceeds the maximum of 16384");
+  if (bits < 2 || bits > 4) fail(VC_ERR_INVALID_ARGUMENT, "bits must be 2, 3, or 4");
+}
+
+// Growable device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t need, hipStream_t s, bool keep = false) {
+    if (need <= bytes) return;
+    size_t nb = std::max(need, bytes + bytes / 2);
+    void* q = nullptr;
+    hip_check(hipMalloc(&q, nb), "hipMalloc");
+    if (keep && p != nullptr && bytes > 0) hip_check(hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, s), "copy");
+    if (p != nullptr) {
+      hip_check(hipStreamSynchronize(s), "sync");
+      hip_check(hipFree(p), "hipFree");
+    }
+    p = q;
+    bytes = nb;
+  }
+  void release() {
+    if (p != nullptr) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as(size_t byte_offset = 0) const {
+    return reinterpret_cast<T*>(static_cast<char*>(p) + byte_offset);
+  }
+};
+
+// Carves one scratch allocation into aligned sub-buffers.
+struct Carve {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    const size_t at = off;
+    off = (off + bytes + 255) & ~size_t(255);
+    return at;
+  }
+};
+
+// Device copies of the rotation tables, per (device, dim).
+struct DevRotation {
+  uint32_t* inv_perm = nullptr;
+  float* signs = nullptr;
+};
+std::mutex g_rot_mu;
+std::map<std::pair<int, size_t>, DevRotation> g_dev_rot;
+
+DevRotation device_rotation(size_t dim) {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard lock(g_rot_mu);
+  auto& r = g_dev_rot[{dev, dim}];
+  if (r.inv_perm == nullptr) {
+    const auto& rt = vchip::rotation_tables(dim);
+    hip_check(hipMalloc(&r.inv_perm, dim * 4), "hipMalloc");
+    hip_check(hipMalloc(&r.signs, dim * 4), "hipMalloc");
+    hip_check(hipMemcpy(r.inv_perm, rt.inv_perm.data(), dim * 4, hipMemcpyHostToDevice), "copy");
+    hip_check(hipMemcpy(r.signs, rt.signs.data(), dim * 4, hipMemcpyHostToDevice), "copy");
+  }
+  return r;
+}
+
+vchip::CodeTables make_tables(size_t dim, size_t bits) {
+  vchip::CodeTables t{};
+  const auto& rt = vchip::rotation_tables(dim);
+  const auto& cb = vchip::codebook(bits, dim);
+  std::fill(std::begin(t.boundaries), std::end(t.boundaries), 0.f);
+  std::fill(std::begin(t.centroids), std::end(t.centroids), 0.f);
+  std::copy(cb.boundaries.begin(), cb.boundaries.end(), t.boundaries);
+  std::copy(cb.centroids.begin(), cb.centroids.end(), t.centroids);
+  t.dim = int(dim);
+  t.bits = int(bits);
+  t.block = int(rt.block);
+  t.groups = int(vchip::byte_groups(bits, dim));
+  t.groups16 = (t.groups + 15) / 16;
+  t.inv_sqrt_block = rt.inv_sqrt_block;
+  const DevRotation dr = device_rotation(dim);
+  t.inv_perm = dr.inv_perm;
+  t.signs = dr.signs;
+  return t;
+}
+
+// Calibration from device-resident sample rows into device shift/scale/inv.
+void calibrate_device(const vchip::CodeTables& t, const float* sample_dev, size_t n, float* shift,
+                      float* scale, float* inv_scale, DevBuf& scratch, hipStream_t s) {
+  if (n < 2) fail(VC_ERR_INVALID_ARGUMENT, "calibration needs at least 2 rows");
+  const size_t D = size_t(t.dim);
+  Carve c;
+  const size_t o_rot = c.take(n * D * 4);
+  const size_t o_keys = c.take(n * D * 4);
+  scratch.reserve(c.off, s);
+  vchip::CodeTables rot_only = t;
+  rot_only.shift = rot_only.scale = rot_only.inv_scale = nullptr;
+  hip_check(vchip::launch_encode(rot_only, sample_dev, n, 0, nullptr, nullptr, scratch.as<float>(o_rot), nullptr, s),
+            "encode(rotate)");
+  const auto anchors = vchip::tq_anchors(size_t(t.bits), D, n);
+  hip_check(vchip::launch_calibrate(scratch.as<float>(o_rot), n, t.dim, anchors.lo_idx, anchors.hi_idx,
+                                    anchors.qc_lo, anchors.qc_span, scratch.as<uint32_t>(o_keys), shift, scale,
+                                    inv_scale, s),
+            "calibrate");
+}
+
+// Scan a tiled code set: queries (device) -> results (device).
+struct SearchOut {
+  size_t k_eff;
+};
+
+SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const float* alpha, size_t n,
+                       const float* q_dev, size_t nq, size_t k, const uint64_t* id_map_dev, float* out_s,
+                       uint64_t* out_i, DevBuf& scratch, hipStream_t s, uint8_t** luts_out = nullptr,
+                       size_t* luts_off = nullptr) {
+  SearchOut r{std::min(k, n)};
+  if (nq == 0 || r.k_eff == 0) return r;
+  if (r.k_eff > size_t(vchip::kMaxTopK))
+    fail(VC_ERR_INVALID_ARGUMENT, "k exceeds the GPU top-k capacity of 2048");
+  if (n > 0xFFFFFFFFull) fail(VC_ERR_INVALID_ARGUMENT, "index exceeds 2^32 vectors");
+  const int ke = int(r.k_eff);
+  const vchip::ScanPlan plan = vchip::plan_scan(t.groups16, n, nq, ke, g_forced_tile.load());
+  if (plan.tile == 0) fail(VC_ERR_INVALID_ARGUMENT, "no scan configuration fits LDS for this k / tile");
+  const size_t G = size_t(t.groups);
+  Carve c;
+  const size_t o_luts = c.take(nq * G * 32);
+  const size_t o_ls = c.take(nq * 4), o_lb = c.take(nq * 4), o_bc = c.take(nq * 4), o_bt = c.take(nq * 4);
+  const size_t parts = nq * size_t(plan.n_ranges);
+  const size_t o_ps = c.take(parts * ke * 4), o_pi = c.take(parts * ke * 4), o_pc = c.take(parts * 4);
+  scratch.reserve(c.off, s);
+  hip_check(vchip::launch_prepare_queries(t, q_dev, nq, scratch.as<uint8_t>(o_luts), scratch.as<float>(o_ls),
+                                          scratch.as<float>(o_lb), scratch.as<float>(o_bc),
+                                          scratch.as<float>(o_bt), s),
+            "prepare_queries");
+  std::unique_lock probe_lock(g_probe.mu);
+  const bool probe = g_probe.on;
+  if (!probe) probe_lock.unlock();
+  if (probe) hip_check(hipEventRecord(g_probe.start, s), "hipEventRecord");
+  hip_check(vchip::launch_scan(plan, tiled, alpha, n, t.groups, t.groups16, scratch.as<uint8_t>(o_luts),
+                               scratch.as<float>(o_ls), scratch.as<float>(o_bt), nq, ke, scratch.as<float>(o_ps),
+                               scratch.as<uint32_t>(o_pi), scratch.as<int>(o_pc), s),
+            "scan");
+  if (probe) {
+    hip_check(hipEventRecord(g_probe.stop, s), "hipEventRecord");
+    g_probe.recorded = true;
+    g_probe.tile = plan.tile;
+    g_probe.ranges = plan.n_ranges;
+    probe_lock.unlock();
+  }
+  hip_check(vchip::launch_merge_ranges(scratch.as<float>(o_ps), scratch.as<uint32_t>(o_pi), scratch.as<int>(o_pc),
+                                       nq, plan.n_ranges, ke, ke, id_map_dev, out_s, out_i, s),
+            "merge_ranges");
+  if (luts_out != nullptr) {
+    *luts_out = scratch.as<uint8_t>(0);
+    *luts_off = o_luts;
+  }
+  return r;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct vc_index {
+  size_t dim = 0, bits = 0;
+  vchip::CodeTables t{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevBuf calib;      // shift | scale | inv_scale (3 x dim floats)
+  bool calibrated = false;
+  std::vector<float> h_shift, h_scale;
+  DevBuf tiled;      // codes, tiled layout
+  DevBuf alpha;
+  size_t cap = 0;    // vector capacity of tiled/alpha (multiple of 64)
+  size_t n_encoded = 0;
+  DevBuf pending;    // staged raw vectors
+  size_t n_pending = 0;
+  bool prepared = false;
+  DevBuf scratch;
+  DevBuf io;         // host-call staging
+  std::mutex mu;
+};
+
+extern "C" {
+
+const char* vc_last_error(void) { return g_last_error.c_str(); }
+
+const char* vc_version(void) { return "vectorcache-hip 0.1.0 (gfx950)"; }
+
+vc_status vc_index_create(size_t dim, size_t bits, vc_index** out) {
+  return guarded([&] {
+    if (out == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "out is null");
+    check_dim_bits(dim, bits);
+    auto* ix = new vc_index();
+    try {
+      ix->dim = dim;
+      ix->bits = bits;
+      hip_check(hipGetDevice(&ix->device), "hipGetDevice");
+      hip_check(hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking), "hipStreamCreate");
+      ix->t = make_tables(dim, bits);
+      ix->h_shift.assign(dim, 0.f);
+      ix->h_scale.assign(dim, 1.f);
+    } catch (...) {
+      if (ix->stream) (void)hipStreamDestroy(ix->stream);
+      delete ix;
+      throw;
+    }
+    *out = ix;
+  });
+}
+
+vc_status vc_index_destroy(vc_index* ix) {
+  return guarded([&] {
+    if (ix == nullptr) return;
+    (void)hipStreamSynchronize(ix->stream);
+    ix->calib.release();
+    ix->tiled.release();
+    ix->alpha.release();
+    ix->pending.release();
+    ix->scratch.release();
+    ix->io.release();
+    (void)hipStreamDestroy(ix->stream);
+    delete ix;
+  });
+}
+
+static void index_calibrate(vc_index* ix, const float* sample_dev, size_t n) {
+  if (ix->n_encoded > 0) fail(VC_ERR_STATE, "calibrate() must precede prepare() of any vector");
+  const size_t D = ix->dim;
+  ix->calib.reserve(3 * D * 4, ix->stream);
+  float* sh = ix->calib.as<float>(0);
+  float* sc = ix->calib.as<float>(D * 4);
+  float* is = ix->calib.as<float>(2 * D * 4);
+  calibrate_device(ix->t, sample_dev, n, sh, sc, is, ix->scratch, ix->stream);
+  hip_check(hipMemcpyAsync(ix->h_shift.data(), sh, D * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+  hip_check(hipMemcpyAsync(ix->h_scale.data(), sc, D * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+  hip_check(hipStreamSynchronize(ix->stream), "sync");
+  ix->t.shift = sh;
+  ix->t.scale = sc;
+  ix->t.inv_scale = is;
+  ix->calibrated = true;
+  ix->prepared = false;
+}
+
+vc_status vc_index_calibrate(vc_index* ix, const float* sample, size_t n) {
+  return guarded([&] {
+    if (ix == nullptr || (sample == nullptr && n > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    if (n < 2) fail(VC_ERR_INVALID_ARGUMENT, "calibration needs at least 2 rows");
+    ix->io.reserve(n * ix->dim * 4, ix->stream);
+    hip_check(hipMemcpyAsync(ix->io.p, sample, n * ix->dim * 4, hipMemcpyHostToDevice, ix->stream), "copy");
+    index_calibrate(ix, ix->io.as<float>(), n);
+  });
+}
+
+vc_status vc_index_calibrate_device(vc_index* ix, const float* sample_dev, size_t n) {
+  return guarded([&] {
+    if (ix == nullptr || (sample_dev == nullptr && n > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    index_calibrate(ix, sample_dev, n);
+  });
+}
+
+static void index_add(vc_index* ix, const float* v, size_t n, hipMemcpyKind kind) {
+  if (n == 0) return;
+  const size_t row = ix->dim * 4;
+  ix->pending.reserve((ix->n_pending + n) * row, ix->stream, /*keep=*/true);
+  hip_check(hipMemcpyAsync(ix->pending.as<char>(ix->n_pending * row), v, n * row, kind, ix->stream), "copy");
+  if (kind == hipMemcpyHostToDevice) hip_check(hipStreamSynchronize(ix->stream), "sync");
+  ix->n_pending += n;
+  ix->prepared = false;
+}
+
+vc_status vc_index_add(vc_index* ix, const float* vectors, size_t n) {
+  return guarded([&] {
+    if (ix == nullptr || (vectors == nullptr && n > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    index_add(ix, vectors, n, hipMemcpyHostToDevice);
+  });
+}
+
+vc_status vc_index_add_device(vc_index* ix, const float* vectors_dev, size_t n) {
+  return guarded([&] {
+    if (ix == nullptr || (vectors_dev == nullptr && n > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    hip_check(hipDeviceSynchronize(), "sync caller work");  // caller's producer stream may differ
+    index_add(ix, vectors_dev, n, hipMemcpyDeviceToDevice);
+  });
+}
+
+vc_status vc_index_prepare(vc_index* ix) {
+  return guarded([&] {
+    if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");
+    std::lock_guard lock(ix->mu);
+    if (ix->n_pending > 0) {
+      const size_t total = ix->n_encoded + ix->n_pending;
+      if (total > 0xFFFFFFFFull) fail(VC_ERR_INVALID_ARGUMENT, "index exceeds 2^32 vectors");
+      if (total > ix->cap) {
+        size_t ncap = std::max(total, ix->cap + ix->cap / 2);
+        ncap = (ncap + 63) / 64 * 64;
+        ix->tiled.reserve(vchip::tiled_bytes(ncap, ix->t.groups16), ix->stream, /*keep=*/true);
+        ix->alpha.reserve(ncap * 4, ix->stream, /*keep=*/true);
+        ix->cap = ncap;
+      }
+      hip_check(vchip::launch_encode(ix->t, ix->pending.as<float>(), ix->n_pending, ix->n_encoded,
+                                     ix->tiled.as<uint8_t>(), nullptr, nullptr, ix->alpha.as<float>(), ix->stream),
+                "encode");
+      hip_check(hipStreamSynchronize(ix->stream), "sync");
+      ix->n_encoded = total;
+      ix->n_pending = 0;
+    }
+    ix->prepared = true;
+  });
+}
+
+static size_t index_search_dev(vc_index* ix, const float* q_dev, size_t nq, size_t k, float* s_dev,
+                               uint64_t* i_dev, hipStream_t s) {
+  if (!ix->prepared) fail(VC_ERR_STATE, "search() before prepare()");
+  const auto r = search_tiled(ix->t, ix->tiled.as<uint8_t>(), ix->alpha.as<float>(), ix->n_encoded, q_dev, nq, k,
+                              nullptr, s_dev, i_dev, ix->scratch, s);
+  return r.k_eff;
+}
+
+vc_status vc_index_search(vc_index* ix, const float* queries, size_t nq, size_t k, float* scores, uint64_t* ids,
+                          size_t* k_out) {
+  return guarded([&] {
+    if (ix == nullptr || (queries == nullptr && nq > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    if (!ix->prepared) fail(VC_ERR_STATE, "search() before prepare()");
+    const size_t ke = std::min(k, ix->n_encoded);
+    if (k_out) *k_out = ke;
+    if (nq == 0 || ke == 0) return;
+    if (scores == nullptr || ids == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null output");
+    Carve c;
+    const size_t oq = c.take(nq * ix->dim * 4), os = c.take(nq * ke * 4), oi = c.take(nq * ke * 8);
+    ix->io.reserve(c.off, ix->stream);
+    hip_check(hipMemcpyAsync(ix->io.as<float>(oq), queries, nq * ix->dim * 4, hipMemcpyHostToDevice, ix->stream),
+              "copy");
+    index_search_dev(ix, ix->io.as<float>(oq), nq, k, ix->io.as<float>(os), ix->io.as<uint64_t>(oi), ix->stream);
+    hip_check(hipMemcpyAsync(scores, ix->io.as<float>(os), nq * ke * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipMemcpyAsync(ids, ix->io.as<uint64_t>(oi), nq * ke * 8, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipStreamSynchronize(ix->stream), "sync");
+  });
+}
+
+vc_status vc_index_search_device(vc_index* ix, const float* queries_dev, size_t nq, size_t k, float* scores_dev,
+                                 uint64_t* ids_dev, size_t* k_out, void* stream) {
+  return guarded([&] {
+    if (ix == nullptr || (queries_dev == nullptr && nq > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ix->stream;
+    const size_t ke = index_search_dev(ix, queries_dev, nq, k, scores_dev, ids_dev, s);
+    if (k_out) *k_out = ke;
+  });
+}
+
+size_t vc_index_dim(const vc_index* ix) { return ix ? ix->dim : 0; }
+size_t vc_index_size(const vc_index* ix) { return ix ? ix->n_encoded + ix->n_pending : 0; }
+size_t vc_index_bits(const vc_index* ix) { return ix ? ix->bits : 0; }
+int vc_index_is_prepared(const vc_index* ix) { return ix && ix->prepared ? 1 : 0; }
+int vc_index_is_calibrated(const vc_index* ix) { return ix && ix->calibrated ? 1 : 0; }
+size_t vc_index_codes_bytes(const vc_index* ix) {
+  return ix ? ix->n_encoded * size_t(ix->t.groups) : 0;
+}
+size_t vc_index_index_bytes(const vc_index* ix) {
+  return ix ? vc_index_codes_bytes(ix) + ix->n_encoded * 4 : 0;
+}
+
+// ---- exports --------------------------------------------------------------
+static void export_flat_dev(vc_index* ix, uint8_t* flat_dev) {
+  hip_check(vchip::launch_tiled_to_flat(ix->tiled.as<uint8_t>(), ix->n_encoded, ix->t.groups, ix->t.groups16,
+                                        flat_dev, ix->stream),
+            "tiled_to_flat");
+}
+
+static void require_prepared(vc_index* ix) {
+  if (ix->n_pending > 0 || !ix->prepared) fail(VC_ERR_STATE, "export before prepare()");
+}
+
+vc_status vc_index_export_flat(vc_index* ix, uint8_t* flat_host) {
+  return guarded([&] {
+    if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");
+    std::lock_guard lock(ix->mu);
+    require_prepared(ix);
+    const size_t bytes = ix->n_encoded * size_t(ix->t.groups);
+    if (bytes == 0) return;
+    ix->io.reserve(bytes, ix->stream);
+    export_flat_dev(ix, ix->io.as<uint8_t>());
+    hip_check(hipMemcpyAsync(flat_host, ix->io.p, bytes, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipStreamSynchronize(ix->stream), "sync");
+  });
+}
+
+vc_status vc_index_export_perm0(vc_index* ix, uint8_t* perm0_host) {
+  return guarded([&] {
+    if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");
+    std::lock_guard lock(ix->mu);
+    require_prepared(ix);
+    const size_t n = ix->n_encoded, G = size_t(ix->t.groups);
+    if (n == 0) return;
+    const size_t pbytes = (n + 191) / 192 * G * 192;
+    Carve c;
+    const size_t of = c.take(n * G), op = c.take(pbytes);
+    ix->io.reserve(c.off, ix->stream);
+    export_flat_dev(ix, ix->io.as<uint8_t>(of));
+    hip_check(vchip::launch_flat_to_perm0(ix->io.as<uint8_t>(of), n, int(G), ix->io.as<uint8_t>(op), ix->stream),
+              "flat_to_perm0");
+    hip_check(hipMemcpyAsync(perm0_host, ix->io.as<uint8_t>(op), pbytes, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipStreamSynchronize(ix->stream), "sync");
+  });
+}
+
+vc_status vc_index_export_packed(vc_index* ix, uint8_t* packed_host) {
+  return guarded([&] {
+    if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");
+    std::lock_guard lock(ix->mu);
+    require_prepared(ix);
+    const size_t n = ix->n_encoded, G = size_t(ix->t.groups), bpr = ix->bits * ix->dim / 8;
+    if (n == 0) return;
+    Carve c;
+    const size_t of = c.take(n * G), op = c.take(n * bpr);
+    ix->io.reserve(c.off, ix->stream);
+    export_flat_dev(ix, ix->io.as<uint8_t>(of));
+    hip_check(vchip::launch_flat_to_packed(ix->io.as<uint8_t>(of), n, int(ix->dim), int(ix->bits),
+                                           ix->io.as<uint8_t>(op), ix->stream),
+              "flat_to_packed");
+    hip_check(hipMemcpyAsync(packed_host, ix->io.as<uint8_t>(op), n * bpr, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipStreamSynchronize(ix->stream), "sync");
+  });
+}
+
+vc_status vc_index_export_alpha(vc_index* ix, float* alpha_host) {
+  return guarded([&] {
+    if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");
+    std::lock_guard lock(ix->mu);
+    require_prepared(ix);
+    if (ix->n_encoded == 0) return;
+    hip_check(hipMemcpyAsync(alpha_host, ix->alpha.p, ix->n_encoded * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipStreamSynchronize(ix->stream), "sync");
+  });
+}
+
+vc_status vc_index_get_calibration(const vc_index* ix, float* shift_host, float* scale_host) {
+  return guarded([&] {
+    if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");
+    if (shift_host) std::memcpy(shift_host, ix->h_shift.data(), ix->dim * 4);
+    if (scale_host) std::memcpy(scale_host, ix->h_scale.data(), ix->dim * 4);
+  });
+}
+
+vc_status vc_index_prepare_queries(vc_index* ix, const float* queries, size_t nq, uint8_t* luts, float* lut_scale,
+                                   float* lut_bias, float* bias_corr) {
+  return guarded([&] {
+    if (ix == nullptr || (queries == nullptr && nq > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    if (nq == 0) return;
+    const size_t D = ix->dim, G = size_t(ix->t.groups);
+    Carve c;
+    const size_t oq = c.take(nq * D * 4), ol = c.take(nq * G * 32), os = c.take(nq * 4), ob = c.take(nq * 4),
+                 oc = c.take(nq * 4), ot = c.take(nq * 4);
+    ix->io.reserve(c.off, ix->stream);
+    hip_check(hipMemcpyAsync(ix->io.as<float>(oq), queries, nq * D * 4, hipMemcpyHostToDevice, ix->stream), "copy");
+    hip_check(vchip::launch_prepare_queries(ix->t, ix->io.as<float>(oq), nq, ix->io.as<uint8_t>(ol),
+                                            ix->io.as<float>(os), ix->io.as<float>(ob), ix->io.as<float>(oc),
+                                            ix->io.as<float>(ot), ix->stream),
+              "prepare_queries");
+    hip_check(hipMemcpyAsync(luts, ix->io.as<uint8_t>(ol), nq * G * 32, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipMemcpyAsync(lut_scale, ix->io.as<float>(os), nq * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipMemcpyAsync(lut_bias, ix->io.as<float>(ob), nq * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipMemcpyAsync(bias_corr, ix->io.as<float>(oc), nq * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipStreamSynchronize(ix->stream), "sync");
+  });
+}
+
+// ---- stateless entry points ------------------------------------------------
+vc_status vc_rotation_tables(size_t dim, uint32_t* perm, float* signs, float* inv_sqrt_block, size_t* block) {
+  return guarded([&] {
+    const auto& rt = vchip::rotation_tables(dim);
+    if (perm) std::memcpy(perm, rt.perm.data(), dim * 4);
+    if (signs) std::memcpy(signs, rt.signs.data(), dim * 4);
+    if (inv_sqrt_block) *inv_sqrt_block = rt.inv_sqrt_block;
+    if (block) *block = rt.block;
+  });
+}
+
+vc_status vc_codebook(size_t bits, size_t dim, float* boundaries, float* centroids) {
+  return guarded([&] {
+    const auto& cb = vchip::codebook(bits, dim);
+    if (boundaries) std::memcpy(boundaries, cb.boundaries.data(), cb.boundaries.size() * 4);
+    if (centroids) std::memcpy(centroids, cb.centroids.data(), cb.centroids.size() * 4);
+  });
+}
+
+namespace {
+struct Scoped {
+  hipStream_t s = nullptr;
+  DevBuf a, scratch;
+  Scoped() { hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"); }
+  ~Scoped() {
+    (void)hipStreamSynchronize(s);
+    a.release();
+    scratch.release();
+    (void)hipStreamDestroy(s);
+  }
+};
+}  // namespace
+
+vc_status vc_encode(const float* vectors, size_t n, size_t dim, size_t bits, const float* shift, const float* scale,
+                    uint8_t* packed_out, float* alpha_out) {
+  return guarded([&] {
+    check_dim_bits(dim, bits);
+    if (n == 0) return;
+    if (vectors == nullptr || packed_out == nullptr || alpha_out == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    if ((shift == nullptr) != (scale == nullptr)) fail(VC_ERR_INVALID_ARGUMENT, "shift and scale go together");
+    Scoped sc;
+    vchip::CodeTables t = make_tables(dim, bits);
+    const size_t bpr = bits * dim / 8;
+    Carve c;
+    const size_t ox = c.take(n * dim * 4), op = c.take(n * bpr), oa = c.take(n * 4), ocal = c.take(3 * dim * 4);
+    sc.a.reserve(c.off, sc.s);
+    hip_check(hipMemcpyAsync(sc.a.as<float>(ox), vectors, n * dim * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    if (shift != nullptr) {
+      std::vector<float> inv(dim);
+      for (size_t d = 0; d < dim; ++d) inv[d] = 1.f / scale[d];
+      hip_check(hipMemcpyAsync(sc.a.as<float>(ocal), shift, dim * 4, hipMemcpyHostToDevice, sc.s), "copy");
+      hip_check(hipMemcpyAsync(sc.a.as<float>(ocal + dim * 4), scale, dim * 4, hipMemcpyHostToDevice, sc.s), "copy");
+      hip_check(hipMemcpyAsync(sc.a.as<float>(ocal + 2 * dim * 4), inv.data(), dim * 4, hipMemcpyHostToDevice, sc.s),
+                "copy");
+      hip_check(hipStreamSynchronize(sc.s), "sync");
+      t.shift = sc.a.as<float>(ocal);
+      t.scale = sc.a.as<float>(ocal + dim * 4);
+      t.inv_scale = sc.a.as<float>(ocal + 2 * dim * 4);
+    }
+    hip_check(vchip::launch_encode(t, sc.a.as<float>(ox), n, 0, nullptr, sc.a.as<uint8_t>(op), nullptr,
+                                   sc.a.as<float>(oa), sc.s),
+              "encode");
+    hip_check(hipMemcpyAsync(packed_out, sc.a.as<uint8_t>(op), n * bpr, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipMemcpyAsync(alpha_out, sc.a.as<float>(oa), n * 4, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipStreamSynchronize(sc.s), "sync");
+  });
+}
+
+vc_status vc_fit_calibration(const float* sample, size_t n, size_t dim, size_t bits, float* shift_out,
+                             float* scale_out) {
+  return guarded([&] {
+    check_dim_bits(dim, bits);
+    if (n < 2) fail(VC_ERR_INVALID_ARGUMENT, "calibration needs at least 2 rows");
+    if (sample == nullptr || shift_out == nullptr || scale_out == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    Scoped sc;
+    vchip::CodeTables t = make_tables(dim, bits);
+    Carve c;
+    const size_t ox = c.take(n * dim * 4), ocal = c.take(3 * dim * 4);
+    sc.a.reserve(c.off, sc.s);
+    hip_check(hipMemcpyAsync(sc.a.as<float>(ox), sample, n * dim * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    float* sh = sc.a.as<float>(ocal);
+    calibrate_device(t, sc.a.as<float>(ox), n, sh, sh + dim, sh + 2 * dim, sc.scratch, sc.s);
+    hip_check(hipMemcpyAsync(shift_out, sh, dim * 4, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipMemcpyAsync(scale_out, sh + dim, dim * 4, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipStreamSynchronize(sc.s), "sync");
+  });
+}
+
+vc_status vc_search_perm0(const float* queries, size_t nq, size_t dim, size_t bits, size_t k, const uint8_t* perm0,
+                          const float* alpha, size_t n, const uint64_t* id_map, const float* shift, const float* scale,
+                          float* scores, uint64_t* ids, size_t* k_out) {
+  return guarded([&] {
+    check_dim_bits(dim, bits);
+    const size_t ke = std::min(k, n);
+    if (k_out) *k_out = ke;
+    if (nq == 0 || ke == 0) return;
+    if (queries == nullptr || perm0 == nullptr || alpha == nullptr || scores == nullptr || ids == nullptr)
+      fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    if ((shift == nullptr) != (scale == nullptr)) fail(VC_ERR_INVALID_ARGUMENT, "shift and scale go together");
+    Scoped sc;
+    vchip::CodeTables t = make_tables(dim, bits);
+    const size_t G = size_t(t.groups);
+    const size_t pbytes = (n + 191) / 192 * G * 192;
+    Carve c;
+    const size_t oq = c.take(nq * dim * 4), op = c.take(pbytes), of = c.take(n * G),
+                 ot = c.take(vchip::tiled_bytes(n, t.groups16)), oa = c.take(n * 4), om = c.take(id_map ? n * 8 : 0),
+                 os = c.take(nq * ke * 4), oi = c.take(nq * ke * 8), ocal = c.take(3 * dim * 4);
+    sc.a.reserve(c.off, sc.s);
+    hip_check(hipMemcpyAsync(sc.a.as<float>(oq), queries, nq * dim * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(hipMemcpyAsync(sc.a.as<uint8_t>(op), perm0, pbytes, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(hipMemcpyAsync(sc.a.as<float>(oa), alpha, n * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    if (id_map) hip_check(hipMemcpyAsync(sc.a.as<uint64_t>(om), id_map, n * 8, hipMemcpyHostToDevice, sc.s), "copy");
+    if (shift != nullptr) {
+      hip_check(hipMemcpyAsync(sc.a.as<float>(ocal), shift, dim * 4, hipMemcpyHostToDevice, sc.s), "copy");
+      hip_check(hipMemcpyAsync(sc.a.as<float>(ocal + dim * 4), scale, dim * 4, hipMemcpyHostToDevice, sc.s), "copy");
+      t.shift = sc.a.as<float>(ocal);
+      t.scale = sc.a.as<float>(ocal + dim * 4);
+      t.inv_scale = nullptr;  // not used by query preparation
+    }
+    hip_check(vchip::launch_perm0_to_flat(sc.a.as<uint8_t>(op), n, int(G), sc.a.as<uint8_t>(of), sc.s), "perm0_to_flat");
+    hip_check(vchip::launch_flat_to_tiled(sc.a.as<uint8_t>(of), n, int(G), t.groups16, sc.a.as<uint8_t>(ot), sc.s),
+              "flat_to_tiled");
+    search_tiled(t, sc.a.as<uint8_t>(ot), sc.a.as<float>(oa), n, sc.a.as<float>(oq), nq, k,
+                 id_map ? sc.a.as<uint64_t>(om) : nullptr, sc.a.as<float>(os), sc.a.as<uint64_t>(oi), sc.scratch, sc.s);
+    hip_check(hipMemcpyAsync(scores, sc.a.as<float>(os), nq * ke * 4, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipMemcpyAsync(ids, sc.a.as<uint64_t>(oi), nq * ke * 8, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipStreamSynchronize(sc.s), "sync");
+  });
+}
+
+vc_status vc_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t* dst_ids, size_t k_src,
+                           const float* src_scores, const uint64_t* src_ids) {
+  return guarded([&] {
+    if (nq == 0 || k_src == 0 || k_dst == 0) return;  // k_dst == 0: the caller takes src as is
+    if (!dst_scores || !dst_ids || !src_scores || !src_ids) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    if (k_dst + k_src > 8192) fail(VC_ERR_INVALID_ARGUMENT, "k_dst + k_src exceeds 8192");
+    Scoped sc;
+    Carve c;
+    const size_t ods = c.take(nq * k_dst * 4), odi = c.take(nq * k_dst * 8), oss = c.take(nq * k_src * 4),
+                 osi = c.take(nq * k_src * 8);
+    sc.a.reserve(c.off, sc.s);
+    hip_check(hipMemcpyAsync(sc.a.as<float>(ods), dst_scores, nq * k_dst * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(hipMemcpyAsync(sc.a.as<uint64_t>(odi), dst_ids, nq * k_dst * 8, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(hipMemcpyAsync(sc.a.as<float>(oss), src_scores, nq * k_src * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(hipMemcpyAsync(sc.a.as<uint64_t>(osi), src_ids, nq * k_src * 8, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(vchip::launch_merge_results(nq, int(k_dst), sc.a.as<float>(ods), sc.a.as<uint64_t>(odi), int(k_src),
+                                          sc.a.as<float>(oss), sc.a.as<uint64_t>(osi), sc.s),
+              "merge_results");
+    hip_check(hipMemcpyAsync(dst_scores, sc.a.as<float>(ods), nq * k_dst * 4, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipMemcpyAsync(dst_ids, sc.a.as<uint64_t>(odi), nq * k_dst * 8, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipStreamSynchronize(sc.s), "sync");
+  });
+}
+
+vc_status vc_set_scan_tile(size_t tile) {
+  return guarded([&] {
+    if (tile != 0 && tile != 1 && tile != 4 && tile != 8) fail(VC_ERR_INVALID_ARGUMENT, "tile must be 0, 1, 4 or 8");
+    g_forced_tile.store(int(tile));
+  });
+}
+
+vc_status vc_set_profiling(int on) {
+  return guarded([&] {
+    std::lock_guard lock(g_probe.mu);
+    if (on && g_probe.start == nullptr) {
+      hip_check(hipEventCreate(&g_probe.start), "hipEventCreate");
+      hip_check(hipEventCreate(&g_probe.stop), "hipEventCreate");
+    }
+    g_probe.on = on != 0;
+    g_probe.recorded = false;
+  });
+}
+
+vc_status vc_last_scan_stats(double* ms, size_t* tile, size_t* n_ranges) {
+  return guarded([&] {
+    std::lock_guard lock(g_probe.mu);
+    if (!g_probe.recorded) fail(VC_ERR_STATE, "no profiled scan recorded");
+    hip_check(hipEventSynchronize(g_probe.stop), "hipEventSynchronize");
+    float t = 0.f;
+    hip_check(hipEventElapsedTime(&t, g_probe.start, g_probe.stop), "hipEventElapsedTime");
+    if (ms) *ms = double(t);
+    if (tile) *tile = size_t(g_probe.tile);
+    if (n_ranges) *n_ranges = size_t(g_probe.ranges);
+  });
+}
+
+size_t vc_scan_tile_for(size_t dim, size_t bits, size_t nq) {
+  try {
+    const size_t G = vchip::byte_groups(bits, dim);
+    const auto p = vchip::plan_scan(int((G + 15) / 16), 1u << 20, nq, 10, g_forced_tile.load());
+    return size_t(p.tile);
+  } catch (...) {
+    return 0;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,48 @@
+// Host-side precompute for the TurboQuant path: rotation tables, Lloyd-Max
+// codebook, TQ+ anchor probabilities. Everything here is bit-identical to the
+// reference's host code (see vc_host.cpp for the per-function citations).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace vchip {
+
+inline constexpr std::size_t kFastScanBlock = 192;   // reference kBlock
+inline constexpr std::size_t kWindowGroups = 256;    // reference kFlushEvery
+inline constexpr std::size_t kMaxDim = 16384;
+inline constexpr float kMinInputNorm = 1e-10f;
+inline constexpr double kDegenerateInnerEps = 1e-12;
+
+struct RotationTables {
+  std::size_t dim = 0;
+  std::size_t block = 0;            // largest power of two dividing dim
+  float inv_sqrt_block = 0.f;
+  std::vector<std::uint32_t> perm;  // y[i] = x[perm[i]] ...
+  std::vector<std::uint32_t> inv_perm;
+  std::vector<float> signs;
+};
+
+struct Codebook {
+  std::vector<float> boundaries;  // 2^b - 1
+  std::vector<float> centroids;   // 2^b
+};
+
+// Throws std::invalid_argument on bad dim.
+const RotationTables& rotation_tables(std::size_t dim);
+// Throws std::invalid_argument on bad bits/dim. Memoized, thread-safe.
+const Codebook& codebook(std::size_t bits, std::size_t dim);
+// I_t(a, a), the regularized incomplete beta function.
+double beta_cdf_symmetric(double a, double t);
+
+struct TqAnchors {
+  float qc_lo, qc_span;     // -c_outer, 2 c_outer
+  std::size_t lo_idx, hi_idx;
+};
+// Order-statistic ranks and target span for TQ+ from n sample rows.
+TqAnchors tq_anchors(std::size_t bits, std::size_t dim, std::size_t n);
+
+inline std::size_t byte_groups(std::size_t bits, std::size_t dim) { return dim / (8 / bits); }
+
+}  // namespace vchip

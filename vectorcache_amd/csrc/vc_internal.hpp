@@ -1,0 +1,99 @@
+// Internal launch interface between the C ABI (vc_capi.cpp) and the gfx950
+// kernels (vc_kernels.hip). Not installed; not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace vchip {
+
+// Device-side constant tables of one (dim, bits) configuration.
+struct CodeTables {
+  float boundaries[15];
+  float centroids[16];
+  int dim, bits, block, groups, groups16;  // groups16 = ceil(groups / 16)
+  float inv_sqrt_block;
+  const std::uint32_t* inv_perm;  // device, dim
+  const float* signs;             // device, dim (indexed by destination)
+  const float* shift;             // device, dim; null = identity calibration
+  const float* scale;             // device, dim
+  const float* inv_scale;         // device, dim
+};
+
+// Internal scan layout ("tiled"): for vector v and byte group g the flat byte
+// (hi nibble = first code of the group) lives at
+//   ((v / 64) * groups16 + g / 16) * 1024 + (v % 64) * 16 + g % 16,
+// i.e. 64-vector tiles, each lane's 16 consecutive groups contiguous, so one
+// 16-byte load per lane covers 16 groups and a wavefront reads 1 KiB
+// contiguously. Groups >= `groups` in the last 16-chunk are zero.
+inline std::size_t tiled_bytes(std::size_t n_vectors_cap, int groups16) {
+  return ((n_vectors_cap + 63) / 64) * std::size_t(groups16) * 1024;
+}
+
+// encode: norm -> rotation -> quantize -> alpha -> codes.
+// Outputs may be null: tiled (written at vector offset `out_offset`),
+// packed (bit-planes, row-major n x bits*dim/8), rotated (n x dim floats:
+// the calibration path), alpha (written at out_offset).
+hipError_t launch_encode(const CodeTables& t, const float* x, std::size_t n, std::size_t out_offset,
+                         std::uint8_t* tiled, std::uint8_t* packed, float* rotated, float* alpha,
+                         hipStream_t s);
+
+// TQ+ per-dimension order statistics of rotated sample rows (n x dim):
+// keys -> radix select -> shift/scale.
+hipError_t launch_calibrate(const float* rotated, std::size_t n, int dim, std::size_t lo_idx,
+                            std::size_t hi_idx, float qc_lo, float qc_span, std::uint32_t* keys_scratch,
+                            float* shift, float* scale, float* inv_scale, hipStream_t s);
+
+// Query preparation: per query rotation (+ TQ+), LUT build and quantization.
+// luts: nq x groups*32 u8. bias_total = lut_bias + bias_corr.
+hipError_t launch_prepare_queries(const CodeTables& t, const float* q, std::size_t nq,
+                                  std::uint8_t* luts, float* lut_scale, float* lut_bias,
+                                  float* bias_corr, float* bias_total, hipStream_t s);
+
+struct ScanPlan {
+  int tile;          // queries per workgroup (1, 4 or 8)
+  int n_ranges;      // vector ranges (workgroups per query tile)
+  int range_len;     // vectors per range (multiple of 64)
+  int lut_chunk16;   // 16-group chunks of LUT resident in LDS (all groups if it fits)
+  int cand_cap;      // candidate buffer per query
+  std::size_t lds_bytes;
+};
+
+// Plan for nq queries over n vectors with top-k (k <= kMaxTopK).
+inline constexpr int kMaxTopK = 2048;
+ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int forced_tile);
+
+// Scan + per-range exact top-k. Partial lists: part_s/part_i [nq][n_ranges][k],
+// part_cnt [nq][n_ranges].
+hipError_t launch_scan(const ScanPlan& p, const std::uint8_t* tiled, const float* alpha,
+                       std::size_t n, int groups, int groups16, const std::uint8_t* luts,
+                       const float* lut_scale, const float* bias_total, std::size_t nq, int k,
+                       float* part_s, std::uint32_t* part_i, int* part_cnt, hipStream_t s);
+
+// Merge per-range lists into final rows of k_out entries (padded with 0/0).
+// id_map (device, may be null) maps vector positions to reported ids.
+hipError_t launch_merge_ranges(const float* part_s, const std::uint32_t* part_i, const int* part_cnt,
+                               std::size_t nq, int n_ranges, int k, int k_out,
+                               const std::uint64_t* id_map, float* out_s, std::uint64_t* out_i,
+                               hipStream_t s);
+
+// merge_search_results on device buffers.
+hipError_t launch_merge_results(std::size_t nq, int k_dst, float* dst_s, std::uint64_t* dst_i,
+                                int k_src, const float* src_s, const std::uint64_t* src_i,
+                                hipStream_t s);
+
+// Format conversions.
+hipError_t launch_tiled_to_flat(const std::uint8_t* tiled, std::size_t n, int groups, int groups16,
+                                std::uint8_t* flat, hipStream_t s);
+hipError_t launch_flat_to_tiled(const std::uint8_t* flat, std::size_t n, int groups, int groups16,
+                                std::uint8_t* tiled, hipStream_t s);
+hipError_t launch_flat_to_perm0(const std::uint8_t* flat, std::size_t n, int groups,
+                                std::uint8_t* perm0, hipStream_t s);
+hipError_t launch_perm0_to_flat(const std::uint8_t* perm0, std::size_t n, int groups,
+                                std::uint8_t* flat, hipStream_t s);
+hipError_t launch_flat_to_packed(const std::uint8_t* flat, std::size_t n, int dim, int bits,
+                                 std::uint8_t* packed, hipStream_t s);
+
+}  // namespace vchip

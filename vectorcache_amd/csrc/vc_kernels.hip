@@ -1,0 +1,1053 @@
+// gfx950 kernels of the TurboQuant encode + search path.
+//
+// Compiled with -ffp-contract=off: every fused multiply-add that the
+// reference CPU path performs is an explicit __fmaf_rn / fma here, every
+// other multiply and add is rounded on its own, so scores, alpha and LUTs are
+// bit-identical to the reference's AVX2 PERM0 path.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "vc_internal.hpp"
+
+namespace vchip {
+namespace {
+
+constexpr float kMinNorm = 1e-10f;
+constexpr int kCandCap = 512;   // per-query candidate buffer of the scan
+constexpr int kScanThreads = 1024;
+constexpr int kLdsBudget = 160 * 1024;
+
+// ---------------------------------------------------------------------------
+// Shared device helpers
+// ---------------------------------------------------------------------------
+
+// Rotation of one row held in LDS by a single wavefront (one-round SRHT of
+// the reference, src/transform/rotation.cpp:38-121,185-205):
+//   y[i] = (x[perm[i]] * inv) * sign[i]      (scattered through inv_perm)
+//   per block of B: butterflies len = 1 .. B/2, (a + b, a - b)
+//   y *= 1 / sqrt(B)
+// The butterfly pairs and their order are the reference's, so the float
+// result is identical however lanes share the work.
+__device__ void rotate_row_lds(const CodeTables& t, const float* __restrict__ xr, float inv,
+                               float* __restrict__ y, int lane) {
+  const int D = t.dim;
+  for (int j = lane; j < D; j += 64) {
+    const int i = (int)t.inv_perm[j];
+    y[i] = (xr[j] * inv) * t.signs[i];
+  }
+  __syncthreads();
+  const int B = t.block, halfB = B >> 1, npairs = D >> 1;
+  for (int len = 1; len < B; len <<= 1) {
+    for (int p = lane; p < npairs; p += 64) {
+      const int blk = p / halfB, q = p - blk * halfB;
+      const int i = blk * B + (q / len) * 2 * len + (q % len);
+      const float u = y[i], v = y[i + len];
+      y[i] = u + v;
+      y[i + len] = u - v;
+    }
+    __syncthreads();
+  }
+  for (int i = lane; i < D; i += 64) y[i] = y[i] * t.inv_sqrt_block;
+  __syncthreads();
+}
+
+// Row norm of the reference (src/encode/encode.cpp:16-26): 8 interleaved
+// chains c[j % 8] += x_j^2 (fused), tree ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)).
+// Called with lanes grouped by 8: lane & 7 is the chain. Returns sqrtf.
+__device__ float norm8(const float* __restrict__ xr, int D, int chain, bool active) {
+  float c = 0.f;
+  if (active)
+    for (int j = chain; j < D; j += 8) c = __fmaf_rn(xr[j], xr[j], c);
+  c = c + __shfl_xor(c, 1);
+  c = c + __shfl_xor(c, 2);
+  c = c + __shfl_xor(c, 4);
+  return __fsqrt_rn(c);
+}
+
+// ---------------------------------------------------------------------------
+// Encode (src/encode/encode.cpp:112-231): one wavefront per workgroup, rows
+// in batches of 8 (the norm phase keeps all 64 lanes busy as 8 rows x 8
+// chains). Alpha uses the reference's 4 double chains in element order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void encode_kernel(CodeTables t, const float* __restrict__ x,
+                                                    long n, long out_offset,
+                                                    uint8_t* __restrict__ tiled,
+                                                    uint8_t* __restrict__ packed,
+                                                    float* __restrict__ rotated,
+                                                    float* __restrict__ alpha) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int D = t.dim;
+  float* y = reinterpret_cast<float*>(smem);
+  uint8_t* code = smem + (size_t)D * 4;
+  const int lane = threadIdx.x;
+  const bool quant = tiled != nullptr || packed != nullptr || alpha != nullptr;
+  const int nbounds = (1 << t.bits) - 1;
+
+  for (long batch = blockIdx.x; batch * 8 < n; batch += gridDim.x) {
+    const long r8 = batch * 8 + (lane >> 3);
+    const float nrm_mine = norm8(x + (r8 < n ? r8 : 0) * (long)D, D, lane & 7, r8 < n);
+    const int rows = (int)((n - batch * 8) < 8 ? (n - batch * 8) : 8);
+    for (int rr = 0; rr < rows; ++rr) {
+      const long row = batch * 8 + rr;
+      const float nv = __shfl(nrm_mine, rr * 8);
+      const float inv = nv > kMinNorm ? __fdiv_rn(1.f, nv) : 0.f;
+      __syncthreads();
+      rotate_row_lds(t, x + row * (long)D, inv, y, lane);
+      if (rotated != nullptr)
+        for (int i = lane; i < D; i += 64) rotated[row * (long)D + i] = y[i];
+      if (!quant) continue;
+      // quantize: code = #{boundaries b : value > b} (strict)
+      for (int i = lane; i < D; i += 64) {
+        const float v = y[i];
+        const float c = t.shift ? (v + t.shift[i]) * t.scale[i] : v;
+        unsigned k = 0;
+        for (int b = 0; b < nbounds; ++b) k += c > t.boundaries[b] ? 1u : 0u;
+        code[i] = (uint8_t)k;
+      }
+      __syncthreads();
+      // alpha = norm / (float) inner, inner = (d0 + d1) + (d2 + d3) over 4
+      // sequential double chains d[j % 4] (fused multiply-add).
+      double d = 0.0;
+      if (lane < 4) {
+        for (int j = lane; j < D; j += 4) {
+          const double yv = (double)y[j];
+          const int k = code[j];
+          if (t.shift == nullptr) {
+            d = fma(yv, (double)t.centroids[k], d);
+          } else {
+            const double rec = fma((double)t.centroids[k], (double)t.inv_scale[j], -(double)t.shift[j]);
+            d = fma(yv, rec, d);
+          }
+        }
+      }
+      const double pair = d + __shfl_xor(d, 1);
+      const double inner = pair + __shfl_xor(pair, 2);
+      const long v = out_offset + row;
+      if (lane == 0 && alpha != nullptr) alpha[v] = inner > 1e-12 ? __fdiv_rn(nv, (float)inner) : 0.f;
+      if (tiled != nullptr) {
+        const int G = t.groups, G16 = t.groups16;
+        for (int c16 = lane; c16 < G16; c16 += 64) {
+          uint32_t w[4] = {0u, 0u, 0u, 0u};
+          for (int e = 0; e < 16; ++e) {
+            const int g = c16 * 16 + e;
+            uint32_t byte = 0;
+            if (g < G) {
+              if (t.bits == 2) {
+                const uint8_t* c4 = code + g * 4;
+                byte = (uint32_t(c4[0]) << 6) | (uint32_t(c4[1]) << 4) | (uint32_t(c4[2]) << 2) | c4[3];
+              } else {
+                byte = (uint32_t(code[2 * g]) << 4) | code[2 * g + 1];
+              }
+            }
+            w[e >> 2] |= byte << ((e & 3) * 8);
+          }
+          uint4* dst = reinterpret_cast<uint4*>(tiled + ((v >> 6) * G16 + c16) * 1024 + (v & 63) * 16);
+          *dst = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+      }
+      if (packed != nullptr) {
+        const int bpp = D / 8;
+        for (int c8 = lane; c8 < bpp; c8 += 64) {
+          for (int p = 0; p < t.bits; ++p) {
+            uint32_t byte = 0;
+            for (int kk = 0; kk < 8; ++kk) byte |= ((uint32_t(code[c8 * 8 + kk]) >> p) & 1u) << (7 - kk);
+            packed[row * (long)(t.bits * bpp) + p * bpp + c8] = (uint8_t)byte;
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// TQ+ calibration (src/encode/encode.cpp:142-150,282-326)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t f32_key(float x) {
+  const uint32_t b = __float_as_uint(x);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float f32_unkey(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// rotated (n x D row-major) -> keys (D x n), 32x32 tiles through LDS.
+__global__ __launch_bounds__(256) void keys_transpose_kernel(const float* __restrict__ rot, long n,
+                                                             int D, uint32_t* __restrict__ keys) {
+  __shared__ uint32_t tile[32][33];
+  const long r0 = (long)blockIdx.y * 32;
+  const int d0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int i = ty; i < 32; i += 8) {
+    const long r = r0 + i;
+    const int d = d0 + tx;
+    if (r < n && d < D) tile[i][tx] = f32_key(rot[r * D + d]);
+  }
+  __syncthreads();
+  for (int i = ty; i < 32; i += 8) {
+    const int d = d0 + i;
+    const long r = r0 + tx;
+    if (r < n && d < D) keys[(long)d * n + r] = tile[tx][i];
+  }
+}
+
+// Per dimension: the lo_idx-th and hi_idx-th smallest keys by 4-pass radix
+// select (8 bits per pass, both ranks per pass), then the TQ+ shift/scale.
+__global__ __launch_bounds__(256) void radix_select_kernel(const uint32_t* __restrict__ keys, long n,
+                                                           long lo_idx, long hi_idx, float qc_lo,
+                                                           float qc_span, float* __restrict__ shift,
+                                                           float* __restrict__ scale,
+                                                           float* __restrict__ inv_scale) {
+  __shared__ uint32_t hist[2][256];
+  __shared__ uint32_t prefix[2];
+  __shared__ unsigned long long rank[2];
+  const int d = blockIdx.x;
+  const uint32_t* col = keys + (long)d * n;
+  if (threadIdx.x == 0) {
+    prefix[0] = prefix[1] = 0u;
+    rank[0] = (unsigned long long)lo_idx;
+    rank[1] = (unsigned long long)hi_idx;
+  }
+  uint32_t mask = 0u;
+  for (int sh = 24; sh >= 0; sh -= 8) {
+    for (int i = threadIdx.x; i < 512; i += 256) (&hist[0][0])[i] = 0u;
+    __syncthreads();
+    const uint32_t p0 = prefix[0], p1 = prefix[1];
+    for (long i = threadIdx.x; i < n; i += 256) {
+      const uint32_t k = col[i];
+      const uint32_t b = (k >> sh) & 255u;
+      if ((k & mask) == p0) atomicAdd(&hist[0][b], 1u);
+      if ((k & mask) == p1) atomicAdd(&hist[1][b], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+      const int w = threadIdx.x;
+      unsigned long long r = rank[w], below = 0;
+      for (int b = 0; b < 256; ++b) {
+        const unsigned long long h = hist[w][b];
+        if (below + h > r) {
+          prefix[w] |= uint32_t(b) << sh;
+          rank[w] = r - below;
+          break;
+        }
+        below += h;
+      }
+    }
+    __syncthreads();
+    mask |= 255u << sh;
+  }
+  if (threadIdx.x == 0) {
+    const float q_lo = f32_unkey(prefix[0]);
+    const float q_hi = f32_unkey(prefix[1]);
+    const float span = q_hi - q_lo;
+    float s = 1.f, sft = 0.f;
+    if (span > 1e-12f && qc_span > 0.f) {
+      s = __fdiv_rn(qc_span, span);
+      sft = __fdiv_rn(qc_lo, s) - q_lo;
+    }
+    shift[d] = sft;
+    scale[d] = s;
+    inv_scale[d] = __fdiv_rn(1.f, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Query preparation (src/search/search.cpp:149-219,762-860): one wavefront
+// per query.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float lut_value(const float* __restrict__ q, const CodeTables& t, int base,
+                                           int cpn, unsigned nib) {
+  const unsigned mask = (1u << t.bits) - 1u;
+  float s = 0.f;
+  for (int c = 0; c < cpn; ++c) {
+    const unsigned code = (nib >> ((cpn - 1 - c) * t.bits)) & mask;
+    s = __fmaf_rn(q[base + c], t.centroids[code], s);
+  }
+  return s;
+}
+
+__global__ __launch_bounds__(64) void prepare_queries_kernel(CodeTables t, const float* __restrict__ q,
+                                                             long nq, uint8_t* __restrict__ luts,
+                                                             float* __restrict__ lut_scale,
+                                                             float* __restrict__ lut_bias,
+                                                             float* __restrict__ bias_corr,
+                                                             float* __restrict__ bias_total) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int D = t.dim, G = t.groups;
+  float* y = reinterpret_cast<float*>(smem);
+  float* mins = y + D;  // 2 * G
+  __shared__ float s_bias, s_bc;
+  const int lane = threadIdx.x;
+  const int cpb = 8 / t.bits, cpn = cpb / 2;
+  for (long qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+    const float* xr = q + qi * (long)D;
+    const float nrm = __shfl(norm8(xr, D, lane & 7, lane < 8), 0);
+    const float inv = nrm > kMinNorm ? __fdiv_rn(1.f, nrm) : 0.f;
+    __syncthreads();
+    rotate_row_lds(t, xr, inv, y, lane);
+    if (t.shift != nullptr) {
+      // bias_corr = (float) -sum (double) y_d * shift_d, sequential in d
+      if (lane == 0) {
+        double acc = 0.0;
+        for (int d = 0; d < D; ++d) acc = fma(-(double)y[d], (double)t.shift[d], acc);
+        s_bc = (float)acc;
+      }
+      __syncthreads();
+      for (int d = lane; d < D; d += 64) y[d] = __fdiv_rn(y[d], t.scale[d]);
+    } else if (lane == 0) {
+      s_bc = 0.f;
+    }
+    __syncthreads();
+    // pass 1: per group the min/max of both 16-entry halves; global span
+    float span = 0.f;
+    for (int g = lane; g < G; g += 64) {
+      for (int half = 0; half < 2; ++half) {
+        float mn = FLT_MAX, mx = -FLT_MAX;
+        for (unsigned nib = 0; nib < 16; ++nib) {
+          const float s = lut_value(y, t, g * cpb + half * cpn, cpn, nib);
+          mn = s < mn ? s : mn;
+          mx = s > mx ? s : mx;
+        }
+        mins[2 * g + half] = mn;
+        span = fmaxf(span, mx - mn);
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) span = fmaxf(span, __shfl_xor(span, off));
+    __syncthreads();
+    if (lane == 0) {
+      float b = 0.f;  // sequential over groups, as the reference
+      for (int g = 0; g < G; ++g) b = b + (mins[2 * g] + mins[2 * g + 1]);
+      s_bias = b;
+    }
+    __syncthreads();
+    float scale = span > 0.f ? __fdiv_rn(span, 127.f) : 1.f;
+    float iscale = 1.f;
+    if (scale >= FLT_MIN)
+      iscale = __fdiv_rn(1.f, scale);
+    else
+      scale = 1.f;
+    // pass 2: quantized entries
+    uint8_t* out = luts + qi * (long)G * 32;
+    for (int g = lane; g < G; g += 64) {
+      uint32_t w[8];
+      for (int half = 0; half < 2; ++half) {
+        const float mn = mins[2 * g + half];
+        for (int e = 0; e < 16; e += 4) {
+          uint32_t word = 0;
+          for (int b = 0; b < 4; ++b) {
+            const float s = lut_value(y, t, g * cpb + half * cpn, cpn, (unsigned)(e + b));
+            float r = roundf((s - mn) * iscale);
+            r = r < 0.f ? 0.f : (r > 127.f ? 127.f : r);
+            word |= uint32_t(r) << (8 * b);
+          }
+          w[half * 4 + e / 4] = word;
+        }
+      }
+      uint4* dst = reinterpret_cast<uint4*>(out + (long)g * 32);
+      dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+      dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    }
+    if (lane == 0) {
+      lut_scale[qi] = scale;
+      lut_bias[qi] = s_bias;
+      bias_corr[qi] = s_bc;
+      bias_total[qi] = s_bias + s_bc;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LUT scan with exact per-range top-k (src/search/search_avx2.cpp:95-222,
+// src/search/search.cpp:600-758).
+//
+// Workgroup = 1024 threads, one vector per thread per step, T queries per
+// workgroup. The T queries' LUTs sit in LDS interleaved by query:
+// entry (g, e, t) at byte (g * 32 + e) * T + t, so one 32-bit (T = 4) or
+// 64-bit (T = 8) LDS read fetches a nibble's entry for all T queries. Byte
+// sums are accumulated SWAR in packed u16 lanes (exact: a 256-group window sum
+// is <= 256 * 254 = 65024), flushed per window into the float chain
+// fa = fmaf(scale, (float) A_w, fa), score = fa * alpha.
+// Top-k per query: candidates that beat the current k-th best under
+// (score desc, id asc) go to an LDS buffer, merged by rank into the sorted
+// list when it fills. Exact; ties resolve to the lower id.
+// ---------------------------------------------------------------------------
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ bool better(float s, uint32_t id, float ts, uint32_t ti) {
+  return s > ts || (s == ts && id < ti);
+}
+
+struct TopkShared {
+  float* top_s;     // [T][k]
+  uint32_t* top_i;  // [T][k]
+  float* nw_s;      // [T][k]
+  uint32_t* nw_i;   // [T][k]
+  float* cand_s;    // [T][cap]
+  uint32_t* cand_i; // [T][cap]
+  int* cnt;         // [T]
+  int* m;           // [T]
+  float* thr_s;     // [T]
+  uint32_t* thr_i;  // [T]
+};
+
+// Merge query t's candidates into its sorted list (all threads participate).
+__device__ void topk_merge(const TopkShared& sh, int t, int k, int cap) {
+  const int tid = threadIdx.x;
+  const int c = min(sh.cnt[t], cap);
+  const int m = sh.m[t];
+  float* ts = sh.top_s + t * k;
+  uint32_t* ti = sh.top_i + t * k;
+  float* ns = sh.nw_s + t * k;
+  uint32_t* ni = sh.nw_i + t * k;
+  const float* cs = sh.cand_s + t * cap;
+  const uint32_t* ci = sh.cand_i + t * cap;
+  for (int e = tid; e < c; e += blockDim.x) {
+    const float s = cs[e];
+    const uint32_t id = ci[e];
+    int lo = 0, hi = m;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (better(ts[mid], ti[mid], s, id)) lo = mid + 1; else hi = mid;
+    }
+    int rank = lo;
+    for (int f = 0; f < c && rank < k; ++f) rank += better(cs[f], ci[f], s, id) ? 1 : 0;
+    if (rank < k) { ns[rank] = s; ni[rank] = id; }
+  }
+  for (int j = tid; j < m; j += blockDim.x) {
+    const float s = ts[j];
+    const uint32_t id = ti[j];
+    int rank = j;
+    for (int f = 0; f < c && rank < k; ++f) rank += better(cs[f], ci[f], s, id) ? 1 : 0;
+    if (rank < k) { ns[rank] = s; ni[rank] = id; }
+  }
+  __syncthreads();
+  const int m2 = min(k, m + c);
+  for (int j = tid; j < m2; j += blockDim.x) { ts[j] = ns[j]; ti[j] = ni[j]; }
+  __syncthreads();
+  if (tid == 0) {
+    sh.m[t] = m2;
+    sh.cnt[t] = 0;
+    if (m2 == k) { sh.thr_s[t] = ts[k - 1]; sh.thr_i[t] = ti[k - 1]; }
+  }
+  __syncthreads();
+}
+
+template <int T>
+struct LutWord;
+template <> struct LutWord<1> { typedef uint8_t type; };
+template <> struct LutWord<4> { typedef uint32_t type; };
+template <> struct LutWord<8> { typedef uint2 type; };
+
+template <int T>
+__global__ __launch_bounds__(kScanThreads) void scan_topk_kernel(
+    const uint8_t* __restrict__ tiled, const float* __restrict__ alpha, long n, int G, int G16,
+    int chunk16, const uint8_t* __restrict__ luts, const float* __restrict__ lut_scale,
+    const float* __restrict__ bias_total, long nq, int k, int cap, int n_ranges, long range_len,
+    float* __restrict__ part_s, uint32_t* __restrict__ part_i, int* __restrict__ part_cnt) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x;
+  const long n_qt = (nq + T - 1) / T;
+  const long L = blockIdx.x;
+  const int r = (int)(L / n_qt);   // range-major: concurrent workgroups share codes
+  const long qt = L % n_qt;
+  const long q0 = qt * T;
+  const long r_begin = (long)r * range_len;
+  const long r_end = min(n, r_begin + range_len);
+
+  // ---- LDS carve-up ----
+  const size_t lut_bytes = (size_t)chunk16 * 16 * 32 * T;
+  uint8_t* lut_s = smem;
+  size_t off = (lut_bytes + 15) & ~size_t(15);
+  TopkShared sh;
+  sh.top_s = reinterpret_cast<float*>(smem + off); off += (size_t)T * k * 4;
+  sh.top_i = reinterpret_cast<uint32_t*>(smem + off); off += (size_t)T * k * 4;
+  sh.nw_s = reinterpret_cast<float*>(smem + off); off += (size_t)T * k * 4;
+  sh.nw_i = reinterpret_cast<uint32_t*>(smem + off); off += (size_t)T * k * 4;
+  sh.cand_s = reinterpret_cast<float*>(smem + off); off += (size_t)T * cap * 4;
+  sh.cand_i = reinterpret_cast<uint32_t*>(smem + off); off += (size_t)T * cap * 4;
+  sh.cnt = reinterpret_cast<int*>(smem + off); off += T * 4;
+  sh.m = reinterpret_cast<int*>(smem + off); off += T * 4;
+  sh.thr_s = reinterpret_cast<float*>(smem + off); off += T * 4;
+  sh.thr_i = reinterpret_cast<uint32_t*>(smem + off); off += T * 4;
+
+  if (tid < T) {
+    sh.cnt[tid] = 0;
+    sh.m[tid] = 0;
+    sh.thr_s[tid] = -INFINITY;
+    sh.thr_i[tid] = 0u;
+  }
+  float scl[T], bias[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    const long q = q0 + t < nq ? q0 + t : nq - 1;
+    scl[t] = lut_scale[q];
+    bias[t] = bias_total[q];
+  }
+
+  // Fill the LDS LUT for 16-group chunks [c0, c0 + chunk16): interleave the
+  // T queries; groups >= G (padding of the last chunk) are zero.
+  auto load_lut = [&](int c0) {
+    const int g0 = c0 * 16;
+    const int ng = min(chunk16, G16 - c0) * 16;
+    const int words = ng * 32 * T / 4;
+    uint32_t* dst = reinterpret_cast<uint32_t*>(lut_s);
+    for (int w = tid; w < words; w += blockDim.x) {
+      uint32_t word = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int byte = w * 4 + b;
+        const int t = byte % T;
+        const int e = (byte / T) % 32;
+        const int g = byte / (T * 32) + g0;
+        const long q = q0 + t;
+        uint32_t val = 0;
+        if (g < G && q < nq) val = luts[q * (long)G * 32 + (long)g * 32 + e];
+        word |= val << (8 * b);
+      }
+      dst[w] = word;
+    }
+  };
+  const bool one_chunk = chunk16 >= G16;
+  if (one_chunk) load_lut(0);
+  __syncthreads();
+
+  typedef typename LutWord<T>::type W;
+  for (long base = r_begin; base < r_end; base += blockDim.x) {
+    const long v = base + tid;
+    const bool valid = v < r_end;
+    const uint4* cp = reinterpret_cast<const uint4*>(tiled) + ((valid ? v : 0) >> 6) * (long)G16 * 64 +
+                      ((valid ? v : 0) & 63);
+    float fa[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) fa[t] = bias[t];
+
+    for (int c0 = 0; c0 < G16; c0 += chunk16) {
+      if (!one_chunk) {
+        __syncthreads();
+        load_lut(c0);
+        __syncthreads();
+      }
+      const int c1 = min(G16, c0 + chunk16);
+      // accumulators of the current window
+      uint32_t accA[T >= 4 ? T / 4 : 1], accB[T >= 4 ? T / 4 : 1];
+#pragma unroll
+      for (int h = 0; h < (T >= 4 ? T / 4 : 1); ++h) { accA[h] = 0u; accB[h] = 0u; }
+      for (int g16 = c0; g16 < c1; ++g16) {
+        const uint4 c = valid ? cp[(long)g16 * 64] : make_uint4(0u, 0u, 0u, 0u);
+        const W* lw = reinterpret_cast<const W*>(lut_s) + (long)(g16 - c0) * 16 * 32;
+        const uint32_t words[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const uint32_t byte = (words[e >> 2] >> ((e & 3) * 8)) & 255u;
+          const W rh = lw[e * 32 + (byte >> 4)];
+          const W rl = lw[e * 32 + 16 + (byte & 15u)];
+          if constexpr (T == 1) {
+            accA[0] += uint32_t(rh) + uint32_t(rl);
+          } else if constexpr (T == 4) {
+            const uint32_t rr = rh + rl;  // bytes <= 254: no carries
+            u16x2 a = __builtin_bit_cast(u16x2, accA[0]) + __builtin_bit_cast(u16x2, rr);
+            u16x2 b = __builtin_bit_cast(u16x2, accB[0]) + (__builtin_bit_cast(u16x2, rr) >> (unsigned short)8);
+            accA[0] = __builtin_bit_cast(uint32_t, a);
+            accB[0] = __builtin_bit_cast(uint32_t, b);
+          } else {
+            const uint32_t r0 = rh.x + rl.x, r1 = rh.y + rl.y;
+            u16x2 a0 = __builtin_bit_cast(u16x2, accA[0]) + __builtin_bit_cast(u16x2, r0);
+            u16x2 b0 = __builtin_bit_cast(u16x2, accB[0]) + (__builtin_bit_cast(u16x2, r0) >> (unsigned short)8);
+            u16x2 a1 = __builtin_bit_cast(u16x2, accA[1]) + __builtin_bit_cast(u16x2, r1);
+            u16x2 b1 = __builtin_bit_cast(u16x2, accB[1]) + (__builtin_bit_cast(u16x2, r1) >> (unsigned short)8);
+            accA[0] = __builtin_bit_cast(uint32_t, a0);
+            accB[0] = __builtin_bit_cast(uint32_t, b0);
+            accA[1] = __builtin_bit_cast(uint32_t, a1);
+            accB[1] = __builtin_bit_cast(uint32_t, b1);
+          }
+        }
+        if (((g16 + 1) & 15) == 0 || g16 == G16 - 1) {
+          // window flush: exact u16 sums per query -> float chain
+          if constexpr (T == 1) {
+            fa[0] = __fmaf_rn(scl[0], (float)accA[0], fa[0]);
+            accA[0] = 0u;
+          } else {
+#pragma unroll
+            for (int h = 0; h < T / 4; ++h) {
+              const u16x2 a = __builtin_bit_cast(u16x2, accA[h]);
+              const u16x2 b = __builtin_bit_cast(u16x2, accB[h]);
+              const uint32_t s0 = (unsigned short)(a.x - (unsigned short)(b.x << 8));
+              const uint32_t s1 = b.x;
+              const uint32_t s2 = (unsigned short)(a.y - (unsigned short)(b.y << 8));
+              const uint32_t s3 = b.y;
+              fa[4 * h + 0] = __fmaf_rn(scl[4 * h + 0], (float)s0, fa[4 * h + 0]);
+              fa[4 * h + 1] = __fmaf_rn(scl[4 * h + 1], (float)s1, fa[4 * h + 1]);
+              fa[4 * h + 2] = __fmaf_rn(scl[4 * h + 2], (float)s2, fa[4 * h + 2]);
+              fa[4 * h + 3] = __fmaf_rn(scl[4 * h + 3], (float)s3, fa[4 * h + 3]);
+              accA[h] = 0u;
+              accB[h] = 0u;
+            }
+          }
+        }
+      }
+    }
+
+    // scores; -0.f reported as +0.f
+    float s[T];
+    const float av = valid ? alpha[v] : 0.f;
+#pragma unroll
+    for (int t = 0; t < T; ++t) s[t] = valid ? fa[t] * av + 0.f : -INFINITY;
+
+    // admission with overflow retry
+    uint32_t pend = 0u;
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+      if (valid && q0 + t < nq && better(s[t], (uint32_t)v, sh.thr_s[t], sh.thr_i[t])) pend |= 1u << t;
+    while (true) {
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        if (pend & (1u << t)) {
+          const int slot = atomicAdd(&sh.cnt[t], 1);
+          if (slot < cap) {
+            sh.cand_s[t * cap + slot] = s[t];
+            sh.cand_i[t * cap + slot] = (uint32_t)v;
+            pend &= ~(1u << t);
+          }
+        }
+      }
+      __syncthreads();
+      bool overflow = false;
+      bool need[T];
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int c = sh.cnt[t];
+        overflow |= c > cap;
+        need[t] = c > cap / 2;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < T; ++t)
+        if (need[t]) topk_merge(sh, t, k, cap);
+      if (!overflow) break;
+#pragma unroll
+      for (int t = 0; t < T; ++t)
+        if ((pend & (1u << t)) && !better(s[t], (uint32_t)v, sh.thr_s[t], sh.thr_i[t])) pend &= ~(1u << t);
+    }
+  }
+  // final merge and output
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < T; ++t)
+    if (sh.cnt[t] > 0) topk_merge(sh, t, k, cap);
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    const long q = q0 + t;
+    if (q >= nq) continue;
+    const long slot = q * n_ranges + r;
+    const int m = sh.m[t];
+    for (int j = tid; j < m; j += blockDim.x) {
+      part_s[slot * k + j] = sh.top_s[t * k + j];
+      part_i[slot * k + j] = sh.top_i[t * k + j];
+    }
+    if (tid == 0) part_cnt[slot] = m;
+  }
+}
+
+// Merge the per-range sorted lists of one query (one wavefront per query).
+constexpr int kMaxListsPerLane = 8;  // n_ranges <= 512
+
+__global__ __launch_bounds__(64) void merge_ranges_kernel(const float* __restrict__ part_s,
+                                                          const uint32_t* __restrict__ part_i,
+                                                          const int* __restrict__ part_cnt, long nq,
+                                                          int R, int k, int k_out,
+                                                          const uint64_t* __restrict__ id_map,
+                                                          float* __restrict__ out_s,
+                                                          uint64_t* __restrict__ out_i) {
+  const int lane = threadIdx.x;
+  for (long q = blockIdx.x; q < nq; q += gridDim.x) {
+    int head[kMaxListsPerLane], len[kMaxListsPerLane];
+#pragma unroll
+    for (int j = 0; j < kMaxListsPerLane; ++j) {
+      const int r = lane + 64 * j;
+      head[j] = 0;
+      len[j] = r < R ? part_cnt[q * R + r] : 0;
+    }
+    int j_out = 0;
+    for (; j_out < k_out; ++j_out) {
+      bool have = false;
+      float bs = 0.f;
+      uint32_t bi = 0u;
+      int bj = -1;
+#pragma unroll
+      for (int j = 0; j < kMaxListsPerLane; ++j) {
+        if (head[j] < len[j]) {
+          const long p = (q * R + lane + 64 * j) * (long)k + head[j];
+          const float s = part_s[p];
+          const uint32_t id = part_i[p];
+          if (!have || better(s, id, bs, bi)) { have = true; bs = s; bi = id; bj = j; }
+        }
+      }
+      // wave argmax under (score desc, id asc)
+      bool wh = have;
+      float ws = bs;
+      uint32_t wi = bi;
+      for (int o = 32; o > 0; o >>= 1) {
+        const int oh = __shfl_xor((int)wh, o);
+        const float os = __shfl_xor(ws, o);
+        const uint32_t oi = (uint32_t)__shfl_xor((int)wi, o);
+        if (oh && (!wh || better(os, oi, ws, wi))) { wh = true; ws = os; wi = oi; }
+      }
+      if (!wh) break;
+      if (have && bs == ws && bi == wi) {
+#pragma unroll
+        for (int j = 0; j < kMaxListsPerLane; ++j) if (j == bj) ++head[j];
+      }
+      if (lane == 0) {
+        out_s[q * k_out + j_out] = ws;
+        out_i[q * k_out + j_out] = id_map ? id_map[wi] : (uint64_t)wi;
+      }
+    }
+    for (int j = j_out + lane; j < k_out; j += 64) {
+      out_s[q * k_out + j] = 0.f;
+      out_i[q * k_out + j] = 0u;
+    }
+  }
+}
+
+struct MergeLayout {
+  size_t cs, ci, keep, os, oi, total;
+};
+__host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~size_t(15); }
+__host__ __device__ inline MergeLayout merge_layout(int m, int kd) {
+  MergeLayout l;
+  l.cs = 0;
+  l.ci = align16(l.cs + (size_t)m * 4);
+  l.keep = align16(l.ci + (size_t)m * 8);
+  l.os = align16(l.keep + (size_t)m * 4);
+  l.oi = align16(l.os + (size_t)kd * 4);
+  l.total = align16(l.oi + (size_t)kd * 8);
+  return l;
+}
+
+// merge_search_results (src/search/search.cpp:1551-1600): one workgroup per
+// query row; union, a duplicated id keeps its higher score, rank by
+// (score desc, id asc), pad with (0, 0).
+__global__ __launch_bounds__(256) void merge_results_kernel(long nq, int kd, float* __restrict__ ds,
+                                                            uint64_t* __restrict__ di, int ks,
+                                                            const float* __restrict__ ss,
+                                                            const uint64_t* __restrict__ si) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int m = kd + ks;
+  const MergeLayout lay = merge_layout(m, kd);
+  float* cs = reinterpret_cast<float*>(smem + lay.cs);
+  uint64_t* ci = reinterpret_cast<uint64_t*>(smem + lay.ci);
+  int* keep = reinterpret_cast<int*>(smem + lay.keep);
+  float* os = reinterpret_cast<float*>(smem + lay.os);
+  uint64_t* oi = reinterpret_cast<uint64_t*>(smem + lay.oi);
+  for (long q = blockIdx.x; q < nq; q += gridDim.x) {
+    for (int e = threadIdx.x; e < m; e += blockDim.x) {
+      cs[e] = e < kd ? ds[q * kd + e] : ss[q * ks + (e - kd)];
+      ci[e] = e < kd ? di[q * kd + e] : si[q * ks + (e - kd)];
+    }
+    __syncthreads();
+    // an entry is dropped if another entry with the same id has a higher
+    // score, or the same score and a lower position (one survivor per id)
+    for (int e = threadIdx.x; e < m; e += blockDim.x) {
+      int kp = 1;
+      for (int f = 0; f < m; ++f)
+        if (f != e && ci[f] == ci[e] && (cs[f] > cs[e] || (cs[f] == cs[e] && f < e))) { kp = 0; break; }
+      keep[e] = kp;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kd; j += blockDim.x) { os[j] = 0.f; oi[j] = 0u; }
+    __syncthreads();
+    for (int e = threadIdx.x; e < m; e += blockDim.x) {
+      if (!keep[e]) continue;
+      int rank = 0;
+      for (int f = 0; f < m; ++f)
+        if (keep[f] && (cs[f] > cs[e] || (cs[f] == cs[e] && ci[f] < ci[e]))) ++rank;
+      if (rank < kd) { os[rank] = cs[e]; oi[rank] = ci[e]; }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kd; j += blockDim.x) { ds[q * kd + j] = os[j]; di[q * kd + j] = oi[j]; }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Format conversions (src/pack/pack.cpp:98-186,403-420)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long tiled_offset(long v, int g, int G16) {
+  return ((v >> 6) * G16 + (g >> 4)) * 1024 + (v & 63) * 16 + (g & 15);
+}
+
+__global__ void tiled_to_flat_kernel(const uint8_t* __restrict__ tiled, long n, int G, int G16,
+                                     uint8_t* __restrict__ flat) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n * G; i += (long)gridDim.x * blockDim.x) {
+    const long v = i / G;
+    const int g = (int)(i - v * G);
+    flat[i] = tiled[tiled_offset(v, g, G16)];
+  }
+}
+
+__global__ void flat_to_tiled_kernel(const uint8_t* __restrict__ flat, long n, int G, int G16,
+                                     uint8_t* __restrict__ tiled) {
+  const long ntiles = (n + 63) / 64;
+  const long total = ntiles * 64 * (long)G16 * 16;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    // i enumerates tiled bytes: tile, chunk, lane, e
+    const long tile = i / ((long)G16 * 1024);
+    const long rem = i - tile * G16 * 1024;
+    const int c16 = (int)(rem / 1024);
+    const int lane = (int)((rem / 16) & 63);
+    const int e = (int)(rem & 15);
+    const long v = tile * 64 + lane;
+    const int g = c16 * 16 + e;
+    tiled[i] = (v < n && g < G) ? flat[v * G + g] : 0;
+  }
+}
+
+__constant__ unsigned char kPerm0Dev[16] = {0, 8, 1, 9, 2, 10, 3, 11, 4, 12, 5, 13, 6, 14, 7, 15};
+__constant__ unsigned char kInvPerm0Dev[16] = {0, 2, 4, 6, 8, 10, 12, 14, 1, 3, 5, 7, 9, 11, 13, 15};
+
+// PERM0 192x1: block bk, group g, 32-lane chunk ch, j < 16, va = 192 bk +
+// 32 ch + P[j], vb = va + 16: byte j = hi(va) | hi(vb) << 4, byte 16 + j =
+// lo(va) | lo(vb) << 4; vectors >= n read as 0.
+__global__ void flat_to_perm0_kernel(const uint8_t* __restrict__ flat, long n, int G,
+                                     uint8_t* __restrict__ out) {
+  const long nb = (n + 191) / 192;
+  const long total = nb * G * 96;  // (bk, g, ch, j)
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(i & 15);
+    const int ch = (int)((i >> 4) % 6);
+    const long bg = (i >> 4) / 6;  // bk * G + g
+    const long bk = bg / G;
+    const int g = (int)(bg - bk * G);
+    const long va = bk * 192 + ch * 32 + kPerm0Dev[j];
+    const long vb = va + 16;
+    const uint32_t a = va < n ? flat[va * G + g] : 0u;
+    const uint32_t b = vb < n ? flat[vb * G + g] : 0u;
+    uint8_t* o = out + bg * 192 + ch * 32;
+    o[j] = (uint8_t)((a >> 4) | ((b >> 4) << 4));
+    o[16 + j] = (uint8_t)((a & 15u) | ((b & 15u) << 4));
+  }
+}
+
+__global__ void perm0_to_flat_kernel(const uint8_t* __restrict__ blk, long n, int G,
+                                     uint8_t* __restrict__ flat) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n * G; i += (long)gridDim.x * blockDim.x) {
+    const long v = i / G;
+    const int g = (int)(i - v * G);
+    const long bk = v / 192;
+    const int lane = (int)(v - bk * 192);
+    const int ch = lane / 32, l = lane & 31;
+    const int j = kInvPerm0Dev[l & 15];
+    const uint8_t* tb = blk + (bk * G + g) * 192 + ch * 32;
+    const uint32_t hp = tb[j], lp = tb[16 + j];
+    const uint32_t hi = l < 16 ? (hp & 15u) : (hp >> 4);
+    const uint32_t lo = l < 16 ? (lp & 15u) : (lp >> 4);
+    flat[i] = (uint8_t)((hi << 4) | lo);
+  }
+}
+
+// flat byte groups -> bit-plane rows (the encode() output format).
+__global__ void flat_to_packed_kernel(const uint8_t* __restrict__ flat, long n, int D, int bits,
+                                      uint8_t* __restrict__ packed) {
+  const int G = D / (8 / bits), bpp = D / 8;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n * bpp; i += (long)gridDim.x * blockDim.x) {
+    const long v = i / bpp;
+    const int c8 = (int)(i - v * bpp);
+    unsigned codes[8];
+    for (int kk = 0; kk < 8; ++kk) {
+      const int j = c8 * 8 + kk;
+      if (bits == 2) {
+        codes[kk] = (flat[v * G + j / 4] >> (6 - 2 * (j & 3))) & 3u;
+      } else {
+        const uint32_t byte = flat[v * G + j / 2];
+        codes[kk] = (j & 1) ? (byte & 15u) : (byte >> 4);
+      }
+    }
+    for (int p = 0; p < bits; ++p) {
+      uint32_t byte = 0;
+      for (int kk = 0; kk < 8; ++kk) byte |= ((codes[kk] >> p) & 1u) << (7 - kk);
+      packed[v * (long)(bits * bpp) + p * bpp + c8] = (uint8_t)byte;
+    }
+  }
+}
+
+int grid_for(long work, int threads, int cap = 65536) {
+  long g = (work + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_encode(const CodeTables& t, const float* x, std::size_t n, std::size_t out_offset,
+                         std::uint8_t* tiled, std::uint8_t* packed, float* rotated, float* alpha,
+                         hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t lds = (size_t)t.dim * 5;
+  const int grid = grid_for((long)(n + 7) / 8, 1, 16384);
+  hipLaunchKernelGGL(encode_kernel, dim3(grid), dim3(64), lds, s, t, x, (long)n, (long)out_offset, tiled,
+                     packed, rotated, alpha);
+  return hipGetLastError();
+}
+
+hipError_t launch_calibrate(const float* rotated, std::size_t n, int dim, std::size_t lo_idx,
+                            std::size_t hi_idx, float qc_lo, float qc_span, std::uint32_t* keys,
+                            float* shift, float* scale, float* inv_scale, hipStream_t s) {
+  dim3 tg((dim + 31) / 32, (unsigned)((n + 31) / 32));
+  hipLaunchKernelGGL(keys_transpose_kernel, tg, dim3(256), 0, s, rotated, (long)n, dim, keys);
+  hipLaunchKernelGGL(radix_select_kernel, dim3(dim), dim3(256), 0, s, keys, (long)n, (long)lo_idx,
+                     (long)hi_idx, qc_lo, qc_span, shift, scale, inv_scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_prepare_queries(const CodeTables& t, const float* q, std::size_t nq,
+                                  std::uint8_t* luts, float* lut_scale, float* lut_bias,
+                                  float* bias_corr, float* bias_total, hipStream_t s) {
+  if (nq == 0) return hipSuccess;
+  const size_t lds = (size_t)t.dim * 4 + (size_t)t.groups * 8;
+  const int grid = grid_for((long)nq, 1, 16384);
+  hipLaunchKernelGGL(prepare_queries_kernel, dim3(grid), dim3(64), lds, s, t, q, (long)nq, luts, lut_scale,
+                     lut_bias, bias_corr, bias_total);
+  return hipGetLastError();
+}
+
+static size_t topk_lds(int T, int k, int cap) { return (size_t)T * (4 * (size_t)k + 2 * (size_t)cap) * 4 + 16 * T + 64; }
+
+ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int forced_tile) {
+  ScanPlan p{};
+  p.cand_cap = kCandCap;
+  const int tiles[3] = {8, 4, 1};
+  for (int ti = 0; ti < 3; ++ti) {
+    const int T = tiles[ti];
+    if (forced_tile != 0 && T != forced_tile) continue;
+    if (forced_tile == 0) {
+      if (T == 8 && nq < 8) continue;
+      if (T == 4 && nq < 4) continue;
+    }
+    const size_t tk = topk_lds(T, k, p.cand_cap);
+    if (tk + 16 * 32 * (size_t)T * 16 > (size_t)kLdsBudget) continue;  // not even one window
+    const size_t lut_room = (size_t)kLdsBudget - tk - 64;
+    int chunk = (int)(lut_room / (16 * 32 * (size_t)T));
+    if (chunk >= groups16) {
+      chunk = groups16;
+    } else {
+      chunk = (chunk / 16) * 16;
+      // multi-chunk mode reloads the LUT per step: only worth it for T = 1
+      if (forced_tile == 0 && T != 1) continue;
+    }
+    if (chunk < 1) continue;
+    p.tile = T;
+    p.lut_chunk16 = chunk;
+    p.lds_bytes = (((size_t)chunk * 16 * 32 * T + 15) & ~size_t(15)) + tk;
+    break;
+  }
+  if (p.tile == 0) return p;  // caller reports the error
+  const long n_qt = (long)((nq + p.tile - 1) / p.tile);
+  long R = (512 + n_qt - 1) / n_qt;
+  const long max_by_n = (long)((n + 1023) / 1024);
+  if (R > max_by_n) R = max_by_n;
+  if (R > 512) R = 512;
+  if (R < 1) R = 1;
+  long len = (long)((n + R - 1) / R);
+  len = (len + 63) / 64 * 64;
+  R = (long)((n + len - 1) / len);
+  if (R < 1) R = 1;
+  p.n_ranges = (int)R;
+  p.range_len = (int)len;
+  return p;
+}
+
+hipError_t launch_scan(const ScanPlan& p, const std::uint8_t* tiled, const float* alpha, std::size_t n,
+                       int groups, int groups16, const std::uint8_t* luts, const float* lut_scale,
+                       const float* bias_total, std::size_t nq, int k, float* part_s,
+                       std::uint32_t* part_i, int* part_cnt, hipStream_t s) {
+  const long n_qt = (long)((nq + p.tile - 1) / p.tile);
+  const dim3 grid((unsigned)(n_qt * p.n_ranges));
+  const dim3 block(kScanThreads);
+#define VC_LAUNCH_SCAN(TT)                                                                            \
+  do {                                                                                                \
+    (void)hipFuncSetAttribute((const void*)scan_topk_kernel<TT>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                        (int)p.lds_bytes);                                                            \
+    hipLaunchKernelGGL(scan_topk_kernel<TT>, grid, block, p.lds_bytes, s, tiled, alpha, (long)n, groups,  \
+                       groups16, p.lut_chunk16, luts, lut_scale, bias_total, (long)nq, k, p.cand_cap,  \
+                       p.n_ranges, (long)p.range_len, part_s, part_i, part_cnt);                      \
+  } while (0)
+  if (p.tile == 8)
+    VC_LAUNCH_SCAN(8);
+  else if (p.tile == 4)
+    VC_LAUNCH_SCAN(4);
+  else
+    VC_LAUNCH_SCAN(1);
+#undef VC_LAUNCH_SCAN
+  return hipGetLastError();
+}
+
+hipError_t launch_merge_ranges(const float* part_s, const std::uint32_t* part_i, const int* part_cnt,
+                               std::size_t nq, int n_ranges, int k, int k_out, const std::uint64_t* id_map,
+                               float* out_s, std::uint64_t* out_i, hipStream_t s) {
+  if (nq == 0 || k_out == 0) return hipSuccess;
+  const int grid = grid_for((long)nq, 1, 65536);
+  hipLaunchKernelGGL(merge_ranges_kernel, dim3(grid), dim3(64), 0, s, part_s, part_i, part_cnt, (long)nq,
+                     n_ranges, k, k_out, id_map, out_s, out_i);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge_results(std::size_t nq, int k_dst, float* dst_s, std::uint64_t* dst_i, int k_src,
+                                const float* src_s, const std::uint64_t* src_i, hipStream_t s) {
+  if (nq == 0) return hipSuccess;
+  const size_t lds = merge_layout(k_dst + k_src, k_dst).total;
+  const int grid = grid_for((long)nq, 1, 65536);
+  hipLaunchKernelGGL(merge_results_kernel, dim3(grid), dim3(256), lds, s, (long)nq, k_dst, dst_s, dst_i, k_src,
+                     src_s, src_i);
+  return hipGetLastError();
+}
+
+hipError_t launch_tiled_to_flat(const std::uint8_t* tiled, std::size_t n, int groups, int groups16,
+                                std::uint8_t* flat, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(tiled_to_flat_kernel, dim3(grid_for((long)n * groups, 256)), dim3(256), 0, s, tiled,
+                     (long)n, groups, groups16, flat);
+  return hipGetLastError();
+}
+
+hipError_t launch_flat_to_tiled(const std::uint8_t* flat, std::size_t n, int groups, int groups16,
+                                std::uint8_t* tiled, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const long total = (long)((n + 63) / 64) * 64 * groups16 * 16;
+  hipLaunchKernelGGL(flat_to_tiled_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, flat, (long)n, groups,
+                     groups16, tiled);
+  return hipGetLastError();
+}
+
+hipError_t launch_flat_to_perm0(const std::uint8_t* flat, std::size_t n, int groups, std::uint8_t* perm0,
+                                hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const long total = (long)((n + 191) / 192) * groups * 96;
+  hipLaunchKernelGGL(flat_to_perm0_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, flat, (long)n, groups,
+                     perm0);
+  return hipGetLastError();
+}
+
+hipError_t launch_perm0_to_flat(const std::uint8_t* perm0, std::size_t n, int groups, std::uint8_t* flat,
+                                hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(perm0_to_flat_kernel, dim3(grid_for((long)n * groups, 256)), dim3(256), 0, s, perm0,
+                     (long)n, groups, flat);
+  return hipGetLastError();
+}
+
+hipError_t launch_flat_to_packed(const std::uint8_t* flat, std::size_t n, int dim, int bits,
+                                 std::uint8_t* packed, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(flat_to_packed_kernel, dim3(grid_for((long)n * (dim / 8), 256)), dim3(256), 0, s, flat,
+                     (long)n, dim, bits, packed);
+  return hipGetLastError();
+}
+
+}  // namespace vchip

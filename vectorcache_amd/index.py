@@ -1,0 +1,286 @@
+"""TurboQuantIndex: the plain flat TurboQuant index on the GPU.
+
+Mirrors the reference's index class shape (include/vectorcache/index.hpp:18-78)
+and the README's calibrate()/add()/prepare()/search() flow for the plain flat
+path; every call goes through the C ABI of libvectorcache_hip.so. Errors map
+onto the reference's exception types: std::invalid_argument -> ValueError,
+std::runtime_error (call order) -> IndexStateError, std::out_of_range ->
+IndexError.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi
+from ._capi import P, S, check, lib
+
+
+def _f32(a, cols=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if cols is not None:
+        if a.size % cols != 0:
+            raise ValueError(f"input size {a.size} is not a multiple of dim {cols}")
+        a = a.reshape(-1, cols)
+    return a
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+@dataclass
+class SearchResults:
+    """nq x k row-major results (reference search.hpp:77-84); rows sorted by
+    (score desc, id asc)."""
+
+    scores: np.ndarray
+    ids: np.ndarray
+
+    @property
+    def k(self) -> int:
+        return self.scores.shape[1]
+
+    @property
+    def nq(self) -> int:
+        return self.scores.shape[0]
+
+
+class TurboQuantIndex:
+    def __init__(self, dim: int, bits: int = 4):
+        h = P()
+        check(lib.vc_index_create(S(dim), S(bits), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            check(lib.vc_index_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- reference accessors ------------------------------------------------
+    def dim(self) -> int:
+        return int(lib.vc_index_dim(self._h))
+
+    def size(self) -> int:
+        return int(lib.vc_index_size(self._h))
+
+    def bit_width(self) -> int:
+        return int(lib.vc_index_bits(self._h))
+
+    def codes_bytes(self) -> int:
+        return int(lib.vc_index_codes_bytes(self._h))
+
+    def index_bytes(self) -> int:
+        return int(lib.vc_index_index_bytes(self._h))
+
+    @property
+    def prepared(self) -> bool:
+        return bool(lib.vc_index_is_prepared(self._h))
+
+    @property
+    def calibrated(self) -> bool:
+        return bool(lib.vc_index_is_calibrated(self._h))
+
+    # ---- build ----------------------------------------------------------------
+    def calibrate(self, sample):
+        """TQ+ calibration from a sample (fit_calibration semantics)."""
+        if _is_torch(sample):
+            t = sample.contiguous().float()
+            check(lib.vc_index_calibrate_device(self._h, P(t.data_ptr()), S(t.numel() // self.dim())))
+            return
+        a = _f32(sample, self.dim())
+        check(lib.vc_index_calibrate(self._h, _ptr(a), S(a.shape[0])))
+
+    def add(self, vectors):
+        if _is_torch(vectors) and vectors.is_cuda:
+            t = vectors.contiguous().float()
+            if t.numel() % self.dim() != 0:
+                raise ValueError("input size is not a multiple of dim")
+            check(lib.vc_index_add_device(self._h, P(t.data_ptr()), S(t.numel() // self.dim())))
+            return
+        a = _f32(vectors, self.dim())
+        check(lib.vc_index_add(self._h, _ptr(a), S(a.shape[0])))
+
+    def prepare(self):
+        check(lib.vc_index_prepare(self._h))
+
+    # ---- search ---------------------------------------------------------------
+    def search(self, queries, k: int) -> SearchResults:
+        q = _f32(queries, self.dim())
+        nq = q.shape[0]
+        kk = min(int(k), self.size())
+        scores = np.zeros((nq, max(kk, 0)), np.float32)
+        ids = np.zeros((nq, max(kk, 0)), np.uint64)
+        kout = S()
+        check(lib.vc_index_search(self._h, _ptr(q), S(nq), S(k), _ptr(scores), _ptr(ids), ctypes.byref(kout)))
+        return SearchResults(scores[:, : kout.value], ids[:, : kout.value])
+
+    def search_device(self, queries, k: int, scores=None, ids=None, stream=None):
+        """Device-resident search: queries is a torch CUDA tensor (nq x dim).
+        Returns torch tensors (scores f32, ids int64 holding the u64 ids)."""
+        import torch
+
+        q = queries.contiguous().float()
+        nq = q.numel() // self.dim()
+        kk = min(int(k), self.size())
+        if scores is None:
+            scores = torch.empty((nq, kk), dtype=torch.float32, device=q.device)
+        if ids is None:
+            ids = torch.empty((nq, kk), dtype=torch.int64, device=q.device)
+        kout = S()
+        st = P(stream) if stream is not None else None
+        check(lib.vc_index_search_device(self._h, P(q.data_ptr()), S(nq), S(k), P(scores.data_ptr()),
+                                         P(ids.data_ptr()), ctypes.byref(kout), st))
+        return scores, ids
+
+    # ---- exports in the reference formats ---------------------------------------
+    def export_packed(self) -> np.ndarray:
+        out = np.zeros((self.size(), self.bit_width() * self.dim() // 8), np.uint8)
+        check(lib.vc_index_export_packed(self._h, _ptr(out)))
+        return out
+
+    def export_flat(self) -> np.ndarray:
+        G = self.dim() // (8 // self.bit_width())
+        out = np.zeros((self.size(), G), np.uint8)
+        check(lib.vc_index_export_flat(self._h, _ptr(out)))
+        return out
+
+    def export_perm0(self) -> np.ndarray:
+        G = self.dim() // (8 // self.bit_width())
+        nb = (self.size() + 191) // 192
+        out = np.zeros(nb * G * 192, np.uint8)
+        check(lib.vc_index_export_perm0(self._h, _ptr(out)))
+        return out
+
+    def export_alpha(self) -> np.ndarray:
+        out = np.zeros(self.size(), np.float32)
+        check(lib.vc_index_export_alpha(self._h, _ptr(out)))
+        return out
+
+    def calibration(self):
+        sh = np.zeros(self.dim(), np.float32)
+        sc = np.zeros(self.dim(), np.float32)
+        check(lib.vc_index_get_calibration(self._h, _ptr(sh), _ptr(sc)))
+        return sh, sc
+
+    def prepare_queries(self, queries):
+        """prepare_queries state: dict(luts, lut_scale, lut_bias, bias_corr)."""
+        q = _f32(queries, self.dim())
+        nq = q.shape[0]
+        G = self.dim() // (8 // self.bit_width())
+        luts = np.zeros((nq, G * 32), np.uint8)
+        ls = np.zeros(nq, np.float32)
+        lb = np.zeros(nq, np.float32)
+        bc = np.zeros(nq, np.float32)
+        check(lib.vc_index_prepare_queries(self._h, _ptr(q), S(nq), _ptr(luts), _ptr(ls), _ptr(lb), _ptr(bc)))
+        return dict(luts=luts, lut_scale=ls, lut_bias=lb, bias_corr=bc)
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+# ---- stateless functions ---------------------------------------------------------
+def rotation_tables(dim: int):
+    perm = np.zeros(dim, np.uint32)
+    signs = np.zeros(dim, np.float32)
+    isb = ctypes.c_float()
+    blk = S()
+    check(lib.vc_rotation_tables(S(dim), _ptr(perm), _ptr(signs), ctypes.byref(isb), ctypes.byref(blk)))
+    return perm, signs, np.float32(isb.value), int(blk.value)
+
+
+def codebook(bits: int, dim: int):
+    b = np.zeros(15, np.float32)
+    c = np.zeros(16, np.float32)
+    check(lib.vc_codebook(S(bits), S(dim), _ptr(b), _ptr(c)))
+    L = 1 << bits
+    return b[: L - 1].copy(), c[:L].copy()
+
+
+def encode(vectors, bits: int, shift=None, scale=None):
+    """encode(): bit-plane codes (n x bits*dim/8) and alpha (n)."""
+    x = _f32(vectors)
+    n, dim = x.shape
+    packed = np.zeros((n, bits * dim // 8), np.uint8)
+    alpha = np.zeros(n, np.float32)
+    sh = None if shift is None else _f32(shift)
+    sc = None if scale is None else _f32(scale)
+    check(lib.vc_encode(_ptr(x), S(n), S(dim), S(bits), _ptr(sh), _ptr(sc), _ptr(packed), _ptr(alpha)))
+    return packed, alpha
+
+
+def fit_calibration(sample, bits: int):
+    x = _f32(sample)
+    n, dim = x.shape
+    shift = np.zeros(dim, np.float32)
+    scale = np.zeros(dim, np.float32)
+    check(lib.vc_fit_calibration(_ptr(x), S(n), S(dim), S(bits), _ptr(shift), _ptr(scale)))
+    return shift, scale
+
+
+def search_perm0(queries, k: int, bits: int, perm0, alpha, id_map=None, shift=None, scale=None) -> SearchResults:
+    """score_prepared over PERM0 blocked codes (optionally with an id map)."""
+    q = _f32(queries)
+    nq, dim = q.shape
+    alpha = _f32(alpha)
+    n = alpha.size
+    perm0 = np.ascontiguousarray(perm0, np.uint8)
+    kk = min(int(k), n)
+    scores = np.zeros((nq, kk), np.float32)
+    ids = np.zeros((nq, kk), np.uint64)
+    im = None if id_map is None else np.ascontiguousarray(id_map, np.uint64)
+    sh = None if shift is None else _f32(shift)
+    sc = None if scale is None else _f32(scale)
+    kout = S()
+    check(lib.vc_search_perm0(_ptr(q), S(nq), S(dim), S(bits), S(k), _ptr(perm0), _ptr(alpha), S(n), _ptr(im),
+                              _ptr(sh), _ptr(sc), _ptr(scores), _ptr(ids), ctypes.byref(kout)))
+    return SearchResults(scores[:, : kout.value], ids[:, : kout.value])
+
+
+def merge_search_results(dst: SearchResults, src: SearchResults) -> SearchResults:
+    """merge_search_results (reference search.cpp:1551-1600) on the GPU."""
+    if src.nq == 0 or src.k == 0:
+        return dst
+    if dst.nq == 0 or dst.k == 0:
+        return SearchResults(src.scores.copy(), src.ids.copy())
+    if dst.nq != src.nq:
+        raise ValueError("merge_search_results: nq mismatch")
+    ds = np.ascontiguousarray(dst.scores, np.float32).copy()
+    di = np.ascontiguousarray(dst.ids, np.uint64).copy()
+    ss = np.ascontiguousarray(src.scores, np.float32)
+    si = np.ascontiguousarray(src.ids, np.uint64)
+    check(lib.vc_merge_results(S(dst.nq), S(dst.k), _ptr(ds), _ptr(di), S(src.k), _ptr(ss), _ptr(si)))
+    return SearchResults(ds, di)
+
+
+def set_scan_tile(tile: int):
+    check(lib.vc_set_scan_tile(S(tile)))
+
+
+def scan_tile_for(dim: int, bits: int, nq: int) -> int:
+    return int(lib.vc_scan_tile_for(S(dim), S(bits), S(nq)))
+
+
+def set_profiling(on: bool):
+    check(lib.vc_set_profiling(1 if on else 0))
+
+
+def last_scan_stats():
+    """(ms, tile, n_ranges) of the last profiled scan kernel (HIP events)."""
+    ms = ctypes.c_double()
+    tile = S()
+    ranges = S()
+    check(lib.vc_last_scan_stats(ctypes.byref(ms), ctypes.byref(tile), ctypes.byref(ranges)))
+    return ms.value, int(tile.value), int(ranges.value)
+
+
+version = _capi.version
