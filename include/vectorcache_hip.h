@@ -128,6 +128,10 @@ vc_status vc_codebook(size_t bits, size_t dim, float* boundaries, float* centroi
 vc_status vc_encode(const float* vectors, size_t n, size_t dim, size_t bits, const float* shift,
                     const float* scale, uint8_t* packed_out, float* alpha_out);
 
+/* rotate_batch_into (src/encode/encode.cpp:159-183) on the GPU: each row
+ * scaled by 1/||row|| (0 below 1e-10) and rotated; rotated_out n x dim. */
+vc_status vc_rotate(const float* vectors, size_t n, size_t dim, float* rotated_out);
+
 /* fit_calibration() on the GPU (src/encode/encode.cpp:327-335). */
 vc_status vc_fit_calibration(const float* sample, size_t n, size_t dim, size_t bits,
                              float* shift_out, float* scale_out);
@@ -158,6 +162,9 @@ size_t vc_scan_tile_for(size_t dim, size_t bits, size_t nq);
  * vc_last_scan_stats waits for the last such kernel and reports its duration
  * in ms, its query tile and the number of vector ranges (work split) it used. */
 vc_status vc_set_profiling(int on);
+/* Test hook (no reference counterpart): flags bit 0 makes every later encode
+ * take the exact sequential alpha path instead of the certified fast one. */
+vc_status vc_set_debug(unsigned flags);
 vc_status vc_last_scan_stats(double* ms, size_t* tile, size_t* n_ranges);
 
 #ifdef __cplusplus

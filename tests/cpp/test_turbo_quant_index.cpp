@@ -1,0 +1,115 @@
+// C++ API checks of vectorcache::TurboQuantIndex (the host side above the C ABI).
+// Restates the reference's own index tests for the plain flat path
+// (reference tests/test_index.cpp:11-113): ids in range, rows sorted, a
+// self-query returns its own id first, batched search equals per-query
+// search, plus the reference's error behaviour (std::invalid_argument,
+// std::runtime_error) and merge_search_results semantics.
+// Prints "OK <checks>" and exits 0 on success.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <span>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "vectorcache/turbo_quant_index.hpp"
+
+namespace {
+int g_checks = 0;
+void expect(bool ok, const std::string& what) {
+  ++g_checks;
+  if (!ok) {
+    std::fprintf(stderr, "FAIL: %s\n", what.c_str());
+    std::exit(1);
+  }
+}
+
+template <typename E, typename F>
+void expect_throw(F&& f, const std::string& what) {
+  try {
+    f();
+  } catch (const E&) {
+    ++g_checks;
+    return;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "FAIL: %s threw the wrong type: %s\n", what.c_str(), e.what());
+    std::exit(1);
+  }
+  std::fprintf(stderr, "FAIL: %s did not throw\n", what.c_str());
+  std::exit(1);
+}
+
+std::vector<float> gaussian(std::size_t n, std::size_t dim, unsigned seed) {
+  std::mt19937 rng(seed);
+  std::normal_distribution<float> nd(0.f, 1.f);
+  std::vector<float> v(n * dim);
+  for (auto& x : v) x = nd(rng);
+  return v;
+}
+}  // namespace
+
+int main() {
+  using vectorcache::SearchResults;
+  using vectorcache::TurboQuantIndex;
+  const std::size_t dim = 128, n = 2000, nq = 16, k = 10;
+  auto data = gaussian(n, dim, 42);
+
+  expect_throw<std::invalid_argument>([] { TurboQuantIndex bad(100, 4); }, "dim not a multiple of 8");
+  expect_throw<std::invalid_argument>([] { TurboQuantIndex bad(128, 5); }, "bits out of range");
+
+  TurboQuantIndex index(dim, 4);
+  expect(index.dim() == dim && index.bit_width() == 4, "accessors");
+  expect_throw<std::invalid_argument>([&] { index.add(std::span<const float>(data.data(), 130)); },
+                                      "span not a multiple of dim");
+  index.add(data);
+  expect(index.size() == n, "size after add");
+  expect_throw<std::runtime_error>([&] { (void)index.search(std::span<const float>(data.data(), dim), k); },
+                                   "search before prepare");
+  index.prepare();
+  expect(index.codes_bytes() == n * dim / 2, "codes_bytes");
+
+  std::span<const float> queries(data.data(), nq * dim);  // the first nq vectors query themselves
+  SearchResults res = index.search(queries, k);
+  expect(res.nq == nq && res.k == k && res.ids.size() == nq * k, "result shape");
+  for (std::size_t q = 0; q < nq; ++q) {
+    expect(res.ids[q * k] == q, "self-query returns its own id first");
+    for (std::size_t j = 0; j < k; ++j) {
+      expect(res.ids[q * k + j] < n, "ids in range");
+      if (j > 0) {
+        const float a = res.scores[q * k + j - 1], b = res.scores[q * k + j];
+        expect(a > b || (a == b && res.ids[q * k + j - 1] < res.ids[q * k + j]), "rows sorted (score desc, id asc)");
+      }
+    }
+  }
+  for (std::size_t q = 0; q < nq; ++q) {
+    SearchResults one = index.search(std::span<const float>(data.data() + q * dim, dim), k);
+    for (std::size_t j = 0; j < k; ++j) {
+      expect(one.ids[j] == res.ids[q * k + j], "batched ids == per-query ids");
+      expect(one.scores[j] == res.scores[q * k + j], "batched scores == per-query scores");
+    }
+  }
+  SearchResults big = index.search(queries, n + 50);
+  expect(big.k == n, "k clipped to size()");
+
+  // merge: dst and src share ids; the union keeps the k best, once per id
+  SearchResults dst = index.search(std::span<const float>(data.data(), dim), 5);
+  SearchResults src = index.search(std::span<const float>(data.data(), dim), 8);
+  vectorcache::merge_search_results(dst, src);
+  expect(dst.k == 5, "merge keeps dst.k");
+  for (std::size_t j = 0; j < 5; ++j) expect(dst.ids[j] == src.ids[j], "merge of overlapping rows");
+
+  // TQ+ calibrated index
+  TurboQuantIndex tq(dim, 2);
+  auto sample = gaussian(500, dim, 7);
+  tq.calibrate(sample);
+  tq.add(data);
+  tq.prepare();
+  SearchResults rt = tq.search(queries, k);
+  for (std::size_t q = 0; q < nq; ++q) expect(rt.ids[q * k] == q, "TQ+ self-query");
+  expect_throw<std::runtime_error>([&] { tq.calibrate(sample); }, "calibrate after prepare");
+
+  std::printf("OK %d\n", g_checks);
+  return 0;
+}

@@ -26,6 +26,17 @@ def test_version(vc):
     assert "gfx950" in vc.version()
 
 
+def test_rotation_matches_golden(vc, golden_cases):
+    from conftest import load_golden
+
+    g = load_golden("rotation.npz")
+    for d in (128, 200, 1536):
+        rot = vc.rotate(g[f"rowin_{d}"][None, :])
+        assert bits_equal(rot[0], g[f"rotated_{d}"]), d
+    for c, gc, x, q, sample in golden_cases:
+        assert sha(vc.rotate(x)) == bytes(gc["rotated_sha"]).hex(), c
+
+
 def test_encode_matches_golden(vc, golden_cases):
     for c, g, x, q, sample in golden_cases:
         sh, sc = (g["shift"], g["scale"]) if c["tq"] else (None, None)
@@ -63,6 +74,19 @@ def test_index_exports_and_search_match_golden(vc, golden_cases):
             r = ix.search(q, k)
             assert np.array_equal(r.ids, g[f"ids_k{k}"]), (c, k)
             assert scores_equal(r.scores, g[f"scores_k{k}"]), (c, k)
+
+
+def test_exact_alpha_path_matches_golden(vc, golden_cases):
+    """Every vector through the reference's sequential alpha chains (the path
+    the certified fast alpha falls back to) gives the same bits."""
+    vc.set_debug(1)
+    try:
+        for c, g, x, q, sample in golden_cases:
+            ix = _index(vc, c, x, g, sample)
+            assert bits_equal(ix.export_alpha(), g["alpha"]), c
+            assert np.array_equal(ix.export_packed(), g["packed"]), c
+    finally:
+        vc.set_debug(0)
 
 
 @pytest.mark.parametrize("tile", [1, 4, 8])

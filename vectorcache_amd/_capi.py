@@ -64,11 +64,13 @@ _SIGS = {
     "vc_codebook": ([S, S, P, P], I),
     "vc_encode": ([P, S, S, S, P, P, P, P], I),
     "vc_fit_calibration": ([P, S, S, S, P, P], I),
+    "vc_rotate": ([P, S, S, P], I),
     "vc_search_perm0": ([P, S, S, S, S, P, P, S, P, P, P, P, P, ctypes.POINTER(S)], I),
     "vc_merge_results": ([S, S, P, P, S, P, P], I),
     "vc_set_scan_tile": ([S], I),
     "vc_scan_tile_for": ([S, S, S], S),
     "vc_set_profiling": ([I], I),
+    "vc_set_debug": ([ctypes.c_uint], I),
     "vc_last_scan_stats": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(S), ctypes.POINTER(S)], I),
 }
 

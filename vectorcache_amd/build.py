@@ -28,7 +28,7 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = [
     "-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off", "-fno-fast-math",
-    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fhip-fp32-correctly-rounded-divide-sqrt", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{ROOT / 'include'}", f"-I{CSRC}", "-Wall", "-Wno-unused-function",
 ]
 HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off", "-fno-fast-math", f"-I{CSRC}", "-Wall"]
@@ -53,7 +53,19 @@ def build(verbose: bool = True) -> Path:
     tmp = OUT.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs])
     tmp.replace(OUT)
+    build_cpp_check()
     return OUT
+
+
+CPP_CHECK_SRC = ROOT / "tests" / "cpp" / "test_turbo_quant_index.cpp"
+CPP_CHECK_BIN = ROOT / "tests" / "cpp" / "test_turbo_quant_index"
+
+
+def build_cpp_check() -> Path:
+    """The C++ API check program (g++ against the C ABI only)."""
+    _run(["g++", "-O2", "-std=c++20", f"-I{ROOT / 'include'}", f"-I{CSRC / 'include'}", CPP_CHECK_SRC,
+          "-o", CPP_CHECK_BIN, f"-L{PKG}", "-lvectorcache_hip", "-Wl,-rpath,$ORIGIN/../../vectorcache_amd"])
+    return CPP_CHECK_BIN
 
 
 if __name__ == "__main__":

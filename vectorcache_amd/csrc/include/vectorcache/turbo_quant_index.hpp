@@ -1,0 +1,120 @@
+// C++20 host API of the MI355X TurboQuant path, over the C ABI of
+// libvectorcache_hip.so (include/vectorcache_hip.h).
+//
+// Mirrors the reference's shapes so callers switch by changing the include:
+//  * TurboQuantIndex: ctor(dim, bits), add(span), prepare(), search(span, k),
+//    dim(), size(), bit_width(), codes_bytes(), index_bytes()
+//    (reference include/vectorcache/index.hpp:18-78) plus calibrate(span)
+//    (README.md:105-111; fit_calibration semantics, encode.hpp:33-35);
+//  * SearchResults: row-major nq x k scores/ids, rows sorted by
+//    (score desc, id asc) (reference include/vectorcache/search/search.hpp:77-84);
+//  * merge_search_results (search.hpp:191-192).
+// Exceptions follow the reference: std::invalid_argument for bad dim/bits or
+// a span that is not a multiple of dim, std::runtime_error for search before
+// prepare and for device failures, std::out_of_range for index errors.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <span>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "vectorcache_hip.h"
+
+namespace vectorcache {
+
+struct SearchResults {
+  std::vector<float> scores;         // nq * k
+  std::vector<std::uint64_t> ids;    // nq * k
+  std::size_t k = 0;
+  std::size_t nq = 0;
+  std::vector<std::size_t> counts;   // empty: every row has k hits
+};
+
+namespace detail {
+inline void throw_on(vc_status st) {
+  if (st == VC_OK) return;
+  const std::string msg = vc_last_error();
+  switch (st) {
+    case VC_ERR_INVALID_ARGUMENT: throw std::invalid_argument(msg);
+    case VC_ERR_OUT_OF_RANGE: throw std::out_of_range(msg);
+    default: throw std::runtime_error(msg);
+  }
+}
+}  // namespace detail
+
+class TurboQuantIndex {
+ public:
+  TurboQuantIndex(std::size_t dim, std::size_t bits) { detail::throw_on(vc_index_create(dim, bits, &h_)); }
+  ~TurboQuantIndex() {
+    if (h_ != nullptr) vc_index_destroy(h_);
+  }
+  TurboQuantIndex(const TurboQuantIndex&) = delete;
+  TurboQuantIndex& operator=(const TurboQuantIndex&) = delete;
+  TurboQuantIndex(TurboQuantIndex&& o) noexcept : h_(std::exchange(o.h_, nullptr)) {}
+  TurboQuantIndex& operator=(TurboQuantIndex&& o) noexcept {
+    if (this != &o) {
+      if (h_ != nullptr) vc_index_destroy(h_);
+      h_ = std::exchange(o.h_, nullptr);
+    }
+    return *this;
+  }
+
+  std::size_t dim() const { return vc_index_dim(h_); }
+  std::size_t bit_width() const { return vc_index_bits(h_); }
+  std::size_t size() const { return vc_index_size(h_); }
+  std::size_t codes_bytes() const { return vc_index_codes_bytes(h_); }
+  std::size_t index_bytes() const { return vc_index_index_bytes(h_); }
+  bool prepared() const { return vc_index_is_prepared(h_) != 0; }
+
+  void calibrate(std::span<const float> sample) {
+    check_rows(sample);
+    detail::throw_on(vc_index_calibrate(h_, sample.data(), sample.size() / dim()));
+  }
+  void add(std::span<const float> vectors) {
+    check_rows(vectors);
+    detail::throw_on(vc_index_add(h_, vectors.data(), vectors.size() / dim()));
+  }
+  void prepare() { detail::throw_on(vc_index_prepare(h_)); }
+
+  SearchResults search(std::span<const float> queries, std::size_t k) const {
+    check_rows(queries);
+    SearchResults r;
+    r.nq = queries.size() / dim();
+    const std::size_t kk = std::min(k, size());
+    r.scores.resize(r.nq * kk);
+    r.ids.resize(r.nq * kk);
+    std::size_t k_out = 0;
+    detail::throw_on(vc_index_search(h_, queries.data(), r.nq, k, r.scores.data(), r.ids.data(), &k_out));
+    r.k = k_out;
+    r.scores.resize(r.nq * k_out);
+    r.ids.resize(r.nq * k_out);
+    return r;
+  }
+
+  vc_index* handle() const { return h_; }
+
+ private:
+  void check_rows(std::span<const float> v) const {
+    if (v.size() % dim() != 0) throw std::invalid_argument("vectors size must be a multiple of dim");
+  }
+  vc_index* h_ = nullptr;
+};
+
+// Keep the top dst.k of dst U src per query (higher score wins, tie -> lower
+// id, a duplicated id keeps its higher score).
+inline void merge_search_results(SearchResults& dst, const SearchResults& src) {
+  if (src.nq == 0 || src.k == 0) return;
+  if (dst.nq == 0 || dst.k == 0) {
+    dst = src;
+    return;
+  }
+  if (dst.nq != src.nq) throw std::invalid_argument("merge_search_results: nq mismatch");
+  detail::throw_on(vc_merge_results(dst.nq, dst.k, dst.scores.data(), dst.ids.data(), src.k, src.scores.data(),
+                                    src.ids.data()));
+}
+
+}  // namespace vectorcache

@@ -13,6 +13,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "vc_host.hpp"
@@ -117,7 +118,9 @@ struct Carve {
 struct DevRotation {
   uint32_t* inv_perm = nullptr;
   float* signs = nullptr;
+  uint32_t* perm_sign = nullptr;  // inv_perm[j] | sign-of-destination << 31
 };
+std::atomic<int> g_debug_flags{0};
 std::mutex g_rot_mu;
 std::map<std::pair<int, size_t>, DevRotation> g_dev_rot;
 
@@ -132,8 +135,33 @@ DevRotation device_rotation(size_t dim) {
     hip_check(hipMalloc(&r.signs, dim * 4), "hipMalloc");
     hip_check(hipMemcpy(r.inv_perm, rt.inv_perm.data(), dim * 4, hipMemcpyHostToDevice), "copy");
     hip_check(hipMemcpy(r.signs, rt.signs.data(), dim * 4, hipMemcpyHostToDevice), "copy");
+    std::vector<uint32_t> ps(dim);
+    for (size_t j = 0; j < dim; ++j) {
+      const uint32_t i = rt.inv_perm[j];
+      ps[j] = i | (rt.signs[i] < 0.f ? 0x80000000u : 0u);
+    }
+    hip_check(hipMalloc(&r.perm_sign, dim * 4), "hipMalloc");
+    hip_check(hipMemcpy(r.perm_sign, ps.data(), dim * 4, hipMemcpyHostToDevice), "copy");
   }
   return r;
+}
+
+// Device copies of the quantizer cell tables, per (device, bits, dim).
+std::map<std::tuple<int, size_t, size_t>, void*> g_dev_cells;
+
+const void* device_cells(size_t bits, size_t dim) {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  const auto& ct = vchip::cell_table(bits, dim);
+  if (!ct.ok) return nullptr;
+  std::lock_guard lock(g_rot_mu);
+  void*& p = g_dev_cells[{dev, bits, dim}];
+  if (p == nullptr) {
+    static_assert(sizeof(vchip::CellEntry) == 16, "cell entry layout");
+    hip_check(hipMalloc(&p, sizeof(ct.cells)), "hipMalloc");
+    hip_check(hipMemcpy(p, ct.cells, sizeof(ct.cells), hipMemcpyHostToDevice), "copy");
+  }
+  return p;
 }
 
 vchip::CodeTables make_tables(size_t dim, size_t bits) {
@@ -153,6 +181,13 @@ vchip::CodeTables make_tables(size_t dim, size_t bits) {
   const DevRotation dr = device_rotation(dim);
   t.inv_perm = dr.inv_perm;
   t.signs = dr.signs;
+  t.perm_sign = dr.perm_sign;
+  const auto& ct = vchip::cell_table(bits, dim);
+  t.cells = device_cells(bits, dim);
+  t.use_cells = t.cells != nullptr ? 1 : 0;
+  t.cell_lo = ct.lo;
+  t.cell_inv_w = ct.inv_w;
+  t.debug_flags = g_debug_flags.load();
   return t;
 }
 
@@ -199,6 +234,7 @@ SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const f
   const size_t o_ls = c.take(nq * 4), o_lb = c.take(nq * 4), o_bc = c.take(nq * 4), o_bt = c.take(nq * 4);
   const size_t parts = nq * size_t(plan.n_ranges);
   const size_t o_ps = c.take(parts * ke * 4), o_pi = c.take(parts * ke * 4), o_pc = c.take(parts * 4);
+  const size_t o_lt = c.take(vchip::scan_lut_bytes(plan, t.groups16, nq));
   scratch.reserve(c.off, s);
   hip_check(vchip::launch_prepare_queries(t, q_dev, nq, scratch.as<uint8_t>(o_luts), scratch.as<float>(o_ls),
                                           scratch.as<float>(o_lb), scratch.as<float>(o_bc),
@@ -209,7 +245,8 @@ SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const f
   if (!probe) probe_lock.unlock();
   if (probe) hip_check(hipEventRecord(g_probe.start, s), "hipEventRecord");
   hip_check(vchip::launch_scan(plan, tiled, alpha, n, t.groups, t.groups16, scratch.as<uint8_t>(o_luts),
-                               scratch.as<float>(o_ls), scratch.as<float>(o_bt), nq, ke, scratch.as<float>(o_ps),
+                               scratch.as<uint8_t>(o_lt), scratch.as<float>(o_ls), scratch.as<float>(o_bt), nq, ke,
+                               scratch.as<float>(o_ps),
                                scratch.as<uint32_t>(o_pi), scratch.as<int>(o_pc), s),
             "scan");
   if (probe) {
@@ -373,6 +410,7 @@ vc_status vc_index_prepare(vc_index* ix) {
         ix->alpha.reserve(ncap * 4, ix->stream, /*keep=*/true);
         ix->cap = ncap;
       }
+      ix->t.debug_flags = g_debug_flags.load();
       hip_check(vchip::launch_encode(ix->t, ix->pending.as<float>(), ix->n_pending, ix->n_encoded,
                                      ix->tiled.as<uint8_t>(), nullptr, nullptr, ix->alpha.as<float>(), ix->stream),
                 "encode");
@@ -611,6 +649,24 @@ vc_status vc_encode(const float* vectors, size_t n, size_t dim, size_t bits, con
   });
 }
 
+vc_status vc_rotate(const float* vectors, size_t n, size_t dim, float* rotated_out) {
+  return guarded([&] {
+    check_dim_bits(dim, 4);
+    if (n == 0) return;
+    if (vectors == nullptr || rotated_out == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    Scoped sc;
+    vchip::CodeTables t = make_tables(dim, 4);
+    Carve c;
+    const size_t ox = c.take(n * dim * 4), orot = c.take(n * dim * 4);
+    sc.a.reserve(c.off, sc.s);
+    hip_check(hipMemcpyAsync(sc.a.as<float>(ox), vectors, n * dim * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(vchip::launch_encode(t, sc.a.as<float>(ox), n, 0, nullptr, nullptr, sc.a.as<float>(orot), nullptr, sc.s),
+              "encode(rotate)");
+    hip_check(hipMemcpyAsync(rotated_out, sc.a.as<float>(orot), n * dim * 4, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipStreamSynchronize(sc.s), "sync");
+  });
+}
+
 vc_status vc_fit_calibration(const float* sample, size_t n, size_t dim, size_t bits, float* shift_out,
                              float* scale_out) {
   return guarded([&] {
@@ -702,6 +758,10 @@ vc_status vc_set_scan_tile(size_t tile) {
     if (tile != 0 && tile != 1 && tile != 4 && tile != 8) fail(VC_ERR_INVALID_ARGUMENT, "tile must be 0, 1, 4 or 8");
     g_forced_tile.store(int(tile));
   });
+}
+
+vc_status vc_set_debug(unsigned flags) {
+  return guarded([&] { g_debug_flags.store(int(flags)); });
 }
 
 vc_status vc_set_profiling(int on) {

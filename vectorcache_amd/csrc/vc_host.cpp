@@ -6,6 +6,7 @@
 
 #include <array>
 #include <cmath>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -204,7 +205,44 @@ std::unique_ptr<Codebook> lloyd_max(std::size_t bits, std::size_t dim) {
   return cb;
 }
 
+// Device cell index of a value (the same float operations as the kernel).
+int cell_of(float v, float lo, float inv_w) {
+  const float f = (v - lo) * inv_w;
+  if (f >= float(kCells - 1)) return kCells - 1;
+  if (f >= 0.f) return int(f);
+  return 0;
+}
+
+std::unique_ptr<CellTable> make_cell_table(const Codebook& cb) {
+  auto t = std::make_unique<CellTable>();
+  const auto& b = cb.boundaries;
+  const std::size_t nb = b.size();
+  const float w = (b[nb - 1] - b[0]) / float(kCells - 2);
+  t->lo = b[0] - w;
+  t->inv_w = 1.f / w;
+  std::vector<int> cell(nb);
+  bool ok = w > 0.f && std::isfinite(t->inv_w);
+  for (std::size_t i = 0; ok && i < nb; ++i) {
+    cell[i] = cell_of(b[i], t->lo, t->inv_w);
+    if (i > 0 && cell[i] <= cell[i - 1]) ok = false;  // two boundaries share a cell
+  }
+  t->ok = ok;
+  if (!ok) return t;
+  const std::size_t top = cb.centroids.size() - 1;
+  for (int c = 0; c < kCells; ++c) {
+    std::uint32_t base = 0;
+    float bound = std::numeric_limits<float>::infinity();
+    for (std::size_t i = 0; i < nb; ++i) {
+      if (cell[i] < c) ++base;
+      if (cell[i] == c) bound = b[i];
+    }
+    t->cells[c] = CellEntry{base, bound, cb.centroids[base], cb.centroids[std::min<std::size_t>(base + 1, top)]};
+  }
+  return t;
+}
+
 std::mutex g_mu;
+std::map<std::pair<std::size_t, std::size_t>, std::unique_ptr<CellTable>> g_cells;
 std::map<std::size_t, std::unique_ptr<RotationTables>> g_rot;
 std::map<std::pair<std::size_t, std::size_t>, std::unique_ptr<Codebook>> g_cb;
 
@@ -225,6 +263,14 @@ const Codebook& codebook(std::size_t bits, std::size_t dim) {
   std::lock_guard lock(g_mu);
   auto& slot = g_cb[{bits, dim}];
   if (!slot) slot = lloyd_max(bits, dim);
+  return *slot;
+}
+
+const CellTable& cell_table(std::size_t bits, std::size_t dim) {
+  const Codebook& cb = codebook(bits, dim);
+  std::lock_guard lock(g_mu);
+  auto& slot = g_cells[{bits, dim}];
+  if (!slot) slot = make_cell_table(cb);
   return *slot;
 }
 

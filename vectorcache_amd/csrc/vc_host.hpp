@@ -36,6 +36,24 @@ const Codebook& codebook(std::size_t bits, std::size_t dim);
 // I_t(a, a), the regularized incomplete beta function.
 double beta_cdf_symmetric(double a, double t);
 
+// Quantizer cell table: a monotone cell function c(v) = clamp(trunc((v - lo) *
+// inv_w), 0, 255) evaluated in float exactly as the device does; every cell
+// holds at most one boundary, so code(v) = base[c] + (v > bound[c]) equals the
+// reference's count of boundaries below v for every float v (NaN -> 0).
+struct CellEntry {
+  std::uint32_t base;
+  float bound;   // +inf when the cell holds no boundary
+  float c_lo;    // centroid[base]
+  float c_hi;    // centroid[base + 1] (c_lo when base is the top code)
+};
+inline constexpr int kCells = 256;
+struct CellTable {
+  float lo = 0.f, inv_w = 0.f;
+  bool ok = false;  // false: the device counts boundaries one by one
+  CellEntry cells[kCells];
+};
+const CellTable& cell_table(std::size_t bits, std::size_t dim);
+
 struct TqAnchors {
   float qc_lo, qc_span;     // -c_outer, 2 c_outer
   std::size_t lo_idx, hi_idx;

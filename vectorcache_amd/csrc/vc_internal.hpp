@@ -20,6 +20,11 @@ struct CodeTables {
   const float* shift;             // device, dim; null = identity calibration
   const float* scale;             // device, dim
   const float* inv_scale;         // device, dim
+  const std::uint32_t* perm_sign;  // device, dim: inv_perm[j] | (signs[inv_perm[j]] < 0) << 31
+  const void* cells;              // device, 256 x {u32 base, f32 bound, f32 c_lo, f32 c_hi}
+  float cell_lo, cell_inv_w;
+  int use_cells;
+  int debug_flags;                // bit 0: take the sequential alpha path for every vector
 };
 
 // Internal scan layout ("tiled"): for vector v and byte group g the flat byte
@@ -58,6 +63,7 @@ struct ScanPlan {
   int range_len;     // vectors per range (multiple of 64)
   int lut_chunk16;   // 16-group chunks of LUT resident in LDS (all groups if it fits)
   int cand_cap;      // candidate buffer per query
+  int vec_per_thread;  // vectors per thread per step (2 when the LUT is reloaded in chunks)
   std::size_t lds_bytes;
 };
 
@@ -65,12 +71,17 @@ struct ScanPlan {
 inline constexpr int kMaxTopK = 2048;
 ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int forced_tile);
 
-// Scan + per-range exact top-k. Partial lists: part_s/part_i [nq][n_ranges][k],
-// part_cnt [nq][n_ranges].
+// Bytes of the query-interleaved LUT scratch the scan needs.
+std::size_t scan_lut_bytes(const ScanPlan& p, int groups16, std::size_t nq);
+
+// Scan + per-range exact top-k. luts: nq x groups*32 (prepare_queries);
+// lut_t: scratch of scan_lut_bytes. Partial lists: part_s/part_i
+// [nq][n_ranges][k], part_cnt [nq][n_ranges].
 hipError_t launch_scan(const ScanPlan& p, const std::uint8_t* tiled, const float* alpha,
                        std::size_t n, int groups, int groups16, const std::uint8_t* luts,
-                       const float* lut_scale, const float* bias_total, std::size_t nq, int k,
-                       float* part_s, std::uint32_t* part_i, int* part_cnt, hipStream_t s);
+                       std::uint8_t* lut_t, const float* lut_scale, const float* bias_total,
+                       std::size_t nq, int k, float* part_s, std::uint32_t* part_i, int* part_cnt,
+                       hipStream_t s);
 
 // Merge per-range lists into final rows of k_out entries (padded with 0/0).
 // id_map (device, may be null) maps vector positions to reported ids.

@@ -16,6 +16,12 @@ namespace vchip {
 namespace {
 
 constexpr float kMinNorm = 1e-10f;
+
+// Correctly rounded sqrt / division (IEEE, as the reference's sqrtf and '/').
+// Note: HIP's __fsqrt_rn maps to the approximate native sqrt unless
+// OCML_BASIC_ROUNDED_OPERATIONS is defined, so it is not used here.
+__device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float div_rn(float a, float b) { return a / b; }
 constexpr int kCandCap = 512;   // per-query candidate buffer of the scan
 constexpr int kScanThreads = 1024;
 constexpr int kLdsBudget = 160 * 1024;
@@ -64,7 +70,7 @@ __device__ float norm8(const float* __restrict__ xr, int D, int chain, bool acti
   c = c + __shfl_xor(c, 1);
   c = c + __shfl_xor(c, 2);
   c = c + __shfl_xor(c, 4);
-  return __fsqrt_rn(c);
+  return sqrt_rn(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -93,7 +99,7 @@ __global__ __launch_bounds__(64) void encode_kernel(CodeTables t, const float* _
     for (int rr = 0; rr < rows; ++rr) {
       const long row = batch * 8 + rr;
       const float nv = __shfl(nrm_mine, rr * 8);
-      const float inv = nv > kMinNorm ? __fdiv_rn(1.f, nv) : 0.f;
+      const float inv = nv > kMinNorm ? div_rn(1.f, nv) : 0.f;
       __syncthreads();
       rotate_row_lds(t, x + row * (long)D, inv, y, lane);
       if (rotated != nullptr)
@@ -126,7 +132,7 @@ __global__ __launch_bounds__(64) void encode_kernel(CodeTables t, const float* _
       const double pair = d + __shfl_xor(d, 1);
       const double inner = pair + __shfl_xor(pair, 2);
       const long v = out_offset + row;
-      if (lane == 0 && alpha != nullptr) alpha[v] = inner > 1e-12 ? __fdiv_rn(nv, (float)inner) : 0.f;
+      if (lane == 0 && alpha != nullptr) alpha[v] = inner > 1e-12 ? div_rn(nv, (float)inner) : 0.f;
       if (tiled != nullptr) {
         const int G = t.groups, G16 = t.groups16;
         for (int c16 = lane; c16 < G16; c16 += 64) {
@@ -159,6 +165,216 @@ __global__ __launch_bounds__(64) void encode_kernel(CodeTables t, const float* _
         }
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Encode, tiled fast path: blocks of B = 64 R elements, one wavefront per row,
+// 16 rows (a quarter of a 64-vector code tile) per wavefront iteration.
+//   * norms: 8 rows x 8 chains at a time (all 64 lanes, the reference order);
+//   * rotation: scatter through inv_perm into LDS, then per block lane l holds
+//     elements l R .. l R + R - 1: butterflies with len < R in registers, the
+//     rest across lanes (partner l ^ len / R) in the reference's stage order;
+//   * quantize through the cell table (one LDS read per element);
+//   * alpha: the reference's 4 sequential double chains give a value within
+//     (D/4 + 3) u sum|t| of the exact sum; a plain-double wavefront sum with
+//     its own bound brackets it. When both ends of the bracket round to the
+//     same float, that float IS the reference's (float) inner; otherwise the
+//     wavefront runs the reference's chains exactly (rare);
+//   * codes are staged per row in LDS and written as whole 16-byte pieces.
+// ---------------------------------------------------------------------------
+struct CellDev {
+  uint32_t base;
+  float bound, c_lo, c_hi;
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ unsigned quant_code(const CodeTables& t, const CellDev* __restrict__ cells, float v,
+                                               float& cen) {
+  if (t.use_cells) {
+    const float f = (v - t.cell_lo) * t.cell_inv_w;
+    int ci = 0;
+    if (f >= 255.f) ci = 255;
+    else if (f >= 0.f) ci = (int)f;
+    const CellDev e = cells[ci];
+    const bool up = v > e.bound;
+    cen = up ? e.c_hi : e.c_lo;
+    return e.base + (up ? 1u : 0u);
+  }
+  unsigned k = 0;
+  for (int b = 0; b < (1 << t.bits) - 1; ++b) k += v > t.boundaries[b] ? 1u : 0u;
+  cen = t.centroids[k];
+  return k;
+}
+
+constexpr int kEncodeWaves = 2;
+constexpr int kEncodeRows = 16;
+
+__host__ __device__ inline size_t enc_align(size_t v) { return (v + 15) & ~size_t(15); }
+__host__ __device__ inline size_t enc_wave_bytes(int D, int G16) {
+  return enc_align((size_t)D * 4) + enc_align((size_t)kEncodeRows * G16 * 16) + 2 * kEncodeRows * 4;
+}
+__host__ __device__ inline size_t enc_lds_bytes(int D, int G16) {
+  return enc_align((size_t)D * 4) + 256 * 16 + kEncodeWaves * enc_wave_bytes(D, G16);
+}
+
+template <int R>
+__global__ __launch_bounds__(64 * kEncodeWaves) void encode_tile_kernel(CodeTables t, const float* __restrict__ x,
+                                                                        long n, long out_offset,
+                                                                        uint8_t* __restrict__ tiled,
+                                                                        float* __restrict__ alpha_out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int D = t.dim, G16 = t.groups16, B = t.block, nblk = D / B;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t* ps = reinterpret_cast<uint32_t*>(smem);
+  CellDev* cells = reinterpret_cast<CellDev*>(smem + enc_align((size_t)D * 4));
+  unsigned char* wb = smem + enc_align((size_t)D * 4) + 256 * 16 + wave * enc_wave_bytes(D, G16);
+  float* y = reinterpret_cast<float*>(wb);
+  uint8_t* codes = wb + enc_align((size_t)D * 4);
+  float* nrm = reinterpret_cast<float*>(codes + enc_align((size_t)kEncodeRows * G16 * 16));
+  float* alp = nrm + kEncodeRows;
+
+  for (int j = threadIdx.x; j < D; j += blockDim.x) ps[j] = t.perm_sign[j];
+  if (t.use_cells)
+    for (int j = threadIdx.x; j < 256 * 4; j += blockDim.x)
+      reinterpret_cast<uint32_t*>(cells)[j] = reinterpret_cast<const uint32_t*>(t.cells)[j];
+  __syncthreads();
+
+  const int row_bytes = G16 * 16;
+  const int cpb = t.bits == 2 ? 4 : 2;  // codes per byte
+  const long s_begin = out_offset / kEncodeRows, s_end = (out_offset + n + kEncodeRows - 1) / kEncodeRows;
+  const long stride = (long)gridDim.x * kEncodeWaves;
+  for (long st = s_begin + (long)blockIdx.x * kEncodeWaves + wave; st < s_end; st += stride) {
+    const long v0 = st * kEncodeRows;
+    for (int i = lane; i < kEncodeRows * row_bytes / 4; i += 64) reinterpret_cast<uint32_t*>(codes)[i] = 0u;
+    for (int rnd = 0; rnd < kEncodeRows / 8; ++rnd) {
+      const int q = rnd * 8 + (lane >> 3);
+      const long row = v0 + q - out_offset;
+      const bool ok = row >= 0 && row < n;
+      const float nv = norm8(x + (ok ? row : 0) * (long)D, D, lane & 7, ok);
+      if ((lane & 7) == 0) nrm[q] = nv;
+    }
+    wave_lds_sync();
+    for (int q = 0; q < kEncodeRows; ++q) {
+      const long row = v0 + q - out_offset;
+      if (row < 0 || row >= n) continue;  // uniform across the wavefront
+      const float nv = nrm[q];
+      const float inv = nv > kMinNorm ? div_rn(1.f, nv) : 0.f;
+      const float* xr = x + row * (long)D;
+      for (int j = lane; j < D; j += 64) {
+        const uint32_t w = ps[j];
+        y[w & 0x7fffffffu] = (xr[j] * inv) * ((w >> 31) ? -1.f : 1.f);
+      }
+      wave_lds_sync();
+      double sum = 0.0, asum = 0.0;
+      for (int b = 0; b < nblk; ++b) {
+        float* yb = y + b * B + lane * R;
+        float v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = yb[r];
+#pragma unroll
+        for (int len = 1; len < R; len <<= 1) {
+#pragma unroll
+          for (int i = 0; i < R; i += 2 * len) {
+#pragma unroll
+            for (int j = i; j < i + len; ++j) {
+              const float a = v[j], c = v[j + len];
+              v[j] = a + c;
+              v[j + len] = a - c;
+            }
+          }
+        }
+#pragma unroll
+        for (int lm = 1; lm < 64; lm <<= 1) {
+          const bool upper = (lane & lm) != 0;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const float p = __shfl_xor(v[r], lm);
+            v[r] = upper ? p - v[r] : v[r] + p;
+          }
+        }
+        uint64_t cw = 0;  // this lane's codes of the block, packed as flat bytes
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const float yv = v[r] * t.inv_sqrt_block;
+          yb[r] = yv;  // post-rotation values, read only by the exact alpha path
+          const int j = b * B + lane * R + r;
+          float cen;
+          const unsigned k = quant_code(t, cells, t.shift ? (yv + t.shift[j]) * t.scale[j] : yv, cen);
+          double term;
+          if (t.shift == nullptr)
+            term = (double)yv * (double)cen;
+          else
+            term = (double)yv * fma((double)cen, (double)t.inv_scale[j], -(double)t.shift[j]);
+          sum += term;
+          asum += fabs(term);
+          const int slot = r % cpb;  // position inside its byte, first code highest
+          const int shift = (cpb - 1 - slot) * (8 / cpb);
+          cw |= (uint64_t)k << (8 * (r / cpb) + shift);
+        }
+        uint8_t* dst = codes + q * row_bytes + (b * B + lane * R) / cpb;
+#pragma unroll
+        for (int i = 0; i < R / cpb; ++i) dst[i] = (uint8_t)(cw >> (8 * i));
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        sum += __shfl_xor(sum, o);
+        asum += __shfl_xor(asum, o);
+      }
+      const double u = 1.1102230246251565e-16;  // 2^-53
+      double bound = (double)(D / 4 + D / 64 + 16) * u * asum * 1.01;
+      bound += 2.0 * u * (fabs(sum) + bound);
+      const double lo = sum - bound, hi = sum + bound;
+      float a = 0.f;
+      bool exact = (t.debug_flags & 1) == 0;
+      if (exact && lo > 1e-12 && __double2float_rn(lo) == __double2float_rn(hi))
+        a = div_rn(nv, __double2float_rn(hi));
+      else if (exact && hi <= 1e-12)
+        a = 0.f;
+      else
+        exact = false;
+      if (!exact) {
+        // the reference's chains, in order (src/encode/encode.cpp:112-157)
+        wave_lds_sync();
+        double d = 0.0;
+        if (lane < 4) {
+          for (int j = lane; j < D; j += 4) {
+            const float yv = y[j];
+            float cen;
+            (void)quant_code(t, cells, t.shift ? (yv + t.shift[j]) * t.scale[j] : yv, cen);
+            if (t.shift == nullptr)
+              d = fma((double)yv, (double)cen, d);
+            else
+              d = fma((double)yv, fma((double)cen, (double)t.inv_scale[j], -(double)t.shift[j]), d);
+          }
+        }
+        const double pr = d + __shfl_xor(d, 1);
+        const double inner = pr + __shfl_xor(pr, 2);
+        a = inner > 1e-12 ? div_rn(nv, (float)inner) : 0.f;
+      }
+      if (lane == 0) alp[q] = a;
+      wave_lds_sync();
+    }
+    // write the 16 rows' codes as 16-byte pieces and their alphas
+    for (int c16 = lane >> 4; c16 < G16; c16 += 4) {
+      const int q = lane & 15;
+      const long v = v0 + q;
+      const long row = v - out_offset;
+      if (row >= 0 && row < n) {
+        const uint4 val = *reinterpret_cast<const uint4*>(codes + q * row_bytes + c16 * 16);
+        *reinterpret_cast<uint4*>(tiled + ((v >> 6) * G16 + c16) * 1024 + (v & 63) * 16) = val;
+      }
+    }
+    if (lane < kEncodeRows) {
+      const long v = v0 + lane;
+      const long row = v - out_offset;
+      if (row >= 0 && row < n) alpha_out[v] = alp[lane];
+    }
+    wave_lds_sync();
   }
 }
 
@@ -244,12 +460,12 @@ __global__ __launch_bounds__(256) void radix_select_kernel(const uint32_t* __res
     const float span = q_hi - q_lo;
     float s = 1.f, sft = 0.f;
     if (span > 1e-12f && qc_span > 0.f) {
-      s = __fdiv_rn(qc_span, span);
-      sft = __fdiv_rn(qc_lo, s) - q_lo;
+      s = div_rn(qc_span, span);
+      sft = div_rn(qc_lo, s) - q_lo;
     }
     shift[d] = sft;
     scale[d] = s;
-    inv_scale[d] = __fdiv_rn(1.f, s);
+    inv_scale[d] = div_rn(1.f, s);
   }
 }
 
@@ -284,7 +500,7 @@ __global__ __launch_bounds__(64) void prepare_queries_kernel(CodeTables t, const
   for (long qi = blockIdx.x; qi < nq; qi += gridDim.x) {
     const float* xr = q + qi * (long)D;
     const float nrm = __shfl(norm8(xr, D, lane & 7, lane < 8), 0);
-    const float inv = nrm > kMinNorm ? __fdiv_rn(1.f, nrm) : 0.f;
+    const float inv = nrm > kMinNorm ? div_rn(1.f, nrm) : 0.f;
     __syncthreads();
     rotate_row_lds(t, xr, inv, y, lane);
     if (t.shift != nullptr) {
@@ -295,7 +511,7 @@ __global__ __launch_bounds__(64) void prepare_queries_kernel(CodeTables t, const
         s_bc = (float)acc;
       }
       __syncthreads();
-      for (int d = lane; d < D; d += 64) y[d] = __fdiv_rn(y[d], t.scale[d]);
+      for (int d = lane; d < D; d += 64) y[d] = div_rn(y[d], t.scale[d]);
     } else if (lane == 0) {
       s_bc = 0.f;
     }
@@ -322,10 +538,10 @@ __global__ __launch_bounds__(64) void prepare_queries_kernel(CodeTables t, const
       s_bias = b;
     }
     __syncthreads();
-    float scale = span > 0.f ? __fdiv_rn(span, 127.f) : 1.f;
+    float scale = span > 0.f ? div_rn(span, 127.f) : 1.f;
     float iscale = 1.f;
     if (scale >= FLT_MIN)
-      iscale = __fdiv_rn(1.f, scale);
+      iscale = div_rn(1.f, scale);
     else
       scale = 1.f;
     // pass 2: quantized entries
@@ -441,10 +657,29 @@ template <> struct LutWord<1> { typedef uint8_t type; };
 template <> struct LutWord<4> { typedef uint32_t type; };
 template <> struct LutWord<8> { typedef uint2 type; };
 
-template <int T>
+// Query-interleaved LUTs of one query tile: byte (g * 32 + e) * T + t holds
+// entry e of group g of query q0 + t; padded groups / queries are zero.
+__global__ void interleave_luts_kernel(const uint8_t* __restrict__ luts, long nq, int G, int G16, int T,
+                                       uint8_t* __restrict__ out) {
+  const long n_qt = (nq + T - 1) / T;
+  const long per_tile = (long)G16 * 16 * 32 * T;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n_qt * per_tile; i += (long)gridDim.x * blockDim.x) {
+    const long qt = i / per_tile;
+    const long r = i - qt * per_tile;
+    const int t = (int)(r % T);
+    const int e = (int)((r / T) & 31);
+    const int g = (int)(r / (T * 32));
+    const long q = qt * T + t;
+    out[i] = (g < G && q < nq) ? luts[q * (long)G * 32 + (long)g * 32 + e] : 0;
+  }
+}
+
+__device__ __forceinline__ uint32_t nib(uint32_t w, int bitpos) { return __builtin_amdgcn_ubfe(w, bitpos, 4); }
+
+template <int T, int V>
 __global__ __launch_bounds__(kScanThreads) void scan_topk_kernel(
-    const uint8_t* __restrict__ tiled, const float* __restrict__ alpha, long n, int G, int G16,
-    int chunk16, const uint8_t* __restrict__ luts, const float* __restrict__ lut_scale,
+    const uint8_t* __restrict__ tiled, const float* __restrict__ alpha, long n, int G16, int chunk16,
+    const uint8_t* __restrict__ lut_t, const float* __restrict__ lut_scale,
     const float* __restrict__ bias_total, long nq, int k, int cap, int n_ranges, long range_len,
     float* __restrict__ part_s, uint32_t* __restrict__ part_i, int* __restrict__ part_cnt) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -487,42 +722,36 @@ __global__ __launch_bounds__(kScanThreads) void scan_topk_kernel(
     bias[t] = bias_total[q];
   }
 
-  // Fill the LDS LUT for 16-group chunks [c0, c0 + chunk16): interleave the
-  // T queries; groups >= G (padding of the last chunk) are zero.
+  // LDS LUT for 16-group chunks [c0, c0 + chunk16): a straight copy of the
+  // query tile's interleaved LUT.
+  const uint4* tile_lut = reinterpret_cast<const uint4*>(lut_t + qt * ((long)G16 * 16 * 32 * T));
   auto load_lut = [&](int c0) {
-    const int g0 = c0 * 16;
-    const int ng = min(chunk16, G16 - c0) * 16;
-    const int words = ng * 32 * T / 4;
-    uint32_t* dst = reinterpret_cast<uint32_t*>(lut_s);
-    for (int w = tid; w < words; w += blockDim.x) {
-      uint32_t word = 0;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int byte = w * 4 + b;
-        const int t = byte % T;
-        const int e = (byte / T) % 32;
-        const int g = byte / (T * 32) + g0;
-        const long q = q0 + t;
-        uint32_t val = 0;
-        if (g < G && q < nq) val = luts[q * (long)G * 32 + (long)g * 32 + e];
-        word |= val << (8 * b);
-      }
-      dst[w] = word;
-    }
+    const int n16 = min(chunk16, G16 - c0) * 16 * 32 * T / 16;
+    const uint4* src = tile_lut + (long)c0 * 16 * 32 * T / 16;
+    uint4* dst = reinterpret_cast<uint4*>(lut_s);
+    for (int w = tid; w < n16; w += blockDim.x) dst[w] = src[w];
   };
   const bool one_chunk = chunk16 >= G16;
   if (one_chunk) load_lut(0);
   __syncthreads();
 
+  constexpr int H = T >= 4 ? T / 4 : 1;   // 32-bit LUT words per entry
   typedef typename LutWord<T>::type W;
-  for (long base = r_begin; base < r_end; base += blockDim.x) {
-    const long v = base + tid;
-    const bool valid = v < r_end;
-    const uint4* cp = reinterpret_cast<const uint4*>(tiled) + ((valid ? v : 0) >> 6) * (long)G16 * 64 +
-                      ((valid ? v : 0) & 63);
-    float fa[T];
+  const long step = (long)blockDim.x * V;
+  for (long base = r_begin; base < r_end; base += step) {
+    long vv[V];
+    bool valid[V];
+    const uint4* cp[V];
+    float fa[V][T];
 #pragma unroll
-    for (int t = 0; t < T; ++t) fa[t] = bias[t];
+    for (int j = 0; j < V; ++j) {
+      vv[j] = base + tid + (long)j * blockDim.x;
+      valid[j] = vv[j] < r_end;
+      const long vs = valid[j] ? vv[j] : 0;
+      cp[j] = reinterpret_cast<const uint4*>(tiled) + (vs >> 6) * (long)G16 * 64 + (vs & 63);
+#pragma unroll
+      for (int t = 0; t < T; ++t) fa[j][t] = bias[t];
+    }
 
     for (int c0 = 0; c0 < G16; c0 += chunk16) {
       if (!one_chunk) {
@@ -531,105 +760,112 @@ __global__ __launch_bounds__(kScanThreads) void scan_topk_kernel(
         __syncthreads();
       }
       const int c1 = min(G16, c0 + chunk16);
-      // accumulators of the current window
-      uint32_t accA[T >= 4 ? T / 4 : 1], accB[T >= 4 ? T / 4 : 1];
+      uint32_t accA[V][H], accB[V][H];
 #pragma unroll
-      for (int h = 0; h < (T >= 4 ? T / 4 : 1); ++h) { accA[h] = 0u; accB[h] = 0u; }
+      for (int j = 0; j < V; ++j)
+#pragma unroll
+        for (int h = 0; h < H; ++h) { accA[j][h] = 0u; accB[j][h] = 0u; }
       for (int g16 = c0; g16 < c1; ++g16) {
-        const uint4 c = valid ? cp[(long)g16 * 64] : make_uint4(0u, 0u, 0u, 0u);
         const W* lw = reinterpret_cast<const W*>(lut_s) + (long)(g16 - c0) * 16 * 32;
-        const uint32_t words[4] = {c.x, c.y, c.z, c.w};
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-          const uint32_t byte = (words[e >> 2] >> ((e & 3) * 8)) & 255u;
-          const W rh = lw[e * 32 + (byte >> 4)];
-          const W rl = lw[e * 32 + 16 + (byte & 15u)];
-          if constexpr (T == 1) {
-            accA[0] += uint32_t(rh) + uint32_t(rl);
-          } else if constexpr (T == 4) {
-            const uint32_t rr = rh + rl;  // bytes <= 254: no carries
-            u16x2 a = __builtin_bit_cast(u16x2, accA[0]) + __builtin_bit_cast(u16x2, rr);
-            u16x2 b = __builtin_bit_cast(u16x2, accB[0]) + (__builtin_bit_cast(u16x2, rr) >> (unsigned short)8);
-            accA[0] = __builtin_bit_cast(uint32_t, a);
-            accB[0] = __builtin_bit_cast(uint32_t, b);
-          } else {
-            const uint32_t r0 = rh.x + rl.x, r1 = rh.y + rl.y;
-            u16x2 a0 = __builtin_bit_cast(u16x2, accA[0]) + __builtin_bit_cast(u16x2, r0);
-            u16x2 b0 = __builtin_bit_cast(u16x2, accB[0]) + (__builtin_bit_cast(u16x2, r0) >> (unsigned short)8);
-            u16x2 a1 = __builtin_bit_cast(u16x2, accA[1]) + __builtin_bit_cast(u16x2, r1);
-            u16x2 b1 = __builtin_bit_cast(u16x2, accB[1]) + (__builtin_bit_cast(u16x2, r1) >> (unsigned short)8);
-            accA[0] = __builtin_bit_cast(uint32_t, a0);
-            accB[0] = __builtin_bit_cast(uint32_t, b0);
-            accA[1] = __builtin_bit_cast(uint32_t, a1);
-            accB[1] = __builtin_bit_cast(uint32_t, b1);
+        for (int j = 0; j < V; ++j) {
+          const uint4 c = valid[j] ? cp[j][(long)g16 * 64] : make_uint4(0u, 0u, 0u, 0u);
+          const uint32_t words[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            const uint32_t w = words[e >> 2];
+            const int bp = (e & 3) * 8;
+            const W rh = lw[e * 32 + nib(w, bp + 4)];
+            const W rl = lw[e * 32 + 16 + nib(w, bp)];
+            if constexpr (T == 1) {
+              accA[j][0] += uint32_t(rh) + uint32_t(rl);
+            } else if constexpr (T == 4) {
+              const uint32_t rr = rh + rl;  // bytes <= 254: no carries
+              accA[j][0] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accA[j][0]) + __builtin_bit_cast(u16x2, rr));
+              accB[j][0] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accB[j][0]) +
+                                                            (__builtin_bit_cast(u16x2, rr) >> (unsigned short)8));
+            } else {
+              const uint32_t r0 = rh.x + rl.x, r1 = rh.y + rl.y;
+              accA[j][0] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accA[j][0]) + __builtin_bit_cast(u16x2, r0));
+              accB[j][0] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accB[j][0]) +
+                                                            (__builtin_bit_cast(u16x2, r0) >> (unsigned short)8));
+              accA[j][1] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accA[j][1]) + __builtin_bit_cast(u16x2, r1));
+              accB[j][1] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accB[j][1]) +
+                                                            (__builtin_bit_cast(u16x2, r1) >> (unsigned short)8));
+            }
           }
         }
         if (((g16 + 1) & 15) == 0 || g16 == G16 - 1) {
-          // window flush: exact u16 sums per query -> float chain
-          if constexpr (T == 1) {
-            fa[0] = __fmaf_rn(scl[0], (float)accA[0], fa[0]);
-            accA[0] = 0u;
-          } else {
+          // window flush: exact u16 sums per query -> the reference's float chain
 #pragma unroll
-            for (int h = 0; h < T / 4; ++h) {
-              const u16x2 a = __builtin_bit_cast(u16x2, accA[h]);
-              const u16x2 b = __builtin_bit_cast(u16x2, accB[h]);
-              const uint32_t s0 = (unsigned short)(a.x - (unsigned short)(b.x << 8));
-              const uint32_t s1 = b.x;
-              const uint32_t s2 = (unsigned short)(a.y - (unsigned short)(b.y << 8));
-              const uint32_t s3 = b.y;
-              fa[4 * h + 0] = __fmaf_rn(scl[4 * h + 0], (float)s0, fa[4 * h + 0]);
-              fa[4 * h + 1] = __fmaf_rn(scl[4 * h + 1], (float)s1, fa[4 * h + 1]);
-              fa[4 * h + 2] = __fmaf_rn(scl[4 * h + 2], (float)s2, fa[4 * h + 2]);
-              fa[4 * h + 3] = __fmaf_rn(scl[4 * h + 3], (float)s3, fa[4 * h + 3]);
-              accA[h] = 0u;
-              accB[h] = 0u;
+          for (int j = 0; j < V; ++j) {
+            if constexpr (T == 1) {
+              fa[j][0] = __fmaf_rn(scl[0], (float)accA[j][0], fa[j][0]);
+              accA[j][0] = 0u;
+            } else {
+#pragma unroll
+              for (int h = 0; h < H; ++h) {
+                const u16x2 a = __builtin_bit_cast(u16x2, accA[j][h]);
+                const u16x2 b = __builtin_bit_cast(u16x2, accB[j][h]);
+                const uint32_t s0 = (unsigned short)(a.x - (unsigned short)(b.x << 8));
+                const uint32_t s1 = b.x;
+                const uint32_t s2 = (unsigned short)(a.y - (unsigned short)(b.y << 8));
+                const uint32_t s3 = b.y;
+                fa[j][4 * h + 0] = __fmaf_rn(scl[4 * h + 0], (float)s0, fa[j][4 * h + 0]);
+                fa[j][4 * h + 1] = __fmaf_rn(scl[4 * h + 1], (float)s1, fa[j][4 * h + 1]);
+                fa[j][4 * h + 2] = __fmaf_rn(scl[4 * h + 2], (float)s2, fa[j][4 * h + 2]);
+                fa[j][4 * h + 3] = __fmaf_rn(scl[4 * h + 3], (float)s3, fa[j][4 * h + 3]);
+                accA[j][h] = 0u;
+                accB[j][h] = 0u;
+              }
             }
           }
         }
       }
     }
 
-    // scores; -0.f reported as +0.f
-    float s[T];
-    const float av = valid ? alpha[v] : 0.f;
+    // scores (-0.f reported as +0.f) and admission with overflow retry
 #pragma unroll
-    for (int t = 0; t < T; ++t) s[t] = valid ? fa[t] * av + 0.f : -INFINITY;
-
-    // admission with overflow retry
-    uint32_t pend = 0u;
+    for (int j = 0; j < V; ++j) {
+      float s[T];
+      const float av = valid[j] ? alpha[vv[j]] : 0.f;
 #pragma unroll
-    for (int t = 0; t < T; ++t)
-      if (valid && q0 + t < nq && better(s[t], (uint32_t)v, sh.thr_s[t], sh.thr_i[t])) pend |= 1u << t;
-    while (true) {
+      for (int t = 0; t < T; ++t) s[t] = valid[j] ? fa[j][t] * av + 0.f : -INFINITY;
+      const uint32_t vid = (uint32_t)vv[j];
+      uint32_t pend = 0u;
 #pragma unroll
-      for (int t = 0; t < T; ++t) {
-        if (pend & (1u << t)) {
-          const int slot = atomicAdd(&sh.cnt[t], 1);
-          if (slot < cap) {
-            sh.cand_s[t * cap + slot] = s[t];
-            sh.cand_i[t * cap + slot] = (uint32_t)v;
-            pend &= ~(1u << t);
+      for (int t = 0; t < T; ++t)
+        if (valid[j] && q0 + t < nq && better(s[t], vid, sh.thr_s[t], sh.thr_i[t])) pend |= 1u << t;
+      while (true) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          if (pend & (1u << t)) {
+            const int slot = atomicAdd(&sh.cnt[t], 1);
+            if (slot < cap) {
+              sh.cand_s[t * cap + slot] = s[t];
+              sh.cand_i[t * cap + slot] = vid;
+              pend &= ~(1u << t);
+            }
           }
         }
+        __syncthreads();
+        bool overflow = false;
+        bool need[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          const int c = sh.cnt[t];
+          overflow |= c > cap;
+          need[t] = c > cap / 2;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+          if (need[t]) topk_merge(sh, t, k, cap);
+        if (!overflow) break;
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+          if ((pend & (1u << t)) && !better(s[t], vid, sh.thr_s[t], sh.thr_i[t])) pend &= ~(1u << t);
       }
-      __syncthreads();
-      bool overflow = false;
-      bool need[T];
-#pragma unroll
-      for (int t = 0; t < T; ++t) {
-        const int c = sh.cnt[t];
-        overflow |= c > cap;
-        need[t] = c > cap / 2;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int t = 0; t < T; ++t)
-        if (need[t]) topk_merge(sh, t, k, cap);
-      if (!overflow) break;
-#pragma unroll
-      for (int t = 0; t < T; ++t)
-        if ((pend & (1u << t)) && !better(s[t], (uint32_t)v, sh.thr_s[t], sh.thr_i[t])) pend &= ~(1u << t);
     }
   }
   // final merge and output
@@ -889,6 +1125,27 @@ hipError_t launch_encode(const CodeTables& t, const float* x, std::size_t n, std
                          std::uint8_t* tiled, std::uint8_t* packed, float* rotated, float* alpha,
                          hipStream_t s) {
   if (n == 0) return hipSuccess;
+  const int R = t.block / 64;
+  const bool fast_shape = (R == 2 && t.bits != 2) || R == 4 || R == 8 || R == 16;
+  if (tiled != nullptr && alpha != nullptr && packed == nullptr && rotated == nullptr && fast_shape &&
+      t.perm_sign != nullptr && (t.bits != 2 || R >= 4) && enc_lds_bytes(t.dim, t.groups16) <= (size_t)kLdsBudget) {
+    const size_t lds = enc_lds_bytes(t.dim, t.groups16);
+    const long tiles = (long)((out_offset + n + kEncodeRows - 1) / kEncodeRows - out_offset / kEncodeRows);
+    const int grid = grid_for((tiles + kEncodeWaves - 1) / kEncodeWaves, 1, 8192);
+#define VC_ENC(RR)                                                                                          \
+  do {                                                                                                      \
+    (void)hipFuncSetAttribute((const void*)encode_tile_kernel<RR>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                              (int)lds);                                                                    \
+    hipLaunchKernelGGL(encode_tile_kernel<RR>, dim3(grid), dim3(64 * kEncodeWaves), lds, s, t, x, (long)n,      \
+                       (long)out_offset, tiled, alpha);                                                     \
+  } while (0)
+    if (R == 2) VC_ENC(2);
+    else if (R == 4) VC_ENC(4);
+    else if (R == 8) VC_ENC(8);
+    else VC_ENC(16);
+#undef VC_ENC
+    return hipGetLastError();
+  }
   const size_t lds = (size_t)t.dim * 5;
   const int grid = grid_for((long)(n + 7) / 8, 1, 16384);
   hipLaunchKernelGGL(encode_kernel, dim3(grid), dim3(64), lds, s, t, x, (long)n, (long)out_offset, tiled,
@@ -937,9 +1194,11 @@ ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int force
     if (chunk >= groups16) {
       chunk = groups16;
     } else {
-      chunk = (chunk / 16) * 16;
-      // multi-chunk mode reloads the LUT per step: only worth it for T = 1
-      if (forced_tile == 0 && T != 1) continue;
+      // windows (16 chunks of 16 groups) never straddle a reload; balance the
+      // chunks so the last one is not a sliver
+      const int max_chunk = (chunk / 16) * 16;
+      const int n_chunks = (groups16 + max_chunk - 1) / max_chunk;
+      chunk = ((groups16 + n_chunks - 1) / n_chunks + 15) / 16 * 16;
     }
     if (chunk < 1) continue;
     p.tile = T;
@@ -948,9 +1207,12 @@ ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int force
     break;
   }
   if (p.tile == 0) return p;  // caller reports the error
+  const bool multi = p.lut_chunk16 < groups16;
+  p.vec_per_thread = multi ? 2 : 1;  // amortize LUT reloads over more vectors
+  const long step = (long)kScanThreads * p.vec_per_thread;
   const long n_qt = (long)((nq + p.tile - 1) / p.tile);
   long R = (512 + n_qt - 1) / n_qt;
-  const long max_by_n = (long)((n + 1023) / 1024);
+  const long max_by_n = (long)((n + step - 1) / step);
   if (R > max_by_n) R = max_by_n;
   if (R > 512) R = 512;
   if (R < 1) R = 1;
@@ -963,27 +1225,37 @@ ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int force
   return p;
 }
 
+std::size_t scan_lut_bytes(const ScanPlan& p, int groups16, std::size_t nq) {
+  const std::size_t n_qt = (nq + p.tile - 1) / p.tile;
+  return n_qt * (std::size_t)groups16 * 16 * 32 * p.tile;
+}
+
 hipError_t launch_scan(const ScanPlan& p, const std::uint8_t* tiled, const float* alpha, std::size_t n,
-                       int groups, int groups16, const std::uint8_t* luts, const float* lut_scale,
-                       const float* bias_total, std::size_t nq, int k, float* part_s,
+                       int groups, int groups16, const std::uint8_t* luts, std::uint8_t* lut_t,
+                       const float* lut_scale, const float* bias_total, std::size_t nq, int k, float* part_s,
                        std::uint32_t* part_i, int* part_cnt, hipStream_t s) {
   const long n_qt = (long)((nq + p.tile - 1) / p.tile);
+  hipLaunchKernelGGL(interleave_luts_kernel, dim3(grid_for((long)scan_lut_bytes(p, groups16, nq), 256)), dim3(256),
+                     0, s, luts, (long)nq, groups, groups16, p.tile, lut_t);
   const dim3 grid((unsigned)(n_qt * p.n_ranges));
   const dim3 block(kScanThreads);
-#define VC_LAUNCH_SCAN(TT)                                                                            \
-  do {                                                                                                \
-    (void)hipFuncSetAttribute((const void*)scan_topk_kernel<TT>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                        (int)p.lds_bytes);                                                            \
-    hipLaunchKernelGGL(scan_topk_kernel<TT>, grid, block, p.lds_bytes, s, tiled, alpha, (long)n, groups,  \
-                       groups16, p.lut_chunk16, luts, lut_scale, bias_total, (long)nq, k, p.cand_cap,  \
-                       p.n_ranges, (long)p.range_len, part_s, part_i, part_cnt);                      \
+#define VC_LAUNCH_SCAN(TT, VV)                                                                          \
+  do {                                                                                                  \
+    (void)hipFuncSetAttribute((const void*)scan_topk_kernel<TT, VV>,                                     \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_bytes);            \
+    hipLaunchKernelGGL((scan_topk_kernel<TT, VV>), grid, block, p.lds_bytes, s, tiled, alpha, (long)n,    \
+                       groups16, p.lut_chunk16, lut_t, lut_scale, bias_total, (long)nq, k, p.cand_cap,  \
+                       p.n_ranges, (long)p.range_len, part_s, part_i, part_cnt);                        \
   } while (0)
-  if (p.tile == 8)
-    VC_LAUNCH_SCAN(8);
-  else if (p.tile == 4)
-    VC_LAUNCH_SCAN(4);
-  else
-    VC_LAUNCH_SCAN(1);
+  if (p.vec_per_thread == 2) {
+    if (p.tile == 8) VC_LAUNCH_SCAN(8, 2);
+    else if (p.tile == 4) VC_LAUNCH_SCAN(4, 2);
+    else VC_LAUNCH_SCAN(1, 2);
+  } else {
+    if (p.tile == 8) VC_LAUNCH_SCAN(8, 1);
+    else if (p.tile == 4) VC_LAUNCH_SCAN(4, 1);
+    else VC_LAUNCH_SCAN(1, 1);
+  }
 #undef VC_LAUNCH_SCAN
   return hipGetLastError();
 }

@@ -218,6 +218,15 @@ def encode(vectors, bits: int, shift=None, scale=None):
     return packed, alpha
 
 
+def rotate(vectors):
+    """rotate_batch_into: rows scaled by 1/||row|| and rotated (n x dim)."""
+    x = _f32(vectors)
+    n, dim = x.shape
+    out = np.zeros((n, dim), np.float32)
+    check(lib.vc_rotate(_ptr(x), S(n), S(dim), _ptr(out)))
+    return out
+
+
 def fit_calibration(sample, bits: int):
     x = _f32(sample)
     n, dim = x.shape
@@ -268,6 +277,11 @@ def set_scan_tile(tile: int):
 
 def scan_tile_for(dim: int, bits: int, nq: int) -> int:
     return int(lib.vc_scan_tile_for(S(dim), S(bits), S(nq)))
+
+
+def set_debug(flags: int):
+    """Test hook: bit 0 forces the exact sequential alpha path in encode."""
+    check(lib.vc_set_debug(ctypes.c_uint(flags)))
 
 
 def set_profiling(on: bool):
