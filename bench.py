@@ -185,6 +185,24 @@ def main():
     achieved = scan_bytes / scan_avg_s / 1e9
     lookups = nq * n * 2 * G / scan_avg_s
 
+    # ---- bandwidth regime: small query batches, scan kernel alone ----
+    bw = []
+    for nqb in (1, 8):
+        qb = q[:nqb].contiguous()
+        sb = torch.empty((nqb, k), dtype=torch.float32, device=dev)
+        ib = torch.empty((nqb, k), dtype=torch.int64, device=dev)
+        ms_b = []
+        for it in range(12):
+            ix.search_device(qb, k, sb, ib, stream=stream)
+            ms_i, tile_b, ranges_b = vc.last_scan_stats()
+            if it >= 2:
+                ms_b.append(ms_i)
+        t_b = float(np.median(ms_b)) / 1e3
+        bytes_b = math.ceil(nqb / tile_b) * n * (G + 4) + nqb * G * 32
+        bw.append({"nq": nqb, "query_tile": tile_b, "vector_ranges": ranges_b, "scan_ms": t_b * 1e3,
+                   "achieved_gbs": bytes_b / t_b / 1e9, "frac": bytes_b / t_b / 1e9 / HBM_PEAK_GBS})
+    log(f"bandwidth regime {bw}")
+
     # ---- quality: recall@10 vs FP64 exact on a query sample ----
     ns = min(200, nq)
     es, ei = exact_topk(x, q[:ns], 10)
@@ -248,6 +266,7 @@ def main():
         },
         "encode": {"vectors_per_s": n / enc_s, "ms": enc_s * 1e3, "gb_per_s": enc_bytes / enc_s / 1e9,
                    "frac": enc_bytes / enc_s / 1e9 / HBM_PEAK_GBS},
+        "roofline_bandwidth_regime": bw,
         "recall_at_10_vs_fp64": recall,
         "latency": lat,
         "cpu_baseline": cpu,

@@ -765,58 +765,70 @@ __global__ __launch_bounds__(kScanThreads) void scan_topk_kernel(
       for (int j = 0; j < V; ++j)
 #pragma unroll
         for (int h = 0; h < H; ++h) { accA[j][h] = 0u; accB[j][h] = 0u; }
-      for (int g16 = c0; g16 < c1; ++g16) {
-        const W* lw = reinterpret_cast<const W*>(lut_s) + (long)(g16 - c0) * 16 * 32;
+      constexpr int NB = V == 1 ? 4 : 2;  // 16-byte code loads in flight per vector
+      for (int gb = c0; gb < c1; gb += NB) {
+        uint4 cb[V][NB];
 #pragma unroll
-        for (int j = 0; j < V; ++j) {
-          const uint4 c = valid[j] ? cp[j][(long)g16 * 64] : make_uint4(0u, 0u, 0u, 0u);
-          const uint32_t words[4] = {c.x, c.y, c.z, c.w};
+        for (int j = 0; j < V; ++j)
 #pragma unroll
-          for (int e = 0; e < 16; ++e) {
-            const uint32_t w = words[e >> 2];
-            const int bp = (e & 3) * 8;
-            const W rh = lw[e * 32 + nib(w, bp + 4)];
-            const W rl = lw[e * 32 + 16 + nib(w, bp)];
-            if constexpr (T == 1) {
-              accA[j][0] += uint32_t(rh) + uint32_t(rl);
-            } else if constexpr (T == 4) {
-              const uint32_t rr = rh + rl;  // bytes <= 254: no carries
-              accA[j][0] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accA[j][0]) + __builtin_bit_cast(u16x2, rr));
-              accB[j][0] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accB[j][0]) +
-                                                            (__builtin_bit_cast(u16x2, rr) >> (unsigned short)8));
-            } else {
-              const uint32_t r0 = rh.x + rl.x, r1 = rh.y + rl.y;
-              accA[j][0] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accA[j][0]) + __builtin_bit_cast(u16x2, r0));
-              accB[j][0] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accB[j][0]) +
-                                                            (__builtin_bit_cast(u16x2, r0) >> (unsigned short)8));
-              accA[j][1] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accA[j][1]) + __builtin_bit_cast(u16x2, r1));
-              accB[j][1] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accB[j][1]) +
-                                                            (__builtin_bit_cast(u16x2, r1) >> (unsigned short)8));
-            }
-          }
-        }
-        if (((g16 + 1) & 15) == 0 || g16 == G16 - 1) {
-          // window flush: exact u16 sums per query -> the reference's float chain
+          for (int u = 0; u < NB; ++u)
+            cb[j][u] = (valid[j] && gb + u < c1) ? cp[j][(long)(gb + u) * 64] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+          const int g16 = gb + u;
+          if (g16 >= c1) break;
+          const W* lw = reinterpret_cast<const W*>(lut_s) + (long)(g16 - c0) * 16 * 32;
 #pragma unroll
           for (int j = 0; j < V; ++j) {
-            if constexpr (T == 1) {
-              fa[j][0] = __fmaf_rn(scl[0], (float)accA[j][0], fa[j][0]);
-              accA[j][0] = 0u;
-            } else {
+            const uint4 c = cb[j][u];
+            const uint32_t words[4] = {c.x, c.y, c.z, c.w};
 #pragma unroll
-              for (int h = 0; h < H; ++h) {
-                const u16x2 a = __builtin_bit_cast(u16x2, accA[j][h]);
-                const u16x2 b = __builtin_bit_cast(u16x2, accB[j][h]);
-                const uint32_t s0 = (unsigned short)(a.x - (unsigned short)(b.x << 8));
-                const uint32_t s1 = b.x;
-                const uint32_t s2 = (unsigned short)(a.y - (unsigned short)(b.y << 8));
-                const uint32_t s3 = b.y;
-                fa[j][4 * h + 0] = __fmaf_rn(scl[4 * h + 0], (float)s0, fa[j][4 * h + 0]);
-                fa[j][4 * h + 1] = __fmaf_rn(scl[4 * h + 1], (float)s1, fa[j][4 * h + 1]);
-                fa[j][4 * h + 2] = __fmaf_rn(scl[4 * h + 2], (float)s2, fa[j][4 * h + 2]);
-                fa[j][4 * h + 3] = __fmaf_rn(scl[4 * h + 3], (float)s3, fa[j][4 * h + 3]);
-                accA[j][h] = 0u;
-                accB[j][h] = 0u;
+            for (int e = 0; e < 16; ++e) {
+              const uint32_t w = words[e >> 2];
+              const int bp = (e & 3) * 8;
+              const W rh = lw[e * 32 + nib(w, bp + 4)];
+              const W rl = lw[e * 32 + 16 + nib(w, bp)];
+              if constexpr (T == 1) {
+                accA[j][0] += uint32_t(rh) + uint32_t(rl);
+              } else if constexpr (T == 4) {
+                const uint32_t rr = rh + rl;  // bytes <= 254: no carries
+                accA[j][0] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accA[j][0]) + __builtin_bit_cast(u16x2, rr));
+                accB[j][0] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accB[j][0]) +
+                                                              (__builtin_bit_cast(u16x2, rr) >> (unsigned short)8));
+              } else {
+                const uint32_t r0 = rh.x + rl.x, r1 = rh.y + rl.y;
+                accA[j][0] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accA[j][0]) + __builtin_bit_cast(u16x2, r0));
+                accB[j][0] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accB[j][0]) +
+                                                              (__builtin_bit_cast(u16x2, r0) >> (unsigned short)8));
+                accA[j][1] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accA[j][1]) + __builtin_bit_cast(u16x2, r1));
+                accB[j][1] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, accB[j][1]) +
+                                                              (__builtin_bit_cast(u16x2, r1) >> (unsigned short)8));
+              }
+            }
+          }
+          if (((g16 + 1) & 15) == 0 || g16 == G16 - 1) {
+            // window flush: exact u16 sums per query -> the reference's float chain
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+              if constexpr (T == 1) {
+                fa[j][0] = __fmaf_rn(scl[0], (float)accA[j][0], fa[j][0]);
+                accA[j][0] = 0u;
+              } else {
+#pragma unroll
+                for (int h = 0; h < H; ++h) {
+                  const u16x2 a = __builtin_bit_cast(u16x2, accA[j][h]);
+                  const u16x2 b = __builtin_bit_cast(u16x2, accB[j][h]);
+                  const uint32_t s0 = (unsigned short)(a.x - (unsigned short)(b.x << 8));
+                  const uint32_t s1 = b.x;
+                  const uint32_t s2 = (unsigned short)(a.y - (unsigned short)(b.y << 8));
+                  const uint32_t s3 = b.y;
+                  fa[j][4 * h + 0] = __fmaf_rn(scl[4 * h + 0], (float)s0, fa[j][4 * h + 0]);
+                  fa[j][4 * h + 1] = __fmaf_rn(scl[4 * h + 1], (float)s1, fa[j][4 * h + 1]);
+                  fa[j][4 * h + 2] = __fmaf_rn(scl[4 * h + 2], (float)s2, fa[j][4 * h + 2]);
+                  fa[j][4 * h + 3] = __fmaf_rn(scl[4 * h + 3], (float)s3, fa[j][4 * h + 3]);
+                  accA[j][h] = 0u;
+                  accB[j][h] = 0u;
+                }
               }
             }
           }
