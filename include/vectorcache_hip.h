@@ -106,6 +106,9 @@ size_t vc_index_index_bytes(const vc_index* index);
 vc_status vc_index_export_packed(vc_index* index, uint8_t* packed_host);
 vc_status vc_index_export_flat(vc_index* index, uint8_t* flat_host);
 vc_status vc_index_export_perm0(vc_index* index, uint8_t* perm0_host);
+/* seq: 192x1 sequential blocked bytes (ceil(n/192) x G x 192, repack_seq,
+ *      src/pack/pack.cpp:133-154,358-364). */
+vc_status vc_index_export_seq(vc_index* index, uint8_t* seq_host);
 vc_status vc_index_export_alpha(vc_index* index, float* alpha_host);
 vc_status vc_index_get_calibration(const vc_index* index, float* shift_host, float* scale_host);
 

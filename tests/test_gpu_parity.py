@@ -64,6 +64,7 @@ def test_index_exports_and_search_match_golden(vc, golden_cases):
         assert np.array_equal(ix.export_packed(), g["packed"]), c
         assert np.array_equal(ix.export_flat(), g["flat"]), c
         assert sha(ix.export_perm0()) == bytes(g["perm0_sha"]).hex(), c
+        assert sha(ix.export_seq()) == bytes(g["seq_sha"]).hex(), c
         assert bits_equal(ix.export_alpha(), g["alpha"]), c
         prep = ix.prepare_queries(q)
         assert np.array_equal(prep["luts"], g["prep_luts"]), c

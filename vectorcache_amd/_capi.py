@@ -57,6 +57,7 @@ _SIGS = {
     "vc_index_export_packed": ([P, P], I),
     "vc_index_export_flat": ([P, P], I),
     "vc_index_export_perm0": ([P, P], I),
+    "vc_index_export_seq": ([P, P], I),
     "vc_index_export_alpha": ([P, P], I),
     "vc_index_get_calibration": ([P, P, P], I),
     "vc_index_prepare_queries": ([P, P, S, P, P, P, P], I),

@@ -519,6 +519,25 @@ vc_status vc_index_export_perm0(vc_index* ix, uint8_t* perm0_host) {
   });
 }
 
+vc_status vc_index_export_seq(vc_index* ix, uint8_t* seq_host) {
+  return guarded([&] {
+    if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");
+    std::lock_guard lock(ix->mu);
+    require_prepared(ix);
+    const size_t n = ix->n_encoded, G = size_t(ix->t.groups);
+    if (n == 0) return;
+    const size_t sbytes = (n + 191) / 192 * G * 192;
+    Carve c;
+    const size_t of = c.take(n * G), os = c.take(sbytes);
+    ix->io.reserve(c.off, ix->stream);
+    export_flat_dev(ix, ix->io.as<uint8_t>(of));
+    hip_check(vchip::launch_flat_to_seq(ix->io.as<uint8_t>(of), n, int(G), ix->io.as<uint8_t>(os), ix->stream),
+              "flat_to_seq");
+    hip_check(hipMemcpyAsync(seq_host, ix->io.as<uint8_t>(os), sbytes, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipStreamSynchronize(ix->stream), "sync");
+  });
+}
+
 vc_status vc_index_export_packed(vc_index* ix, uint8_t* packed_host) {
   return guarded([&] {
     if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");

@@ -102,6 +102,8 @@ hipError_t launch_flat_to_tiled(const std::uint8_t* flat, std::size_t n, int gro
                                 std::uint8_t* tiled, hipStream_t s);
 hipError_t launch_flat_to_perm0(const std::uint8_t* flat, std::size_t n, int groups,
                                 std::uint8_t* perm0, hipStream_t s);
+hipError_t launch_flat_to_seq(const std::uint8_t* flat, std::size_t n, int groups, std::uint8_t* seq,
+                              hipStream_t s);
 hipError_t launch_perm0_to_flat(const std::uint8_t* perm0, std::size_t n, int groups,
                                 std::uint8_t* flat, hipStream_t s);
 hipError_t launch_flat_to_packed(const std::uint8_t* flat, std::size_t n, int dim, int bits,

@@ -45,11 +45,13 @@ __device__ void rotate_row_lds(const CodeTables& t, const float* __restrict__ xr
     y[i] = (xr[j] * inv) * t.signs[i];
   }
   __syncthreads();
-  const int B = t.block, halfB = B >> 1, npairs = D >> 1;
-  for (int len = 1; len < B; len <<= 1) {
+  const int B = t.block, npairs = D >> 1;
+  const int lhb = __ffs(B) - 2;  // log2(B / 2)
+  for (int lg = 0; (1 << lg) < B; ++lg) {
+    const int len = 1 << lg;
     for (int p = lane; p < npairs; p += 64) {
-      const int blk = p / halfB, q = p - blk * halfB;
-      const int i = blk * B + (q / len) * 2 * len + (q % len);
+      const int blk = p >> lhb, q = p & ((B >> 1) - 1);
+      const int i = blk * B + ((q >> lg) << (lg + 1)) + (q & (len - 1));
       const float u = y[i], v = y[i + len];
       y[i] = u + v;
       y[i + len] = u - v;
@@ -60,13 +62,45 @@ __device__ void rotate_row_lds(const CodeTables& t, const float* __restrict__ xr
   __syncthreads();
 }
 
+// Norm of a row staged in LDS: lanes 0..7 run the reference's chains.
+__device__ float norm8_lds(const float* __restrict__ x, int D, int lane) {
+  float c = 0.f;
+  if (lane < 8) {
+    constexpr int kAhead = 32;
+    const int steps = D / 8;
+    for (int t0 = 0; t0 < steps; t0 += kAhead) {
+      float v[kAhead];
+#pragma unroll
+      for (int u = 0; u < kAhead; ++u) v[u] = t0 + u < steps ? x[lane + 8 * (t0 + u)] : 0.f;
+#pragma unroll
+      for (int u = 0; u < kAhead; ++u)
+        if (t0 + u < steps) c = __fmaf_rn(v[u], v[u], c);
+    }
+  }
+  c = c + __shfl_xor(c, 1);
+  c = c + __shfl_xor(c, 2);
+  c = c + __shfl_xor(c, 4);
+  return __shfl(sqrt_rn(c), 0);
+}
+
 // Row norm of the reference (src/encode/encode.cpp:16-26): 8 interleaved
 // chains c[j % 8] += x_j^2 (fused), tree ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)).
 // Called with lanes grouped by 8: lane & 7 is the chain. Returns sqrtf.
 __device__ float norm8(const float* __restrict__ xr, int D, int chain, bool active) {
+  // the chain is sequential; its loads are not: fetch 32 ahead
+  constexpr int kAhead = 32;
   float c = 0.f;
-  if (active)
-    for (int j = chain; j < D; j += 8) c = __fmaf_rn(xr[j], xr[j], c);
+  const int steps = D / 8;
+  if (active) {
+    for (int t0 = 0; t0 < steps; t0 += kAhead) {
+      float v[kAhead];
+#pragma unroll
+      for (int u = 0; u < kAhead; ++u) v[u] = t0 + u < steps ? xr[chain + 8 * (t0 + u)] : 0.f;
+#pragma unroll
+      for (int u = 0; u < kAhead; ++u)
+        if (t0 + u < steps) c = __fmaf_rn(v[u], v[u], c);
+    }
+  }
   c = c + __shfl_xor(c, 1);
   c = c + __shfl_xor(c, 2);
   c = c + __shfl_xor(c, 4);
@@ -212,8 +246,8 @@ __device__ __forceinline__ unsigned quant_code(const CodeTables& t, const CellDe
   return k;
 }
 
-constexpr int kEncodeWaves = 2;
-constexpr int kEncodeRows = 16;
+constexpr int kEncodeWaves = 4;
+constexpr int kEncodeRows = 8;
 
 __host__ __device__ inline size_t enc_align(size_t v) { return (v + 15) & ~size_t(15); }
 __host__ __device__ inline size_t enc_wave_bytes(int D, int G16) {
@@ -266,6 +300,7 @@ __global__ __launch_bounds__(64 * kEncodeWaves) void encode_tile_kernel(CodeTabl
       const float nv = nrm[q];
       const float inv = nv > kMinNorm ? div_rn(1.f, nv) : 0.f;
       const float* xr = x + row * (long)D;
+#pragma unroll 8
       for (int j = lane; j < D; j += 64) {
         const uint32_t w = ps[j];
         y[w & 0x7fffffffu] = (xr[j] * inv) * ((w >> 31) ? -1.f : 1.f);
@@ -291,11 +326,12 @@ __global__ __launch_bounds__(64 * kEncodeWaves) void encode_tile_kernel(CodeTabl
         }
 #pragma unroll
         for (int lm = 1; lm < 64; lm <<= 1) {
-          const bool upper = (lane & lm) != 0;
+          // lower lane: a + b; upper lane: a - b = partner - self; fma with +-1 is exact
+          const float sg = (lane & lm) ? -1.f : 1.f;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const float p = __shfl_xor(v[r], lm);
-            v[r] = upper ? p - v[r] : v[r] + p;
+            v[r] = __fmaf_rn(sg, v[r], p);
           }
         }
         uint64_t cw = 0;  // this lane's codes of the block, packed as flat bytes
@@ -360,8 +396,8 @@ __global__ __launch_bounds__(64 * kEncodeWaves) void encode_tile_kernel(CodeTabl
       wave_lds_sync();
     }
     // write the 16 rows' codes as 16-byte pieces and their alphas
-    for (int c16 = lane >> 4; c16 < G16; c16 += 4) {
-      const int q = lane & 15;
+    for (int c16 = lane / kEncodeRows; c16 < G16; c16 += 64 / kEncodeRows) {
+      const int q = lane % kEncodeRows;
       const long v = v0 + q;
       const long row = v - out_offset;
       if (row >= 0 && row < n) {
@@ -493,16 +529,19 @@ __global__ __launch_bounds__(64) void prepare_queries_kernel(CodeTables t, const
   extern __shared__ __align__(16) unsigned char smem[];
   const int D = t.dim, G = t.groups;
   float* y = reinterpret_cast<float*>(smem);
-  float* mins = y + D;  // 2 * G
+  float* mins = y + D;       // 2 * G
+  float* xs = mins + 2 * G;  // D: the staged query row
   __shared__ float s_bias, s_bc;
   const int lane = threadIdx.x;
   const int cpb = 8 / t.bits, cpn = cpb / 2;
   for (long qi = blockIdx.x; qi < nq; qi += gridDim.x) {
     const float* xr = q + qi * (long)D;
-    const float nrm = __shfl(norm8(xr, D, lane & 7, lane < 8), 0);
-    const float inv = nrm > kMinNorm ? div_rn(1.f, nrm) : 0.f;
     __syncthreads();
-    rotate_row_lds(t, xr, inv, y, lane);
+    for (int j = lane; j < D; j += 64) xs[j] = xr[j];
+    __syncthreads();
+    const float nrm = norm8_lds(xs, D, lane);
+    const float inv = nrm > kMinNorm ? div_rn(1.f, nrm) : 0.f;
+    rotate_row_lds(t, xs, inv, y, lane);
     if (t.shift != nullptr) {
       // bias_corr = (float) -sum (double) y_d * shift_d, sequential in d
       if (lane == 0) {
@@ -1079,6 +1118,19 @@ __global__ void flat_to_perm0_kernel(const uint8_t* __restrict__ flat, long n, i
   }
 }
 
+// sequential 192x1: out[(bk * G + g) * 192 + lane] = flat[bk * 192 + lane][g], 0 past n
+__global__ void flat_to_seq_kernel(const uint8_t* __restrict__ flat, long n, int G, uint8_t* __restrict__ out) {
+  const long total = (n + 191) / 192 * (long)G * 192;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int lane = (int)(i % 192);
+    const long bg = i / 192;
+    const long bk = bg / G;
+    const int g = (int)(bg - bk * G);
+    const long v = bk * 192 + lane;
+    out[i] = v < n ? flat[v * G + g] : 0;
+  }
+}
+
 __global__ void perm0_to_flat_kernel(const uint8_t* __restrict__ blk, long n, int G,
                                      uint8_t* __restrict__ flat) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n * G; i += (long)gridDim.x * blockDim.x) {
@@ -1179,7 +1231,7 @@ hipError_t launch_prepare_queries(const CodeTables& t, const float* q, std::size
                                   std::uint8_t* luts, float* lut_scale, float* lut_bias,
                                   float* bias_corr, float* bias_total, hipStream_t s) {
   if (nq == 0) return hipSuccess;
-  const size_t lds = (size_t)t.dim * 4 + (size_t)t.groups * 8;
+  const size_t lds = (size_t)t.dim * 8 + (size_t)t.groups * 8;
   const int grid = grid_for((long)nq, 1, 16384);
   hipLaunchKernelGGL(prepare_queries_kernel, dim3(grid), dim3(64), lds, s, t, q, (long)nq, luts, lut_scale,
                      lut_bias, bias_corr, bias_total);
@@ -1315,6 +1367,14 @@ hipError_t launch_flat_to_perm0(const std::uint8_t* flat, std::size_t n, int gro
   const long total = (long)((n + 191) / 192) * groups * 96;
   hipLaunchKernelGGL(flat_to_perm0_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, flat, (long)n, groups,
                      perm0);
+  return hipGetLastError();
+}
+
+hipError_t launch_flat_to_seq(const std::uint8_t* flat, std::size_t n, int groups, std::uint8_t* seq,
+                              hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const long total = (long)((n + 191) / 192) * groups * 192;
+  hipLaunchKernelGGL(flat_to_seq_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, flat, (long)n, groups, seq);
   return hipGetLastError();
 }
 

@@ -160,6 +160,14 @@ class TurboQuantIndex:
         check(lib.vc_index_export_perm0(self._h, _ptr(out)))
         return out
 
+    def export_seq(self) -> np.ndarray:
+        """repack_seq bytes: sequential 192x1 blocks."""
+        G = self.dim() // (8 // self.bit_width())
+        nb = (self.size() + 191) // 192
+        out = np.zeros(nb * G * 192, np.uint8)
+        check(lib.vc_index_export_seq(self._h, _ptr(out)))
+        return out
+
     def export_alpha(self) -> np.ndarray:
         out = np.zeros(self.size(), np.float32)
         check(lib.vc_index_export_alpha(self._h, _ptr(out)))
