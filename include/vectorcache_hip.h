@@ -156,7 +156,8 @@ vc_status vc_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t*
 
 /* ---- tuning / introspection ------------------------------------------------
  * Query-tile size T the scan kernel uses for (dim, bits, nq) — 0 = automatic.
- * vc_set_scan_tile forces T in {1, 4, 8} for later searches (0 restores auto). */
+ * vc_set_scan_tile caps T at 1, 2, 4 or 8 for later searches (0 restores the
+ * automatic choice); a k too large for that tile's LDS budget takes a smaller one. */
 vc_status vc_set_scan_tile(size_t tile);
 size_t vc_scan_tile_for(size_t dim, size_t bits, size_t nq);
 
@@ -166,7 +167,11 @@ size_t vc_scan_tile_for(size_t dim, size_t bits, size_t nq);
  * in ms, its query tile and the number of vector ranges (work split) it used. */
 vc_status vc_set_profiling(int on);
 /* Test hook (no reference counterpart): flags bit 0 makes every later encode
- * take the exact sequential alpha path instead of the certified fast one. */
+ * take the exact sequential alpha path instead of the certified fast one;
+ * bit 1 makes searches use the nibble-table scan kernel (two 16-entry lookups
+ * per code byte) instead of the byte-pair table kernel; bits 16-31, when not
+ * zero, set the number of vector ranges each query tile is split into (at
+ * most 512) in place of the automatic split. */
 vc_status vc_set_debug(unsigned flags);
 vc_status vc_last_scan_stats(double* ms, size_t* tile, size_t* n_ranges);
 

@@ -90,18 +90,23 @@ def test_exact_alpha_path_matches_golden(vc, golden_cases):
         vc.set_debug(0)
 
 
-@pytest.mark.parametrize("tile", [1, 4, 8])
-def test_scan_tiles_agree_with_golden(vc, golden_cases, tile):
+@pytest.mark.parametrize("kernel,tile", [("pair", 1), ("pair", 2), ("pair", 4), ("pair", 8),
+                                         ("nibble", 1), ("nibble", 4), ("nibble", 8)])
+def test_scan_tiles_agree_with_golden(vc, golden_cases, kernel, tile):
+    """Every scan kernel (byte-pair tables, nibble tables) at every query tile."""
     vc.set_scan_tile(tile)
+    vc.set_debug(2 if kernel == "nibble" else 0)
     try:
         for c, g, x, q, sample in golden_cases:
             ix = _index(vc, c, x, g, sample)
-            for k in (10, 64):
+            assert vc.scan_tile_for(c["dim"], c["bits"], q.shape[0]) == tile
+            for k in (10, 64, 1000):
                 r = ix.search(q, k)
-                assert np.array_equal(r.ids, g[f"ids_k{k}"]), (c, k, tile)
-                assert scores_equal(r.scores, g[f"scores_k{k}"]), (c, k, tile)
+                assert np.array_equal(r.ids, g[f"ids_k{k}"]), (c, k, kernel, tile)
+                assert scores_equal(r.scores, g[f"scores_k{k}"]), (c, k, kernel, tile)
     finally:
         vc.set_scan_tile(0)
+        vc.set_debug(0)
 
 
 def test_idmap_scoring_matches_golden(vc, golden_cases):
@@ -172,6 +177,32 @@ def test_random_vs_oracle(vc, oracle, bits, dim, n, nq, k):
     r = ix.search(q, k)
     assert np.array_equal(r.ids, i_o)
     assert scores_equal(r.scores, s_o)
+
+
+@pytest.mark.parametrize("kernel,tile,ranges", [("nibble", 1, 8), ("nibble", 8, 2), ("pair", 1, 8),
+                                                ("pair", 2, 8), ("pair", 4, 3), ("pair", 8, 2)])
+def test_multi_step_ranges_match_oracle(vc, oracle, kernel, tile, ranges):
+    """Few vector ranges (forced), so every workgroup scans many steps and runs
+    many rounds of candidate admission and list merges, over several query
+    tiles: the 100k x 128 4-bit workload against the oracle."""
+    from vectorcache_amd import synthetic
+
+    x = synthetic.flat100k_db()
+    q = synthetic.flat100k_queries()[:16]
+    ix = vc.TurboQuantIndex(128, 4)
+    ix.add(x)
+    ix.prepare()
+    vc.set_scan_tile(tile)
+    vc.set_debug((2 if kernel == "nibble" else 0) | (ranges << 16))
+    try:
+        for k in (10, 100):
+            s_o, i_o = _oracle_search(oracle, x, q, 4, k)
+            r = ix.search(q, k)
+            assert np.array_equal(r.ids, i_o), (kernel, tile, ranges, k)
+            assert scores_equal(r.scores, s_o), (kernel, tile, ranges, k)
+    finally:
+        vc.set_scan_tile(0)
+        vc.set_debug(0)
 
 
 def test_ties_resolve_to_lower_id(vc, oracle):

@@ -121,6 +121,7 @@ struct DevRotation {
   uint32_t* perm_sign = nullptr;  // inv_perm[j] | sign-of-destination << 31
 };
 std::atomic<int> g_debug_flags{0};
+
 std::mutex g_rot_mu;
 std::map<std::pair<int, size_t>, DevRotation> g_dev_rot;
 
@@ -226,8 +227,16 @@ SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const f
     fail(VC_ERR_INVALID_ARGUMENT, "k exceeds the GPU top-k capacity of 2048");
   if (n > 0xFFFFFFFFull) fail(VC_ERR_INVALID_ARGUMENT, "index exceeds 2^32 vectors");
   const int ke = int(r.k_eff);
-  const vchip::ScanPlan plan = vchip::plan_scan(t.groups16, n, nq, ke, g_forced_tile.load());
+  vchip::ScanPlan plan = vchip::plan_scan(t.groups16, n, nq, ke, g_forced_tile.load(), (g_debug_flags.load() & 2) != 0);
   if (plan.tile == 0) fail(VC_ERR_INVALID_ARGUMENT, "no scan configuration fits LDS for this k / tile");
+  if ((g_debug_flags.load() & 8) != 0 && plan.pair) plan.vec_per_thread = 3 - plan.vec_per_thread;  // test hook
+  if (const size_t fr = std::min<size_t>(unsigned(g_debug_flags.load()) >> 16, 512); fr != 0 && n <= (size_t(1) << 30)) {
+    // test hook: flags >> 16 = vector ranges per query tile (at most 512)
+    size_t len = (n + fr - 1) / fr;
+    len = (len + 63) / 64 * 64;
+    plan.range_len = int(len);
+    plan.n_ranges = int((n + len - 1) / len);
+  }
   const size_t G = size_t(t.groups);
   Carve c;
   const size_t o_luts = c.take(nq * G * 32);
@@ -774,7 +783,8 @@ vc_status vc_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t*
 
 vc_status vc_set_scan_tile(size_t tile) {
   return guarded([&] {
-    if (tile != 0 && tile != 1 && tile != 4 && tile != 8) fail(VC_ERR_INVALID_ARGUMENT, "tile must be 0, 1, 4 or 8");
+    if (tile != 0 && tile != 1 && tile != 2 && tile != 4 && tile != 8)
+      fail(VC_ERR_INVALID_ARGUMENT, "tile must be 0, 1, 2, 4 or 8");
     g_forced_tile.store(int(tile));
   });
 }
@@ -811,7 +821,8 @@ vc_status vc_last_scan_stats(double* ms, size_t* tile, size_t* n_ranges) {
 size_t vc_scan_tile_for(size_t dim, size_t bits, size_t nq) {
   try {
     const size_t G = vchip::byte_groups(bits, dim);
-    const auto p = vchip::plan_scan(int((G + 15) / 16), 1u << 20, nq, 10, g_forced_tile.load());
+    const auto p = vchip::plan_scan(int((G + 15) / 16), 1u << 20, nq, 10, g_forced_tile.load(),
+                                    (g_debug_flags.load() & 2) != 0);
     return size_t(p.tile);
   } catch (...) {
     return 0;

@@ -58,7 +58,8 @@ hipError_t launch_prepare_queries(const CodeTables& t, const float* q, std::size
                                   float* bias_corr, float* bias_total, hipStream_t s);
 
 struct ScanPlan {
-  int tile;          // queries per workgroup (1, 4 or 8)
+  int tile;          // queries per workgroup (1, 2, 4 or 8)
+  int pair;          // 1: byte-pair table kernel, 0: nibble-table kernel
   int n_ranges;      // vector ranges (workgroups per query tile)
   int range_len;     // vectors per range (multiple of 64)
   int lut_chunk16;   // 16-group chunks of LUT resident in LDS (all groups if it fits)
@@ -69,9 +70,10 @@ struct ScanPlan {
 
 // Plan for nq queries over n vectors with top-k (k <= kMaxTopK).
 inline constexpr int kMaxTopK = 2048;
-ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int forced_tile);
+// nibble = true selects the nibble-table kernel (two lookups per code byte).
+ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int forced_tile, bool nibble);
 
-// Bytes of the query-interleaved LUT scratch the scan needs.
+// Bytes of the per-query-tile LUT rows (interleaved / pair source) the scan needs.
 std::size_t scan_lut_bytes(const ScanPlan& p, int groups16, std::size_t nq);
 
 // Scan + per-range exact top-k. luts: nq x groups*32 (prepare_queries);

@@ -6,6 +6,7 @@
 // bit-identical to the reference's AVX2 PERM0 path.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -651,14 +652,26 @@ struct TopkShared {
 // Merge query t's candidates into its sorted list (all threads participate).
 __device__ void topk_merge(const TopkShared& sh, int t, int k, int cap) {
   const int tid = threadIdx.x;
-  const int c = min(sh.cnt[t], cap);
-  const int m = sh.m[t];
+  const int c = __builtin_amdgcn_readfirstlane(min(sh.cnt[t], cap));
+  const int m = __builtin_amdgcn_readfirstlane(sh.m[t]);
   float* ts = sh.top_s + t * k;
   uint32_t* ti = sh.top_i + t * k;
   float* ns = sh.nw_s + t * k;
   uint32_t* ni = sh.nw_i + t * k;
   const float* cs = sh.cand_s + t * cap;
   const uint32_t* ci = sh.cand_i + t * cap;
+  // rank = start + number of candidates better than (s, id); four candidates
+  // per step (16-byte LDS reads; the buffer is 16-byte aligned and cap is a
+  // multiple of 32, so reads past c stay inside it and are masked)
+  auto rank_of = [&](float s, uint32_t id, int rank) {
+    for (int f = 0; f < c && rank < k; f += 4) {
+      const float4 s4 = *reinterpret_cast<const float4*>(cs + f);
+      const uint4 i4 = *reinterpret_cast<const uint4*>(ci + f);
+      rank += (better(s4.x, i4.x, s, id) ? 1 : 0) + (f + 1 < c && better(s4.y, i4.y, s, id) ? 1 : 0) +
+              (f + 2 < c && better(s4.z, i4.z, s, id) ? 1 : 0) + (f + 3 < c && better(s4.w, i4.w, s, id) ? 1 : 0);
+    }
+    return rank;
+  };
   for (int e = tid; e < c; e += blockDim.x) {
     const float s = cs[e];
     const uint32_t id = ci[e];
@@ -667,15 +680,13 @@ __device__ void topk_merge(const TopkShared& sh, int t, int k, int cap) {
       const int mid = (lo + hi) >> 1;
       if (better(ts[mid], ti[mid], s, id)) lo = mid + 1; else hi = mid;
     }
-    int rank = lo;
-    for (int f = 0; f < c && rank < k; ++f) rank += better(cs[f], ci[f], s, id) ? 1 : 0;
+    const int rank = rank_of(s, id, lo);
     if (rank < k) { ns[rank] = s; ni[rank] = id; }
   }
   for (int j = tid; j < m; j += blockDim.x) {
     const float s = ts[j];
     const uint32_t id = ti[j];
-    int rank = j;
-    for (int f = 0; f < c && rank < k; ++f) rank += better(cs[f], ci[f], s, id) ? 1 : 0;
+    const int rank = rank_of(s, id, j);
     if (rank < k) { ns[rank] = s; ni[rank] = id; }
   }
   __syncthreads();
@@ -688,6 +699,156 @@ __device__ void topk_merge(const TopkShared& sh, int t, int k, int cap) {
     if (m2 == k) { sh.thr_s[t] = ts[k - 1]; sh.thr_i[t] = ti[k - 1]; }
   }
   __syncthreads();
+}
+
+// Carve the top-k state out of LDS at `base` and reset it (threads < T write;
+// the caller synchronizes before first use).
+__device__ __forceinline__ void init_topk(TopkShared& sh, unsigned char* base, int T, int k, int cap) {
+  size_t off = 0;
+  sh.top_s = reinterpret_cast<float*>(base + off); off += (size_t)T * k * 4;
+  sh.top_i = reinterpret_cast<uint32_t*>(base + off); off += (size_t)T * k * 4;
+  sh.nw_s = reinterpret_cast<float*>(base + off); off += (size_t)T * k * 4;
+  sh.nw_i = reinterpret_cast<uint32_t*>(base + off); off += (size_t)T * k * 4;
+  sh.cand_s = reinterpret_cast<float*>(base + off); off += (size_t)T * cap * 4;
+  sh.cand_i = reinterpret_cast<uint32_t*>(base + off); off += (size_t)T * cap * 4;
+  sh.cnt = reinterpret_cast<int*>(base + off); off += T * 4;
+  sh.m = reinterpret_cast<int*>(base + off); off += T * 4;
+  sh.thr_s = reinterpret_cast<float*>(base + off); off += T * 4;
+  sh.thr_i = reinterpret_cast<uint32_t*>(base + off);
+  const int tid = threadIdx.x;
+  if (tid < T) {
+    sh.cnt[tid] = 0;
+    sh.m[tid] = 0;
+    sh.thr_s[tid] = -INFINITY;
+    sh.thr_i[tid] = 0u;
+  }
+}
+
+// Bitonic sort of one key per lane across the wavefront into descending
+// (score, then ascending id) order: lane i ends with the i-th best key.
+__device__ __forceinline__ void wave_sort_desc(float& s, uint32_t& id, int lane) {
+#pragma unroll
+  for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const float os = __shfl_xor(s, stride);
+      const uint32_t oi = (uint32_t)__shfl_xor((int)id, stride);
+      const bool desc = (lane & size) == 0 || size == 64;
+      const bool lower = (lane & stride) == 0;
+      const bool o_better = better(os, oi, s, id);
+      if ((lower == desc) ? o_better : !o_better) { s = os; id = oi; }
+    }
+  }
+}
+
+// Scores of V vectors x T queries (fa * alpha, -0.f reported as +0.f, as the
+// reference's flush_batch_to_fa epilogue) and their admission into the
+// workgroup's top-k lists: strict "better than the current k-th" test,
+// candidate buffer, merge when half full, retry on overflow. Every thread of
+// the workgroup calls it (it synchronizes).
+template <int T, int V>
+__device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], const long (&vv)[V],
+                             const bool (&valid)[V], const float* __restrict__ alpha, long q0, long nq,
+                             int k, int cap) {
+  const int lane = threadIdx.x & 63;
+  const int kw = min(k, 64);
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    float s[T];
+    uint32_t sid[T];
+    const float av = valid[j] ? alpha[vv[j]] : 0.f;
+    const uint32_t vid = (uint32_t)vv[j];
+    uint32_t pend = 0u;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      s[t] = valid[j] ? fa[j][t] * av + 0.f : -INFINITY;
+      sid[t] = vid;
+      const bool cand = valid[j] && q0 + t < nq && better(s[t], vid, sh.thr_s[t], sh.thr_i[t]);
+      // (tiles of one or two queries only: the bandwidth regime, where a
+      // workgroup scans few steps and the first one admits every vector; the
+      // sort would cost larger tiles their register budget)
+      const int nc = T <= 2 ? __popcll(__ballot(cand)) : 0;
+      if (T <= 2 && nc > kw) {
+        // more candidates in this wavefront than can reach the list: keep the
+        // wavefront's best kw (sorted into lanes 0 .. kw-1)
+        if (!cand) { s[t] = -INFINITY; sid[t] = 0xffffffffu; }
+        wave_sort_desc(s[t], sid[t], lane);
+        if (lane < kw) pend |= 1u << t;
+      } else if (cand) {
+        pend |= 1u << t;
+      }
+    }
+    while (true) {
+      // every lane takes part in the atomic (adding 0 when it has nothing to
+      // store): the loop then has no lane-dependent control flow around its
+      // barriers, only the uniform exit below
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const bool want = ((pend >> t) & 1u) != 0u;
+        // one LDS atomic per wavefront: base + rank among the wanting lanes
+        const uint64_t wm = __ballot(want);
+        int base = 0;
+        if ((threadIdx.x & 63) == 0) base = atomicAdd(&sh.cnt[t], __popcll(wm));
+        base = __shfl(base, 0);
+        const int slot = base + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(wm >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)wm, 0u));
+        const bool store = want && slot < cap;
+        if (store) {
+          sh.cand_s[t * cap + slot] = s[t];
+          sh.cand_i[t * cap + slot] = sid[t];
+        }
+        pend &= ~((store ? 1u : 0u) << t);
+      }
+      __syncthreads();
+      // Every thread reads the same counts; taking them through readfirstlane
+      // makes the loop exit and the merges (which synchronize) wave-uniform
+      // branches for the compiler as well.
+      int overflow = 0, need = 0;
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int c = sh.cnt[t];
+        overflow |= c > cap ? 1 : 0;
+        need |= (c > cap / 2 ? 1 : 0) << t;
+      }
+      overflow = __builtin_amdgcn_readfirstlane(overflow);
+      need = __builtin_amdgcn_readfirstlane(need);
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < T; ++t)
+        if ((need >> t) & 1) topk_merge(sh, t, k, cap);
+      if (!overflow) break;
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const bool keep = better(s[t], sid[t], sh.thr_s[t], sh.thr_i[t]);
+        pend &= ~((keep ? 0u : 1u) << t);
+      }
+    }
+  }
+}
+
+// Final merge of the remaining candidates and output of this workgroup's
+// sorted lists: part_s/part_i [q][n_ranges][k], part_cnt [q][n_ranges].
+template <int T>
+__device__ void write_partials(const TopkShared& sh, long q0, long nq, int n_ranges, int r, int k, int cap,
+                               float* __restrict__ part_s, uint32_t* __restrict__ part_i,
+                               int* __restrict__ part_cnt) {
+  const int tid = threadIdx.x;
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < T; ++t)
+    if (__builtin_amdgcn_readfirstlane(sh.cnt[t]) > 0) topk_merge(sh, t, k, cap);
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    const long q = q0 + t;
+    if (q >= nq) continue;
+    const long slot = q * n_ranges + r;
+    const int m = sh.m[t];
+    for (int j = tid; j < m; j += blockDim.x) {
+      part_s[slot * k + j] = sh.top_s[t * k + j];
+      part_i[slot * k + j] = sh.top_i[t * k + j];
+    }
+    if (tid == 0) part_cnt[slot] = m;
+  }
 }
 
 template <int T>
@@ -734,25 +895,8 @@ __global__ __launch_bounds__(kScanThreads) void scan_topk_kernel(
   // ---- LDS carve-up ----
   const size_t lut_bytes = (size_t)chunk16 * 16 * 32 * T;
   uint8_t* lut_s = smem;
-  size_t off = (lut_bytes + 15) & ~size_t(15);
   TopkShared sh;
-  sh.top_s = reinterpret_cast<float*>(smem + off); off += (size_t)T * k * 4;
-  sh.top_i = reinterpret_cast<uint32_t*>(smem + off); off += (size_t)T * k * 4;
-  sh.nw_s = reinterpret_cast<float*>(smem + off); off += (size_t)T * k * 4;
-  sh.nw_i = reinterpret_cast<uint32_t*>(smem + off); off += (size_t)T * k * 4;
-  sh.cand_s = reinterpret_cast<float*>(smem + off); off += (size_t)T * cap * 4;
-  sh.cand_i = reinterpret_cast<uint32_t*>(smem + off); off += (size_t)T * cap * 4;
-  sh.cnt = reinterpret_cast<int*>(smem + off); off += T * 4;
-  sh.m = reinterpret_cast<int*>(smem + off); off += T * 4;
-  sh.thr_s = reinterpret_cast<float*>(smem + off); off += T * 4;
-  sh.thr_i = reinterpret_cast<uint32_t*>(smem + off); off += T * 4;
-
-  if (tid < T) {
-    sh.cnt[tid] = 0;
-    sh.m[tid] = 0;
-    sh.thr_s[tid] = -INFINITY;
-    sh.thr_i[tid] = 0u;
-  }
+  init_topk(sh, smem + ((lut_bytes + 15) & ~size_t(15)), T, k, cap);
   float scl[T], bias[T];
 #pragma unroll
   for (int t = 0; t < T; ++t) {
@@ -811,7 +955,11 @@ __global__ __launch_bounds__(kScanThreads) void scan_topk_kernel(
         for (int j = 0; j < V; ++j)
 #pragma unroll
           for (int u = 0; u < NB; ++u)
-            cb[j][u] = (valid[j] && gb + u < c1) ? cp[j][(long)(gb + u) * 64] : make_uint4(0u, 0u, 0u, 0u);
+            // unconditional, clamped (in-bounds) load: no branch and no select
+            // around it, so it stays in flight until its block is consumed.
+            // Blocks past c1 are never consumed; lanes of invalid vectors
+            // (pointing at vector 0) get -inf scores.
+            cb[j][u] = cp[j][(long)min(gb + u, G16 - 1) * 64];
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
           const int g16 = gb + u;
@@ -875,67 +1023,264 @@ __global__ __launch_bounds__(kScanThreads) void scan_topk_kernel(
       }
     }
 
-    // scores (-0.f reported as +0.f) and admission with overflow retry
+    admit_scores<T, V>(sh, fa, vv, valid, alpha, q0, nq, k, cap);
+  }
+  write_partials<T>(sh, q0, nq, n_ranges, r, k, cap, part_s, part_i, part_cnt);
+}
+
+// ---------------------------------------------------------------------------
+// Byte-pair scan. A code byte holds two nibbles of one group (hi = first
+// code). Instead of two 16-entry lookups per byte, each group's two nibble
+// tables are expanded into one 256-entry table over the whole byte:
+//   pair[g][b] = lut[g][b >> 4] + lut[g][16 + (b & 15)]   (<= 254)
+// so one LDS read serves both nibbles of T queries. The window sums (256
+// groups, the reference's kFlushEvery) are the same exact integers, so the
+// float chain fa = fmaf(scale, (float)A_w, fa) and the scores are unchanged.
+//
+// Entries hold T queries: u8 per query for T = 1 and T = 8 (T = 8 splits the
+// bytes into u16 lanes after the read), u16 per query for T = 2 and 4. The
+// table lives in a 64 KiB LDS chunk of CG groups, rebuilt per chunk from an
+// 8 KiB slice of the query tile's source rows ([group][32][T] of u8 or u16),
+// which is staged through LDS one chunk ahead.
+// ---------------------------------------------------------------------------
+constexpr int kPairTableBytes = 65536;
+constexpr int kPairStageBytes = 8192;
+
+template <int T> struct PairFmt { static constexpr bool wide = (T == 2 || T == 4); };
+template <int T> constexpr int pair_entry_bytes() { return T * (PairFmt<T>::wide ? 2 : 1); }
+
+// Source rows of every query tile: element (tile, g, e, t) = lut[q][g][e] of
+// query q = tile * T + t (zero for padded groups / queries), u16 when wide.
+template <int T>
+__global__ void pair_source_kernel(const uint8_t* __restrict__ luts, long nq, int G, int G16,
+                                   uint8_t* __restrict__ out) {
+  const long n_qt = (nq + T - 1) / T;
+  const long per_tile = (long)G16 * 16 * 32 * T;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n_qt * per_tile; i += (long)gridDim.x * blockDim.x) {
+    const long qt = i / per_tile;
+    const long r = i - qt * per_tile;
+    const int t = (int)(r % T);
+    const int e = (int)((r / T) & 31);
+    const int g = (int)(r / (T * 32));
+    const long q = qt * T + t;
+    const uint8_t v = (g < G && q < nq) ? luts[q * (long)G * 32 + (long)g * 32 + e] : 0;
+    if constexpr (PairFmt<T>::wide) reinterpret_cast<uint16_t*>(out)[i] = v;
+    else out[i] = v;
+  }
+}
+
+// Table chunk from the staged source rows of its CG groups. Each thread writes
+// 64 contiguous bytes of table: entries (g, h, l0 .. l0 + 64 / EB - 1), or for
+// T = 1 four whole rows (g, h0 .. h0 + 3).
+template <int T>
+__device__ __forceinline__ void build_pair_chunk(const uint8_t* __restrict__ stage, uint8_t* __restrict__ table) {
+  constexpr int EB = pair_entry_bytes<T>();
+  const int tid = threadIdx.x;
+  if constexpr (EB == 1) {
+    const int g = tid >> 2, h0 = (tid & 3) * 4;
+    const uint32_t his = *reinterpret_cast<const uint32_t*>(stage + g * 32 + h0);
+    const uint4 lo = *reinterpret_cast<const uint4*>(stage + g * 32 + 16);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint32_t bc = ((his >> (8 * r)) & 0xffu) * 0x01010101u;  // bytes <= 127: no carries
+      *reinterpret_cast<uint4*>(table + g * 256 + (h0 + r) * 16) =
+          make_uint4(lo.x + bc, lo.y + bc, lo.z + bc, lo.w + bc);
+    }
+  } else {
+    constexpr int ENT = 64 / EB;          // entries per thread
+    constexpr int PER_ROW = 16 / ENT;     // threads per (g, h) row
+    constexpr int WPE = EB / 4;           // 32-bit words per entry
+    const int row = tid / PER_ROW;        // g * 16 + h
+    const int g = row >> 4, h = row & 15;
+    const int l0 = (tid % PER_ROW) * ENT;
+    uint32_t hw[WPE];
+#pragma unroll
+    for (int w = 0; w < WPE; ++w) hw[w] = reinterpret_cast<const uint32_t*>(stage + (g * 32 + h) * EB)[w];
+    const uint4* ls = reinterpret_cast<const uint4*>(stage + (g * 32 + 16 + l0) * EB);
+    uint4* dst = reinterpret_cast<uint4*>(table + (g * 256 + h * 16 + l0) * EB);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const uint4 l = ls[v];
+      uint32_t o[4] = {l.x, l.y, l.z, l.w};
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const uint32_t hv = hw[(v * 4 + w) % WPE];
+        if constexpr (PairFmt<T>::wide)
+          o[w] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, o[w]) + __builtin_bit_cast(u16x2, hv));
+        else
+          o[w] += hv;  // bytes <= 127: no carries
+      }
+      dst[v] = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+  }
+}
+
+template <int T, int V>
+__global__ __launch_bounds__(kScanThreads, 8) void scan_pair_kernel(
+    const uint8_t* __restrict__ tiled, const float* __restrict__ alpha, long n, int G16,
+    const uint8_t* __restrict__ src, const float* __restrict__ lut_scale,
+    const float* __restrict__ bias_total, long nq, int k, int cap, int n_ranges, long range_len,
+    float* __restrict__ part_s, uint32_t* __restrict__ part_i, int* __restrict__ part_cnt) {
+  constexpr bool kWide = PairFmt<T>::wide;
+  constexpr int EB = pair_entry_bytes<T>();
+  constexpr int CG = kPairTableBytes / (256 * EB);  // groups per table chunk
+  constexpr int CB = CG / 16;                       // 16-group code blocks per chunk
+  static_assert(CG * 32 * EB == kPairStageBytes && kPairStageBytes == kScanThreads * 8, "stage = one u64 per thread");
+  constexpr int NR = (T == 1 && V == 1) ? 4 : 2;   // code blocks in flight per vector
+  static_assert(CB % NR == 0, "a chunk holds whole rounds of the code-block ring");
+  constexpr int NA = T == 1 ? 1 : (kWide ? T / 2 : T / 4);  // 32-bit accumulators per vector (x2 when narrow T=8)
+
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint8_t* table = smem;
+  uint8_t* stage = smem + kPairTableBytes;
+  const int tid = threadIdx.x;
+  const long n_qt = (nq + T - 1) / T;
+  const long L = blockIdx.x;
+  const int r = (int)(L / n_qt);   // range-major: concurrent workgroups share codes
+  const long qt = L % n_qt;
+  const long q0 = qt * T;
+  const long r_begin = (long)r * range_len;
+  const long r_end = min(n, r_begin + range_len);
+
+  TopkShared sh;
+  init_topk(sh, smem + kPairTableBytes + kPairStageBytes, T, k, cap);
+  float scl[T], bias[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    const long q = q0 + t < nq ? q0 + t : nq - 1;
+    scl[t] = lut_scale[q];
+    bias[t] = bias_total[q];
+  }
+
+  const int n_chunks = (G16 + CB - 1) / CB;
+  const uint64_t* tile_src = reinterpret_cast<const uint64_t*>(src + qt * ((long)G16 * 16 * 32 * EB));
+  const long src_words = (long)G16 * 16 * 32 * EB / 8;
+  auto src_word = [&](int c) -> uint64_t {
+    const long i = (long)c * kScanThreads + tid;
+    return i < src_words ? tile_src[i] : 0ull;
+  };
+  reinterpret_cast<uint64_t*>(stage)[tid] = src_word(0);
+  uint64_t sw = src_word(n_chunks > 1 ? 1 : 0);
+  bool built = false;
+
+  const long step = (long)kScanThreads * V;
+  for (long base = r_begin; base < r_end; base += step) {
+    long vv[V];
+    bool valid[V];
+    const uint4* cp[V];
+    float fa[V][T];
+    uint32_t acc[V][NA], acc2[V][NA];
 #pragma unroll
     for (int j = 0; j < V; ++j) {
-      float s[T];
-      const float av = valid[j] ? alpha[vv[j]] : 0.f;
+      vv[j] = base + tid + (long)j * kScanThreads;
+      valid[j] = vv[j] < r_end;
+      const long vs = valid[j] ? vv[j] : 0;
+      cp[j] = reinterpret_cast<const uint4*>(tiled) + (vs >> 6) * (long)G16 * 64 + (vs & 63);
 #pragma unroll
-      for (int t = 0; t < T; ++t) s[t] = valid[j] ? fa[j][t] * av + 0.f : -INFINITY;
-      const uint32_t vid = (uint32_t)vv[j];
-      uint32_t pend = 0u;
+      for (int t = 0; t < T; ++t) fa[j][t] = bias[t];
 #pragma unroll
-      for (int t = 0; t < T; ++t)
-        if (valid[j] && q0 + t < nq && better(s[t], vid, sh.thr_s[t], sh.thr_i[t])) pend |= 1u << t;
-      while (true) {
+      for (int a = 0; a < NA; ++a) { acc[j][a] = 0u; acc2[j][a] = 0u; }
+    }
+    auto load_block = [&](int j, int b) -> uint4 {
+      // unconditional, clamped (in-bounds) load: no branch and no select around
+      // it, so it stays in flight until its block is consumed. Blocks past G16
+      // are never consumed; lanes of invalid vectors (pointing at vector 0)
+      // get -inf scores.
+      return cp[j][(long)min(b, G16 - 1) * 64];
+    };
+    // Code blocks NR ahead. The block loop of a chunk is unrolled and CB is a
+    // multiple of NR, so each block always uses the same ring slot: no
+    // register rotation, and each wait covers exactly the block consumed.
+    uint4 ring[NR][V];
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
-          if (pend & (1u << t)) {
-            const int slot = atomicAdd(&sh.cnt[t], 1);
-            if (slot < cap) {
-              sh.cand_s[t * cap + slot] = s[t];
-              sh.cand_i[t * cap + slot] = vid;
-              pend &= ~(1u << t);
+    for (int u = 0; u < NR; ++u)
+#pragma unroll
+      for (int j = 0; j < V; ++j) ring[u][j] = load_block(j, u);
+
+    for (int c = 0; c < n_chunks; ++c) {
+      if (n_chunks > 1 || !built) {
+        __syncthreads();  // table and stage free (previous chunk's lookups done, stage written)
+        build_pair_chunk<T>(stage, table);
+        __syncthreads();  // table ready; stage consumed
+        if (n_chunks > 1) {
+          reinterpret_cast<uint64_t*>(stage)[tid] = sw;  // rows of chunk c + 1
+          sw = src_word(c + 2 < n_chunks ? c + 2 : (c + 2) - n_chunks);
+        }
+        built = true;
+      }
+#pragma unroll
+      for (int u = 0; u < CB; ++u) {
+        const int bb = c * CB + u;
+        uint4 cw[V];
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+          cw[j] = ring[u % NR][j];
+          ring[u % NR][j] = load_block(j, bb + NR);  // issued even past G16 (clamped, unused)
+        }
+        if (bb < G16) {
+          const uint8_t* tb = table + u * 16 * 256 * EB;
+#pragma unroll
+          for (int j = 0; j < V; ++j) {
+            const uint32_t words[4] = {cw[j].x, cw[j].y, cw[j].z, cw[j].w};
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+              const uint32_t byte = __builtin_amdgcn_ubfe(words[e >> 2], 8 * (e & 3), 8);
+              const uint8_t* p = tb + e * 256 * EB + byte * EB;
+              if constexpr (T == 1) {
+                acc[j][0] += *p;
+              } else if constexpr (kWide) {
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+                  acc[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc[j][a]) +
+                                                               reinterpret_cast<const u16x2*>(p)[a]);
+              } else {
+                const uint2 w2 = *reinterpret_cast<const uint2*>(p);
+                const uint32_t ws[2] = {w2.x, w2.y};
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                  // bytes (0, 2) and (1, 3) of the entry word as u16 lanes (v_perm selectors; 0x0c = zero)
+                  const uint32_t ev = __builtin_amdgcn_perm(0u, ws[a], 0x0c020c00u);
+                  const uint32_t od = __builtin_amdgcn_perm(0u, ws[a], 0x0c030c01u);
+                  acc[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc[j][a]) + __builtin_bit_cast(u16x2, ev));
+                  acc2[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc2[j][a]) + __builtin_bit_cast(u16x2, od));
+                }
+              }
+            }
+          }
+          if (((bb + 1) & 15) == 0 || bb == G16 - 1) {
+            // window flush: exact integer sums per query -> the reference's float chain
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+              if constexpr (T == 1) {
+                fa[j][0] = __fmaf_rn(scl[0], (float)acc[j][0], fa[j][0]);
+              } else if constexpr (kWide) {
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                  const u16x2 sm = __builtin_bit_cast(u16x2, acc[j][a]);
+                  fa[j][2 * a] = __fmaf_rn(scl[2 * a], (float)sm.x, fa[j][2 * a]);
+                  fa[j][2 * a + 1] = __fmaf_rn(scl[2 * a + 1], (float)sm.y, fa[j][2 * a + 1]);
+                }
+              } else {
+#pragma unroll
+                for (int a = 0; a < NA; ++a) {
+                  const u16x2 ev = __builtin_bit_cast(u16x2, acc[j][a]);   // queries 4a, 4a + 2
+                  const u16x2 od = __builtin_bit_cast(u16x2, acc2[j][a]);  // queries 4a + 1, 4a + 3
+                  fa[j][4 * a + 0] = __fmaf_rn(scl[4 * a + 0], (float)ev.x, fa[j][4 * a + 0]);
+                  fa[j][4 * a + 1] = __fmaf_rn(scl[4 * a + 1], (float)od.x, fa[j][4 * a + 1]);
+                  fa[j][4 * a + 2] = __fmaf_rn(scl[4 * a + 2], (float)ev.y, fa[j][4 * a + 2]);
+                  fa[j][4 * a + 3] = __fmaf_rn(scl[4 * a + 3], (float)od.y, fa[j][4 * a + 3]);
+                }
+              }
+#pragma unroll
+              for (int a = 0; a < NA; ++a) { acc[j][a] = 0u; acc2[j][a] = 0u; }
             }
           }
         }
-        __syncthreads();
-        bool overflow = false;
-        bool need[T];
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-          const int c = sh.cnt[t];
-          overflow |= c > cap;
-          need[t] = c > cap / 2;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int t = 0; t < T; ++t)
-          if (need[t]) topk_merge(sh, t, k, cap);
-        if (!overflow) break;
-#pragma unroll
-        for (int t = 0; t < T; ++t)
-          if ((pend & (1u << t)) && !better(s[t], vid, sh.thr_s[t], sh.thr_i[t])) pend &= ~(1u << t);
       }
     }
+    admit_scores<T, V>(sh, fa, vv, valid, alpha, q0, nq, k, cap);
   }
-  // final merge and output
-  __syncthreads();
-#pragma unroll
-  for (int t = 0; t < T; ++t)
-    if (sh.cnt[t] > 0) topk_merge(sh, t, k, cap);
-#pragma unroll
-  for (int t = 0; t < T; ++t) {
-    const long q = q0 + t;
-    if (q >= nq) continue;
-    const long slot = q * n_ranges + r;
-    const int m = sh.m[t];
-    for (int j = tid; j < m; j += blockDim.x) {
-      part_s[slot * k + j] = sh.top_s[t * k + j];
-      part_i[slot * k + j] = sh.top_i[t * k + j];
-    }
-    if (tid == 0) part_cnt[slot] = m;
-  }
+  write_partials<T>(sh, q0, nq, n_ranges, r, k, cap, part_s, part_i, part_cnt);
 }
 
 // Merge the per-range sorted lists of one query (one wavefront per query).
@@ -1240,13 +1585,73 @@ hipError_t launch_prepare_queries(const CodeTables& t, const float* q, std::size
 
 static size_t topk_lds(int T, int k, int cap) { return (size_t)T * (4 * (size_t)k + 2 * (size_t)cap) * 4 + 16 * T + 64; }
 
-ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int forced_tile) {
+// Vector ranges per query tile: enough workgroups to fill every CU slot when
+// the query tiles alone do not; otherwise the split (at most 8) that leaves
+// the smallest tail in the last round of workgroups.
+static void plan_ranges(ScanPlan& p, std::size_t n, std::size_t nq) {
+  const long step = (long)kScanThreads * p.vec_per_thread;
+  const long n_qt = (long)((nq + p.tile - 1) / p.tile);
+  const long slots = 256L * (p.lds_bytes <= (std::size_t)kLdsBudget / 2 ? 2 : 1);
+  const long max_by_n = std::max(1L, (long)((n + step - 1) / step));
+  long R;
+  if (n_qt < slots) {
+    R = (slots + n_qt - 1) / n_qt;
+  } else {
+    R = 1;
+    double best = 1e30;
+    for (long c = 1; c <= 8; ++c) {
+      const double cost = (double)((n_qt * c + slots - 1) / slots) / (double)c;
+      if (cost < best * 0.999) { best = cost; R = c; }
+    }
+  }
+  R = std::min(R, std::min(max_by_n, 512L));
+  if (R < 1) R = 1;
+  long len = (long)((n + R - 1) / R);
+  len = (len + 63) / 64 * 64;
+  R = std::max(1L, (long)((n + len - 1) / len));
+  p.n_ranges = (int)R;
+  p.range_len = (int)len;
+}
+
+ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int forced_tile, bool nibble) {
   ScanPlan p{};
+  if (!nibble) {
+    // Byte-pair kernel. Relative cost of one pass over the codes (LDS gather
+    // rate vs HBM): T = 1 : 2 : 4 : 8 ~ 1 : 1.4 : 1.4 : 2.
+    const int want = nq >= 5 ? 8 : nq >= 3 ? 4 : nq == 2 ? 2 : 1;
+    const std::size_t fixed = (std::size_t)kPairTableBytes + kPairStageBytes;
+    for (int T : {8, 4, 2, 1}) {
+      if (T > (forced_tile != 0 ? forced_tile : want)) continue;  // a forced tile is an upper bound
+      // largest candidate buffer that still lets two workgroups share a CU,
+      // else 512 with one workgroup per CU
+      int cap = kCandCap;
+      std::size_t lds = fixed + topk_lds(T, k, cap);
+      if (lds > (std::size_t)kLdsBudget / 2) {
+        const long room = (long)kLdsBudget / 2 - (long)fixed - (long)topk_lds(T, k, 0);
+        const long c2 = room / (8L * T) / 32 * 32;
+        if (c2 >= 64) {
+          cap = (int)std::min<long>(c2, kCandCap);
+          lds = fixed + topk_lds(T, k, cap);
+        }
+      }
+      if (lds > (std::size_t)kLdsBudget) continue;
+      p.tile = T;
+      p.pair = 1;
+      p.cand_cap = cap;
+      p.lds_bytes = lds;
+      p.lut_chunk16 = 0;
+      p.vec_per_thread = T == 2 ? 2 : 1;  // register budget of 8 waves / SIMD
+      break;
+    }
+    if (p.tile == 0) return p;  // caller reports the error
+    plan_ranges(p, n, nq);
+    return p;
+  }
   p.cand_cap = kCandCap;
   const int tiles[3] = {8, 4, 1};
   for (int ti = 0; ti < 3; ++ti) {
     const int T = tiles[ti];
-    if (forced_tile != 0 && T != forced_tile) continue;
+    if (forced_tile != 0 && T > forced_tile) continue;
     if (forced_tile == 0) {
       if (T == 8 && nq < 8) continue;
       if (T == 4 && nq < 4) continue;
@@ -1271,27 +1676,15 @@ ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int force
     break;
   }
   if (p.tile == 0) return p;  // caller reports the error
-  const bool multi = p.lut_chunk16 < groups16;
-  p.vec_per_thread = multi ? 2 : 1;  // amortize LUT reloads over more vectors
-  const long step = (long)kScanThreads * p.vec_per_thread;
-  const long n_qt = (long)((nq + p.tile - 1) / p.tile);
-  long R = (512 + n_qt - 1) / n_qt;
-  const long max_by_n = (long)((n + step - 1) / step);
-  if (R > max_by_n) R = max_by_n;
-  if (R > 512) R = 512;
-  if (R < 1) R = 1;
-  long len = (long)((n + R - 1) / R);
-  len = (len + 63) / 64 * 64;
-  R = (long)((n + len - 1) / len);
-  if (R < 1) R = 1;
-  p.n_ranges = (int)R;
-  p.range_len = (int)len;
+  p.vec_per_thread = p.lut_chunk16 < groups16 ? 2 : 1;  // amortize LUT reloads over more vectors
+  plan_ranges(p, n, nq);
   return p;
 }
 
 std::size_t scan_lut_bytes(const ScanPlan& p, int groups16, std::size_t nq) {
   const std::size_t n_qt = (nq + p.tile - 1) / p.tile;
-  return n_qt * (std::size_t)groups16 * 16 * 32 * p.tile;
+  const std::size_t per_elem = p.pair && (p.tile == 2 || p.tile == 4) ? 2 : 1;
+  return n_qt * (std::size_t)groups16 * 16 * 32 * p.tile * per_elem;
 }
 
 hipError_t launch_scan(const ScanPlan& p, const std::uint8_t* tiled, const float* alpha, std::size_t n,
@@ -1299,10 +1692,32 @@ hipError_t launch_scan(const ScanPlan& p, const std::uint8_t* tiled, const float
                        const float* lut_scale, const float* bias_total, std::size_t nq, int k, float* part_s,
                        std::uint32_t* part_i, int* part_cnt, hipStream_t s) {
   const long n_qt = (long)((nq + p.tile - 1) / p.tile);
-  hipLaunchKernelGGL(interleave_luts_kernel, dim3(grid_for((long)scan_lut_bytes(p, groups16, nq), 256)), dim3(256),
-                     0, s, luts, (long)nq, groups, groups16, p.tile, lut_t);
   const dim3 grid((unsigned)(n_qt * p.n_ranges));
   const dim3 block(kScanThreads);
+  const long src_elems = n_qt * (long)groups16 * 16 * 32 * p.tile;
+  if (p.pair) {
+#define VC_LAUNCH_PAIR(TT, VV)                                                                            \
+  do {                                                                                                    \
+    hipLaunchKernelGGL(pair_source_kernel<TT>, dim3(grid_for(src_elems, 256)), dim3(256), 0, s, luts,     \
+                       (long)nq, groups, groups16, lut_t);                                                \
+    (void)hipFuncSetAttribute((const void*)scan_pair_kernel<TT, VV>,                                       \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_bytes);              \
+    hipLaunchKernelGGL((scan_pair_kernel<TT, VV>), grid, block, p.lds_bytes, s, tiled, alpha, (long)n,      \
+                       groups16, lut_t, lut_scale, bias_total, (long)nq, k, p.cand_cap, p.n_ranges,       \
+                       (long)p.range_len, part_s, part_i, part_cnt);                                      \
+  } while (0)
+    const bool v2 = p.vec_per_thread == 2;
+    switch (p.tile) {
+      case 8: if (v2) VC_LAUNCH_PAIR(8, 2); else VC_LAUNCH_PAIR(8, 1); break;
+      case 4: if (v2) VC_LAUNCH_PAIR(4, 2); else VC_LAUNCH_PAIR(4, 1); break;
+      case 2: if (v2) VC_LAUNCH_PAIR(2, 2); else VC_LAUNCH_PAIR(2, 1); break;
+      default: if (v2) VC_LAUNCH_PAIR(1, 2); else VC_LAUNCH_PAIR(1, 1); break;
+    }
+#undef VC_LAUNCH_PAIR
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(interleave_luts_kernel, dim3(grid_for(src_elems, 256)), dim3(256),
+                     0, s, luts, (long)nq, groups, groups16, p.tile, lut_t);
 #define VC_LAUNCH_SCAN(TT, VV)                                                                          \
   do {                                                                                                  \
     (void)hipFuncSetAttribute((const void*)scan_topk_kernel<TT, VV>,                                     \

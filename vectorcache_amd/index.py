@@ -288,7 +288,10 @@ def scan_tile_for(dim: int, bits: int, nq: int) -> int:
 
 
 def set_debug(flags: int):
-    """Test hook: bit 0 forces the exact sequential alpha path in encode."""
+    """Test hooks: bit 0 forces the exact sequential alpha path in encode;
+    bit 1 selects the nibble-table scan kernel instead of the byte-pair one;
+    bits 16-31 (when not zero) force the number of vector ranges per query
+    tile (at most 512)."""
     check(lib.vc_set_debug(ctypes.c_uint(flags)))
 
 
