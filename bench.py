@@ -117,13 +117,13 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="flat1m", choices=sorted(WORKLOADS))
-    ap.add_argument("--tile", type=int, default=0, help="force the scan query tile (1, 4, 8)")
+    ap.add_argument("--tile", type=int, default=0, help="cap the scan query tile (1, 2, 4, 8)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-latency", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; the flat scan would shard by rows "
-                                   "with an all-gather of per-GPU top-k (see DESIGN.md)", "n_gpus": args.gpus}))
+                                   "with an all-gather of per-GPU top-k lists", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch
@@ -260,7 +260,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "scan_topk_kernel", "avg_ms": scan_avg_s * 1e3,
+            "kernel": f"scan_pair_kernel (query tile {tile})", "avg_ms": scan_avg_s * 1e3,
             "algorithmic_bytes": scan_bytes,
             "note": f"bytes = ceil(nq/T) passes x N x (G+4) + LUTs; T={tile}; nibble lookups/s = {lookups:.3e}",
         },

@@ -168,6 +168,8 @@ size_t vc_scan_tile_for(size_t dim, size_t bits, size_t nq);
 vc_status vc_set_profiling(int on);
 /* Test hook (no reference counterpart): flags bit 0 makes every later encode
  * take the exact sequential alpha path instead of the certified fast one;
+ * bit 2 makes indexes created later encode with the 8-rows-per-wavefront tile
+ * kernel instead of the two-pass streaming row kernels;
  * bit 1 makes searches use the nibble-table scan kernel (two 16-entry lookups
  * per code byte) instead of the byte-pair table kernel; bits 16-31, when not
  * zero, set the number of vector ranges each query tile is split into (at

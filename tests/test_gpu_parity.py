@@ -77,10 +77,12 @@ def test_index_exports_and_search_match_golden(vc, golden_cases):
             assert scores_equal(r.scores, g[f"scores_k{k}"]), (c, k)
 
 
-def test_exact_alpha_path_matches_golden(vc, golden_cases):
-    """Every vector through the reference's sequential alpha chains (the path
-    the certified fast alpha falls back to) gives the same bits."""
-    vc.set_debug(1)
+@pytest.mark.parametrize("flags", [1, 4, 5])
+def test_exact_alpha_path_matches_golden(vc, golden_cases, flags):
+    """Bit 0: every vector through the reference's sequential alpha chains
+    (the path the certified fast alpha falls back to); bit 2: the 8-rows-per-
+    wavefront tile encode instead of the streaming row encode. Same bits."""
+    vc.set_debug(flags)
     try:
         for c, g, x, q, sample in golden_cases:
             ix = _index(vc, c, x, g, sample)

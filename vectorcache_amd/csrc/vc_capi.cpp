@@ -119,6 +119,7 @@ struct DevRotation {
   uint32_t* inv_perm = nullptr;
   float* signs = nullptr;
   uint32_t* perm_sign = nullptr;  // inv_perm[j] | sign-of-destination << 31
+  uint32_t* perm_gather = nullptr;  // perm[i] | sign[i] << 31 (by destination)
 };
 std::atomic<int> g_debug_flags{0};
 
@@ -143,6 +144,9 @@ DevRotation device_rotation(size_t dim) {
     }
     hip_check(hipMalloc(&r.perm_sign, dim * 4), "hipMalloc");
     hip_check(hipMemcpy(r.perm_sign, ps.data(), dim * 4, hipMemcpyHostToDevice), "copy");
+    for (size_t i = 0; i < dim; ++i) ps[i] = rt.perm[i] | (rt.signs[i] < 0.f ? 0x80000000u : 0u);
+    hip_check(hipMalloc(&r.perm_gather, dim * 4), "hipMalloc");
+    hip_check(hipMemcpy(r.perm_gather, ps.data(), dim * 4, hipMemcpyHostToDevice), "copy");
   }
   return r;
 }
@@ -183,6 +187,7 @@ vchip::CodeTables make_tables(size_t dim, size_t bits) {
   t.inv_perm = dr.inv_perm;
   t.signs = dr.signs;
   t.perm_sign = dr.perm_sign;
+  t.perm_gather = dr.perm_gather;
   const auto& ct = vchip::cell_table(bits, dim);
   t.cells = device_cells(bits, dim);
   t.use_cells = t.cells != nullptr ? 1 : 0;

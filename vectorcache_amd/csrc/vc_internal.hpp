@@ -21,6 +21,7 @@ struct CodeTables {
   const float* scale;             // device, dim
   const float* inv_scale;         // device, dim
   const std::uint32_t* perm_sign;  // device, dim: inv_perm[j] | (signs[inv_perm[j]] < 0) << 31
+  const std::uint32_t* perm_gather;  // device, dim: perm[i] | (signs[i] < 0) << 31
   const void* cells;              // device, 256 x {u32 base, f32 bound, f32 c_lo, f32 c_hi}
   float cell_lo, cell_inv_w;
   int use_cells;

@@ -416,6 +416,257 @@ __global__ __launch_bounds__(64 * kEncodeWaves) void encode_tile_kernel(CodeTabl
 }
 
 // ---------------------------------------------------------------------------
+// Streaming encode in two passes over the rows.
+//
+// norm_rows_kernel: the reference's row norm (8 fused chains c[j % 8] and the
+// tree ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)), src/encode/encode.cpp:16-26) for
+// 8 rows per wavefront, lane = (row, chain); the chains are sequential, their
+// loads are issued kNormAhead at a time.
+//
+// encode_row_kernel: one wavefront per row. The row is prefetched into
+// registers one row ahead, staged in the wavefront's LDS buffer, gathered into
+// rotation order (y[i] = (x[perm[i]] * inv) * sign[i]), transformed and
+// quantized in registers, and its codes leave as 16-byte pieces.
+// ---------------------------------------------------------------------------
+constexpr int kNormAhead = 48;
+
+__global__ __launch_bounds__(256) void norm_rows_kernel(const float* __restrict__ x, long n, int D,
+                                                        float* __restrict__ norms) {
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 3, c = lane & 7;
+  const long n_waves = (long)gridDim.x * (blockDim.x >> 6);
+  const int steps = D / 8;
+  for (long g = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g * 8 < n; g += n_waves) {
+    const long row = g * 8 + q;
+    const bool ok = row < n;
+    const float* xr = x + (ok ? row : 0) * (long)D + c;
+    float acc = 0.f;
+    for (int t0 = 0; t0 < steps; t0 += kNormAhead) {
+      float v[kNormAhead];
+#pragma unroll
+      for (int u = 0; u < kNormAhead; ++u) v[u] = xr[8 * min(t0 + u, steps - 1)];  // clamped: always in bounds
+#pragma unroll
+      for (int u = 0; u < kNormAhead; ++u)
+        if (t0 + u < steps) acc = __fmaf_rn(v[u], v[u], acc);
+    }
+    acc = acc + __shfl_xor(acc, 1);
+    acc = acc + __shfl_xor(acc, 2);
+    acc = acc + __shfl_xor(acc, 4);
+    if (ok && c == 0) norms[row] = sqrt_rn(acc);
+  }
+}
+
+constexpr int kRowMaxWaves = 16;
+
+// per wavefront: the row (D floats) and a staging area that holds the row's
+// codes, later (exact alpha path only) one rotated block at a time
+__host__ __device__ inline size_t row_stage_bytes(int D, int G16, int B) {
+  const size_t c = (size_t)G16 * 16, y = (size_t)B * 4;
+  return enc_align(c > y ? c : y);
+}
+__host__ __device__ inline size_t row_wave_bytes(int D, int G16, int B) {
+  return enc_align((size_t)D * 4) + row_stage_bytes(D, G16, B);
+}
+// shared by the workgroup: gather table, cell table, TQ+ shift | scale | 1/scale
+__host__ __device__ inline size_t row_shared_bytes(int D, bool tq) {
+  return enc_align((size_t)D * 4) + 256 * 16 + (tq ? 3 * enc_align((size_t)D * 4) : 0);
+}
+
+// Value of lane ^ LM: DPP quad permutes for 1 and 2, ds_swizzle (xor within
+// 32 lanes) for 4, 8 and 16, a lane permute for 32.
+template <int LM>
+__device__ __forceinline__ float lane_xor(float v) {
+  const int b = __builtin_bit_cast(int, v);
+  if constexpr (LM == 1) return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(b, 0xB1, 0xF, 0xF, false));
+  else if constexpr (LM == 2) return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(b, 0x4E, 0xF, 0xF, false));
+  else if constexpr (LM < 32) return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(b, (LM << 10) | 0x1F));
+  else return __shfl_xor(v, 32);
+}
+
+// One cross-lane butterfly stage: lower lane a + b, upper lane a - b =
+// partner - self (fma with +-1 is exact).
+template <int R, int LM>
+__device__ __forceinline__ void lane_stage(float (&v)[R], int lane) {
+  const float sg = (lane & LM) ? -1.f : 1.f;
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = __fmaf_rn(sg, v[r], lane_xor<LM>(v[r]));
+}
+
+// Block b of the rotation for this lane: v[r] = y[b * B + lane * R + r]
+// (gathered from the row in X, butterflies in the lane then across lanes,
+// scaled by 1 / sqrt(B)); the reference's pairs and order
+// (src/transform/rotation.cpp:38-121,185-205).
+template <int R>
+__device__ __forceinline__ void rotate_block(const CodeTables& t, const uint32_t* __restrict__ pg,
+                                             const float* __restrict__ X, float inv, int b, int lane, float (&v)[R]) {
+  const int B = t.block;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t w = pg[b * B + lane * R + r];
+    v[r] = (X[w & 0x7fffffffu] * inv) * ((w >> 31) ? -1.f : 1.f);
+  }
+#pragma unroll
+  for (int len = 1; len < R; len <<= 1) {
+#pragma unroll
+    for (int i = 0; i < R; i += 2 * len) {
+#pragma unroll
+      for (int j = i; j < i + len; ++j) {
+        const float a = v[j], c = v[j + len];
+        v[j] = a + c;
+        v[j + len] = a - c;
+      }
+    }
+  }
+  lane_stage<R, 1>(v, lane);
+  lane_stage<R, 2>(v, lane);
+  lane_stage<R, 4>(v, lane);
+  lane_stage<R, 8>(v, lane);
+  lane_stage<R, 16>(v, lane);
+  lane_stage<R, 32>(v, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = v[r] * t.inv_sqrt_block;
+}
+
+// R = block / 64 values per lane per rotation block; NXM >= D / 64 values per
+// lane per row. alpha_io[out_offset + row] holds the row norm on entry and
+// alpha on exit.
+template <int R, int NXM>
+__global__ __launch_bounds__(64 * kRowMaxWaves) void encode_row_kernel(CodeTables t, const float* __restrict__ x,
+                                                                    long n, long out_offset,
+                                                                    uint8_t* __restrict__ tiled,
+                                                                    float* __restrict__ alpha_io) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int D = t.dim, G16 = t.groups16, B = t.block;
+  const int NX = D / 64, nblk = D / B;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+  const bool tq = t.shift != nullptr;
+  uint32_t* pg = reinterpret_cast<uint32_t*>(smem);
+  CellDev* cells = reinterpret_cast<CellDev*>(smem + enc_align((size_t)D * 4));
+  float* tsh = reinterpret_cast<float*>(smem + enc_align((size_t)D * 4) + 256 * 16);
+  float* tsc = tsh + enc_align((size_t)D * 4) / 4;
+  float* tis = tsc + enc_align((size_t)D * 4) / 4;
+  unsigned char* wb = smem + row_shared_bytes(D, tq) + wave * row_wave_bytes(D, G16, B);
+  float* X = reinterpret_cast<float*>(wb);
+  unsigned char* stage = wb + enc_align((size_t)D * 4);
+  uint8_t* codes = stage;
+  float* ys = reinterpret_cast<float*>(stage);
+
+  for (int j = threadIdx.x; j < D; j += blockDim.x) pg[j] = t.perm_gather[j];
+  if (t.use_cells)
+    for (int j = threadIdx.x; j < 256 * 4; j += blockDim.x)
+      reinterpret_cast<uint32_t*>(cells)[j] = reinterpret_cast<const uint32_t*>(t.cells)[j];
+  if (tq)
+    for (int j = threadIdx.x; j < D; j += blockDim.x) {
+      tsh[j] = t.shift[j];
+      tsc[j] = t.scale[j];
+      tis[j] = t.inv_scale[j];
+    }
+  __syncthreads();
+
+  const int cpb = t.bits == 2 ? 4 : 2;  // codes per byte
+  const long stride = (long)gridDim.x * waves;
+  for (long row = (long)blockIdx.x * waves + wave; row < n; row += stride) {
+    {
+      // the row, lane + 64 * i (clamped loads: always in bounds), into X
+      const float* xr = x + row * (long)D + lane;
+      float xv[NXM];
+#pragma unroll
+      for (int i = 0; i < NXM; ++i) xv[i] = xr[64 * min(i, NX - 1)];
+#pragma unroll
+      for (int i = 0; i < NXM; ++i)
+        if (i < NX) X[lane + 64 * i] = xv[i];
+    }
+    // bytes past the last group of the final 16-byte piece stay zero
+    for (int i = (t.groups & ~15) + lane; i < G16 * 16; i += 64) codes[i] = 0;
+    const float nv = alpha_io[out_offset + row];
+    wave_lds_sync();
+    const float inv = nv > kMinNorm ? div_rn(1.f, nv) : 0.f;
+
+    double sum = 0.0, asum = 0.0;
+#pragma unroll 1
+    for (int b = 0; b < nblk; ++b) {
+      float v[R];
+      rotate_block<R>(t, pg, X, inv, b, lane, v);
+      uint64_t cw = 0;  // this lane's codes of the block, packed as flat bytes
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float yv = v[r];
+        const int j = b * B + lane * R + r;
+        float cen;
+        const unsigned k = quant_code(t, cells, tq ? (yv + tsh[j]) * tsc[j] : yv, cen);
+        double term;
+        if (!tq)
+          term = (double)yv * (double)cen;
+        else
+          term = (double)yv * fma((double)cen, (double)tis[j], -(double)tsh[j]);
+        sum += term;
+        asum += fabs(term);
+        const int slot = r % cpb;  // position inside its byte, first code highest
+        const int shift = (cpb - 1 - slot) * (8 / cpb);
+        cw |= (uint64_t)k << (8 * (r / cpb) + shift);
+      }
+      uint8_t* dst = codes + (b * B + lane * R) / cpb;
+#pragma unroll
+      for (int i = 0; i < R / cpb; ++i) dst[i] = (uint8_t)(cw >> (8 * i));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      sum += __shfl_xor(sum, o);
+      asum += __shfl_xor(asum, o);
+    }
+    wave_lds_sync();  // the row's codes are complete
+    const long v = out_offset + row;
+    for (int c16 = lane; c16 < G16; c16 += 64)
+      *reinterpret_cast<uint4*>(tiled + ((v >> 6) * G16 + c16) * 1024 + (v & 63) * 16) =
+          *reinterpret_cast<const uint4*>(codes + c16 * 16);
+
+    // certified alpha (the bound of encode_tile_kernel), else the reference's
+    // sequential chains over the rotated values, one block at a time
+    const double u = 1.1102230246251565e-16;  // 2^-53
+    double bound = (double)(D / 4 + D / 64 + 16) * u * asum * 1.01;
+    bound += 2.0 * u * (fabs(sum) + bound);
+    const double lo = sum - bound, hi = sum + bound;
+    float a = 0.f;
+    bool exact = (t.debug_flags & 1) == 0;
+    if (exact && lo > 1e-12 && __double2float_rn(lo) == __double2float_rn(hi))
+      a = div_rn(nv, __double2float_rn(hi));
+    else if (exact && hi <= 1e-12)
+      a = 0.f;
+    else
+      exact = false;
+    if (!exact) {
+      wave_lds_sync();  // codes written out: the staging area takes rotated blocks
+      double d = 0.0;   // lane m < 4: chain m
+#pragma unroll 1
+      for (int b = 0; b < nblk; ++b) {
+        float yb[R];
+        rotate_block<R>(t, pg, X, inv, b, lane, yb);
+#pragma unroll
+        for (int r = 0; r < R; ++r) ys[lane * R + r] = yb[r];
+        wave_lds_sync();
+        if (lane < 4) {
+          for (int jj = lane; jj < B; jj += 4) {
+            const int j = b * B + jj;
+            const float yv = ys[jj];
+            float cen;
+            (void)quant_code(t, cells, tq ? (yv + tsh[j]) * tsc[j] : yv, cen);
+            if (!tq)
+              d = fma((double)yv, (double)cen, d);
+            else
+              d = fma((double)yv, fma((double)cen, (double)tis[j], -(double)tsh[j]), d);
+          }
+        }
+        wave_lds_sync();
+      }
+      const double pr = d + __shfl_xor(d, 1);
+      const double inner = pr + __shfl_xor(pr, 2);
+      a = inner > 1e-12 ? div_rn(nv, (float)inner) : 0.f;
+    }
+    if (lane == 0) alpha_io[v] = a;
+    wave_lds_sync();  // X and the staging area reused by the next row
+  }
+}
+
+// ---------------------------------------------------------------------------
 // TQ+ calibration (src/encode/encode.cpp:142-150,282-326)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t f32_key(float x) {
@@ -1048,6 +1299,13 @@ constexpr int kPairStageBytes = 8192;
 
 template <int T> struct PairFmt { static constexpr bool wide = (T == 2 || T == 4); };
 template <int T> constexpr int pair_entry_bytes() { return T * (PairFmt<T>::wide ? 2 : 1); }
+// 16-group code blocks per 64 KiB table chunk
+__host__ __device__ constexpr int pair_chunk_blocks(int T) { return kPairTableBytes / (256 * T * ((T == 2 || T == 4) ? 2 : 1)) / 16; }
+// 16-group blocks of source rows per query tile: padded to whole chunks
+__host__ __device__ inline long pair_src_blocks(int T, int G16) {
+  const int cb = pair_chunk_blocks(T);
+  return (long)((G16 + cb - 1) / cb) * cb;
+}
 
 // Source rows of every query tile: element (tile, g, e, t) = lut[q][g][e] of
 // query q = tile * T + t (zero for padded groups / queries), u16 when wide.
@@ -1055,7 +1313,7 @@ template <int T>
 __global__ void pair_source_kernel(const uint8_t* __restrict__ luts, long nq, int G, int G16,
                                    uint8_t* __restrict__ out) {
   const long n_qt = (nq + T - 1) / T;
-  const long per_tile = (long)G16 * 16 * 32 * T;
+  const long per_tile = pair_src_blocks(T, G16) * 16 * 32 * T;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n_qt * per_tile; i += (long)gridDim.x * blockDim.x) {
     const long qt = i / per_tile;
     const long r = i - qt * per_tile;
@@ -1144,21 +1402,22 @@ __global__ __launch_bounds__(kScanThreads, 8) void scan_pair_kernel(
 
   TopkShared sh;
   init_topk(sh, smem + kPairTableBytes + kPairStageBytes, T, k, cap);
-  float scl[T], bias[T];
-#pragma unroll
-  for (int t = 0; t < T; ++t) {
-    const long q = q0 + t < nq ? q0 + t : nq - 1;
-    scl[t] = lut_scale[q];
-    bias[t] = bias_total[q];
+  // per-query LUT scale and bias in LDS (read at window flushes and step
+  // starts only; keeps them out of the scalar register budget)
+  float* scl = reinterpret_cast<float*>(smem + kPairTableBytes + kPairStageBytes +
+                                        (size_t)T * (4 * (size_t)k + 2 * (size_t)cap) * 4 + 16 * T);
+  float* bias = scl + T;
+  if (tid < T) {
+    const long q = q0 + tid < nq ? q0 + tid : nq - 1;
+    scl[tid] = lut_scale[q];
+    bias[tid] = bias_total[q];
   }
+  __syncthreads();
 
+  // the source rows are padded to whole chunks, so every load below is in bounds
   const int n_chunks = (G16 + CB - 1) / CB;
-  const uint64_t* tile_src = reinterpret_cast<const uint64_t*>(src + qt * ((long)G16 * 16 * 32 * EB));
-  const long src_words = (long)G16 * 16 * 32 * EB / 8;
-  auto src_word = [&](int c) -> uint64_t {
-    const long i = (long)c * kScanThreads + tid;
-    return i < src_words ? tile_src[i] : 0ull;
-  };
+  const uint64_t* tile_src = reinterpret_cast<const uint64_t*>(src + qt * ((long)n_chunks * CB * 16 * 32 * EB));
+  auto src_word = [&](int c) -> uint64_t { return tile_src[(long)c * kScanThreads + tid]; };
   reinterpret_cast<uint64_t*>(stage)[tid] = src_word(0);
   uint64_t sw = src_word(n_chunks > 1 ? 1 : 0);
   bool built = false;
@@ -1535,6 +1794,44 @@ hipError_t launch_encode(const CodeTables& t, const float* x, std::size_t n, std
                          hipStream_t s) {
   if (n == 0) return hipSuccess;
   const int R = t.block / 64;
+  const int NX = t.dim / 64;
+  if (tiled != nullptr && alpha != nullptr && packed == nullptr && rotated == nullptr && t.dim % 64 == 0 &&
+      t.perm_gather != nullptr && (R == 2 || R == 4 || R == 8 || R == 16) && (t.bits != 2 || R >= 4) &&
+      NX <= 64 && (t.debug_flags & 4) == 0 &&
+      row_shared_bytes(t.dim, t.shift != nullptr) + row_wave_bytes(t.dim, t.groups16, t.block) <= (size_t)kLdsBudget) {
+    // two streaming passes: row norms into alpha[], then one wavefront per row
+    const size_t shared = row_shared_bytes(t.dim, t.shift != nullptr);
+    const size_t per_wave = row_wave_bytes(t.dim, t.groups16, t.block);
+    const int waves = (int)std::min<size_t>(kRowMaxWaves, ((size_t)kLdsBudget - std::min(shared, (size_t)kLdsBudget)) / per_wave);
+    hipLaunchKernelGGL(norm_rows_kernel, dim3(grid_for((long)(n + 31) / 32, 1, 16384)), dim3(256), 0, s, x, (long)n,
+                       t.dim, alpha + out_offset);
+    const size_t lds = shared + (size_t)waves * per_wave;
+    const int grid = grid_for((long)(n + waves - 1) / waves, 1, 4096);
+#define VC_ROW(RR, NN)                                                                                        \
+  do {                                                                                                        \
+    (void)hipFuncSetAttribute((const void*)encode_row_kernel<RR, NN>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                              (int)lds);                                                                      \
+    hipLaunchKernelGGL((encode_row_kernel<RR, NN>), dim3(grid), dim3(64 * waves), lds, s, t, x, (long)n,          \
+                       (long)out_offset, tiled, alpha);                                                       \
+  } while (0)
+    if (NX <= 8) {
+      if (R == 2) VC_ROW(2, 8);
+      else if (R == 4) VC_ROW(4, 8);
+      else VC_ROW(8, 8);
+    } else if (NX <= 32) {
+      if (R == 2) VC_ROW(2, 32);
+      else if (R == 4) VC_ROW(4, 32);
+      else if (R == 8) VC_ROW(8, 32);
+      else VC_ROW(16, 32);
+    } else {
+      if (R == 2) VC_ROW(2, 64);
+      else if (R == 4) VC_ROW(4, 64);
+      else if (R == 8) VC_ROW(8, 64);
+      else VC_ROW(16, 64);
+    }
+#undef VC_ROW
+    return hipGetLastError();
+  }
   const bool fast_shape = (R == 2 && t.bits != 2) || R == 4 || R == 8 || R == 16;
   if (tiled != nullptr && alpha != nullptr && packed == nullptr && rotated == nullptr && fast_shape &&
       t.perm_sign != nullptr && (t.bits != 2 || R >= 4) && enc_lds_bytes(t.dim, t.groups16) <= (size_t)kLdsBudget) {
@@ -1583,7 +1880,9 @@ hipError_t launch_prepare_queries(const CodeTables& t, const float* q, std::size
   return hipGetLastError();
 }
 
-static size_t topk_lds(int T, int k, int cap) { return (size_t)T * (4 * (size_t)k + 2 * (size_t)cap) * 4 + 16 * T + 64; }
+// top-k state of a workgroup (lists, candidates, counters) + per-query scale
+// and bias for the pair kernel
+static size_t topk_lds(int T, int k, int cap) { return (size_t)T * (4 * (size_t)k + 2 * (size_t)cap) * 4 + 24 * T + 64; }
 
 // Vector ranges per query tile: enough workgroups to fill every CU slot when
 // the query tiles alone do not; otherwise the split (at most 8) that leaves
@@ -1616,9 +1915,9 @@ static void plan_ranges(ScanPlan& p, std::size_t n, std::size_t nq) {
 ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int forced_tile, bool nibble) {
   ScanPlan p{};
   if (!nibble) {
-    // Byte-pair kernel. Relative cost of one pass over the codes (LDS gather
-    // rate vs HBM): T = 1 : 2 : 4 : 8 ~ 1 : 1.4 : 1.4 : 2.
-    const int want = nq >= 5 ? 8 : nq >= 3 ? 4 : nq == 2 ? 2 : 1;
+    // Byte-pair kernel. Measured on 1M x 1536 4-bit: one query per tile is
+    // fastest up to two queries, a 4-query tile for three or four, 8 above.
+    const int want = nq >= 5 ? 8 : nq >= 3 ? 4 : 1;
     const std::size_t fixed = (std::size_t)kPairTableBytes + kPairStageBytes;
     for (int T : {8, 4, 2, 1}) {
       if (T > (forced_tile != 0 ? forced_tile : want)) continue;  // a forced tile is an upper bound
@@ -1640,7 +1939,7 @@ ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int force
       p.cand_cap = cap;
       p.lds_bytes = lds;
       p.lut_chunk16 = 0;
-      p.vec_per_thread = T == 2 ? 2 : 1;  // register budget of 8 waves / SIMD
+      p.vec_per_thread = T == 1 ? 1 : 2;  // measured; T = 1 keeps a deeper code prefetch instead
       break;
     }
     if (p.tile == 0) return p;  // caller reports the error
@@ -1683,8 +1982,11 @@ ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int force
 
 std::size_t scan_lut_bytes(const ScanPlan& p, int groups16, std::size_t nq) {
   const std::size_t n_qt = (nq + p.tile - 1) / p.tile;
-  const std::size_t per_elem = p.pair && (p.tile == 2 || p.tile == 4) ? 2 : 1;
-  return n_qt * (std::size_t)groups16 * 16 * 32 * p.tile * per_elem;
+  if (p.pair) {
+    const std::size_t per_elem = (p.tile == 2 || p.tile == 4) ? 2 : 1;
+    return n_qt * (std::size_t)pair_src_blocks(p.tile, groups16) * 16 * 32 * p.tile * per_elem;
+  }
+  return n_qt * (std::size_t)groups16 * 16 * 32 * p.tile;
 }
 
 hipError_t launch_scan(const ScanPlan& p, const std::uint8_t* tiled, const float* alpha, std::size_t n,
@@ -1696,9 +1998,10 @@ hipError_t launch_scan(const ScanPlan& p, const std::uint8_t* tiled, const float
   const dim3 block(kScanThreads);
   const long src_elems = n_qt * (long)groups16 * 16 * 32 * p.tile;
   if (p.pair) {
+    const long pair_elems = n_qt * pair_src_blocks(p.tile, groups16) * 16 * 32 * p.tile;
 #define VC_LAUNCH_PAIR(TT, VV)                                                                            \
   do {                                                                                                    \
-    hipLaunchKernelGGL(pair_source_kernel<TT>, dim3(grid_for(src_elems, 256)), dim3(256), 0, s, luts,     \
+    hipLaunchKernelGGL(pair_source_kernel<TT>, dim3(grid_for(pair_elems, 256)), dim3(256), 0, s, luts,    \
                        (long)nq, groups, groups16, lut_t);                                                \
     (void)hipFuncSetAttribute((const void*)scan_pair_kernel<TT, VV>,                                       \
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_bytes);              \

@@ -289,6 +289,7 @@ def scan_tile_for(dim: int, bits: int, nq: int) -> int:
 
 def set_debug(flags: int):
     """Test hooks: bit 0 forces the exact sequential alpha path in encode;
+    bit 2 selects the tile encode kernel for indexes created afterwards;
     bit 1 selects the nibble-table scan kernel instead of the byte-pair one;
     bits 16-31 (when not zero) force the number of vector ranges per query
     tile (at most 512)."""
