@@ -772,80 +772,118 @@ __device__ __forceinline__ float lut_value(const float* __restrict__ q, const Co
   return s;
 }
 
-__global__ __launch_bounds__(64) void prepare_queries_kernel(CodeTables t, const float* __restrict__ q,
-                                                             long nq, uint8_t* __restrict__ luts,
-                                                             float* __restrict__ lut_scale,
-                                                             float* __restrict__ lut_bias,
-                                                             float* __restrict__ bias_corr,
-                                                             float* __restrict__ bias_total) {
+// One query per 256-thread workgroup (src/search/search.cpp:149-219,762-860):
+// row norm (8 chains) -> rotation -> TQ+ (bias_corr, y / scale) -> per group
+// the 16 + 16 nibble sums, their mins and the global span -> scale = span /
+// 127, u8 entries, bias = sequential sum of the group mins. The chains and the
+// two sequential sums run on one lane each in the reference's order; every
+// other step is spread over the workgroup.
+constexpr int kPrepThreads = 256;
+
+__global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTables t, const float* __restrict__ q,
+                                                                      long nq, uint8_t* __restrict__ luts,
+                                                                      float* __restrict__ lut_scale,
+                                                                      float* __restrict__ lut_bias,
+                                                                      float* __restrict__ bias_corr,
+                                                                      float* __restrict__ bias_total) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int D = t.dim, G = t.groups;
-  float* y = reinterpret_cast<float*>(smem);
-  float* mins = y + D;       // 2 * G
-  float* xs = mins + 2 * G;  // D: the staged query row
-  __shared__ float s_bias, s_bc;
-  const int lane = threadIdx.x;
+  const int D = t.dim, G = t.groups, B = t.block;
+  const bool tq = t.shift != nullptr;
+  float* y = reinterpret_cast<float*>(smem);   // D: rotated query
+  float* xs = y + D;                            // D: the staged query row
+  float* mins = xs + D;                         // 2 * G
+  float* tsh = mins + 2 * G;                    // D (TQ+)
+  float* tsc = tsh + D;                         // D (TQ+)
+  __shared__ float s_norm, s_bias, s_bc, s_span[kPrepThreads / 64];
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
   const int cpb = 8 / t.bits, cpn = cpb / 2;
+  if (tq)
+    for (int j = tid; j < D; j += nt) {
+      tsh[j] = t.shift[j];
+      tsc[j] = t.scale[j];
+    }
   for (long qi = blockIdx.x; qi < nq; qi += gridDim.x) {
     const float* xr = q + qi * (long)D;
     __syncthreads();
-    for (int j = lane; j < D; j += 64) xs[j] = xr[j];
+    for (int j = tid; j < D; j += nt) xs[j] = xr[j];
     __syncthreads();
-    const float nrm = norm8_lds(xs, D, lane);
+    if (wave == 0) {
+      const float nrm = norm8_lds(xs, D, lane);
+      if (lane == 0) s_norm = nrm;
+    }
+    __syncthreads();
+    const float nrm = s_norm;
     const float inv = nrm > kMinNorm ? div_rn(1.f, nrm) : 0.f;
-    rotate_row_lds(t, xs, inv, y, lane);
-    if (t.shift != nullptr) {
+    // y[i] = (x[perm[i]] * inv) * sign[i], scattered through inv_perm
+    for (int j = tid; j < D; j += nt) {
+      const uint32_t w = t.perm_sign[j];
+      y[w & 0x7fffffffu] = (xs[j] * inv) * ((w >> 31) ? -1.f : 1.f);
+    }
+    __syncthreads();
+    const int npairs = D >> 1, lhb = __ffs(B) - 2;  // log2(B / 2)
+    for (int lg = 0; (1 << lg) < B; ++lg) {
+      const int len = 1 << lg;
+      for (int p = tid; p < npairs; p += nt) {
+        const int blk = p >> lhb, qq = p & ((B >> 1) - 1);
+        const int i = blk * B + ((qq >> lg) << (lg + 1)) + (qq & (len - 1));
+        const float u = y[i], v = y[i + len];
+        y[i] = u + v;
+        y[i + len] = u - v;
+      }
+      __syncthreads();
+    }
+    for (int i = tid; i < D; i += nt) y[i] = y[i] * t.inv_sqrt_block;
+    __syncthreads();
+    if (tq) {
       // bias_corr = (float) -sum (double) y_d * shift_d, sequential in d
-      if (lane == 0) {
+      if (tid == 0) {
         double acc = 0.0;
-        for (int d = 0; d < D; ++d) acc = fma(-(double)y[d], (double)t.shift[d], acc);
+        for (int d = 0; d < D; ++d) acc = fma(-(double)y[d], (double)tsh[d], acc);
         s_bc = (float)acc;
       }
       __syncthreads();
-      for (int d = lane; d < D; d += 64) y[d] = div_rn(y[d], t.scale[d]);
-    } else if (lane == 0) {
+      for (int d = tid; d < D; d += nt) y[d] = div_rn(y[d], tsc[d]);
+    } else if (tid == 0) {
       s_bc = 0.f;
     }
     __syncthreads();
     // pass 1: per group the min/max of both 16-entry halves; global span
     float span = 0.f;
-    for (int g = lane; g < G; g += 64) {
+    for (int g = tid; g < G; g += nt) {
       for (int half = 0; half < 2; ++half) {
         float mn = FLT_MAX, mx = -FLT_MAX;
         for (unsigned nib = 0; nib < 16; ++nib) {
-          const float s = lut_value(y, t, g * cpb + half * cpn, cpn, nib);
-          mn = s < mn ? s : mn;
-          mx = s > mx ? s : mx;
+          const float sv = lut_value(y, t, g * cpb + half * cpn, cpn, nib);
+          mn = sv < mn ? sv : mn;
+          mx = sv > mx ? sv : mx;
         }
         mins[2 * g + half] = mn;
         span = fmaxf(span, mx - mn);
       }
     }
     for (int off = 32; off > 0; off >>= 1) span = fmaxf(span, __shfl_xor(span, off));
+    if (lane == 0) s_span[wave] = span;
     __syncthreads();
-    if (lane == 0) {
-      float b = 0.f;  // sequential over groups, as the reference
-      for (int g = 0; g < G; ++g) b = b + (mins[2 * g] + mins[2 * g + 1]);
-      s_bias = b;
-    }
-    __syncthreads();
+    span = s_span[0];
+    for (int w = 1; w < nt / 64; ++w) span = fmaxf(span, s_span[w]);
     float scale = span > 0.f ? div_rn(span, 127.f) : 1.f;
     float iscale = 1.f;
     if (scale >= FLT_MIN)
       iscale = div_rn(1.f, scale);
     else
       scale = 1.f;
-    // pass 2: quantized entries
+    // pass 2: quantized entries; the bias (sequential over groups, as the
+    // reference) runs on the last lane after its own groups
     uint8_t* out = luts + qi * (long)G * 32;
-    for (int g = lane; g < G; g += 64) {
+    for (int g = tid; g < G; g += nt) {
       uint32_t w[8];
       for (int half = 0; half < 2; ++half) {
         const float mn = mins[2 * g + half];
         for (int e = 0; e < 16; e += 4) {
           uint32_t word = 0;
           for (int b = 0; b < 4; ++b) {
-            const float s = lut_value(y, t, g * cpb + half * cpn, cpn, (unsigned)(e + b));
-            float r = roundf((s - mn) * iscale);
+            const float sv = lut_value(y, t, g * cpb + half * cpn, cpn, (unsigned)(e + b));
+            float r = roundf((sv - mn) * iscale);
             r = r < 0.f ? 0.f : (r > 127.f ? 127.f : r);
             word |= uint32_t(r) << (8 * b);
           }
@@ -856,13 +894,14 @@ __global__ __launch_bounds__(64) void prepare_queries_kernel(CodeTables t, const
       dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
       dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
     }
-    if (lane == 0) {
+    if (tid == nt - 1) {
+      float bsum = 0.f;
+      for (int g = 0; g < G; ++g) bsum = bsum + (mins[2 * g] + mins[2 * g + 1]);
       lut_scale[qi] = scale;
-      lut_bias[qi] = s_bias;
+      lut_bias[qi] = bsum;
       bias_corr[qi] = s_bc;
-      bias_total[qi] = s_bias + s_bc;
+      bias_total[qi] = bsum + s_bc;
     }
-    __syncthreads();
   }
 }
 
@@ -1542,8 +1581,73 @@ __global__ __launch_bounds__(kScanThreads, 8) void scan_pair_kernel(
   write_partials<T>(sh, q0, nq, n_ranges, r, k, cap, part_s, part_i, part_cnt);
 }
 
-// Merge the per-range sorted lists of one query (one wavefront per query).
+// Merge the per-range sorted lists of one query into its final row (one
+// wavefront): each lane holds up to 8 lists and keeps their head entries in
+// registers; every output is a wave argmax under (score desc, id asc).
+// ps / pi: the query's lists [R][k]; pc: their lengths [R].
 constexpr int kMaxListsPerLane = 8;  // n_ranges <= 512
+
+__device__ void merge_query(const float* __restrict__ ps, const uint32_t* __restrict__ pi,
+                            const int* __restrict__ pc, int R, int k, int k_out,
+                            const uint64_t* __restrict__ id_map, float* __restrict__ out_s,
+                            uint64_t* __restrict__ out_i, int lane) {
+  int head[kMaxListsPerLane], len[kMaxListsPerLane];
+  float hs[kMaxListsPerLane];
+  uint32_t hid[kMaxListsPerLane];
+#pragma unroll
+  for (int j = 0; j < kMaxListsPerLane; ++j) {
+    const int r = lane + 64 * j;
+    head[j] = 0;
+    len[j] = r < R ? pc[r] : 0;
+    hs[j] = len[j] > 0 ? ps[(long)r * k] : 0.f;
+    hid[j] = len[j] > 0 ? pi[(long)r * k] : 0u;
+  }
+  int j_out = 0;
+  for (; j_out < k_out; ++j_out) {
+    bool have = false;
+    float bs = 0.f;
+    uint32_t bi = 0u;
+    int bj = -1;
+#pragma unroll
+    for (int j = 0; j < kMaxListsPerLane; ++j)
+      if (head[j] < len[j] && (!have || better(hs[j], hid[j], bs, bi))) {
+        have = true;
+        bs = hs[j];
+        bi = hid[j];
+        bj = j;
+      }
+    bool wh = have;
+    float ws = bs;
+    uint32_t wi = bi;
+    for (int o = 32; o > 0; o >>= 1) {
+      const int oh = __shfl_xor((int)wh, o);
+      const float os = __shfl_xor(ws, o);
+      const uint32_t oi = (uint32_t)__shfl_xor((int)wi, o);
+      if (oh && (!wh || better(os, oi, ws, wi))) { wh = true; ws = os; wi = oi; }
+    }
+    if (!wh) break;  // the same on every lane
+    if (have && bs == ws && bi == wi) {
+#pragma unroll
+      for (int j = 0; j < kMaxListsPerLane; ++j)
+        if (j == bj) {
+          ++head[j];
+          if (head[j] < len[j]) {
+            const long p = (long)(lane + 64 * j) * k + head[j];
+            hs[j] = ps[p];
+            hid[j] = pi[p];
+          }
+        }
+    }
+    if (lane == 0) {
+      out_s[j_out] = ws;
+      out_i[j_out] = id_map ? id_map[wi] : (uint64_t)wi;
+    }
+  }
+  for (int j = j_out + lane; j < k_out; j += 64) {
+    out_s[j] = 0.f;
+    out_i[j] = 0u;
+  }
+}
 
 __global__ __launch_bounds__(64) void merge_ranges_kernel(const float* __restrict__ part_s,
                                                           const uint32_t* __restrict__ part_i,
@@ -1552,54 +1656,37 @@ __global__ __launch_bounds__(64) void merge_ranges_kernel(const float* __restric
                                                           const uint64_t* __restrict__ id_map,
                                                           float* __restrict__ out_s,
                                                           uint64_t* __restrict__ out_i) {
-  const int lane = threadIdx.x;
+  for (long q = blockIdx.x; q < nq; q += gridDim.x)
+    merge_query(part_s + q * R * (long)k, part_i + q * R * (long)k, part_cnt + q * R, R, k, k_out, id_map,
+                out_s + q * k_out, out_i + q * k_out, threadIdx.x);
+}
+
+// The same with the query's lists first copied to LDS by the whole workgroup
+// (one coalesced pass), when they fit.
+constexpr int kMergeStageBytes = 48 * 1024;
+
+__global__ __launch_bounds__(256) void merge_ranges_lds_kernel(const float* __restrict__ part_s,
+                                                               const uint32_t* __restrict__ part_i,
+                                                               const int* __restrict__ part_cnt, long nq,
+                                                               int R, int k, int k_out,
+                                                               const uint64_t* __restrict__ id_map,
+                                                               float* __restrict__ out_s,
+                                                               uint64_t* __restrict__ out_i) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = R * k;
+  float* ls = reinterpret_cast<float*>(smem);
+  uint32_t* li = reinterpret_cast<uint32_t*>(ls + n);
+  int* lc = reinterpret_cast<int*>(li + n);
   for (long q = blockIdx.x; q < nq; q += gridDim.x) {
-    int head[kMaxListsPerLane], len[kMaxListsPerLane];
-#pragma unroll
-    for (int j = 0; j < kMaxListsPerLane; ++j) {
-      const int r = lane + 64 * j;
-      head[j] = 0;
-      len[j] = r < R ? part_cnt[q * R + r] : 0;
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+      ls[e] = part_s[q * n + e];
+      li[e] = part_i[q * n + e];
     }
-    int j_out = 0;
-    for (; j_out < k_out; ++j_out) {
-      bool have = false;
-      float bs = 0.f;
-      uint32_t bi = 0u;
-      int bj = -1;
-#pragma unroll
-      for (int j = 0; j < kMaxListsPerLane; ++j) {
-        if (head[j] < len[j]) {
-          const long p = (q * R + lane + 64 * j) * (long)k + head[j];
-          const float s = part_s[p];
-          const uint32_t id = part_i[p];
-          if (!have || better(s, id, bs, bi)) { have = true; bs = s; bi = id; bj = j; }
-        }
-      }
-      // wave argmax under (score desc, id asc)
-      bool wh = have;
-      float ws = bs;
-      uint32_t wi = bi;
-      for (int o = 32; o > 0; o >>= 1) {
-        const int oh = __shfl_xor((int)wh, o);
-        const float os = __shfl_xor(ws, o);
-        const uint32_t oi = (uint32_t)__shfl_xor((int)wi, o);
-        if (oh && (!wh || better(os, oi, ws, wi))) { wh = true; ws = os; wi = oi; }
-      }
-      if (!wh) break;
-      if (have && bs == ws && bi == wi) {
-#pragma unroll
-        for (int j = 0; j < kMaxListsPerLane; ++j) if (j == bj) ++head[j];
-      }
-      if (lane == 0) {
-        out_s[q * k_out + j_out] = ws;
-        out_i[q * k_out + j_out] = id_map ? id_map[wi] : (uint64_t)wi;
-      }
-    }
-    for (int j = j_out + lane; j < k_out; j += 64) {
-      out_s[q * k_out + j] = 0.f;
-      out_i[q * k_out + j] = 0u;
-    }
+    for (int r = threadIdx.x; r < R; r += blockDim.x) lc[r] = part_cnt[q * R + r];
+    __syncthreads();
+    if (threadIdx.x < 64)
+      merge_query(ls, li, lc, R, k, k_out, id_map, out_s + q * k_out, out_i + q * k_out, threadIdx.x);
   }
 }
 
@@ -1873,10 +1960,13 @@ hipError_t launch_prepare_queries(const CodeTables& t, const float* q, std::size
                                   std::uint8_t* luts, float* lut_scale, float* lut_bias,
                                   float* bias_corr, float* bias_total, hipStream_t s) {
   if (nq == 0) return hipSuccess;
-  const size_t lds = (size_t)t.dim * 8 + (size_t)t.groups * 8;
+  const size_t lds = (size_t)t.dim * 8 + (size_t)t.groups * 8 + (t.shift != nullptr ? (size_t)t.dim * 8 : 0);
   const int grid = grid_for((long)nq, 1, 16384);
-  hipLaunchKernelGGL(prepare_queries_kernel, dim3(grid), dim3(64), lds, s, t, q, (long)nq, luts, lut_scale,
-                     lut_bias, bias_corr, bias_total);
+  if (lds > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)prepare_queries_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+  hipLaunchKernelGGL(prepare_queries_kernel, dim3(grid), dim3(kPrepThreads), lds, s, t, q, (long)nq, luts,
+                     lut_scale, lut_bias, bias_corr, bias_total);
   return hipGetLastError();
 }
 
@@ -2047,8 +2137,14 @@ hipError_t launch_merge_ranges(const float* part_s, const std::uint32_t* part_i,
                                float* out_s, std::uint64_t* out_i, hipStream_t s) {
   if (nq == 0 || k_out == 0) return hipSuccess;
   const int grid = grid_for((long)nq, 1, 65536);
-  hipLaunchKernelGGL(merge_ranges_kernel, dim3(grid), dim3(64), 0, s, part_s, part_i, part_cnt, (long)nq,
-                     n_ranges, k, k_out, id_map, out_s, out_i);
+  const size_t staged = (size_t)n_ranges * k * 8 + (size_t)n_ranges * 4;
+  if (n_ranges > 1 && staged <= (size_t)kMergeStageBytes) {
+    hipLaunchKernelGGL(merge_ranges_lds_kernel, dim3(grid), dim3(256), staged, s, part_s, part_i, part_cnt,
+                       (long)nq, n_ranges, k, k_out, id_map, out_s, out_i);
+  } else {
+    hipLaunchKernelGGL(merge_ranges_kernel, dim3(grid), dim3(64), 0, s, part_s, part_i, part_cnt, (long)nq,
+                       n_ranges, k, k_out, id_map, out_s, out_i);
+  }
   return hipGetLastError();
 }
 
