@@ -111,6 +111,40 @@ def cpu_baseline(ix, q_np, gpu_ids, gpu_scores, bits, shift, scale, budget_s=12.
     }
 
 
+def copy_probe(dev, nbytes=1 << 30, reps=5):
+    """Device-to-device copy of a known size: the achievable HBM ceiling
+    (read + write) and the reference size for scaling rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "gb_per_s": 2 * nbytes / ms / 1e6}
+
+
+def committed_traffic(workload, nq, tile):
+    """HBM bytes per launch of the dominant scan launch, from the committed
+    counter profile of this workload (FETCH_SIZE and WRITE_SIZE, each scaled
+    by what the counter reads for the copy probe's known size); None when
+    no profile of this workload and query tile is committed."""
+    p = ROOT / "profiles" / f"scan_pair_kernel_{workload}_traffic.json"
+    if not p.exists():
+        return None, None
+    d = json.loads(p.read_text())
+    if d.get("nq") != nq or d.get("query_tile") != tile:
+        return None, None
+    return d["traffic_bytes_per_launch"], str(p.relative_to(ROOT))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,6 +263,9 @@ def main():
             lat[f"batch{b}"] = {"p50_ms": float(np.percentile(ts, 50)), "p99_ms": float(np.percentile(ts, 99))}
         log(f"latency {lat}")
 
+    probe = copy_probe(dev)
+    log(f"copy probe {probe}")
+
     # ---- CPU baseline (reference build) ----
     cpu = None
     if not args.no_cpu:
@@ -239,6 +276,7 @@ def main():
         except Exception as e:  # reported, never fatal for the GPU figure
             cpu = {"error": str(e)}
 
+    traffic, traffic_src = committed_traffic(args.workload, nq, tile)
     out = {
         "metric": METRIC,
         "value": qps,
@@ -259,7 +297,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
             "kernel": f"scan_pair_kernel (query tile {tile})", "avg_ms": scan_avg_s * 1e3,
             "algorithmic_bytes": scan_bytes,
             "note": f"bytes = ceil(nq/T) passes x N x (G+4) + LUTs; T={tile}; nibble lookups/s = {lookups:.3e}",
@@ -267,6 +305,7 @@ def main():
         "encode": {"vectors_per_s": n / enc_s, "ms": enc_s * 1e3, "gb_per_s": enc_bytes / enc_s / 1e9,
                    "frac": enc_bytes / enc_s / 1e9 / HBM_PEAK_GBS},
         "roofline_bandwidth_regime": bw,
+        "copy_probe": probe,
         "recall_at_10_vs_fp64": recall,
         "latency": lat,
         "cpu_baseline": cpu,

@@ -182,11 +182,13 @@ def test_random_vs_oracle(vc, oracle, bits, dim, n, nq, k):
 
 
 @pytest.mark.parametrize("kernel,tile,ranges", [("nibble", 1, 8), ("nibble", 8, 2), ("pair", 1, 8),
-                                                ("pair", 2, 8), ("pair", 4, 3), ("pair", 8, 2)])
+                                                ("pair", 2, 8), ("pair", 4, 3), ("pair", 8, 2),
+                                                ("pair-v", 1, 8), ("pair-v", 8, 2)])
 def test_multi_step_ranges_match_oracle(vc, oracle, kernel, tile, ranges):
     """Few vector ranges (forced), so every workgroup scans many steps and runs
     many rounds of candidate admission and list merges, over several query
-    tiles: the 100k x 128 4-bit workload against the oracle."""
+    tiles: the 100k x 128 4-bit workload against the oracle. "pair-v" is the
+    byte-pair kernel's other vectors-per-thread variant."""
     from vectorcache_amd import synthetic
 
     x = synthetic.flat100k_db()
@@ -195,7 +197,7 @@ def test_multi_step_ranges_match_oracle(vc, oracle, kernel, tile, ranges):
     ix.add(x)
     ix.prepare()
     vc.set_scan_tile(tile)
-    vc.set_debug((2 if kernel == "nibble" else 0) | (ranges << 16))
+    vc.set_debug({"nibble": 2, "pair": 0, "pair-v": 8}[kernel] | (ranges << 16))
     try:
         for k in (10, 100):
             s_o, i_o = _oracle_search(oracle, x, q, 4, k)

@@ -10,6 +10,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <utility>
 
 #include "vc_internal.hpp"
 
@@ -564,8 +565,14 @@ __global__ __launch_bounds__(64 * kRowMaxWaves) void encode_row_kernel(CodeTable
   __syncthreads();
 
   const int cpb = t.bits == 2 ? 4 : 2;  // codes per byte
-  const long stride = (long)gridDim.x * waves;
-  for (long row = (long)blockIdx.x * waves + wave; row < n; row += stride) {
+  // A workgroup encodes whole 64-row output tiles, so every byte of a tile's
+  // 1 KiB code lines is written through one L2 (one XCD) before eviction.
+  const long tile0 = out_offset >> 6;
+  const long ntile = ((out_offset + n + 63) >> 6) - tile0;
+  for (long ti = blockIdx.x; ti < ntile; ti += gridDim.x)
+  for (int tr = wave; tr < 64; tr += waves) {
+    const long row = ((tile0 + ti) << 6) + tr - out_offset;
+    if (row < 0 || row >= n) continue;  // wave-uniform
     {
       // the row, lane + 64 * i (clamped loads: always in bounds), into X
       const float* xr = x + row * (long)D + lane;
@@ -761,13 +768,13 @@ __global__ __launch_bounds__(256) void radix_select_kernel(const uint32_t* __res
 // Query preparation (src/search/search.cpp:149-219,762-860): one wavefront
 // per query.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float lut_value(const float* __restrict__ q, const CodeTables& t, int base,
-                                           int cpn, unsigned nib) {
-  const unsigned mask = (1u << t.bits) - 1u;
+__device__ __forceinline__ float lut_value(const float* __restrict__ q, const float* __restrict__ cent, int bits,
+                                           int base, int cpn, unsigned nib) {
+  const unsigned mask = (1u << bits) - 1u;
   float s = 0.f;
   for (int c = 0; c < cpn; ++c) {
-    const unsigned code = (nib >> ((cpn - 1 - c) * t.bits)) & mask;
-    s = __fmaf_rn(q[base + c], t.centroids[code], s);
+    const unsigned code = (nib >> ((cpn - 1 - c) * bits)) & mask;
+    s = __fmaf_rn(q[base + c], cent[code], s);
   }
   return s;
 }
@@ -780,6 +787,24 @@ __device__ __forceinline__ float lut_value(const float* __restrict__ q, const Co
 // other step is spread over the workgroup.
 constexpr int kPrepThreads = 256;
 
+// dst[j] = src[j] for j < n over the workgroup, 8 loads in flight per thread
+__device__ __forceinline__ void stage_to_lds(float* __restrict__ dst, const float* __restrict__ src, int n) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int base = 0; base < n; base += 8 * nt) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = base + u * nt + tid;
+      v[u] = j < n ? src[j] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = base + u * nt + tid;
+      if (j < n) dst[j] = v[u];
+    }
+  }
+}
+
 __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTables t, const float* __restrict__ q,
                                                                       long nq, uint8_t* __restrict__ luts,
                                                                       float* __restrict__ lut_scale,
@@ -790,22 +815,22 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTable
   const int D = t.dim, G = t.groups, B = t.block;
   const bool tq = t.shift != nullptr;
   float* y = reinterpret_cast<float*>(smem);   // D: rotated query
-  float* xs = y + D;                            // D: the staged query row
+  float* xs = y + D;                            // D: the staged query row, then y before the TQ+ division
   float* mins = xs + D;                         // 2 * G
   float* tsh = mins + 2 * G;                    // D (TQ+)
   float* tsc = tsh + D;                         // D (TQ+)
-  __shared__ float s_norm, s_bias, s_bc, s_span[kPrepThreads / 64];
+  __shared__ float s_norm, s_bsum, s_bc, s_span[kPrepThreads / 64], s_cent[16];
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
   const int cpb = 8 / t.bits, cpn = cpb / 2;
-  if (tq)
-    for (int j = tid; j < D; j += nt) {
-      tsh[j] = t.shift[j];
-      tsc[j] = t.scale[j];
-    }
+  if (tid < (1 << t.bits)) s_cent[tid] = t.centroids[tid];
+  if (tq) {
+    stage_to_lds(tsh, t.shift, D);
+    stage_to_lds(tsc, t.scale, D);
+  }
   for (long qi = blockIdx.x; qi < nq; qi += gridDim.x) {
     const float* xr = q + qi * (long)D;
     __syncthreads();
-    for (int j = tid; j < D; j += nt) xs[j] = xr[j];
+    stage_to_lds(xs, xr, D);
     __syncthreads();
     if (wave == 0) {
       const float nrm = norm8_lds(xs, D, lane);
@@ -815,9 +840,18 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTable
     const float nrm = s_norm;
     const float inv = nrm > kMinNorm ? div_rn(1.f, nrm) : 0.f;
     // y[i] = (x[perm[i]] * inv) * sign[i], scattered through inv_perm
-    for (int j = tid; j < D; j += nt) {
-      const uint32_t w = t.perm_sign[j];
-      y[w & 0x7fffffffu] = (xs[j] * inv) * ((w >> 31) ? -1.f : 1.f);
+    for (int base = 0; base < D; base += 8 * nt) {
+      uint32_t w[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = base + u * nt + tid;
+        w[u] = j < D ? t.perm_sign[j] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = base + u * nt + tid;
+        if (j < D) y[w[u] & 0x7fffffffu] = (xs[j] * inv) * ((w[u] >> 31) ? -1.f : 1.f);
+      }
     }
     __syncthreads();
     const int npairs = D >> 1, lhb = __ffs(B) - 2;  // log2(B / 2)
@@ -832,19 +866,14 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTable
       }
       __syncthreads();
     }
-    for (int i = tid; i < D; i += nt) y[i] = y[i] * t.inv_sqrt_block;
-    __syncthreads();
-    if (tq) {
-      // bias_corr = (float) -sum (double) y_d * shift_d, sequential in d
-      if (tid == 0) {
-        double acc = 0.0;
-        for (int d = 0; d < D; ++d) acc = fma(-(double)y[d], (double)tsh[d], acc);
-        s_bc = (float)acc;
+    for (int i = tid; i < D; i += nt) {
+      const float r = y[i] * t.inv_sqrt_block;
+      if (tq) {
+        xs[i] = r;  // kept for bias_corr
+        y[i] = div_rn(r, tsc[i]);
+      } else {
+        y[i] = r;
       }
-      __syncthreads();
-      for (int d = tid; d < D; d += nt) y[d] = div_rn(y[d], tsc[d]);
-    } else if (tid == 0) {
-      s_bc = 0.f;
     }
     __syncthreads();
     // pass 1: per group the min/max of both 16-entry halves; global span
@@ -853,7 +882,7 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTable
       for (int half = 0; half < 2; ++half) {
         float mn = FLT_MAX, mx = -FLT_MAX;
         for (unsigned nib = 0; nib < 16; ++nib) {
-          const float sv = lut_value(y, t, g * cpb + half * cpn, cpn, nib);
+          const float sv = lut_value(y, s_cent, t.bits, g * cpb + half * cpn, cpn, nib);
           mn = sv < mn ? sv : mn;
           mx = sv > mx ? sv : mx;
         }
@@ -872,35 +901,67 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTable
       iscale = div_rn(1.f, scale);
     else
       scale = 1.f;
-    // pass 2: quantized entries; the bias (sequential over groups, as the
-    // reference) runs on the last lane after its own groups
+    // pass 2 on waves 0-1: quantized entries. Meanwhile, with no barrier in
+    // between, wave 2 lane 0 runs the bias (sequential over groups, as the
+    // reference) and wave 3 lane 0 the TQ+ bias_corr chain (sequential in d).
     uint8_t* out = luts + qi * (long)G * 32;
-    for (int g = tid; g < G; g += nt) {
-      uint32_t w[8];
-      for (int half = 0; half < 2; ++half) {
-        const float mn = mins[2 * g + half];
-        for (int e = 0; e < 16; e += 4) {
-          uint32_t word = 0;
-          for (int b = 0; b < 4; ++b) {
-            const float sv = lut_value(y, t, g * cpb + half * cpn, cpn, (unsigned)(e + b));
-            float r = roundf((sv - mn) * iscale);
-            r = r < 0.f ? 0.f : (r > 127.f ? 127.f : r);
-            word |= uint32_t(r) << (8 * b);
+    if (wave < 2) {
+      for (int g = tid; g < G; g += 128) {
+        uint32_t w[8];
+        for (int half = 0; half < 2; ++half) {
+          const float mn = mins[2 * g + half];
+          for (int e = 0; e < 16; e += 4) {
+            uint32_t word = 0;
+            for (int b = 0; b < 4; ++b) {
+              const float sv = lut_value(y, s_cent, t.bits, g * cpb + half * cpn, cpn, (unsigned)(e + b));
+              float r = roundf((sv - mn) * iscale);
+              r = r < 0.f ? 0.f : (r > 127.f ? 127.f : r);
+              word |= uint32_t(r) << (8 * b);
+            }
+            w[half * 4 + e / 4] = word;
           }
-          w[half * 4 + e / 4] = word;
         }
+        uint4* dst = reinterpret_cast<uint4*>(out + (long)g * 32);
+        dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+        dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
       }
-      uint4* dst = reinterpret_cast<uint4*>(out + (long)g * 32);
-      dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
-      dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
-    }
-    if (tid == nt - 1) {
+    } else if (wave == 2 && lane == 0) {
       float bsum = 0.f;
-      for (int g = 0; g < G; ++g) bsum = bsum + (mins[2 * g] + mins[2 * g + 1]);
+      int g = 0;
+      for (; g + 8 <= G; g += 8) {
+        float m[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) m[u] = mins[2 * g + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) bsum = bsum + (m[2 * u] + m[2 * u + 1]);
+      }
+      for (; g < G; ++g) bsum = bsum + (mins[2 * g] + mins[2 * g + 1]);
       lut_scale[qi] = scale;
       lut_bias[qi] = bsum;
+      s_bsum = bsum;
+    } else if (wave == 3 && lane == 0) {
+      // bias_corr = (float) -sum (double) y_d * shift_d, sequential in d
+      double acc = 0.0;
+      if (tq) {
+        int d = 0;
+        for (; d + 8 <= D; d += 8) {
+          float yv[8], sv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            yv[u] = xs[d + u];
+            sv[u] = tsh[d + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc = fma(-(double)yv[u], (double)sv[u], acc);
+        }
+        for (; d < D; ++d) acc = fma(-(double)xs[d], (double)tsh[d], acc);
+      }
+      s_bc = (float)acc;
+    }
+    __syncthreads();
+    if (tid == 0) {
       bias_corr[qi] = s_bc;
-      bias_total[qi] = bsum + s_bc;
+      bias_total[qi] = s_bsum + s_bc;
     }
   }
 }
@@ -1331,7 +1392,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_topk_kernel(
 // bytes into u16 lanes after the read), u16 per query for T = 2 and 4. The
 // table lives in a 64 KiB LDS chunk of CG groups, rebuilt per chunk from an
 // 8 KiB slice of the query tile's source rows ([group][32][T] of u8 or u16),
-// which is staged through LDS one chunk ahead.
+// which is copied into an LDS stage by DMA one chunk ahead.
 // ---------------------------------------------------------------------------
 constexpr int kPairTableBytes = 65536;
 constexpr int kPairStageBytes = 8192;
@@ -1412,24 +1473,174 @@ __device__ __forceinline__ void build_pair_chunk(const uint8_t* __restrict__ sta
   }
 }
 
+// f(integral_constant<int, I>) for I = 0 .. N-1, unrolled at compile time.
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Byte e of w times 2^SH in one VALU op: the SDWA operand select picks the
+// byte, so the LDS address of a table entry costs one instruction.
+template <int E, int SH>
+__device__ __forceinline__ uint32_t byte_scaled(uint32_t w) {
+  static_assert(E >= 0 && E < 4 && SH >= 0 && SH < 8, "byte and shift");
+  uint32_t r;
+  if constexpr (SH == 0) {
+    r = __builtin_amdgcn_ubfe(w, 8 * E, 8);
+  } else if constexpr (E == 0) {
+    asm("v_lshlrev_b32_sdwa %0, %2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0"
+        : "=v"(r) : "v"(w), "i"(SH));
+  } else if constexpr (E == 1) {
+    asm("v_lshlrev_b32_sdwa %0, %2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_1"
+        : "=v"(r) : "v"(w), "i"(SH));
+  } else if constexpr (E == 2) {
+    asm("v_lshlrev_b32_sdwa %0, %2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_2"
+        : "=v"(r) : "v"(w), "i"(SH));
+  } else {
+    asm("v_lshlrev_b32_sdwa %0, %2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_3"
+        : "=v"(r) : "v"(w), "i"(SH));
+  }
+  return r;
+}
+
+// 32-bit accumulators per vector (u16 lanes; narrow T = 8 keeps a second set)
+template <int T> constexpr int pair_acc_words() { return T == 1 ? 1 : (PairFmt<T>::wide ? T / 2 : T / 4); }
+
+// One table chunk of the scan: CB code blocks of each of the thread's V
+// vectors against the chunk's table, window flushes into the float chains,
+// code-ring reloads; first, when dma_src is set, the DMA of the next chunk's
+// source rows into the stage (global_load_lds: wavefront w copies bytes
+// [1024 w, 1024 (w + 1)), 16 per lane, with no registers in flight).
+// The DMA is complete at the barrier before the next build. (The compiler
+// still retires it, with every load in flight, at the first wait for a code
+// block after it: measured faster than staging the rows through registers,
+// which costs the two registers that push the two-vector variant into spills.)
+template <int T, int V, int NR>
+__device__ __forceinline__ void pair_chunk(const uint8_t* __restrict__ table, uint8_t* __restrict__ stage,
+                                           const uint8_t* __restrict__ dma_src, const float* __restrict__ scl,
+                                           const uint4* const (&cp)[V], int c, int G16, uint4 (&ring)[NR][V],
+                                           uint32_t (&acc)[V][pair_acc_words<T>()],
+                                           uint32_t (&acc2)[V][pair_acc_words<T>()], float (&fa)[V][T]) {
+  constexpr bool kWide = PairFmt<T>::wide;
+  constexpr int EB = pair_entry_bytes<T>();
+  constexpr int CB = kPairTableBytes / (256 * EB) / 16;  // 16-group code blocks per chunk
+  constexpr int NA = pair_acc_words<T>();
+  constexpr int SH = EB == 8 ? 3 : EB == 4 ? 2 : 0;  // log2 of the entry size
+  const int tid = threadIdx.x;
+  if (dma_src != nullptr) {
+    const int w = tid >> 6;
+    if (w < kPairStageBytes / 1024)
+      __builtin_amdgcn_global_load_lds(dma_src + w * 1024 + (tid & 63) * 16, stage + w * 1024, 16, 0, 0);
+  }
+  auto load_block = [&](int j, int b) -> uint4 {
+    // unconditional, clamped (in-bounds) load: no branch and no select around
+    // it, so it stays in flight until its block is consumed. Blocks past G16
+    // are never consumed; lanes of invalid vectors (pointing at vector 0)
+    // get -inf scores.
+    return cp[j][(long)min(b, G16 - 1) * 64];
+  };
+#pragma unroll
+  for (int u = 0; u < CB; ++u) {
+    const int bb = c * CB + u;
+    const bool live = bb < G16;  // uniform
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      if (live) {
+        const uint8_t* tb = table + u * 16 * 256 * EB;
+        const uint32_t words[4] = {ring[u % NR][j].x, ring[u % NR][j].y, ring[u % NR][j].z, ring[u % NR][j].w};
+        static_for<16>([&](auto ec) {
+          constexpr int e = decltype(ec)::value;
+          const uint8_t* p = tb + e * 256 * EB + byte_scaled<e & 3, SH>(words[e >> 2]);
+          if constexpr (T == 1) {
+            acc[j][0] += *p;
+          } else if constexpr (kWide) {
+            // the whole entry in one read (its offset folds into the instruction)
+            uint32_t ent[NA];
+            if constexpr (NA == 2) {
+              const uint2 w2 = *reinterpret_cast<const uint2*>(p);
+              ent[0] = w2.x;
+              ent[1] = w2.y;
+            } else {
+              ent[0] = *reinterpret_cast<const uint32_t*>(p);
+            }
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+              acc[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc[j][a]) +
+                                                           __builtin_bit_cast(u16x2, ent[a]));
+          } else {
+            const uint2 w2 = *reinterpret_cast<const uint2*>(p);
+            const uint32_t ws[2] = {w2.x, w2.y};
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+              // bytes (0, 2) and (1, 3) of the entry word as u16 lanes (v_perm selectors; 0x0c = zero)
+              const uint32_t ev = __builtin_amdgcn_perm(0u, ws[a], 0x0c020c00u);
+              const uint32_t od = __builtin_amdgcn_perm(0u, ws[a], 0x0c030c01u);
+              acc[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc[j][a]) + __builtin_bit_cast(u16x2, ev));
+              acc2[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc2[j][a]) + __builtin_bit_cast(u16x2, od));
+            }
+          }
+        });
+      }
+      // the slot is reloaded only after its block is consumed: no second copy
+      // of the block stays live (issued even past G16: clamped, unused)
+      ring[u % NR][j] = load_block(j, bb + NR);
+    }
+    if (live && (((bb + 1) & 15) == 0 || bb == G16 - 1)) {
+      // window flush: exact integer sums per query -> the reference's float chain
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        if constexpr (T == 1) {
+          fa[j][0] = __fmaf_rn(scl[0], (float)acc[j][0], fa[j][0]);
+        } else if constexpr (kWide) {
+#pragma unroll
+          for (int a = 0; a < NA; ++a) {
+            const u16x2 sm = __builtin_bit_cast(u16x2, acc[j][a]);
+            fa[j][2 * a] = __fmaf_rn(scl[2 * a], (float)sm.x, fa[j][2 * a]);
+            fa[j][2 * a + 1] = __fmaf_rn(scl[2 * a + 1], (float)sm.y, fa[j][2 * a + 1]);
+          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < NA; ++a) {
+            const u16x2 ev = __builtin_bit_cast(u16x2, acc[j][a]);   // queries 4a, 4a + 2
+            const u16x2 od = __builtin_bit_cast(u16x2, acc2[j][a]);  // queries 4a + 1, 4a + 3
+            fa[j][4 * a + 0] = __fmaf_rn(scl[4 * a + 0], (float)ev.x, fa[j][4 * a + 0]);
+            fa[j][4 * a + 1] = __fmaf_rn(scl[4 * a + 1], (float)od.x, fa[j][4 * a + 1]);
+            fa[j][4 * a + 2] = __fmaf_rn(scl[4 * a + 2], (float)ev.y, fa[j][4 * a + 2]);
+            fa[j][4 * a + 3] = __fmaf_rn(scl[4 * a + 3], (float)od.y, fa[j][4 * a + 3]);
+          }
+        }
+#pragma unroll
+        for (int a = 0; a < NA; ++a) { acc[j][a] = 0u; acc2[j][a] = 0u; }
+      }
+    }
+  }
+}
+
+#ifndef VC_PAIR_MIN_WAVES
+#define VC_PAIR_MIN_WAVES 8  // waves per SIMD the register budget must allow
+#endif
 template <int T, int V>
-__global__ __launch_bounds__(kScanThreads, 8) void scan_pair_kernel(
+__global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void scan_pair_kernel(
     const uint8_t* __restrict__ tiled, const float* __restrict__ alpha, long n, int G16,
     const uint8_t* __restrict__ src, const float* __restrict__ lut_scale,
     const float* __restrict__ bias_total, long nq, int k, int cap, int n_ranges, long range_len,
     float* __restrict__ part_s, uint32_t* __restrict__ part_i, int* __restrict__ part_cnt) {
-  constexpr bool kWide = PairFmt<T>::wide;
   constexpr int EB = pair_entry_bytes<T>();
   constexpr int CG = kPairTableBytes / (256 * EB);  // groups per table chunk
   constexpr int CB = CG / 16;                       // 16-group code blocks per chunk
-  static_assert(CG * 32 * EB == kPairStageBytes && kPairStageBytes == kScanThreads * 8, "stage = one u64 per thread");
-  constexpr int NR = (T == 1 && V == 1) ? 4 : 2;   // code blocks in flight per vector
+  static_assert(CG * 32 * EB == kPairStageBytes && kPairStageBytes == kScanThreads * 8, "stage = 8 bytes per thread");
+  constexpr int NR = (T == 1 && V == 1) ? 4 : 2;  // code blocks in flight per vector
   static_assert(CB % NR == 0, "a chunk holds whole rounds of the code-block ring");
-  constexpr int NA = T == 1 ? 1 : (kWide ? T / 2 : T / 4);  // 32-bit accumulators per vector (x2 when narrow T=8)
+  constexpr int NA = pair_acc_words<T>();
 
+  // the table at a fixed LDS address, so entry offsets fold into the reads
+  __shared__ __align__(16) uint8_t table[kPairTableBytes];
+  __shared__ __align__(16) uint8_t stage[kPairStageBytes];
   extern __shared__ __align__(16) unsigned char smem[];
-  uint8_t* table = smem;
-  uint8_t* stage = smem + kPairTableBytes;
   const int tid = threadIdx.x;
   const long n_qt = (nq + T - 1) / T;
   const long L = blockIdx.x;
@@ -1440,25 +1651,23 @@ __global__ __launch_bounds__(kScanThreads, 8) void scan_pair_kernel(
   const long r_end = min(n, r_begin + range_len);
 
   TopkShared sh;
-  init_topk(sh, smem + kPairTableBytes + kPairStageBytes, T, k, cap);
+  init_topk(sh, smem, T, k, cap);
   // per-query LUT scale and bias in LDS (read at window flushes and step
   // starts only; keeps them out of the scalar register budget)
-  float* scl = reinterpret_cast<float*>(smem + kPairTableBytes + kPairStageBytes +
-                                        (size_t)T * (4 * (size_t)k + 2 * (size_t)cap) * 4 + 16 * T);
+  float* scl = reinterpret_cast<float*>(smem + (size_t)T * (4 * (size_t)k + 2 * (size_t)cap) * 4 + 16 * T);
   float* bias = scl + T;
   if (tid < T) {
     const long q = q0 + tid < nq ? q0 + tid : nq - 1;
     scl[tid] = lut_scale[q];
     bias[tid] = bias_total[q];
   }
-  __syncthreads();
 
-  // the source rows are padded to whole chunks, so every load below is in bounds
+  // the source rows are padded to whole chunks, so every DMA is in bounds;
+  // chunk 0's rows are staged by plain loads before the first build
   const int n_chunks = (G16 + CB - 1) / CB;
-  const uint64_t* tile_src = reinterpret_cast<const uint64_t*>(src + qt * ((long)n_chunks * CB * 16 * 32 * EB));
-  auto src_word = [&](int c) -> uint64_t { return tile_src[(long)c * kScanThreads + tid]; };
-  reinterpret_cast<uint64_t*>(stage)[tid] = src_word(0);
-  uint64_t sw = src_word(n_chunks > 1 ? 1 : 0);
+  const uint8_t* tile_src = src + qt * ((long)n_chunks * kPairStageBytes);
+  reinterpret_cast<uint64_t*>(stage)[tid] = reinterpret_cast<const uint64_t*>(tile_src)[tid];
+  __syncthreads();
   bool built = false;
 
   const long step = (long)kScanThreads * V;
@@ -1479,13 +1688,6 @@ __global__ __launch_bounds__(kScanThreads, 8) void scan_pair_kernel(
 #pragma unroll
       for (int a = 0; a < NA; ++a) { acc[j][a] = 0u; acc2[j][a] = 0u; }
     }
-    auto load_block = [&](int j, int b) -> uint4 {
-      // unconditional, clamped (in-bounds) load: no branch and no select around
-      // it, so it stays in flight until its block is consumed. Blocks past G16
-      // are never consumed; lanes of invalid vectors (pointing at vector 0)
-      // get -inf scores.
-      return cp[j][(long)min(b, G16 - 1) * 64];
-    };
     // Code blocks NR ahead. The block loop of a chunk is unrolled and CB is a
     // multiple of NR, so each block always uses the same ring slot: no
     // register rotation, and each wait covers exactly the block consumed.
@@ -1493,88 +1695,21 @@ __global__ __launch_bounds__(kScanThreads, 8) void scan_pair_kernel(
 #pragma unroll
     for (int u = 0; u < NR; ++u)
 #pragma unroll
-      for (int j = 0; j < V; ++j) ring[u][j] = load_block(j, u);
+      for (int j = 0; j < V; ++j) ring[u][j] = cp[j][(long)min(u, G16 - 1) * 64];
 
     for (int c = 0; c < n_chunks; ++c) {
+      const uint8_t* dma = nullptr;
       if (n_chunks > 1 || !built) {
-        __syncthreads();  // table and stage free (previous chunk's lookups done, stage written)
+        __syncthreads();  // table free (previous chunk's lookups done); stage DMA complete
         build_pair_chunk<T>(stage, table);
         __syncthreads();  // table ready; stage consumed
-        if (n_chunks > 1) {
-          reinterpret_cast<uint64_t*>(stage)[tid] = sw;  // rows of chunk c + 1
-          sw = src_word(c + 2 < n_chunks ? c + 2 : (c + 2) - n_chunks);
-        }
+        // rows of the next chunk (of the next step after the last one); none
+        // after the last chunk of the last step
+        if (n_chunks > 1 && (c + 1 < n_chunks || base + step < r_end))
+          dma = tile_src + (long)(c + 1 < n_chunks ? c + 1 : 0) * kPairStageBytes;
         built = true;
       }
-#pragma unroll
-      for (int u = 0; u < CB; ++u) {
-        const int bb = c * CB + u;
-        uint4 cw[V];
-#pragma unroll
-        for (int j = 0; j < V; ++j) {
-          cw[j] = ring[u % NR][j];
-          ring[u % NR][j] = load_block(j, bb + NR);  // issued even past G16 (clamped, unused)
-        }
-        if (bb < G16) {
-          const uint8_t* tb = table + u * 16 * 256 * EB;
-#pragma unroll
-          for (int j = 0; j < V; ++j) {
-            const uint32_t words[4] = {cw[j].x, cw[j].y, cw[j].z, cw[j].w};
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-              const uint32_t byte = __builtin_amdgcn_ubfe(words[e >> 2], 8 * (e & 3), 8);
-              const uint8_t* p = tb + e * 256 * EB + byte * EB;
-              if constexpr (T == 1) {
-                acc[j][0] += *p;
-              } else if constexpr (kWide) {
-#pragma unroll
-                for (int a = 0; a < NA; ++a)
-                  acc[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc[j][a]) +
-                                                               reinterpret_cast<const u16x2*>(p)[a]);
-              } else {
-                const uint2 w2 = *reinterpret_cast<const uint2*>(p);
-                const uint32_t ws[2] = {w2.x, w2.y};
-#pragma unroll
-                for (int a = 0; a < NA; ++a) {
-                  // bytes (0, 2) and (1, 3) of the entry word as u16 lanes (v_perm selectors; 0x0c = zero)
-                  const uint32_t ev = __builtin_amdgcn_perm(0u, ws[a], 0x0c020c00u);
-                  const uint32_t od = __builtin_amdgcn_perm(0u, ws[a], 0x0c030c01u);
-                  acc[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc[j][a]) + __builtin_bit_cast(u16x2, ev));
-                  acc2[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc2[j][a]) + __builtin_bit_cast(u16x2, od));
-                }
-              }
-            }
-          }
-          if (((bb + 1) & 15) == 0 || bb == G16 - 1) {
-            // window flush: exact integer sums per query -> the reference's float chain
-#pragma unroll
-            for (int j = 0; j < V; ++j) {
-              if constexpr (T == 1) {
-                fa[j][0] = __fmaf_rn(scl[0], (float)acc[j][0], fa[j][0]);
-              } else if constexpr (kWide) {
-#pragma unroll
-                for (int a = 0; a < NA; ++a) {
-                  const u16x2 sm = __builtin_bit_cast(u16x2, acc[j][a]);
-                  fa[j][2 * a] = __fmaf_rn(scl[2 * a], (float)sm.x, fa[j][2 * a]);
-                  fa[j][2 * a + 1] = __fmaf_rn(scl[2 * a + 1], (float)sm.y, fa[j][2 * a + 1]);
-                }
-              } else {
-#pragma unroll
-                for (int a = 0; a < NA; ++a) {
-                  const u16x2 ev = __builtin_bit_cast(u16x2, acc[j][a]);   // queries 4a, 4a + 2
-                  const u16x2 od = __builtin_bit_cast(u16x2, acc2[j][a]);  // queries 4a + 1, 4a + 3
-                  fa[j][4 * a + 0] = __fmaf_rn(scl[4 * a + 0], (float)ev.x, fa[j][4 * a + 0]);
-                  fa[j][4 * a + 1] = __fmaf_rn(scl[4 * a + 1], (float)od.x, fa[j][4 * a + 1]);
-                  fa[j][4 * a + 2] = __fmaf_rn(scl[4 * a + 2], (float)ev.y, fa[j][4 * a + 2]);
-                  fa[j][4 * a + 3] = __fmaf_rn(scl[4 * a + 3], (float)od.y, fa[j][4 * a + 3]);
-                }
-              }
-#pragma unroll
-              for (int a = 0; a < NA; ++a) { acc[j][a] = 0u; acc2[j][a] = 0u; }
-            }
-          }
-        }
-      }
+      pair_chunk<T, V, NR>(table, stage, dma, scl, cp, c, G16, ring, acc, acc2, fa);
     }
     admit_scores<T, V>(sh, fa, vv, valid, alpha, q0, nq, k, cap);
   }
@@ -1679,11 +1814,9 @@ __global__ __launch_bounds__(256) void merge_ranges_lds_kernel(const float* __re
   int* lc = reinterpret_cast<int*>(li + n);
   for (long q = blockIdx.x; q < nq; q += gridDim.x) {
     __syncthreads();
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-      ls[e] = part_s[q * n + e];
-      li[e] = part_i[q * n + e];
-    }
-    for (int r = threadIdx.x; r < R; r += blockDim.x) lc[r] = part_cnt[q * R + r];
+    stage_to_lds(ls, part_s + q * n, n);
+    stage_to_lds(reinterpret_cast<float*>(li), reinterpret_cast<const float*>(part_i + q * n), n);
+    stage_to_lds(reinterpret_cast<float*>(lc), reinterpret_cast<const float*>(part_cnt + q * R), R);
     __syncthreads();
     if (threadIdx.x < 64)
       merge_query(ls, li, lc, R, k, k_out, id_map, out_s + q * k_out, out_i + q * k_out, threadIdx.x);
@@ -1893,7 +2026,8 @@ hipError_t launch_encode(const CodeTables& t, const float* x, std::size_t n, std
     hipLaunchKernelGGL(norm_rows_kernel, dim3(grid_for((long)(n + 31) / 32, 1, 16384)), dim3(256), 0, s, x, (long)n,
                        t.dim, alpha + out_offset);
     const size_t lds = shared + (size_t)waves * per_wave;
-    const int grid = grid_for((long)(n + waves - 1) / waves, 1, 4096);
+    const long ntile = (long)((out_offset + n + 63) / 64 - out_offset / 64);
+    const int grid = grid_for(ntile, 1, 4096);
 #define VC_ROW(RR, NN)                                                                                        \
   do {                                                                                                        \
     (void)hipFuncSetAttribute((const void*)encode_row_kernel<RR, NN>, hipFuncAttributeMaxDynamicSharedMemorySize, \
@@ -2094,11 +2228,12 @@ hipError_t launch_scan(const ScanPlan& p, const std::uint8_t* tiled, const float
     hipLaunchKernelGGL(pair_source_kernel<TT>, dim3(grid_for(pair_elems, 256)), dim3(256), 0, s, luts,    \
                        (long)nq, groups, groups16, lut_t);                                                \
     (void)hipFuncSetAttribute((const void*)scan_pair_kernel<TT, VV>,                                       \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds_bytes);              \
-    hipLaunchKernelGGL((scan_pair_kernel<TT, VV>), grid, block, p.lds_bytes, s, tiled, alpha, (long)n,      \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)pair_dyn);                 \
+    hipLaunchKernelGGL((scan_pair_kernel<TT, VV>), grid, block, pair_dyn, s, tiled, alpha, (long)n,         \
                        groups16, lut_t, lut_scale, bias_total, (long)nq, k, p.cand_cap, p.n_ranges,       \
                        (long)p.range_len, part_s, part_i, part_cnt);                                      \
   } while (0)
+    const size_t pair_dyn = p.lds_bytes - kPairTableBytes - kPairStageBytes;  // table and stage are static LDS
     const bool v2 = p.vec_per_thread == 2;
     switch (p.tile) {
       case 8: if (v2) VC_LAUNCH_PAIR(8, 2); else VC_LAUNCH_PAIR(8, 1); break;

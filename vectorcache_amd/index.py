@@ -291,6 +291,7 @@ def set_debug(flags: int):
     """Test hooks: bit 0 forces the exact sequential alpha path in encode;
     bit 2 selects the tile encode kernel for indexes created afterwards;
     bit 1 selects the nibble-table scan kernel instead of the byte-pair one;
+    bit 3 swaps the byte-pair kernel's vectors-per-thread variant (1 <-> 2);
     bits 16-31 (when not zero) force the number of vector ranges per query
     tile (at most 512)."""
     check(lib.vc_set_debug(ctypes.c_uint(flags)))
