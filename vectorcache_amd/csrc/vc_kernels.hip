@@ -1572,14 +1572,17 @@ __device__ __forceinline__ void pair_chunk(const uint8_t* __restrict__ table, ui
               acc[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc[j][a]) +
                                                            __builtin_bit_cast(u16x2, ent[a]));
           } else {
+            // Entry word a = bytes of queries 4a .. 4a+3. acc adds the word
+            // as it is, as u16 lanes: (q0 + 256 q1, q2 + 256 q3) mod 2^16;
+            // acc2 adds the odd bytes alone: (q1, q3) (v_perm; 0x0c = zero).
+            // Every window sum is < 2^16, so the flush recovers the even
+            // queries exactly as acc - 256 acc2 (mod 2^16).
             const uint2 w2 = *reinterpret_cast<const uint2*>(p);
             const uint32_t ws[2] = {w2.x, w2.y};
 #pragma unroll
             for (int a = 0; a < NA; ++a) {
-              // bytes (0, 2) and (1, 3) of the entry word as u16 lanes (v_perm selectors; 0x0c = zero)
-              const uint32_t ev = __builtin_amdgcn_perm(0u, ws[a], 0x0c020c00u);
               const uint32_t od = __builtin_amdgcn_perm(0u, ws[a], 0x0c030c01u);
-              acc[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc[j][a]) + __builtin_bit_cast(u16x2, ev));
+              acc[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc[j][a]) + __builtin_bit_cast(u16x2, ws[a]));
               acc2[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc2[j][a]) + __builtin_bit_cast(u16x2, od));
             }
           }
@@ -1605,8 +1608,8 @@ __device__ __forceinline__ void pair_chunk(const uint8_t* __restrict__ table, ui
         } else {
 #pragma unroll
           for (int a = 0; a < NA; ++a) {
-            const u16x2 ev = __builtin_bit_cast(u16x2, acc[j][a]);   // queries 4a, 4a + 2
             const u16x2 od = __builtin_bit_cast(u16x2, acc2[j][a]);  // queries 4a + 1, 4a + 3
+            const u16x2 ev = __builtin_bit_cast(u16x2, acc[j][a]) - (od << (unsigned short)8);  // queries 4a, 4a + 2
             fa[j][4 * a + 0] = __fmaf_rn(scl[4 * a + 0], (float)ev.x, fa[j][4 * a + 0]);
             fa[j][4 * a + 1] = __fmaf_rn(scl[4 * a + 1], (float)od.x, fa[j][4 * a + 1]);
             fa[j][4 * a + 2] = __fmaf_rn(scl[4 * a + 2], (float)ev.y, fa[j][4 * a + 2]);
