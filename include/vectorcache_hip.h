@@ -148,6 +148,31 @@ vc_status vc_search_perm0(const float* queries, size_t nq, size_t dim, size_t bi
                           const uint64_t* id_map, const float* shift, const float* scale,
                           float* scores, uint64_t* ids, size_t* k_out);
 
+/* score_prepared_into (src/search/search.cpp:1156-1267, include/vectorcache/
+ * search/search.hpp:163-177) over one bucket given as PERM0 blocks + alpha:
+ * the nq queries (host, nq x dim) are the prepared batch; only the queries in
+ * query_indices (non-empty, each < nq, no repeats) are scored, each into its
+ * persistent top-k heap: heap_s / heap_i [nq][k] and heap_sz / heap_min /
+ * heap_mi [nq], the reference's arrays, updated in place exactly as the
+ * reference's AVX2 PERM0 path updates them (strict admission above heap_min,
+ * the k < 128 rescan heap and the k >= 128 binary heap; ids = id_map[v], or v
+ * when id_map is NULL; score_offsets[qi], when not NULL, added after
+ * alpha x LUT). n == 0 or k == 0 changes nothing. Not supported: the
+ * buffered collector (heap_mi == SIZE_MAX) and k > 4096 (VC_ERR_INVALID_ARGUMENT).
+ * Empty query_indices -> VC_ERR_INVALID_ARGUMENT, an index >= nq ->
+ * VC_ERR_OUT_OF_RANGE, as the reference throws. */
+vc_status vc_score_perm0_into(const float* queries, size_t nq, size_t dim, size_t bits, size_t k,
+                              const uint8_t* perm0, const float* alpha, size_t n, const uint64_t* id_map,
+                              const float* shift, const float* scale, const size_t* query_indices,
+                              size_t n_indices, float* heap_s, uint64_t* heap_i, size_t* heap_sz,
+                              float* heap_min, size_t* heap_mi, const float* score_offsets);
+
+/* heaps_to_search_results (src/search/search.cpp:1269-1291): each of the nq
+ * heaps' heap_sz entries by (score desc, id asc) into scores / ids [nq][k],
+ * the rest of each row (0, 0). */
+vc_status vc_heaps_to_results(size_t nq, size_t k, const float* heap_s, const uint64_t* heap_i,
+                              const size_t* heap_sz, float* scores, uint64_t* ids);
+
 /* merge_search_results (src/search/search.cpp:1551-1600) on the GPU:
  * dst (nq x k_dst) <- top k_dst of dst U src by (score desc, id asc), a
  * duplicated id keeping its higher score, short rows padded with (0, 0). */

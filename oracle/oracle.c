@@ -820,3 +820,172 @@ int oc_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t* dst_i
   free(c);
   return 0;
 }
+
+/* ---- persistent per-query heaps ------------------------------------------
+ * topk_worse / rescan_min / topk_sift_down / topk_heapify /
+ * heap_push_or_replace (reference include/vectorcache/search/topk_heap.hpp:
+ * 19-66,113-157): the k < 128 "rescan" heap and the k >= 128 binary heap.
+ * Admission is strict (score > heap_min); while the heap fills, heap_min
+ * keeps the caller's value. The buffered collector (heap_mi == SIZE_MAX,
+ * :96-111) keeps its best k/2 through std::nth_element, whose order is the
+ * standard library's; it is not restated (OC_ERR_UNSUPPORTED). */
+#define OC_ERR_UNSUPPORTED (-3)
+
+static int oc_worse(float sa, uint64_t ia, float sb, uint64_t ib) {
+  if (sa != sb) return sa < sb;
+  return ia > ib;
+}
+
+static void oc_rescan_min(const float* hs, const uint64_t* hi, size_t k, float* hmin, size_t* hmi) {
+  size_t mi = 0;
+  for (size_t h = 1; h < k; ++h)
+    if (oc_worse(hs[h], hi[h], hs[mi], hi[mi])) mi = h;
+  *hmin = hs[mi];
+  *hmi = mi;
+}
+
+static void oc_sift_down(float* hs, uint64_t* hi, size_t k, size_t i) {
+  for (;;) {
+    const size_t l = 2 * i + 1, r = 2 * i + 2;
+    size_t w = i;
+    if (l < k && oc_worse(hs[l], hi[l], hs[w], hi[w])) w = l;
+    if (r < k && oc_worse(hs[r], hi[r], hs[w], hi[w])) w = r;
+    if (w == i) return;
+    const float ts = hs[i];
+    hs[i] = hs[w];
+    hs[w] = ts;
+    const uint64_t ti = hi[i];
+    hi[i] = hi[w];
+    hi[w] = ti;
+    i = w;
+  }
+}
+
+static void oc_heap_push(float* hs, uint64_t* hi, size_t* hsz, float* hmin, size_t* hmi, size_t k,
+                         float s, uint64_t id) {
+  if (k < 128) {
+    if (*hsz < k) {
+      if (!(s > *hmin)) return;
+      hs[*hsz] = s;
+      hi[*hsz] = id;
+      ++*hsz;
+      if (*hsz == k) oc_rescan_min(hs, hi, k, hmin, hmi);
+    } else if (s > *hmin) {
+      hs[*hmi] = s;
+      hi[*hmi] = id;
+      oc_rescan_min(hs, hi, k, hmin, hmi);
+    }
+    return;
+  }
+  if (*hsz < k) {
+    if (!(s > *hmin)) return;
+    hs[*hsz] = s;
+    hi[*hsz] = id;
+    ++*hsz;
+    if (*hsz == k) {
+      for (size_t i = k / 2; i-- > 0;) oc_sift_down(hs, hi, k, i);
+      *hmin = hs[0];
+      *hmi = 0;
+    }
+  } else if (s > *hmin) {
+    hs[0] = s;
+    hi[0] = id;
+    oc_sift_down(hs, hi, k, 0);
+    *hmin = hs[0];
+    *hmi = 0;
+  }
+}
+
+/* score_prepared_into (reference src/search/search.cpp:1156-1267) on the AVX2
+ * PERM0 path (src/search/search_avx2.cpp:72-225): for each listed query, the
+ * bucket's vectors in order, by 32-vector sub-tiles (kFlushChunk) with the
+ * same window chain as oc_scan_topk; p = fa * alpha. A sub-tile that is whole
+ * and starts with a full heap (:163-206) scores p, plus the offset only when
+ * the offset is non-zero; every other sub-tile (:208-230) scores p + offset.
+ * Ids are id_map[v] (or v). Heaps [nq][k] and their state [nq] are updated in
+ * place; query indices must be < nq. */
+int oc_score_into(const float* q, size_t nq, size_t dim, size_t k, size_t bits,
+                  const uint8_t* blocked_perm0, const float* alphas, size_t n,
+                  const uint64_t* id_map, const float* shift, const float* scale,
+                  const size_t* qidx, size_t n_qidx, float* heap_s, uint64_t* heap_i,
+                  size_t* heap_sz, float* heap_min, size_t* heap_mi, const float* offsets) {
+  if (dim == 0 || dim % 8 != 0 || bits < 2 || bits > 4 || n_qidx == 0) return -1;
+  for (size_t i = 0; i < n_qidx; ++i)
+    if (qidx[i] >= nq) return -2;
+  if (n == 0 || k == 0) return 0;
+  for (size_t i = 0; i < n_qidx; ++i)
+    if (heap_mi[qidx[i]] == (size_t)-1) return OC_ERR_UNSUPPORTED;
+  const size_t G = groups_of(bits, dim);
+  const size_t W = (G + OC_FLUSH_EVERY - 1) / OC_FLUSH_EVERY;
+  float* q_rot = (float*)malloc(nq * dim * sizeof(float) + 1);
+  float* bc = (float*)malloc(nq * sizeof(float) + 1);
+  float* ls = (float*)malloc(nq * sizeof(float) + 1);
+  float* lb = (float*)malloc(nq * sizeof(float) + 1);
+  uint8_t* luts = (uint8_t*)malloc(nq * G * 32 + 1);
+  uint8_t* flat = (uint8_t*)malloc(n * G + 1);
+  int rc = oc_prepare_queries(q, nq, dim, bits, shift, scale, q_rot, bc, luts, ls, lb);
+  if (rc == 0) {
+    oc_unpack_perm0(blocked_perm0, n, G, flat);
+    for (size_t ii = 0; ii < n_qidx; ++ii) {
+      const size_t gq = qidx[ii];
+      const uint8_t* lut = luts + gq * G * 32;
+      const float bias = lb[gq] + bc[gq];
+      const float off = offsets ? offsets[gq] : 0.f;
+      float* hs = heap_s + gq * k;
+      uint64_t* hi = heap_i + gq * k;
+      for (size_t cb = 0; cb < n; cb += OC_CHUNK) {
+        const size_t end = cb + OC_CHUNK < n ? cb + OC_CHUNK : n;
+        const int fast = heap_sz[gq] >= k && end - cb == OC_CHUNK;
+        for (size_t v = cb; v < end; ++v) {
+          float fa = bias;
+          const uint8_t* row = flat + v * G;
+          for (size_t w = 0; w < W; ++w) {
+            uint32_t acc = 0;
+            const size_t g1 = (w + 1) * OC_FLUSH_EVERY < G ? (w + 1) * OC_FLUSH_EVERY : G;
+            for (size_t g = w * OC_FLUSH_EVERY; g < g1; ++g) {
+              const unsigned byte = row[g];
+              acc += lut[g * 32 + (byte >> 4)] + lut[g * 32 + 16 + (byte & 15u)];
+            }
+            fa = fmaf(ls[gq], (float)acc, fa);
+          }
+          const float p = fa * alphas[v];
+          const uint64_t id = id_map ? id_map[v] : (uint64_t)v;
+          if (fast) {
+            const float s = off != 0.f ? p + off : p;
+            if (s > heap_min[gq]) oc_heap_push(hs, hi, &heap_sz[gq], &heap_min[gq], &heap_mi[gq], k, s, id);
+          } else {
+            oc_heap_push(hs, hi, &heap_sz[gq], &heap_min[gq], &heap_mi[gq], k, p + off, id);
+          }
+        }
+      }
+    }
+  }
+  free(q_rot);
+  free(bc);
+  free(ls);
+  free(lb);
+  free(luts);
+  free(flat);
+  return rc;
+}
+
+/* heaps_to_search_results (reference src/search/search.cpp:1269-1291): each
+ * heap's heap_sz entries by (score desc, id asc), the rest of the row (0, 0). */
+int oc_heaps_to_results(size_t nq, size_t k, const float* heap_s, const uint64_t* heap_i,
+                        const size_t* heap_sz, float* scores_out, uint64_t* ids_out) {
+  oc_cand* c = (oc_cand*)malloc((k ? k : 1) * sizeof(oc_cand));
+  for (size_t qi = 0; qi < nq; ++qi) {
+    const size_t m = heap_sz[qi] < k ? heap_sz[qi] : k;
+    for (size_t j = 0; j < m; ++j) {
+      c[j].s = heap_s[qi * k + j];
+      c[j].id = heap_i[qi * k + j];
+    }
+    qsort(c, m, sizeof(oc_cand), cand_cmp);
+    for (size_t j = 0; j < k; ++j) {
+      scores_out[qi * k + j] = j < m ? c[j].s : 0.f;
+      ids_out[qi * k + j] = j < m ? c[j].id : 0;
+    }
+  }
+  free(c);
+  return 0;
+}

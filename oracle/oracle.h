@@ -41,6 +41,13 @@ int oc_search_flat(const float* q, size_t nq, size_t dim, size_t k, size_t bits,
                    const uint8_t* blocked_perm0, const float* alphas, size_t n,
                    const float* shift, const float* scale, float* scores_out, uint64_t* ids_out,
                    size_t* k_out);
+int oc_score_into(const float* q, size_t nq, size_t dim, size_t k, size_t bits,
+                  const uint8_t* blocked_perm0, const float* alphas, size_t n,
+                  const uint64_t* id_map, const float* shift, const float* scale,
+                  const size_t* qidx, size_t n_qidx, float* heap_s, uint64_t* heap_i,
+                  size_t* heap_sz, float* heap_min, size_t* heap_mi, const float* offsets);
+int oc_heaps_to_results(size_t nq, size_t k, const float* heap_s, const uint64_t* heap_i,
+                        const size_t* heap_sz, float* scores_out, uint64_t* ids_out);
 int oc_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t* dst_ids, size_t k_src,
                      const float* src_scores, const uint64_t* src_ids);
 void oc_set_threads(int n);

@@ -239,6 +239,51 @@ class Oracle:
                     "merge_results")
         return ds, di
 
+    def score_into(self, q, k, bits, blocked_perm0, alphas, qidx, heaps, id_map=None, shift=None, scale=None,
+                   offsets=None):
+        """score_prepared_into over one bucket (PERM0 blocks + alphas): heaps is
+        (heap_s [nq, k] f32, heap_i [nq, k] u64, heap_sz [nq] u64, heap_min [nq]
+        f32, heap_mi [nq] u64); returns the updated copies."""
+        q = np.ascontiguousarray(q, np.float32)
+        nq, dim = q.shape
+        alphas = np.ascontiguousarray(alphas, np.float32)
+        n = alphas.size
+        blocked_perm0 = np.ascontiguousarray(blocked_perm0, np.uint8)
+        qidx = np.ascontiguousarray(qidx, np.uint64)
+        hs, hi, hsz, hmin, hmi = (np.ascontiguousarray(heaps[0], np.float32).copy(),
+                                  np.ascontiguousarray(heaps[1], np.uint64).copy(),
+                                  np.ascontiguousarray(heaps[2], np.uint64).copy(),
+                                  np.ascontiguousarray(heaps[3], np.float32).copy(),
+                                  np.ascontiguousarray(heaps[4], np.uint64).copy())
+        im = None if id_map is None else np.ascontiguousarray(id_map, np.uint64)
+        sh = None if shift is None else np.ascontiguousarray(shift, np.float32)
+        sc = None if scale is None else np.ascontiguousarray(scale, np.float32)
+        off = None if offsets is None else np.ascontiguousarray(offsets, np.float32)
+        if self.kind == "reference":
+            nb = (n + 191) // 192
+            rc = self.lib.vcref_score_into(_ptr(q), _S(nq), _S(dim), _S(k), _S(bits), _ptr(blocked_perm0), _S(nb),
+                                           _ptr(alphas), _S(n), _ptr(im), _ptr(sh), _ptr(sc), _ptr(qidx),
+                                           _S(qidx.size), _ptr(hs), _ptr(hi), _ptr(hsz), _ptr(hmin), _ptr(hmi),
+                                           _ptr(off))
+        else:
+            rc = self.lib.oc_score_into(_ptr(q), _S(nq), _S(dim), _S(k), _S(bits), _ptr(blocked_perm0),
+                                        _ptr(alphas), _S(n), _ptr(im), _ptr(sh), _ptr(sc), _ptr(qidx),
+                                        _S(qidx.size), _ptr(hs), _ptr(hi), _ptr(hsz), _ptr(hmin), _ptr(hmi),
+                                        _ptr(off))
+        self._check(rc, "score_into")
+        return hs, hi, hsz, hmin, hmi
+
+    def heaps_to_results(self, heap_s, heap_i, heap_sz):
+        hs = np.ascontiguousarray(heap_s, np.float32)
+        hi = np.ascontiguousarray(heap_i, np.uint64)
+        hsz = np.ascontiguousarray(heap_sz, np.uint64)
+        nq, k = hs.shape
+        scores = np.zeros((nq, k), np.float32)
+        ids = np.zeros((nq, k), np.uint64)
+        self._check(self._f("heaps_to_results")(_S(nq), _S(k), _ptr(hs), _ptr(hi), _ptr(hsz), _ptr(scores),
+                                                _ptr(ids)), "heaps_to_results")
+        return scores, ids
+
 
 def available(kind: str) -> bool:
     if kind == "port":

@@ -276,4 +276,47 @@ int vcref_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t* ds
   });
 }
 
+// score_prepared_into() (reference src/search/search.cpp:1156-1267): the nq
+// queries are prepared as one batch; the heaps [nq][k] (heap_s, heap_i) and
+// their state [nq] (heap_sz, heap_min, heap_mi) are updated in place for the
+// queries listed in qidx only; offsets may be null.
+int vcref_score_into(const float* q, size_t nq, size_t dim, size_t k, size_t bits,
+                     const uint8_t* blocked, size_t n_blocks, const float* alphas, size_t n,
+                     const uint64_t* id_map, const float* shift, const float* scale,
+                     const size_t* qidx, size_t n_qidx, float* heap_s, uint64_t* heap_i,
+                     size_t* heap_sz, float* heap_min, size_t* heap_mi, const float* offsets) {
+  return guarded([&] {
+    vc::Rotation rot(dim);
+    auto [b, c] = vc::codebook(bits, dim);
+    std::span<const float> sh, sc;
+    if (shift != nullptr && scale != nullptr) {
+      sh = std::span<const float>(shift, dim);
+      sc = std::span<const float>(scale, dim);
+    }
+    auto prep = vc::prepare_queries(std::span<const float>(q, nq * dim), nq, dim, rot, c, bits, sh,
+                                    sc);
+    if (prep.backend != vc::SearchBackendKind::Avx2) {
+      throw std::runtime_error("reference did not select the AVX2 PERM0 backend");
+    }
+    const auto geo = vc::blocked_geometry(n, bits, dim);
+    std::span<const uint64_t> im;
+    if (id_map != nullptr) im = std::span<const uint64_t>(id_map, n);
+    vc::score_prepared_into(prep, k, std::span<const uint8_t>(blocked, geo.blocked_len), n_blocks,
+                            std::span<const float>(alphas, n), im,
+                            std::span<const size_t>(qidx, n_qidx), heap_s, heap_i, heap_sz,
+                            heap_min, heap_mi, offsets);
+  });
+}
+
+// heaps_to_search_results() (reference src/search/search.cpp:1269-1291).
+int vcref_heaps_to_results(size_t nq, size_t k, const float* heap_s, const uint64_t* heap_i,
+                           const size_t* heap_sz, float* scores_out, uint64_t* ids_out) {
+  return guarded([&] {
+    vc::SearchResults out;
+    vc::heaps_to_search_results(out, nq, k, heap_s, heap_i, heap_sz);
+    std::memcpy(scores_out, out.scores.data(), nq * k * sizeof(float));
+    std::memcpy(ids_out, out.ids.data(), nq * k * sizeof(uint64_t));
+  });
+}
+
 }  // extern "C"

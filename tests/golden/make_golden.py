@@ -4,12 +4,14 @@ Run in a container that has the reference sources (it builds oracle/_ref from
 them with oracle/Makefile and loads it); the outputs are data only:
 inputs plus the reference's outputs for the plain flat TurboQuant path
 (rotation tables, codebooks, encode, TQ+ calibration, flat bytes, PERM0
-blocks, per-query LUTs, search_flat top-k, id_map scoring, result merging),
-run on the reference's AVX2 PERM0 backend (TURBOVEC_NO_VNNI=1).
+blocks, per-query LUTs, search_flat top-k, id_map scoring, result merging,
+score_prepared_into over chained buckets), run on the reference's AVX2 PERM0
+backend (TURBOVEC_NO_VNNI=1).
 
 Large outputs are stored as SHA-256 digests of their little-endian bytes.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py              # everything
+    python tests/golden/make_golden.py score_into   # score_into.npz only
 """
 from __future__ import annotations
 
@@ -46,9 +48,81 @@ ROT_DIMS = [8, 16, 24, 64, 128, 200, 256, 384, 768, 1024, 1536, 4096]
 CB_DIMS = [8, 16, 64, 128, 200, 256, 512, 768, 1024, 1536, 3072, 16384]
 
 
+# score_prepared_into scenarios: (name, bits, dim, tq, bucket sizes, ks)
+SCORE_INTO = [("a", 4, 64, False, (300, 200, 300), (1, 7, 150)),
+              ("b", 2, 128, True, (250, 97), (10,))]
+
+
+def score_into_fixture(R):
+    """Chained score_prepared_into calls per scenario: buckets of one dataset
+    (with duplicated rows and zero rows, so scores tie and zero scores occur),
+    each with its own id map, query subset (unordered) and score offsets
+    (none, then a mix of 0, -0, positive and negative, then all +0); every
+    heap starts empty at -inf except one query's, seeded at a finite
+    threshold. Stores the inputs and every heap array after every call."""
+    out = {}
+    for name, bits, dim, tq, sizes, ks in SCORE_INTO:
+        r = np.random.default_rng(4100 + dim)
+        n = sum(sizes)
+        centers = r.standard_normal((24, dim)).astype(np.float32)
+        x = (centers[r.integers(0, 24, n)] + 0.3 * r.standard_normal((n, dim))).astype(np.float32)
+        x[7] = 0.0
+        x[n - 3] = 0.0
+        x[20:26] = x[19]
+        x[sizes[0] + 5: sizes[0] + 9] = x[3]
+        nq = 6
+        q = r.standard_normal((nq, dim)).astype(np.float32)
+        shift = scale = None
+        if tq:
+            shift, scale = R.fit_calibration(x[:200], bits)
+        packed, alpha = R.encode(x, bits, shift, scale)
+        out[f"{name}_q"] = q
+        out[f"{name}_bits"] = np.array([bits], np.int64)
+        if tq:
+            out[f"{name}_shift"] = shift
+            out[f"{name}_scale"] = scale
+        qsets = [np.array([3, 0, 5, 1], np.uint64), np.array([5, 2, 0, 4, 1, 3], np.uint64),
+                 np.array([1, 5], np.uint64)]
+        offs = [None, np.array([0.0, -0.0, 0.25, -1.5, 3e-3, 0.0], np.float32), np.zeros(nq, np.float32)]
+        start = 0
+        calls = []
+        for ci, sz in enumerate(sizes):
+            pk = packed[start:start + sz]
+            perm0 = R.repack_perm0(pk, bits, dim)
+            al = alpha[start:start + sz].copy()
+            id_map = (np.arange(sz, dtype=np.uint64) * (ci + 2) + 1000 * (ci + 1)).astype(np.uint64)
+            if ci == 2:
+                id_map = id_map[::-1].copy()
+            calls.append((perm0, al, id_map, qsets[ci % 3], offs[ci % 3]))
+            out[f"{name}_c{ci}_perm0"] = perm0
+            out[f"{name}_c{ci}_alpha"] = al
+            out[f"{name}_c{ci}_idmap"] = id_map
+            out[f"{name}_c{ci}_qidx"] = qsets[ci % 3]
+            if offs[ci % 3] is not None:
+                out[f"{name}_c{ci}_offsets"] = offs[ci % 3]
+            start += sz
+        for k in ks:
+            heaps = (np.zeros((nq, k), np.float32), np.zeros((nq, k), np.uint64), np.zeros(nq, np.uint64),
+                     np.full(nq, -np.inf, np.float32), np.zeros(nq, np.uint64))
+            heaps[3][4] = np.float32(0.05)
+            for ci, (perm0, al, id_map, qidx, off) in enumerate(calls):
+                heaps = R.score_into(q, k, bits, perm0, al, qidx, heaps, id_map=id_map, shift=shift, scale=scale,
+                                     offsets=off)
+                for hn, h in zip(("hs", "hi", "hsz", "hmin", "hmi"), heaps):
+                    out[f"{name}_k{k}_c{ci}_{hn}"] = h
+            s, i = R.heaps_to_results(heaps[0], heaps[1], heaps[2])
+            out[f"{name}_k{k}_scores"] = s
+            out[f"{name}_k{k}_ids"] = i
+    np.savez_compressed(OUT / "score_into.npz", **out)
+
+
 def main():
     subprocess.run(["make", "-C", str(ROOT / "oracle"), "ref"], check=True, stdout=subprocess.DEVNULL)
     R = Oracle("reference")
+    score_into_fixture(R)
+    if len(sys.argv) > 1 and sys.argv[1] == "score_into":
+        print("wrote score_into.npz")
+        return
 
     rot = {}
     for d in ROT_DIMS:

@@ -96,3 +96,27 @@ def test_compute_fails_loudly_without_a_device(capi):
         vc.TurboQuantIndex(128, 4)
     with pytest.raises(capi.VectorCacheError):
         vc.encode(np.ones((4, 128), np.float32), 4)
+
+
+def test_score_into_argument_errors_without_device(capi):
+    """score_prepared_into's argument checks (the reference's exceptions) come
+    before any device work: empty query_indices -> ValueError
+    (std::invalid_argument), an index >= nq -> IndexError (std::out_of_range),
+    n == 0 or k == 0 -> no change."""
+    import vectorcache_amd.index as vc
+
+    q = np.zeros((3, 64), np.float32)
+    heaps = vc.empty_heaps(3, 5)
+    with pytest.raises(ValueError):
+        vc.score_perm0_into(q, 5, 4, np.zeros(0, np.uint8), np.zeros(0, np.float32), [], heaps)
+    with pytest.raises(IndexError):
+        vc.score_perm0_into(q, 5, 4, np.zeros(0, np.uint8), np.zeros(0, np.float32), [0, 3], heaps)
+    before = [h.copy() for h in heaps]
+    vc.score_perm0_into(q, 5, 4, np.zeros(0, np.uint8), np.zeros(0, np.float32), [2, 0], heaps)  # n == 0
+    assert all(np.array_equal(a, b) for a, b in zip(before, heaps))
+    heaps[4][1] = np.uint64(2**64 - 1)  # the buffered collector marker
+    with pytest.raises(ValueError):
+        vc.score_perm0_into(q, 5, 4, np.zeros(32 * 192, np.uint8), np.ones(1, np.float32), [1], heaps)
+    heaps[4][1] = 0
+    with pytest.raises(ValueError):  # a repeated index
+        vc.score_perm0_into(q, 5, 4, np.zeros(32 * 192, np.uint8), np.ones(1, np.float32), [1, 1], heaps)

@@ -275,3 +275,49 @@ def test_api_errors_and_edges(vc):
     assert empty.search(x[:3], 5).k == 0
     with pytest.raises(ValueError):
         vc.TurboQuantIndex(64, 4).calibrate(x[:1])  # needs >= 2 rows
+
+
+# ------------------------------------------------- score_prepared_into (heaps)
+def test_score_into_matches_golden(vc):
+    """Chained score_prepared_into calls on the GPU: every heap array (scores,
+    ids, sizes, thresholds, worst positions) after every call equals the
+    reference's, and heaps_to_search_results of the last state too."""
+    from conftest import load_golden
+    from util import empty_heaps, heaps_equal, score_into_scenarios
+
+    g = load_golden("score_into.npz")
+    checked = 0
+    for name, bits, dim, k, q, shift, scale, calls, want in score_into_scenarios(g):
+        heaps = empty_heaps(q.shape[0], k, {4: 0.05})
+        for ci, (perm0, alpha, id_map, qidx, off) in enumerate(calls):
+            vc.score_perm0_into(q, k, bits, perm0, alpha, qidx, heaps, id_map=id_map, shift=shift, scale=scale,
+                                score_offsets=off)
+            assert heaps_equal(heaps, want[ci]), (name, k, ci)
+            checked += 1
+        r = vc.heaps_to_results(heaps)
+        assert bits_equal(r.scores, want["final"][0]) and np.array_equal(r.ids, want["final"][1]), (name, k)
+    assert checked >= 10
+
+
+@pytest.mark.parametrize("bits,dim,k,n", [(4, 256, 10, 3000), (4, 768, 300, 1500), (3, 128, 64, 2500)])
+def test_score_into_matches_oracle(vc, oracle, bits, dim, k, n):
+    """Larger buckets (many sub-tiles, several 256-group windows at dim 768),
+    two chained calls with offsets, against the oracle."""
+    from vectorcache_amd import synthetic
+
+    from util import empty_heaps, heaps_equal
+
+    x = synthetic.mixture(n, dim, 32, 0.3, 500 + dim, 501 + n)
+    x[50:90] = x[49]
+    q = synthetic.mixture(7, dim, 32, 0.3, 500 + dim, 502)
+    packed, alpha = oracle.encode(x, bits)
+    half = n // 2 + 17
+    hg, ho = empty_heaps(7, k, {6: 0.1}), empty_heaps(7, k, {6: 0.1})
+    for ci, (a, b) in enumerate(((0, half), (half, n))):
+        perm0 = oracle.repack_perm0(packed[a:b], bits, dim)
+        qidx = np.array([6, 2, 0, 5] if ci == 0 else [1, 2, 3, 4, 5, 6, 0], np.uint64)
+        off = np.linspace(-1, 1, 7).astype(np.float32) if ci else None
+        idm = (np.arange(b - a, dtype=np.uint64) * 3 + (7 if ci else 100000))
+        ho = oracle.score_into(q, k, bits, perm0, alpha[a:b], qidx, ho, id_map=idm, offsets=off)
+        vc.score_perm0_into(q, k, bits, perm0, alpha[a:b], qidx, hg, id_map=idm, score_offsets=off)
+        assert heaps_equal(hg, ho), (bits, dim, k, ci)

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from conftest import load_golden
-from util import bits_equal, digest, sha
+from util import bits_equal, digest, scores_equal, sha
 
 
 # ---------------------------------------------------------------- golden vectors
@@ -243,3 +243,78 @@ def test_topk_is_full_sort(oracle):
         sums = oracle.window_sums(lut[qi], flat)[:, 0]
         want = (scale[qi] * sums.astype(np.float64) + bias[qi]).astype(np.float32)  # exact here
         assert bits_equal(np.sort(want)[::-1], s_all[qi])
+
+
+# ------------------------------------------------- score_prepared_into (heaps)
+def test_score_into_golden(oracle):
+    """Chained score_prepared_into calls (several buckets, id maps, query
+    subsets, offsets, a seeded threshold; rescan heap k < 128 and binary heap
+    k >= 128): every heap array after every call equals the reference's."""
+    from util import empty_heaps, heaps_equal, score_into_scenarios
+
+    g = load_golden("score_into.npz")
+    n_checked = 0
+    for name, bits, dim, k, q, shift, scale, calls, want in score_into_scenarios(g):
+        heaps = empty_heaps(q.shape[0], k, {4: 0.05})
+        for ci, (perm0, alpha, id_map, qidx, off) in enumerate(calls):
+            heaps = oracle.score_into(q, k, bits, perm0, alpha, qidx, heaps, id_map=id_map, shift=shift,
+                                      scale=scale, offsets=off)
+            assert heaps_equal(heaps, want[ci]), (name, k, ci)
+            n_checked += 1
+        s, i = oracle.heaps_to_results(heaps[0], heaps[1], heaps[2])
+        assert bits_equal(s, want["final"][0]) and np.array_equal(i, want["final"][1]), (name, k)
+    assert n_checked >= 10
+
+
+@pytest.mark.parametrize("k", [5, 40, 200])
+def test_score_into_single_bucket_is_full_sort(oracle, k):
+    """One bucket into empty heaps (threshold -inf), ids increasing with the
+    scan order: strict admission in scan order keeps, among equal scores, the
+    earlier (lower) id, so the heaps hold the exact top-k of a full sort by
+    (score desc, id asc) -- an independent check of the heap semantics."""
+    from vectorcache_amd import synthetic
+
+    from util import empty_heaps
+
+    n, dim, bits = 700, 64, 4
+    x = synthetic.mixture(n, dim, 16, 0.3, 91, 92)
+    x[100:140] = x[99]                       # ties
+    q = synthetic.mixture(4, dim, 16, 0.3, 91, 93)
+    packed, alpha = oracle.encode(x, bits)
+    perm0 = oracle.repack_perm0(packed, bits, dim)
+    id_map = (np.arange(n, dtype=np.uint64) * 5 + 2)
+    heaps = oracle.score_into(q, k, bits, perm0, alpha, np.arange(4, dtype=np.uint64), empty_heaps(4, k),
+                              id_map=id_map)
+    s, i = oracle.heaps_to_results(heaps[0], heaps[1], heaps[2])
+    prep = oracle.prepare_queries(q, bits)
+    flat = oracle.codes_flat(packed, bits, dim)
+    s_all, i_all = oracle.scan_topk(prep["luts"], prep["lut_scale"], prep["lut_bias"] + prep["bias_corr"], flat,
+                                    alpha, n)
+    for qi in range(4):
+        order = np.lexsort((i_all[qi], -s_all[qi]))[:k]
+        assert np.array_equal(i[qi], id_map[i_all[qi][order]]), (k, qi)
+        assert scores_equal(s[qi], s_all[qi][order]), (k, qi)
+
+
+def test_score_into_vs_reference_build(oracle, ref):
+    """Random chained calls, the oracle against the reference build."""
+    from vectorcache_amd import synthetic
+
+    from util import empty_heaps, heaps_equal
+
+    r = np.random.default_rng(77)
+    for bits, dim, k in ((4, 128, 9), (3, 96, 130), (2, 64, 33)):
+        n = 420
+        x = synthetic.mixture(n, dim, 8, 0.2, dim, bits)
+        x[r.integers(0, n, 30)] = x[5]
+        q = synthetic.mixture(5, dim, 8, 0.2, dim, bits + 50)
+        packed, alpha = ref.encode(x, bits)
+        ho, hr = empty_heaps(5, k, {2: -0.25}), empty_heaps(5, k, {2: -0.25})
+        for ci, (a, b) in enumerate(((0, 160), (160, 161), (161, 420))):
+            perm0 = ref.repack_perm0(packed[a:b], bits, dim)
+            qidx = r.permutation(5)[: 2 + ci].astype(np.uint64)
+            off = (r.standard_normal(5) * (ci % 2)).astype(np.float32)
+            idm = r.permutation(10 * n)[: b - a].astype(np.uint64)
+            hr = ref.score_into(q, k, bits, perm0, alpha[a:b], qidx, hr, id_map=idm, offsets=off)
+            ho = oracle.score_into(q, k, bits, perm0, alpha[a:b], qidx, ho, id_map=idm, offsets=off)
+            assert heaps_equal(hr, ho), (bits, dim, k, ci)

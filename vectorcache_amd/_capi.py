@@ -68,6 +68,8 @@ _SIGS = {
     "vc_rotate": ([P, S, S, P], I),
     "vc_search_perm0": ([P, S, S, S, S, P, P, S, P, P, P, P, P, ctypes.POINTER(S)], I),
     "vc_merge_results": ([S, S, P, P, S, P, P], I),
+    "vc_score_perm0_into": ([P, S, S, S, S, P, P, S, P, P, P, P, S, P, P, P, P, P, P], I),
+    "vc_heaps_to_results": ([S, S, P, P, P, P, P], I),
     "vc_set_scan_tile": ([S], I),
     "vc_scan_tile_for": ([S, S, S], S),
     "vc_set_profiling": ([I], I),

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -758,6 +759,141 @@ vc_status vc_search_perm0(const float* queries, size_t nq, size_t dim, size_t bi
                  id_map ? sc.a.as<uint64_t>(om) : nullptr, sc.a.as<float>(os), sc.a.as<uint64_t>(oi), sc.scratch, sc.s);
     hip_check(hipMemcpyAsync(scores, sc.a.as<float>(os), nq * ke * 4, hipMemcpyDeviceToHost, sc.s), "copy");
     hip_check(hipMemcpyAsync(ids, sc.a.as<uint64_t>(oi), nq * ke * 8, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipStreamSynchronize(sc.s), "sync");
+  });
+}
+
+vc_status vc_score_perm0_into(const float* queries, size_t nq, size_t dim, size_t bits, size_t k,
+                              const uint8_t* perm0, const float* alpha, size_t n, const uint64_t* id_map,
+                              const float* shift, const float* scale, const size_t* query_indices,
+                              size_t n_indices, float* heap_s, uint64_t* heap_i, size_t* heap_sz,
+                              float* heap_min, size_t* heap_mi, const float* score_offsets) {
+  return guarded([&] {
+    check_dim_bits(dim, bits);
+    if (n_indices == 0 || query_indices == nullptr)
+      fail(VC_ERR_INVALID_ARGUMENT, "score_prepared_into requires non-empty query_indices");
+    for (size_t i = 0; i < n_indices; ++i)
+      if (query_indices[i] >= nq) fail(VC_ERR_OUT_OF_RANGE, "score_prepared_into: query_indices out of range");
+    if (n == 0 || k == 0) return;
+    if (queries == nullptr || perm0 == nullptr || alpha == nullptr || heap_s == nullptr || heap_i == nullptr ||
+        heap_sz == nullptr || heap_min == nullptr || heap_mi == nullptr)
+      fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    if ((shift == nullptr) != (scale == nullptr)) fail(VC_ERR_INVALID_ARGUMENT, "shift and scale go together");
+    if (k > size_t(vchip::kMaxHeapK)) fail(VC_ERR_INVALID_ARGUMENT, "k exceeds the GPU heap capacity of 4096");
+    if (n > 0xFFFFFFFFull) fail(VC_ERR_INVALID_ARGUMENT, "bucket exceeds 2^32 vectors");
+    {
+      std::vector<size_t> sorted(query_indices, query_indices + n_indices);
+      std::sort(sorted.begin(), sorted.end());
+      if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+        fail(VC_ERR_INVALID_ARGUMENT, "score_prepared_into: duplicate query index");
+    }
+    for (size_t i = 0; i < n_indices; ++i) {
+      const size_t qi = query_indices[i];
+      if (heap_mi[qi] == SIZE_MAX) fail(VC_ERR_INVALID_ARGUMENT, "buffered top-k collectors are not supported");
+      if (heap_sz[qi] > k) fail(VC_ERR_INVALID_ARGUMENT, "heap_sz exceeds k");
+      if (heap_sz[qi] == k && heap_mi[qi] >= k) fail(VC_ERR_INVALID_ARGUMENT, "heap_mi out of range");
+    }
+    const size_t m = n_indices;
+    // the listed queries, their offsets and heaps, gathered in list order
+    std::vector<float> hq(m * dim), hoff(m), hs(m * k), hmin(m);
+    std::vector<uint64_t> hi(m * k), hsz(m), hmi(m);
+    for (size_t i = 0; i < m; ++i) {
+      const size_t qi = query_indices[i];
+      std::memcpy(&hq[i * dim], queries + qi * dim, dim * 4);
+      hoff[i] = score_offsets ? score_offsets[qi] : 0.f;
+      std::memcpy(&hs[i * k], heap_s + qi * k, k * 4);
+      std::memcpy(&hi[i * k], heap_i + qi * k, k * 8);
+      hsz[i] = heap_sz[qi];
+      hmin[i] = heap_min[qi];
+      hmi[i] = heap_mi[qi];
+    }
+    Scoped sc;
+    vchip::CodeTables t = make_tables(dim, bits);
+    const size_t G = size_t(t.groups);
+    const size_t pbytes = (n + 191) / 192 * G * 192;
+    Carve c;
+    const size_t oq = c.take(m * dim * 4), op = c.take(pbytes), of = c.take(n * G),
+                 ot = c.take(vchip::tiled_bytes(n, t.groups16)), oa = c.take(n * 4), om = c.take(id_map ? n * 8 : 0),
+                 ocal = c.take(3 * dim * 4), oluts = c.take(m * G * 32), ols = c.take(m * 4), olb = c.take(m * 4),
+                 obc = c.take(m * 4), obt = c.take(m * 4), opv = c.take(m * n * 4), ooff = c.take(m * 4),
+                 ohs = c.take(m * k * 4), ohi = c.take(m * k * 8), ohsz = c.take(m * 8), ohmin = c.take(m * 4),
+                 ohmi = c.take(m * 8);
+    sc.a.reserve(c.off, sc.s);
+    auto up = [&](size_t off, const void* src, size_t bytes) {
+      hip_check(hipMemcpyAsync(sc.a.as<uint8_t>(off), src, bytes, hipMemcpyHostToDevice, sc.s), "copy");
+    };
+    up(oq, hq.data(), m * dim * 4);
+    up(op, perm0, pbytes);
+    up(oa, alpha, n * 4);
+    if (id_map) up(om, id_map, n * 8);
+    up(ooff, hoff.data(), m * 4);
+    up(ohs, hs.data(), m * k * 4);
+    up(ohi, hi.data(), m * k * 8);
+    up(ohsz, hsz.data(), m * 8);
+    up(ohmin, hmin.data(), m * 4);
+    up(ohmi, hmi.data(), m * 8);
+    if (shift != nullptr) {
+      up(ocal, shift, dim * 4);
+      up(ocal + dim * 4, scale, dim * 4);
+      t.shift = sc.a.as<float>(ocal);
+      t.scale = sc.a.as<float>(ocal + dim * 4);
+      t.inv_scale = nullptr;  // not used by query preparation
+    }
+    hip_check(vchip::launch_perm0_to_flat(sc.a.as<uint8_t>(op), n, int(G), sc.a.as<uint8_t>(of), sc.s), "perm0_to_flat");
+    hip_check(vchip::launch_flat_to_tiled(sc.a.as<uint8_t>(of), n, int(G), t.groups16, sc.a.as<uint8_t>(ot), sc.s),
+              "flat_to_tiled");
+    hip_check(vchip::launch_prepare_queries(t, sc.a.as<float>(oq), m, sc.a.as<uint8_t>(oluts), sc.a.as<float>(ols),
+                                            sc.a.as<float>(olb), sc.a.as<float>(obc), sc.a.as<float>(obt), sc.s),
+              "prepare_queries");
+    hip_check(vchip::launch_bucket_scores(sc.a.as<uint8_t>(ot), sc.a.as<float>(oa), n, int(G), t.groups16,
+                                          sc.a.as<uint8_t>(oluts), sc.a.as<float>(ols), sc.a.as<float>(obt), m,
+                                          sc.a.as<float>(opv), sc.s),
+              "bucket_scores");
+    hip_check(vchip::launch_heap_replay(sc.a.as<float>(opv), n, int(k), id_map ? sc.a.as<uint64_t>(om) : nullptr,
+                                        sc.a.as<float>(ooff), m, sc.a.as<float>(ohs), sc.a.as<uint64_t>(ohi),
+                                        sc.a.as<uint64_t>(ohsz), sc.a.as<float>(ohmin), sc.a.as<uint64_t>(ohmi), sc.s),
+              "heap_replay");
+    auto down = [&](void* dst, size_t off, size_t bytes) {
+      hip_check(hipMemcpyAsync(dst, sc.a.as<uint8_t>(off), bytes, hipMemcpyDeviceToHost, sc.s), "copy");
+    };
+    down(hs.data(), ohs, m * k * 4);
+    down(hi.data(), ohi, m * k * 8);
+    down(hsz.data(), ohsz, m * 8);
+    down(hmin.data(), ohmin, m * 4);
+    down(hmi.data(), ohmi, m * 8);
+    hip_check(hipStreamSynchronize(sc.s), "sync");
+    for (size_t i = 0; i < m; ++i) {
+      const size_t qi = query_indices[i];
+      std::memcpy(heap_s + qi * k, &hs[i * k], k * 4);
+      std::memcpy(heap_i + qi * k, &hi[i * k], k * 8);
+      heap_sz[qi] = size_t(hsz[i]);
+      heap_min[qi] = hmin[i];
+      heap_mi[qi] = size_t(hmi[i]);
+    }
+  });
+}
+
+vc_status vc_heaps_to_results(size_t nq, size_t k, const float* heap_s, const uint64_t* heap_i,
+                              const size_t* heap_sz, float* scores, uint64_t* ids) {
+  return guarded([&] {
+    if (nq == 0 || k == 0) return;
+    if (!heap_s || !heap_i || !heap_sz || !scores || !ids) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    if (k > size_t(vchip::kMaxHeapK)) fail(VC_ERR_INVALID_ARGUMENT, "k exceeds the GPU heap capacity of 4096");
+    Scoped sc;
+    Carve c;
+    const size_t ohs = c.take(nq * k * 4), ohi = c.take(nq * k * 8), ohsz = c.take(nq * 8), oos = c.take(nq * k * 4),
+                 ooi = c.take(nq * k * 8);
+    sc.a.reserve(c.off, sc.s);
+    std::vector<uint64_t> hsz(heap_sz, heap_sz + nq);
+    hip_check(hipMemcpyAsync(sc.a.as<float>(ohs), heap_s, nq * k * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(hipMemcpyAsync(sc.a.as<uint64_t>(ohi), heap_i, nq * k * 8, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(hipMemcpyAsync(sc.a.as<uint64_t>(ohsz), hsz.data(), nq * 8, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(vchip::launch_heaps_to_results(nq, int(k), sc.a.as<float>(ohs), sc.a.as<uint64_t>(ohi),
+                                             sc.a.as<uint64_t>(ohsz), sc.a.as<float>(oos), sc.a.as<uint64_t>(ooi),
+                                             sc.s),
+              "heaps_to_results");
+    hip_check(hipMemcpyAsync(scores, sc.a.as<float>(oos), nq * k * 4, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipMemcpyAsync(ids, sc.a.as<uint64_t>(ooi), nq * k * 8, hipMemcpyDeviceToHost, sc.s), "copy");
     hip_check(hipStreamSynchronize(sc.s), "sync");
   });
 }

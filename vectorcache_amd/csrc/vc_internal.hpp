@@ -94,6 +94,17 @@ hipError_t launch_merge_ranges(const float* part_s, const std::uint32_t* part_i,
                                hipStream_t s);
 
 // merge_search_results on device buffers.
+// score_prepared_into for one bucket: p [nq][n] = fa * alpha, then the heap
+// replay into heaps [nq][k] (state arrays [nq]); k <= kMaxHeapK.
+constexpr int kMaxHeapK = 4096;
+hipError_t launch_bucket_scores(const std::uint8_t* tiled, const float* alpha, std::size_t n, int groups,
+                                int groups16, const std::uint8_t* luts, const float* lut_scale,
+                                const float* bias_total, std::size_t nq, float* p_out, hipStream_t s);
+hipError_t launch_heap_replay(const float* p, std::size_t n, int k, const std::uint64_t* id_map,
+                              const float* offsets, std::size_t nq, float* hs, std::uint64_t* hi,
+                              std::uint64_t* hsz, float* hmin, std::uint64_t* hmi, hipStream_t s);
+hipError_t launch_heaps_to_results(std::size_t nq, int k, const float* hs, const std::uint64_t* hi,
+                                   const std::uint64_t* hsz, float* out_s, std::uint64_t* out_i, hipStream_t s);
 hipError_t launch_merge_results(std::size_t nq, int k_dst, float* dst_s, std::uint64_t* dst_i,
                                 int k_src, const float* src_s, const std::uint64_t* src_i,
                                 hipStream_t s);

@@ -1826,6 +1826,231 @@ __global__ __launch_bounds__(256) void merge_ranges_lds_kernel(const float* __re
   }
 }
 
+// ---------------------------------------------------------------------------
+// score_prepared_into (src/search/search.cpp:1156-1267) for one bucket, the
+// reference's AVX2 PERM0 path (src/search/search_avx2.cpp:72-230):
+//   1. bucket_scores_kernel: p[i][v] = fa * alpha[v], fa = bias + bias_corr
+//      followed by fmaf(scale, window sum, fa) per 256-group window;
+//   2. heap_replay_kernel: per listed query, the vectors in order by 32-vector
+//      sub-tiles into the query's persistent heap (topk_heap.hpp:19-157).
+// ---------------------------------------------------------------------------
+constexpr int kWindowGroups = 256;  // kFlushEvery
+
+// grid (ceil(n / 256), queries), 256 threads; one window of LUT rows in LDS.
+__global__ __launch_bounds__(256) void bucket_scores_kernel(const uint8_t* __restrict__ tiled,
+                                                            const float* __restrict__ alpha, long n, int G,
+                                                            int G16, const uint8_t* __restrict__ luts,
+                                                            const float* __restrict__ lut_scale,
+                                                            const float* __restrict__ bias_total,
+                                                            float* __restrict__ p_out) {
+  __shared__ __align__(16) uint8_t lut[kWindowGroups * 32];
+  const int qi = blockIdx.y;
+  const long v = (long)blockIdx.x * 256 + threadIdx.x;
+  const long vs = v < n ? v : n - 1;  // lanes past n read a valid row and store nothing
+  const uint8_t* ql = luts + (long)qi * G * 32;
+  const float scale = lut_scale[qi];
+  float fa = bias_total[qi];
+  for (int g0 = 0; g0 < G; g0 += kWindowGroups) {
+    const int g1 = min(G, g0 + kWindowGroups);
+    __syncthreads();
+    for (int i = threadIdx.x; i < (g1 - g0) * 8; i += 256)
+      reinterpret_cast<uint32_t*>(lut)[i] = reinterpret_cast<const uint32_t*>(ql + (long)g0 * 32)[i];
+    __syncthreads();
+    uint32_t acc = 0;
+    for (int b = g0 / 16; b * 16 < g1; ++b) {
+      const uint4 cw = *reinterpret_cast<const uint4*>(tiled + ((vs >> 6) * G16 + b) * 1024 + (vs & 63) * 16);
+      const uint32_t words[4] = {cw.x, cw.y, cw.z, cw.w};
+      const int ge = min(16, g1 - b * 16);  // padded groups of the last block are skipped
+      for (int e = 0; e < ge; ++e) {
+        const uint32_t byte = __builtin_amdgcn_ubfe(words[e >> 2], 8 * (e & 3), 8);
+        const int gl = b * 16 + e - g0;
+        acc += lut[gl * 32 + (byte >> 4)] + lut[gl * 32 + 16 + (byte & 15u)];
+      }
+    }
+    fa = __fmaf_rn(scale, (float)acc, fa);
+  }
+  if (v < n) p_out[(long)qi * n + v] = fa * alpha[v];
+}
+
+// topk_worse (topk_heap.hpp:19-22): lower score is worse; on equal scores the
+// larger id
+__device__ __forceinline__ bool heap_worse(float sa, uint64_t ia, float sb, uint64_t ib) {
+  return sa != sb ? sa < sb : ia > ib;
+}
+
+// rescan_min (topk_heap.hpp:26-36) by one wavefront: the first index of the
+// worst entry
+__device__ __forceinline__ void heap_rescan(const float* hs, const uint64_t* hi, int k, int lane, float& hmin,
+                                            long& hmi) {
+  int mi = -1;
+  float ms = 0.f;
+  uint64_t mid = 0;
+  for (int j = lane; j < k; j += 64) {
+    const float sj = hs[j];
+    const uint64_t ij = hi[j];
+    if (mi < 0 || heap_worse(sj, ij, ms, mid)) {
+      mi = j;
+      ms = sj;
+      mid = ij;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const int omi = __shfl_xor(mi, o);
+    const float os = __shfl_xor(ms, o);
+    const uint64_t oid = ((uint64_t)(uint32_t)__shfl_xor((int)(uint32_t)(mid >> 32), o) << 32) |
+                         (uint32_t)__shfl_xor((int)(uint32_t)mid, o);
+    if (omi >= 0 && (mi < 0 || heap_worse(os, oid, ms, mid) || (!heap_worse(ms, mid, os, oid) && omi < mi))) {
+      mi = omi;
+      ms = os;
+      mid = oid;
+    }
+  }
+  hmin = ms;
+  hmi = mi;
+}
+
+// topk_sift_down (topk_heap.hpp:38-58), one lane
+__device__ void heap_sift_down(float* hs, uint64_t* hi, int k, int i) {
+  for (;;) {
+    const int l = 2 * i + 1, r = 2 * i + 2;
+    int w = i;
+    if (l < k && heap_worse(hs[l], hi[l], hs[w], hi[w])) w = l;
+    if (r < k && heap_worse(hs[r], hi[r], hs[w], hi[w])) w = r;
+    if (w == i) return;
+    const float ts = hs[i];
+    hs[i] = hs[w];
+    hs[w] = ts;
+    const uint64_t ti = hi[i];
+    hi[i] = hi[w];
+    hi[w] = ti;
+    i = w;
+  }
+}
+
+// One wavefront per listed query; its heap (k entries) and state in LDS.
+// heap_sz / heap_min / heap_mi are computed identically by every lane.
+__global__ __launch_bounds__(64) void heap_replay_kernel(const float* __restrict__ p, long n, int k,
+                                                         const uint64_t* __restrict__ id_map,
+                                                         const float* __restrict__ offsets,
+                                                         float* __restrict__ hs_g, uint64_t* __restrict__ hi_g,
+                                                         uint64_t* __restrict__ hsz_g, float* __restrict__ hmin_g,
+                                                         uint64_t* __restrict__ hmi_g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float* hs = reinterpret_cast<float*>(smem);
+  uint64_t* hi = reinterpret_cast<uint64_t*>(smem + (((size_t)k * 4 + 15) & ~size_t(15)));
+  const int qi = blockIdx.x, lane = threadIdx.x;
+  for (int j = lane; j < k; j += 64) {
+    hs[j] = hs_g[(long)qi * k + j];
+    hi[j] = hi_g[(long)qi * k + j];
+  }
+  long hsz = (long)hsz_g[qi];
+  float hmin = hmin_g[qi];
+  long hmi = (long)hmi_g[qi];
+  const float off = offsets[qi];
+  const float* pq = p + (long)qi * n;
+  __syncthreads();
+  for (long cb = 0; cb < n; cb += 32) {
+    const int cnt = (int)min(32L, n - cb);
+    // a whole sub-tile starting with a full heap adds the offset only when it
+    // is non-zero (search_avx2.cpp:163-180); every other sub-tile adds it
+    const bool fast = hsz >= k && cnt == 32;
+    float sv = 0.f;
+    if (lane < cnt) {
+      const float pv = pq[cb + lane];
+      sv = fast ? (off != 0.f ? pv + off : pv) : pv + off;
+    }
+    uint64_t pend = __ballot(lane < cnt && sv > hmin);
+    while (pend != 0) {
+      const int l = (int)__builtin_ctzll(pend);
+      const float s = __shfl(sv, l);
+      const uint64_t id = id_map != nullptr ? id_map[cb + l] : (uint64_t)(cb + l);
+      // heap_push_or_replace (topk_heap.hpp:113-157); s > hmin holds here
+      if (k < 128) {
+        const long at = hsz < k ? hsz : hmi;
+        if (lane == 0) {
+          hs[at] = s;
+          hi[at] = id;
+        }
+        if (hsz < k) ++hsz;
+        __syncthreads();
+        if (hsz == k) heap_rescan(hs, hi, k, lane, hmin, hmi);
+      } else if (hsz < k) {
+        if (lane == 0) {
+          hs[hsz] = s;
+          hi[hsz] = id;
+          if (hsz + 1 == k)
+            for (int i = k / 2; i-- > 0;) heap_sift_down(hs, hi, k, i);
+        }
+        ++hsz;
+        __syncthreads();
+        if (hsz == k) {
+          hmin = hs[0];
+          hmi = 0;
+        }
+      } else {
+        if (lane == 0) {
+          hs[0] = s;
+          hi[0] = id;
+          heap_sift_down(hs, hi, k, 0);
+        }
+        __syncthreads();
+        hmin = hs[0];
+        hmi = 0;
+      }
+      pend &= pend - 1;
+      pend &= __ballot(sv > hmin);  // the threshold only rises
+    }
+  }
+  __syncthreads();
+  for (int j = lane; j < k; j += 64) {
+    hs_g[(long)qi * k + j] = hs[j];
+    hi_g[(long)qi * k + j] = hi[j];
+  }
+  if (lane == 0) {
+    hsz_g[qi] = (uint64_t)hsz;
+    hmin_g[qi] = hmin;
+    hmi_g[qi] = (uint64_t)hmi;
+  }
+}
+
+// heaps_to_search_results (src/search/search.cpp:1269-1291): per query the
+// heap_sz entries by (score desc, id asc) -- each entry's rank counted against
+// all others, equal pairs ordered by position -- then (0, 0) to k.
+__global__ __launch_bounds__(256) void heaps_to_results_kernel(long nq, int k, const float* __restrict__ hs_g,
+                                                               const uint64_t* __restrict__ hi_g,
+                                                               const uint64_t* __restrict__ hsz_g,
+                                                               float* __restrict__ out_s,
+                                                               uint64_t* __restrict__ out_i) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float* hs = reinterpret_cast<float*>(smem);
+  uint64_t* hi = reinterpret_cast<uint64_t*>(smem + (((size_t)k * 4 + 15) & ~size_t(15)));
+  for (long q = blockIdx.x; q < nq; q += gridDim.x) {
+    const int m = (int)min((uint64_t)k, hsz_g[q]);
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+      hs[j] = hs_g[q * k + j];
+      hi[j] = hi_g[q * k + j];
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+      const float sj = hs[j];
+      const uint64_t ij = hi[j];
+      int rank = 0;
+      for (int i = 0; i < m; ++i) {
+        const float si = hs[i];
+        const uint64_t ii = hi[i];
+        rank += (si > sj || (si == sj && (ii < ij || (ii == ij && i < j)))) ? 1 : 0;
+      }
+      out_s[q * k + rank] = sj;
+      out_i[q * k + rank] = ij;
+    }
+    for (int j = m + threadIdx.x; j < k; j += blockDim.x) {
+      out_s[q * k + j] = 0.f;
+      out_i[q * k + j] = 0u;
+    }
+  }
+}
+
 struct MergeLayout {
   size_t cs, ci, keep, os, oi, total;
 };
@@ -2293,6 +2518,39 @@ hipError_t launch_merge_results(std::size_t nq, int k_dst, float* dst_s, std::ui
   const int grid = grid_for((long)nq, 1, 65536);
   hipLaunchKernelGGL(merge_results_kernel, dim3(grid), dim3(256), lds, s, (long)nq, k_dst, dst_s, dst_i, k_src,
                      src_s, src_i);
+  return hipGetLastError();
+}
+
+hipError_t launch_bucket_scores(const std::uint8_t* tiled, const float* alpha, std::size_t n, int groups,
+                                int groups16, const std::uint8_t* luts, const float* lut_scale,
+                                const float* bias_total, std::size_t nq, float* p_out, hipStream_t s) {
+  if (n == 0 || nq == 0) return hipSuccess;
+  hipLaunchKernelGGL(bucket_scores_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)nq), dim3(256), 0, s,
+                     tiled, alpha, (long)n, groups, groups16, luts, lut_scale, bias_total, p_out);
+  return hipGetLastError();
+}
+
+std::size_t heap_lds_bytes(int k) { return (((std::size_t)k * 4 + 15) & ~std::size_t(15)) + (std::size_t)k * 8; }
+
+hipError_t launch_heap_replay(const float* p, std::size_t n, int k, const std::uint64_t* id_map,
+                              const float* offsets, std::size_t nq, float* hs, std::uint64_t* hi,
+                              std::uint64_t* hsz, float* hmin, std::uint64_t* hmi, hipStream_t s) {
+  if (n == 0 || nq == 0 || k == 0) return hipSuccess;
+  const size_t lds = heap_lds_bytes(k);
+  (void)hipFuncSetAttribute((const void*)heap_replay_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(heap_replay_kernel, dim3((unsigned)nq), dim3(64), lds, s, p, (long)n, k, id_map, offsets, hs,
+                     hi, hsz, hmin, hmi);
+  return hipGetLastError();
+}
+
+hipError_t launch_heaps_to_results(std::size_t nq, int k, const float* hs, const std::uint64_t* hi,
+                                   const std::uint64_t* hsz, float* out_s, std::uint64_t* out_i, hipStream_t s) {
+  if (nq == 0 || k == 0) return hipSuccess;
+  const size_t lds = heap_lds_bytes(k);
+  (void)hipFuncSetAttribute((const void*)heaps_to_results_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+  hipLaunchKernelGGL(heaps_to_results_kernel, dim3((unsigned)std::min<size_t>(nq, 65536)), dim3(256), lds, s,
+                     (long)nq, k, hs, hi, hsz, out_s, out_i);
   return hipGetLastError();
 }
 

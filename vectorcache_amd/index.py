@@ -263,6 +263,53 @@ def search_perm0(queries, k: int, bits: int, perm0, alpha, id_map=None, shift=No
     return SearchResults(scores[:, : kout.value], ids[:, : kout.value])
 
 
+def empty_heaps(nq: int, k: int):
+    """Fresh per-query heaps in the reference's layout: (heap_s [nq, k] f32,
+    heap_i [nq, k] u64, heap_sz [nq] u64, heap_min [nq] f32 = -inf,
+    heap_mi [nq] u64)."""
+    return (np.zeros((nq, k), np.float32), np.zeros((nq, k), np.uint64), np.zeros(nq, np.uint64),
+            np.full(nq, -np.inf, np.float32), np.zeros(nq, np.uint64))
+
+
+def score_perm0_into(queries, k: int, bits: int, perm0, alpha, query_indices, heaps, id_map=None, shift=None,
+                     scale=None, score_offsets=None):
+    """score_prepared_into (reference search.cpp:1156-1267) over one bucket of
+    PERM0 blocks + alpha: the listed queries' heaps (see empty_heaps) are
+    updated in place and returned."""
+    q = _f32(queries)
+    nq, dim = q.shape
+    alpha = _f32(alpha)
+    n = alpha.size
+    perm0 = np.ascontiguousarray(perm0, np.uint8)
+    qi = np.ascontiguousarray(query_indices, np.uint64)
+    hs, hi, hsz, hmin, hmi = heaps
+    for a, dt in ((hs, np.float32), (hi, np.uint64), (hsz, np.uint64), (hmin, np.float32), (hmi, np.uint64)):
+        if a.dtype != dt or not a.flags.c_contiguous:
+            raise ValueError("heaps must be C-contiguous arrays of the empty_heaps() types")
+    if hs.shape != (nq, k) or hi.shape != (nq, k) or hsz.shape != (nq,) or hmin.shape != (nq,) or hmi.shape != (nq,):
+        raise ValueError("heap shapes must be [nq, k] and [nq]")
+    im = None if id_map is None else np.ascontiguousarray(id_map, np.uint64)
+    sh = None if shift is None else _f32(shift)
+    sc = None if scale is None else _f32(scale)
+    off = None if score_offsets is None else _f32(score_offsets)
+    check(lib.vc_score_perm0_into(_ptr(q), S(nq), S(dim), S(bits), S(k), _ptr(perm0), _ptr(alpha), S(n), _ptr(im),
+                                  _ptr(sh), _ptr(sc), _ptr(qi), S(qi.size), _ptr(hs), _ptr(hi), _ptr(hsz), _ptr(hmin),
+                                  _ptr(hmi), _ptr(off)))
+    return heaps
+
+
+def heaps_to_results(heaps) -> SearchResults:
+    """heaps_to_search_results (reference search.cpp:1269-1291) on the GPU."""
+    hs, hi, hsz = heaps[0], heaps[1], heaps[2]
+    nq, k = hs.shape
+    scores = np.zeros((nq, k), np.float32)
+    ids = np.zeros((nq, k), np.uint64)
+    check(lib.vc_heaps_to_results(S(nq), S(k), _ptr(np.ascontiguousarray(hs, np.float32)),
+                                  _ptr(np.ascontiguousarray(hi, np.uint64)), _ptr(np.ascontiguousarray(hsz, np.uint64)),
+                                  _ptr(scores), _ptr(ids)))
+    return SearchResults(scores, ids)
+
+
 def merge_search_results(dst: SearchResults, src: SearchResults) -> SearchResults:
     """merge_search_results (reference search.cpp:1551-1600) on the GPU."""
     if src.nq == 0 or src.k == 0:
