@@ -173,6 +173,20 @@ vc_status vc_score_perm0_into(const float* queries, size_t nq, size_t dim, size_
 vc_status vc_heaps_to_results(size_t nq, size_t k, const float* heap_s, const uint64_t* heap_i,
                               const size_t* heap_sz, float* scores, uint64_t* ids);
 
+/* float_rescore_topk (src/search/float_rescore.cpp:80-164): each of the nq
+ * result rows (scores / ids, nq x k) re-scored in place with the float inner
+ * product of its query (nq x dim) and the corpus row of each id (n_corpus x
+ * dim; -inf for an id >= n_corpus), summed in the reference's AVX2 order, then
+ * its first min(k, counts[q]) entries (k when counts is NULL) ordered by
+ * (score desc, id asc); entries past that are left as they are. k <= 4096. */
+vc_status vc_float_rescore(size_t nq, size_t k, float* scores, uint64_t* ids, const size_t* counts,
+                           const float* queries, const float* corpus, size_t n_corpus, size_t dim);
+/* Same on device buffers (counts_dev: u64 or NULL), ordered on `stream`
+ * (NULL: the default stream); the call returns once the work is enqueued. */
+vc_status vc_float_rescore_device(size_t nq, size_t k, float* scores_dev, uint64_t* ids_dev,
+                                  const uint64_t* counts_dev, const float* queries_dev, const float* corpus_dev,
+                                  size_t n_corpus, size_t dim, void* stream);
+
 /* merge_search_results (src/search/search.cpp:1551-1600) on the GPU:
  * dst (nq x k_dst) <- top k_dst of dst U src by (score desc, id asc), a
  * duplicated id keeping its higher score, short rows padded with (0, 0). */

@@ -989,3 +989,49 @@ int oc_heaps_to_results(size_t nq, size_t k, const float* heap_s, const uint64_t
   free(c);
   return 0;
 }
+
+/* float_rescore_topk (reference src/search/float_rescore.cpp:80-164): each
+ * row's first n = min(k, counts[q]) (or k) candidates get the float inner
+ * product of the query with the corpus row (-inf for an id >= n_corpus), in
+ * the reference's AVX2 order (:36-57): 8-lane accumulators acc0 / acc1 over
+ * 16-float steps (fused), acc0 + acc1, an 8-float fused tail, then the
+ * horizontal sum ((v0+v4)+(v1+v5)) + ((v2+v6)+(v3+v7)) (:24-33); then the n
+ * entries are ordered by (score desc, id asc). Entries past n are untouched. */
+static float oc_dot_avx2_order(const float* a, const float* b, size_t dim) {
+  float acc0[8] = {0}, acc1[8] = {0};
+  size_t i = 0;
+  for (; i + 16 <= dim; i += 16)
+    for (int l = 0; l < 8; ++l) {
+      acc0[l] = fmaf(a[i + l], b[i + l], acc0[l]);
+      acc1[l] = fmaf(a[i + 8 + l], b[i + 8 + l], acc1[l]);
+    }
+  for (int l = 0; l < 8; ++l) acc0[l] = acc0[l] + acc1[l];
+  for (; i + 8 <= dim; i += 8)
+    for (int l = 0; l < 8; ++l) acc0[l] = fmaf(a[i + l], b[i + l], acc0[l]);
+  float lo[4];
+  for (int l = 0; l < 4; ++l) lo[l] = acc0[l] + acc0[l + 4];
+  float s = (lo[0] + lo[1]) + (lo[2] + lo[3]);
+  for (; i < dim; ++i) s = fmaf(a[i], b[i], s); /* dim % 8 != 0 only */
+  return s;
+}
+
+int oc_float_rescore(size_t nq, size_t k, float* scores, uint64_t* ids, const size_t* counts,
+                     const float* queries, const float* corpus, size_t n_corpus, size_t dim) {
+  if (nq == 0 || k == 0 || dim == 0) return 0;
+  oc_cand* c = (oc_cand*)malloc(k * sizeof(oc_cand));
+  for (size_t q = 0; q < nq; ++q) {
+    const size_t n = counts ? (counts[q] < k ? counts[q] : k) : k;
+    for (size_t j = 0; j < n; ++j) {
+      const uint64_t id = ids[q * k + j];
+      c[j].id = id;
+      c[j].s = id < n_corpus ? oc_dot_avx2_order(queries + q * dim, corpus + id * dim, dim) : -INFINITY;
+    }
+    qsort(c, n, sizeof(oc_cand), cand_cmp);
+    for (size_t j = 0; j < n; ++j) {
+      scores[q * k + j] = c[j].s;
+      ids[q * k + j] = c[j].id;
+    }
+  }
+  free(c);
+  return 0;
+}

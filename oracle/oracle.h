@@ -48,6 +48,8 @@ int oc_score_into(const float* q, size_t nq, size_t dim, size_t k, size_t bits,
                   size_t* heap_sz, float* heap_min, size_t* heap_mi, const float* offsets);
 int oc_heaps_to_results(size_t nq, size_t k, const float* heap_s, const uint64_t* heap_i,
                         const size_t* heap_sz, float* scores_out, uint64_t* ids_out);
+int oc_float_rescore(size_t nq, size_t k, float* scores, uint64_t* ids, const size_t* counts,
+                     const float* queries, const float* corpus, size_t n_corpus, size_t dim);
 int oc_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t* dst_ids, size_t k_src,
                      const float* src_scores, const uint64_t* src_ids);
 void oc_set_threads(int n);

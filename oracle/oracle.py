@@ -284,6 +284,19 @@ class Oracle:
                                                 _ptr(ids)), "heaps_to_results")
         return scores, ids
 
+    def float_rescore(self, scores, ids, queries, corpus, counts=None):
+        """float_rescore_topk: returns re-scored, re-ordered copies of the rows."""
+        sc = np.ascontiguousarray(scores, np.float32).copy()
+        ii = np.ascontiguousarray(ids, np.uint64).copy()
+        nq, k = sc.shape
+        q = np.ascontiguousarray(queries, np.float32)
+        cor = np.ascontiguousarray(corpus, np.float32)
+        dim = q.shape[1]
+        cnt = None if counts is None else np.ascontiguousarray(counts, np.uint64)
+        self._check(self._f("float_rescore")(_S(nq), _S(k), _ptr(sc), _ptr(ii), _ptr(cnt), _ptr(q), _ptr(cor),
+                                             _S(cor.shape[0]), _S(dim)), "float_rescore")
+        return sc, ii
+
 
 def available(kind: str) -> bool:
     if kind == "port":

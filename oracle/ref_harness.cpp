@@ -19,6 +19,7 @@
 #include "vectorcache/encode/encode.hpp"
 #include "vectorcache/pack/pack.hpp"
 #include "vectorcache/quantize/codebook.hpp"
+#include "vectorcache/search/float_rescore.hpp"
 #include "vectorcache/search/search.hpp"
 #include "vectorcache/transform/rotation.hpp"
 
@@ -316,6 +317,26 @@ int vcref_heaps_to_results(size_t nq, size_t k, const float* heap_s, const uint6
     vc::heaps_to_search_results(out, nq, k, heap_s, heap_i, heap_sz);
     std::memcpy(scores_out, out.scores.data(), nq * k * sizeof(float));
     std::memcpy(ids_out, out.ids.data(), nq * k * sizeof(uint64_t));
+  });
+}
+
+// float_rescore_topk() (reference src/search/float_rescore.cpp:80-164): the
+// nq x k rows re-scored in place; counts may be null (every row has k hits).
+int vcref_float_rescore(size_t nq, size_t k, float* scores, uint64_t* ids, const size_t* counts,
+                        const float* queries, const float* corpus, size_t n_corpus, size_t dim) {
+  return guarded([&] {
+    vc::SearchResults out;
+    out.nq = nq;
+    out.k = k;
+    out.scores.resize_uninitialized(nq * k);
+    out.ids.resize_uninitialized(nq * k);
+    std::memcpy(out.scores.data(), scores, nq * k * sizeof(float));
+    std::memcpy(out.ids.data(), ids, nq * k * sizeof(uint64_t));
+    if (counts != nullptr) out.counts.assign(counts, counts + nq);
+    vc::float_rescore_topk(out, std::span<const float>(queries, nq * dim),
+                           std::span<const float>(corpus, n_corpus * dim), dim);
+    std::memcpy(scores, out.scores.data(), nq * k * sizeof(float));
+    std::memcpy(ids, out.ids.data(), nq * k * sizeof(uint64_t));
   });
 }
 

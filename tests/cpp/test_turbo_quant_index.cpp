@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <limits>
 #include <random>
 #include <span>
 #include <stdexcept>
@@ -99,6 +100,39 @@ int main() {
   vectorcache::merge_search_results(dst, src);
   expect(dst.k == 5, "merge keeps dst.k");
   for (std::size_t j = 0; j < 5; ++j) expect(dst.ids[j] == src.ids[j], "merge of overlapping rows");
+
+  // score_prepared_into over the index's own codes as one bucket, into empty
+  // heaps (threshold -inf): ids increase with the scan order, so the heaps
+  // hold the exact top-k -- the same rows as search()
+  {
+    const std::size_t G = dim / 2, nb = (n + 191) / 192;
+    std::vector<std::uint8_t> perm0(nb * G * 192);
+    std::vector<float> alpha(n);
+    expect(vc_index_export_perm0(index.handle(), perm0.data()) == VC_OK, "export perm0");
+    expect(vc_index_export_alpha(index.handle(), alpha.data()) == VC_OK, "export alpha");
+    for (std::size_t kk : {std::size_t(7), std::size_t(200)}) {
+      std::vector<float> hs(nq * kk), hmin(nq, -std::numeric_limits<float>::infinity());
+      std::vector<std::uint64_t> hi(nq * kk);
+      std::vector<std::size_t> hsz(nq, 0), hmi(nq, 0), qidx(nq);
+      for (std::size_t q = 0; q < nq; ++q) qidx[q] = nq - 1 - q;
+      vectorcache::score_perm0_into(queries, dim, 4, kk, perm0, alpha, {}, qidx, hs.data(), hi.data(), hsz.data(),
+                                    hmin.data(), hmi.data());
+      SearchResults hr;
+      vectorcache::heaps_to_search_results(hr, nq, kk, hs.data(), hi.data(), hsz.data());
+      SearchResults sr = index.search(queries, kk);
+      expect(hr.ids == sr.ids && hr.scores == sr.scores, "score_prepared_into into empty heaps == search");
+      std::vector<std::size_t> none;
+      expect_throw<std::invalid_argument>([&] {
+        vectorcache::score_perm0_into(queries, dim, 4, kk, perm0, alpha, {}, none, hs.data(), hi.data(), hsz.data(),
+                                      hmin.data(), hmi.data());
+      }, "score_prepared_into: empty query_indices");
+      std::vector<std::size_t> bad{nq};
+      expect_throw<std::out_of_range>([&] {
+        vectorcache::score_perm0_into(queries, dim, 4, kk, perm0, alpha, {}, bad, hs.data(), hi.data(), hsz.data(),
+                                      hmin.data(), hmi.data());
+      }, "score_prepared_into: query index out of range");
+    }
+  }
 
   // TQ+ calibrated index
   TurboQuantIndex tq(dim, 2);

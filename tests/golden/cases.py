@@ -23,3 +23,19 @@ def case_data(bits: int, dim: int, n: int, idx: int):
     q[4] = 0.0                                  # zero query
     sample = synthetic.mixture(400, dim, 16, 0.3, 7000 + idx, 7300 + idx)
     return x, q, sample
+
+
+RESCORE_DIMS = (8, 24, 128, 1536)
+
+
+def rescore_inputs(dim):
+    """Seeded float_rescore_topk inputs: corpus (300 x dim), 6 queries, rows of
+    12 candidates with ids past the corpus and a repeated id, per-row counts."""
+    r = np.random.default_rng(6100 + dim)
+    corpus = r.standard_normal((300, dim)).astype(np.float32)
+    q = r.standard_normal((6, dim)).astype(np.float32)
+    ids = r.integers(0, 320, (6, 12)).astype(np.uint64)
+    ids[1, 3] = ids[1, 4]
+    scores = r.standard_normal((6, 12)).astype(np.float32)
+    counts = np.array([12, 5, 0, 12, 7, 1], np.uint64)
+    return corpus, q, scores, ids, counts

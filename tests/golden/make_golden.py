@@ -5,13 +5,14 @@ them with oracle/Makefile and loads it); the outputs are data only:
 inputs plus the reference's outputs for the plain flat TurboQuant path
 (rotation tables, codebooks, encode, TQ+ calibration, flat bytes, PERM0
 blocks, per-query LUTs, search_flat top-k, id_map scoring, result merging,
-score_prepared_into over chained buckets), run on the reference's AVX2 PERM0
-backend (TURBOVEC_NO_VNNI=1).
+score_prepared_into over chained buckets, float_rescore_topk), run on the
+reference's AVX2 PERM0 backend (TURBOVEC_NO_VNNI=1).
 
 Large outputs are stored as SHA-256 digests of their little-endian bytes.
 
     python tests/golden/make_golden.py              # everything
     python tests/golden/make_golden.py score_into   # score_into.npz only
+    python tests/golden/make_golden.py rescore      # float_rescore.npz only
 """
 from __future__ import annotations
 
@@ -30,7 +31,7 @@ sys.path.insert(0, str(Path(__file__).resolve().parent))
 
 from oracle import Oracle  # noqa: E402
 from vectorcache_amd import synthetic  # noqa: E402
-from cases import CASES, KS, case_data  # noqa: E402
+from cases import CASES, KS, RESCORE_DIMS, case_data, rescore_inputs  # noqa: E402
 
 OUT = Path(__file__).resolve().parent
 
@@ -116,12 +117,27 @@ def score_into_fixture(R):
     np.savez_compressed(OUT / "score_into.npz", **out)
 
 
+def rescore_fixture(R):
+    out = {}
+    for dim in RESCORE_DIMS:
+        corpus, q, scores, ids, counts = rescore_inputs(dim)
+        for tag, cnt in (("all", None), ("counts", counts)):
+            s, i = R.float_rescore(scores, ids, q, corpus, cnt)
+            out[f"d{dim}_{tag}_scores"] = s
+            out[f"d{dim}_{tag}_ids"] = i
+    np.savez_compressed(OUT / "float_rescore.npz", **out)
+
+
 def main():
     subprocess.run(["make", "-C", str(ROOT / "oracle"), "ref"], check=True, stdout=subprocess.DEVNULL)
     R = Oracle("reference")
-    score_into_fixture(R)
-    if len(sys.argv) > 1 and sys.argv[1] == "score_into":
-        print("wrote score_into.npz")
+    only = sys.argv[1] if len(sys.argv) > 1 else None
+    if only in (None, "score_into"):
+        score_into_fixture(R)
+    if only in (None, "rescore"):
+        rescore_fixture(R)
+    if only is not None:
+        print("wrote", only)
         return
 
     rot = {}

@@ -318,3 +318,34 @@ def test_score_into_vs_reference_build(oracle, ref):
             hr = ref.score_into(q, k, bits, perm0, alpha[a:b], qidx, hr, id_map=idm, offsets=off)
             ho = oracle.score_into(q, k, bits, perm0, alpha[a:b], qidx, ho, id_map=idm, offsets=off)
             assert heaps_equal(hr, ho), (bits, dim, k, ci)
+
+
+# ------------------------------------------------------------ float re-rank
+def test_float_rescore_golden(oracle):
+    """float_rescore_topk: every dim's rows (all entries, and per-row counts)
+    equal the reference's, bit for bit; ids past the corpus score -inf."""
+    from cases import RESCORE_DIMS, rescore_inputs
+
+    g = load_golden("float_rescore.npz")
+    for dim in RESCORE_DIMS:
+        corpus, q, scores, ids, counts = rescore_inputs(dim)
+        for tag, cnt in (("all", None), ("counts", counts)):
+            s, i = oracle.float_rescore(scores, ids, q, corpus, cnt)
+            assert bits_equal(s, g[f"d{dim}_{tag}_scores"]) and np.array_equal(i, g[f"d{dim}_{tag}_ids"]), (dim, tag)
+
+
+def test_float_rescore_is_the_inner_product(oracle):
+    """Each re-scored entry is the float inner product to within the error of
+    a float sum of dim products, and rows come out sorted (score desc, id asc)."""
+    r = np.random.default_rng(8)
+    dim = 1536
+    corpus = r.standard_normal((200, dim)).astype(np.float32)
+    q = r.standard_normal((4, dim)).astype(np.float32)
+    ids = r.integers(0, 200, (4, 20)).astype(np.uint64)
+    s, i = oracle.float_rescore(np.zeros((4, 20), np.float32), ids, q, corpus)
+    exact = np.einsum("qd,qkd->qk", q.astype(np.float64), corpus[i.astype(np.int64)].astype(np.float64))
+    bound = dim * 2.0 ** -23 * np.einsum("qd,qkd->qk", np.abs(q), np.abs(corpus[i.astype(np.int64)]))
+    assert np.all(np.abs(s - exact) <= bound)
+    for row_s, row_i in zip(s, i):
+        order = np.lexsort((row_i, -row_s))
+        assert np.array_equal(order, np.arange(20))

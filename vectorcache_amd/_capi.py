@@ -70,6 +70,8 @@ _SIGS = {
     "vc_merge_results": ([S, S, P, P, S, P, P], I),
     "vc_score_perm0_into": ([P, S, S, S, S, P, P, S, P, P, P, P, S, P, P, P, P, P, P], I),
     "vc_heaps_to_results": ([S, S, P, P, P, P, P], I),
+    "vc_float_rescore": ([S, S, P, P, P, P, P, S, S], I),
+    "vc_float_rescore_device": ([S, S, P, P, P, P, P, S, S, P], I),
     "vc_set_scan_tile": ([S], I),
     "vc_scan_tile_for": ([S, S, S], S),
     "vc_set_profiling": ([I], I),

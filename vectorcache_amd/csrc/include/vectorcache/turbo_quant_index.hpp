@@ -8,7 +8,9 @@
 //    (README.md:105-111; fit_calibration semantics, encode.hpp:33-35);
 //  * SearchResults: row-major nq x k scores/ids, rows sorted by
 //    (score desc, id asc) (reference include/vectorcache/search/search.hpp:77-84);
-//  * merge_search_results (search.hpp:191-192).
+//  * merge_search_results (search.hpp:191-192);
+//  * heaps_to_search_results (search.hpp:186-189) and score_perm0_into, the
+//    score_prepared_into of one bucket given as PERM0 blocks (search.hpp:163-177).
 // Exceptions follow the reference: std::invalid_argument for bad dim/bits or
 // a span that is not a multiple of dim, std::runtime_error for search before
 // prepare and for device failures, std::out_of_range for index errors.
@@ -115,6 +117,44 @@ inline void merge_search_results(SearchResults& dst, const SearchResults& src) {
   if (dst.nq != src.nq) throw std::invalid_argument("merge_search_results: nq mismatch");
   detail::throw_on(vc_merge_results(dst.nq, dst.k, dst.scores.data(), dst.ids.data(), src.k, src.scores.data(),
                                     src.ids.data()));
+}
+
+// heaps_to_search_results (reference search.hpp:186-189): each heap's
+// heap_sz[q] entries sorted by (score desc, id asc) into row q of out; the rest
+// of the row is (0, 0).
+inline void heaps_to_search_results(SearchResults& out, std::size_t nq, std::size_t k, const float* heap_s,
+                                    const std::uint64_t* heap_i, const std::size_t* heap_sz) {
+  out.nq = nq;
+  out.k = k;
+  out.scores.assign(nq * k, 0.f);
+  out.ids.assign(nq * k, 0);
+  out.counts.clear();
+  if (nq == 0 || k == 0) return;
+  detail::throw_on(vc_heaps_to_results(nq, k, heap_s, heap_i, heap_sz, out.scores.data(), out.ids.data()));
+}
+
+// score_prepared_into (reference search.hpp:163-177) for one bucket given as
+// PERM0 blocks + per-vector scales: `queries` (nq x dim rows; TQ+ shift/scale
+// or both empty) are the batch the reference's PreparedQueries would be built
+// from. Only query_indices are scored, each into its heap (heap_s / heap_i
+// [nq][k], heap_sz / heap_min / heap_mi [nq]), updated in place as the
+// reference updates them; score_offsets (length nq) is added after alpha x LUT.
+// Throws as the reference: std::invalid_argument for empty query_indices or
+// an id_map of the wrong size, std::out_of_range for an index >= nq.
+inline void score_perm0_into(std::span<const float> queries, std::size_t dim, std::size_t bits, std::size_t k,
+                             std::span<const std::uint8_t> perm0, std::span<const float> scales,
+                             std::span<const std::uint64_t> id_map, std::span<const std::size_t> query_indices,
+                             float* heap_s, std::uint64_t* heap_i, std::size_t* heap_sz, float* heap_min,
+                             std::size_t* heap_mi, const float* score_offsets = nullptr,
+                             std::span<const float> shift = {}, std::span<const float> scale = {}) {
+  if (dim == 0 || queries.size() % dim != 0) throw std::invalid_argument("queries size must be a multiple of dim");
+  if (!id_map.empty() && id_map.size() != scales.size())
+    throw std::invalid_argument("id_map size must equal number of vectors in range");
+  detail::throw_on(vc_score_perm0_into(queries.data(), queries.size() / dim, dim, bits, k, perm0.data(),
+                                       scales.data(), scales.size(), id_map.empty() ? nullptr : id_map.data(),
+                                       shift.empty() ? nullptr : shift.data(), scale.empty() ? nullptr : scale.data(),
+                                       query_indices.data(), query_indices.size(), heap_s, heap_i, heap_sz, heap_min,
+                                       heap_mi, score_offsets));
 }
 
 }  // namespace vectorcache

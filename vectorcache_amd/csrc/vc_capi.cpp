@@ -898,6 +898,48 @@ vc_status vc_heaps_to_results(size_t nq, size_t k, const float* heap_s, const ui
   });
 }
 
+vc_status vc_float_rescore(size_t nq, size_t k, float* scores, uint64_t* ids, const size_t* counts,
+                           const float* queries, const float* corpus, size_t n_corpus, size_t dim) {
+  return guarded([&] {
+    if (nq == 0 || k == 0 || dim == 0) return;
+    if (!scores || !ids || !queries || (!corpus && n_corpus != 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    if (k > size_t(vchip::kMaxHeapK)) fail(VC_ERR_INVALID_ARGUMENT, "k exceeds the GPU capacity of 4096");
+    Scoped sc;
+    Carve c;
+    const size_t os = c.take(nq * k * 4), oi = c.take(nq * k * 8), oc = c.take(counts ? nq * 8 : 0),
+                 oq = c.take(nq * dim * 4), ox = c.take(n_corpus * dim * 4);
+    sc.a.reserve(c.off, sc.s);
+    std::vector<uint64_t> cnt;
+    if (counts) cnt.assign(counts, counts + nq);
+    hip_check(hipMemcpyAsync(sc.a.as<float>(os), scores, nq * k * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(hipMemcpyAsync(sc.a.as<uint64_t>(oi), ids, nq * k * 8, hipMemcpyHostToDevice, sc.s), "copy");
+    if (counts) hip_check(hipMemcpyAsync(sc.a.as<uint64_t>(oc), cnt.data(), nq * 8, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(hipMemcpyAsync(sc.a.as<float>(oq), queries, nq * dim * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    if (n_corpus) hip_check(hipMemcpyAsync(sc.a.as<float>(ox), corpus, n_corpus * dim * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(vchip::launch_float_rescore(nq, int(k), sc.a.as<float>(os), sc.a.as<uint64_t>(oi),
+                                          counts ? sc.a.as<uint64_t>(oc) : nullptr, sc.a.as<float>(oq),
+                                          sc.a.as<float>(ox), n_corpus, int(dim), sc.s),
+              "float_rescore");
+    hip_check(hipMemcpyAsync(scores, sc.a.as<float>(os), nq * k * 4, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipMemcpyAsync(ids, sc.a.as<uint64_t>(oi), nq * k * 8, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipStreamSynchronize(sc.s), "sync");
+  });
+}
+
+vc_status vc_float_rescore_device(size_t nq, size_t k, float* scores_dev, uint64_t* ids_dev,
+                                  const uint64_t* counts_dev, const float* queries_dev, const float* corpus_dev,
+                                  size_t n_corpus, size_t dim, void* stream) {
+  return guarded([&] {
+    if (nq == 0 || k == 0 || dim == 0) return;
+    if (!scores_dev || !ids_dev || !queries_dev || (!corpus_dev && n_corpus != 0))
+      fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    if (k > size_t(vchip::kMaxHeapK)) fail(VC_ERR_INVALID_ARGUMENT, "k exceeds the GPU capacity of 4096");
+    hip_check(vchip::launch_float_rescore(nq, int(k), scores_dev, ids_dev, counts_dev, queries_dev, corpus_dev,
+                                          n_corpus, int(dim), static_cast<hipStream_t>(stream)),
+              "float_rescore");
+  });
+}
+
 vc_status vc_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t* dst_ids, size_t k_src,
                            const float* src_scores, const uint64_t* src_ids) {
   return guarded([&] {

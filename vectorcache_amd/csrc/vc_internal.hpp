@@ -105,6 +105,10 @@ hipError_t launch_heap_replay(const float* p, std::size_t n, int k, const std::u
                               std::uint64_t* hsz, float* hmin, std::uint64_t* hmi, hipStream_t s);
 hipError_t launch_heaps_to_results(std::size_t nq, int k, const float* hs, const std::uint64_t* hi,
                                    const std::uint64_t* hsz, float* out_s, std::uint64_t* out_i, hipStream_t s);
+// float_rescore_topk on device rows: scores / ids [nq][k], counts [nq] or null.
+hipError_t launch_float_rescore(std::size_t nq, int k, float* scores, std::uint64_t* ids,
+                                const std::uint64_t* counts, const float* queries, const float* corpus,
+                                std::size_t n_corpus, int dim, hipStream_t s);
 hipError_t launch_merge_results(std::size_t nq, int k_dst, float* dst_s, std::uint64_t* dst_i,
                                 int k_src, const float* src_s, const std::uint64_t* src_i,
                                 hipStream_t s);

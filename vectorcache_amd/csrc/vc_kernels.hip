@@ -2051,6 +2051,66 @@ __global__ __launch_bounds__(256) void heaps_to_results_kernel(long nq, int k, c
   }
 }
 
+// float_rescore_topk (src/search/float_rescore.cpp:80-164): one workgroup
+// per query row. Each candidate is scored by 16 lanes in the reference's AVX2
+// order (:36-57): lanes 0-7 are acc0 and 8-15 acc1 over 16-float steps
+// (fused), then acc0 + acc1, the 8-float fused tail on acc0, and the
+// horizontal sum ((v0+v4)+(v1+v5)) + ((v2+v6)+(v3+v7)) (:24-33); -inf for an id
+// outside the corpus. The row's first n entries are then ordered by
+// (score desc, id asc) by rank, entries past n left as they are.
+__global__ __launch_bounds__(256) void float_rescore_kernel(long nq, int k, float* __restrict__ scores,
+                                                            uint64_t* __restrict__ ids,
+                                                            const uint64_t* __restrict__ counts,
+                                                            const float* __restrict__ queries,
+                                                            const float* __restrict__ corpus, long n_corpus,
+                                                            int dim) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float* cs = reinterpret_cast<float*>(smem);
+  uint64_t* ci = reinterpret_cast<uint64_t*>(smem + (((size_t)k * 4 + 15) & ~size_t(15)));
+  const int tid = threadIdx.x, l = tid & 15;
+  for (long q = blockIdx.x; q < nq; q += gridDim.x) {
+    const int n = counts != nullptr ? (int)min((uint64_t)k, counts[q]) : k;
+    const float* qa = queries + q * dim;
+    __syncthreads();
+    for (int j = tid >> 4; j < n; j += blockDim.x >> 4) {  // the 16 lanes of a group share j
+      const uint64_t id = ids[q * k + j];
+      float s = -INFINITY;
+      if (id < (uint64_t)n_corpus) {
+        const float* b = corpus + (long)id * dim;
+        float acc = 0.f;
+        int i = 0;
+        for (; i + 16 <= dim; i += 16) acc = __fmaf_rn(qa[i + l], b[i + l], acc);
+        const float a1 = __shfl_down(acc, 8, 16);
+        if (l < 8) acc = acc + a1;
+        for (; i + 8 <= dim; i += 8)
+          if (l < 8) acc = __fmaf_rn(qa[i + l], b[i + l], acc);
+        const float a4 = __shfl_down(acc, 4, 16);
+        const float lo = acc + a4;  // lanes 0-3: v_l + v_(l+4)
+        const float lo1 = __shfl_down(lo, 1, 16), lo2 = __shfl_down(lo, 2, 16), lo3 = __shfl_down(lo, 3, 16);
+        s = (lo + lo1) + (lo2 + lo3);  // lane 0
+        for (; i < dim; ++i) s = __fmaf_rn(qa[i], b[i], s);
+      }
+      if (l == 0) {
+        cs[j] = s;
+        ci[j] = id;
+      }
+    }
+    __syncthreads();
+    for (int j = tid; j < n; j += blockDim.x) {
+      const float sj = cs[j];
+      const uint64_t ij = ci[j];
+      int rank = 0;
+      for (int i = 0; i < n; ++i) {
+        const float si = cs[i];
+        const uint64_t ii = ci[i];
+        rank += (si > sj || (si == sj && (ii < ij || (ii == ij && i < j)))) ? 1 : 0;
+      }
+      scores[q * k + rank] = sj;
+      ids[q * k + rank] = ij;
+    }
+  }
+}
+
 struct MergeLayout {
   size_t cs, ci, keep, os, oi, total;
 };
@@ -2551,6 +2611,17 @@ hipError_t launch_heaps_to_results(std::size_t nq, int k, const float* hs, const
                             (int)lds);
   hipLaunchKernelGGL(heaps_to_results_kernel, dim3((unsigned)std::min<size_t>(nq, 65536)), dim3(256), lds, s,
                      (long)nq, k, hs, hi, hsz, out_s, out_i);
+  return hipGetLastError();
+}
+
+hipError_t launch_float_rescore(std::size_t nq, int k, float* scores, std::uint64_t* ids,
+                                const std::uint64_t* counts, const float* queries, const float* corpus,
+                                std::size_t n_corpus, int dim, hipStream_t s) {
+  if (nq == 0 || k == 0 || dim == 0) return hipSuccess;
+  const size_t lds = heap_lds_bytes(k);
+  (void)hipFuncSetAttribute((const void*)float_rescore_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(float_rescore_kernel, dim3((unsigned)std::min<size_t>(nq, 65536)), dim3(256), lds, s, (long)nq,
+                     k, scores, ids, counts, queries, corpus, (long)n_corpus, dim);
   return hipGetLastError();
 }
 

@@ -310,6 +310,35 @@ def heaps_to_results(heaps) -> SearchResults:
     return SearchResults(scores, ids)
 
 
+def float_rescore_topk(results: SearchResults, queries, corpus, counts=None) -> SearchResults:
+    """float_rescore_topk (reference float_rescore.cpp:80-164) on the GPU:
+    re-scored, re-ordered copies of the rows. With torch CUDA tensors
+    (scores f32, ids int64, queries, corpus) it runs in place on the device,
+    ordered on torch's current stream."""
+    if _is_torch(results.scores) and results.scores.is_cuda:
+        import torch
+
+        sc, ii = results.scores, results.ids
+        nq, k = sc.shape
+        q = queries.contiguous().float()
+        cor = corpus.contiguous().float()
+        cnt = None if counts is None else counts.contiguous().to(torch.int64)
+        st = torch.cuda.current_stream().cuda_stream
+        check(lib.vc_float_rescore_device(S(nq), S(k), P(sc.data_ptr()), P(ii.data_ptr()),
+                                          None if cnt is None else P(cnt.data_ptr()), P(q.data_ptr()),
+                                          P(cor.data_ptr()), S(cor.shape[0]), S(q.shape[1]), P(st)))
+        return results
+    sc = np.ascontiguousarray(results.scores, np.float32).copy()
+    ii = np.ascontiguousarray(results.ids, np.uint64).copy()
+    nq, k = sc.shape
+    q = _f32(queries)
+    cor = _f32(corpus)
+    cnt = None if counts is None else np.ascontiguousarray(counts, np.uint64)
+    check(lib.vc_float_rescore(S(nq), S(k), _ptr(sc), _ptr(ii), _ptr(cnt), _ptr(q), _ptr(cor), S(cor.shape[0]),
+                               S(q.shape[1])))
+    return SearchResults(sc, ii)
+
+
 def merge_search_results(dst: SearchResults, src: SearchResults) -> SearchResults:
     """merge_search_results (reference search.cpp:1551-1600) on the GPU."""
     if src.nq == 0 or src.k == 0:
