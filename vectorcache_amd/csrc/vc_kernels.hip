@@ -73,7 +73,7 @@ __device__ float norm8_lds(const float* __restrict__ x, int D, int lane) {
     for (int t0 = 0; t0 < steps; t0 += kAhead) {
       float v[kAhead];
 #pragma unroll
-      for (int u = 0; u < kAhead; ++u) v[u] = t0 + u < steps ? x[lane + 8 * (t0 + u)] : 0.f;
+      for (int u = 0; u < kAhead; ++u) v[u] = x[lane + 8 * min(t0 + u, steps - 1)];  // in bounds, no branch
 #pragma unroll
       for (int u = 0; u < kAhead; ++u)
         if (t0 + u < steps) c = __fmaf_rn(v[u], v[u], c);
@@ -805,6 +805,43 @@ __device__ __forceinline__ void stage_to_lds(float* __restrict__ dst, const floa
   }
 }
 
+// The 16 + 16 LUT values of group g (lut_value order: the fused products of
+// the group's rotated coordinates with the codebook, src/search/search.cpp:
+// 149-219), the codebook in registers.
+template <int CPN>
+__device__ __forceinline__ void group_values(const float* __restrict__ y, const float (&cr)[16], int bits, int g,
+                                             float (&sv)[2][16]) {
+  const unsigned mask = (1u << bits) - 1u;
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const int base = g * (2 * CPN) + half * CPN;
+    float yv[CPN];
+#pragma unroll
+    for (int c = 0; c < CPN; ++c) yv[c] = y[base + c];
+#pragma unroll
+    for (int nib = 0; nib < 16; ++nib) {
+      float acc = 0.f;
+#pragma unroll
+      for (int c = 0; c < CPN; ++c) acc = __fmaf_rn(yv[c], cr[(nib >> ((CPN - 1 - c) * bits)) & mask], acc);
+      sv[half][nib] = acc;
+    }
+  }
+}
+
+// Query rotation by one wavefront per rotation block (the encode kernel's
+// rotate_block: gather, in-lane and cross-lane butterflies, 1/sqrt(B)).
+template <int R>
+__device__ __forceinline__ void rotate_query_blocks(const CodeTables& t, const float* __restrict__ xs, float inv,
+                                                    float* __restrict__ yout, int wave, int waves, int lane) {
+  const int nblk = t.dim / t.block;
+  for (int b = wave; b < nblk; b += waves) {
+    float v[R];
+    rotate_block<R>(t, t.perm_gather, xs, inv, b, lane, v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) yout[b * t.block + lane * R + r] = v[r];
+  }
+}
+
 __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTables t, const float* __restrict__ q,
                                                                       long nq, uint8_t* __restrict__ luts,
                                                                       float* __restrict__ lut_scale,
@@ -814,19 +851,24 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTable
   extern __shared__ __align__(16) unsigned char smem[];
   const int D = t.dim, G = t.groups, B = t.block;
   const bool tq = t.shift != nullptr;
-  float* y = reinterpret_cast<float*>(smem);   // D: rotated query
+  float* y = reinterpret_cast<float*>(smem);   // D: rotated query (divided by the TQ+ scale)
   float* xs = y + D;                            // D: the staged query row, then y before the TQ+ division
   float* mins = xs + D;                         // 2 * G
   float* tsh = mins + 2 * G;                    // D (TQ+)
   float* tsc = tsh + D;                         // D (TQ+)
-  __shared__ float s_norm, s_bsum, s_bc, s_span[kPrepThreads / 64], s_cent[16];
-  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ float s_norm, s_bsum, s_bc, s_span[kPrepThreads / 64];
+  __shared__ double s_red[2][kPrepThreads / 64];
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, waves = nt >> 6;
   const int cpb = 8 / t.bits, cpn = cpb / 2;
-  if (tid < (1 << t.bits)) s_cent[tid] = t.centroids[tid];
+  float cr[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) cr[i] = t.centroids[i & ((1 << t.bits) - 1)];
   if (tq) {
     stage_to_lds(tsh, t.shift, D);
     stage_to_lds(tsc, t.scale, D);
   }
+  const int R = B / 64;
+  const bool wave_rot = t.perm_gather != nullptr && (R == 2 || R == 4 || R == 8 || R == 16);
   for (long qi = blockIdx.x; qi < nq; qi += gridDim.x) {
     const float* xr = q + qi * (long)D;
     __syncthreads();
@@ -839,52 +881,103 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTable
     __syncthreads();
     const float nrm = s_norm;
     const float inv = nrm > kMinNorm ? div_rn(1.f, nrm) : 0.f;
-    // y[i] = (x[perm[i]] * inv) * sign[i], scattered through inv_perm
-    for (int base = 0; base < D; base += 8 * nt) {
-      uint32_t w[8];
+    if (wave_rot) {
+      // y[i] = (x[perm[i]] * inv) * sign[i], butterflies, 1/sqrt(B): into y
+      if (R == 2) rotate_query_blocks<2>(t, xs, inv, y, wave, waves, lane);
+      else if (R == 4) rotate_query_blocks<4>(t, xs, inv, y, wave, waves, lane);
+      else if (R == 8) rotate_query_blocks<8>(t, xs, inv, y, wave, waves, lane);
+      else rotate_query_blocks<16>(t, xs, inv, y, wave, waves, lane);
+      __syncthreads();
+    } else {
+      // y[i] = (x[perm[i]] * inv) * sign[i], scattered through inv_perm
+      for (int base = 0; base < D; base += 8 * nt) {
+        uint32_t w[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int j = base + u * nt + tid;
-        w[u] = j < D ? t.perm_sign[j] : 0u;
-      }
+        for (int u = 0; u < 8; ++u) {
+          const int j = base + u * nt + tid;
+          w[u] = j < D ? t.perm_sign[j] : 0u;
+        }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int j = base + u * nt + tid;
-        if (j < D) y[w[u] & 0x7fffffffu] = (xs[j] * inv) * ((w[u] >> 31) ? -1.f : 1.f);
-      }
-    }
-    __syncthreads();
-    const int npairs = D >> 1, lhb = __ffs(B) - 2;  // log2(B / 2)
-    for (int lg = 0; (1 << lg) < B; ++lg) {
-      const int len = 1 << lg;
-      for (int p = tid; p < npairs; p += nt) {
-        const int blk = p >> lhb, qq = p & ((B >> 1) - 1);
-        const int i = blk * B + ((qq >> lg) << (lg + 1)) + (qq & (len - 1));
-        const float u = y[i], v = y[i + len];
-        y[i] = u + v;
-        y[i + len] = u - v;
+        for (int u = 0; u < 8; ++u) {
+          const int j = base + u * nt + tid;
+          if (j < D) y[w[u] & 0x7fffffffu] = (xs[j] * inv) * ((w[u] >> 31) ? -1.f : 1.f);
+        }
       }
       __syncthreads();
-    }
-    for (int i = tid; i < D; i += nt) {
-      const float r = y[i] * t.inv_sqrt_block;
-      if (tq) {
-        xs[i] = r;  // kept for bias_corr
-        y[i] = div_rn(r, tsc[i]);
-      } else {
-        y[i] = r;
+      const int npairs = D >> 1, lhb = __ffs(B) - 2;  // log2(B / 2)
+      for (int lg = 0; (1 << lg) < B; ++lg) {
+        const int len = 1 << lg;
+        for (int p = tid; p < npairs; p += nt) {
+          const int blk = p >> lhb, qq = p & ((B >> 1) - 1);
+          const int i = blk * B + ((qq >> lg) << (lg + 1)) + (qq & (len - 1));
+          const float u = y[i], v = y[i + len];
+          y[i] = u + v;
+          y[i + len] = u - v;
+        }
+        __syncthreads();
       }
+      for (int i = tid; i < D; i += nt) y[i] = y[i] * t.inv_sqrt_block;
+      __syncthreads();
+    }
+    if (tq) {
+      // keep y for bias_corr, divide by the TQ+ scale; bias_corr =
+      // (float) -sum_d (double) y_d * shift_d, sequential in d in the
+      // reference: the products are exact in double, so a tree sum plus the
+      // error bound of the sequential one certifies the float result
+      double sum = 0.0, asum = 0.0;
+      for (int i = tid; i < D; i += nt) {
+        const float r = y[i];
+        xs[i] = r;
+        y[i] = div_rn(r, tsc[i]);
+        const double term = -(double)r * (double)tsh[i];
+        sum += term;
+        asum += fabs(term);
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        sum += __shfl_xor(sum, o);
+        asum += __shfl_xor(asum, o);
+      }
+      if (lane == 0) {
+        s_red[0][wave] = sum;
+        s_red[1][wave] = asum;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        double ts = 0.0, ta = 0.0;
+        for (int w = 0; w < waves; ++w) {
+          ts += s_red[0][w];
+          ta += s_red[1][w];
+        }
+        const double u = 1.1102230246251565e-16;  // 2^-53
+        double bound = (double)(D + 64) * u * ta * 1.01;
+        bound += 2.0 * u * (fabs(ts) + bound);
+        const float lo = (float)(ts - bound), hi = (float)(ts + bound);
+        if (lo == hi) {
+          s_bc = hi;
+        } else {
+          // not certified: the reference's sequential chain
+          double acc = 0.0;
+          for (int d = 0; d < D; ++d) acc = fma(-(double)xs[d], (double)tsh[d], acc);
+          s_bc = (float)acc;
+        }
+      }
+    } else if (tid == 0) {
+      s_bc = 0.f;
     }
     __syncthreads();
     // pass 1: per group the min/max of both 16-entry halves; global span
     float span = 0.f;
     for (int g = tid; g < G; g += nt) {
+      float sv[2][16];
+      if (cpn == 1) group_values<1>(y, cr, t.bits, g, sv);
+      else group_values<2>(y, cr, t.bits, g, sv);
+#pragma unroll
       for (int half = 0; half < 2; ++half) {
         float mn = FLT_MAX, mx = -FLT_MAX;
-        for (unsigned nib = 0; nib < 16; ++nib) {
-          const float sv = lut_value(y, s_cent, t.bits, g * cpb + half * cpn, cpn, nib);
-          mn = sv < mn ? sv : mn;
-          mx = sv > mx ? sv : mx;
+#pragma unroll
+        for (int nib = 0; nib < 16; ++nib) {
+          mn = sv[half][nib] < mn ? sv[half][nib] : mn;
+          mx = sv[half][nib] > mx ? sv[half][nib] : mx;
         }
         mins[2 * g + half] = mn;
         span = fmaxf(span, mx - mn);
@@ -894,27 +987,33 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTable
     if (lane == 0) s_span[wave] = span;
     __syncthreads();
     span = s_span[0];
-    for (int w = 1; w < nt / 64; ++w) span = fmaxf(span, s_span[w]);
+    for (int w = 1; w < waves; ++w) span = fmaxf(span, s_span[w]);
     float scale = span > 0.f ? div_rn(span, 127.f) : 1.f;
     float iscale = 1.f;
     if (scale >= FLT_MIN)
       iscale = div_rn(1.f, scale);
     else
       scale = 1.f;
-    // pass 2 on waves 0-1: quantized entries. Meanwhile, with no barrier in
-    // between, wave 2 lane 0 runs the bias (sequential over groups, as the
-    // reference) and wave 3 lane 0 the TQ+ bias_corr chain (sequential in d).
+    // pass 2 (all but the last wavefront): quantized entries; meanwhile lane 0
+    // of the last wavefront runs the bias, sequential over groups as in the
+    // reference
     uint8_t* out = luts + qi * (long)G * 32;
-    if (wave < 2) {
-      for (int g = tid; g < G; g += 128) {
+    if (wave < waves - 1) {
+      const int pt = nt - 64;
+      for (int g = tid; g < G; g += pt) {
+        float sv[2][16];
+        if (cpn == 1) group_values<1>(y, cr, t.bits, g, sv);
+        else group_values<2>(y, cr, t.bits, g, sv);
         uint32_t w[8];
+#pragma unroll
         for (int half = 0; half < 2; ++half) {
           const float mn = mins[2 * g + half];
+#pragma unroll
           for (int e = 0; e < 16; e += 4) {
             uint32_t word = 0;
+#pragma unroll
             for (int b = 0; b < 4; ++b) {
-              const float sv = lut_value(y, s_cent, t.bits, g * cpb + half * cpn, cpn, (unsigned)(e + b));
-              float r = roundf((sv - mn) * iscale);
+              float r = roundf((sv[half][e + b] - mn) * iscale);
               r = r < 0.f ? 0.f : (r > 127.f ? 127.f : r);
               word |= uint32_t(r) << (8 * b);
             }
@@ -925,7 +1024,7 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTable
         dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
         dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
       }
-    } else if (wave == 2 && lane == 0) {
+    } else if (lane == 0) {
       float bsum = 0.f;
       int g = 0;
       for (; g + 8 <= G; g += 8) {
@@ -939,24 +1038,6 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTable
       lut_scale[qi] = scale;
       lut_bias[qi] = bsum;
       s_bsum = bsum;
-    } else if (wave == 3 && lane == 0) {
-      // bias_corr = (float) -sum (double) y_d * shift_d, sequential in d
-      double acc = 0.0;
-      if (tq) {
-        int d = 0;
-        for (; d + 8 <= D; d += 8) {
-          float yv[8], sv[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            yv[u] = xs[d + u];
-            sv[u] = tsh[d + u];
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) acc = fma(-(double)yv[u], (double)sv[u], acc);
-        }
-        for (; d < D; ++d) acc = fma(-(double)xs[d], (double)tsh[d], acc);
-      }
-      s_bc = (float)acc;
     }
     __syncthreads();
     if (tid == 0) {
