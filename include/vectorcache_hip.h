@@ -212,7 +212,9 @@ vc_status vc_set_profiling(int on);
  * bit 1 makes searches use the nibble-table scan kernel (two 16-entry lookups
  * per code byte) instead of the byte-pair table kernel; bit 3 makes the
  * byte-pair kernel take the other of its two vectors-per-thread variants
- * (1 <-> 2) from the one the planner picks; bits 16-31, when not
+ * (1 <-> 2) from the one the planner picks; bit 4 makes query preparation
+ * of indexes created later always take the reference's sequential bias_corr
+ * chain instead of the certified parallel sum; bits 16-31, when not
  * zero, set the number of vector ranges each query tile is split into (at
  * most 512) in place of the automatic split. */
 vc_status vc_set_debug(unsigned flags);

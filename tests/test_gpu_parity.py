@@ -77,6 +77,24 @@ def test_index_exports_and_search_match_golden(vc, golden_cases):
             assert scores_equal(r.scores, g[f"scores_k{k}"]), (c, k)
 
 
+def test_sequential_bias_corr_matches_golden(vc, golden_cases):
+    """Bit 4: query preparation always through the reference's sequential
+    bias_corr chain (the path the certified parallel sum falls back to):
+    the same LUTs, scales, biases and bias_corr."""
+    vc.set_debug(16)
+    try:
+        for c, g, x, q, sample in golden_cases:
+            if not c["tq"]:
+                continue
+            ix = _index(vc, c, x, g, sample)
+            prep = ix.prepare_queries(q)
+            assert np.array_equal(prep["luts"], g["prep_luts"]), c
+            assert bits_equal(prep["lut_bias"], g["prep_lut_bias"]), c
+            assert bits_equal(prep["bias_corr"], g["prep_bias_corr"]), c
+    finally:
+        vc.set_debug(0)
+
+
 @pytest.mark.parametrize("flags", [1, 4, 5])
 def test_exact_alpha_path_matches_golden(vc, golden_cases, flags):
     """Bit 0: every vector through the reference's sequential alpha chains

@@ -952,7 +952,7 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTable
         double bound = (double)(D + 64) * u * ta * 1.01;
         bound += 2.0 * u * (fabs(ts) + bound);
         const float lo = (float)(ts - bound), hi = (float)(ts + bound);
-        if (lo == hi) {
+        if (lo == hi && (t.debug_flags & 16) == 0) {  // flags bit 4: always the chain (test hook)
           s_bc = hi;
         } else {
           // not certified: the reference's sequential chain
@@ -1588,6 +1588,21 @@ __device__ __forceinline__ uint32_t byte_scaled(uint32_t w) {
   return r;
 }
 
+// acc (two words, low first) += w as one 64-bit integer add. kAsm writes the
+// instruction out: the compiler splits an add of two separately computed
+// words into a 64-bit add and a 32-bit add.
+template <bool kAsm = false>
+__device__ __forceinline__ void acc64_add(uint32_t (&acc)[2], uint2 w) {
+  uint64_t a = ((uint64_t)acc[1] << 32) | acc[0];
+  const uint64_t b = ((uint64_t)w.y << 32) | w.x;
+  if constexpr (kAsm)
+    asm("v_lshl_add_u64 %0, %1, 0, %2" : "=v"(a) : "v"(b), "v"(a));
+  else
+    a += b;
+  acc[0] = (uint32_t)a;
+  acc[1] = (uint32_t)(a >> 32);
+}
+
 // 32-bit accumulators per vector (u16 lanes; narrow T = 8 keeps a second set)
 template <int T> constexpr int pair_acc_words() { return T == 1 ? 1 : (PairFmt<T>::wide ? T / 2 : T / 4); }
 
@@ -1638,34 +1653,25 @@ __device__ __forceinline__ void pair_chunk(const uint8_t* __restrict__ table, ui
           const uint8_t* p = tb + e * 256 * EB + byte_scaled<e & 3, SH>(words[e >> 2]);
           if constexpr (T == 1) {
             acc[j][0] += *p;
-          } else if constexpr (kWide) {
-            // the whole entry in one read (its offset folds into the instruction)
-            uint32_t ent[NA];
-            if constexpr (NA == 2) {
-              const uint2 w2 = *reinterpret_cast<const uint2*>(p);
-              ent[0] = w2.x;
-              ent[1] = w2.y;
-            } else {
-              ent[0] = *reinterpret_cast<const uint32_t*>(p);
-            }
-#pragma unroll
-            for (int a = 0; a < NA; ++a)
-              acc[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc[j][a]) +
-                                                           __builtin_bit_cast(u16x2, ent[a]));
-          } else {
-            // Entry word a = bytes of queries 4a .. 4a+3. acc adds the word
-            // as it is, as u16 lanes: (q0 + 256 q1, q2 + 256 q3) mod 2^16;
-            // acc2 adds the odd bytes alone: (q1, q3) (v_perm; 0x0c = zero).
-            // Every window sum is < 2^16, so the flush recovers the even
-            // queries exactly as acc - 256 acc2 (mod 2^16).
+          } else if constexpr (kWide && NA == 2) {
+            // the whole entry in one read (its offset folds into the
+            // instruction); its four u16 lanes added by one 64-bit add (every
+            // lane's window sum is < 2^16, so no carry crosses a lane)
             const uint2 w2 = *reinterpret_cast<const uint2*>(p);
-            const uint32_t ws[2] = {w2.x, w2.y};
-#pragma unroll
-            for (int a = 0; a < NA; ++a) {
-              const uint32_t od = __builtin_amdgcn_perm(0u, ws[a], 0x0c030c01u);
-              acc[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc[j][a]) + __builtin_bit_cast(u16x2, ws[a]));
-              acc2[j][a] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc2[j][a]) + __builtin_bit_cast(u16x2, od));
-            }
+            acc64_add(acc[j], w2);
+          } else if constexpr (kWide) {
+            const uint32_t e0 = *reinterpret_cast<const uint32_t*>(p);
+            acc[j][0] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, acc[j][0]) + __builtin_bit_cast(u16x2, e0));
+          } else {
+            // Entry = bytes of queries 0 .. 7. acc adds it as it is, one
+            // 64-bit add: sum over q of 2^(8q) S_q (mod 2^64); acc2 adds the
+            // odd queries' bytes alone as u16 lanes (S_1, S_3, S_5, S_7; two
+            // v_perm, 0x0c = zero). Every window sum S_q is < 2^16, so the
+            // flush recovers the even queries exactly from acc - 2^8 acc2.
+            const uint2 w2 = *reinterpret_cast<const uint2*>(p);
+            acc64_add(acc[j], w2);
+            acc64_add<true>(acc2[j], make_uint2(__builtin_amdgcn_perm(0u, w2.x, 0x0c030c01u),
+                                          __builtin_amdgcn_perm(0u, w2.y, 0x0c030c01u)));
           }
         });
       }
@@ -1687,10 +1693,14 @@ __device__ __forceinline__ void pair_chunk(const uint8_t* __restrict__ table, ui
             fa[j][2 * a + 1] = __fmaf_rn(scl[2 * a + 1], (float)sm.y, fa[j][2 * a + 1]);
           }
         } else {
+          const uint64_t a64 = ((uint64_t)acc[j][1] << 32) | acc[j][0];
+          const uint64_t b64 = ((uint64_t)acc2[j][1] << 32) | acc2[j][0];
+          const uint64_t e64 = a64 - (b64 << 8);  // sum over q of 2^(16q) S_2q
+          const uint32_t evw[2] = {(uint32_t)e64, (uint32_t)(e64 >> 32)};
 #pragma unroll
           for (int a = 0; a < NA; ++a) {
             const u16x2 od = __builtin_bit_cast(u16x2, acc2[j][a]);  // queries 4a + 1, 4a + 3
-            const u16x2 ev = __builtin_bit_cast(u16x2, acc[j][a]) - (od << (unsigned short)8);  // queries 4a, 4a + 2
+            const u16x2 ev = __builtin_bit_cast(u16x2, evw[a]);      // queries 4a, 4a + 2
             fa[j][4 * a + 0] = __fmaf_rn(scl[4 * a + 0], (float)ev.x, fa[j][4 * a + 0]);
             fa[j][4 * a + 1] = __fmaf_rn(scl[4 * a + 1], (float)od.x, fa[j][4 * a + 1]);
             fa[j][4 * a + 2] = __fmaf_rn(scl[4 * a + 2], (float)ev.y, fa[j][4 * a + 2]);
