@@ -1508,49 +1508,50 @@ __global__ void pair_source_kernel(const uint8_t* __restrict__ luts, long nq, in
   }
 }
 
-// Table chunk from the staged source rows of its CG groups. Each thread writes
-// 64 contiguous bytes of table: entries (g, h, l0 .. l0 + 64 / EB - 1), or for
-// T = 1 four whole rows (g, h0 .. h0 + 3).
+// Table chunk from the staged source rows of its CG groups. The table is
+// 4096 16-byte pieces; instruction v of thread t writes piece v * 1024 + t,
+// so each wavefront store covers 1 KiB of consecutive bytes (no bank
+// conflicts). Piece c holds entries (g, h, l0 .. l0 + 16 / EB - 1) of its
+// first entry c * 16 / EB = (g, h, l0): lo[g][l0 ..] + hi[g][h].
 template <int T>
 __device__ __forceinline__ void build_pair_chunk(const uint8_t* __restrict__ stage, uint8_t* __restrict__ table) {
   constexpr int EB = pair_entry_bytes<T>();
+  constexpr int EPC = 16 / EB;  // entries per piece
+  // piece v * 1024 + t lies 4 * EPC groups after piece t: its source rows are
+  // a fixed 4 * EPC * 32 * EB = 2 KiB further into the stage
+  constexpr int kVStep = 4 * EPC * 32 * EB;
+  static_assert(kScanThreads * EPC % 256 == 0 && kVStep == 2048, "pieces of one thread differ in g only");
   const int tid = threadIdx.x;
-  if constexpr (EB == 1) {
-    const int g = tid >> 2, h0 = (tid & 3) * 4;
-    const uint32_t his = *reinterpret_cast<const uint32_t*>(stage + g * 32 + h0);
-    const uint4 lo = *reinterpret_cast<const uint4*>(stage + g * 32 + 16);
+  const int i0 = tid * EPC;
+  const int g = i0 >> 8, h = (i0 >> 4) & 15, l0 = i0 & 15;
+  const uint8_t* lo_src = stage + (g * 32 + 16 + l0) * EB;
+  const uint8_t* hi_src = stage + (g * 32 + h) * EB;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const uint32_t bc = ((his >> (8 * r)) & 0xffu) * 0x01010101u;  // bytes <= 127: no carries
-      *reinterpret_cast<uint4*>(table + g * 256 + (h0 + r) * 16) =
-          make_uint4(lo.x + bc, lo.y + bc, lo.z + bc, lo.w + bc);
-    }
-  } else {
-    constexpr int ENT = 64 / EB;          // entries per thread
-    constexpr int PER_ROW = 16 / ENT;     // threads per (g, h) row
-    constexpr int WPE = EB / 4;           // 32-bit words per entry
-    const int row = tid / PER_ROW;        // g * 16 + h
-    const int g = row >> 4, h = row & 15;
-    const int l0 = (tid % PER_ROW) * ENT;
-    uint32_t hw[WPE];
+  for (int v = 0; v < kPairTableBytes / 16 / kScanThreads; ++v) {
+    const int c = v * kScanThreads + tid;
+    const uint4 lo = *reinterpret_cast<const uint4*>(lo_src + v * kVStep);
+    uint32_t o[4] = {lo.x, lo.y, lo.z, lo.w};
+    if constexpr (EB == 1) {
+      const uint32_t bc = (uint32_t)hi_src[v * kVStep] * 0x01010101u;  // bytes <= 127: no carries
 #pragma unroll
-    for (int w = 0; w < WPE; ++w) hw[w] = reinterpret_cast<const uint32_t*>(stage + (g * 32 + h) * EB)[w];
-    const uint4* ls = reinterpret_cast<const uint4*>(stage + (g * 32 + 16 + l0) * EB);
-    uint4* dst = reinterpret_cast<uint4*>(table + (g * 256 + h * 16 + l0) * EB);
+      for (int w = 0; w < 4; ++w) o[w] += bc;
+    } else {
+      constexpr int WPE = EB / 4;  // 32-bit words per entry
+      uint32_t hw[WPE];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const uint4 l = ls[v];
-      uint32_t o[4] = {l.x, l.y, l.z, l.w};
+      for (int w = 0; w < WPE; ++w) hw[w] = reinterpret_cast<const uint32_t*>(hi_src + v * kVStep)[w];
 #pragma unroll
       for (int w = 0; w < 4; ++w) {
-        const uint32_t hv = hw[(v * 4 + w) % WPE];
         if constexpr (PairFmt<T>::wide)
-          o[w] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, o[w]) + __builtin_bit_cast(u16x2, hv));
+          o[w] = __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, o[w]) + __builtin_bit_cast(u16x2, hw[w % WPE]));
         else
-          o[w] += hv;  // bytes <= 127: no carries
+          o[w] += hw[w % WPE];  // bytes <= 127: no carries
       }
-      dst[v] = make_uint4(o[0], o[1], o[2], o[3]);
     }
+    reinterpret_cast<uint4*>(table)[c] = make_uint4(o[0], o[1], o[2], o[3]);
+    // one piece at a time: the build runs with the code ring and the score
+    // chains live, so it must not hold four pieces' sources in registers
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
