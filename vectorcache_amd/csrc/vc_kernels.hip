@@ -1079,7 +1079,15 @@ struct TopkShared {
   int* m;           // [T]
   float* thr_s;     // [T]
   uint32_t* thr_i;  // [T]
+  int* bound;       // [T] score bound (order-preserving int image of a float)
 };
+
+// Order-preserving map of floats to ints (for LDS atomicMax) and back.
+__device__ __forceinline__ int float_order(float f) {
+  const int b = __float_as_int(f);
+  return b >= 0 ? b : (b ^ 0x7fffffff);
+}
+__device__ __forceinline__ float order_float(int o) { return __int_as_float(o >= 0 ? o : (o ^ 0x7fffffff)); }
 
 // Merge query t's candidates into its sorted list (all threads participate).
 __device__ void topk_merge(const TopkShared& sh, int t, int k, int cap) {
@@ -1092,15 +1100,22 @@ __device__ void topk_merge(const TopkShared& sh, int t, int k, int cap) {
   uint32_t* ni = sh.nw_i + t * k;
   const float* cs = sh.cand_s + t * cap;
   const uint32_t* ci = sh.cand_i + t * cap;
-  // rank = start + number of candidates better than (s, id); four candidates
-  // per step (16-byte LDS reads; the buffer is 16-byte aligned and cap is a
-  // multiple of 32, so reads past c stay inside it and are masked)
+  // rank = start + number of candidates better than (s, id) (16-byte LDS
+  // reads; the buffer is 16-byte aligned and cap is a multiple of 32, so
+  // reads past c stay inside it and are masked)
+  // Eight candidates per step, their reads issued together: the early exit
+  // (rank >= k) makes each step wait for its reads, and the LDS is shared with
+  // another workgroup's lookups, so a chain of single reads is slow.
   auto rank_of = [&](float s, uint32_t id, int rank) {
-    for (int f = 0; f < c && rank < k; f += 4) {
-      const float4 s4 = *reinterpret_cast<const float4*>(cs + f);
-      const uint4 i4 = *reinterpret_cast<const uint4*>(ci + f);
-      rank += (better(s4.x, i4.x, s, id) ? 1 : 0) + (f + 1 < c && better(s4.y, i4.y, s, id) ? 1 : 0) +
-              (f + 2 < c && better(s4.z, i4.z, s, id) ? 1 : 0) + (f + 3 < c && better(s4.w, i4.w, s, id) ? 1 : 0);
+    for (int f = 0; f < c && rank < k; f += 8) {
+      const float4 sa = *reinterpret_cast<const float4*>(cs + f);
+      const float4 sb = *reinterpret_cast<const float4*>(cs + f + 4);
+      const uint4 ia = *reinterpret_cast<const uint4*>(ci + f);
+      const uint4 ib = *reinterpret_cast<const uint4*>(ci + f + 4);
+      rank += (better(sa.x, ia.x, s, id) ? 1 : 0) + (f + 1 < c && better(sa.y, ia.y, s, id) ? 1 : 0) +
+              (f + 2 < c && better(sa.z, ia.z, s, id) ? 1 : 0) + (f + 3 < c && better(sa.w, ia.w, s, id) ? 1 : 0) +
+              (f + 4 < c && better(sb.x, ib.x, s, id) ? 1 : 0) + (f + 5 < c && better(sb.y, ib.y, s, id) ? 1 : 0) +
+              (f + 6 < c && better(sb.z, ib.z, s, id) ? 1 : 0) + (f + 7 < c && better(sb.w, ib.w, s, id) ? 1 : 0);
     }
     return rank;
   };
@@ -1146,13 +1161,15 @@ __device__ __forceinline__ void init_topk(TopkShared& sh, unsigned char* base, i
   sh.cnt = reinterpret_cast<int*>(base + off); off += T * 4;
   sh.m = reinterpret_cast<int*>(base + off); off += T * 4;
   sh.thr_s = reinterpret_cast<float*>(base + off); off += T * 4;
-  sh.thr_i = reinterpret_cast<uint32_t*>(base + off);
+  sh.thr_i = reinterpret_cast<uint32_t*>(base + off); off += T * 4;
+  sh.bound = reinterpret_cast<int*>(base + off);
   const int tid = threadIdx.x;
   if (tid < T) {
     sh.cnt[tid] = 0;
     sh.m[tid] = 0;
     sh.thr_s[tid] = -INFINITY;
     sh.thr_i[tid] = 0u;
+    sh.bound[tid] = float_order(-INFINITY);
   }
 }
 
@@ -1195,7 +1212,9 @@ __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], cons
     for (int t = 0; t < T; ++t) {
       s[t] = valid[j] ? fa[j][t] * av + 0.f : -INFINITY;
       sid[t] = vid;
-      const bool cand = valid[j] && q0 + t < nq && better(s[t], vid, sh.thr_s[t], sh.thr_i[t]);
+      // below the bound: worse than k vectors already offered (see below)
+      const bool cand = valid[j] && q0 + t < nq && better(s[t], vid, sh.thr_s[t], sh.thr_i[t]) &&
+                        !(s[t] < order_float(sh.bound[t]));
       // (tiles of one or two queries only: the bandwidth regime, where a
       // workgroup scans few steps and the first one admits every vector; the
       // sort would cost larger tiles their register budget)
@@ -1206,6 +1225,12 @@ __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], cons
         if (!cand) { s[t] = -INFINITY; sid[t] = 0xffffffffu; }
         wave_sort_desc(s[t], sid[t], lane);
         if (lane < kw) pend |= 1u << t;
+        // The wavefront offers its k best; every later vector scoring below
+        // the k-th of them is worse than k offered vectors (each of which
+        // stays in the lists or is displaced by a better one), so it can
+        // never enter the top k: that score bounds the admission from below
+        // (an equal score may still win on its id).
+        if (kw == k && lane == k - 1) atomicMax(&sh.bound[t], float_order(s[t]));
       } else if (cand) {
         pend |= 1u << t;
       }
@@ -1749,7 +1774,7 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void scan_pair_ker
   init_topk(sh, smem, T, k, cap);
   // per-query LUT scale and bias in LDS (read at window flushes and step
   // starts only; keeps them out of the scalar register budget)
-  float* scl = reinterpret_cast<float*>(smem + (size_t)T * (4 * (size_t)k + 2 * (size_t)cap) * 4 + 16 * T);
+  float* scl = reinterpret_cast<float*>(smem + (size_t)T * (4 * (size_t)k + 2 * (size_t)cap) * 4 + 20 * T);
   float* bias = scl + T;
   if (tid < T) {
     const long q = q0 + tid < nq ? q0 + tid : nq - 1;
@@ -2486,7 +2511,7 @@ hipError_t launch_prepare_queries(const CodeTables& t, const float* q, std::size
 
 // top-k state of a workgroup (lists, candidates, counters) + per-query scale
 // and bias for the pair kernel
-static size_t topk_lds(int T, int k, int cap) { return (size_t)T * (4 * (size_t)k + 2 * (size_t)cap) * 4 + 24 * T + 64; }
+static size_t topk_lds(int T, int k, int cap) { return (size_t)T * (4 * (size_t)k + 2 * (size_t)cap) * 4 + 28 * T + 64; }
 
 // Vector ranges per query tile: enough workgroups to fill every CU slot when
 // the query tiles alone do not; otherwise the split (at most 8) that leaves
