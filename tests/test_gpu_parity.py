@@ -77,6 +77,22 @@ def test_index_exports_and_search_match_golden(vc, golden_cases):
             assert scores_equal(r.scores, g[f"scores_k{k}"]), (c, k)
 
 
+def test_gpu32_layout_export_and_import(vc, oracle, golden_cases):
+    """The reference CUDA index's code words: export equals the oracle's
+    layout of the reference's flat bytes; imported back as PERM0 blocks they
+    equal the PERM0 export and search like the index itself."""
+    for c, g, x, q, sample in golden_cases:
+        ix = _index(vc, c, x, g, sample)
+        w = ix.export_gpu32()
+        assert np.array_equal(w, oracle.flat_to_gpu32(g["flat"])), c
+        p = vc.gpu32_to_perm0(w, c["n"], c["dim"], c["bits"])
+        assert np.array_equal(p, ix.export_perm0()), c
+        sh, sc = (g["shift"], g["scale"]) if c["tq"] else (None, None)
+        r = vc.search_perm0(q, 10, c["bits"], p, ix.export_alpha(), shift=sh, scale=sc)
+        assert np.array_equal(r.ids, g["ids_k10"]), c
+        assert scores_equal(r.scores, g["scores_k10"]), c
+
+
 def test_sequential_bias_corr_matches_golden(vc, golden_cases):
     """Bit 4: query preparation always through the reference's sequential
     bias_corr chain (the path the certified parallel sum falls back to):
@@ -243,6 +259,29 @@ def test_ties_resolve_to_lower_id(vc, oracle):
     for row_s, row_i in zip(res.scores, res.ids):
         same = row_s[1:] == row_s[:-1]
         assert np.all(row_i[1:][same] > row_i[:-1][same])
+
+
+@pytest.mark.parametrize("nq,k", [(1, 10), (2, 64)])
+def test_ties_at_the_admission_bound(vc, oracle, nq, k):
+    """Heavy score ties with one or two queries (one-query tiles: wavefront
+    sorts and the lower score bound they set) over many steps per workgroup:
+    vectors tied with the bound are still admitted and resolve to the lower
+    ids, as in the oracle."""
+    r = np.random.default_rng(5)
+    base = r.standard_normal((40, 128)).astype(np.float32)
+    x = base[r.integers(0, 40, 60000)]
+    q = r.standard_normal((nq, 128)).astype(np.float32)
+    s_o, i_o = _oracle_search(oracle, x, q, 4, k)
+    ix = vc.TurboQuantIndex(128, 4)
+    ix.add(x)
+    ix.prepare()
+    vc.set_debug(2 << 16)  # two vector ranges: about 30 steps per workgroup
+    try:
+        res = ix.search(q, k)
+    finally:
+        vc.set_debug(0)
+    assert np.array_equal(res.ids, i_o)
+    assert scores_equal(res.scores, s_o)
 
 
 def test_incremental_add_equals_bulk_and_batch_equals_single(vc):
