@@ -110,6 +110,14 @@ vc_status vc_index_export_perm0(vc_index* index, uint8_t* perm0_host);
  *      src/pack/pack.cpp:133-154,358-364). */
 vc_status vc_index_export_seq(vc_index* index, uint8_t* seq_host);
 vc_status vc_index_export_alpha(vc_index* index, float* alpha_host);
+/* gpu32: the reference CUDA index's code layout, u32 words
+ *        [ceil(n/32)][ceil(G/4)][32] (src/gpu/kernels.cuh:18-21, written by
+ *        encode_coop, src/gpu/kernels.cu:111-133): word (tile, g4, lane) holds
+ *        byte groups 4 g4 .. 4 g4 + 3 of vector 32 tile + lane, little-endian;
+ *        groups past G and vectors past n are zero. vc_gpu32_words(n, dim,
+ *        bits) words (0 for a bad dim / bits). */
+vc_status vc_index_export_gpu32(vc_index* index, uint32_t* codes_host);
+size_t vc_gpu32_words(size_t n, size_t dim, size_t bits);
 vc_status vc_index_get_calibration(const vc_index* index, float* shift_host, float* scale_host);
 
 /* Per-query state as prepare_queries builds it (src/search/search.cpp:762-860):
@@ -147,6 +155,13 @@ vc_status vc_search_perm0(const float* queries, size_t nq, size_t dim, size_t bi
                           const uint8_t* perm0, const float* alpha, size_t n,
                           const uint64_t* id_map, const float* shift, const float* scale,
                           float* scores, uint64_t* ids, size_t* k_out);
+
+/* Import of codes in the reference CUDA index's layout (see gpu32 above) as
+ * PERM0 blocks (ceil(n/192) x G x 192 bytes), the input of vc_search_perm0 /
+ * vc_score_perm0_into: the same bytes the reference's repack_perm0 makes from
+ * those codes' flat rows (src/pack/pack.cpp:388-401). Bytes of groups past G
+ * in the last word of a row are ignored. */
+vc_status vc_gpu32_to_perm0(const uint32_t* codes, size_t n, size_t dim, size_t bits, uint8_t* perm0_out);
 
 /* score_prepared_into (src/search/search.cpp:1156-1267, include/vectorcache/
  * search/search.hpp:163-177) over one bucket given as PERM0 blocks + alpha:

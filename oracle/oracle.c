@@ -550,6 +550,37 @@ int oc_unpack_perm0(const uint8_t* blocked, size_t n, size_t G, uint8_t* flat_ou
   return 0;
 }
 
+/* The reference CUDA index's code words (src/gpu/kernels.cuh:18-21,
+ * src/gpu/kernels.cu:111-133, zero-filled at allocation :587-595): for vector
+ * vi, tile = vi / 32, lane = vi % 32, word (tile * ceil(G/4) + g4) * 32 + lane
+ * = bytes[0] | bytes[1] << 8 | bytes[2] << 16 | bytes[3] << 24 of groups
+ * 4 g4 .. 4 g4 + 3 (0 past G). */
+int oc_flat_to_gpu32(const uint8_t* flat, size_t n, size_t G, uint32_t* words) {
+  const size_t nbg4 = (G + 3) / 4, n_tiles = (n + 31) / 32;
+  memset(words, 0, n_tiles * nbg4 * 32 * sizeof(uint32_t));
+  for (size_t vi = 0; vi < n; ++vi) {
+    const size_t tile = vi / 32, lane = vi - tile * 32;
+    for (size_t g4 = 0; g4 < nbg4; ++g4) {
+      uint32_t bytes[4] = {0, 0, 0, 0};
+      for (size_t t = 0; t < 4; ++t) {
+        const size_t g = g4 * 4 + t;
+        if (g >= G) break;
+        bytes[t] = flat[vi * G + g];
+      }
+      words[(tile * nbg4 + g4) * 32 + lane] = bytes[0] | (bytes[1] << 8) | (bytes[2] << 16) | (bytes[3] << 24);
+    }
+  }
+  return 0;
+}
+
+int oc_gpu32_to_flat(const uint32_t* words, size_t n, size_t G, uint8_t* flat_out) {
+  const size_t nbg4 = (G + 3) / 4;
+  for (size_t vi = 0; vi < n; ++vi)
+    for (size_t g = 0; g < G; ++g)
+      flat_out[vi * G + g] = (uint8_t)(words[((vi / 32) * nbg4 + g / 4) * 32 + vi % 32] >> (8 * (g % 4)));
+  return 0;
+}
+
 /* sequential 192x1 layout (reference src/pack/pack.cpp:133-154) */
 int oc_pack_seq(const uint8_t* flat, size_t n, size_t G, uint8_t* out) {
   const size_t nb = (n + OC_BLOCK - 1) / OC_BLOCK;

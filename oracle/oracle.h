@@ -28,6 +28,8 @@ int oc_codes_flat(const uint8_t* packed, size_t n, size_t bits, size_t dim, uint
 int oc_pack_perm0(const uint8_t* flat, size_t n, size_t n_groups, uint8_t* out);
 int oc_pack_seq(const uint8_t* flat, size_t n, size_t n_groups, uint8_t* out);
 int oc_unpack_perm0(const uint8_t* blocked, size_t n, size_t n_groups, uint8_t* flat_out);
+int oc_flat_to_gpu32(const uint8_t* flat, size_t n, size_t n_groups, uint32_t* words);
+int oc_gpu32_to_flat(const uint32_t* words, size_t n, size_t n_groups, uint8_t* flat_out);
 int oc_prepare_queries(const float* q, size_t nq, size_t dim, size_t bits, const float* shift,
                        const float* scale, float* q_rot_out, float* bias_corr_out,
                        uint8_t* luts_out, float* lut_scale_out, float* lut_bias_out);

@@ -137,6 +137,21 @@ class Oracle:
             self._check(self.lib.oc_pack_perm0(_ptr(flat), _S(n), _S(G), _ptr(out)), "pack_perm0")
         return out
 
+    def flat_to_gpu32(self, flat):
+        """Row-major byte groups (n x G) -> the reference CUDA index's u32
+        words [ceil(n/32)][ceil(G/4)][32]."""
+        flat = np.ascontiguousarray(flat, np.uint8)
+        n, G = flat.shape
+        out = np.zeros(((n + 31) // 32) * ((G + 3) // 4) * 32, np.uint32)
+        self._check(self.lib.oc_flat_to_gpu32(_ptr(flat), _S(n), _S(G), _ptr(out)), "flat_to_gpu32")
+        return out
+
+    def gpu32_to_flat(self, words, n, G):
+        words = np.ascontiguousarray(words, np.uint32)
+        out = np.zeros((n, G), np.uint8)
+        self._check(self.lib.oc_gpu32_to_flat(_ptr(words), _S(n), _S(G), _ptr(out)), "gpu32_to_flat")
+        return out
+
     def repack_seq(self, packed, bits, dim):
         packed = np.ascontiguousarray(packed, np.uint8)
         n = packed.shape[0]

@@ -349,3 +349,40 @@ def test_float_rescore_is_the_inner_product(oracle):
     for row_s, row_i in zip(s, i):
         order = np.lexsort((row_i, -row_s))
         assert np.array_equal(order, np.arange(20))
+
+
+def test_gpu32_layout_known_answer(oracle):
+    """The reference CUDA index's code words (src/gpu/kernels.cu:111-133),
+    written out by hand for 33 vectors of 5 byte groups: 2 tiles of 32 lanes,
+    2 words per lane (groups 0-3, then group 4 alone), little-endian bytes,
+    zero past the last group and the last vector."""
+    n, G = 33, 5
+    flat = (np.arange(n * G, dtype=np.uint32) % 251 + 1).astype(np.uint8).reshape(n, G)
+    w = oracle.flat_to_gpu32(flat)
+    assert w.size == 128
+    b = flat.astype(np.uint32)
+    # vector 0: tile 0, lane 0 -> words 0 (groups 0-3) and 32 (group 4)
+    assert w[0] == b[0, 0] | b[0, 1] << 8 | b[0, 2] << 16 | b[0, 3] << 24
+    assert w[32] == b[0, 4]
+    # vector 1: lane 1 -> words 1 and 33
+    assert w[1] == b[1, 0] | b[1, 1] << 8 | b[1, 2] << 16 | b[1, 3] << 24
+    assert w[33] == b[1, 4]
+    # vector 31: lane 31 -> words 31 and 63
+    assert w[31] == b[31, 0] | b[31, 1] << 8 | b[31, 2] << 16 | b[31, 3] << 24
+    assert w[63] == b[31, 4]
+    # vector 32: tile 1, lane 0 -> words 64 and 96
+    assert w[64] == b[32, 0] | b[32, 1] << 8 | b[32, 2] << 16 | b[32, 3] << 24
+    assert w[96] == b[32, 4]
+    # lanes 1-31 of tile 1 hold no vector
+    assert not w[65:96].any() and not w[97:128].any()
+
+
+def test_gpu32_round_trip(oracle):
+    r = np.random.default_rng(17)
+    for n, G in [(1, 1), (31, 3), (32, 4), (1000, 24), (257, 384)]:
+        flat = r.integers(0, 256, (n, G), dtype=np.uint8)
+        w = oracle.flat_to_gpu32(flat)
+        assert w.size == (n + 31) // 32 * ((G + 3) // 4) * 32
+        assert np.array_equal(oracle.gpu32_to_flat(w, n, G), flat)
+        # a bijection between the valid bytes: every flat byte appears once
+        assert int(np.count_nonzero(w.view(np.uint8))) == int(np.count_nonzero(flat))

@@ -59,6 +59,9 @@ _SIGS = {
     "vc_index_export_perm0": ([P, P], I),
     "vc_index_export_seq": ([P, P], I),
     "vc_index_export_alpha": ([P, P], I),
+    "vc_index_export_gpu32": ([P, P], I),
+    "vc_gpu32_words": ([S, S, S], S),
+    "vc_gpu32_to_perm0": ([P, S, S, S, P], I),
     "vc_index_get_calibration": ([P, P, P], I),
     "vc_index_prepare_queries": ([P, P, S, P, P, P, P], I),
     "vc_rotation_tables": ([S, P, P, P, ctypes.POINTER(S)], I),

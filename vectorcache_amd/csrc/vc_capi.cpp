@@ -553,6 +553,26 @@ vc_status vc_index_export_seq(vc_index* ix, uint8_t* seq_host) {
   });
 }
 
+vc_status vc_index_export_gpu32(vc_index* ix, uint32_t* codes_host) {
+  return guarded([&] {
+    if (ix == nullptr || codes_host == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    require_prepared(ix);
+    const size_t n = ix->n_encoded, G = size_t(ix->t.groups);
+    if (n == 0) return;
+    const size_t words = vchip::gpu32_words(n, int(G));
+    Carve c;
+    const size_t of = c.take(n * G), ow = c.take(words * 4);
+    ix->io.reserve(c.off, ix->stream);
+    export_flat_dev(ix, ix->io.as<uint8_t>(of));
+    hip_check(vchip::launch_flat_to_gpu32(ix->io.as<uint8_t>(of), n, int(G), ix->io.as<uint32_t>(ow), ix->stream),
+              "flat_to_gpu32");
+    hip_check(hipMemcpyAsync(codes_host, ix->io.as<uint32_t>(ow), words * 4, hipMemcpyDeviceToHost, ix->stream),
+              "copy");
+    hip_check(hipStreamSynchronize(ix->stream), "sync");
+  });
+}
+
 vc_status vc_index_export_packed(vc_index* ix, uint8_t* packed_host) {
   return guarded([&] {
     if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");
@@ -759,6 +779,34 @@ vc_status vc_search_perm0(const float* queries, size_t nq, size_t dim, size_t bi
                  id_map ? sc.a.as<uint64_t>(om) : nullptr, sc.a.as<float>(os), sc.a.as<uint64_t>(oi), sc.scratch, sc.s);
     hip_check(hipMemcpyAsync(scores, sc.a.as<float>(os), nq * ke * 4, hipMemcpyDeviceToHost, sc.s), "copy");
     hip_check(hipMemcpyAsync(ids, sc.a.as<uint64_t>(oi), nq * ke * 8, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipStreamSynchronize(sc.s), "sync");
+  });
+}
+
+size_t vc_gpu32_words(size_t n, size_t dim, size_t bits) {
+  if (dim == 0 || dim % 8 != 0 || dim > 16384 || (bits != 2 && bits != 3 && bits != 4)) return 0;
+  const size_t G = bits == 2 ? dim / 4 : dim / 2;
+  return vchip::gpu32_words(n, int(G));
+}
+
+vc_status vc_gpu32_to_perm0(const uint32_t* codes, size_t n, size_t dim, size_t bits, uint8_t* perm0_out) {
+  return guarded([&] {
+    check_dim_bits(dim, bits);
+    if (n == 0) return;
+    if (codes == nullptr || perm0_out == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    const vchip::CodeTables t = make_tables(dim, bits);
+    const size_t G = size_t(t.groups);
+    const size_t words = vchip::gpu32_words(n, int(G)), pbytes = (n + 191) / 192 * G * 192;
+    Scoped sc;
+    Carve c;
+    const size_t ow = c.take(words * 4), of = c.take(n * G), op = c.take(pbytes);
+    sc.a.reserve(c.off, sc.s);
+    hip_check(hipMemcpyAsync(sc.a.as<uint32_t>(ow), codes, words * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(vchip::launch_gpu32_to_flat(sc.a.as<uint32_t>(ow), n, int(G), sc.a.as<uint8_t>(of), sc.s),
+              "gpu32_to_flat");
+    hip_check(vchip::launch_flat_to_perm0(sc.a.as<uint8_t>(of), n, int(G), sc.a.as<uint8_t>(op), sc.s),
+              "flat_to_perm0");
+    hip_check(hipMemcpyAsync(perm0_out, sc.a.as<uint8_t>(op), pbytes, hipMemcpyDeviceToHost, sc.s), "copy");
     hip_check(hipStreamSynchronize(sc.s), "sync");
   });
 }

@@ -118,6 +118,12 @@ hipError_t launch_tiled_to_flat(const std::uint8_t* tiled, std::size_t n, int gr
                                 std::uint8_t* flat, hipStream_t s);
 hipError_t launch_flat_to_tiled(const std::uint8_t* flat, std::size_t n, int groups, int groups16,
                                 std::uint8_t* tiled, hipStream_t s);
+// The reference CUDA index's layout: ceil(n/32) x ceil(G/4) x 32 u32 words.
+inline std::size_t gpu32_words(std::size_t n, int groups) { return (n + 31) / 32 * ((groups + 3) / 4) * 32; }
+hipError_t launch_flat_to_gpu32(const std::uint8_t* flat, std::size_t n, int groups, std::uint32_t* words,
+                                hipStream_t s);
+hipError_t launch_gpu32_to_flat(const std::uint32_t* words, std::size_t n, int groups, std::uint8_t* flat,
+                                hipStream_t s);
 hipError_t launch_flat_to_perm0(const std::uint8_t* flat, std::size_t n, int groups,
                                 std::uint8_t* perm0, hipStream_t s);
 hipError_t launch_flat_to_seq(const std::uint8_t* flat, std::size_t n, int groups, std::uint8_t* seq,

@@ -2321,6 +2321,41 @@ __global__ void flat_to_tiled_kernel(const uint8_t* __restrict__ flat, long n, i
   }
 }
 
+// The reference CUDA index's code layout (src/gpu/kernels.cuh:18-21,
+// src/gpu/kernels.cu:111-133): u32 words [ceil(n/32)][ceil(G/4)][32], word
+// (tile, g4, lane) = the byte groups 4 g4 .. 4 g4 + 3 of vector 32 tile +
+// lane, little-endian; groups past G and vectors past n are zero.
+__global__ void flat_to_gpu32_kernel(const uint8_t* __restrict__ flat, long n, int G, int nbg4,
+                                     uint32_t* __restrict__ words) {
+  const long total = (n + 31) / 32 * nbg4 * 32;
+  for (long w = blockIdx.x * (long)blockDim.x + threadIdx.x; w < total; w += (long)gridDim.x * blockDim.x) {
+    const int lane = (int)(w & 31);
+    const long tg = w >> 5;
+    const long tile = tg / nbg4;
+    const int g4 = (int)(tg - tile * nbg4);
+    const long v = tile * 32 + lane;
+    uint32_t x = 0u;
+    if (v < n) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int g = g4 * 4 + t;
+        if (g < G) x |= (uint32_t)flat[v * G + g] << (8 * t);
+      }
+    }
+    words[w] = x;
+  }
+}
+
+__global__ void gpu32_to_flat_kernel(const uint32_t* __restrict__ words, long n, int G, int nbg4,
+                                     uint8_t* __restrict__ flat) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n * G; i += (long)gridDim.x * blockDim.x) {
+    const long v = i / G;
+    const int g = (int)(i - v * G);
+    const uint32_t x = words[((v >> 5) * nbg4 + (g >> 2)) * 32 + (v & 31)];
+    flat[i] = (uint8_t)(x >> (8 * (g & 3)));
+  }
+}
+
 __constant__ unsigned char kPerm0Dev[16] = {0, 8, 1, 9, 2, 10, 3, 11, 4, 12, 5, 13, 6, 14, 7, 15};
 __constant__ unsigned char kInvPerm0Dev[16] = {0, 2, 4, 6, 8, 10, 12, 14, 1, 3, 5, 7, 9, 11, 13, 15};
 
@@ -2747,6 +2782,24 @@ hipError_t launch_tiled_to_flat(const std::uint8_t* tiled, std::size_t n, int gr
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(tiled_to_flat_kernel, dim3(grid_for((long)n * groups, 256)), dim3(256), 0, s, tiled,
                      (long)n, groups, groups16, flat);
+  return hipGetLastError();
+}
+
+hipError_t launch_flat_to_gpu32(const std::uint8_t* flat, std::size_t n, int groups, std::uint32_t* words,
+                                hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const int nbg4 = (groups + 3) / 4;
+  const long total = (long)((n + 31) / 32) * nbg4 * 32;
+  hipLaunchKernelGGL(flat_to_gpu32_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, flat, (long)n, groups, nbg4,
+                     words);
+  return hipGetLastError();
+}
+
+hipError_t launch_gpu32_to_flat(const std::uint32_t* words, std::size_t n, int groups, std::uint8_t* flat,
+                                hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(gpu32_to_flat_kernel, dim3(grid_for((long)n * groups, 256)), dim3(256), 0, s, words, (long)n,
+                     groups, (groups + 3) / 4, flat);
   return hipGetLastError();
 }
 

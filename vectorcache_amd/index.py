@@ -160,6 +160,13 @@ class TurboQuantIndex:
         check(lib.vc_index_export_perm0(self._h, _ptr(out)))
         return out
 
+    def export_gpu32(self) -> np.ndarray:
+        """The reference CUDA index's code words: u32 [ceil(n/32)][ceil(G/4)][32]
+        (src/gpu/kernels.cu:111-133), flattened."""
+        out = np.zeros(int(lib.vc_gpu32_words(S(self.size()), S(self.dim()), S(self.bit_width()))), np.uint32)
+        check(lib.vc_index_export_gpu32(self._h, _ptr(out)))
+        return out
+
     def export_seq(self) -> np.ndarray:
         """repack_seq bytes: sequential 192x1 blocks."""
         G = self.dim() // (8 // self.bit_width())
@@ -261,6 +268,20 @@ def search_perm0(queries, k: int, bits: int, perm0, alpha, id_map=None, shift=No
     check(lib.vc_search_perm0(_ptr(q), S(nq), S(dim), S(bits), S(k), _ptr(perm0), _ptr(alpha), S(n), _ptr(im),
                               _ptr(sh), _ptr(sc), _ptr(scores), _ptr(ids), ctypes.byref(kout)))
     return SearchResults(scores[:, : kout.value], ids[:, : kout.value])
+
+
+def gpu32_to_perm0(codes, n: int, dim: int, bits: int) -> np.ndarray:
+    """Codes in the reference CUDA index's layout (u32 words
+    [ceil(n/32)][ceil(G/4)][32]) as PERM0 blocks, the input of search_perm0 and
+    score_perm0_into."""
+    words = int(lib.vc_gpu32_words(S(n), S(dim), S(bits)))
+    codes = np.ascontiguousarray(codes, np.uint32).reshape(-1)
+    if words == 0 or codes.size != words:
+        raise ValueError(f"expected {words} code words for n={n}, dim={dim}, bits={bits}, got {codes.size}")
+    G = dim // (8 // bits)
+    out = np.zeros((n + 191) // 192 * G * 192, np.uint8)
+    check(lib.vc_gpu32_to_perm0(_ptr(codes), S(n), S(dim), S(bits), _ptr(out)))
+    return out
 
 
 def empty_heaps(nq: int, k: int):
