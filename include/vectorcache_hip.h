@@ -235,6 +235,51 @@ vc_status vc_set_profiling(int on);
 vc_status vc_set_debug(unsigned flags);
 vc_status vc_last_scan_stats(double* ms, size_t* tile, size_t* n_ranges);
 
+/* ---- on-disk formats of the recall runs (host code; no device) -------------
+ * NPY float32 (rows, dim) matrices through mmap (NpyReader, include/
+ * vectorcache/datasets/npy.hpp:33-65, src/datasets/npy.cpp): vc_npy_open maps
+ * the file and checks it as the reference does (version 1.0 or 2.0, descr
+ * '<f4', two dimensions, data complete); vc_npy_data is the zero-copy
+ * row-major view, valid until vc_npy_close; vc_npy_read_rows copies rows
+ * [first, first + n) clipped to the row count (rows copied in *n_read).
+ * File and format errors return VC_ERR_STATE with the reference's message
+ * (its vectorcache::Error, a std::runtime_error). */
+typedef struct vc_npy vc_npy;
+vc_status vc_npy_open(const char* path, vc_npy** out, size_t* count, size_t* dim);
+const float* vc_npy_data(const vc_npy* npy);
+vc_status vc_npy_read_rows(const vc_npy* npy, size_t first, size_t n, float* out, size_t* n_read);
+vc_status vc_npy_close(vc_npy* npy);
+/* write_npy_f32_matrix (src/datasets/npy.cpp:336-366): the same bytes
+ * (version 1.0; 10 + the header dict padded with blanks to a multiple of 16,
+ * then a newline); parent directories are created. */
+vc_status vc_npy_write_f32(const char* path, size_t rows, size_t cols, const float* data);
+
+/* VCEX exact top-k cache of query-bench (tools/query-bench.cpp:146-340):
+ * "VCEX", u32 version 1, u64 query_n, k, index_n, seed, three u32-length
+ * strings (source label, split, query split), then query_n x k u64 ids.
+ * vc_vcex_save writes path.tmp and renames it (VC_ERR_STATE when any step
+ * fails); vc_vcex_load sets *loaded = 1 and fills ids (query_n x k) only when
+ * the file exists, is version 1 and was made for exactly this key, else 0.
+ * vc_vcex_source_label (cache_source_label: "npy-<stem>-<FNV-1a 64 of the
+ * normalized path, hex>", or `dataset` for an empty path) and
+ * vc_vcex_cache_path (exact_topk_cache_path: ".cache/exact_topk/<label>_
+ * <split>_n<N>_qs-<query split>_ql<nq>_k<k>_seed<seed>.bin", characters
+ * other than [A-Za-z0-9_-] in the strings replaced by '_') report the length
+ * in *len and write the NUL-terminated string when cap > *len. */
+typedef struct {
+  const char* source_label;
+  const char* split;
+  const char* query_split;
+  uint64_t index_n;
+  uint64_t query_n;
+  uint64_t k;
+  uint64_t seed;
+} vc_vcex_key;
+vc_status vc_vcex_save(const char* path, const vc_vcex_key* key, const uint64_t* ids);
+vc_status vc_vcex_load(const char* path, const vc_vcex_key* key, uint64_t* ids, int* loaded);
+vc_status vc_vcex_source_label(const char* npy_path, const char* dataset, char* out, size_t cap, size_t* len);
+vc_status vc_vcex_cache_path(const vc_vcex_key* key, char* out, size_t cap, size_t* len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <filesystem>
 #include <limits>
 #include <random>
 #include <span>
@@ -15,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "vectorcache/datasets/npy.hpp"
 #include "vectorcache/turbo_quant_index.hpp"
 
 namespace {
@@ -144,6 +146,28 @@ int main() {
   for (std::size_t q = 0; q < nq; ++q) expect(rt.ids[q * k] == q, "TQ+ self-query");
   expect_throw<std::runtime_error>([&] { tq.calibrate(sample); }, "calibrate after prepare");
 
+  // NpyReader / write_npy_f32_matrix (reference include/vectorcache/datasets/npy.hpp:33-65)
+  {
+    namespace ds = vectorcache::datasets;
+    const std::filesystem::path p = std::filesystem::temp_directory_path() / "vc_cpp_api_check" / "m.npy";
+    std::vector<float> m(5 * 3);
+    for (std::size_t i = 0; i < m.size(); ++i) m[i] = float(i) * 0.5f;
+    ds::write_npy_f32_matrix(p, 5, 3, m.data());
+    auto r = ds::NpyReader::open(p, "m");
+    expect(r.meta().count == 5 && r.meta().dim == 3 && std::string(r.meta().label) == "m", "npy meta");
+    auto v = r.next_vector_span();
+    expect(v && v->size() == 3 && (*v)[2] == 1.0f, "npy first row");
+    std::vector<float> row(3);
+    expect(r.next_vector_into(row) && row[0] == 1.5f, "npy second row");
+    std::vector<float> wrong(4);
+    expect_throw<std::runtime_error>([&] { (void)r.next_vector_into(wrong); }, "npy row size mismatch");
+    std::vector<float> rest(9);
+    expect(r.copy_rows_into(rest, 10) == 3 && rest[8] == 7.0f, "npy bulk copy");
+    expect(!r.next_vector_span(), "npy end");
+    expect_throw<std::runtime_error>([&] { (void)ds::NpyReader::open(p.parent_path() / "none.npy", "x"); },
+                                     "npy missing file");
+    std::filesystem::remove_all(p.parent_path());
+  }
   std::printf("OK %d\n", g_checks);
   return 0;
 }

@@ -32,7 +32,16 @@ class DeviceOutOfMemory(VectorCacheError):
 
 
 class IndexStateError(RuntimeError):
-    """The reference's std::runtime_error for call-order errors (search before prepare)."""
+    """The reference's std::runtime_error for call-order errors (search before
+    prepare) and for file / format errors (its vectorcache::Error)."""
+
+
+class VcexKey(ctypes.Structure):
+    """vc_vcex_key: the key of a VCEX exact top-k cache file."""
+
+    _fields_ = [("source_label", ctypes.c_char_p), ("split", ctypes.c_char_p), ("query_split", ctypes.c_char_p),
+                ("index_n", ctypes.c_uint64), ("query_n", ctypes.c_uint64), ("k", ctypes.c_uint64),
+                ("seed", ctypes.c_uint64)]
 
 
 _SIGS = {
@@ -80,6 +89,15 @@ _SIGS = {
     "vc_set_profiling": ([I], I),
     "vc_set_debug": ([ctypes.c_uint], I),
     "vc_last_scan_stats": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(S), ctypes.POINTER(S)], I),
+    "vc_npy_open": ([ctypes.c_char_p, ctypes.POINTER(P), ctypes.POINTER(S), ctypes.POINTER(S)], I),
+    "vc_npy_data": ([P], ctypes.POINTER(ctypes.c_float)),
+    "vc_npy_read_rows": ([P, S, S, P, ctypes.POINTER(S)], I),
+    "vc_npy_close": ([P], I),
+    "vc_npy_write_f32": ([ctypes.c_char_p, S, S, P], I),
+    "vc_vcex_save": ([ctypes.c_char_p, ctypes.POINTER(VcexKey), P], I),
+    "vc_vcex_load": ([ctypes.c_char_p, ctypes.POINTER(VcexKey), P, ctypes.POINTER(I)], I),
+    "vc_vcex_source_label": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, S, ctypes.POINTER(S)], I),
+    "vc_vcex_cache_path": ([ctypes.POINTER(VcexKey), ctypes.c_char_p, S, ctypes.POINTER(S)], I),
 }
 
 for _name, (_args, _res) in _SIGS.items():

@@ -3,7 +3,7 @@
     python -m vectorcache_amd.build          # or vectorcache_amd.build.build()
 
 * vc_kernels.hip, vc_capi.cpp  -> hipcc --offload-arch=gfx950
-* vc_host.cpp                  -> g++ (host precompute, contraction off)
+* vc_host.cpp, vc_io.cpp       -> g++ (host precompute, contraction off; file formats)
 
 Floating-point flags are part of the numerics contract: -ffp-contract=off
 everywhere (fused operations are explicit in the source), and packed FP32
@@ -47,9 +47,10 @@ def build(verbose: bool = True) -> Path:
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         _run([HIPCC, *HIP_FLAGS, *lang, "-c", CSRC / src, "-o", obj])
         objs.append(obj)
-    obj = BUILD / "vc_host.cpp.o"
-    _run(["g++", *HOST_FLAGS, "-c", CSRC / "vc_host.cpp", "-o", obj])
-    objs.append(obj)
+    for src in ("vc_host.cpp", "vc_io.cpp"):
+        obj = BUILD / (src + ".o")
+        _run(["g++", *HOST_FLAGS, "-c", CSRC / src, "-o", obj])
+        objs.append(obj)
     tmp = OUT.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs])
     tmp.replace(OUT)

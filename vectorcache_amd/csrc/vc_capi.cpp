@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <stdexcept>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "vc_host.hpp"
+#include "vc_io.hpp"
 #include "vc_internal.hpp"
 #include "vectorcache_hip.h"
 
@@ -1061,3 +1063,101 @@ size_t vc_scan_tile_for(size_t dim, size_t bits, size_t nq) {
 }
 
 }  // extern "C"
+
+// ---- on-disk formats (host) -------------------------------------------------
+struct vc_npy {
+  std::unique_ptr<vcio::NpyMatrix> m;
+};
+
+template <typename F>
+vc_status guarded_io(F&& f) {
+  return guarded([&] {
+    try {
+      f();
+    } catch (const vcio::IoError& e) {
+      fail(VC_ERR_STATE, e.what());
+    }
+  });
+}
+
+static vcio::ExactTopkKey vcex_key(const vc_vcex_key* k) {
+  if (k == nullptr || k->source_label == nullptr || k->split == nullptr || k->query_split == nullptr)
+    fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+  vcio::ExactTopkKey key;
+  key.source_label = k->source_label;
+  key.split = k->split;
+  key.query_split = k->query_split;
+  key.index_n = k->index_n;
+  key.query_n = k->query_n;
+  key.k = k->k;
+  key.seed = k->seed;
+  return key;
+}
+
+static void put_string(const std::string& s, char* out, size_t cap, size_t* len) {
+  if (len == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+  *len = s.size();
+  if (out != nullptr && cap > s.size()) std::memcpy(out, s.c_str(), s.size() + 1);
+}
+
+vc_status vc_npy_open(const char* path, vc_npy** out, size_t* count, size_t* dim) {
+  return guarded_io([&] {
+    if (path == nullptr || out == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    auto h = std::make_unique<vc_npy>();
+    h->m = std::make_unique<vcio::NpyMatrix>(path);
+    if (count) *count = h->m->count();
+    if (dim) *dim = h->m->dim();
+    *out = h.release();
+  });
+}
+
+const float* vc_npy_data(const vc_npy* npy) { return npy ? npy->m->data() : nullptr; }
+
+vc_status vc_npy_read_rows(const vc_npy* npy, size_t first, size_t n, float* out, size_t* n_read) {
+  return guarded_io([&] {
+    if (npy == nullptr || (out == nullptr && n > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    const size_t got = npy->m->copy_rows(first, n, out);
+    if (n_read) *n_read = got;
+  });
+}
+
+vc_status vc_npy_close(vc_npy* npy) {
+  delete npy;
+  return VC_OK;
+}
+
+vc_status vc_npy_write_f32(const char* path, size_t rows, size_t cols, const float* data) {
+  return guarded_io([&] {
+    if (path == nullptr || (data == nullptr && rows * cols > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    vcio::write_npy_f32(path, rows, cols, data);
+  });
+}
+
+vc_status vc_vcex_save(const char* path, const vc_vcex_key* key, const uint64_t* ids) {
+  return guarded_io([&] {
+    const vcio::ExactTopkKey k = vcex_key(key);
+    if (path == nullptr || (ids == nullptr && k.query_n * k.k > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    if (!vcio::save_exact_topk(path, k, ids)) fail(VC_ERR_STATE, std::string("failed to write exact top-k cache ") + path);
+  });
+}
+
+vc_status vc_vcex_load(const char* path, const vc_vcex_key* key, uint64_t* ids, int* loaded) {
+  return guarded_io([&] {
+    const vcio::ExactTopkKey k = vcex_key(key);
+    if (path == nullptr || loaded == nullptr || (ids == nullptr && k.query_n * k.k > 0))
+      fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::vector<uint64_t> v;
+    *loaded = vcio::load_exact_topk(path, k, v) ? 1 : 0;
+    if (*loaded && !v.empty()) std::memcpy(ids, v.data(), v.size() * 8);
+  });
+}
+
+vc_status vc_vcex_source_label(const char* npy_path, const char* dataset, char* out, size_t cap, size_t* len) {
+  return guarded_io([&] {
+    put_string(vcio::cache_source_label(npy_path ? npy_path : "", dataset ? dataset : ""), out, cap, len);
+  });
+}
+
+vc_status vc_vcex_cache_path(const vc_vcex_key* key, char* out, size_t cap, size_t* len) {
+  return guarded_io([&] { put_string(vcio::exact_topk_cache_path(vcex_key(key)), out, cap, len); });
+}
