@@ -163,6 +163,24 @@ vc_status vc_search_perm0(const float* queries, size_t nq, size_t dim, size_t bi
  * in the last word of a row are ignored. */
 vc_status vc_gpu32_to_perm0(const uint32_t* codes, size_t n, size_t dim, size_t bits, uint8_t* perm0_out);
 
+/* score_prepared with every argument the reference's AVX2 path honours
+ * (src/search/search.cpp:1072-1154, include/vectorcache/search/search.hpp:
+ * 156-165): as vc_search_perm0, over the queries listed in query_indices
+ * (n_indices of them, repeats allowed; NULL = all nq; an index >= nq is
+ * VC_ERR_OUT_OF_RANGE), and, when both residual_norms (n) and residual_scales
+ * (nq) are given, each kept score s of vector v becomes
+ * fmaf(residual_scales[query], residual_norms[v], s) and the row is ranked
+ * again by (score desc, v asc) before ids go through id_map (the landmark
+ * path's correction: the top-k membership is decided on the LUT scores).
+ * Output nq_out x min(k, n), nq_out = n_indices or nq; *k_out = 0 when n or
+ * nq_out is 0. scale_resid, hit bitmaps, buffered collectors and seeds are
+ * arguments of the reference's VNNI backends only and have no counterpart. */
+vc_status vc_score_perm0(const float* queries, size_t nq, size_t dim, size_t bits, size_t k,
+                         const uint8_t* perm0, const float* alpha, size_t n, const uint64_t* id_map,
+                         const float* shift, const float* scale, const size_t* query_indices,
+                         size_t n_indices, const float* residual_norms, const float* residual_scales,
+                         float* scores, uint64_t* ids, size_t* k_out);
+
 /* score_prepared_into (src/search/search.cpp:1156-1267, include/vectorcache/
  * search/search.hpp:163-177) over one bucket given as PERM0 blocks + alpha:
  * the nq queries (host, nq x dim) are the prepared batch; only the queries in

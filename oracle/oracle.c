@@ -813,6 +813,82 @@ int oc_search_flat(const float* q, size_t nq, size_t dim, size_t k, size_t bits,
   return rc;
 }
 
+/* score_prepared on the AVX2 path with every argument (reference
+ * src/search/search.cpp:1072-1154): the listed queries (query_indices, or all
+ * nq; each < nq, else -2), top k_eff = min(k, n) by local position; then,
+ * when both residual_norms (n) and residual_scales (nq) are given, each kept
+ * score s of vector vi < n becomes fmaf(residual_scales[query],
+ * residual_norms[vi], s) (the reference's `+= rs * resid[vi]` under its
+ * contraction flags, :1123-1131) and the row is sorted again by (score desc,
+ * position asc) (:1132-1146); ids go through id_map last (remap_ids). */
+int oc_score_prepared(const float* q, size_t nq, size_t dim, size_t k, size_t bits,
+                      const uint8_t* blocked_perm0, const float* alphas, size_t n, const float* shift,
+                      const float* scale, const uint64_t* id_map, const size_t* query_indices,
+                      size_t n_indices, const float* residual_norms, const float* residual_scales,
+                      float* scores_out, uint64_t* ids_out, size_t* k_out) {
+  if (dim == 0 || dim % 8 != 0 || bits < 2 || bits > 4) return -1;
+  if (query_indices != NULL)
+    for (size_t i = 0; i < n_indices; ++i)
+      if (query_indices[i] >= nq) return -2;
+  const size_t nq_out = query_indices != NULL ? n_indices : nq;
+  *k_out = 0;
+  if (n == 0 || nq_out == 0) return 0;
+  const size_t G = groups_of(bits, dim);
+  float* q_rot = (float*)malloc(nq * dim * sizeof(float) + 1);
+  float* bc = (float*)malloc(nq * sizeof(float) + 1);
+  float* ls = (float*)malloc(nq * sizeof(float) + 1);
+  float* lb = (float*)malloc(nq * sizeof(float) + 1);
+  uint8_t* luts = (uint8_t*)malloc(nq * G * 32 + 1);
+  float* ls_o = (float*)malloc(nq_out * sizeof(float) + 1);
+  float* lb_o = (float*)malloc(nq_out * sizeof(float) + 1);
+  uint8_t* luts_o = (uint8_t*)malloc(nq_out * G * 32 + 1);
+  uint8_t* flat = (uint8_t*)malloc(n * G + 1);
+  int rc = oc_prepare_queries(q, nq, dim, bits, shift, scale, q_rot, bc, luts, ls, lb);
+  if (rc == 0) {
+    for (size_t l = 0; l < nq_out; ++l) {
+      const size_t g = query_indices != NULL ? query_indices[l] : l;
+      ls_o[l] = ls[g];
+      lb_o[l] = lb[g] + bc[g];
+      memcpy(luts_o + l * G * 32, luts + g * G * 32, G * 32);
+    }
+    oc_unpack_perm0(blocked_perm0, n, G, flat);
+    rc = oc_scan_topk(luts_o, ls_o, lb_o, nq_out, flat, n, G, alphas, NULL, k, scores_out, ids_out, k_out);
+  }
+  const size_t ke = *k_out;
+  if (rc == 0 && residual_norms != NULL && residual_scales != NULL && ke > 0) {
+    oc_cand* row = (oc_cand*)malloc(ke * sizeof(oc_cand));
+    for (size_t l = 0; l < nq_out; ++l) {
+      const float rs = residual_scales[query_indices != NULL ? query_indices[l] : l];
+      for (size_t j = 0; j < ke; ++j) {
+        const uint64_t vi = ids_out[l * ke + j];
+        float s = scores_out[l * ke + j];
+        if (vi < n) s = fmaf(rs, residual_norms[vi], s);
+        row[j].s = s;
+        row[j].id = vi;
+      }
+      qsort(row, ke, sizeof(oc_cand), cand_cmp);
+      for (size_t j = 0; j < ke; ++j) {
+        scores_out[l * ke + j] = row[j].s;
+        ids_out[l * ke + j] = row[j].id;
+      }
+    }
+    free(row);
+  }
+  if (rc == 0 && id_map != NULL)
+    for (size_t i = 0; i < nq_out * ke; ++i)
+      if (ids_out[i] < n) ids_out[i] = id_map[ids_out[i]];
+  free(q_rot);
+  free(bc);
+  free(ls);
+  free(lb);
+  free(luts);
+  free(ls_o);
+  free(lb_o);
+  free(luts_o);
+  free(flat);
+  return rc;
+}
+
 /* merge_search_results (reference src/search/search.cpp:1551-1600): union of
  * both rows, duplicates by id keep the higher score, top k_dst by (score desc,
  * id asc), short rows padded with (0.f, 0). */

@@ -52,6 +52,11 @@ int oc_heaps_to_results(size_t nq, size_t k, const float* heap_s, const uint64_t
                         const size_t* heap_sz, float* scores_out, uint64_t* ids_out);
 int oc_float_rescore(size_t nq, size_t k, float* scores, uint64_t* ids, const size_t* counts,
                      const float* queries, const float* corpus, size_t n_corpus, size_t dim);
+int oc_score_prepared(const float* q, size_t nq, size_t dim, size_t k, size_t bits,
+                      const uint8_t* blocked_perm0, const float* alphas, size_t n, const float* shift,
+                      const float* scale, const uint64_t* id_map, const size_t* query_indices,
+                      size_t n_indices, const float* residual_norms, const float* residual_scales,
+                      float* scores_out, uint64_t* ids_out, size_t* k_out);
 int oc_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t* dst_ids, size_t k_src,
                      const float* src_scores, const uint64_t* src_ids);
 void oc_set_threads(int n);

@@ -215,6 +215,37 @@ class Oracle:
         ko = int(kout.value)
         return scores[: nq * ko].reshape(nq, ko), ids[: nq * ko].reshape(nq, ko)
 
+    def score_prepared(self, q, k, bits, blocked_perm0, alphas, shift=None, scale=None, id_map=None,
+                       query_indices=None, residual_norms=None, residual_scales=None):
+        """score_prepared with id_map, query_indices and the residual
+        post-correction (AVX2 path). Returns (scores, ids) nq_out x k_eff."""
+        q = np.ascontiguousarray(q, np.float32)
+        nq, dim = q.shape
+        alphas = np.ascontiguousarray(alphas, np.float32)
+        n = alphas.size
+        blocked_perm0 = np.ascontiguousarray(blocked_perm0, np.uint8)
+        qi = None if query_indices is None else np.ascontiguousarray(query_indices, np.uint64)
+        nq_out = nq if qi is None else qi.size
+        kk = max(min(k, n), 0)
+        scores = np.zeros(max(nq_out * kk, 1), np.float32)
+        ids = np.zeros(max(nq_out * kk, 1), np.uint64)
+        kout = ctypes.c_size_t()
+        cv = lambda a, t: None if a is None else np.ascontiguousarray(a, t)
+        sh, sc, im = cv(shift, np.float32), cv(scale, np.float32), cv(id_map, np.uint64)
+        rn, rs = cv(residual_norms, np.float32), cv(residual_scales, np.float32)
+        args = (_ptr(im), _ptr(qi), _S(0 if qi is None else qi.size), _ptr(rn), _ptr(rs), _ptr(scores), _ptr(ids),
+                ctypes.byref(kout))
+        if self.kind == "reference":
+            nb = (n + 191) // 192
+            rc = self.lib.vcref_score_prepared(_ptr(q), _S(nq), _S(dim), _S(k), _S(bits), _ptr(blocked_perm0),
+                                               _S(nb), _ptr(alphas), _S(n), _ptr(sh), _ptr(sc), *args)
+        else:
+            rc = self.lib.oc_score_prepared(_ptr(q), _S(nq), _S(dim), _S(k), _S(bits), _ptr(blocked_perm0),
+                                            _ptr(alphas), _S(n), _ptr(sh), _ptr(sc), *args)
+        self._check(rc, "score_prepared")
+        ko = int(kout.value)
+        return scores[: nq_out * ko].reshape(nq_out, ko), ids[: nq_out * ko].reshape(nq_out, ko)
+
     def scan_topk(self, luts, lut_scale, lut_bias_total, flat, alphas, k, id_map=None):
         """Scan + top-k from prepared LUTs and row-major flat codes (port only)."""
         luts = np.ascontiguousarray(luts, np.uint8)

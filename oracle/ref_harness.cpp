@@ -254,6 +254,43 @@ int vcref_search_flat_idmap(const float* q, size_t nq, size_t dim, size_t k, siz
   });
 }
 
+// score_prepared() with every argument of the AVX2 path (reference
+// src/search/search.cpp:1072-1154): id_map, query_indices and the residual
+// post-correction (residual_norms per vector, residual_scales per query).
+// Writes nq_out * k_eff scores/ids, nq_out = n_indices or nq.
+int vcref_score_prepared(const float* q, size_t nq, size_t dim, size_t k, size_t bits,
+                         const uint8_t* blocked, size_t n_blocks, const float* alphas, size_t n,
+                         const float* shift, const float* scale, const uint64_t* id_map,
+                         const size_t* query_indices, size_t n_indices, const float* residual_norms,
+                         const float* residual_scales, float* scores_out, uint64_t* ids_out,
+                         size_t* k_out) {
+  return guarded([&] {
+    vc::Rotation rot(dim);
+    auto [b, c] = vc::codebook(bits, dim);
+    std::span<const float> sh, sc;
+    if (shift != nullptr && scale != nullptr) {
+      sh = std::span<const float>(shift, dim);
+      sc = std::span<const float>(scale, dim);
+    }
+    auto prep = vc::prepare_queries(std::span<const float>(q, nq * dim), nq, dim, rot, c, bits, sh, sc);
+    const auto geo = vc::blocked_geometry(n, bits, dim);
+    std::span<const uint64_t> im;
+    if (id_map != nullptr) im = std::span<const uint64_t>(id_map, n);
+    std::span<const size_t> qi;
+    if (query_indices != nullptr) qi = std::span<const size_t>(query_indices, n_indices);
+    std::span<const float> rn, rs;
+    if (residual_norms != nullptr) rn = std::span<const float>(residual_norms, n);
+    if (residual_scales != nullptr) rs = std::span<const float>(residual_scales, nq);
+    auto res = vc::score_prepared(prep, k, std::span<const uint8_t>(blocked, geo.blocked_len), n_blocks,
+                                  std::span<const float>(alphas, n), im, qi, rn, rs);
+    *k_out = res.k;
+    if (res.k > 0) {
+      std::memcpy(scores_out, res.scores.data(), res.nq * res.k * sizeof(float));
+      std::memcpy(ids_out, res.ids.data(), res.nq * res.k * sizeof(uint64_t));
+    }
+  });
+}
+
 // merge_search_results() (reference src/search/search.cpp:1551-1600).
 int vcref_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t* dst_ids,
                         size_t k_src, const float* src_scores, const uint64_t* src_ids) {

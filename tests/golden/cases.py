@@ -28,6 +28,31 @@ def case_data(bits: int, dim: int, n: int, idx: int):
 RESCORE_DIMS = (8, 24, 128, 1536)
 
 
+# score_prepared with every argument: (bits, dim, n, nq, k)
+SCORE_PREPARED_CASES = ((4, 128, 3000, 5, 10), (2, 256, 1000, 4, 64), (3, 128, 777, 3, 200))
+
+
+def score_prepared_inputs(bits, dim, n, nq):
+    """Seeded vectors, queries, an id map, a query subset with a repeat, and
+    residual norms / scales (the landmark path's arguments)."""
+    x = synthetic.mixture(n, dim, 16, 0.3, 6300 + n, 6400 + dim)
+    q = synthetic.mixture(nq, dim, 16, 0.3, 6300 + n, 6500 + nq)
+    r = np.random.default_rng(6600 + n)
+    id_map = (r.permutation(n).astype(np.uint64) * 7 + 3).astype(np.uint64)
+    query_indices = np.array([nq - 1, 0, nq - 1], np.uint64)
+    residual_norms = (r.random(n) * 0.5).astype(np.float32)
+    residual_scales = (r.random(nq) * 2).astype(np.float32)
+    return x, q, id_map, query_indices, residual_norms, residual_scales
+
+
+def SCORE_PREPARED_VARIANTS(id_map, qi, rn, rs):
+    """(tag, keyword arguments) of the score_prepared golden outputs."""
+    return (("plain", {}), ("idmap", dict(id_map=id_map)), ("subset", dict(query_indices=qi)),
+            ("resid", dict(residual_norms=rn, residual_scales=rs)),
+            ("all", dict(id_map=id_map, query_indices=qi, residual_norms=rn, residual_scales=rs)),
+            ("norms_only", dict(residual_norms=rn)))
+
+
 def rescore_inputs(dim):
     """Seeded float_rescore_topk inputs: corpus (300 x dim), 6 queries, rows of
     12 candidates with ids past the corpus and a repeated id, per-row counts."""

@@ -31,7 +31,8 @@ sys.path.insert(0, str(Path(__file__).resolve().parent))
 
 from oracle import Oracle  # noqa: E402
 from vectorcache_amd import synthetic  # noqa: E402
-from cases import CASES, KS, RESCORE_DIMS, case_data, rescore_inputs  # noqa: E402
+from cases import (CASES, KS, RESCORE_DIMS, SCORE_PREPARED_CASES, SCORE_PREPARED_VARIANTS,  # noqa: E402
+                   case_data, rescore_inputs, score_prepared_inputs)
 
 OUT = Path(__file__).resolve().parent
 
@@ -128,6 +129,21 @@ def rescore_fixture(R):
     np.savez_compressed(OUT / "float_rescore.npz", **out)
 
 
+def score_prepared_fixture(R):
+    """score_prepared (AVX2 path) with id_map, query_indices and the residual
+    post-correction, from the reference build."""
+    out = {}
+    for ci, (bits, dim, n, nq, k) in enumerate(SCORE_PREPARED_CASES):
+        x, q, id_map, qi, rn, rs = score_prepared_inputs(bits, dim, n, nq)
+        packed, alpha = R.encode(x, bits)
+        perm0 = R.repack_perm0(packed, bits, dim)
+        for tag, kw in SCORE_PREPARED_VARIANTS(id_map, qi, rn, rs):
+            s, i = R.score_prepared(q, k, bits, perm0, alpha, **kw)
+            out[f"c{ci}_{tag}_scores"] = s
+            out[f"c{ci}_{tag}_ids"] = i
+    np.savez_compressed(OUT / "score_prepared.npz", **out)
+
+
 def main():
     subprocess.run(["make", "-C", str(ROOT / "oracle"), "ref"], check=True, stdout=subprocess.DEVNULL)
     R = Oracle("reference")
@@ -136,6 +152,8 @@ def main():
         score_into_fixture(R)
     if only in (None, "rescore"):
         rescore_fixture(R)
+    if only in (None, "score_prepared"):
+        score_prepared_fixture(R)
     if only is not None:
         print("wrote", only)
         return

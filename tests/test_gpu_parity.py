@@ -8,6 +8,7 @@ bias, scores and top-k ids (ties by lower id). Scores compare with -0.0 ==
 import numpy as np
 import pytest
 
+from conftest import load_golden
 from util import bits_equal, scores_equal, sha
 
 pytestmark = pytest.mark.gpu
@@ -91,6 +92,34 @@ def test_gpu32_layout_export_and_import(vc, oracle, golden_cases):
         r = vc.search_perm0(q, 10, c["bits"], p, ix.export_alpha(), shift=sh, scale=sc)
         assert np.array_equal(r.ids, g["ids_k10"]), c
         assert scores_equal(r.scores, g["scores_k10"]), c
+
+
+def test_score_prepared_arguments_match_golden(vc, oracle):
+    """score_prepared with id_map, query subsets (with a repeat) and the
+    residual post-correction, against the reference build's golden outputs;
+    then a larger random case against the oracle."""
+    from cases import SCORE_PREPARED_CASES, SCORE_PREPARED_VARIANTS, score_prepared_inputs
+
+    g = load_golden("score_prepared.npz")
+    for ci, (bits, dim, n, nq, k) in enumerate(SCORE_PREPARED_CASES):
+        x, q, *extra = score_prepared_inputs(bits, dim, n, nq)
+        packed, alpha = oracle.encode(x, bits)
+        perm0 = oracle.repack_perm0(packed, bits, dim)
+        for tag, kw in SCORE_PREPARED_VARIANTS(*extra):
+            r = vc.score_perm0(q, k, bits, perm0, alpha, **kw)
+            assert np.array_equal(r.ids, g[f"c{ci}_{tag}_ids"]), (ci, tag)
+            assert bits_equal(r.scores, g[f"c{ci}_{tag}_scores"]), (ci, tag)
+    x, q, id_map, qi, rn, rs = score_prepared_inputs(4, 768, 20000, 9)
+    packed, alpha = oracle.encode(x, 4)
+    perm0 = oracle.repack_perm0(packed, 4, 768)
+    qi = np.array([8, 3, 3, 0, 5], np.uint64)
+    kw = dict(id_map=id_map, query_indices=qi, residual_norms=rn, residual_scales=rs)
+    for k in (10, 700):
+        s_o, i_o = oracle.score_prepared(q, k, 4, perm0, alpha, **kw)
+        r = vc.score_perm0(q, k, 4, perm0, alpha, **kw)
+        assert np.array_equal(r.ids, i_o) and bits_equal(r.scores, s_o), k
+    with pytest.raises(IndexError):
+        vc.score_perm0(q, 10, 4, perm0, alpha, query_indices=np.array([9], np.uint64))
 
 
 def test_sequential_bias_corr_matches_golden(vc, golden_cases):

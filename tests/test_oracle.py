@@ -11,6 +11,7 @@
 """
 import ctypes
 import math
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -386,3 +387,53 @@ def test_gpu32_round_trip(oracle):
         assert np.array_equal(oracle.gpu32_to_flat(w, n, G), flat)
         # a bijection between the valid bytes: every flat byte appears once
         assert int(np.count_nonzero(w.view(np.uint8))) == int(np.count_nonzero(flat))
+
+
+def _score_prepared_codes(oracle, bits, dim, n, nq):
+    from cases import score_prepared_inputs
+
+    x, q, id_map, qi, rn, rs = score_prepared_inputs(bits, dim, n, nq)
+    packed, alpha = oracle.encode(x, bits)
+    return q, oracle.repack_perm0(packed, bits, dim), alpha, (id_map, qi, rn, rs)
+
+
+def test_score_prepared_golden(oracle):
+    """score_prepared with id_map, query subsets (with a repeat) and the
+    residual post-correction: bit-identical to the reference build's outputs."""
+    from cases import SCORE_PREPARED_CASES, SCORE_PREPARED_VARIANTS
+
+    g = load_golden("score_prepared.npz")
+    for ci, (bits, dim, n, nq, k) in enumerate(SCORE_PREPARED_CASES):
+        q, perm0, alpha, extra = _score_prepared_codes(oracle, bits, dim, n, nq)
+        for tag, kw in SCORE_PREPARED_VARIANTS(*extra):
+            s, i = oracle.score_prepared(q, k, bits, perm0, alpha, **kw)
+            assert np.array_equal(i, g[f"c{ci}_{tag}_ids"]), (ci, tag)
+            assert bits_equal(s, g[f"c{ci}_{tag}_scores"]), (ci, tag)
+
+
+def test_score_prepared_residual_is_a_rerank(oracle):
+    """What the mathematics fixes: the correction keeps the LUT top-k
+    membership, adds rs * ||r|| per kept vector (one rounding) and re-sorts."""
+    q, perm0, alpha, (id_map, qi, rn, rs) = _score_prepared_codes(oracle, 4, 128, 3000, 5)
+    s0, i0 = oracle.score_prepared(q, 50, 4, perm0, alpha)
+    s1, i1 = oracle.score_prepared(q, 50, 4, perm0, alpha, residual_norms=rn, residual_scales=rs)
+    for r in range(q.shape[0]):
+        assert set(i0[r].tolist()) == set(i1[r].tolist())
+        exact = {int(v): Fraction(float(s)) + Fraction(float(rs[r])) * Fraction(float(rn[v]))
+                 for s, v in zip(s0[r], i0[r])}
+        for s, v in zip(s1[r], i1[r]):
+            # s is the float32 nearest the exact s0 + rs * ||r|| (one rounding)
+            e = exact[int(v)]
+            lo, hi = np.nextafter(s, np.float32(-np.inf)), np.nextafter(s, np.float32(np.inf))
+            d = abs(Fraction(float(s)) - e)
+            assert d <= abs(Fraction(float(lo)) - e) and d <= abs(Fraction(float(hi)) - e), (r, int(v))
+        assert np.all(np.diff(s1[r]) <= 0)
+
+
+def test_score_prepared_vs_reference_build(oracle, ref):
+    for bits, dim, n, nq, k in [(4, 256, 2500, 6, 33), (2, 128, 900, 3, 900)]:
+        q, perm0, alpha, (id_map, qi, rn, rs) = _score_prepared_codes(oracle, bits, dim, n, nq)
+        for kw in ({}, dict(id_map=id_map, query_indices=qi, residual_norms=rn, residual_scales=rs)):
+            a = oracle.score_prepared(q, k, bits, perm0, alpha, **kw)
+            b = ref.score_prepared(q, k, bits, perm0, alpha, **kw)
+            assert np.array_equal(a[1], b[1]) and bits_equal(a[0], b[0]), (bits, dim, list(kw))

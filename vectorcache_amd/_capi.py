@@ -70,6 +70,7 @@ _SIGS = {
     "vc_index_export_alpha": ([P, P], I),
     "vc_index_export_gpu32": ([P, P], I),
     "vc_gpu32_words": ([S, S, S], S),
+    "vc_score_perm0": ([P, S, S, S, S, P, P, S, P, P, P, P, S, P, P, P, P, ctypes.POINTER(S)], I),
     "vc_gpu32_to_perm0": ([P, S, S, S, P], I),
     "vc_index_get_calibration": ([P, P, P], I),
     "vc_index_prepare_queries": ([P, P, S, P, P, P, P], I),

@@ -785,6 +785,82 @@ vc_status vc_search_perm0(const float* queries, size_t nq, size_t dim, size_t bi
   });
 }
 
+vc_status vc_score_perm0(const float* queries, size_t nq, size_t dim, size_t bits, size_t k, const uint8_t* perm0,
+                         const float* alpha, size_t n, const uint64_t* id_map, const float* shift,
+                         const float* scale, const size_t* query_indices, size_t n_indices,
+                         const float* residual_norms, const float* residual_scales, float* scores, uint64_t* ids,
+                         size_t* k_out) {
+  return guarded([&] {
+    check_dim_bits(dim, bits);
+    if (query_indices != nullptr)
+      for (size_t i = 0; i < n_indices; ++i)
+        if (query_indices[i] >= nq) fail(VC_ERR_OUT_OF_RANGE, "score_prepared: query_indices out of range");
+    const size_t nq_out = query_indices != nullptr ? n_indices : nq;
+    if (k_out) *k_out = 0;
+    if (n == 0 || nq_out == 0) return;
+    const size_t ke = std::min(k, n);
+    if (k_out) *k_out = ke;
+    if (ke == 0) return;
+    if (queries == nullptr || perm0 == nullptr || alpha == nullptr || scores == nullptr || ids == nullptr)
+      fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    if ((shift == nullptr) != (scale == nullptr)) fail(VC_ERR_INVALID_ARGUMENT, "shift and scale go together");
+    const bool resid = residual_norms != nullptr && residual_scales != nullptr;
+    // the listed queries (repeats allowed) and their residual scales, in order
+    std::vector<float> qsel, rs_sel;
+    const float* qh = queries;
+    if (query_indices != nullptr) {
+      qsel.resize(nq_out * dim);
+      for (size_t l = 0; l < nq_out; ++l)
+        std::memcpy(qsel.data() + l * dim, queries + query_indices[l] * dim, dim * 4);
+      qh = qsel.data();
+    }
+    if (resid) {
+      rs_sel.resize(nq_out);
+      for (size_t l = 0; l < nq_out; ++l) rs_sel[l] = residual_scales[query_indices ? query_indices[l] : l];
+    }
+    Scoped sc;
+    vchip::CodeTables t = make_tables(dim, bits);
+    const size_t G = size_t(t.groups);
+    const size_t pbytes = (n + 191) / 192 * G * 192;
+    Carve c;
+    const size_t oq = c.take(nq_out * dim * 4), op = c.take(pbytes), of = c.take(n * G),
+                 ot = c.take(vchip::tiled_bytes(n, t.groups16)), oa = c.take(n * 4), om = c.take(id_map ? n * 8 : 0),
+                 os = c.take(nq_out * ke * 4), oi = c.take(nq_out * ke * 8), ocal = c.take(3 * dim * 4),
+                 orn = c.take(resid ? n * 4 : 0), ors = c.take(resid ? nq_out * 4 : 0);
+    sc.a.reserve(c.off, sc.s);
+    hip_check(hipMemcpyAsync(sc.a.as<float>(oq), qh, nq_out * dim * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(hipMemcpyAsync(sc.a.as<uint8_t>(op), perm0, pbytes, hipMemcpyHostToDevice, sc.s), "copy");
+    hip_check(hipMemcpyAsync(sc.a.as<float>(oa), alpha, n * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    if (id_map) hip_check(hipMemcpyAsync(sc.a.as<uint64_t>(om), id_map, n * 8, hipMemcpyHostToDevice, sc.s), "copy");
+    if (resid) {
+      hip_check(hipMemcpyAsync(sc.a.as<float>(orn), residual_norms, n * 4, hipMemcpyHostToDevice, sc.s), "copy");
+      hip_check(hipMemcpyAsync(sc.a.as<float>(ors), rs_sel.data(), nq_out * 4, hipMemcpyHostToDevice, sc.s), "copy");
+    }
+    if (shift != nullptr) {
+      hip_check(hipMemcpyAsync(sc.a.as<float>(ocal), shift, dim * 4, hipMemcpyHostToDevice, sc.s), "copy");
+      hip_check(hipMemcpyAsync(sc.a.as<float>(ocal + dim * 4), scale, dim * 4, hipMemcpyHostToDevice, sc.s), "copy");
+      t.shift = sc.a.as<float>(ocal);
+      t.scale = sc.a.as<float>(ocal + dim * 4);
+      t.inv_scale = nullptr;  // not used by query preparation
+    }
+    hip_check(vchip::launch_perm0_to_flat(sc.a.as<uint8_t>(op), n, int(G), sc.a.as<uint8_t>(of), sc.s), "perm0_to_flat");
+    hip_check(vchip::launch_flat_to_tiled(sc.a.as<uint8_t>(of), n, int(G), t.groups16, sc.a.as<uint8_t>(ot), sc.s),
+              "flat_to_tiled");
+    // with the residual correction the rows are ranked on positions first and
+    // mapped afterwards (remap_ids follows the correction)
+    const uint64_t* im_dev = id_map ? sc.a.as<uint64_t>(om) : nullptr;
+    search_tiled(t, sc.a.as<uint8_t>(ot), sc.a.as<float>(oa), n, sc.a.as<float>(oq), nq_out, k,
+                 resid ? nullptr : im_dev, sc.a.as<float>(os), sc.a.as<uint64_t>(oi), sc.scratch, sc.s);
+    if (resid)
+      hip_check(vchip::launch_residual_post(nq_out, int(ke), sc.a.as<float>(os), sc.a.as<uint64_t>(oi),
+                                            sc.a.as<float>(orn), sc.a.as<float>(ors), n, im_dev, sc.s),
+                "residual_post");
+    hip_check(hipMemcpyAsync(scores, sc.a.as<float>(os), nq_out * ke * 4, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipMemcpyAsync(ids, sc.a.as<uint64_t>(oi), nq_out * ke * 8, hipMemcpyDeviceToHost, sc.s), "copy");
+    hip_check(hipStreamSynchronize(sc.s), "sync");
+  });
+}
+
 size_t vc_gpu32_words(size_t n, size_t dim, size_t bits) {
   if (dim == 0 || dim % 8 != 0 || dim > 16384 || (bits != 2 && bits != 3 && bits != 4)) return 0;
   const size_t G = bits == 2 ? dim / 4 : dim / 2;

@@ -124,6 +124,9 @@ hipError_t launch_flat_to_gpu32(const std::uint8_t* flat, std::size_t n, int gro
                                 hipStream_t s);
 hipError_t launch_gpu32_to_flat(const std::uint32_t* words, std::size_t n, int groups, std::uint8_t* flat,
                                 hipStream_t s);
+// Residual post-correction + re-rank + id remap of rows x k results in place.
+hipError_t launch_residual_post(std::size_t rows, int k, float* scores, std::uint64_t* ids, const float* rn,
+                                const float* rs, std::size_t n, const std::uint64_t* id_map, hipStream_t s);
 hipError_t launch_flat_to_perm0(const std::uint8_t* flat, std::size_t n, int groups,
                                 std::uint8_t* perm0, hipStream_t s);
 hipError_t launch_flat_to_seq(const std::uint8_t* flat, std::size_t n, int groups, std::uint8_t* seq,

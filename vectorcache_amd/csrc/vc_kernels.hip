@@ -2755,6 +2755,50 @@ hipError_t launch_heap_replay(const float* p, std::size_t n, int k, const std::u
   return hipGetLastError();
 }
 
+// score_prepared's residual post-correction on the AVX2 path (src/search/
+// search.cpp:1123-1146), one workgroup per row of k (score, position): each
+// score of a position v < n becomes fmaf(rs[row], rn[v], score), the row is
+// ranked again by (score desc, position asc), and positions go through id_map
+// (when set) as they are written.
+__global__ __launch_bounds__(256) void residual_post_kernel(long rows, int k, float* __restrict__ scores,
+                                                            uint64_t* __restrict__ ids,
+                                                            const float* __restrict__ rn,
+                                                            const float* __restrict__ rs, long n,
+                                                            const uint64_t* __restrict__ id_map) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint64_t* si = reinterpret_cast<uint64_t*>(smem);
+  float* ss = reinterpret_cast<float*>(si + k);
+  for (long row = blockIdx.x; row < rows; row += gridDim.x) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+      const uint64_t v = ids[row * k + j];
+      float s = scores[row * k + j];
+      if (rn != nullptr && v < (uint64_t)n) s = __fmaf_rn(rs[row], rn[v], s);
+      ss[j] = s;
+      si[j] = v;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+      const float s = ss[j];
+      const uint64_t v = si[j];
+      int rank = 0;
+      for (int i = 0; i < k; ++i) rank += (ss[i] > s || (ss[i] == s && si[i] < v)) ? 1 : 0;
+      scores[row * k + rank] = s;
+      ids[row * k + rank] = (id_map != nullptr && v < (uint64_t)n) ? id_map[v] : v;
+    }
+  }
+}
+
+hipError_t launch_residual_post(std::size_t rows, int k, float* scores, std::uint64_t* ids, const float* rn,
+                                const float* rs, std::size_t n, const std::uint64_t* id_map, hipStream_t s) {
+  if (rows == 0 || k == 0) return hipSuccess;
+  const size_t lds = (size_t)k * 12;
+  (void)hipFuncSetAttribute((const void*)residual_post_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(residual_post_kernel, dim3((unsigned)std::min<size_t>(rows, 65536)), dim3(256), lds, s,
+                     (long)rows, k, scores, ids, rn, rs, (long)n, id_map);
+  return hipGetLastError();
+}
+
 hipError_t launch_heaps_to_results(std::size_t nq, int k, const float* hs, const std::uint64_t* hi,
                                    const std::uint64_t* hsz, float* out_s, std::uint64_t* out_i, hipStream_t s) {
   if (nq == 0 || k == 0) return hipSuccess;

@@ -270,6 +270,31 @@ def search_perm0(queries, k: int, bits: int, perm0, alpha, id_map=None, shift=No
     return SearchResults(scores[:, : kout.value], ids[:, : kout.value])
 
 
+def score_perm0(queries, k: int, bits: int, perm0, alpha, id_map=None, shift=None, scale=None,
+                query_indices=None, residual_norms=None, residual_scales=None) -> SearchResults:
+    """score_prepared over PERM0 blocked codes with every argument of the
+    reference's AVX2 path: id map, query subset (rows in its order) and the
+    residual post-correction (residual_norms per vector, residual_scales per
+    query)."""
+    q = _f32(queries)
+    nq, dim = q.shape
+    alpha = _f32(alpha)
+    n = alpha.size
+    perm0 = np.ascontiguousarray(perm0, np.uint8)
+    qi = None if query_indices is None else np.ascontiguousarray(query_indices, np.uint64)
+    nq_out = nq if qi is None else qi.size
+    kk = min(int(k), n)
+    scores = np.zeros((nq_out, kk), np.float32)
+    ids = np.zeros((nq_out, kk), np.uint64)
+    cv = lambda a, t: None if a is None else np.ascontiguousarray(a, t)
+    kout = S()
+    check(lib.vc_score_perm0(_ptr(q), S(nq), S(dim), S(bits), S(k), _ptr(perm0), _ptr(alpha), S(n),
+                             _ptr(cv(id_map, np.uint64)), _ptr(cv(shift, np.float32)), _ptr(cv(scale, np.float32)),
+                             _ptr(qi), S(0 if qi is None else qi.size), _ptr(cv(residual_norms, np.float32)),
+                             _ptr(cv(residual_scales, np.float32)), _ptr(scores), _ptr(ids), ctypes.byref(kout)))
+    return SearchResults(scores[:, : kout.value], ids[:, : kout.value])
+
+
 def gpu32_to_perm0(codes, n: int, dim: int, bits: int) -> np.ndarray:
     """Codes in the reference CUDA index's layout (u32 words
     [ceil(n/32)][ceil(G/4)][32]) as PERM0 blocks, the input of search_perm0 and
