@@ -13,7 +13,7 @@ small-batch p50/p99 latency, and the reference's own CPU search (oracle/_ref,
 built from the reference sources) on a bounded sample of the same queries,
 which also checks the GPU's ids/scores on that sample.
 
-    python bench.py [--steps K] [--warmup W] [--workload flat1m|flat100k]
+    python bench.py [--steps K] [--warmup W] [--workload flat1m|flat1m_k64|flat1m_b2|flat100k]
 """
 from __future__ import annotations
 
@@ -36,6 +36,8 @@ METRIC = "QPS @ recall@10≥0.95 (learned vs fixed probe); p99 ms; % HBM rooflin
 WORKLOADS = {
     # name: (n, dim, bits, calibrate_rows, nq, k)
     "flat1m": (1_000_000, 1536, 4, 20_000, 10_000, 10),
+    "flat1m_k64": (1_000_000, 1536, 4, 20_000, 10_000, 64),
+    "flat1m_b2": (1_000_000, 1536, 2, 20_000, 10_000, 10),
     "flat100k": (100_000, 128, 4, 0, 1_000, 10),
 }
 

@@ -1071,8 +1071,6 @@ __device__ __forceinline__ bool better(float s, uint32_t id, float ts, uint32_t 
 struct TopkShared {
   float* top_s;     // [T][k]
   uint32_t* top_i;  // [T][k]
-  float* nw_s;      // [T][k]
-  uint32_t* nw_i;   // [T][k]
   float* cand_s;    // [T][cap]
   uint32_t* cand_i; // [T][cap]
   int* cnt;         // [T]
@@ -1096,8 +1094,6 @@ __device__ void topk_merge(const TopkShared& sh, int t, int k, int cap) {
   const int m = __builtin_amdgcn_readfirstlane(sh.m[t]);
   float* ts = sh.top_s + t * k;
   uint32_t* ti = sh.top_i + t * k;
-  float* ns = sh.nw_s + t * k;
-  uint32_t* ni = sh.nw_i + t * k;
   const float* cs = sh.cand_s + t * cap;
   const uint32_t* ci = sh.cand_i + t * cap;
   // rank = start + number of candidates better than (s, id) (16-byte LDS
@@ -1119,26 +1115,38 @@ __device__ void topk_merge(const TopkShared& sh, int t, int k, int cap) {
     }
     return rank;
   };
-  for (int e = tid; e < c; e += blockDim.x) {
-    const float s = cs[e];
-    const uint32_t id = ci[e];
+  // Every rank first, the writes after a barrier, so the merged list is
+  // written in place: a thread holds at most one candidate (cap <= the
+  // workgroup size) and two list entries (k <= twice the workgroup size).
+  const int nt = blockDim.x;
+  float es = 0.f, ls0 = 0.f, ls1 = 0.f;
+  uint32_t ei = 0u, li0 = 0u, li1 = 0u;
+  int er = k, lr0 = k, lr1 = k;
+  if (tid < c) {
+    es = cs[tid];
+    ei = ci[tid];
     int lo = 0, hi = m;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
-      if (better(ts[mid], ti[mid], s, id)) lo = mid + 1; else hi = mid;
+      if (better(ts[mid], ti[mid], es, ei)) lo = mid + 1; else hi = mid;
     }
-    const int rank = rank_of(s, id, lo);
-    if (rank < k) { ns[rank] = s; ni[rank] = id; }
+    er = rank_of(es, ei, lo);
   }
-  for (int j = tid; j < m; j += blockDim.x) {
-    const float s = ts[j];
-    const uint32_t id = ti[j];
-    const int rank = rank_of(s, id, j);
-    if (rank < k) { ns[rank] = s; ni[rank] = id; }
+  if (tid < m) {
+    ls0 = ts[tid];
+    li0 = ti[tid];
+    lr0 = rank_of(ls0, li0, tid);
+  }
+  if (tid + nt < m) {
+    ls1 = ts[tid + nt];
+    li1 = ti[tid + nt];
+    lr1 = rank_of(ls1, li1, tid + nt);
   }
   __syncthreads();
+  if (er < k) { ts[er] = es; ti[er] = ei; }
+  if (lr0 < k) { ts[lr0] = ls0; ti[lr0] = li0; }
+  if (lr1 < k) { ts[lr1] = ls1; ti[lr1] = li1; }
   const int m2 = min(k, m + c);
-  for (int j = tid; j < m2; j += blockDim.x) { ts[j] = ns[j]; ti[j] = ni[j]; }
   __syncthreads();
   if (tid == 0) {
     sh.m[t] = m2;
@@ -1154,8 +1162,6 @@ __device__ __forceinline__ void init_topk(TopkShared& sh, unsigned char* base, i
   size_t off = 0;
   sh.top_s = reinterpret_cast<float*>(base + off); off += (size_t)T * k * 4;
   sh.top_i = reinterpret_cast<uint32_t*>(base + off); off += (size_t)T * k * 4;
-  sh.nw_s = reinterpret_cast<float*>(base + off); off += (size_t)T * k * 4;
-  sh.nw_i = reinterpret_cast<uint32_t*>(base + off); off += (size_t)T * k * 4;
   sh.cand_s = reinterpret_cast<float*>(base + off); off += (size_t)T * cap * 4;
   sh.cand_i = reinterpret_cast<uint32_t*>(base + off); off += (size_t)T * cap * 4;
   sh.cnt = reinterpret_cast<int*>(base + off); off += T * 4;
@@ -1774,7 +1780,7 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void scan_pair_ker
   init_topk(sh, smem, T, k, cap);
   // per-query LUT scale and bias in LDS (read at window flushes and step
   // starts only; keeps them out of the scalar register budget)
-  float* scl = reinterpret_cast<float*>(smem + (size_t)T * (4 * (size_t)k + 2 * (size_t)cap) * 4 + 20 * T);
+  float* scl = reinterpret_cast<float*>(smem + (size_t)T * (2 * (size_t)k + 2 * (size_t)cap) * 4 + 20 * T);
   float* bias = scl + T;
   if (tid < T) {
     const long q = q0 + tid < nq ? q0 + tid : nq - 1;
@@ -2546,7 +2552,8 @@ hipError_t launch_prepare_queries(const CodeTables& t, const float* q, std::size
 
 // top-k state of a workgroup (lists, candidates, counters) + per-query scale
 // and bias for the pair kernel
-static size_t topk_lds(int T, int k, int cap) { return (size_t)T * (4 * (size_t)k + 2 * (size_t)cap) * 4 + 28 * T + 64; }
+static size_t topk_lds(int T, int k, int cap) { return (size_t)T * (2 * (size_t)k + 2 * (size_t)cap) * 4 + 28 * T + 64; }
+static_assert(kCandCap <= kScanThreads && kMaxTopK <= 2 * kScanThreads, "topk_merge: ranks held in registers");
 
 // Vector ranges per query tile: enough workgroups to fill every CU slot when
 // the query tiles alone do not; otherwise the split (at most 8) that leaves
@@ -2585,14 +2592,15 @@ ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int force
     const std::size_t fixed = (std::size_t)kPairTableBytes + kPairStageBytes;
     for (int T : {8, 4, 2, 1}) {
       if (T > (forced_tile != 0 ? forced_tile : want)) continue;  // a forced tile is an upper bound
-      // largest candidate buffer that still lets two workgroups share a CU,
-      // else 512 with one workgroup per CU
+      // largest candidate buffer (at least 32) that still lets two
+      // workgroups share a CU, else 512 with one workgroup per CU (measured:
+      // k = 64 at T = 8 scans faster with 32 candidates and two workgroups)
       int cap = kCandCap;
       std::size_t lds = fixed + topk_lds(T, k, cap);
       if (lds > (std::size_t)kLdsBudget / 2) {
         const long room = (long)kLdsBudget / 2 - (long)fixed - (long)topk_lds(T, k, 0);
         const long c2 = room / (8L * T) / 32 * 32;
-        if (c2 >= 64) {
+        if (c2 >= 32) {
           cap = (int)std::min<long>(c2, kCandCap);
           lds = fixed + topk_lds(T, k, cap);
         }
