@@ -39,6 +39,7 @@ typedef enum {
 } vc_status;
 
 typedef struct vc_index vc_index;
+typedef struct vc_lm_index vc_lm_index;
 
 /* Thread-local message of the last failing call on this thread. */
 const char* vc_last_error(void);
@@ -126,6 +127,54 @@ vc_status vc_index_prepare_queries(vc_index* index, const float* queries, size_t
                                    uint8_t* luts, float* lut_scale, float* lut_bias,
                                    float* bias_corr);
 
+/* ---- landmark index --------------------------------------------------------
+ * The reference's LandmarkIndex (include/vectorcache/index.hpp:16-68,
+ * src/index.cpp:120-289), which its TurboQuantIndex aliases, on the AVX2
+ * path: add() encodes alpha * y-hat, projects it onto an orthonormal landmark
+ * basis and keeps the residual norm; prepare() encodes the landmark rows at
+ * scan_dim = n_landmarks rounded up to 8; search() rotates the queries,
+ * projects them onto the 4-bit basis, scans and applies the residual
+ * post-correction (src/search/search.cpp:1304-1546). Results equal the
+ * reference's bit for bit. Rows added after prepare() find the earlier rows
+ * zeroed at the next prepare(), as in the reference.
+ * create: dim a multiple of 8 (<= 16384), scan_bits and rerank_bits in 2..4
+ * (rerank_bits 0 = scan_bits), 1 <= n_landmarks <= dim (the reference's
+ * default is 1024). */
+vc_status vc_lm_index_create(size_t dim, size_t scan_bits, size_t n_landmarks, uint64_t landmark_seed,
+                             size_t rerank_bits, vc_lm_index** out);
+vc_status vc_lm_index_destroy(vc_lm_index* index);
+vc_status vc_lm_index_add(vc_lm_index* index, const float* vectors, size_t n);
+vc_status vc_lm_index_add_device(vc_lm_index* index, const float* vectors_dev, size_t n);
+vc_status vc_lm_index_prepare(vc_lm_index* index);
+/* search(queries, k, float_corpus): k_eff = min(k, size()) <= 2048; with a
+ * float corpus (size() x dim rows, may be NULL) the top k is re-scored by exact
+ * inner products (float_rescore_topk). VC_ERR_STATE before prepare(). */
+vc_status vc_lm_index_search(vc_lm_index* index, const float* queries, size_t nq, size_t k,
+                             const float* float_corpus, float* scores, uint64_t* ids, size_t* k_out);
+vc_status vc_lm_index_search_device(vc_lm_index* index, const float* queries_dev, size_t nq, size_t k,
+                                    const float* float_corpus_dev, float* scores_dev, uint64_t* ids_dev,
+                                    size_t* k_out, void* stream);
+/* The query side alone: landmark rows q_land (nq x scan_dim, zero past
+ * n_landmarks) and residual scales q_perp (nq), as search() computes them. */
+vc_status vc_lm_index_query_rows(vc_lm_index* index, const float* queries, size_t nq, float* q_land,
+                                 float* q_perp);
+/* Per-vector state: residual norms (size()), and after prepare() the scan
+ * scales (alpha) and flat codes (n x scan_dim / (8 / scan_bits) bytes); each
+ * output may be NULL. */
+vc_status vc_lm_index_export(vc_lm_index* index, float* residual_norms, float* scales, uint8_t* codes_flat);
+size_t vc_lm_index_dim(const vc_lm_index* index);
+size_t vc_lm_index_size(const vc_lm_index* index);
+size_t vc_lm_index_scan_bits(const vc_lm_index* index);
+size_t vc_lm_index_rerank_bits(const vc_lm_index* index);
+size_t vc_lm_index_n_landmarks(const vc_lm_index* index);
+size_t vc_lm_index_scan_dim(const vc_lm_index* index);
+int vc_lm_index_is_prepared(const vc_lm_index* index);
+/* codes_bytes() / index_bytes() as the reference reports them (its PERM0
+ * blocks of 192 vectors; index_bytes adds residual norms, scales and the
+ * scale/residual pairs). */
+size_t vc_lm_index_codes_bytes(const vc_lm_index* index);
+size_t vc_lm_index_index_bytes(const vc_lm_index* index);
+
 /* ---- stateless entry points (host buffers) ---------------------------------
  * Host precompute, bit-identical to the reference:
  * Rotation tables (src/transform/rotation.cpp:139-160), codebook
@@ -133,6 +182,13 @@ vc_status vc_index_prepare_queries(vc_index* index, const float* queries, size_t
 vc_status vc_rotation_tables(size_t dim, uint32_t* perm, float* signs, float* inv_sqrt_block,
                              size_t* block);
 vc_status vc_codebook(size_t bits, size_t dim, float* boundaries, float* centroids);
+/* make_ortho_landmarks (src/transform/landmarks.cpp:180-198): L dim x m floats,
+ * column j at L[j * dim]; 0 < m <= dim, dim a multiple of 8. */
+vc_status vc_landmarks_make(size_t dim, size_t m, uint64_t seed, float* L);
+/* quantize_landmarks (src/transform/landmarks.cpp:236-269): codes
+ * m x (pad16(dim) / 2) bytes (level + 7 per nibble, low nibble = even dim),
+ * scales m. */
+vc_status vc_landmarks_quantize(const float* L, size_t dim, size_t m, uint8_t* codes, float* scales);
 
 /* encode() on the GPU (src/encode/encode.cpp:294-307): n x dim host rows ->
  * bit-plane codes and alpha. shift/scale: TQ+ calibration or both NULL. */

@@ -1142,3 +1142,422 @@ int oc_float_rescore(size_t nq, size_t k, float* scores, uint64_t* ids, const si
   free(c);
   return 0;
 }
+
+/* ==== landmark index (reference src/index.cpp:120-289,
+ * src/transform/landmarks.cpp, src/search/search.cpp:1304-1546) ==========
+ * On the AVX2 path (TURBOVEC_NO_VNNI=1) planes_wanted() is false
+ * (src/pack/pack.cpp:441-449 needs the vector-major kernels), so no sign
+ * planes exist and search is: rotate the query, project it onto the 4-bit
+ * landmark basis, LUTs from those rows, the PERM0 scan with the residual
+ * post-correction. Float semantics follow the reference's object code under
+ * its flags (-O3 -mavx2 -mfma -ffp-contract=fast): where the compiler
+ * vectorized a reduction it multiplies (rounded) and adds in element order. */
+
+/* mt19937-64 (Matsumoto and Nishimura; the std::mt19937_64 parameters). */
+typedef struct {
+  uint64_t mt[312];
+  int i;
+} oc_mt64;
+
+static void mt64_seed(oc_mt64* g, uint64_t s) {
+  g->mt[0] = s;
+  for (int i = 1; i < 312; ++i) g->mt[i] = 6364136223846793005ULL * (g->mt[i - 1] ^ (g->mt[i - 1] >> 62)) + (uint64_t)i;
+  g->i = 312;
+}
+
+static uint64_t mt64_next(oc_mt64* g) {
+  if (g->i >= 312) {
+    for (int k = 0; k < 312; ++k) {
+      const uint64_t y = (g->mt[k] & 0xFFFFFFFF80000000ULL) | (g->mt[(k + 1) % 312] & 0x7FFFFFFFULL);
+      g->mt[k] = g->mt[(k + 156) % 312] ^ (y >> 1) ^ ((y & 1ULL) ? 0xB5026F5AA96619E9ULL : 0ULL);
+    }
+    g->i = 0;
+  }
+  uint64_t y = g->mt[g->i++];
+  y ^= (y >> 29) & 0x5555555555555555ULL;
+  y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
+  y ^= (y << 37) & 0xFFF7EEE000000000ULL;
+  y ^= y >> 43;
+  return y;
+}
+
+/* libstdc++'s normal_distribution<double> (Marsaglia polar method) over
+ * generate_canonical<double, 53>: u = double(x) 2^-64 (below 1); as compiled
+ * into the reference, r2 = fma(a, a, b * b); returns b f, keeps a f. */
+typedef struct {
+  oc_mt64 g;
+  double saved;
+  int have;
+} oc_gauss;
+
+static double gauss_uniform(oc_gauss* s) {
+  double u = (double)mt64_next(&s->g) * 0x1p-64;
+  if (u >= 1.0) u = nextafter(1.0, 0.0);
+  return u;
+}
+
+static double gauss_next(oc_gauss* s) {
+  if (s->have) {
+    s->have = 0;
+    return s->saved + 0.0;
+  }
+  double a, b, r2;
+  do {
+    a = fma(2.0, gauss_uniform(s), -1.0);
+    b = fma(2.0, gauss_uniform(s), -1.0);
+    r2 = fma(a, a, b * b);
+  } while (r2 > 1.0 || r2 == 0.0);
+  const double f = sqrt(-2.0 * log(r2) / r2);
+  s->saved = a * f;
+  s->have = 1;
+  return b * f + 0.0;
+}
+
+/* make_ortho_landmarks (landmarks.cpp:134-198): Gaussian dim x m (column
+ * major, filled in order), modified Gram-Schmidt in double: dot and norm sums
+ * of rounded products in element order, a_i = fma(-dot, a_j, a_i). */
+int oc_ortho_landmarks(size_t dim, size_t m, uint64_t seed, float* L) {
+  if (dim == 0 || m == 0 || m > dim || dim % 8 != 0) return -1;
+  double* A = (double*)malloc(dim * m * sizeof(double));
+  if (!A) return -1;
+  oc_gauss gs;
+  memset(&gs, 0, sizeof(gs));
+  mt64_seed(&gs.g, seed);
+  for (size_t i = 0; i < dim * m; ++i) A[i] = gauss_next(&gs);
+  int rc = 0;
+  for (size_t i = 0; i < m && rc == 0; ++i) {
+    double* ai = A + i * dim;
+    for (size_t j = 0; j < i; ++j) {
+      const double* aj = A + j * dim;
+      double dot = 0.0;
+      for (size_t r = 0; r < dim; ++r) dot = dot + ai[r] * aj[r];
+      for (size_t r = 0; r < dim; ++r) ai[r] = fma(-dot, aj[r], ai[r]);
+    }
+    double nrm = 0.0;
+    for (size_t r = 0; r < dim; ++r) nrm = nrm + ai[r] * ai[r];
+    nrm = sqrt(nrm);
+    if (nrm < 1e-15) rc = -3;
+    const double inv = 1.0 / nrm;
+    for (size_t r = 0; r < dim; ++r) ai[r] = ai[r] * inv;
+  }
+  for (size_t i = 0; rc == 0 && i < dim * m; ++i) L[i] = (float)A[i];
+  free(A);
+  return rc;
+}
+
+static size_t pad16(size_t n) { return (n + 15u) & ~(size_t)15u; }
+
+/* quantize_landmarks (landmarks.cpp:236-269): per column amax; scale =
+ * amax / 7, level = clamp(round(x * (7 / amax)), -7, 7); byte = (odd + 7) << 4
+ * | (even + 7); columns of pad16(dim) / 2 bytes, zero-filled. */
+int oc_quantize_landmarks(const float* L, size_t dim, size_t m, uint8_t* codes, float* scales) {
+  const size_t bpc = pad16(dim) / 2;
+  memset(codes, 0, bpc * m);
+  for (size_t j = 0; j < m; ++j) {
+    const float* col = L + j * dim;
+    float amax = 0.f;
+    for (size_t i = 0; i < dim; ++i) amax = fmaxf(amax, fabsf(col[i]));
+    scales[j] = 0.f;
+    if (amax < 1e-30f) continue;
+    scales[j] = amax / 7.f;
+    const float inv = 7.f / amax;
+    for (size_t i = 0; i < dim; i += 2) {
+      const int e = (int)fmaxf(-7.f, fminf(7.f, roundf(col[i] * inv)));
+      const int o = i + 1 < dim ? (int)fmaxf(-7.f, fminf(7.f, roundf(col[i + 1] * inv))) : 0;
+      codes[j * bpc + i / 2] = (uint8_t)(((o + 7) << 4) | (e + 7));
+    }
+  }
+  return 0;
+}
+
+/* dot_f32_avx2 (landmarks.cpp:104-121): 8 fused chains, then
+ * ((a0 + a4) + (a1 + a5)) + ((a2 + a6) + (a3 + a7)); dim % 8 == 0. */
+static float dot8(const float* a, const float* b, size_t dim) {
+  float c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t i = 0; i < dim; i += 8)
+    for (int l = 0; l < 8; ++l) c[l] = fmaf(a[i + l], b[i + l], c[l]);
+  return ((c[0] + c[4]) + (c[1] + c[5])) + ((c[2] + c[6]) + (c[3] + c[7]));
+}
+
+/* project_rows_onto_landmarks (landmarks.cpp:200-234) */
+int oc_project_rows(const float* Y, size_t n, size_t dim, const float* L, size_t m, float* out,
+                    size_t stride) {
+  if (dim % 8 != 0 || stride < m) return -1;
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)n; ++i) {
+    for (size_t j = 0; j < stride; ++j)
+      out[(size_t)i * stride + j] = j < m ? dot8(Y + (size_t)i * dim, L + j * dim, dim) : 0.f;
+  }
+  return 0;
+}
+
+/* project_onto_landmarks_q4 (landmarks.cpp:271-348) with the codes/scales of
+ * quantize_landmarks: q level = clamp(round(x * (127 / amax)), -127, 127),
+ * integer dots over pad16(dim), q_land = (q_scale * scale_j) * (float) dot. */
+static void project_q4_rows(const float* q_rot, size_t nq, size_t dim, const uint8_t* codes,
+                            const float* scales, size_t m, float* q_land, size_t stride) {
+  const size_t dp = pad16(dim), bpc = dp / 2;
+#pragma omp parallel
+  {
+    int8_t* qi8 = (int8_t*)malloc(dp);
+#pragma omp for schedule(static)
+    for (long long qi = 0; qi < (long long)nq; ++qi) {
+      const float* q = q_rot + (size_t)qi * dim;
+      float amax = 0.f;
+      for (size_t i = 0; i < dim; ++i) amax = fmaxf(amax, fabsf(q[i]));
+      memset(qi8, 0, dp);
+      float qs = 0.f;
+      if (!(amax < 1e-30f)) {
+        qs = amax / 127.f;
+        const float inv = 127.f / amax;
+        for (size_t i = 0; i < dim; ++i) qi8[i] = (int8_t)fmaxf(-127.f, fminf(127.f, roundf(q[i] * inv)));
+      }
+      float* dst = q_land + (size_t)qi * stride;
+      for (size_t j = 0; j < stride; ++j) {
+        if (j >= m) {
+          dst[j] = 0.f;
+          continue;
+        }
+        int32_t dot = 0;
+        for (size_t b = 0; b < bpc; ++b) {
+          const uint8_t byte = codes[j * bpc + b];
+          dot += (int32_t)qi8[2 * b] * ((int32_t)(byte & 15u) - 7);
+          dot += (int32_t)qi8[2 * b + 1] * ((int32_t)(byte >> 4) - 7);
+        }
+        dst[j] = (qs * scales[j]) * (float)dot;
+      }
+    }
+    free(qi8);
+  }
+}
+
+int oc_project_q4(const float* q_rot, size_t nq, size_t dim, const float* L, size_t m, float* q_land,
+                  size_t stride) {
+  if (stride < m) return -1;
+  const size_t bpc = pad16(dim) / 2;
+  uint8_t* codes = (uint8_t*)malloc(bpc * m);
+  float* sc = (float*)malloc(m * sizeof(float));
+  oc_quantize_landmarks(L, dim, m, codes, sc);
+  project_q4_rows(q_rot, nq, dim, codes, sc, m, q_land, stride);
+  free(codes);
+  free(sc);
+  return 0;
+}
+
+/* l2_normalize_into (index.cpp:21-50): 8 fused chains summed in order, the
+ * tail in double, norm = (float) sqrt(double); below 1e-10 the row is zero
+ * and the norm 0. */
+static float lm_unit(const float* src, size_t d, float* dst) {
+  float c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  size_t i = 0;
+  for (; i + 8 <= d; i += 8)
+    for (int l = 0; l < 8; ++l) c[l] = fmaf(src[i + l], src[i + l], c[l]);
+  float t = c[0];
+  for (int l = 1; l < 8; ++l) t = t + c[l];
+  double s = (double)t;
+  for (; i < d; ++i) s = fma((double)src[i], (double)src[i], s);
+  const float nrm = (float)sqrt(s);
+  if (nrm < kMinInputNorm) {
+    memset(dst, 0, d * sizeof(float));
+    return 0.f;
+  }
+  const float inv = 1.f / nrm;
+  for (size_t k = 0; k < d; ++k) dst[k] = src[k] * inv;
+  return nrm;
+}
+
+/* encode_scaled_yhat (index.cpp:88-117): unit row, rotation, level = #{b : y
+ * >= b}, alpha = (float) (dot / |c|^2) (double sums in order), row = alpha *
+ * c, returns the sum of the rounded squares in order (dim % 8 == 0). */
+static float lm_yhat(const float* src, size_t dim, const oc_rot* r, const float* bnd, size_t nb,
+                     const float* cen, float* unit, float* y) {
+  lm_unit(src, dim, unit);
+  rotate_with(dim, r->perm, r->signs, r->block, r->isb, unit, 1.f, y);
+  double dot = 0.0, cc = 0.0;
+  for (size_t i = 0; i < dim; ++i) {
+    unsigned k = 0;
+    for (size_t b = 0; b < nb; ++b) k += (y[i] >= bnd[b]) ? 1u : 0u;
+    const double c = (double)cen[k];
+    dot = dot + (double)y[i] * c;
+    cc = cc + c * c;
+    unit[i] = cen[k];
+  }
+  const float alpha = (cc > kDegenerateInnerEps) ? (float)(dot / cc) : 0.f;
+  float vn = 0.f;
+  for (size_t i = 0; i < dim; ++i) {
+    const float vi = alpha * unit[i];
+    y[i] = vi;
+    vn = vn + vi * vi;
+  }
+  return vn;
+}
+
+/* norm_sq_f32 (index.cpp:60-82): 8 fused chains over m & ~7, summed in order;
+ * the tail as compiled: 4 rounded squares in order when >= 4 remain, then
+ * fused multiply-adds. */
+static float lm_norm_sq(const float* x, size_t m) {
+  float c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const size_t m8 = m & ~(size_t)7;
+  for (size_t i = 0; i < m8; i += 8)
+    for (int l = 0; l < 8; ++l) c[l] = fmaf(x[i + l], x[i + l], c[l]);
+  float s = c[0];
+  for (int l = 1; l < 8; ++l) s = s + c[l];
+  size_t j = m8;
+  if (m - j >= 4) {
+    for (int u = 0; u < 4; ++u) s = s + x[j + u] * x[j + u];
+    j += 4;
+  }
+  for (; j < m; ++j) s = fmaf(x[j], x[j], s);
+  return s;
+}
+
+/* LandmarkIndex end to end with the reference's state rules: rows added after
+ * a prepare() find the earlier rows zeroed (landmark_rows_ was released and is
+ * resized), and prepare() encodes every row. Arguments as vcref_lm_search. */
+int oc_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
+                 const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
+                 size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
+                 uint64_t* ids_out, size_t* k_out, size_t* info) {
+  if (rerank_bits == 0) rerank_bits = scan_bits;
+  if (dim == 0 || dim % 8 != 0 || scan_bits < 2 || scan_bits > 4 || rerank_bits < 2 || rerank_bits > 4 ||
+      m == 0 || m > dim)
+    return -1;
+  const size_t S = (m + 7u) & ~(size_t)7u;
+  const size_t G = groups_of(scan_bits, S);
+  float* L = (float*)malloc(dim * m * sizeof(float));
+  if (oc_ortho_landmarks(dim, m, seed, L) != 0) {
+    free(L);
+    return -1;
+  }
+  const size_t bpc = pad16(dim) / 2;
+  uint8_t* lq = (uint8_t*)malloc(bpc * m);
+  float* lsc = (float*)malloc(m * sizeof(float));
+  oc_quantize_landmarks(L, dim, m, lq, lsc);
+  float bnd[15], cen[16], lbnd[15], lcen[16];
+  oc_codebook(rerank_bits, dim, bnd, cen);
+  oc_codebook(scan_bits, S, lbnd, lcen);
+  oc_rot r;
+  rot_make(&r, dim);
+  size_t n_total = 0;
+  for (size_t b = 0; b < n_batches; ++b) n_total += batch_rows[b];
+  float* rows = (float*)calloc(n_total * S + 1, sizeof(float));
+  float* rn = (float*)calloc(n_total + 1, sizeof(float));
+  float* alpha = (float*)calloc(n_total + 1, sizeof(float));
+  uint8_t* flat = (uint8_t*)calloc(n_total * G + 1, 1);
+  size_t n = 0, n_enc = 0;
+  int rows_live = 1;
+  for (size_t b = 0; b < n_batches; ++b) {
+    if (!rows_live) {
+      memset(rows, 0, n_total * S * sizeof(float));
+      rows_live = 1;
+    }
+    const size_t nb = batch_rows[b];
+#pragma omp parallel
+    {
+      float* unit = (float*)malloc(dim * sizeof(float));
+      float* y = (float*)malloc(dim * sizeof(float));
+#pragma omp for schedule(static)
+      for (long long ii = 0; ii < (long long)nb; ++ii) {
+        const size_t i = n + (size_t)ii;
+        const float vn = lm_yhat(x + i * dim, dim, &r, bnd, ((size_t)1 << rerank_bits) - 1, cen, unit, y);
+        float* row = rows + i * S;
+        for (size_t j = 0; j < S; ++j) row[j] = j < m ? dot8(y, L + j * dim, dim) : 0.f;
+        const float d = vn - lm_norm_sq(row, m);
+        rn[i] = sqrtf(0.f < d ? d : 0.f);
+      }
+      free(unit);
+      free(y);
+    }
+    n += nb;
+    if (prepare_between || b + 1 == n_batches) {
+      /* prepare(): every row, then the rows are released */
+      const size_t bpr = scan_bits * (S / 8);
+      uint8_t* packed = (uint8_t*)malloc(n * bpr + 1);
+#pragma omp parallel
+      {
+        float* unit = (float*)malloc(S * sizeof(float));
+#pragma omp for schedule(static)
+        for (long long ii = 0; ii < (long long)n; ++ii) {
+          const float nrm = lm_unit(rows + (size_t)ii * S, S, unit);
+          alpha[ii] = quantize_row(unit, S, scan_bits, lbnd, lcen, NULL, NULL, NULL, 1, nrm,
+                                   packed + (size_t)ii * bpr);
+        }
+        free(unit);
+      }
+      oc_codes_flat(packed, n, scan_bits, S, flat);
+      free(packed);
+      n_enc = n;
+      rows_live = 0;
+    }
+  }
+  int rc = 0;
+  const size_t ke = k < n ? k : n;
+  *k_out = ke;
+  if (nq > 0 && ke > 0) {
+    float* q_rot = (float*)malloc(nq * dim * sizeof(float));
+    float* q_land = (float*)malloc(nq * S * sizeof(float));
+    float* q_perp = (float*)malloc(nq * sizeof(float));
+    uint8_t* luts = (uint8_t*)malloc(nq * G * 32);
+    float* ls = (float*)malloc(nq * sizeof(float));
+    float* lb = (float*)malloc(nq * sizeof(float));
+    for (size_t i = 0; i < nq; ++i) {
+      const float nv = oc_row_norm(q + i * dim, dim);
+      const float inv = (nv > kMinInputNorm) ? (1.f / nv) : 0.f;
+      rotate_with(dim, r.perm, r.signs, r.block, r.isb, q + i * dim, inv, q_rot + i * dim);
+    }
+    project_q4_rows(q_rot, nq, dim, lq, lsc, m, q_land, S);
+    float* fv = (float*)malloc(G * 34 * sizeof(float));
+    for (size_t i = 0; i < nq; ++i) {
+      double q2 = 0.0, l2 = 0.0;
+      for (size_t d = 0; d < dim; ++d) q2 = q2 + (double)q_rot[i * dim + d] * (double)q_rot[i * dim + d];
+      for (size_t j = 0; j < m; ++j) l2 = l2 + (double)q_land[i * S + j] * (double)q_land[i * S + j];
+      const double d2 = q2 - l2;
+      q_perp[i] = (float)sqrt(0.0 < d2 ? d2 : 0.0);
+      float bias;
+      build_lut(q_land + i * S, S, scan_bits, lcen, luts + i * G * 32, &ls[i], &bias, fv);
+      lb[i] = bias + 0.f;
+    }
+    free(fv);
+    size_t kk = 0;
+    rc = oc_scan_topk(luts, ls, lb, nq, flat, n_enc, G, alpha, NULL, ke, scores_out, ids_out, &kk);
+    /* residual post-correction and re-rank (search.cpp:1123-1146) */
+    oc_cand* rowc = (oc_cand*)malloc(ke * sizeof(oc_cand));
+    for (size_t i = 0; rc == 0 && i < nq; ++i) {
+      for (size_t j = 0; j < ke; ++j) {
+        const uint64_t vi = ids_out[i * ke + j];
+        float s = scores_out[i * ke + j];
+        if (vi < n_enc) s = fmaf(q_perp[i], rn[vi], s);
+        rowc[j].s = s;
+        rowc[j].id = vi;
+      }
+      qsort(rowc, ke, sizeof(oc_cand), cand_cmp);
+      for (size_t j = 0; j < ke; ++j) {
+        scores_out[i * ke + j] = rowc[j].s;
+        ids_out[i * ke + j] = rowc[j].id;
+      }
+    }
+    free(rowc);
+    if (rc == 0 && float_corpus != NULL)
+      rc = oc_float_rescore(nq, ke, scores_out, ids_out, NULL, q, float_corpus, n, dim);
+    free(q_rot);
+    free(q_land);
+    free(q_perp);
+    free(luts);
+    free(ls);
+    free(lb);
+  }
+  if (info != NULL) {
+    info[0] = S;
+    info[1] = (n_enc + 191) / 192 * 192 * G;
+    info[2] = info[1] + 4 * n + 12 * n_enc;
+  }
+  rot_free(&r);
+  free(L);
+  free(lq);
+  free(lsc);
+  free(rows);
+  free(rn);
+  free(alpha);
+  free(flat);
+  return rc;
+}

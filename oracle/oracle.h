@@ -60,6 +60,17 @@ int oc_score_prepared(const float* q, size_t nq, size_t dim, size_t k, size_t bi
 int oc_merge_results(size_t nq, size_t k_dst, float* dst_scores, uint64_t* dst_ids, size_t k_src,
                      const float* src_scores, const uint64_t* src_ids);
 void oc_set_threads(int n);
+/* landmark index (reference src/index.cpp, src/transform/landmarks.cpp) */
+int oc_ortho_landmarks(size_t dim, size_t m, uint64_t seed, float* L);
+int oc_quantize_landmarks(const float* L, size_t dim, size_t m, uint8_t* codes, float* scales);
+int oc_project_rows(const float* Y, size_t n, size_t dim, const float* L, size_t m, float* out,
+                    size_t stride);
+int oc_project_q4(const float* q_rot, size_t nq, size_t dim, const float* L, size_t m, float* q_land,
+                  size_t stride);
+int oc_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
+                 const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
+                 size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
+                 uint64_t* ids_out, size_t* k_out, size_t* info);
 /* the ChaCha block function (8 rounds for the rotation, 20 for RFC 7539 tests) */
 void oc_chacha_block(const uint32_t in[16], uint32_t out[16], int rounds);
 

@@ -343,6 +343,59 @@ class Oracle:
                                              _S(cor.shape[0]), _S(dim)), "float_rescore")
         return sc, ii
 
+    # ---- landmark index ----------------------------------------------------
+    def ortho_landmarks(self, dim, m, seed):
+        L = np.zeros(dim * m, np.float32)
+        self._check(self._f("ortho_landmarks")(_S(dim), _S(m), ctypes.c_uint64(seed), _ptr(L)), "ortho_landmarks")
+        return L.reshape(m, dim)
+
+    def quantize_landmarks(self, L):
+        L = np.ascontiguousarray(L, np.float32)
+        m, dim = L.shape
+        codes = np.zeros((m, ((dim + 15) // 16 * 16) // 2), np.uint8)
+        scales = np.zeros(m, np.float32)
+        self._check(self._f("quantize_landmarks")(_ptr(L), _S(dim), _S(m), _ptr(codes), _ptr(scales)),
+                    "quantize_landmarks")
+        return codes, scales
+
+    def project_rows(self, Y, L, stride):
+        Y = np.ascontiguousarray(Y, np.float32)
+        L = np.ascontiguousarray(L, np.float32)
+        out = np.zeros((Y.shape[0], stride), np.float32)
+        self._check(self._f("project_rows")(_ptr(Y), _S(Y.shape[0]), _S(Y.shape[1]), _ptr(L), _S(L.shape[0]),
+                                            _ptr(out), _S(stride)), "project_rows")
+        return out
+
+    def project_q4(self, q_rot, L, stride):
+        q = np.ascontiguousarray(q_rot, np.float32)
+        L = np.ascontiguousarray(L, np.float32)
+        out = np.zeros((q.shape[0], stride), np.float32)
+        self._check(self._f("project_q4")(_ptr(q), _S(q.shape[0]), _S(q.shape[1]), _ptr(L), _S(L.shape[0]),
+                                          _ptr(out), _S(stride)), "project_q4")
+        return out
+
+    def lm_search(self, x, q, k, scan_bits=4, n_landmarks=1024, seed=0, rerank_bits=0, batches=None,
+                  prepare_between=False, float_corpus=None):
+        """LandmarkIndex(dim, scan_bits, n_landmarks, seed, rerank_bits): add the
+        rows of x (in `batches` row counts), prepare, search. Returns scores,
+        ids, info {scan_dim, codes_bytes, index_bytes}."""
+        x = np.ascontiguousarray(x, np.float32)
+        q = np.ascontiguousarray(q, np.float32)
+        n, dim = x.shape
+        nq = q.shape[0]
+        br = np.asarray(batches if batches is not None else [n], np.uint64)
+        scores = np.zeros(max(1, nq * k), np.float32)
+        ids = np.zeros(max(1, nq * k), np.uint64)
+        ko = ctypes.c_size_t()
+        info = np.zeros(3, np.uint64)
+        fc = None if float_corpus is None else np.ascontiguousarray(float_corpus, np.float32)
+        self._check(self._f("lm_search")(_ptr(x), _ptr(br), _S(len(br)), ctypes.c_int(int(prepare_between)),
+                                         _ptr(q), _S(nq), _S(dim), _S(scan_bits), _S(n_landmarks),
+                                         ctypes.c_uint64(seed), _S(rerank_bits), _S(k), _ptr(fc), _ptr(scores),
+                                         _ptr(ids), ctypes.byref(ko), _ptr(info)), "lm_search")
+        ke = ko.value
+        return (scores[: nq * ke].reshape(nq, ke), ids[: nq * ke].reshape(nq, ke),
+                {"scan_dim": int(info[0]), "codes_bytes": int(info[1]), "index_bytes": int(info[2])})
 
 def available(kind: str) -> bool:
     if kind == "port":

@@ -17,10 +17,12 @@
 
 #include "vectorcache/constants.hpp"
 #include "vectorcache/encode/encode.hpp"
+#include "vectorcache/index.hpp"
 #include "vectorcache/pack/pack.hpp"
 #include "vectorcache/quantize/codebook.hpp"
 #include "vectorcache/search/float_rescore.hpp"
 #include "vectorcache/search/search.hpp"
+#include "vectorcache/transform/landmarks.hpp"
 #include "vectorcache/transform/rotation.hpp"
 
 #ifdef _OPENMP
@@ -374,6 +376,75 @@ int vcref_float_rescore(size_t nq, size_t k, float* scores, uint64_t* ids, const
                            std::span<const float>(corpus, n_corpus * dim), dim);
     std::memcpy(scores, out.scores.data(), nq * k * sizeof(float));
     std::memcpy(ids, out.ids.data(), nq * k * sizeof(uint64_t));
+  });
+}
+
+// make_ortho_landmarks() (reference src/transform/landmarks.cpp:180-198):
+// L is dim x m, column j at L[j * dim].
+int vcref_ortho_landmarks(size_t dim, size_t m, uint64_t seed, float* L) {
+  return guarded([&] {
+    auto v = vc::make_ortho_landmarks(dim, m, seed);
+    std::memcpy(L, v.data(), dim * m * sizeof(float));
+  });
+}
+
+// quantize_landmarks() (reference src/transform/landmarks.cpp:236-269):
+// codes m x (pad16(dim) / 2) bytes, scales m.
+int vcref_quantize_landmarks(const float* L, size_t dim, size_t m, uint8_t* codes, float* scales) {
+  return guarded([&] {
+    auto q = vc::quantize_landmarks(std::span<const float>(L, dim * m), dim, m);
+    std::memcpy(codes, q.codes.data(), q.codes.size());
+    std::memcpy(scales, q.scales.data(), m * sizeof(float));
+  });
+}
+
+// project_rows_onto_landmarks() (reference src/transform/landmarks.cpp:200-234).
+int vcref_project_rows(const float* Y, size_t n, size_t dim, const float* L, size_t m, float* out,
+                       size_t out_stride) {
+  return guarded([&] { vc::project_rows_onto_landmarks(Y, n, dim, L, m, out, out_stride); });
+}
+
+// project_onto_landmarks_q4() (reference src/transform/landmarks.cpp:271-348)
+// with the 4-bit basis quantize_landmarks() makes of L.
+int vcref_project_q4(const float* q_rot, size_t nq, size_t dim, const float* L, size_t m, float* q_land,
+                     size_t scan_dim) {
+  return guarded([&] {
+    auto q = vc::quantize_landmarks(std::span<const float>(L, dim * m), dim, m);
+    vc::project_onto_landmarks_q4(q_rot, nq, dim, q, q_land, scan_dim);
+  });
+}
+
+// LandmarkIndex end to end (reference src/index.cpp:120-289): the rows of x
+// added in batches of batch_rows[b], prepare() after every batch when
+// prepare_between, else once at the end, then search() with the optional
+// float corpus.
+// info[0..2] = scan_dim, codes_bytes(), index_bytes().
+int vcref_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
+                    const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
+                    size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
+                    uint64_t* ids_out, size_t* k_out, size_t* info) {
+  return guarded([&] {
+    vc::LandmarkIndex ix(dim, scan_bits, m, seed, rerank_bits);
+    size_t off = 0;
+    for (size_t b = 0; b < n_batches; ++b) {
+      ix.add(std::span<const float>(x + off * dim, batch_rows[b] * dim));
+      off += batch_rows[b];
+      if (prepare_between) ix.prepare();
+    }
+    if (!prepare_between) ix.prepare();  // a second prepare() with no add reads released rows
+    std::span<const float> fc;
+    if (float_corpus != nullptr) fc = std::span<const float>(float_corpus, off * dim);
+    auto res = ix.search(std::span<const float>(q, nq * dim), k, fc);
+    *k_out = res.k;
+    if (res.k > 0 && nq > 0) {
+      std::memcpy(scores_out, res.scores.data(), res.nq * res.k * sizeof(float));
+      std::memcpy(ids_out, res.ids.data(), res.nq * res.k * sizeof(uint64_t));
+    }
+    if (info != nullptr) {
+      info[0] = ix.scan_dim();
+      info[1] = ix.codes_bytes();
+      info[2] = ix.index_bytes();
+    }
   });
 }
 

@@ -64,3 +64,37 @@ def rescore_inputs(dim):
     scores = r.standard_normal((6, 12)).astype(np.float32)
     counts = np.array([12, 5, 0, 12, 7, 1], np.uint64)
     return corpus, q, scores, ids, counts
+
+
+# Landmark index cases: (tag, n, nq, dim, n_landmarks, scan_bits, rerank_bits,
+# seed, k, batches, prepare_between, float corpus)
+LANDMARK_CASES = (
+    ("m20", 500, 7, 64, 20, 4, 0, 11, 10, None, False, False),
+    ("m100", 700, 5, 128, 100, 4, 4, 3, 25, None, False, False),
+    ("b2", 400, 4, 96, 96, 2, 3, 0, 7, None, False, False),
+    ("b3k", 300, 3, 64, 13, 3, 2, 5, 300, None, False, False),
+    ("batched", 600, 6, 256, 64, 4, 4, 9, 10, (200, 250, 150), False, False),
+    ("reprep", 600, 6, 256, 64, 4, 4, 9, 10, (200, 250, 150), True, False),
+    ("rescore", 300, 4, 64, 36, 4, 4, 1, 5, None, False, True),
+    ("d1536", 1500, 8, 1536, 1024, 4, 4, 0, 10, None, False, False),
+)
+
+
+def landmark_inputs(n, nq, dim):
+    """Seeded vectors (row scales varied) and queries with the structured rows
+    the reference's quantizers tie on: a zero row and rows whose rotation has
+    exact zeros (constant and two-level rows), some repeated as queries."""
+    r = np.random.default_rng(7100 + n + dim)
+    x = synthetic.mixture(n, dim, 16, 0.3, 7200 + n, 7300 + dim)
+    x *= r.uniform(0.5, 2.0, (n, 1)).astype(np.float32)
+    q = synthetic.mixture(nq, dim, 16, 0.3, 7200 + n, 7400 + nq)
+    x[0] = 1.0
+    x[1, ::2] = 2.0
+    x[1, 1::2] = 0.0
+    x[2, : dim // 2] = -1.0
+    x[2, dim // 2:] = 0.5
+    x[3] = 0.0
+    q[0] = x[0]
+    if nq > 2:
+        q[2] = x[2]
+    return np.ascontiguousarray(x, np.float32), np.ascontiguousarray(q, np.float32)

@@ -31,8 +31,8 @@ sys.path.insert(0, str(Path(__file__).resolve().parent))
 
 from oracle import Oracle  # noqa: E402
 from vectorcache_amd import synthetic  # noqa: E402
-from cases import (CASES, KS, RESCORE_DIMS, SCORE_PREPARED_CASES, SCORE_PREPARED_VARIANTS,  # noqa: E402
-                   case_data, rescore_inputs, score_prepared_inputs)
+from cases import (CASES, KS, LANDMARK_CASES, RESCORE_DIMS, SCORE_PREPARED_CASES, SCORE_PREPARED_VARIANTS,  # noqa: E402
+                   case_data, landmark_inputs, rescore_inputs, score_prepared_inputs)
 
 OUT = Path(__file__).resolve().parent
 
@@ -144,6 +144,34 @@ def score_prepared_fixture(R):
     np.savez_compressed(OUT / "score_prepared.npz", **out)
 
 
+def landmark_fixture(R):
+    """LandmarkIndex end to end (reference src/index.cpp:120-289) per case, the
+    landmark basis and its 4-bit form for the small shapes, and the q4 query
+    projection."""
+    out = {}
+    for tag, n, nq, dim, m, bits, rb, seed, k, batches, pb, fc in LANDMARK_CASES:
+        x, q = landmark_inputs(n, nq, dim)
+        s, i, info = R.lm_search(x, q, k, scan_bits=bits, n_landmarks=m, seed=seed, rerank_bits=rb,
+                                 batches=batches, prepare_between=pb, float_corpus=x if fc else None)
+        out[f"{tag}_scores"] = s
+        out[f"{tag}_ids"] = i
+        out[f"{tag}_info"] = np.array([info["scan_dim"], info["codes_bytes"], info["index_bytes"]], np.uint64)
+    for dim, m, seed in ((64, 20, 0), (128, 100, 7)):
+        L = R.ortho_landmarks(dim, m, seed)
+        codes, scales = R.quantize_landmarks(L)
+        if dim * m <= 4096:
+            out[f"L_{dim}_{m}_{seed}"] = L
+        out[f"L_{dim}_{m}_{seed}_sha"] = _d(L)
+        out[f"Lcodes_{dim}_{m}_{seed}"] = codes
+        out[f"Lscales_{dim}_{m}_{seed}"] = scales
+        qr = np.random.default_rng(dim + m).standard_normal((6, dim)).astype(np.float32)
+        qr[5] = 0.0
+        out[f"qrot_{dim}_{m}_{seed}"] = qr
+        out[f"qland_{dim}_{m}_{seed}"] = R.project_q4(qr, L, (m + 7) // 8 * 8)
+    out["L_1536_1024_0_sha"] = _d(R.ortho_landmarks(1536, 1024, 0))
+    np.savez_compressed(OUT / "landmark.npz", **out)
+
+
 def main():
     subprocess.run(["make", "-C", str(ROOT / "oracle"), "ref"], check=True, stdout=subprocess.DEVNULL)
     R = Oracle("reference")
@@ -154,6 +182,8 @@ def main():
         rescore_fixture(R)
     if only in (None, "score_prepared"):
         score_prepared_fixture(R)
+    if only in (None, "landmark"):
+        landmark_fixture(R)
     if only is not None:
         print("wrote", only)
         return

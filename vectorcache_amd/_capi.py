@@ -85,6 +85,26 @@ _SIGS = {
     "vc_heaps_to_results": ([S, S, P, P, P, P, P], I),
     "vc_float_rescore": ([S, S, P, P, P, P, P, S, S], I),
     "vc_float_rescore_device": ([S, S, P, P, P, P, P, S, S, P], I),
+    "vc_landmarks_make": ([S, S, ctypes.c_uint64, P], I),
+    "vc_landmarks_quantize": ([P, S, S, P, P], I),
+    "vc_lm_index_create": ([S, S, S, ctypes.c_uint64, S, ctypes.POINTER(P)], I),
+    "vc_lm_index_destroy": ([P], I),
+    "vc_lm_index_add": ([P, P, S], I),
+    "vc_lm_index_add_device": ([P, P, S], I),
+    "vc_lm_index_prepare": ([P], I),
+    "vc_lm_index_search": ([P, P, S, S, P, P, P, ctypes.POINTER(S)], I),
+    "vc_lm_index_search_device": ([P, P, S, S, P, P, P, ctypes.POINTER(S), P], I),
+    "vc_lm_index_query_rows": ([P, P, S, P, P], I),
+    "vc_lm_index_export": ([P, P, P, P], I),
+    "vc_lm_index_dim": ([P], S),
+    "vc_lm_index_size": ([P], S),
+    "vc_lm_index_scan_bits": ([P], S),
+    "vc_lm_index_rerank_bits": ([P], S),
+    "vc_lm_index_n_landmarks": ([P], S),
+    "vc_lm_index_scan_dim": ([P], S),
+    "vc_lm_index_is_prepared": ([P], I),
+    "vc_lm_index_codes_bytes": ([P], S),
+    "vc_lm_index_index_bytes": ([P], S),
     "vc_set_scan_tile": ([S], I),
     "vc_scan_tile_for": ([S, S, S], S),
     "vc_set_profiling": ([I], I),

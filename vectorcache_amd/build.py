@@ -3,7 +3,7 @@
     python -m vectorcache_amd.build          # or vectorcache_amd.build.build()
 
 * vc_kernels.hip, vc_capi.cpp  -> hipcc --offload-arch=gfx950
-* vc_host.cpp, vc_io.cpp       -> g++ (host precompute, contraction off; file formats)
+* vc_host.cpp, vc_landmark_host.cpp, vc_io.cpp -> g++ (host precompute, contraction off; file formats)
 
 Floating-point flags are part of the numerics contract: -ffp-contract=off
 everywhere (fused operations are explicit in the source), and packed FP32
@@ -31,7 +31,7 @@ HIP_FLAGS = [
     "-fhip-fp32-correctly-rounded-divide-sqrt", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{ROOT / 'include'}", f"-I{CSRC}", "-Wall", "-Wno-unused-function",
 ]
-HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off", "-fno-fast-math", f"-I{CSRC}", "-Wall"]
+HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-mfma", "-ffp-contract=off", "-fno-fast-math", f"-I{CSRC}", "-Wall"]
 
 
 def _run(cmd):
@@ -47,7 +47,7 @@ def build(verbose: bool = True) -> Path:
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         _run([HIPCC, *HIP_FLAGS, *lang, "-c", CSRC / src, "-o", obj])
         objs.append(obj)
-    for src in ("vc_host.cpp", "vc_io.cpp"):
+    for src in ("vc_host.cpp", "vc_landmark_host.cpp", "vc_io.cpp"):
         obj = BUILD / (src + ".o")
         _run(["g++", *HOST_FLAGS, "-c", CSRC / src, "-o", obj])
         objs.append(obj)

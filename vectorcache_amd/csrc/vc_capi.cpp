@@ -656,6 +656,23 @@ vc_status vc_codebook(size_t bits, size_t dim, float* boundaries, float* centroi
   });
 }
 
+vc_status vc_landmarks_make(size_t dim, size_t m, uint64_t seed, float* L) {
+  return guarded([&] {
+    if (L == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "L is null");
+    const auto& v = vchip::ortho_landmarks(dim, m, seed);
+    std::memcpy(L, v.data(), v.size() * 4);
+  });
+}
+
+vc_status vc_landmarks_quantize(const float* L, size_t dim, size_t m, uint8_t* codes, float* scales) {
+  return guarded([&] {
+    if (L == nullptr || codes == nullptr || scales == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    const auto q = vchip::quantize_landmarks(L, dim, m);
+    std::memcpy(codes, q.codes.data(), q.codes.size());
+    std::memcpy(scales, q.scales.data(), m * 4);
+  });
+}
+
 namespace {
 struct Scoped {
   hipStream_t s = nullptr;
@@ -1237,3 +1254,316 @@ vc_status vc_vcex_source_label(const char* npy_path, const char* dataset, char* 
 vc_status vc_vcex_cache_path(const vc_vcex_key* key, char* out, size_t cap, size_t* len) {
   return guarded_io([&] { put_string(vcio::exact_topk_cache_path(vcex_key(key)), out, cap, len); });
 }
+
+// ---------------------------------------------------------------------------
+// Landmark index (reference LandmarkIndex, src/index.cpp:120-289, on the AVX2
+// path). HBM state: the landmark basis L (dim x m floats, column-major) and
+// its 4-bit form (m columns of dim_pad / 2 bytes + m scales); landmark rows
+// (n x scan_dim floats) from add() until prepare(); residual norms (n);
+// after prepare() the scan codes (tiled, scan_dim / (8 / scan_bits) groups)
+// and alpha (n). prepare() encodes every row and releases the rows, so rows
+// added after a prepare() find the earlier ones zeroed, as in the reference
+// (its landmark_rows_ is released and resized with zeros).
+// ---------------------------------------------------------------------------
+struct vc_lm_index {
+  size_t dim = 0, scan_bits = 0, rerank_bits = 0, m = 0, scan_dim = 0, dim_pad = 0;
+  uint64_t seed = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  vchip::CodeTables t_add{};   // (dim, rerank_bits): in-order norm + rotation, y-hat codebook
+  vchip::CodeTables t_query{}; // (dim): query norm + rotation
+  vchip::CodeTables t_scan{};  // (scan_dim, scan_bits): landmark rows, prerotated
+  DevBuf basis;                // L | codes | scales
+  size_t o_codes = 0, o_scales = 0;
+  DevBuf rows;                 // n x scan_dim while live
+  bool rows_live = true;
+  DevBuf rn;                   // residual norms
+  DevBuf tiled, alpha;
+  size_t cap = 0;              // vector capacity of tiled / alpha
+  size_t n = 0, n_encoded = 0;
+  bool prepared = false;
+  DevBuf work, qwork, scratch, io;
+  std::mutex mu;
+};
+
+namespace {
+
+constexpr size_t kLmAddChunk = 16384;  // rows per add() pass (rotated rows staged in HBM)
+
+void lm_add(vc_lm_index* ix, const float* v, size_t n_new, bool host) {
+  if (n_new == 0) return;
+  const size_t D = ix->dim, S = ix->scan_dim, base = ix->n, total = ix->n + n_new;
+  if (total > 0xFFFFFFFFull) fail(VC_ERR_INVALID_ARGUMENT, "index exceeds 2^32 vectors");
+  hipStream_t s = ix->stream;
+  if (!ix->rows_live) {
+    ix->rows.reserve(total * S * 4, s);
+    hip_check(hipMemsetAsync(ix->rows.p, 0, base * S * 4, s), "memset");
+    ix->rows_live = true;
+  } else {
+    ix->rows.reserve(total * S * 4, s, /*keep=*/true);
+  }
+  ix->rn.reserve(total * 4, s, /*keep=*/true);
+  const size_t chunk = std::min(n_new, kLmAddChunk);
+  Carve c;
+  const size_t ox = c.take(host ? chunk * D * 4 : 0), orot = c.take(chunk * D * 4), ovn = c.take(chunk * 4);
+  ix->work.reserve(c.off, s);
+  for (size_t off = 0; off < n_new; off += chunk) {
+    const size_t cn = std::min(chunk, n_new - off);
+    const float* x = v + off * D;
+    if (host) {
+      hip_check(hipMemcpyAsync(ix->work.as<float>(ox), x, cn * D * 4, hipMemcpyHostToDevice, s), "copy");
+      x = ix->work.as<float>(ox);
+    }
+    float* rot = ix->work.as<float>(orot);
+    float* vn = ix->work.as<float>(ovn);
+    float* dst = ix->rows.as<float>((base + off) * S * 4);
+    hip_check(vchip::launch_encode(ix->t_add, x, cn, 0, nullptr, nullptr, rot, nullptr, s), "encode(rotate)");
+    hip_check(vchip::launch_lm_yhat(ix->t_add, rot, cn, vn, s), "lm_yhat");
+    hip_check(vchip::launch_lm_project(rot, cn, int(D), ix->basis.as<float>(), int(ix->m), dst, int(S), s),
+              "lm_project");
+    hip_check(vchip::launch_lm_resid(dst, cn, int(ix->m), int(S), vn, ix->rn.as<float>((base + off) * 4), s),
+              "lm_resid");
+    if (host) hip_check(hipStreamSynchronize(s), "sync");  // the staging buffer is reused
+  }
+  hip_check(hipStreamSynchronize(s), "sync");
+  ix->n = total;
+  ix->prepared = false;
+}
+
+// Rotated query rows -> landmark rows (nq x scan_dim) and |q_perp| (nq), in qwork.
+void lm_query_rows(vc_lm_index* ix, const float* q_dev, size_t nq, float** q_land, float** q_perp, hipStream_t s) {
+  Carve c;
+  const size_t orot = c.take(nq * ix->dim * 4), oland = c.take(nq * ix->scan_dim * 4), operp = c.take(nq * 4);
+  ix->qwork.reserve(c.off, s);
+  float* rot = ix->qwork.as<float>(orot);
+  *q_land = ix->qwork.as<float>(oland);
+  *q_perp = ix->qwork.as<float>(operp);
+  hip_check(vchip::launch_encode(ix->t_query, q_dev, nq, 0, nullptr, nullptr, rot, nullptr, s), "encode(rotate)");
+  hip_check(vchip::launch_lm_query(rot, nq, int(ix->dim), int(ix->dim_pad), ix->basis.as<uint8_t>(ix->o_codes),
+                                   ix->basis.as<float>(ix->o_scales), int(ix->m), *q_land, int(ix->scan_dim),
+                                   *q_perp, s),
+            "lm_query");
+}
+
+size_t lm_search_dev(vc_lm_index* ix, const float* q_dev, size_t nq, size_t k, const float* corpus_dev,
+                     float* s_dev, uint64_t* i_dev, hipStream_t s) {
+  if (!ix->prepared) fail(VC_ERR_STATE, "call prepare() before search");
+  const size_t ke = std::min(k, ix->n_encoded);
+  if (nq == 0 || ke == 0) return ke;
+  if (ke > size_t(vchip::kMaxTopK)) fail(VC_ERR_INVALID_ARGUMENT, "k exceeds the GPU top-k capacity of 2048");
+  float *q_land = nullptr, *q_perp = nullptr;
+  lm_query_rows(ix, q_dev, nq, &q_land, &q_perp, s);
+  search_tiled(ix->t_scan, ix->tiled.as<uint8_t>(), ix->alpha.as<float>(), ix->n_encoded, q_land, nq, ke, nullptr,
+               s_dev, i_dev, ix->scratch, s);
+  hip_check(vchip::launch_residual_post(nq, int(ke), s_dev, i_dev, ix->rn.as<float>(), q_perp, ix->n_encoded,
+                                        nullptr, s),
+            "residual_post");
+  if (corpus_dev != nullptr)
+    hip_check(vchip::launch_float_rescore(nq, int(ke), s_dev, i_dev, nullptr, q_dev, corpus_dev, ix->n, int(ix->dim), s),
+              "float_rescore");
+  return ke;
+}
+
+}  // namespace
+
+extern "C" {
+
+vc_status vc_lm_index_create(size_t dim, size_t scan_bits, size_t n_landmarks, uint64_t landmark_seed,
+                             size_t rerank_bits, vc_lm_index** out) {
+  return guarded([&] {
+    if (out == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "out is null");
+    const size_t rb = rerank_bits == 0 ? scan_bits : rerank_bits;
+    if (dim == 0 || dim % 8 != 0 || dim > vchip::kMaxDim)
+      fail(VC_ERR_INVALID_ARGUMENT, "dim must be a multiple of 8 in [8, MAX_DIM]");
+    if (scan_bits < 2 || scan_bits > 4) fail(VC_ERR_INVALID_ARGUMENT, "scan_bits must be 2, 3, or 4");
+    if (rb < 2 || rb > 4) fail(VC_ERR_INVALID_ARGUMENT, "rerank_bits must be 2, 3, or 4");
+    if (n_landmarks == 0 || n_landmarks > dim) fail(VC_ERR_INVALID_ARGUMENT, "n_landmarks must be in 1..dim");
+    auto* ix = new vc_lm_index();
+    try {
+      ix->dim = dim;
+      ix->scan_bits = scan_bits;
+      ix->rerank_bits = rb;
+      ix->m = n_landmarks;
+      ix->scan_dim = (n_landmarks + 7) & ~size_t(7);
+      ix->dim_pad = (dim + 15) & ~size_t(15);
+      ix->seed = landmark_seed;
+      hip_check(hipGetDevice(&ix->device), "hipGetDevice");
+      hip_check(hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking), "hipStreamCreate");
+      ix->t_add = make_tables(dim, rb);
+      ix->t_add.norm_seq = 1;
+      ix->t_query = make_tables(dim, rb);
+      ix->t_scan = make_tables(ix->scan_dim, scan_bits);
+      ix->t_scan.prerotated = 1;
+      ix->t_scan.norm_seq = 1;
+      const auto& L = vchip::ortho_landmarks(dim, n_landmarks, landmark_seed);
+      const auto q = vchip::quantize_landmarks(L.data(), dim, n_landmarks);
+      Carve c;
+      const size_t oL = c.take(L.size() * 4);
+      ix->o_codes = c.take(q.codes.size());
+      ix->o_scales = c.take(q.scales.size() * 4);
+      ix->basis.reserve(c.off, ix->stream);
+      hip_check(hipMemcpyAsync(ix->basis.as<float>(oL), L.data(), L.size() * 4, hipMemcpyHostToDevice, ix->stream), "copy");
+      hip_check(hipMemcpyAsync(ix->basis.as<uint8_t>(ix->o_codes), q.codes.data(), q.codes.size(),
+                               hipMemcpyHostToDevice, ix->stream), "copy");
+      hip_check(hipMemcpyAsync(ix->basis.as<float>(ix->o_scales), q.scales.data(), q.scales.size() * 4,
+                               hipMemcpyHostToDevice, ix->stream), "copy");
+      hip_check(hipStreamSynchronize(ix->stream), "sync");
+    } catch (...) {
+      if (ix->stream) (void)hipStreamDestroy(ix->stream);
+      ix->basis.release();
+      delete ix;
+      throw;
+    }
+    *out = ix;
+  });
+}
+
+vc_status vc_lm_index_destroy(vc_lm_index* ix) {
+  return guarded([&] {
+    if (ix == nullptr) return;
+    (void)hipStreamSynchronize(ix->stream);
+    for (DevBuf* b : {&ix->basis, &ix->rows, &ix->rn, &ix->tiled, &ix->alpha, &ix->work, &ix->qwork, &ix->scratch,
+                      &ix->io})
+      b->release();
+    (void)hipStreamDestroy(ix->stream);
+    delete ix;
+  });
+}
+
+vc_status vc_lm_index_add(vc_lm_index* ix, const float* vectors, size_t n) {
+  return guarded([&] {
+    if (ix == nullptr || (vectors == nullptr && n > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    lm_add(ix, vectors, n, true);
+  });
+}
+
+vc_status vc_lm_index_add_device(vc_lm_index* ix, const float* vectors_dev, size_t n) {
+  return guarded([&] {
+    if (ix == nullptr || (vectors_dev == nullptr && n > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    hip_check(hipDeviceSynchronize(), "sync caller work");  // the producer's stream may differ
+    lm_add(ix, vectors_dev, n, false);
+  });
+}
+
+vc_status vc_lm_index_prepare(vc_lm_index* ix) {
+  return guarded([&] {
+    if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");
+    std::lock_guard lock(ix->mu);
+    if (ix->rows_live) {
+      // every row (zeros for rows added before an earlier prepare)
+      const size_t n = ix->n;
+      if (n > ix->cap) {
+        size_t ncap = std::max(n, ix->cap + ix->cap / 2);
+        ncap = (ncap + 63) / 64 * 64;
+        ix->tiled.reserve(vchip::tiled_bytes(ncap, ix->t_scan.groups16), ix->stream);
+        ix->alpha.reserve(ncap * 4, ix->stream);
+        ix->cap = ncap;
+      }
+      ix->t_scan.debug_flags = g_debug_flags.load();
+      if (n > 0)
+        hip_check(vchip::launch_encode(ix->t_scan, ix->rows.as<float>(), n, 0, ix->tiled.as<uint8_t>(), nullptr,
+                                       nullptr, ix->alpha.as<float>(), ix->stream),
+                  "encode(landmark rows)");
+      hip_check(hipStreamSynchronize(ix->stream), "sync");
+      ix->rows.release();
+      ix->rows_live = false;
+      ix->n_encoded = n;
+    }
+    ix->prepared = true;
+  });
+}
+
+vc_status vc_lm_index_search(vc_lm_index* ix, const float* queries, size_t nq, size_t k, const float* float_corpus,
+                             float* scores, uint64_t* ids, size_t* k_out) {
+  return guarded([&] {
+    if (ix == nullptr || (queries == nullptr && nq > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    if (!ix->prepared) fail(VC_ERR_STATE, "call prepare() before search");
+    const size_t ke = std::min(k, ix->n_encoded);
+    if (k_out) *k_out = ke;
+    if (nq == 0 || ke == 0) return;
+    if (scores == nullptr || ids == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null output");
+    Carve c;
+    const size_t oq = c.take(nq * ix->dim * 4), os = c.take(nq * ke * 4), oi = c.take(nq * ke * 8),
+                 ox = c.take(float_corpus ? ix->n * ix->dim * 4 : 0);
+    ix->io.reserve(c.off, ix->stream);
+    hip_check(hipMemcpyAsync(ix->io.as<float>(oq), queries, nq * ix->dim * 4, hipMemcpyHostToDevice, ix->stream),
+              "copy");
+    if (float_corpus)
+      hip_check(hipMemcpyAsync(ix->io.as<float>(ox), float_corpus, ix->n * ix->dim * 4, hipMemcpyHostToDevice,
+                               ix->stream),
+                "copy");
+    lm_search_dev(ix, ix->io.as<float>(oq), nq, k, float_corpus ? ix->io.as<float>(ox) : nullptr,
+                  ix->io.as<float>(os), ix->io.as<uint64_t>(oi), ix->stream);
+    hip_check(hipMemcpyAsync(scores, ix->io.as<float>(os), nq * ke * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipMemcpyAsync(ids, ix->io.as<uint64_t>(oi), nq * ke * 8, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipStreamSynchronize(ix->stream), "sync");
+  });
+}
+
+vc_status vc_lm_index_search_device(vc_lm_index* ix, const float* queries_dev, size_t nq, size_t k,
+                                    const float* float_corpus_dev, float* scores_dev, uint64_t* ids_dev,
+                                    size_t* k_out, void* stream) {
+  return guarded([&] {
+    if (ix == nullptr || (queries_dev == nullptr && nq > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ix->stream;
+    const size_t ke = lm_search_dev(ix, queries_dev, nq, k, float_corpus_dev, scores_dev, ids_dev, s);
+    if (k_out) *k_out = ke;
+  });
+}
+
+vc_status vc_lm_index_query_rows(vc_lm_index* ix, const float* queries, size_t nq, float* q_land, float* q_perp) {
+  return guarded([&] {
+    if (ix == nullptr || (nq > 0 && (queries == nullptr || q_land == nullptr || q_perp == nullptr)))
+      fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    if (nq == 0) return;
+    ix->io.reserve(nq * ix->dim * 4, ix->stream);
+    hip_check(hipMemcpyAsync(ix->io.p, queries, nq * ix->dim * 4, hipMemcpyHostToDevice, ix->stream), "copy");
+    float *ql = nullptr, *qp = nullptr;
+    lm_query_rows(ix, ix->io.as<float>(), nq, &ql, &qp, ix->stream);
+    hip_check(hipMemcpyAsync(q_land, ql, nq * ix->scan_dim * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipMemcpyAsync(q_perp, qp, nq * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipStreamSynchronize(ix->stream), "sync");
+  });
+}
+
+vc_status vc_lm_index_export(vc_lm_index* ix, float* residual_norms, float* scales, uint8_t* codes_flat) {
+  return guarded([&] {
+    if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");
+    std::lock_guard lock(ix->mu);
+    if (residual_norms && ix->n)
+      hip_check(hipMemcpyAsync(residual_norms, ix->rn.p, ix->n * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+    if ((scales || codes_flat) && !ix->prepared) fail(VC_ERR_STATE, "call prepare() before exporting codes");
+    const size_t ne = ix->n_encoded, G = size_t(ix->t_scan.groups);
+    if (scales && ne)
+      hip_check(hipMemcpyAsync(scales, ix->alpha.p, ne * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+    if (codes_flat && ne) {
+      ix->io.reserve(ne * G, ix->stream);
+      hip_check(vchip::launch_tiled_to_flat(ix->tiled.as<uint8_t>(), ne, int(G), ix->t_scan.groups16,
+                                            ix->io.as<uint8_t>(), ix->stream),
+                "tiled_to_flat");
+      hip_check(hipMemcpyAsync(codes_flat, ix->io.p, ne * G, hipMemcpyDeviceToHost, ix->stream), "copy");
+    }
+    hip_check(hipStreamSynchronize(ix->stream), "sync");
+  });
+}
+
+size_t vc_lm_index_dim(const vc_lm_index* ix) { return ix ? ix->dim : 0; }
+size_t vc_lm_index_size(const vc_lm_index* ix) { return ix ? ix->n : 0; }
+size_t vc_lm_index_scan_bits(const vc_lm_index* ix) { return ix ? ix->scan_bits : 0; }
+size_t vc_lm_index_rerank_bits(const vc_lm_index* ix) { return ix ? ix->rerank_bits : 0; }
+size_t vc_lm_index_n_landmarks(const vc_lm_index* ix) { return ix ? ix->m : 0; }
+size_t vc_lm_index_scan_dim(const vc_lm_index* ix) { return ix ? ix->scan_dim : 0; }
+int vc_lm_index_is_prepared(const vc_lm_index* ix) { return ix && ix->prepared ? 1 : 0; }
+size_t vc_lm_index_codes_bytes(const vc_lm_index* ix) {
+  return ix ? (ix->n_encoded + 191) / 192 * 192 * size_t(ix->t_scan.groups) : 0;
+}
+size_t vc_lm_index_index_bytes(const vc_lm_index* ix) {
+  return ix ? vc_lm_index_codes_bytes(ix) + 4 * ix->n + 12 * ix->n_encoded : 0;
+}
+
+}  // extern "C"

@@ -63,4 +63,23 @@ TqAnchors tq_anchors(std::size_t bits, std::size_t dim, std::size_t n);
 
 inline std::size_t byte_groups(std::size_t bits, std::size_t dim) { return dim / (8 / bits); }
 
+// ---- landmark basis (vc_landmark_host.cpp) ----------------------------------
+inline constexpr std::size_t kDefaultLandmarks = 1024;
+
+// Orthonormal landmark matrix L (dim x m floats, column j at L[j * dim]):
+// seeded Gaussian matrix, modified Gram-Schmidt in double. Memoized per
+// (dim, m, seed), thread-safe. Throws std::invalid_argument unless
+// 0 < m <= dim, std::runtime_error on a degenerate column.
+const std::vector<float>& ortho_landmarks(std::size_t dim, std::size_t m, std::uint64_t seed);
+
+// Per-column absmax 4-bit quantization of L. codes: m columns of dim_pad / 2
+// bytes (dim_pad = dim rounded up to 16), byte i of a column holds dims 2i
+// (low nibble) and 2i + 1 (high nibble) as level + 7, levels in [-7, 7].
+struct QuantizedLandmarks {
+  std::vector<std::uint8_t> codes;
+  std::vector<float> scales;  // m
+  std::size_t dim = 0, dim_pad = 0, m = 0;
+};
+QuantizedLandmarks quantize_landmarks(const float* L, std::size_t dim, std::size_t m);
+
 }  // namespace vchip

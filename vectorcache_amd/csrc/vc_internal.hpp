@@ -26,6 +26,12 @@ struct CodeTables {
   float cell_lo, cell_inv_w;
   int use_cells;
   int debug_flags;                // bit 0: take the sequential alpha path for every vector
+  // Landmark index modes (zero for the plain flat path):
+  int prerotated;  // rows are already in the rotated basis: encode scales them by
+                   // 1 / norm without rotating; prepare_queries builds LUTs from the
+                   // rows as given (no norm, no rotation)
+  int norm_seq;    // encode: the 8 norm chains are summed in order and rows with
+                   // norm < 1e-10 become zero (the landmark l2-normalize)
 };
 
 // Internal scan layout ("tiled"): for vector v and byte group g the flat byte
@@ -127,6 +133,22 @@ hipError_t launch_gpu32_to_flat(const std::uint32_t* words, std::size_t n, int g
 // Residual post-correction + re-rank + id remap of rows x k results in place.
 hipError_t launch_residual_post(std::size_t rows, int k, float* scores, std::uint64_t* ids, const float* rn,
                                 const float* rs, std::size_t n, const std::uint64_t* id_map, hipStream_t s);
+// ---- landmark index -------------------------------------------------------
+// rows (n x dim, rotated unit rows in place) -> alpha * centroid rows
+// (x >= boundary levels of t's codebook), vnorm2[n] = their squared norms.
+hipError_t launch_lm_yhat(const CodeTables& t, float* rows, std::size_t n, float* vnorm2, hipStream_t s);
+// out[r][j] = <Y_r, L_j> in the 8-lane order, j < m; zero for m <= j < stride.
+hipError_t launch_lm_project(const float* Y, std::size_t n, int dim, const float* L, int m, float* out,
+                             int stride, hipStream_t s);
+// rn[r] = sqrt(max(0, vnorm2[r] - |rows_r[0:m)|^2)), rows with the given stride.
+hipError_t launch_lm_resid(const float* rows, std::size_t n, int m, int stride, const float* vnorm2, float* rn,
+                           hipStream_t s);
+// Query projection onto the 4-bit basis: q_land nq x stride (zero past m) and
+// q_perp[nq] from rotated query rows q_rot (nq x dim). codes: m columns of
+// dim_pad / 2 bytes.
+hipError_t launch_lm_query(const float* q_rot, std::size_t nq, int dim, int dim_pad, const std::uint8_t* codes,
+                           const float* lscales, int m, float* q_land, int stride, float* q_perp, hipStream_t s);
+
 hipError_t launch_flat_to_perm0(const std::uint8_t* flat, std::size_t n, int groups,
                                 std::uint8_t* perm0, hipStream_t s);
 hipError_t launch_flat_to_seq(const std::uint8_t* flat, std::size_t n, int groups, std::uint8_t* seq,

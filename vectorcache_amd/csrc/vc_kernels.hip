@@ -87,8 +87,11 @@ __device__ float norm8_lds(const float* __restrict__ x, int D, int lane) {
 
 // Row norm of the reference (src/encode/encode.cpp:16-26): 8 interleaved
 // chains c[j % 8] += x_j^2 (fused), tree ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)).
+// seq: the chains summed in order c0 + c1 + ... + c7 instead (the landmark
+// index's l2-normalize, src/index.cpp:21-50, whose double sqrt of a float
+// sum rounds to the same float as sqrtf).
 // Called with lanes grouped by 8: lane & 7 is the chain. Returns sqrtf.
-__device__ float norm8(const float* __restrict__ xr, int D, int chain, bool active) {
+__device__ float norm8(const float* __restrict__ xr, int D, int chain, bool active, bool seq = false) {
   // the chain is sequential; its loads are not: fetch 32 ahead
   constexpr int kAhead = 32;
   float c = 0.f;
@@ -102,6 +105,13 @@ __device__ float norm8(const float* __restrict__ xr, int D, int chain, bool acti
       for (int u = 0; u < kAhead; ++u)
         if (t0 + u < steps) c = __fmaf_rn(v[u], v[u], c);
     }
+  }
+  if (seq) {
+    const int base = (int)(threadIdx.x & 63) & ~7;
+    float sum = __shfl(c, base);
+#pragma unroll
+    for (int j = 1; j < 8; ++j) sum = sum + __shfl(c, base + j);
+    return sqrt_rn(sum);
   }
   c = c + __shfl_xor(c, 1);
   c = c + __shfl_xor(c, 2);
@@ -130,14 +140,21 @@ __global__ __launch_bounds__(64) void encode_kernel(CodeTables t, const float* _
 
   for (long batch = blockIdx.x; batch * 8 < n; batch += gridDim.x) {
     const long r8 = batch * 8 + (lane >> 3);
-    const float nrm_mine = norm8(x + (r8 < n ? r8 : 0) * (long)D, D, lane & 7, r8 < n);
+    const float nrm_mine = norm8(x + (r8 < n ? r8 : 0) * (long)D, D, lane & 7, r8 < n, t.norm_seq != 0);
     const int rows = (int)((n - batch * 8) < 8 ? (n - batch * 8) : 8);
     for (int rr = 0; rr < rows; ++rr) {
       const long row = batch * 8 + rr;
       const float nv = __shfl(nrm_mine, rr * 8);
-      const float inv = nv > kMinNorm ? div_rn(1.f, nv) : 0.f;
+      float inv = nv > kMinNorm ? div_rn(1.f, nv) : 0.f;
+      if (t.norm_seq) inv = nv < kMinNorm ? 0.f : div_rn(1.f, nv);
       __syncthreads();
-      rotate_row_lds(t, x + row * (long)D, inv, y, lane);
+      if (t.prerotated) {
+        // rows already in the rotated basis (landmark rows): unit = x * inv
+        for (int i = lane; i < D; i += 64) y[i] = x[row * (long)D + i] * inv;
+        __syncthreads();
+      } else {
+        rotate_row_lds(t, x + row * (long)D, inv, y, lane);
+      }
       if (rotated != nullptr)
         for (int i = lane; i < D; i += 64) rotated[row * (long)D + i] = y[i];
       if (!quant) continue;
@@ -872,6 +889,11 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTable
   for (long qi = blockIdx.x; qi < nq; qi += gridDim.x) {
     const float* xr = q + qi * (long)D;
     __syncthreads();
+    if (t.prerotated) {
+      // LUTs of the rows as given (landmark query rows, src/search/search.cpp:973-1050)
+      stage_to_lds(y, xr, D);
+      __syncthreads();
+    } else {
     stage_to_lds(xs, xr, D);
     __syncthreads();
     if (wave == 0) {
@@ -919,6 +941,7 @@ __global__ __launch_bounds__(kPrepThreads) void prepare_queries_kernel(CodeTable
       for (int i = tid; i < D; i += nt) y[i] = y[i] * t.inv_sqrt_block;
       __syncthreads();
     }
+    }  // rotated rows
     if (tq) {
       // keep y for bias_corr, divide by the TQ+ scale; bias_corr =
       // (float) -sum_d (double) y_d * shift_d, sequential in d in the
@@ -2443,6 +2466,255 @@ __global__ void flat_to_packed_kernel(const uint8_t* __restrict__ flat, long n, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Landmark index (reference src/index.cpp:151-265 and the AVX2 path of
+// src/search/search.cpp:1304-1546, where sign planes are never built).
+// ---------------------------------------------------------------------------
+
+// alpha * y-hat rows (src/index.cpp:88-117), one thread per row, in place:
+// level = #{b : x >= b}, alpha = (float) (dot / |c|^2) over sequential double
+// sums (the products are exact in double); the squared norm of the scaled row
+// sums ROUNDED squares in element order, which is what the reference's
+// vectorized fold computes for rows of dim % 8 == 0.
+__global__ __launch_bounds__(256) void lm_yhat_kernel(CodeTables t, float* __restrict__ rows, long n,
+                                                      float* __restrict__ vnorm2) {
+  __shared__ float cen[16];
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 16; ++i) cen[i] = t.centroids[i];
+  __syncthreads();
+  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int D = t.dim, nb = (1 << t.bits) - 1;
+  float* row = rows + r * (long)D;
+  auto level = [&](float x) {
+    int k = 0;
+    for (int b = 0; b < nb; ++b) k += x >= t.boundaries[b] ? 1 : 0;
+    return k;
+  };
+  double dot = 0.0, cc = 0.0;
+  for (int i = 0; i < D; i += 4) {
+    const float4 v = *reinterpret_cast<const float4*>(row + i);
+    const float xs[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double c = (double)cen[level(xs[u])];
+      dot = dot + (double)xs[u] * c;
+      cc = cc + c * c;
+    }
+  }
+  const float alpha = cc > 1e-12 ? (float)(dot / cc) : 0.f;
+  float vn = 0.f;
+  for (int i = 0; i < D; i += 4) {
+    const float4 v = *reinterpret_cast<const float4*>(row + i);
+    float xs[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float vi = alpha * cen[level(xs[u])];
+      xs[u] = vi;
+      vn = vn + vi * vi;
+    }
+    *reinterpret_cast<float4*>(row + i) = make_float4(xs[0], xs[1], xs[2], xs[3]);
+  }
+  vnorm2[r] = vn;
+}
+
+// Rows onto landmarks (src/transform/landmarks.cpp:104-121,200-234): out[r][j]
+// = <Y_r, L_j> as the AVX2 dot: 8 chains acc[l] = fma(y[8t + l], L[8t + l],
+// acc[l]) in t order, then ((a0 + a4) + (a1 + a5)) + ((a2 + a6) + (a3 + a7)).
+// Workgroup: 32 rows x 32 landmarks. Lane l = tid & 7 runs chain l of 4 rows
+// x 8 landmarks; the 8 chains of an output meet through lane shuffles.
+constexpr int kLmK = 64;             // dims per LDS stage
+constexpr int kLmPitch = kLmK + 2;   // row pitch: the 4 row blocks of a half-wave hit distinct banks
+__global__ __launch_bounds__(256) void lm_project_kernel(const float* __restrict__ Y, long n, int D,
+                                                         const float* __restrict__ L, int m,
+                                                         float* __restrict__ out, int stride) {
+  __shared__ float ys[32 * kLmPitch], ls[32 * kLmPitch];
+  const int tid = threadIdx.x, l = tid & 7, oct = tid >> 3, rb = oct & 7, cb = oct >> 3;
+  const long r0 = (long)blockIdx.x * 32;
+  const int j0 = (int)blockIdx.y * 32;
+  float acc[4][8];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[i][c] = 0.f;
+  for (int k0 = 0; k0 < D; k0 += kLmK) {
+    const int kc = min(kLmK, D - k0);  // a multiple of 8
+#pragma unroll
+    for (int e = tid; e < 32 * kLmK / 4; e += 256) {
+      const int rr = e / (kLmK / 4), c4 = (e % (kLmK / 4)) * 4;
+      float4 yv = make_float4(0.f, 0.f, 0.f, 0.f), lv = yv;
+      if (c4 < kc) {
+        if (r0 + rr < n) yv = *reinterpret_cast<const float4*>(Y + (r0 + rr) * (long)D + k0 + c4);
+        if (j0 + rr < m) lv = *reinterpret_cast<const float4*>(L + (long)(j0 + rr) * D + k0 + c4);
+      }
+      float* yd = ys + rr * kLmPitch + c4;
+      float* ld = ls + rr * kLmPitch + c4;
+      yd[0] = yv.x; yd[1] = yv.y; yd[2] = yv.z; yd[3] = yv.w;
+      ld[0] = lv.x; ld[1] = lv.y; ld[2] = lv.z; ld[3] = lv.w;
+    }
+    __syncthreads();
+    const int steps = kc / 8;
+    for (int s8 = 0; s8 < steps; ++s8) {
+      float yv[4], lv[8];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) yv[i] = ys[(rb * 4 + i) * kLmPitch + s8 * 8 + l];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) lv[c] = ls[(cb * 8 + c) * kLmPitch + s8 * 8 + l];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[i][c] = __fmaf_rn(yv[i], lv[c], acc[i][c]);
+    }
+    __syncthreads();
+  }
+  const int j = j0 + cb * 8 + l;  // lane l writes landmark cb * 8 + l of its 4 rows
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float mine = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      float v = acc[i][c];
+      v = v + __shfl_xor(v, 4);
+      v = v + __shfl_xor(v, 1);
+      v = v + __shfl_xor(v, 2);
+      mine = c == l ? v : mine;
+    }
+    const long r = r0 + rb * 4 + i;
+    if (r < n && j < stride) out[r * stride + j] = j < m ? mine : 0.f;
+  }
+}
+
+// Residual norms (src/index.cpp:60-82,200-210), one thread per row:
+// |row[0:m)|^2 as the AVX2 norm_sq (8 fused chains over m & ~7 entries,
+// summed in order; of the tail, a block of 4 rounded squares added in order
+// when at least 4 remain, then fused multiply-adds), rn = sqrtf(max(0,
+// vnorm2 - that)).
+__global__ __launch_bounds__(256) void lm_resid_kernel(const float* __restrict__ rows, long n, int m,
+                                                       int stride, const float* __restrict__ vnorm2,
+                                                       float* __restrict__ rn) {
+  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const float* x = rows + r * (long)stride;
+  float c[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const int m8 = m & ~7;
+  for (int j = 0; j < m8; j += 8) {
+    const float4 a = *reinterpret_cast<const float4*>(x + j);
+    const float4 b = *reinterpret_cast<const float4*>(x + j + 4);
+    c[0] = __fmaf_rn(a.x, a.x, c[0]);
+    c[1] = __fmaf_rn(a.y, a.y, c[1]);
+    c[2] = __fmaf_rn(a.z, a.z, c[2]);
+    c[3] = __fmaf_rn(a.w, a.w, c[3]);
+    c[4] = __fmaf_rn(b.x, b.x, c[4]);
+    c[5] = __fmaf_rn(b.y, b.y, c[5]);
+    c[6] = __fmaf_rn(b.z, b.z, c[6]);
+    c[7] = __fmaf_rn(b.w, b.w, c[7]);
+  }
+  float s = c[0];
+#pragma unroll
+  for (int u = 1; u < 8; ++u) s = s + c[u];
+  int j = m8;
+  if (m - j >= 4) {
+    for (int u = 0; u < 4; ++u) s = s + x[j + u] * x[j + u];
+    j += 4;
+  }
+  for (; j < m; ++j) s = __fmaf_rn(x[j], x[j], s);
+  const float d = vnorm2[r] - s;
+  rn[r] = sqrt_rn(0.f < d ? d : 0.f);
+}
+
+// Query side of the landmark search (src/transform/landmarks.cpp:271-348,
+// src/search/search.cpp:1349-1378), one workgroup per query: int8 levels
+// round(x * (127 / amax)) clamped to +-127, exact integer dots with the 4-bit
+// basis columns (nibble u = level + 7: dot = sum q u - 7 sum q, four bytes
+// per v_dot4), q_land[j] = (q_scale * scale_j) * (float) dot; q_perp =
+// (float) sqrt(max(0, |q|^2 - |q_land|^2)), both sums sequential in double.
+__global__ __launch_bounds__(256) void lm_query_kernel(const float* __restrict__ q_rot, long nq, int D,
+                                                       int dim_pad, const uint8_t* __restrict__ codes,
+                                                       const float* __restrict__ lscales, int m,
+                                                       float* __restrict__ q_land, int stride,
+                                                       float* __restrict__ q_perp) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float* qf = reinterpret_cast<float*>(smem);  // D
+  float* ql = qf + D;                          // stride
+  int* qe = reinterpret_cast<int*>(ql + stride);  // dims 8w + {0, 2, 4, 6} as bytes
+  int* qo = qe + dim_pad / 8;                     // dims 8w + {1, 3, 5, 7}
+  __shared__ float s_amax[4];
+  __shared__ int s_sum[4];
+  __shared__ double s_q2;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int W = dim_pad / 8, bpc = dim_pad / 2;
+  for (long qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+    __syncthreads();
+    const float* xr = q_rot + qi * (long)D;
+    float am = 0.f;
+    for (int i = tid; i < D; i += 256) {
+      const float v = xr[i];
+      qf[i] = v;
+      am = fmaxf(am, fabsf(v));
+    }
+    for (int o = 32; o > 0; o >>= 1) am = fmaxf(am, __shfl_xor(am, o));
+    if (lane == 0) s_amax[wave] = am;
+    __syncthreads();
+    const float amax = fmaxf(fmaxf(s_amax[0], s_amax[1]), fmaxf(s_amax[2], s_amax[3]));
+    const bool zero = amax < 1e-30f;
+    const float qs = zero ? 0.f : div_rn(amax, 127.f);
+    const float inv = zero ? 0.f : div_rn(127.f, amax);
+    int sq = 0;
+    for (int w = tid; w < W; w += 256) {
+      uint32_t ev = 0u, od = 0u;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int d = 8 * w + u;
+        int lv = 0;
+        if (!zero && d < D) lv = (int)fmaxf(-127.f, fminf(127.f, roundf(qf[d] * inv)));
+        sq += lv;
+        const uint32_t b = (uint32_t)lv & 0xffu;
+        if (u & 1) od |= b << (8 * (u >> 1));
+        else ev |= b << (8 * (u >> 1));
+      }
+      qe[w] = (int)ev;
+      qo[w] = (int)od;
+    }
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    if (lane == 0) s_sum[wave] = sq;
+    __syncthreads();
+    const int sumq = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
+    if (tid == 192) {  // |q|^2 in element order while the other lanes take columns
+      double a = 0.0;
+      for (int d = 0; d < D; ++d) {
+        const double v = (double)qf[d];
+        a = a + v * v;
+      }
+      s_q2 = a;
+    }
+    for (int j = tid; j < stride; j += 256) {
+      float v = 0.f;
+      if (j < m) {
+        const uint32_t* col = reinterpret_cast<const uint32_t*>(codes + (long)j * bpc);
+        int acc = 0;
+        for (int w = 0; w < W; ++w) {
+          const uint32_t cw = col[w];
+          acc = __builtin_amdgcn_sdot4((int)(cw & 0x0F0F0F0Fu), qe[w], acc, false);
+          acc = __builtin_amdgcn_sdot4((int)((cw >> 4) & 0x0F0F0F0Fu), qo[w], acc, false);
+        }
+        v = (qs * lscales[j]) * (float)(acc - 7 * sumq);
+      }
+      ql[j] = v;
+      q_land[qi * (long)stride + j] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double l2 = 0.0;
+      for (int j = 0; j < m; ++j) {
+        const double v = (double)ql[j];
+        l2 = l2 + v * v;
+      }
+      const double d = s_q2 - l2;
+      q_perp[qi] = (float)sqrt(0.0 < d ? d : 0.0);
+    }
+  }
+}
+
 int grid_for(long work, int threads, int cap = 65536) {
   long g = (work + threads - 1) / threads;
   if (g < 1) g = 1;
@@ -2461,7 +2733,8 @@ hipError_t launch_encode(const CodeTables& t, const float* x, std::size_t n, std
   if (n == 0) return hipSuccess;
   const int R = t.block / 64;
   const int NX = t.dim / 64;
-  if (tiled != nullptr && alpha != nullptr && packed == nullptr && rotated == nullptr && t.dim % 64 == 0 &&
+  const bool plain = t.prerotated == 0 && t.norm_seq == 0;
+  if (plain && tiled != nullptr && alpha != nullptr && packed == nullptr && rotated == nullptr && t.dim % 64 == 0 &&
       t.perm_gather != nullptr && (R == 2 || R == 4 || R == 8 || R == 16) && (t.bits != 2 || R >= 4) &&
       NX <= 64 && (t.debug_flags & 4) == 0 &&
       row_shared_bytes(t.dim, t.shift != nullptr) + row_wave_bytes(t.dim, t.groups16, t.block) <= (size_t)kLdsBudget) {
@@ -2500,7 +2773,7 @@ hipError_t launch_encode(const CodeTables& t, const float* x, std::size_t n, std
     return hipGetLastError();
   }
   const bool fast_shape = (R == 2 && t.bits != 2) || R == 4 || R == 8 || R == 16;
-  if (tiled != nullptr && alpha != nullptr && packed == nullptr && rotated == nullptr && fast_shape &&
+  if (plain && tiled != nullptr && alpha != nullptr && packed == nullptr && rotated == nullptr && fast_shape &&
       t.perm_sign != nullptr && (t.bits != 2 || R >= 4) && enc_lds_bytes(t.dim, t.groups16) <= (size_t)kLdsBudget) {
     const size_t lds = enc_lds_bytes(t.dim, t.groups16);
     const long tiles = (long)((out_offset + n + kEncodeRows - 1) / kEncodeRows - out_offset / kEncodeRows);
@@ -2894,6 +3167,45 @@ hipError_t launch_flat_to_packed(const std::uint8_t* flat, std::size_t n, int di
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(flat_to_packed_kernel, dim3(grid_for((long)n * (dim / 8), 256)), dim3(256), 0, s, flat,
                      (long)n, dim, bits, packed);
+  return hipGetLastError();
+}
+
+// ---- landmark index ---------------------------------------------------------
+hipError_t launch_lm_yhat(const CodeTables& t, float* rows, std::size_t n, float* vnorm2, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (t.dim % 8 != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(lm_yhat_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t, rows, (long)n, vnorm2);
+  return hipGetLastError();
+}
+
+hipError_t launch_lm_project(const float* Y, std::size_t n, int dim, const float* L, int m, float* out,
+                             int stride, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (dim % 8 != 0 || stride < m) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(lm_project_kernel, dim3((unsigned)((n + 31) / 32), (unsigned)((stride + 31) / 32)), dim3(256),
+                     0, s, Y, (long)n, dim, L, m, out, stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_lm_resid(const float* rows, std::size_t n, int m, int stride, const float* vnorm2, float* rn,
+                           hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (stride % 4 != 0 || stride < m) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(lm_resid_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rows, (long)n, m, stride,
+                     vnorm2, rn);
+  return hipGetLastError();
+}
+
+hipError_t launch_lm_query(const float* q_rot, std::size_t nq, int dim, int dim_pad, const std::uint8_t* codes,
+                           const float* lscales, int m, float* q_land, int stride, float* q_perp, hipStream_t s) {
+  if (nq == 0) return hipSuccess;
+  if (dim_pad % 16 != 0 || dim_pad < dim || stride < m) return hipErrorInvalidValue;
+  const size_t lds = ((size_t)dim + (size_t)stride) * 4 + (size_t)(dim_pad / 8) * 8;
+  if (lds > (size_t)kLdsBudget) return hipErrorInvalidValue;
+  if (lds > 64 * 1024)
+    (void)hipFuncSetAttribute((const void*)lm_query_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(lm_query_kernel, dim3((unsigned)std::min<size_t>(nq, 16384)), dim3(256), lds, s, q_rot,
+                     (long)nq, dim, dim_pad, codes, lscales, m, q_land, stride, q_perp);
   return hipGetLastError();
 }
 
