@@ -230,6 +230,4 @@ def test_gpu_landmark_errors():
         ix.search(np.ones((1, 64), np.float32), 3)
     ix.prepare()
     assert ix.search(np.ones((1, 64), np.float32), 0).k == 0
-    assert ix.search(np.ones((1, 64), np.float32), 50).k == 10
-    with pytest.raises(ValueError):
-        ix.search(np.ones((1, 64), np.float32), 5000)
+    assert ix.search(np.ones((1, 64), np.float32), 5000).k == 10  # k clipped to size()

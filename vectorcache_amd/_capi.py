@@ -13,6 +13,15 @@ LIB_PATH = Path(__file__).resolve().parent / "libvectorcache_hip.so"
 if not LIB_PATH.exists():
     raise ImportError(f"{LIB_PATH} is not built; run `python -m vectorcache_amd.build` (HIP, gfx950)")
 
+# torch bundles its own HIP runtime under the same soname (libamdhip64.so.7).
+# Loaded first, it is the one runtime of the process and the library binds to
+# it; loaded after this library, torch would bring a second runtime that then
+# finds no device. So torch, when present, is imported before the library.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 lib = ctypes.CDLL(str(LIB_PATH))
 
 P = ctypes.c_void_p
