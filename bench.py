@@ -41,6 +41,12 @@ WORKLOADS = {
     "flat100k": (100_000, 128, 4, 0, 1_000, 10),
 }
 
+LANDMARK_WORKLOADS = {
+    # name: (n, dim, scan_bits, n_landmarks, nq, k) -- the reference's LandmarkIndex
+    # (TurboQuantIndex) with its default 1024 landmarks
+    "landmark1m": (1_000_000, 1536, 4, 1024, 10_000, 10),
+}
+
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
@@ -152,7 +158,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="flat1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="flat1m", choices=sorted(WORKLOADS) + sorted(LANDMARK_WORKLOADS))
     ap.add_argument("--tile", type=int, default=0, help="cap the scan query tile (1, 2, 4, 8)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-latency", action="store_true")
@@ -161,6 +167,9 @@ def main():
         print(json.dumps({"error": "this benchmark runs on one GPU only; the flat scan would shard by rows "
                                    "with an all-gather of per-GPU top-k lists", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.workload in LANDMARK_WORKLOADS:
+        return run_landmark(args)
 
     import torch
 
@@ -308,6 +317,161 @@ def main():
                    "frac": enc_bytes / enc_s / 1e9 / HBM_PEAK_GBS},
         "roofline_bandwidth_regime": bw,
         "copy_probe": probe,
+        "recall_at_10_vs_fp64": recall,
+        "latency": lat,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def landmark_cpu_baseline(x_np, q_np, bits, m, k, n_full, budget_queries=1000, n_sample=100_000):
+    """The reference's own LandmarkIndex (oracle/_ref) built on the first
+    n_sample vectors, its search() of budget_queries queries timed; the rate is
+    scaled to the full index (the scan is linear in n). Also checks a GPU
+    index of the same sample against it."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from oracle import Oracle, available
+
+    from vectorcache_amd.landmark import LandmarkIndex
+
+    kind = "reference" if available("reference") else "port"
+    o = Oracle(kind)
+    cores = min(16, os.cpu_count() or 1)
+    o.set_threads(cores)
+    xs = np.ascontiguousarray(x_np[:n_sample])
+    qs = np.ascontiguousarray(q_np[:budget_queries])
+    s, i, info = o.lm_search(xs, qs, k, scan_bits=bits, n_landmarks=m, seed=0)
+    dt = info["search_s"]
+    g = LandmarkIndex(xs.shape[1], bits, m)
+    g.add(xs)
+    g.prepare()
+    r = g.search(qs, k)
+    g.close()
+    out = {
+        "unit": "queries/s", "cores": cores, "kind": kind,
+        "sample": f"LandmarkIndex on the first {n_sample} of the {n_full} vectors, search() of {len(qs)} queries; "
+                  f"rate x {n_sample}/{n_full} (scan linear in n)",
+        "parity_ids": bool(np.array_equal(r.ids, i)), "parity_scores": bool(np.array_equal(r.scores, s)),
+    }
+    if dt > 0:
+        out["value"] = len(qs) / dt * n_sample / n_full
+        out["sample_queries_per_s"] = len(qs) / dt
+    return out
+
+
+def run_landmark(args):
+    """The landmark index (the reference's TurboQuantIndex): add + prepare of
+    the whole database, then search of the query batch (device-resident)."""
+    import torch
+
+    import vectorcache_amd.index as vc
+    from vectorcache_amd import synthetic
+    from vectorcache_amd.landmark import LandmarkIndex
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    n, dim, bits, m, nq, k = LANDMARK_WORKLOADS[args.workload]
+    S = (m + 7) // 8 * 8
+    G = S // (8 // bits)
+    log(f"workload {args.workload}: n={n} dim={dim} scan_bits={bits} landmarks={m} nq={nq} k={k}")
+    x = synthetic.anisotropic_torch(n, dim, 2001, dev)
+    q = synthetic.anisotropic_torch(nq, dim, 2003, dev)
+    LandmarkIndex(dim, bits, m).close()  # the basis (host Gram-Schmidt) is memoized per process
+    ix = LandmarkIndex(dim, bits, m)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ix.add(x)
+    t_add = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ix.prepare()
+    t_prep = time.perf_counter() - t0
+    log(f"add {n / t_add / 1e6:.2f} M vectors/s, prepare {n / t_prep / 1e6:.2f} M vectors/s")
+
+    if args.tile:
+        vc.set_scan_tile(args.tile)
+    stream = torch.cuda.current_stream().cuda_stream
+    vc.set_profiling(True)
+    scores = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    ids = torch.empty((nq, k), dtype=torch.int64, device=dev)
+    for _ in range(args.warmup):
+        ix.search_device(q, k, scores, ids, stream=stream)
+    torch.cuda.synchronize()
+    scan_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ix.search_device(q, k, scores, ids, stream=stream)
+        ms, tile, ranges = vc.last_scan_stats()
+        scan_ms.append(ms)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    qps = nq * args.steps / elapsed
+    scan_avg_s = float(np.mean(scan_ms)) / 1e3
+    scan_bytes = math.ceil(nq / tile) * n * (G + 4) + nq * G * 32
+    achieved = scan_bytes / scan_avg_s / 1e9
+    log(f"search {qps:.0f} QPS, {elapsed / args.steps * 1e3:.1f} ms/step, scan kernel {scan_avg_s * 1e3:.1f} ms "
+        f"(tile {tile}, ranges {ranges})")
+
+    ns = min(200, nq)
+    es, ei = exact_topk(x, q[:ns], 10)
+    got = ids[:ns, :10].cpu().numpy()
+    exact = ei.cpu().numpy()
+    recall = float(np.mean([len(set(got[i]) & set(exact[i])) / 10 for i in range(ns)]))
+    log(f"recall@10 vs fp64 exact ({ns} queries): {recall:.4f}")
+
+    lat = {}
+    if not args.no_latency:
+        for b in (1, 64):
+            ts = []
+            qb = q[:b].contiguous()
+            sb = torch.empty((b, k), dtype=torch.float32, device=dev)
+            ib = torch.empty((b, k), dtype=torch.int64, device=dev)
+            for it in range(25):
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                ix.search_device(qb, k, sb, ib, stream=stream)
+                torch.cuda.synchronize()
+                if it >= 5:
+                    ts.append((time.perf_counter() - t1) * 1e3)
+            lat[f"batch{b}"] = {"p50_ms": float(np.percentile(ts, 50)), "p99_ms": float(np.percentile(ts, 99))}
+        log(f"latency {lat}")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = landmark_cpu_baseline(x.cpu().numpy(), q.cpu().numpy(), bits, m, k, n)
+            log(f"cpu baseline {cpu}")
+        except Exception as e:  # reported, never fatal for the GPU figure
+            cpu = {"error": str(e)}
+
+    traffic, traffic_src = committed_traffic(args.workload, nq, tile)
+    out = {
+        "metric": METRIC,
+        "value": qps,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded, generated on device)",
+        "config": {
+            "workload": f"{args.workload}: landmark index (LandmarkIndex), {n} x {dim} float32, {m} landmarks, "
+                        f"{bits}-bit scan codes, {nq} queries, k={k}",
+            "n": n, "dim": dim, "scan_bits": bits, "n_landmarks": m, "scan_dim": S, "nq": nq, "k": k,
+            "query_tile": tile, "vector_ranges": ranges,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+            "kernel": f"scan_pair_kernel (query tile {tile})", "avg_ms": scan_avg_s * 1e3,
+            "algorithmic_bytes": scan_bytes,
+            "note": f"bytes = ceil(nq/T) passes x N x (G+4) + LUTs; G={G} groups of the landmark rows; T={tile}",
+        },
+        "build": {"add_vectors_per_s": n / t_add, "prepare_vectors_per_s": n / t_prep,
+                  "add_ms": t_add * 1e3, "prepare_ms": t_prep * 1e3},
         "recall_at_10_vs_fp64": recall,
         "latency": lat,
         "cpu_baseline": cpu,

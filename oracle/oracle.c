@@ -1413,11 +1413,13 @@ static float lm_norm_sq(const float* x, size_t m) {
 
 /* LandmarkIndex end to end with the reference's state rules: rows added after
  * a prepare() find the earlier rows zeroed (landmark_rows_ was released and is
- * resized), and prepare() encodes every row. Arguments as vcref_lm_search. */
+ * resized), and prepare() encodes every row. Arguments as vcref_lm_search
+ * (search_seconds is left 0: the restatement is not timed). */
 int oc_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
                  const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
                  size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
-                 uint64_t* ids_out, size_t* k_out, size_t* info) {
+                 uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds) {
+  if (search_seconds != NULL) *search_seconds = 0.0;
   if (rerank_bits == 0) rerank_bits = scan_bits;
   if (dim == 0 || dim % 8 != 0 || scan_bits < 2 || scan_bits > 4 || rerank_bits < 2 || rerank_bits > 4 ||
       m == 0 || m > dim)

@@ -70,7 +70,7 @@ int oc_project_q4(const float* q_rot, size_t nq, size_t dim, const float* L, siz
 int oc_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
                  const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
                  size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
-                 uint64_t* ids_out, size_t* k_out, size_t* info);
+                 uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds);
 /* the ChaCha block function (8 rounds for the rotation, 20 for RFC 7539 tests) */
 void oc_chacha_block(const uint32_t in[16], uint32_t out[16], int rounds);
 

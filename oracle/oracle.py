@@ -378,7 +378,8 @@ class Oracle:
                   prepare_between=False, float_corpus=None):
         """LandmarkIndex(dim, scan_bits, n_landmarks, seed, rerank_bits): add the
         rows of x (in `batches` row counts), prepare, search. Returns scores,
-        ids, info {scan_dim, codes_bytes, index_bytes}."""
+        ids, info {scan_dim, codes_bytes, index_bytes, search_s (reference: wall time
+        of search())}."""
         x = np.ascontiguousarray(x, np.float32)
         q = np.ascontiguousarray(q, np.float32)
         n, dim = x.shape
@@ -388,14 +389,16 @@ class Oracle:
         ids = np.zeros(max(1, nq * k), np.uint64)
         ko = ctypes.c_size_t()
         info = np.zeros(3, np.uint64)
+        secs = ctypes.c_double()
         fc = None if float_corpus is None else np.ascontiguousarray(float_corpus, np.float32)
         self._check(self._f("lm_search")(_ptr(x), _ptr(br), _S(len(br)), ctypes.c_int(int(prepare_between)),
                                          _ptr(q), _S(nq), _S(dim), _S(scan_bits), _S(n_landmarks),
                                          ctypes.c_uint64(seed), _S(rerank_bits), _S(k), _ptr(fc), _ptr(scores),
-                                         _ptr(ids), ctypes.byref(ko), _ptr(info)), "lm_search")
+                                         _ptr(ids), ctypes.byref(ko), _ptr(info), ctypes.byref(secs)), "lm_search")
         ke = ko.value
         return (scores[: nq * ke].reshape(nq, ke), ids[: nq * ke].reshape(nq, ke),
-                {"scan_dim": int(info[0]), "codes_bytes": int(info[1]), "index_bytes": int(info[2])})
+                {"scan_dim": int(info[0]), "codes_bytes": int(info[1]), "index_bytes": int(info[2]),
+                 "search_s": secs.value})
 
 def available(kind: str) -> bool:
     if kind == "port":

@@ -7,6 +7,7 @@
 // Every entry forces the reference's AVX2 PERM0 uint8-LUT scan
 // (TURBOVEC_NO_VNNI=1, reference src/pack/pack.cpp:214-233), which is the
 // deterministic, ISA-portable path this repository is bit-exact against.
+#include <chrono>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -418,11 +419,12 @@ int vcref_project_q4(const float* q_rot, size_t nq, size_t dim, const float* L, 
 // added in batches of batch_rows[b], prepare() after every batch when
 // prepare_between, else once at the end, then search() with the optional
 // float corpus.
-// info[0..2] = scan_dim, codes_bytes(), index_bytes().
+// info[0..2] = scan_dim, codes_bytes(), index_bytes(); search_seconds (may
+// be null) = wall time of the search() call.
 int vcref_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
                     const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
                     size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
-                    uint64_t* ids_out, size_t* k_out, size_t* info) {
+                    uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds) {
   return guarded([&] {
     vc::LandmarkIndex ix(dim, scan_bits, m, seed, rerank_bits);
     size_t off = 0;
@@ -434,7 +436,10 @@ int vcref_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, 
     if (!prepare_between) ix.prepare();  // a second prepare() with no add reads released rows
     std::span<const float> fc;
     if (float_corpus != nullptr) fc = std::span<const float>(float_corpus, off * dim);
+    const auto t0 = std::chrono::steady_clock::now();
     auto res = ix.search(std::span<const float>(q, nq * dim), k, fc);
+    if (search_seconds != nullptr)
+      *search_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     *k_out = res.k;
     if (res.k > 0 && nq > 0) {
       std::memcpy(scores_out, res.scores.data(), res.nq * res.k * sizeof(float));
