@@ -3,7 +3,9 @@
 // (reference tests/test_index.cpp:11-113): ids in range, rows sorted, a
 // self-query returns its own id first, batched search equals per-query
 // search, plus the reference's error behaviour (std::invalid_argument,
-// std::runtime_error) and merge_search_results semantics.
+// std::runtime_error) and merge_search_results semantics; and the same
+// tests for vectorcache::LandmarkIndex, which is what they exercise in the
+// reference (its TurboQuantIndex aliases LandmarkIndex).
 // Prints "OK <checks>" and exits 0 on success.
 #include <cmath>
 #include <cstdio>
@@ -17,6 +19,7 @@
 #include <vector>
 
 #include "vectorcache/datasets/npy.hpp"
+#include "vectorcache/index.hpp"
 #include "vectorcache/turbo_quant_index.hpp"
 
 namespace {
@@ -50,6 +53,72 @@ std::vector<float> gaussian(std::size_t n, std::size_t dim, unsigned seed) {
   std::vector<float> v(n * dim);
   for (auto& x : v) x = nd(rng);
   return v;
+}
+// The reference's LandmarkIndex tests (tests/test_index.cpp:11-113).
+void landmark_checks() {
+  using vectorcache::LandmarkIndex;
+  {  // AddAndSearchReturnsIds
+    const std::size_t dim = 64, n = 256, nq = 4, k = 5;
+    const auto db = gaussian(n, dim, 42);
+    const auto queries = gaussian(nq, dim, 43);
+    LandmarkIndex index(dim, 4, 48);
+    expect(index.scan_dim() == 48 && index.n_landmarks() == 48 && index.rerank_bits() == 4, "landmark accessors");
+    index.add(db);
+    index.prepare();
+    const auto res = index.search(queries, k);
+    expect(res.nq == nq && res.k == k, "landmark result shape");
+    for (std::size_t qi = 0; qi < nq; ++qi)
+      for (std::size_t j = 0; j < k; ++j) {
+        expect(res.ids[qi * k + j] < n, "landmark ids in range");
+        if (j > 0) expect(res.scores[qi * k + j - 1] >= res.scores[qi * k + j], "landmark rows sorted");
+      }
+  }
+  {  // SelfQueryHighScore
+    const std::size_t dim = 128, n = 64;
+    const auto db = gaussian(n, dim, 7);
+    LandmarkIndex index(dim, 4, 96);
+    index.add(db);
+    index.prepare();
+    const auto res = index.search(std::span<const float>(db.data(), dim), 1);
+    expect(res.k == 1 && res.ids[0] == 0 && res.scores[0] > 0.7f, "landmark self-query");
+  }
+  {  // FloatRescoreSelfQuery
+    const std::size_t dim = 128, n = 256, k = 16;
+    auto db = gaussian(n, dim, 11);
+    for (std::size_t i = 0; i < n; ++i) {
+      double s = 0.0;
+      float* row = db.data() + i * dim;
+      for (std::size_t d = 0; d < dim; ++d) s += static_cast<double>(row[d]) * row[d];
+      const float inv = s > 0.0 ? static_cast<float>(1.0 / std::sqrt(s)) : 0.f;
+      for (std::size_t d = 0; d < dim; ++d) row[d] *= inv;
+    }
+    LandmarkIndex index(dim, 4, 64);
+    index.add(db);
+    index.prepare();
+    const auto res = index.search(std::span<const float>(db.data(), dim), k, db);
+    expect(res.k == k && res.ids[0] == 0 && std::fabs(res.scores[0] - 1.0f) < 1e-4f, "landmark float re-rank");
+    for (std::size_t j = 1; j < k; ++j) expect(res.scores[j - 1] >= res.scores[j], "landmark re-ranked sorted");
+  }
+  {  // MultiQueryMatchesSingle
+    const std::size_t dim = 64, n = 512, nq = 8, k = 8;
+    const auto db = gaussian(n, dim, 123);
+    const auto queries = gaussian(nq, dim, 124);
+    LandmarkIndex index(dim, 4, 48);
+    index.add(db);
+    index.prepare();
+    const auto batch = index.search(queries, k);
+    for (std::size_t qi = 0; qi < nq; ++qi) {
+      const auto one = index.search(std::span<const float>(queries.data() + qi * dim, dim), k);
+      for (std::size_t j = 0; j < k; ++j)
+        expect(batch.ids[qi * k + j] == one.ids[j] && batch.scores[qi * k + j] == one.scores[j],
+               "landmark batch equals single");
+    }
+  }
+  expect_throw<std::invalid_argument>([] { LandmarkIndex bad(64, 4, 65); }, "landmarks > dim");
+  expect_throw<std::invalid_argument>([] { LandmarkIndex bad(64, 4, 16, 0, 5); }, "rerank bits");
+  LandmarkIndex idx(64, 4, 16);
+  idx.add(gaussian(10, 64, 5));
+  expect_throw<std::runtime_error>([&] { (void)idx.search(gaussian(1, 64, 6), 3); }, "landmark search before prepare");
 }
 }  // namespace
 
@@ -168,6 +237,7 @@ int main() {
                                      "npy missing file");
     std::filesystem::remove_all(p.parent_path());
   }
+  landmark_checks();
   std::printf("OK %d\n", g_checks);
   return 0;
 }
