@@ -1273,7 +1273,7 @@ struct vc_lm_index {
   vchip::CodeTables t_add{};   // (dim, rerank_bits): in-order norm + rotation, y-hat codebook
   vchip::CodeTables t_query{}; // (dim): query norm + rotation
   vchip::CodeTables t_scan{};  // (scan_dim, scan_bits): landmark rows, prerotated
-  DevBuf basis;                // L | codes | scales
+  DevBuf basis;                // L | codes as words [dim_pad / 8][m] | scales
   size_t o_codes = 0, o_scales = 0;
   DevBuf rows;                 // n x scan_dim while live
   bool rows_live = true;
@@ -1288,7 +1288,10 @@ struct vc_lm_index {
 
 namespace {
 
-constexpr size_t kLmAddChunk = 16384;  // rows per add() pass (rotated rows staged in HBM)
+// rows per add() pass (rotated rows staged in HBM: 768 MiB at dim 1536); the
+// y-hat kernel runs one thread per row, so a pass needs many rows to keep
+// enough wavefronts in flight
+constexpr size_t kLmAddChunk = 131072;
 
 void lm_add(vc_lm_index* ix, const float* v, size_t n_new, bool host) {
   if (n_new == 0) return;
@@ -1339,7 +1342,7 @@ void lm_query_rows(vc_lm_index* ix, const float* q_dev, size_t nq, float** q_lan
   *q_land = ix->qwork.as<float>(oland);
   *q_perp = ix->qwork.as<float>(operp);
   hip_check(vchip::launch_encode(ix->t_query, q_dev, nq, 0, nullptr, nullptr, rot, nullptr, s), "encode(rotate)");
-  hip_check(vchip::launch_lm_query(rot, nq, int(ix->dim), int(ix->dim_pad), ix->basis.as<uint8_t>(ix->o_codes),
+  hip_check(vchip::launch_lm_query(rot, nq, int(ix->dim), int(ix->dim_pad), ix->basis.as<uint32_t>(ix->o_codes),
                                    ix->basis.as<float>(ix->o_scales), int(ix->m), *q_land, int(ix->scan_dim),
                                    *q_perp, s),
             "lm_query");
@@ -1397,13 +1400,18 @@ vc_status vc_lm_index_create(size_t dim, size_t scan_bits, size_t n_landmarks, u
       ix->t_scan.norm_seq = 1;
       const auto& L = vchip::ortho_landmarks(dim, n_landmarks, landmark_seed);
       const auto q = vchip::quantize_landmarks(L.data(), dim, n_landmarks);
+      // column j's u32 word w (dims 8w .. 8w + 7) at w * m + j
+      const size_t W = ix->dim_pad / 8, bpc = ix->dim_pad / 2;
+      std::vector<uint32_t> codes_t(W * n_landmarks);
+      for (size_t j = 0; j < n_landmarks; ++j)
+        for (size_t w = 0; w < W; ++w) std::memcpy(&codes_t[w * n_landmarks + j], &q.codes[j * bpc + 4 * w], 4);
       Carve c;
       const size_t oL = c.take(L.size() * 4);
-      ix->o_codes = c.take(q.codes.size());
+      ix->o_codes = c.take(codes_t.size() * 4);
       ix->o_scales = c.take(q.scales.size() * 4);
       ix->basis.reserve(c.off, ix->stream);
       hip_check(hipMemcpyAsync(ix->basis.as<float>(oL), L.data(), L.size() * 4, hipMemcpyHostToDevice, ix->stream), "copy");
-      hip_check(hipMemcpyAsync(ix->basis.as<uint8_t>(ix->o_codes), q.codes.data(), q.codes.size(),
+      hip_check(hipMemcpyAsync(ix->basis.as<uint32_t>(ix->o_codes), codes_t.data(), codes_t.size() * 4,
                                hipMemcpyHostToDevice, ix->stream), "copy");
       hip_check(hipMemcpyAsync(ix->basis.as<float>(ix->o_scales), q.scales.data(), q.scales.size() * 4,
                                hipMemcpyHostToDevice, ix->stream), "copy");

@@ -2492,11 +2492,12 @@ __global__ __launch_bounds__(256) void lm_yhat_kernel(CodeTables t, float* __res
     return k;
   };
   double dot = 0.0, cc = 0.0;
-  for (int i = 0; i < D; i += 4) {
-    const float4 v = *reinterpret_cast<const float4*>(row + i);
-    const float xs[4] = {v.x, v.y, v.z, v.w};
+  for (int i = 0; i < D; i += 8) {  // D % 8 == 0: two 16-byte loads in flight per step
+    const float4 a = *reinterpret_cast<const float4*>(row + i);
+    const float4 b = *reinterpret_cast<const float4*>(row + i + 4);
+    const float xs[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const double c = (double)cen[level(xs[u])];
       dot = dot + (double)xs[u] * c;
       cc = cc + c * c;
@@ -2504,16 +2505,18 @@ __global__ __launch_bounds__(256) void lm_yhat_kernel(CodeTables t, float* __res
   }
   const float alpha = cc > 1e-12 ? (float)(dot / cc) : 0.f;
   float vn = 0.f;
-  for (int i = 0; i < D; i += 4) {
-    const float4 v = *reinterpret_cast<const float4*>(row + i);
-    float xs[4] = {v.x, v.y, v.z, v.w};
+  for (int i = 0; i < D; i += 8) {
+    const float4 a = *reinterpret_cast<const float4*>(row + i);
+    const float4 b = *reinterpret_cast<const float4*>(row + i + 4);
+    float xs[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const float vi = alpha * cen[level(xs[u])];
       xs[u] = vi;
       vn = vn + vi * vi;
     }
     *reinterpret_cast<float4*>(row + i) = make_float4(xs[0], xs[1], xs[2], xs[3]);
+    *reinterpret_cast<float4*>(row + i + 4) = make_float4(xs[4], xs[5], xs[6], xs[7]);
   }
   vnorm2[r] = vn;
 }
@@ -2628,8 +2631,11 @@ __global__ __launch_bounds__(256) void lm_resid_kernel(const float* __restrict__
 // basis columns (nibble u = level + 7: dot = sum q u - 7 sum q, four bytes
 // per v_dot4), q_land[j] = (q_scale * scale_j) * (float) dot; q_perp =
 // (float) sqrt(max(0, |q|^2 - |q_land|^2)), both sums sequential in double.
+// codes_t: the basis as u32 words [dim_pad / 8][m] (word w of column j at
+// w * m + j), so the lanes of a wavefront, one column each, read consecutive
+// words.
 __global__ __launch_bounds__(256) void lm_query_kernel(const float* __restrict__ q_rot, long nq, int D,
-                                                       int dim_pad, const uint8_t* __restrict__ codes,
+                                                       int dim_pad, const uint32_t* __restrict__ codes_t,
                                                        const float* __restrict__ lscales, int m,
                                                        float* __restrict__ q_land, int stride,
                                                        float* __restrict__ q_perp) {
@@ -2642,7 +2648,7 @@ __global__ __launch_bounds__(256) void lm_query_kernel(const float* __restrict__
   __shared__ int s_sum[4];
   __shared__ double s_q2;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int W = dim_pad / 8, bpc = dim_pad / 2;
+  const int W = dim_pad / 8;
   for (long qi = blockIdx.x; qi < nq; qi += gridDim.x) {
     __syncthreads();
     const float* xr = q_rot + qi * (long)D;
@@ -2690,10 +2696,9 @@ __global__ __launch_bounds__(256) void lm_query_kernel(const float* __restrict__
     for (int j = tid; j < stride; j += 256) {
       float v = 0.f;
       if (j < m) {
-        const uint32_t* col = reinterpret_cast<const uint32_t*>(codes + (long)j * bpc);
         int acc = 0;
         for (int w = 0; w < W; ++w) {
-          const uint32_t cw = col[w];
+          const uint32_t cw = codes_t[(long)w * m + j];
           acc = __builtin_amdgcn_sdot4((int)(cw & 0x0F0F0F0Fu), qe[w], acc, false);
           acc = __builtin_amdgcn_sdot4((int)((cw >> 4) & 0x0F0F0F0Fu), qo[w], acc, false);
         }
@@ -3196,7 +3201,7 @@ hipError_t launch_lm_resid(const float* rows, std::size_t n, int m, int stride, 
   return hipGetLastError();
 }
 
-hipError_t launch_lm_query(const float* q_rot, std::size_t nq, int dim, int dim_pad, const std::uint8_t* codes,
+hipError_t launch_lm_query(const float* q_rot, std::size_t nq, int dim, int dim_pad, const std::uint32_t* codes_t,
                            const float* lscales, int m, float* q_land, int stride, float* q_perp, hipStream_t s) {
   if (nq == 0) return hipSuccess;
   if (dim_pad % 16 != 0 || dim_pad < dim || stride < m) return hipErrorInvalidValue;
@@ -3205,7 +3210,7 @@ hipError_t launch_lm_query(const float* q_rot, std::size_t nq, int dim, int dim_
   if (lds > 64 * 1024)
     (void)hipFuncSetAttribute((const void*)lm_query_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(lm_query_kernel, dim3((unsigned)std::min<size_t>(nq, 16384)), dim3(256), lds, s, q_rot,
-                     (long)nq, dim, dim_pad, codes, lscales, m, q_land, stride, q_perp);
+                     (long)nq, dim, dim_pad, codes_t, lscales, m, q_land, stride, q_perp);
   return hipGetLastError();
 }
 
