@@ -232,3 +232,31 @@ def test_gpu_landmark_errors():
     ix.prepare()
     assert ix.search(np.ones((1, 64), np.float32), 0).k == 0
     assert ix.search(np.ones((1, 64), np.float32), 5000).k == 10  # k clipped to size()
+
+
+@pytest.mark.gpu
+def test_gpu_landmark_edge_cases(oracle):
+    """Empty index, fewer vectors than k, no queries, and add after prepare
+    (the index must be prepared again; earlier rows are zeroed as in the
+    reference) against the oracle."""
+    from vectorcache_amd._capi import IndexStateError
+    from vectorcache_amd.landmark import LandmarkIndex
+
+    ix = LandmarkIndex(64, 4, 20)
+    ix.prepare()
+    assert ix.search(np.ones((3, 64), np.float32), 10).scores.shape == (3, 0)
+    x, q = landmark_inputs(40, 3, 64)
+    ix.add(x[:5])
+    ix.prepare()
+    res = ix.search(q, 10)
+    s, i, _ = oracle.lm_search(x[:5], q, 10, n_landmarks=20, seed=0)
+    assert res.k == 5 and scores_equal(res.scores, s) and np.array_equal(res.ids, i)
+    assert ix.search(np.zeros((0, 64), np.float32), 4).scores.shape == (0, 4)
+    ix.add(x[5:40])
+    with pytest.raises(IndexStateError):
+        ix.search(q, 3)
+    ix.prepare()
+    res = ix.search(q, 7)
+    s, i, _ = oracle.lm_search(x, q, 7, n_landmarks=20, seed=0, batches=(5, 35), prepare_between=True)
+    assert scores_equal(res.scores, s) and np.array_equal(res.ids, i)
+    assert ix.size() == 40
