@@ -119,6 +119,18 @@ def cpu_baseline(ix, q_np, gpu_ids, gpu_scores, bits, shift, scale, budget_s=12.
     }
 
 
+def host_buffer_qps(ix, q, k, reps=2):
+    """Queries per second through the host-buffer boundary (search(span)):
+    query upload, search and result download included; never the reported
+    value, which keeps inputs resident in HBM."""
+    qh = q.cpu().numpy()
+    ix.search(qh, k)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ix.search(qh, k)
+    return qh.shape[0] * reps / (time.perf_counter() - t0)
+
+
 def copy_probe(dev, nbytes=1 << 30, reps=5):
     """Device-to-device copy of a known size: the achievable HBM ceiling
     (read + write) and the reference size for scaling rocprofv3's
@@ -276,6 +288,8 @@ def main():
 
     probe = copy_probe(dev)
     log(f"copy probe {probe}")
+    host_qps = host_buffer_qps(ix, q, k)
+    log(f"host-buffer search {host_qps:.0f} QPS (PCIe-inclusive)")
 
     # ---- CPU baseline (reference build) ----
     cpu = None
@@ -316,6 +330,7 @@ def main():
         "encode": {"vectors_per_s": n / enc_s, "ms": enc_s * 1e3, "gb_per_s": enc_bytes / enc_s / 1e9,
                    "frac": enc_bytes / enc_s / 1e9 / HBM_PEAK_GBS},
         "roofline_bandwidth_regime": bw,
+        "host_buffer_qps": host_qps,
         "copy_probe": probe,
         "recall_at_10_vs_fp64": recall,
         "latency": lat,
@@ -435,6 +450,9 @@ def run_landmark(args):
             lat[f"batch{b}"] = {"p50_ms": float(np.percentile(ts, 50)), "p99_ms": float(np.percentile(ts, 99))}
         log(f"latency {lat}")
 
+    host_qps = host_buffer_qps(ix, q, k)
+    log(f"host-buffer search {host_qps:.0f} QPS (PCIe-inclusive)")
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -474,6 +492,7 @@ def run_landmark(args):
                   "add_ms": t_add * 1e3, "prepare_ms": t_prep * 1e3},
         "recall_at_10_vs_fp64": recall,
         "latency": lat,
+        "host_buffer_qps": host_qps,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
