@@ -2633,8 +2633,11 @@ __global__ __launch_bounds__(256) void lm_resid_kernel(const float* __restrict__
 // (float) sqrt(max(0, |q|^2 - |q_land|^2)), both sums sequential in double.
 // codes_t: the basis as u32 words [dim_pad / 8][m] (word w of column j at
 // w * m + j), so the lanes of a wavefront, one column each, read consecutive
-// words.
-__global__ __launch_bounds__(256) void lm_query_kernel(const float* __restrict__ q_rot, long nq, int D,
+// words. 1024 threads, one column each for the default 1024 landmarks, with
+// the word loop unrolled so that each lane keeps several loads in flight (a
+// single query is latency-bound otherwise).
+constexpr int kLmQueryThreads = 1024;
+__global__ __launch_bounds__(kLmQueryThreads) void lm_query_kernel(const float* __restrict__ q_rot, long nq, int D,
                                                        int dim_pad, const uint32_t* __restrict__ codes_t,
                                                        const float* __restrict__ lscales, int m,
                                                        float* __restrict__ q_land, int stride,
@@ -2644,8 +2647,9 @@ __global__ __launch_bounds__(256) void lm_query_kernel(const float* __restrict__
   float* ql = qf + D;                          // stride
   int* qe = reinterpret_cast<int*>(ql + stride);  // dims 8w + {0, 2, 4, 6} as bytes
   int* qo = qe + dim_pad / 8;                     // dims 8w + {1, 3, 5, 7}
-  __shared__ float s_amax[4];
-  __shared__ int s_sum[4];
+  constexpr int kWaves = kLmQueryThreads / 64;
+  __shared__ float s_amax[kWaves];
+  __shared__ int s_sum[kWaves];
   __shared__ double s_q2;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int W = dim_pad / 8;
@@ -2653,7 +2657,7 @@ __global__ __launch_bounds__(256) void lm_query_kernel(const float* __restrict__
     __syncthreads();
     const float* xr = q_rot + qi * (long)D;
     float am = 0.f;
-    for (int i = tid; i < D; i += 256) {
+    for (int i = tid; i < D; i += kLmQueryThreads) {
       const float v = xr[i];
       qf[i] = v;
       am = fmaxf(am, fabsf(v));
@@ -2661,12 +2665,14 @@ __global__ __launch_bounds__(256) void lm_query_kernel(const float* __restrict__
     for (int o = 32; o > 0; o >>= 1) am = fmaxf(am, __shfl_xor(am, o));
     if (lane == 0) s_amax[wave] = am;
     __syncthreads();
-    const float amax = fmaxf(fmaxf(s_amax[0], s_amax[1]), fmaxf(s_amax[2], s_amax[3]));
+    float amax = s_amax[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) amax = fmaxf(amax, s_amax[w]);
     const bool zero = amax < 1e-30f;
     const float qs = zero ? 0.f : div_rn(amax, 127.f);
     const float inv = zero ? 0.f : div_rn(127.f, amax);
     int sq = 0;
-    for (int w = tid; w < W; w += 256) {
+    for (int w = tid; w < W; w += kLmQueryThreads) {
       uint32_t ev = 0u, od = 0u;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -2684,8 +2690,10 @@ __global__ __launch_bounds__(256) void lm_query_kernel(const float* __restrict__
     for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
     if (lane == 0) s_sum[wave] = sq;
     __syncthreads();
-    const int sumq = s_sum[0] + s_sum[1] + s_sum[2] + s_sum[3];
-    if (tid == 192) {  // |q|^2 in element order while the other lanes take columns
+    int sumq = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) sumq += s_sum[w];
+    if (tid == kLmQueryThreads - 64) {  // |q|^2 in element order while the other lanes take columns
       double a = 0.0;
       for (int d = 0; d < D; ++d) {
         const double v = (double)qf[d];
@@ -2693,10 +2701,11 @@ __global__ __launch_bounds__(256) void lm_query_kernel(const float* __restrict__
       }
       s_q2 = a;
     }
-    for (int j = tid; j < stride; j += 256) {
+    for (int j = tid; j < stride; j += kLmQueryThreads) {
       float v = 0.f;
       if (j < m) {
         int acc = 0;
+#pragma unroll 8
         for (int w = 0; w < W; ++w) {
           const uint32_t cw = codes_t[(long)w * m + j];
           acc = __builtin_amdgcn_sdot4((int)(cw & 0x0F0F0F0Fu), qe[w], acc, false);
@@ -3209,7 +3218,7 @@ hipError_t launch_lm_query(const float* q_rot, std::size_t nq, int dim, int dim_
   if (lds > (size_t)kLdsBudget) return hipErrorInvalidValue;
   if (lds > 64 * 1024)
     (void)hipFuncSetAttribute((const void*)lm_query_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(lm_query_kernel, dim3((unsigned)std::min<size_t>(nq, 16384)), dim3(256), lds, s, q_rot,
+  hipLaunchKernelGGL(lm_query_kernel, dim3((unsigned)std::min<size_t>(nq, 16384)), dim3(kLmQueryThreads), lds, s, q_rot,
                      (long)nq, dim, dim_pad, codes_t, lscales, m, q_land, stride, q_perp);
   return hipGetLastError();
 }
