@@ -452,6 +452,8 @@ def run_landmark(args):
 
     host_qps = host_buffer_qps(ix, q, k)
     log(f"host-buffer search {host_qps:.0f} QPS (PCIe-inclusive)")
+    probe = copy_probe(dev)
+    log(f"copy probe {probe}")
 
     cpu = None
     if not args.no_cpu:
@@ -493,6 +495,7 @@ def run_landmark(args):
         "recall_at_10_vs_fp64": recall,
         "latency": lat,
         "host_buffer_qps": host_qps,
+        "copy_probe": probe,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
