@@ -1871,6 +1871,46 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void scan_pair_ker
 // ps / pi: the query's lists [R][k]; pc: their lengths [R].
 constexpr int kMaxListsPerLane = 8;  // n_ranges <= 512
 
+// Best (score desc, id asc) entry of a full wavefront, the same on every lane:
+// an all-reduce inside each row of 16 lanes through DPP (quad xor 1, quad xor
+// 2, half-row mirror, row mirror), then the four row winners through
+// readlane. The order is total, so any reduction tree gives the same winner.
+template <int kCtrl>
+__device__ __forceinline__ void dpp_best_step(bool& h, float& s, uint32_t& i) {
+  const int oh = __builtin_amdgcn_update_dpp(0, (int)h, kCtrl, 0xF, 0xF, false);
+  const float os = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s), kCtrl, 0xF, 0xF, false));
+  const uint32_t oi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)i, kCtrl, 0xF, 0xF, false);
+  if (oh && (!h || better(os, oi, s, i))) {
+    h = true;
+    s = os;
+    i = oi;
+  }
+}
+
+__device__ __forceinline__ void wave_best(bool& h, float& s, uint32_t& i) {
+  dpp_best_step<0xB1>(h, s, i);   // quad_perm [1, 0, 3, 2]
+  dpp_best_step<0x4E>(h, s, i);   // quad_perm [2, 3, 0, 1]
+  dpp_best_step<0x141>(h, s, i);  // row_half_mirror
+  dpp_best_step<0x140>(h, s, i);  // row_mirror
+  bool bh = false;
+  float bs = 0.f;
+  uint32_t bi = 0u;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const bool rh = __builtin_amdgcn_readlane((int)h, 16 * r) != 0;
+    const float rs = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s), 16 * r));
+    const uint32_t ri = (uint32_t)__builtin_amdgcn_readlane((int)i, 16 * r);
+    if (rh && (!bh || better(rs, ri, bs, bi))) {
+      bh = true;
+      bs = rs;
+      bi = ri;
+    }
+  }
+  h = bh;
+  s = bs;
+  i = bi;
+}
+
 __device__ void merge_query(const float* __restrict__ ps, const uint32_t* __restrict__ pi,
                             const int* __restrict__ pc, int R, int k, int k_out,
                             const uint64_t* __restrict__ id_map, float* __restrict__ out_s,
@@ -1903,12 +1943,7 @@ __device__ void merge_query(const float* __restrict__ ps, const uint32_t* __rest
     bool wh = have;
     float ws = bs;
     uint32_t wi = bi;
-    for (int o = 32; o > 0; o >>= 1) {
-      const int oh = __shfl_xor((int)wh, o);
-      const float os = __shfl_xor(ws, o);
-      const uint32_t oi = (uint32_t)__shfl_xor((int)wi, o);
-      if (oh && (!wh || better(os, oi, ws, wi))) { wh = true; ws = os; wi = oi; }
-    }
+    wave_best(wh, ws, wi);
     if (!wh) break;  // the same on every lane
     if (have && bs == ws && bi == wi) {
 #pragma unroll
@@ -1949,7 +1984,11 @@ __global__ __launch_bounds__(64) void merge_ranges_kernel(const float* __restric
 // (one coalesced pass), when they fit.
 constexpr int kMergeStageBytes = 48 * 1024;
 
-__global__ __launch_bounds__(256) void merge_ranges_lds_kernel(const float* __restrict__ part_s,
+// Partial lists staged in LDS (scores, ids and counts in one round of loads,
+// eight per thread in flight: the lists come from every XCD's share of the
+// scan, so each round is a trip to memory), then one wavefront merges. Small
+// batches launch it with 1024 threads, large ones with 256.
+__global__ __launch_bounds__(1024) void merge_ranges_lds_kernel(const float* __restrict__ part_s,
                                                                const uint32_t* __restrict__ part_i,
                                                                const int* __restrict__ part_cnt, long nq,
                                                                int R, int k, int k_out,
@@ -1961,11 +2000,33 @@ __global__ __launch_bounds__(256) void merge_ranges_lds_kernel(const float* __re
   float* ls = reinterpret_cast<float*>(smem);
   uint32_t* li = reinterpret_cast<uint32_t*>(ls + n);
   int* lc = reinterpret_cast<int*>(li + n);
+  const int tid = threadIdx.x, nt = blockDim.x;
   for (long q = blockIdx.x; q < nq; q += gridDim.x) {
     __syncthreads();
-    stage_to_lds(ls, part_s + q * n, n);
-    stage_to_lds(reinterpret_cast<float*>(li), reinterpret_cast<const float*>(part_i + q * n), n);
-    stage_to_lds(reinterpret_cast<float*>(lc), reinterpret_cast<const float*>(part_cnt + q * R), R);
+    const float* ss = part_s + q * n;
+    const uint32_t* si = part_i + q * n;
+    const int* sc = part_cnt + q * R;
+    for (int base = 0; base < n || base < R; base += 8 * nt) {
+      float a[8];
+      uint32_t b[8];
+      int c[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = base + u * nt + tid;
+        a[u] = j < n ? ss[j] : 0.f;
+        b[u] = j < n ? si[j] : 0u;
+        c[u] = j < R ? sc[j] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = base + u * nt + tid;
+        if (j < n) {
+          ls[j] = a[u];
+          li[j] = b[u];
+        }
+        if (j < R) lc[j] = c[u];
+      }
+    }
     __syncthreads();
     if (threadIdx.x < 64)
       merge_query(ls, li, lc, R, k, k_out, id_map, out_s + q * k_out, out_i + q * k_out, threadIdx.x);
@@ -3009,7 +3070,7 @@ hipError_t launch_merge_ranges(const float* part_s, const std::uint32_t* part_i,
   const int grid = grid_for((long)nq, 1, 65536);
   const size_t staged = (size_t)n_ranges * k * 8 + (size_t)n_ranges * 4;
   if (n_ranges > 1 && staged <= (size_t)kMergeStageBytes) {
-    hipLaunchKernelGGL(merge_ranges_lds_kernel, dim3(grid), dim3(256), staged, s, part_s, part_i, part_cnt,
+    hipLaunchKernelGGL(merge_ranges_lds_kernel, dim3(grid), dim3(nq <= 512 ? 1024 : 256), staged, s, part_s, part_i, part_cnt,
                        (long)nq, n_ranges, k, k_out, id_map, out_s, out_i);
   } else {
     hipLaunchKernelGGL(merge_ranges_kernel, dim3(grid), dim3(64), 0, s, part_s, part_i, part_cnt, (long)nq,
