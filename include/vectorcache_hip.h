@@ -303,7 +303,9 @@ vc_status vc_set_profiling(int on);
  * byte-pair kernel take the other of its two vectors-per-thread variants
  * (1 <-> 2) from the one the planner picks; bit 4 makes query preparation
  * of indexes created later always take the reference's sequential bias_corr
- * chain instead of the certified parallel sum; bits 16-31, when not
+ * chain instead of the certified parallel sum; bit 5 makes landmark searches
+ * rotate their queries in a separate kernel instead of inside the query
+ * projection kernel; bits 16-31, when not
  * zero, set the number of vector ranges each query tile is split into (at
  * most 512) in place of the automatic split. */
 vc_status vc_set_debug(unsigned flags);

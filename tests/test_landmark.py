@@ -260,3 +260,26 @@ def test_gpu_landmark_edge_cases(oracle):
     s, i, _ = oracle.lm_search(x, q, 7, n_landmarks=20, seed=0, batches=(5, 35), prepare_between=True)
     assert scores_equal(res.scores, s) and np.array_equal(res.ids, i)
     assert ix.size() == 40
+
+
+@pytest.mark.gpu
+def test_gpu_landmark_query_rotation_paths_agree(golden):
+    """The query rotation inside the projection kernel (default) and in its own
+    kernel (debug bit 5) give the same query rows and the golden results."""
+    import vectorcache_amd.index as vci
+
+    case = LANDMARK_CASES[-1]  # dim 1536, 1024 landmarks: rotation blocks of 512
+    tag, n, nq, dim, m, bits, rb, seed, k, batches, pb, fc = case
+    x, q = landmark_inputs(n, nq, dim)
+    ix = _gpu_index(case, x)
+    fused = ix.query_rows(q)
+    res = ix.search(q, k)
+    vci.set_debug(32)
+    try:
+        apart = ix.query_rows(q)
+        res2 = ix.search(q, k)
+    finally:
+        vci.set_debug(0)
+    assert bits_equal(fused[0], apart[0]) and bits_equal(fused[1], apart[1])
+    for r in (res, res2):
+        assert scores_equal(r.scores, golden[f"{tag}_scores"]) and np.array_equal(r.ids, golden[f"{tag}_ids"])

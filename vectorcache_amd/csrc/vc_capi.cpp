@@ -1341,8 +1341,12 @@ void lm_query_rows(vc_lm_index* ix, const float* q_dev, size_t nq, float** q_lan
   float* rot = ix->qwork.as<float>(orot);
   *q_land = ix->qwork.as<float>(oland);
   *q_perp = ix->qwork.as<float>(operp);
-  hip_check(vchip::launch_encode(ix->t_query, q_dev, nq, 0, nullptr, nullptr, rot, nullptr, s), "encode(rotate)");
-  hip_check(vchip::launch_lm_query(rot, nq, int(ix->dim), int(ix->dim_pad), ix->basis.as<uint32_t>(ix->o_codes),
+  const bool fused = (g_debug_flags.load() & 32) == 0 &&  // flags bit 5: separate rotation (test hook)
+                     vchip::lm_query_rotates(ix->t_query, int(ix->dim), int(ix->dim_pad), int(ix->scan_dim));
+  if (!fused)
+    hip_check(vchip::launch_encode(ix->t_query, q_dev, nq, 0, nullptr, nullptr, rot, nullptr, s), "encode(rotate)");
+  hip_check(vchip::launch_lm_query(fused ? &ix->t_query : nullptr, fused ? q_dev : rot, nq, int(ix->dim),
+                                   int(ix->dim_pad), ix->basis.as<uint32_t>(ix->o_codes),
                                    ix->basis.as<float>(ix->o_scales), int(ix->m), *q_land, int(ix->scan_dim),
                                    *q_perp, s),
             "lm_query");

@@ -144,10 +144,15 @@ hipError_t launch_lm_project(const float* Y, std::size_t n, int dim, const float
 hipError_t launch_lm_resid(const float* rows, std::size_t n, int m, int stride, const float* vnorm2, float* rn,
                            hipStream_t s);
 // Query projection onto the 4-bit basis: q_land nq x stride (zero past m) and
-// q_perp[nq] from rotated query rows q_rot (nq x dim). codes_t: the 4-bit
-// basis as u32 words [dim_pad / 8][m] (word w of column j at w * m + j).
-hipError_t launch_lm_query(const float* q_rot, std::size_t nq, int dim, int dim_pad, const std::uint32_t* codes_t,
-                           const float* lscales, int m, float* q_land, int stride, float* q_perp, hipStream_t s);
+// q_perp[nq] from rotated query rows q_rot (nq x dim); with rot (the query
+// tables) non-null, q_rot holds the raw queries and the kernel applies the
+// row norm and rotation itself (only where lm_query_rotates(*rot, ...)).
+// codes_t: the 4-bit basis as u32 words [dim_pad / 8][m] (word w of column j
+// at w * m + j).
+bool lm_query_rotates(const CodeTables& t, int dim, int dim_pad, int stride);
+hipError_t launch_lm_query(const CodeTables* rot, const float* q_rot, std::size_t nq, int dim, int dim_pad,
+                           const std::uint32_t* codes_t, const float* lscales, int m, float* q_land, int stride,
+                           float* q_perp, hipStream_t s);
 
 hipError_t launch_flat_to_perm0(const std::uint8_t* flat, std::size_t n, int groups,
                                 std::uint8_t* perm0, hipStream_t s);

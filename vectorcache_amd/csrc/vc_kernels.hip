@@ -2696,9 +2696,13 @@ __global__ __launch_bounds__(256) void lm_resid_kernel(const float* __restrict__
 // w * m + j), so the lanes of a wavefront, one column each, read consecutive
 // words. 1024 threads, one column each for the default 1024 landmarks, with
 // the word loop unrolled so that each lane keeps several loads in flight (a
-// single query is latency-bound otherwise).
+// single query is latency-bound otherwise). raw_R != 0: q_rot holds the raw
+// queries; the kernel takes the row norm and rotates them itself (one
+// wavefront per rotation block of 64 * raw_R elements, as query preparation
+// does), saving a launch.
 constexpr int kLmQueryThreads = 1024;
-__global__ __launch_bounds__(kLmQueryThreads) void lm_query_kernel(const float* __restrict__ q_rot, long nq, int D,
+__global__ __launch_bounds__(kLmQueryThreads) void lm_query_kernel(CodeTables t, int raw_R,
+                                                                   const float* __restrict__ q_rot, long nq, int D,
                                                        int dim_pad, const uint32_t* __restrict__ codes_t,
                                                        const float* __restrict__ lscales, int m,
                                                        float* __restrict__ q_land, int stride,
@@ -2708,6 +2712,8 @@ __global__ __launch_bounds__(kLmQueryThreads) void lm_query_kernel(const float* 
   float* ql = qf + D;                          // stride
   int* qe = reinterpret_cast<int*>(ql + stride);  // dims 8w + {0, 2, 4, 6} as bytes
   int* qo = qe + dim_pad / 8;                     // dims 8w + {1, 3, 5, 7}
+  float* xs = reinterpret_cast<float*>(qo + dim_pad / 8);  // D: the raw query (raw_R != 0)
+  __shared__ float s_norm;
   constexpr int kWaves = kLmQueryThreads / 64;
   __shared__ float s_amax[kWaves];
   __shared__ int s_sum[kWaves];
@@ -2718,10 +2724,27 @@ __global__ __launch_bounds__(kLmQueryThreads) void lm_query_kernel(const float* 
     __syncthreads();
     const float* xr = q_rot + qi * (long)D;
     float am = 0.f;
-    for (int i = tid; i < D; i += kLmQueryThreads) {
-      const float v = xr[i];
-      qf[i] = v;
-      am = fmaxf(am, fabsf(v));
+    if (raw_R != 0) {
+      stage_to_lds(xs, xr, D);
+      __syncthreads();
+      if (wave == 0) {
+        const float nrm = norm8_lds(xs, D, lane);
+        if (lane == 0) s_norm = nrm;
+      }
+      __syncthreads();
+      const float inv = s_norm > kMinNorm ? div_rn(1.f, s_norm) : 0.f;
+      if (raw_R == 2) rotate_query_blocks<2>(t, xs, inv, qf, wave, kWaves, lane);
+      else if (raw_R == 4) rotate_query_blocks<4>(t, xs, inv, qf, wave, kWaves, lane);
+      else if (raw_R == 8) rotate_query_blocks<8>(t, xs, inv, qf, wave, kWaves, lane);
+      else rotate_query_blocks<16>(t, xs, inv, qf, wave, kWaves, lane);
+      __syncthreads();
+      for (int i = tid; i < D; i += kLmQueryThreads) am = fmaxf(am, fabsf(qf[i]));
+    } else {
+      for (int i = tid; i < D; i += kLmQueryThreads) {
+        const float v = xr[i];
+        qf[i] = v;
+        am = fmaxf(am, fabsf(v));
+      }
     }
     for (int o = 32; o > 0; o >>= 1) am = fmaxf(am, __shfl_xor(am, o));
     if (lane == 0) s_amax[wave] = am;
@@ -3271,15 +3294,27 @@ hipError_t launch_lm_resid(const float* rows, std::size_t n, int m, int stride, 
   return hipGetLastError();
 }
 
-hipError_t launch_lm_query(const float* q_rot, std::size_t nq, int dim, int dim_pad, const std::uint32_t* codes_t,
-                           const float* lscales, int m, float* q_land, int stride, float* q_perp, hipStream_t s) {
+bool lm_query_rotates(const CodeTables& t, int dim, int dim_pad, int stride) {
+  const int R = t.block / 64;
+  const size_t lds = ((size_t)dim * 2 + (size_t)stride) * 4 + (size_t)(dim_pad / 8) * 8;
+  return t.dim == dim && t.perm_gather != nullptr && (R == 2 || R == 4 || R == 8 || R == 16) &&
+         lds <= (size_t)kLdsBudget;
+}
+
+hipError_t launch_lm_query(const CodeTables* rot, const float* q_rot, std::size_t nq, int dim, int dim_pad,
+                           const std::uint32_t* codes_t, const float* lscales, int m, float* q_land, int stride,
+                           float* q_perp, hipStream_t s) {
   if (nq == 0) return hipSuccess;
   if (dim_pad % 16 != 0 || dim_pad < dim || stride < m) return hipErrorInvalidValue;
-  const size_t lds = ((size_t)dim + (size_t)stride) * 4 + (size_t)(dim_pad / 8) * 8;
+  if (rot != nullptr && !lm_query_rotates(*rot, dim, dim_pad, stride)) return hipErrorInvalidValue;
+  const size_t lds = ((size_t)dim * (rot != nullptr ? 2 : 1) + (size_t)stride) * 4 + (size_t)(dim_pad / 8) * 8;
   if (lds > (size_t)kLdsBudget) return hipErrorInvalidValue;
+  const CodeTables t = rot != nullptr ? *rot : CodeTables{};
+  const int raw_R = rot != nullptr ? rot->block / 64 : 0;
   if (lds > 64 * 1024)
     (void)hipFuncSetAttribute((const void*)lm_query_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(lm_query_kernel, dim3((unsigned)std::min<size_t>(nq, 16384)), dim3(kLmQueryThreads), lds, s, q_rot,
+  hipLaunchKernelGGL(lm_query_kernel, dim3((unsigned)std::min<size_t>(nq, 16384)), dim3(kLmQueryThreads), lds, s, t,
+                     raw_R, q_rot,
                      (long)nq, dim, dim_pad, codes_t, lscales, m, q_land, stride, q_perp);
   return hipGetLastError();
 }
