@@ -283,3 +283,39 @@ def test_gpu_landmark_query_rotation_paths_agree(golden):
     assert bits_equal(fused[0], apart[0]) and bits_equal(fused[1], apart[1])
     for r in (res, res2):
         assert scores_equal(r.scores, golden[f"{tag}_scores"]) and np.array_equal(r.ids, golden[f"{tag}_ids"])
+
+
+@pytest.mark.gpu
+def test_gpu_landmark_full_size_properties():
+    """1M x 1536 with the default 1024 landmarks (the benchmark's index), where
+    the oracle is too slow: a batch of queries equals the same queries one at
+    a time bit for bit, device search equals host search, rows are sorted by
+    (score desc, id asc) with ids in range, and a stored vector queried by
+    itself ranks first."""
+    import torch
+
+    from vectorcache_amd import synthetic
+    from vectorcache_amd.landmark import LandmarkIndex
+
+    dev = torch.device("cuda:0")
+    n, dim, k = 1_000_000, 1536, 10
+    x = synthetic.anisotropic_torch(n, dim, 2001, dev)
+    q = synthetic.anisotropic_torch(8, dim, 2003, dev)
+    q[0] = x[123457]
+    ix = LandmarkIndex(dim, 4)
+    ix.add(x)
+    ix.prepare()
+    del x
+    qh = q.cpu().numpy()
+    batch = ix.search(qh, k)
+    for j in range(qh.shape[0]):
+        one = ix.search(qh[j: j + 1], k)
+        assert bits_equal(one.scores[0], batch.scores[j]) and np.array_equal(one.ids[0], batch.ids[j])
+    s, i = ix.search_device(q, k)
+    torch.cuda.synchronize()
+    assert bits_equal(s.cpu().numpy(), batch.scores) and np.array_equal(i.cpu().numpy().view(np.uint64), batch.ids)
+    assert (batch.ids < n).all()
+    for j in range(qh.shape[0]):
+        sc, ids = batch.scores[j], batch.ids[j]
+        assert all(sc[t] > sc[t + 1] or (sc[t] == sc[t + 1] and ids[t] < ids[t + 1]) for t in range(k - 1))
+    assert batch.ids[0, 0] == 123457
