@@ -77,6 +77,8 @@ LANDMARK_CASES = (
     ("reprep", 600, 6, 256, 64, 4, 4, 9, 10, (200, 250, 150), True, False),
     ("rescore", 300, 4, 64, 36, 4, 4, 1, 5, None, False, True),
     ("d1536", 1500, 8, 1536, 1024, 4, 4, 0, 10, None, False, False),
+    ("d768b3", 1200, 5, 768, 256, 3, 2, 4, 64, None, False, False),
+    ("d384b2", 900, 4, 384, 200, 2, 4, 6, 100, (300, 600), False, False),
 )
 
 
