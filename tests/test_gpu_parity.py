@@ -453,3 +453,37 @@ def test_float_rescore_device_after_search(vc, oracle):
     rec_lut = np.mean([len(set(i0[j, :10]) & set(exact[j])) for j in range(32)])
     rec_rr = np.mean([len(set(io[j, :10]) & set(exact[j])) for j in range(32)])
     assert rec_rr >= rec_lut
+
+
+def test_search_device_orders_on_current_stream(vc):
+    """search_device defaults to torch's current stream: fp16, non-contiguous
+    queries made on that stream (no synchronize before the call) give the
+    results of a host search of the same rounded queries; CPU tensors, rows
+    that are not whole and short or mistyped output buffers are rejected."""
+    import torch
+
+    from vectorcache_amd import synthetic
+
+    x = synthetic.mixture(6000, 128, 32, 0.3, 71, 72)
+    q = synthetic.mixture(24, 128, 32, 0.3, 71, 73)
+    q16 = q.astype(np.float16).astype(np.float32)
+    ix = vc.TurboQuantIndex(128, 4)
+    ix.add(x)
+    ix.prepare()
+    want = ix.search(q16, 10)
+    dev = torch.device("cuda:0")
+    torch.cuda.synchronize()
+    # a large producer kernel right before the call: the search must wait for it
+    big = torch.randn((4096, 4096), device=dev)
+    qt = (torch.from_numpy(np.ascontiguousarray(q.T)).to(dev).half() + big[:128, :24].half() * 0)
+    s, i = ix.search_device(qt.t(), 10)
+    got_s, got_i = s.cpu().numpy(), i.cpu().numpy().astype(np.uint64)
+    assert np.array_equal(got_i, want.ids) and scores_equal(got_s, want.scores)
+    with pytest.raises(ValueError):
+        ix.search_device(torch.from_numpy(q), 10)
+    with pytest.raises(ValueError):
+        ix.search_device(torch.from_numpy(q).to(dev)[:, :100], 10)
+    with pytest.raises(ValueError):
+        ix.search_device(torch.from_numpy(q).to(dev), 10, scores=torch.empty((24, 5), device=dev))
+    with pytest.raises(ValueError):
+        ix.search_device(torch.from_numpy(q).to(dev), 10, ids=torch.empty((24, 10), dtype=torch.int32, device=dev))

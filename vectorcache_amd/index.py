@@ -31,6 +31,53 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(P)
 
 
+def _cuda_rows(t, dim: int, name: str):
+    """A CUDA tensor of whole rows of `dim` as contiguous float32 (a copy made
+    on torch's current stream when it is not one already)."""
+    if not (_is_torch(t) and t.is_cuda):
+        raise ValueError(f"{name} must be a CUDA tensor")
+    t = t.contiguous().float()
+    if t.numel() % dim != 0:
+        raise ValueError(f"{name} size {t.numel()} is not a multiple of dim {dim}")
+    return t
+
+
+def _cuda_out(buf, rows: int, cols: int, dtype, name: str, device):
+    """A caller output buffer checked for device, dtype, layout and room for
+    rows x cols, or a new one."""
+    import torch
+
+    if buf is None:
+        return torch.empty((rows, cols), dtype=dtype, device=device)
+    if not (_is_torch(buf) and buf.is_cuda):
+        raise ValueError(f"{name} must be a CUDA tensor")
+    if buf.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {buf.dtype}")
+    if not buf.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if buf.numel() < rows * cols:
+        raise ValueError(f"{name} holds {buf.numel()} elements, {rows} x {cols} needed")
+    return buf
+
+
+def _launch_stream(stream, *tensors):
+    """The HIP stream a device call is ordered on: torch's current stream by
+    default, so the kernels run after the work that produced the inputs and
+    before whatever torch enqueues next. A caller-given stream also keeps the
+    temporaries alive for it."""
+    import torch
+
+    if stream is None:
+        return P(torch.cuda.current_stream().cuda_stream)
+    cur = torch.cuda.current_stream().cuda_stream
+    if int(stream) != int(cur):
+        ext = torch.cuda.ExternalStream(int(stream))
+        for t in tensors:
+            if t is not None:
+                t.record_stream(ext)
+    return P(int(stream))
+
+
 @dataclass
 class SearchResults:
     """nq x k row-major results (reference search.hpp:77-84); rows sorted by
@@ -125,18 +172,17 @@ class TurboQuantIndex:
 
     def search_device(self, queries, k: int, scores=None, ids=None, stream=None):
         """Device-resident search: queries is a torch CUDA tensor (nq x dim).
-        Returns torch tensors (scores f32, ids int64 holding the u64 ids)."""
+        Returns torch tensors (scores f32, ids int64 holding the u64 ids),
+        written in order on `stream` (default: torch's current stream)."""
         import torch
 
-        q = queries.contiguous().float()
+        q = _cuda_rows(queries, self.dim(), "queries")
         nq = q.numel() // self.dim()
         kk = min(int(k), self.size())
-        if scores is None:
-            scores = torch.empty((nq, kk), dtype=torch.float32, device=q.device)
-        if ids is None:
-            ids = torch.empty((nq, kk), dtype=torch.int64, device=q.device)
+        scores = _cuda_out(scores, nq, kk, torch.float32, "scores", q.device)
+        ids = _cuda_out(ids, nq, kk, torch.int64, "ids", q.device)
         kout = S()
-        st = P(stream) if stream is not None else None
+        st = _launch_stream(stream, q)
         check(lib.vc_index_search_device(self._h, P(q.data_ptr()), S(nq), S(k), P(scores.data_ptr()),
                                          P(ids.data_ptr()), ctypes.byref(kout), st))
         return scores, ids

@@ -15,7 +15,7 @@ import ctypes
 import numpy as np
 
 from ._capi import P, S, check, lib
-from .index import SearchResults, _f32, _is_torch, _ptr
+from .index import SearchResults, _cuda_out, _cuda_rows, _f32, _is_torch, _launch_stream, _ptr
 
 DEFAULT_LANDMARKS = 1024  # reference kDefaultLandmarks (include/vectorcache/constants.hpp:28)
 
@@ -111,23 +111,22 @@ class LandmarkIndex:
     def search_device(self, queries, k: int, scores=None, ids=None, float_corpus=None, stream=None):
         """Device-resident search: queries (and the optional float corpus) are
         torch CUDA tensors. Returns torch tensors (scores f32, ids int64 holding
-        the u64 ids)."""
+        the u64 ids), written in order on `stream` (default: torch's current
+        stream)."""
         import torch
 
-        q = queries.contiguous().float()
+        q = _cuda_rows(queries, self.dim(), "queries")
         nq = q.numel() // self.dim()
         kk = min(int(k), self.size())
-        if scores is None:
-            scores = torch.empty((nq, kk), dtype=torch.float32, device=q.device)
-        if ids is None:
-            ids = torch.empty((nq, kk), dtype=torch.int64, device=q.device)
+        scores = _cuda_out(scores, nq, kk, torch.float32, "scores", q.device)
+        ids = _cuda_out(ids, nq, kk, torch.int64, "ids", q.device)
         fc = None
         if float_corpus is not None:
-            fc = float_corpus.contiguous().float()
+            fc = _cuda_rows(float_corpus, self.dim(), "float_corpus")
             if fc.numel() < self.size() * self.dim():
                 raise ValueError("float_corpus must have at least size() * dim rows")
         kout = S()
-        st = P(stream) if stream is not None else None
+        st = _launch_stream(stream, q, fc)
         check(lib.vc_lm_index_search_device(self._h, P(q.data_ptr()), S(nq), S(k),
                                             P(fc.data_ptr()) if fc is not None else None, P(scores.data_ptr()),
                                             P(ids.data_ptr()), ctypes.byref(kout), st))
