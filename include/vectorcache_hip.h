@@ -16,8 +16,8 @@
  * (score desc, id asc); a -0.0f score is reported as +0.0f.
  *
  * "host" pointers are ordinary CPU memory; "device" pointers are HIP device
- * memory on the index's device; `stream` is a hipStream_t (NULL = the index's
- * own stream).
+ * memory on the index's device; `stream` is a hipStream_t (NULL = the null
+ * stream, as in every HIP call; host-buffer calls use the index's own stream).
  */
 #ifndef VECTORCACHE_HIP_H
 #define VECTORCACHE_HIP_H
@@ -82,7 +82,7 @@ vc_status vc_index_prepare(vc_index* index);
 vc_status vc_index_search(vc_index* index, const float* queries, size_t nq, size_t k,
                           float* scores, uint64_t* ids, size_t* k_out);
 /* Same with device-resident queries and outputs, ordered on `stream`
- * (NULL: the index's stream). The call returns once the work is enqueued. */
+ * (NULL: the null stream). The call returns once the work is enqueued. */
 vc_status vc_index_search_device(vc_index* index, const float* queries_dev, size_t nq, size_t k,
                                  float* scores_dev, uint64_t* ids_dev, size_t* k_out,
                                  void* stream);

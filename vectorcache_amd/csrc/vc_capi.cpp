@@ -474,7 +474,7 @@ vc_status vc_index_search_device(vc_index* ix, const float* queries_dev, size_t 
   return guarded([&] {
     if (ix == nullptr || (queries_dev == nullptr && nq > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
     std::lock_guard lock(ix->mu);
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ix->stream;
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
     const size_t ke = index_search_dev(ix, queries_dev, nq, k, scores_dev, ids_dev, s);
     if (k_out) *k_out = ke;
   });
@@ -1521,7 +1521,7 @@ vc_status vc_lm_index_search_device(vc_lm_index* ix, const float* queries_dev, s
   return guarded([&] {
     if (ix == nullptr || (queries_dev == nullptr && nq > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
     std::lock_guard lock(ix->mu);
-    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ix->stream;
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
     const size_t ke = lm_search_dev(ix, queries_dev, nq, k, float_corpus_dev, scores_dev, ids_dev, s);
     if (k_out) *k_out = ke;
   });
