@@ -87,6 +87,22 @@ vc_status vc_index_search_device(vc_index* index, const float* queries_dev, size
                                  float* scores_dev, uint64_t* ids_dev, size_t* k_out,
                                  void* stream);
 
+/* vc_index_search_device without allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments and results as vc_index_search_device; the caller passes inputs
+ * and outputs, and the search's scratch (LUTs, pair-table sources, per-range
+ * top-k lists) lives in one fixed array per GPU of vc_noalloc_scratch_bytes()
+ * bytes (16 MiB), held until the process ends. A search that needs more
+ * (many queries, a large k or dim) returns VC_ERR_INVALID_ARGUMENT and changes
+ * nothing. Nothing in the scratch needs to start at zero. Every call on one
+ * GPU shares that array: calls that overlap on one GPU (two streams, two host
+ * threads on different indexes) race and corrupt each other's results. The
+ * index's device must be the current device. */
+vc_status vc_index_search_device_noalloc(vc_index* index, const float* queries_dev, size_t nq, size_t k,
+                                         float* scores_dev, uint64_t* ids_dev, size_t* k_out, void* stream);
+size_t vc_noalloc_scratch_bytes(void);
+
 /* Accessors (index.hpp:23-40). */
 size_t vc_index_dim(const vc_index* index);
 size_t vc_index_size(const vc_index* index);

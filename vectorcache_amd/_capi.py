@@ -65,6 +65,8 @@ _SIGS = {
     "vc_index_prepare": ([P], I),
     "vc_index_search": ([P, P, S, S, P, P, ctypes.POINTER(S)], I),
     "vc_index_search_device": ([P, P, S, S, P, P, ctypes.POINTER(S), P], I),
+    "vc_index_search_device_noalloc": ([P, P, S, S, P, P, ctypes.POINTER(S), P], I),
+    "vc_noalloc_scratch_bytes": ([], S),
     "vc_index_dim": ([P], S),
     "vc_index_size": ([P], S),
     "vc_index_bits": ([P], S),

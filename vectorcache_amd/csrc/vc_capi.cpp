@@ -225,10 +225,28 @@ struct SearchOut {
   size_t k_eff;
 };
 
+// Scratch of a search: a growable device buffer, or (when `fixed` is set) a
+// region the caller owns of `fixed_cap` bytes, which is never grown.
+struct ScratchArena {
+  DevBuf* grow = nullptr;
+  uint8_t* fixed = nullptr;
+  size_t fixed_cap = 0;
+  uint8_t* take(size_t bytes, hipStream_t s) {
+    if (fixed != nullptr) {
+      if (bytes > fixed_cap)
+        fail(VC_ERR_INVALID_ARGUMENT, "search needs " + std::to_string(bytes) + " bytes of scratch; the fixed scratch "
+                                      "holds " + std::to_string(fixed_cap) + " (fewer queries or a smaller k)");
+      return fixed;
+    }
+    grow->reserve(bytes, s);
+    return grow->as<uint8_t>();
+  }
+};
+
 SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const float* alpha, size_t n,
                        const float* q_dev, size_t nq, size_t k, const uint64_t* id_map_dev, float* out_s,
                        uint64_t* out_i, DevBuf& scratch, hipStream_t s, uint8_t** luts_out = nullptr,
-                       size_t* luts_off = nullptr) {
+                       size_t* luts_off = nullptr, ScratchArena* arena = nullptr) {
   SearchOut r{std::min(k, n)};
   if (nq == 0 || r.k_eff == 0) return r;
   if (r.k_eff > size_t(vchip::kMaxTopK))
@@ -252,19 +270,19 @@ SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const f
   const size_t parts = nq * size_t(plan.n_ranges);
   const size_t o_ps = c.take(parts * ke * 4), o_pi = c.take(parts * ke * 4), o_pc = c.take(parts * 4);
   const size_t o_lt = c.take(vchip::scan_lut_bytes(plan, t.groups16, nq));
-  scratch.reserve(c.off, s);
-  hip_check(vchip::launch_prepare_queries(t, q_dev, nq, scratch.as<uint8_t>(o_luts), scratch.as<float>(o_ls),
-                                          scratch.as<float>(o_lb), scratch.as<float>(o_bc),
-                                          scratch.as<float>(o_bt), s),
+  ScratchArena own{&scratch};
+  uint8_t* base = (arena != nullptr ? arena : &own)->take(c.off, s);
+  auto at = [base](size_t off) { return base + off; };
+  auto atf = [base](size_t off) { return reinterpret_cast<float*>(base + off); };
+  hip_check(vchip::launch_prepare_queries(t, q_dev, nq, at(o_luts), atf(o_ls), atf(o_lb), atf(o_bc), atf(o_bt), s),
             "prepare_queries");
   std::unique_lock probe_lock(g_probe.mu);
   const bool probe = g_probe.on;
   if (!probe) probe_lock.unlock();
   if (probe) hip_check(hipEventRecord(g_probe.start, s), "hipEventRecord");
-  hip_check(vchip::launch_scan(plan, tiled, alpha, n, t.groups, t.groups16, scratch.as<uint8_t>(o_luts),
-                               scratch.as<uint8_t>(o_lt), scratch.as<float>(o_ls), scratch.as<float>(o_bt), nq, ke,
-                               scratch.as<float>(o_ps),
-                               scratch.as<uint32_t>(o_pi), scratch.as<int>(o_pc), s),
+  hip_check(vchip::launch_scan(plan, tiled, alpha, n, t.groups, t.groups16, at(o_luts), at(o_lt), atf(o_ls),
+                               atf(o_bt), nq, ke, atf(o_ps), reinterpret_cast<uint32_t*>(at(o_pi)),
+                               reinterpret_cast<int*>(at(o_pc)), s),
             "scan");
   if (probe) {
     hip_check(hipEventRecord(g_probe.stop, s), "hipEventRecord");
@@ -273,11 +291,12 @@ SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const f
     g_probe.ranges = plan.n_ranges;
     probe_lock.unlock();
   }
-  hip_check(vchip::launch_merge_ranges(scratch.as<float>(o_ps), scratch.as<uint32_t>(o_pi), scratch.as<int>(o_pc),
-                                       nq, plan.n_ranges, ke, ke, id_map_dev, out_s, out_i, s),
+  hip_check(vchip::launch_merge_ranges(atf(o_ps), reinterpret_cast<uint32_t*>(at(o_pi)),
+                                       reinterpret_cast<int*>(at(o_pc)), nq, plan.n_ranges, ke, ke, id_map_dev, out_s,
+                                       out_i, s),
             "merge_ranges");
   if (luts_out != nullptr) {
-    *luts_out = scratch.as<uint8_t>(0);
+    *luts_out = base;
     *luts_off = o_luts;
   }
   return r;
@@ -477,6 +496,56 @@ vc_status vc_index_search_device(vc_index* ix, const float* queries_dev, size_t 
     hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
     const size_t ke = index_search_dev(ix, queries_dev, nq, k, scores_dev, ids_dev, s);
     if (k_out) *k_out = ke;
+  });
+}
+
+}  // extern "C"
+
+// ---- search without allocation ----------------------------------------------
+// Scratch of vc_index_search_device_noalloc: one array per GPU, held until the
+// process ends. Host code never passes `scratch` itself (its host-side name is
+// not a device address); it passes the device address that
+// hipGetSymbolAddress returns for the current device, looked up once per
+// device on the entry point's first call there.
+static constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_noalloc_mu;
+std::map<int, uint8_t*> g_noalloc_ptr;
+
+uint8_t* noalloc_scratch(int device) {
+  std::lock_guard lock(g_noalloc_mu);
+  auto it = g_noalloc_ptr.find(device);
+  if (it != g_noalloc_ptr.end()) return it->second;
+  void* p = nullptr;
+  hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  if (p == nullptr) fail(VC_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned null");
+  g_noalloc_ptr[device] = static_cast<uint8_t*>(p);
+  return static_cast<uint8_t*>(p);
+}
+}  // namespace
+
+extern "C" {
+
+size_t vc_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
+vc_status vc_index_search_device_noalloc(vc_index* ix, const float* queries_dev, size_t nq, size_t k,
+                                         float* scores_dev, uint64_t* ids_dev, size_t* k_out, void* stream) {
+  return guarded([&] {
+    if (ix == nullptr || (queries_dev == nullptr && nq > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    if (!ix->prepared) fail(VC_ERR_STATE, "search() before prepare()");
+    int dev = -1;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != ix->device) fail(VC_ERR_INVALID_ARGUMENT, "the index's device must be current");
+    ScratchArena arena;
+    arena.fixed = noalloc_scratch(dev);
+    arena.fixed_cap = kNoAllocScratchBytes;
+    hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
+    const auto r = search_tiled(ix->t, ix->tiled.as<uint8_t>(), ix->alpha.as<float>(), ix->n_encoded, queries_dev,
+                                nq, k, nullptr, scores_dev, ids_dev, ix->scratch, s, nullptr, nullptr, &arena);
+    if (k_out) *k_out = r.k_eff;
   });
 }
 
