@@ -191,6 +191,104 @@ int vc_lm_index_is_prepared(const vc_lm_index* index);
 size_t vc_lm_index_codes_bytes(const vc_lm_index* index);
 size_t vc_lm_index_index_bytes(const vc_lm_index* index);
 
+/* ---- bucketed index ----------------------------------------------------------
+ * The README's BucketedTurboQuantIndex (README.md:56,114-120; the reference has
+ * prose and an API sketch, no code): rows are L2-normalized and grouped into
+ * buckets; each bucket's rows are encoded with the flat index's encode, as
+ * residuals x_hat - c_b against the bucket's encode centroid (residual = 1) or
+ * as themselves (residual = 0); a query opens its n_probe best buckets by
+ * routing score and ranks their vectors with <q_hat, c_b> + alpha <q_hat, r_hat>.
+ * Each opened bucket is scored exactly as score_prepared_into scores one
+ * bucket (src/search/search.cpp:1156-1267, score_offsets = <q_hat, c_b>); the
+ * result rows are the top k of the opened vectors by (score desc, id asc).
+ * Build (prepare): a coarse quantizer of ceil(n / energy_soft) lists trained
+ * by spherical k-means (kmeans_iters Lloyd iterations on kmeans_sample rows
+ * per list, strided over the rows; the seed offsets the initial centroids),
+ * every row assigned to its nearest list, and every bucket above
+ * max_bucket_size split by one iteration of spherical 2-means until none is.
+ * Routing and assignment run on bf16 matrix cores: they choose buckets only.
+ * The offset is (float) of a sequential double sum over the dimensions of
+ * q_hat[d] c_b[d]; q_hat = q * (1 / ||q||) with the encode's row norm.
+ * prepare() rebuilds from every row added so far (ids = insertion order). */
+typedef struct vc_bucketed vc_bucketed;
+typedef struct {
+  size_t n_probe;          /* buckets a query opens without a learned or per-query count (64) */
+  size_t max_bucket_size;  /* hard cap at the build (800) */
+  size_t energy_soft;      /* mean bucket size the coarse quantizer aims at (400) */
+  size_t expected_n;       /* capacity hint, rows (0) */
+  size_t kmeans_iters;     /* Lloyd iterations of the coarse quantizer (8) */
+  size_t kmeans_sample;    /* training rows per list (64) */
+  uint64_t seed;           /* (4001) */
+  int residual;            /* 1: residual codes with score offsets; 0: codes of the rows (1) */
+} vc_bucket_params;
+void vc_bucket_params_default(vc_bucket_params* params);
+/* dim a multiple of 8, <= 4096; bits in {2, 3, 4}; params NULL = defaults. */
+vc_status vc_bucketed_create(size_t dim, size_t bits, const vc_bucket_params* params, vc_bucketed** out);
+vc_status vc_bucketed_destroy(vc_bucketed* index);
+/* TQ+ calibration: fitted at prepare() on the sample's residuals (or rows). */
+vc_status vc_bucketed_calibrate(vc_bucketed* index, const float* sample, size_t n);
+vc_status vc_bucketed_calibrate_device(vc_bucketed* index, const float* sample_dev, size_t n, void* stream);
+vc_status vc_bucketed_add(vc_bucketed* index, const float* vectors, size_t n);
+/* Device rows, copied in order after `stream` (NULL: the null stream). */
+vc_status vc_bucketed_add_device(vc_bucketed* index, const float* vectors_dev, size_t n, void* stream);
+vc_status vc_bucketed_prepare(vc_bucketed* index);
+/* Host queries; probe counts from the learned model when one is enabled, else
+ * params.n_probe. k_eff = min(k, size()) <= 2048. Rows with fewer than k_eff
+ * opened vectors end with (0, 0). */
+vc_status vc_bucketed_search(vc_bucketed* index, const float* queries, size_t nq, size_t k, float* scores,
+                             uint64_t* ids, size_t* k_out);
+/* Device queries and outputs, ordered on `stream` (NULL: the null stream).
+ * n_probe > 0: every query opens min(n_probe, 512, n_buckets) buckets, or with
+ * probe_counts_dev (int32 [nq]) query q opens clamp(counts[q], 1, that bound);
+ * n_probe = 0: the learned model when enabled, else params.n_probe.
+ * counts_dev (u64 [nq], may be NULL) receives the entries found per row. */
+vc_status vc_bucketed_search_device(vc_bucketed* index, const float* queries_dev, size_t nq, size_t k, size_t n_probe,
+                                    const int32_t* probe_counts_dev, float* scores_dev, uint64_t* ids_dev,
+                                    uint64_t* counts_dev, size_t* k_out, void* stream);
+/* Each query's P best buckets by routing score (bucket ids int32, scores f32,
+ * nq x P, sorted by (score desc, bucket asc)); 1 <= P <= min(512, n_buckets). */
+vc_status vc_bucketed_route_device(vc_bucketed* index, const float* queries_dev, size_t nq, size_t P,
+                                   int32_t* probe_b_dev, float* probe_s_dev, void* stream);
+size_t vc_bucketed_dim(const vc_bucketed* index);
+size_t vc_bucketed_size(const vc_bucketed* index);
+size_t vc_bucketed_bits(const vc_bucketed* index);
+size_t vc_bucketed_n_buckets(const vc_bucketed* index);
+int vc_bucketed_is_prepared(const vc_bucketed* index);
+int vc_bucketed_is_calibrated(const vc_bucketed* index);
+/* The built state (each output may be NULL): bucket sizes (n_buckets), encode
+ * centroids (n_buckets x dim), the id of every position (size(); positions
+ * run bucket by bucket), flat codes by position (size() x G, the reference's
+ * codes_flat_from_packed bytes), alpha by position, TQ+ shift / scale (dim). */
+vc_status vc_bucketed_export(vc_bucketed* index, uint32_t* bucket_sizes, float* centroids, uint64_t* pos_ids,
+                             uint8_t* flat_codes, float* alpha, float* shift, float* scale);
+
+/* ---- learned probe budget (no reference counterpart) -----------------------
+ * An MLP 96 -> 256 -> 256 -> 1 (ReLU) per bucketed index, trained and run on
+ * the GPU with hand-written bf16 matrix-core kernels (fp32 accumulation, fp32
+ * master weights, Adam). Inputs of a query (vc_bucketed_probe_features): its
+ * best routing score, 16 x the gaps to its next 63, and log2(1 + size) / 10 of
+ * its 32 best buckets. Output: log2 of the buckets it needs; once enabled, a
+ * search with n_probe = 0 opens clamp(ceil(2^(y + shift)), 1, 256) buckets,
+ * the model running in one kernel on the routing lists. Parameters: fp32
+ * vc_probe_param_count() values, W1 [256][96], b1, W2 [256][256], b2, w3, b3. */
+size_t vc_probe_param_count(void);
+vc_status vc_bucketed_probe_features(vc_bucketed* index, const float* queries_dev, size_t nq, float* feats_dev,
+                                     void* stream);
+vc_status vc_bucketed_probe_init(vc_bucketed* index, uint64_t seed);
+vc_status vc_bucketed_probe_set_weights(vc_bucketed* index, const float* weights_host);
+vc_status vc_bucketed_probe_get_weights(vc_bucketed* index, float* weights_host);
+/* n_steps Adam steps (lr, beta1 0.9, beta2 0.999, eps 1e-8) on minibatches of
+ * 64 rows: rows order_dev[64 s .. 64 s + 63] of feats_dev (n x 96) and
+ * targets_dev (n); losses_dev (n_steps, may be NULL) the minibatch MSEs. */
+vc_status vc_bucketed_probe_train(vc_bucketed* index, const float* feats_dev, const float* targets_dev, size_t n,
+                                  const int32_t* order_dev, size_t n_steps, float lr, float* losses_dev, void* stream);
+/* The gradients of one minibatch (order_dev[0..63]) without an update. */
+vc_status vc_bucketed_probe_grads(vc_bucketed* index, const float* feats_dev, const float* targets_dev, size_t n,
+                                  const int32_t* order_dev, float* grads_host);
+vc_status vc_bucketed_probe_predict(vc_bucketed* index, const float* feats_dev, size_t n, float* y_dev, void* stream);
+/* on = 1: searches with n_probe = 0 use the model with this log2 shift. */
+vc_status vc_bucketed_probe_enable(vc_bucketed* index, int on, float shift);
+
 /* ---- stateless entry points (host buffers) ---------------------------------
  * Host precompute, bit-identical to the reference:
  * Rotation tables (src/transform/rotation.cpp:139-160), codebook

@@ -1563,3 +1563,42 @@ int oc_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, int
   free(flat);
   return rc;
 }
+
+/* ---- bucketed index ------------------------------------------------------
+ * The reference describes its bucketed index in prose only (README.md:56,
+ * 114-120): rows are L2-normalized, each bucket keeps a frozen encode
+ * centroid c, rows are encoded as residuals x_hat - c, and a bucket's
+ * vectors score <q_hat, c> + alpha <q_hat, r_hat> (score_prepared_into with
+ * score_offsets, src/search/search.cpp:1156-1267). x_hat = x * inv, inv =
+ * 1 / row_norm(x) (0 below 1e-10), each product and difference rounded. */
+int oc_bucket_residuals(const float* x, size_t dim, const uint64_t* pos_rows, const uint32_t* pos_bucket,
+                        size_t n, const float* cent, float* out) {
+  for (size_t p = 0; p < n; ++p) {
+    const float* xr = x + (size_t)pos_rows[p] * dim;
+    const float nv = oc_row_norm(xr, dim);
+    const float inv = (nv > kMinInputNorm) ? (1.f / nv) : 0.f;
+    for (size_t d = 0; d < dim; ++d) {
+      const float v = xr[d] * inv;
+      out[p * dim + d] = cent != NULL ? v - cent[(size_t)pos_bucket[p] * dim + d] : v;
+    }
+  }
+  return 0;
+}
+
+/* Score offset of query q and probed bucket b: (float) of the sequential
+ * double sum over d of q_hat[d] * c_b[d] (each product exact in double). */
+int oc_probe_offsets(const float* q, size_t nq, size_t dim, const float* cent, const int32_t* probe_b,
+                     size_t P, const int32_t* probe_n, float* off) {
+  for (size_t i = 0; i < nq; ++i) {
+    const float* qr = q + i * dim;
+    const float nv = oc_row_norm(qr, dim);
+    const float inv = (nv > kMinInputNorm) ? (1.f / nv) : 0.f;
+    for (size_t j = 0; j < (size_t)probe_n[i] && j < P; ++j) {
+      const float* c = cent + (size_t)probe_b[i * P + j] * dim;
+      double acc = 0.0;
+      for (size_t d = 0; d < dim; ++d) acc = fma((double)(qr[d] * inv), (double)c[d], acc);
+      off[i * P + j] = (float)acc;
+    }
+  }
+  return 0;
+}

@@ -71,6 +71,12 @@ int oc_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, int
                  const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
                  size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
                  uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds);
+/* bucketed index (README.md:56: rows L2-normalized, residuals x_hat - c,
+ * scores <q_hat, c> + alpha <q_hat, r_hat>) */
+int oc_bucket_residuals(const float* x, size_t dim, const uint64_t* pos_rows, const uint32_t* pos_bucket,
+                        size_t n, const float* cent, float* out);
+int oc_probe_offsets(const float* q, size_t nq, size_t dim, const float* cent, const int32_t* probe_b,
+                     size_t P, const int32_t* probe_n, float* off);
 /* the ChaCha block function (8 rounds for the rotation, 20 for RFC 7539 tests) */
 void oc_chacha_block(const uint32_t in[16], uint32_t out[16], int rounds);
 

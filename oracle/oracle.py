@@ -319,6 +319,73 @@ class Oracle:
         self._check(rc, "score_into")
         return hs, hi, hsz, hmin, hmi
 
+    # ---- bucketed index (its scores; bucket choice is not part of the oracle) ----
+    def bucket_residuals(self, x, pos_rows, pos_bucket, cent):
+        """Rows x_hat - c of the given positions (cent None: x_hat)."""
+        x = np.ascontiguousarray(x, np.float32)
+        dim = x.shape[1]
+        pr = np.ascontiguousarray(pos_rows, np.uint64)
+        pbk = np.ascontiguousarray(pos_bucket, np.uint32)
+        c = None if cent is None else np.ascontiguousarray(cent, np.float32)
+        out = np.zeros((pr.size, dim), np.float32)
+        self._check(self._port("bucket_residuals")(_ptr(x), _S(dim), _ptr(pr), _ptr(pbk), _S(pr.size), _ptr(c), _ptr(out)),
+                    "bucket_residuals")
+        return out
+
+    def probe_offsets(self, q, cent, probe_b, probe_n):
+        q = np.ascontiguousarray(q, np.float32)
+        nq, dim = q.shape
+        pb = np.ascontiguousarray(probe_b, np.int32)
+        pn = np.ascontiguousarray(probe_n, np.int32)
+        c = np.ascontiguousarray(cent, np.float32)
+        off = np.zeros(pb.shape, np.float32)
+        self._check(self._port("probe_offsets")(_ptr(q), _S(nq), _S(dim), _ptr(c), _ptr(pb), _S(pb.shape[1]), _ptr(pn),
+                                             _ptr(off)), "probe_offsets")
+        return off
+
+    def _port(self, name):
+        """A function of this repository's C restatement only (the reference
+        has no code for the bucketed index)."""
+        if self.kind != "port":
+            raise ValueError(f"{name} exists in the port oracle only")
+        return getattr(self.lib, "oc_" + name)
+
+    def bucketed_search(self, q, k, bits, bucket_sizes, flat, alpha, pos_ids, cent, probe_b, probe_n, shift=None,
+                        scale=None, residual=True):
+        """The bucketed index's search as a chain of score_prepared_into calls
+        (one per bucket, every query that probes it, offsets <q_hat, c>), then
+        heaps_to_search_results. probe_b / probe_n: each query's probed
+        buckets (the routing is not part of the oracle)."""
+        q = np.ascontiguousarray(q, np.float32)
+        nq = q.shape[0]
+        sizes = np.asarray(bucket_sizes, np.int64)
+        boff = np.concatenate([[0], np.cumsum(sizes)])
+        G = flat.shape[1]
+        pb = np.asarray(probe_b, np.int32)
+        pn = np.asarray(probe_n, np.int32)
+        off = self.probe_offsets(q, cent, pb, pn) if residual else np.zeros(pb.shape, np.float32)
+        heaps = (np.zeros((nq, k), np.float32), np.zeros((nq, k), np.uint64), np.zeros(nq, np.uint64),
+                 np.full(nq, -np.inf, np.float32), np.zeros(nq, np.uint64))
+        users = {}
+        for i in range(nq):
+            for j in range(int(pn[i])):
+                users.setdefault(int(pb[i, j]), []).append((i, j))
+        for b in sorted(users):
+            n = int(sizes[b])
+            if n == 0:
+                continue
+            rows = slice(int(boff[b]), int(boff[b]) + n)
+            fl = np.ascontiguousarray(flat[rows])
+            perm0 = np.zeros(((n + 191) // 192) * G * 192, np.uint8)
+            self._check(self._port("pack_perm0")(_ptr(fl), _S(n), _S(G), _ptr(perm0)), "pack_perm0")
+            qi = np.array([i for i, _ in users[b]], np.uint64)
+            offs = np.zeros(nq, np.float32)
+            for i, j in users[b]:
+                offs[i] = off[i, j]
+            heaps = self.score_into(q, k, bits, perm0, alpha[rows], qi, heaps, id_map=pos_ids[rows], shift=shift,
+                                    scale=scale, offsets=offs)
+        return self.heaps_to_results(heaps[0], heaps[1], heaps[2])
+
     def heaps_to_results(self, heap_s, heap_i, heap_sz):
         hs = np.ascontiguousarray(heap_s, np.float32)
         hi = np.ascontiguousarray(heap_i, np.uint64)

@@ -45,6 +45,14 @@ class IndexStateError(RuntimeError):
     prepare) and for file / format errors (its vectorcache::Error)."""
 
 
+class BucketParams(ctypes.Structure):
+    """vc_bucket_params."""
+
+    _fields_ = [("n_probe", ctypes.c_size_t), ("max_bucket_size", ctypes.c_size_t), ("energy_soft", ctypes.c_size_t),
+                ("expected_n", ctypes.c_size_t), ("kmeans_iters", ctypes.c_size_t), ("kmeans_sample", ctypes.c_size_t),
+                ("seed", ctypes.c_uint64), ("residual", ctypes.c_int)]
+
+
 class VcexKey(ctypes.Structure):
     """vc_vcex_key: the key of a VCEX exact top-k cache file."""
 
@@ -116,6 +124,33 @@ _SIGS = {
     "vc_lm_index_is_prepared": ([P], I),
     "vc_lm_index_codes_bytes": ([P], S),
     "vc_lm_index_index_bytes": ([P], S),
+    "vc_bucket_params_default": ([ctypes.POINTER(BucketParams)], None),
+    "vc_bucketed_create": ([S, S, ctypes.POINTER(BucketParams), ctypes.POINTER(P)], I),
+    "vc_bucketed_destroy": ([P], I),
+    "vc_bucketed_calibrate": ([P, P, S], I),
+    "vc_bucketed_calibrate_device": ([P, P, S, P], I),
+    "vc_bucketed_add": ([P, P, S], I),
+    "vc_bucketed_add_device": ([P, P, S, P], I),
+    "vc_bucketed_prepare": ([P], I),
+    "vc_bucketed_search": ([P, P, S, S, P, P, ctypes.POINTER(S)], I),
+    "vc_bucketed_search_device": ([P, P, S, S, S, P, P, P, P, ctypes.POINTER(S), P], I),
+    "vc_bucketed_route_device": ([P, P, S, S, P, P, P], I),
+    "vc_bucketed_dim": ([P], S),
+    "vc_bucketed_size": ([P], S),
+    "vc_bucketed_bits": ([P], S),
+    "vc_bucketed_n_buckets": ([P], S),
+    "vc_bucketed_is_prepared": ([P], I),
+    "vc_bucketed_is_calibrated": ([P], I),
+    "vc_bucketed_export": ([P, P, P, P, P, P, P, P], I),
+    "vc_probe_param_count": ([], S),
+    "vc_bucketed_probe_features": ([P, P, S, P, P], I),
+    "vc_bucketed_probe_init": ([P, ctypes.c_uint64], I),
+    "vc_bucketed_probe_set_weights": ([P, P], I),
+    "vc_bucketed_probe_get_weights": ([P, P], I),
+    "vc_bucketed_probe_train": ([P, P, P, S, P, S, F, P, P], I),
+    "vc_bucketed_probe_grads": ([P, P, P, S, P, P], I),
+    "vc_bucketed_probe_predict": ([P, P, S, P, P], I),
+    "vc_bucketed_probe_enable": ([P, I, F], I),
     "vc_set_scan_tile": ([S], I),
     "vc_scan_tile_for": ([S, S, S], S),
     "vc_set_profiling": ([I], I),

@@ -22,11 +22,13 @@
 #include "vc_io.hpp"
 #include "vc_internal.hpp"
 #include "vectorcache_hip.h"
+#include "vc_common.hpp"
 
-namespace {
+namespace vcapi {
 
 thread_local std::string g_last_error;
 std::atomic<int> g_forced_tile{0};
+std::atomic<int> g_debug_flags{0};
 
 // scan-kernel instrumentation (vc_set_profiling / vc_last_scan_stats)
 struct ScanProbe {
@@ -37,86 +39,6 @@ struct ScanProbe {
   int tile = 0, ranges = 0;
 } g_probe;
 
-struct VcError {
-  vc_status status;
-  std::string msg;
-};
-
-[[noreturn]] void fail(vc_status st, const std::string& msg) { throw VcError{st, msg}; }
-
-void hip_check(hipError_t e, const char* what) {
-  if (e == hipSuccess) return;
-  const vc_status st = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? VC_ERR_OOM : VC_ERR_DEVICE;
-  fail(st, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <typename F>
-vc_status guarded(F&& f) {
-  try {
-    f();
-    return VC_OK;
-  } catch (const VcError& e) {
-    g_last_error = e.msg;
-    return e.status;
-  } catch (const std::invalid_argument& e) {
-    g_last_error = e.what();
-    return VC_ERR_INVALID_ARGUMENT;
-  } catch (const std::out_of_range& e) {
-    g_last_error = e.what();
-    return VC_ERR_OUT_OF_RANGE;
-  } catch (const std::bad_alloc&) {
-    g_last_error = "host allocation failed";
-    return VC_ERR_OOM;
-  } catch (const std::exception& e) {
-    g_last_error = e.what();
-    return VC_ERR_DEVICE;
-  }
-}
-
-void check_dim_bits(size_t dim, size_t bits) {
-  if (dim == 0 || dim % 8 != 0) fail(VC_ERR_INVALID_ARGUMENT, "dim must be a positive multiple of 8");
-  if (dim > vchip::kMaxDim) fail(VC_ERR_INVALID_ARGUMENT, "dim exceeds the maximum of 16384");
-  if (bits < 2 || bits > 4) fail(VC_ERR_INVALID_ARGUMENT, "bits must be 2, 3, or 4");
-}
-
-// Growable device allocation.
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  void reserve(size_t need, hipStream_t s, bool keep = false) {
-    if (need <= bytes) return;
-    size_t nb = std::max(need, bytes + bytes / 2);
-    void* q = nullptr;
-    hip_check(hipMalloc(&q, nb), "hipMalloc");
-    if (keep && p != nullptr && bytes > 0) hip_check(hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, s), "copy");
-    if (p != nullptr) {
-      hip_check(hipStreamSynchronize(s), "sync");
-      hip_check(hipFree(p), "hipFree");
-    }
-    p = q;
-    bytes = nb;
-  }
-  void release() {
-    if (p != nullptr) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  template <typename T>
-  T* as(size_t byte_offset = 0) const {
-    return reinterpret_cast<T*>(static_cast<char*>(p) + byte_offset);
-  }
-};
-
-// Carves one scratch allocation into aligned sub-buffers.
-struct Carve {
-  size_t off = 0;
-  size_t take(size_t bytes) {
-    const size_t at = off;
-    off = (off + bytes + 255) & ~size_t(255);
-    return at;
-  }
-};
-
 // Device copies of the rotation tables, per (device, dim).
 struct DevRotation {
   uint32_t* inv_perm = nullptr;
@@ -124,8 +46,6 @@ struct DevRotation {
   uint32_t* perm_sign = nullptr;  // inv_perm[j] | sign-of-destination << 31
   uint32_t* perm_gather = nullptr;  // perm[i] | sign[i] << 31 (by destination)
 };
-std::atomic<int> g_debug_flags{0};
-
 std::mutex g_rot_mu;
 std::map<std::pair<int, size_t>, DevRotation> g_dev_rot;
 
@@ -220,33 +140,10 @@ void calibrate_device(const vchip::CodeTables& t, const float* sample_dev, size_
             "calibrate");
 }
 
-// Scan a tiled code set: queries (device) -> results (device).
-struct SearchOut {
-  size_t k_eff;
-};
-
-// Scratch of a search: a growable device buffer, or (when `fixed` is set) a
-// region the caller owns of `fixed_cap` bytes, which is never grown.
-struct ScratchArena {
-  DevBuf* grow = nullptr;
-  uint8_t* fixed = nullptr;
-  size_t fixed_cap = 0;
-  uint8_t* take(size_t bytes, hipStream_t s) {
-    if (fixed != nullptr) {
-      if (bytes > fixed_cap)
-        fail(VC_ERR_INVALID_ARGUMENT, "search needs " + std::to_string(bytes) + " bytes of scratch; the fixed scratch "
-                                      "holds " + std::to_string(fixed_cap) + " (fewer queries or a smaller k)");
-      return fixed;
-    }
-    grow->reserve(bytes, s);
-    return grow->as<uint8_t>();
-  }
-};
-
 SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const float* alpha, size_t n,
                        const float* q_dev, size_t nq, size_t k, const uint64_t* id_map_dev, float* out_s,
-                       uint64_t* out_i, DevBuf& scratch, hipStream_t s, uint8_t** luts_out = nullptr,
-                       size_t* luts_off = nullptr, ScratchArena* arena = nullptr) {
+                       uint64_t* out_i, DevBuf& scratch, hipStream_t s, uint8_t** luts_out, size_t* luts_off,
+                       ScratchArena* arena) {
   SearchOut r{std::min(k, n)};
   if (nq == 0 || r.k_eff == 0) return r;
   if (r.k_eff > size_t(vchip::kMaxTopK))
@@ -302,7 +199,9 @@ SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const f
   return r;
 }
 
-}  // namespace
+}  // namespace vcapi
+
+using namespace vcapi;
 
 // ---------------------------------------------------------------------------
 struct vc_index {

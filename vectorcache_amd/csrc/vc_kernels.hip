@@ -13,6 +13,7 @@
 #include <utility>
 
 #include "vc_internal.hpp"
+#include "vc_ivf.hpp"
 
 namespace vchip {
 namespace {
@@ -117,6 +118,26 @@ __device__ float norm8(const float* __restrict__ xr, int D, int chain, bool acti
   c = c + __shfl_xor(c, 2);
   c = c + __shfl_xor(c, 4);
   return sqrt_rn(c);
+}
+
+// Inverse row norms as the encode computes them (8 fused chains, tree,
+// correctly rounded sqrt; 0 below 1e-10), and optionally the normalized rows
+// x * inv. Eight lanes per row.
+__global__ __launch_bounds__(256) void row_inv_norms_kernel(const float* __restrict__ x, long n, int D,
+                                                            float* __restrict__ inv_out, float* __restrict__ qhat) {
+  const int chain = threadIdx.x & 7;
+  for (long r0 = (blockIdx.x * (long)blockDim.x + threadIdx.x) >> 3; r0 < (n + 7) / 8 * 8;
+       r0 += ((long)gridDim.x * blockDim.x) >> 3) {
+    const bool active = r0 < n;  // whole groups of eight lanes share a row
+    const long r = active ? r0 : 0;
+    const float nrm = norm8(x + r * D, D, chain, active);
+    const float inv = nrm > kMinNorm ? div_rn(1.f, nrm) : 0.f;
+    if (active) {
+      if (chain == 0) inv_out[r] = inv;
+      if (qhat != nullptr)
+        for (int d = chain; d < D; d += 8) qhat[r * D + d] = x[r * D + d] * inv;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1224,10 +1245,13 @@ __device__ __forceinline__ void wave_sort_desc(float& s, uint32_t& id, int lane)
 // workgroup's top-k lists: strict "better than the current k-th" test,
 // candidate buffer, merge when half full, retry on overflow. Every thread of
 // the workgroup calls it (it synchronizes).
+// voff (when not null): a per-vector offset added after alpha, the reference's
+// score_offsets of score_prepared_into (src/search/search_avx2.cpp:160-214:
+// fa * alpha, then + offset, each rounded).
 template <int T, int V>
 __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], const long (&vv)[V],
                              const bool (&valid)[V], const float* __restrict__ alpha, long q0, long nq,
-                             int k, int cap) {
+                             int k, int cap, const float (*voff)[V] = nullptr) {
   const int lane = threadIdx.x & 63;
   const int kw = min(k, 64);
 #pragma unroll
@@ -1239,7 +1263,7 @@ __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], cons
     uint32_t pend = 0u;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-      s[t] = valid[j] ? fa[j][t] * av + 0.f : -INFINITY;
+      s[t] = valid[j] ? fa[j][t] * av + (voff != nullptr ? (*voff)[j] : 0.f) : -INFINITY;
       sid[t] = vid;
       // below the bound: worse than k vectors already offered (see below)
       const bool cand = valid[j] && q0 + t < nq && better(s[t], vid, sh.thr_s[t], sh.thr_i[t]) &&
@@ -2030,6 +2054,152 @@ __global__ __launch_bounds__(1024) void merge_ranges_lds_kernel(const float* __r
     __syncthreads();
     if (threadIdx.x < 64)
       merge_query(ls, li, lc, R, k, k_out, id_map, out_s + q * k_out, out_i + q * k_out, threadIdx.x);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Bucketed scan (the multi-bucket plumbing of score_prepared_into, src/search/
+// search.cpp:1156-1267, driven over many buckets per query in one launch).
+// Codes are ordered by bucket: bucket b holds positions [boff[b], boff[b] +
+// bsz[b]). Query q probes the first probe_n[q] buckets of its probe list
+// probe_b[q][0 .. P), each with the score offset probe_off[q][j] (the
+// query's dot product with the bucket's encode centroid); probe_pre[q][j] is
+// the number of vectors in the list before entry j. The probed vectors of q
+// are cut into work items of item_len[q] consecutive list vectors; item_off
+// [q] .. item_off[q + 1] are q's items (item_off[nq] = all items). Every
+// vector scores fa * alpha + offset exactly as the reference's AVX2 path
+// scores a bucket; each item keeps its exact top-k by (score desc, position
+// asc), and merge_items_kernel merges a query's items and maps positions to
+// ids.
+// ---------------------------------------------------------------------------
+
+constexpr int kBucketScanGrid = 512;  // persistent workgroups (two per CU)
+
+template <int V>
+__global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void bucket_scan_kernel(BucketScanParams a) {
+  constexpr int T = 1;
+  constexpr int CG = kPairTableBytes / 256;  // groups per table chunk
+  constexpr int CB = CG / 16;
+  constexpr int NR = V == 1 ? 4 : 2;
+  static_assert(CB % NR == 0, "a chunk holds whole rounds of the code-block ring");
+  __shared__ __align__(16) uint8_t table[kPairTableBytes];
+  __shared__ __align__(16) uint8_t stage[kPairStageBytes];
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x;
+  const int G16 = a.G16;
+  const int n_chunks = (G16 + CB - 1) / CB;
+  const size_t tk = (size_t)T * (2 * (size_t)a.k + 2 * (size_t)a.cap) * 4 + 28 * T + 64;
+  float* scl = reinterpret_cast<float*>(smem + (size_t)T * (2 * (size_t)a.k + 2 * (size_t)a.cap) * 4 + 20 * T);
+  float* bias = scl + T;
+  int* l_pre = reinterpret_cast<int*>(smem + ((tk + 15) & ~size_t(15)));  // [P + 1]
+  long* l_pos = reinterpret_cast<long*>(l_pre + ((a.P + 2) & ~1));        // [P] first position of entry j
+  float* l_off = reinterpret_cast<float*>(l_pos + a.P);                     // [P]
+  const int n_items = __builtin_amdgcn_readfirstlane(a.item_off[a.nq]);
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    // query of the item: last q with item_off[q] <= it (uniform)
+    int lo = 0, hi = (int)a.nq;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (a.item_off[mid] <= it) lo = mid; else hi = mid;
+    }
+    const int q = __builtin_amdgcn_readfirstlane(lo);
+    const int np = __builtin_amdgcn_readfirstlane(a.probe_n[q]);
+    const int* pre_g = a.probe_pre + (long)q * (a.P + 1);
+    const int total = __builtin_amdgcn_readfirstlane(pre_g[np]);
+    const int len = __builtin_amdgcn_readfirstlane(a.item_len[q]);
+    const int i0 = (it - a.item_off[q]) * len;
+    const int i1 = min(total, i0 + len);
+    __syncthreads();  // the previous item's lists are written out
+    TopkShared sh;
+    init_topk(sh, smem, T, a.k, a.cap);
+    for (int j = tid; j <= np; j += blockDim.x) l_pre[j] = pre_g[j];
+    for (int j = tid; j < np; j += blockDim.x) {
+      l_pos[j] = a.boff[a.probe_b[(long)q * a.P + j]];
+      l_off[j] = a.probe_off[(long)q * a.P + j];
+    }
+    if (tid == 0) {
+      scl[0] = a.lut_scale[q];
+      bias[0] = a.bias_total[q];
+    }
+    const uint8_t* q_src = a.src + (long)q * ((long)n_chunks * kPairStageBytes);
+    reinterpret_cast<uint64_t*>(stage)[tid] = reinterpret_cast<const uint64_t*>(q_src)[tid];
+    __syncthreads();
+    bool built = false;
+    const int step = kScanThreads * V;
+    for (int base = i0; base < i1; base += step) {
+      long vv[V];
+      bool valid[V];
+      float voff[V];
+      const uint4* cp[V];
+      float fa[V][T];
+      uint32_t acc[V][1], acc2[V][1];
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        const int i = base + tid + j * kScanThreads;
+        valid[j] = i < i1;
+        // list entry of local vector i: last e with l_pre[e] <= i
+        int e0 = 0, e1 = np;
+        while (e1 - e0 > 1) {
+          const int mid = (e0 + e1) >> 1;
+          if (l_pre[mid] <= i) e0 = mid; else e1 = mid;
+        }
+        vv[j] = valid[j] ? l_pos[e0] + (i - l_pre[e0]) : 0;
+        voff[j] = valid[j] ? l_off[e0] : 0.f;
+        cp[j] = reinterpret_cast<const uint4*>(a.tiled) + (vv[j] >> 6) * (long)G16 * 64 + (vv[j] & 63);
+        fa[j][0] = bias[0];
+        acc[j][0] = 0u;
+        acc2[j][0] = 0u;
+      }
+      uint4 ring[NR][V];
+#pragma unroll
+      for (int u = 0; u < NR; ++u)
+#pragma unroll
+        for (int j = 0; j < V; ++j) ring[u][j] = cp[j][(long)min(u, G16 - 1) * 64];
+      for (int c = 0; c < n_chunks; ++c) {
+        const uint8_t* dma = nullptr;
+        if (n_chunks > 1 || !built) {
+          __syncthreads();
+          build_pair_chunk<T>(stage, table);
+          __syncthreads();
+          if (n_chunks > 1 && (c + 1 < n_chunks || base + step < i1))
+            dma = q_src + (long)(c + 1 < n_chunks ? c + 1 : 0) * kPairStageBytes;
+          built = true;
+        }
+        pair_chunk<T, V, NR>(table, stage, dma, scl, cp, c, G16, ring, acc, acc2, fa);
+      }
+      admit_scores<T, V>(sh, fa, vv, valid, a.alpha, 0, 1, a.k, a.cap, &voff);
+    }
+    __syncthreads();
+    if (__builtin_amdgcn_readfirstlane(sh.cnt[0]) > 0) topk_merge(sh, 0, a.k, a.cap);
+    const int m = sh.m[0];
+    for (int j = tid; j < m; j += blockDim.x) {
+      a.part_s[(long)it * a.k + j] = sh.top_s[j];
+      a.part_i[(long)it * a.k + j] = sh.top_i[j];
+    }
+    if (tid == 0) a.part_cnt[it] = m;
+  }
+}
+
+// One wavefront per query: merge its items' lists (at most 512) into k_out
+// entries, positions mapped through id_map; counts[q] (when set) = entries
+// found, the rest of the row (0, 0).
+__global__ __launch_bounds__(64) void merge_items_kernel(const float* __restrict__ part_s,
+                                                         const uint32_t* __restrict__ part_i,
+                                                         const int* __restrict__ part_cnt,
+                                                         const int* __restrict__ item_off, long nq, int k,
+                                                         int k_out, const uint64_t* __restrict__ id_map,
+                                                         float* __restrict__ out_s, uint64_t* __restrict__ out_i,
+                                                         uint64_t* __restrict__ counts) {
+  for (long q = blockIdx.x; q < nq; q += gridDim.x) {
+    const int f = item_off[q], R = item_off[q + 1] - f;
+    merge_query(part_s + (long)f * k, part_i + (long)f * k, part_cnt + f, R, k, k_out, id_map, out_s + q * k_out,
+                out_i + q * k_out, threadIdx.x);
+    if (counts != nullptr) {
+      int c = 0;
+      for (int r = threadIdx.x; r < R; r += 64) c += part_cnt[f + r];
+      for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+      if (threadIdx.x == 0) counts[q] = (uint64_t)min(c, k_out);
+    }
   }
 }
 
@@ -3316,6 +3486,49 @@ hipError_t launch_lm_query(const CodeTables* rot, const float* q_rot, std::size_
   hipLaunchKernelGGL(lm_query_kernel, dim3((unsigned)std::min<size_t>(nq, 16384)), dim3(kLmQueryThreads), lds, s, t,
                      raw_R, q_rot,
                      (long)nq, dim, dim_pad, codes_t, lscales, m, q_land, stride, q_perp);
+  return hipGetLastError();
+}
+
+
+hipError_t launch_row_inv_norms(const float* x, std::size_t n, int D, float* inv, float* qhat, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(row_inv_norms_kernel, dim3(grid_for((long)(n + 31) / 32, 1, 16384)), dim3(256), 0, s, x, (long)n,
+                     D, inv, qhat);
+  return hipGetLastError();
+}
+
+std::size_t bucket_src_bytes(int groups16, std::size_t nq) {
+  return nq * (std::size_t)pair_src_blocks(1, groups16) * 16 * 32;
+}
+
+hipError_t launch_bucket_scan(BucketScanParams p, const std::uint8_t* luts, int groups, std::uint8_t* src,
+                              hipStream_t s) {
+  if (p.nq == 0) return hipSuccess;
+  const long elems = p.nq * pair_src_blocks(1, p.G16) * 16 * 32;
+  hipLaunchKernelGGL(pair_source_kernel<1>, dim3(grid_for(elems, 256)), dim3(256), 0, s, luts, p.nq, groups, p.G16,
+                     src);
+  p.src = src;
+  // LDS: 72 KiB of table and stage, the top-k state and the probe list; the
+  // candidate buffer takes what is left of half a CU (two workgroups per CU)
+  const size_t probe = (size_t)4 * ((p.P + 2) & ~1) + 12 * (size_t)p.P;
+  const long room = (long)kLdsBudget / 2 - (long)(kPairTableBytes + kPairStageBytes) - (long)topk_lds(1, p.k, 0) -
+                    (long)probe - 16;
+  int cap = (int)std::min<long>(kCandCap, room / 8 / 32 * 32);
+  if (cap < 64) cap = 64;  // one workgroup per CU then
+  p.cap = cap;
+  const size_t dyn = ((topk_lds(1, p.k, cap) + 15) & ~size_t(15)) + probe;
+  if (dyn + kPairTableBytes + kPairStageBytes > (size_t)kLdsBudget) return hipErrorInvalidValue;
+  (void)hipFuncSetAttribute((const void*)bucket_scan_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  hipLaunchKernelGGL(bucket_scan_kernel<1>, dim3(kBucketScanGrid), dim3(kScanThreads), dyn, s, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge_items(const float* part_s, const std::uint32_t* part_i, const int* part_cnt,
+                              const int* item_off, std::size_t nq, int k, int k_out, const std::uint64_t* id_map,
+                              float* out_s, std::uint64_t* out_i, std::uint64_t* counts, hipStream_t s) {
+  if (nq == 0 || k_out == 0) return hipSuccess;
+  hipLaunchKernelGGL(merge_items_kernel, dim3(grid_for((long)nq, 1, 65536)), dim3(64), 0, s, part_s, part_i,
+                     part_cnt, item_off, (long)nq, k, k_out, id_map, out_s, out_i, counts);
   return hipGetLastError();
 }
 

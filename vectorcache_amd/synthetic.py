@@ -9,6 +9,12 @@ torch variants, which generate on the device.
   (the 100k x 128 flat workload: 256 centers, sigma 0.25, not normalized).
 * ``anisotropic``: x_d ~ N(0, (d+1)^-0.8), then L2-normalized
   (the 1M x 1536 embedding-scale workload).
+* ``zipf_mixture``: centers uniform on the unit sphere, cluster sizes by a
+  Zipf rank law (cluster r drawn with probability r^-s / H), isotropic noise
+  of RMS norm ``sigma`` (each coordinate sigma / sqrt(dim)): the bucketed
+  workload (10M x 768, 20,000 centers, s = 1.1, sigma 0.2). With a
+  per-coordinate sigma of 0.2 the 768-d noise would have norm 5.5 against
+  unit-norm centers and no bucket structure would survive.
 """
 from __future__ import annotations
 
@@ -27,6 +33,19 @@ def mixture(n: int, dim: int, n_centers: int, sigma: float, center_seed: int, se
     r = np.random.default_rng(seed)
     z = r.integers(0, n_centers, n)
     return (c[z] + sigma * r.standard_normal((n, dim))).astype(np.float32)
+
+
+def zipf_weights(n_centers: int, s: float) -> np.ndarray:
+    w = np.arange(1, n_centers + 1, dtype=np.float64) ** -s
+    return w / w.sum()
+
+
+def zipf_mixture(n: int, dim: int, n_centers: int, zipf_s: float, sigma: float, center_seed: int,
+                 seed: int) -> np.ndarray:
+    c = sphere_centers(n_centers, dim, center_seed)
+    r = np.random.default_rng(seed)
+    z = r.choice(n_centers, size=n, p=zipf_weights(n_centers, zipf_s))
+    return (c[z] + (sigma / np.sqrt(dim)) * r.standard_normal((n, dim))).astype(np.float32)
 
 
 def anisotropic(n: int, dim: int, seed: int, normalize: bool = True) -> np.ndarray:
@@ -59,4 +78,25 @@ def anisotropic_torch(n: int, dim: int, seed: int, device="cuda", chunk: int = 1
         e = min(n, s + chunk)
         x = torch.randn((e - s, dim), generator=g, device=device, dtype=torch.float32) * std
         out[s:e] = x / x.norm(dim=1, keepdim=True)
+    return out
+
+
+def zipf_mixture_torch(n: int, dim: int, n_centers: int, zipf_s: float, sigma: float, center_seed: int, seed: int,
+                       device="cuda", chunk: int = 1 << 18):
+    """zipf_mixture generated on the device (torch generators: the same seeds
+    give the same rows on every run with this torch, not the numpy rows)."""
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(center_seed)
+    c = torch.randn((n_centers, dim), generator=g, device=device, dtype=torch.float32)
+    c = c / c.norm(dim=1, keepdim=True)
+    w = torch.from_numpy(zipf_weights(n_centers, zipf_s)).to(device=device, dtype=torch.float64)
+    g.manual_seed(seed)
+    out = torch.empty((n, dim), dtype=torch.float32, device=device)
+    scale = sigma / float(np.sqrt(dim))
+    for s0 in range(0, n, chunk):
+        e = min(n, s0 + chunk)
+        z = torch.multinomial(w, e - s0, replacement=True, generator=g)
+        out[s0:e] = c[z] + scale * torch.randn((e - s0, dim), generator=g, device=device, dtype=torch.float32)
     return out
