@@ -18,6 +18,7 @@ which also checks the GPU's ids/scores on that sample.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -165,12 +166,310 @@ def committed_traffic(workload, nq, tile):
     return d["traffic_bytes_per_launch"], str(p.relative_to(ROOT))
 
 
+BUCKETED_WORKLOADS = {
+    # name: (n, dim, bits, n_centers, zipf_s, sigma, n_train, n_val, n_eval, k, max_bucket_size, energy_soft)
+    "bucketed10m": (10_000_000, 768, 4, 20_000, 1.1, 0.2, 180_000, 20_000, 10_000, 10, 800, 400),
+    "bucketed1m": (1_000_000, 768, 4, 2_000, 1.1, 0.2, 45_000, 5_000, 5_000, 10, 800, 400),
+}
+RECALL_TARGETS = (0.95, 0.9, 0.85, 0.8, 0.75, 0.7, 0.6, 0.5)
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def chunked(n, size):
+    for s in range(0, n, size):
+        yield s, min(n, s + size)
+
+
+def probe_ranks(ix, q, judge_ids, bucket_of_row, P, chunk=10_000):
+    """Rank, in each query's routing order, of the bucket holding each of its
+    judge top-k rows (P = not among its P best buckets)."""
+    import torch
+
+    out = torch.empty(judge_ids.shape, dtype=torch.int32, device=q.device)
+    for s, e in chunked(q.shape[0], chunk):
+        pb, _ = ix.route_device(q[s:e], P)
+        bo = bucket_of_row[judge_ids[s:e]]
+        hit = pb.unsqueeze(1) == bo.unsqueeze(2)
+        r = torch.argmax(hit.to(torch.int8), dim=2).to(torch.int32)
+        out[s:e] = torch.where(hit.any(dim=2), r, torch.full_like(r, P))
+    return out
+
+
+def recall_at(ranks, n_probe):
+    """Mean recall@k against the judge when query i opens n_probe[i] buckets."""
+    import torch
+
+    n = n_probe if torch.is_tensor(n_probe) else torch.full((ranks.shape[0],), int(n_probe), device=ranks.device)
+    return float((ranks < n.view(-1, 1)).float().mean())
+
+
+def learned_counts(y, shift, valid):
+    import torch
+
+    n = torch.clamp(torch.ceil(torch.exp2(torch.clamp(y + shift, max=30.0))), min=1)
+    return torch.minimum(n, valid.to(n.dtype)).to(torch.int32)
+
+
+def run_bucketed(args):
+    """The bucketed index with the learned probe budget: QPS at a recall@10
+    target against the plain flat scan, learned vs fixed n_probe."""
+    import torch
+
+    from vectorcache_amd import synthetic
+    from vectorcache_amd.bucketed import BucketedTurboQuantIndex, BucketParams
+    import vectorcache_amd.index as vc
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    n, dim, bits, nc, zs, sigma, n_tr, n_val, n_ev, k, mbs, es = BUCKETED_WORKLOADS[args.workload]
+    G = dim // (8 // bits)
+    log(f"workload {args.workload}: n={n} dim={dim} bits={bits} centers={nc} zipf={zs} sigma={sigma} "
+        f"train={n_tr} val={n_val} eval={n_ev} k={k}")
+    t0 = time.perf_counter()
+    x = synthetic.zipf_mixture_torch(n, dim, nc, zs, sigma, 4001, 4002, dev)
+    q_tr = synthetic.zipf_mixture_torch(n_tr + n_val, dim, nc, zs, sigma, 4001, 4003, dev)
+    q_ev = synthetic.zipf_mixture_torch(n_ev, dim, nc, zs, sigma, 4001, 4004, dev)
+    torch.cuda.synchronize()
+    log(f"data generated in {time.perf_counter() - t0:.1f} s")
+
+    ix = BucketedTurboQuantIndex(dim, bits, BucketParams(n_probe=64, max_bucket_size=mbs, energy_soft=es,
+                                                         residual=False, seed=4001))
+    ix.add(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ix.prepare()
+    build_s = time.perf_counter() - t0
+    L = ix.n_buckets()
+    st = ix.export()
+    sizes = torch.from_numpy(st["bucket_sizes"].astype(np.int64)).to(dev)
+    pos_ids = torch.from_numpy(st["pos_ids"].astype(np.int64)).to(dev)
+    bucket_of_row = torch.empty(n, dtype=torch.int32, device=dev)
+    bucket_of_row[pos_ids] = torch.repeat_interleave(torch.arange(L, device=dev, dtype=torch.int32), sizes)
+    log(f"build {build_s:.1f} s ({n / build_s / 1e6:.2f} M vectors/s): {L} buckets, sizes "
+        f"{int(sizes.min())}..{int(sizes.max())} mean {float(sizes.float().mean()):.0f}")
+
+    # ---- the judge: the plain flat scan over every code, then routing ranks ----
+    P = min(512, L)
+    t0 = time.perf_counter()
+    q_all = torch.cat([q_tr, q_ev])
+    judge = torch.empty((q_all.shape[0], k), dtype=torch.int64, device=dev)
+    for s, e in chunked(q_all.shape[0], 10_000):
+        _, i = ix.search_exhaustive_device(q_all[s:e], k)
+        judge[s:e] = i
+    torch.cuda.synchronize()
+    judge_s = time.perf_counter() - t0
+    log(f"judge: flat scan of {q_all.shape[0]} queries in {judge_s:.1f} s ({q_all.shape[0] / judge_s:.0f} QPS)")
+    ranks = probe_ranks(ix, q_all, judge, bucket_of_row, P)
+    labels = ranks.max(dim=1).values + 1  # buckets needed for the whole judge top-k (P + 1: not within P)
+    feats = torch.cat([ix.probe_features(q_all[s:e]) for s, e in chunked(q_all.shape[0], 10_000)])
+    n_tv = n_tr + n_val
+    tr, va, ev = slice(0, n_tr), slice(n_tr, n_tv), slice(n_tv, n_tv + n_ev)
+    lab_np = labels.cpu().numpy()
+    log("labels (buckets for recall 1.0) quantiles 10/50/90/99: "
+        f"{np.percentile(lab_np[tr], [10, 50, 90, 99]).tolist()}")
+
+    # ---- train the probe model on the GPU ----
+    targets = torch.log2(torch.clamp(labels, 1, 512).float())
+    ix.probe_init(4005)
+    rng = np.random.default_rng(4006)
+    epochs = 8
+    t0 = time.perf_counter()
+    loss_first = loss_last = None
+    for ep in range(epochs):
+        order = torch.from_numpy(rng.permutation(n_tr).astype(np.int32)[: n_tr // 64 * 64]).to(dev)
+        losses = ix.probe_train(feats[tr], targets[tr], order, lr=1e-3 if ep < 6 else 3e-4)
+        lc = losses.cpu().numpy()
+        loss_first = float(lc[:50].mean()) if loss_first is None else loss_first
+        loss_last = float(lc[-200:].mean())
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log(f"probe model: {epochs} epochs of {n_tr} queries in {train_s:.1f} s, loss {loss_first:.3f} -> {loss_last:.3f}")
+
+    # ---- recall target, fixed n_probe and the learned shift, chosen on the validation split ----
+    valid = torch.clamp(torch.full((q_all.shape[0],), min(256, L), device=dev), max=P)
+    y = ix.probe_predict(feats)
+    fixed_curve = {p: recall_at(ranks[va], p) for p in range(1, min(256, L) + 1)}
+    target = next((t for t in RECALL_TARGETS if max(fixed_curve.values()) >= t), RECALL_TARGETS[-1])
+    n_fixed = next(p for p, r in fixed_curve.items() if r >= target) if max(fixed_curve.values()) >= target else min(
+        256, L)
+    shift = None
+    for s10 in range(-40, 121):
+        sh = s10 / 20.0
+        if recall_at(ranks[va], learned_counts(y[va], sh, valid[va])) >= target:
+            shift = sh
+            break
+    if shift is None:
+        shift = 6.0
+    ix.probe_enable(True, shift)
+    cnt_ev = learned_counts(y[ev], shift, valid[ev])
+    rec_fixed_ev = recall_at(ranks[ev], n_fixed)
+    rec_learned_ev = recall_at(ranks[ev], cnt_ev)
+    reach = {p: recall_at(ranks[ev], p) for p in (1, 2, 4, 8, 16, 32, 64, 128, 256) if p <= L}
+    log(f"recall target {target}: fixed n_probe {n_fixed} (eval recall {rec_fixed_ev:.4f}), learned shift {shift} "
+        f"mean n_probe {float(cnt_ev.float().mean()):.1f} (eval recall {rec_learned_ev:.4f}); curve {reach}")
+
+    # ---- timed: the eval queries, fixed vs learned ----
+    qe = q_ev.contiguous()
+    stream = torch.cuda.current_stream().cuda_stream
+    vc.set_profiling(True)
+    res = {}
+    for mode, n_probe in (("fixed", n_fixed), ("learned", 0)):
+        sc = torch.empty((n_ev, k), dtype=torch.float32, device=dev)
+        ids = torch.empty((n_ev, k), dtype=torch.int64, device=dev)
+        for _ in range(args.warmup):
+            ix.search_device(qe, k, n_probe=n_probe, scores=sc, ids=ids, stream=stream)
+        torch.cuda.synchronize()
+        scan_ms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ix.search_device(qe, k, n_probe=n_probe, scores=sc, ids=ids, stream=stream)
+            scan_ms.append(vc.last_scan_stats()[0])
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        got = ids.cpu().numpy()
+        jd = judge[ev].cpu().numpy()
+        rec = float(np.mean([len(set(got[i]) & set(jd[i])) / k for i in range(n_ev)]))
+        res[mode] = {"qps": n_ev * args.steps / el, "ms_per_step": el / args.steps * 1e3,
+                     "scan_ms": float(np.mean(scan_ms)), "recall_at_10_vs_flat_scan": rec, "ids": ids}
+        log(f"{mode}: {res[mode]['qps']:.0f} QPS, {res[mode]['ms_per_step']:.2f} ms/step, bucket scan "
+            f"{res[mode]['scan_ms']:.2f} ms, recall@10 vs flat scan {rec:.4f}")
+
+    # vectors the bucket scan reads per pass (algorithmic bytes: codes + alpha of the opened buckets, LUT rows)
+    pb_ev = torch.cat([ix.route_device(qe[s:e], min(256, L))[0] for s, e in chunked(n_ev, 10_000)])
+    csz = torch.cumsum(sizes[pb_ev.long()], dim=1)
+    vec_fixed = int(csz[:, n_fixed - 1].sum())
+    vec_learned = int(torch.gather(csz, 1, (cnt_ev.long() - 1).view(-1, 1)).sum())
+    lut_bytes = n_ev * G * 32
+    bytes_l = vec_learned * (G + 4) + lut_bytes
+    bytes_f = vec_fixed * (G + 4) + lut_bytes
+    ach_l = bytes_l / (res["learned"]["scan_ms"] / 1e3) / 1e9
+    ach_f = bytes_f / (res["fixed"]["scan_ms"] / 1e3) / 1e9
+
+    # ---- quality against FP64 exact, latency ----
+    ns = min(200, n_ev)
+    es_, ei = exact_topk(x, qe[:ns], k)
+    got = res["learned"]["ids"][:ns].cpu().numpy()
+    rec64 = float(np.mean([len(set(got[i]) & set(ei.cpu().numpy()[i])) / k for i in range(ns)]))
+    got_j = judge[ev][:ns].cpu().numpy()
+    rec64_flat = float(np.mean([len(set(got_j[i]) & set(ei.cpu().numpy()[i])) / k for i in range(ns)]))
+    lat = {}
+    if not args.no_latency:
+        for b in (1, 64):
+            ts = []
+            qb = qe[:b].contiguous()
+            for it in range(25):
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                ix.search_device(qb, k, stream=stream)
+                torch.cuda.synchronize()
+                if it >= 5:
+                    ts.append((time.perf_counter() - t1) * 1e3)
+            lat[f"batch{b}"] = {"p50_ms": float(np.percentile(ts, 50)), "p99_ms": float(np.percentile(ts, 99))}
+        log(f"latency (learned) {lat}")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = bucketed_cpu_baseline(st, q_ev[: 64].cpu().numpy(), judge[ev][:64].cpu().numpy(), bits, k, n)
+            log(f"cpu baseline {cpu}")
+        except Exception as e:  # reported, never fatal for the GPU figure
+            cpu = {"error": str(e)}
+
+    out = {
+        "metric": METRIC,
+        "value": res["learned"]["qps"],
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["learned"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded, generated on device)",
+        "config": {
+            "workload": f"{args.workload}: bucketed TurboQuant index, {n} x {dim} float32 from {nc} centers on the "
+                        f"unit sphere, Zipf({zs}) cluster sizes, noise RMS {sigma}, {bits}-bit codes of the rows, "
+                        f"max_bucket_size {mbs}, energy_soft {es}; probe MLP 96-256-256-1 trained on {n_tr} queries; "
+                        f"{n_ev} held-out queries, k={k}",
+            "n": n, "dim": dim, "bits": bits, "k": k, "n_buckets": L, "nq": n_ev,
+            "recall_target": target, "recall_judge": "the plain flat scan over every code (exact LUT top-10)",
+            "n_probe_fixed": n_fixed, "n_probe_learned_mean": float(cnt_ev.float().mean()), "learned_shift": shift,
+            "seeds": {"centers": 4001, "rows": 4002, "train_val_queries": 4003, "eval_queries": 4004},
+        },
+        "fixed": {k2: v for k2, v in res["fixed"].items() if k2 != "ids"},
+        "learned": {k2: v for k2, v in res["learned"].items() if k2 != "ids"},
+        "learned_over_fixed_qps": res["learned"]["qps"] / res["fixed"]["qps"],
+        "eval_recall_from_ranks": {"fixed": rec_fixed_ev, "learned": rec_learned_ev},
+        "recall_curve_fixed_n_probe": reach,
+        "label_quantiles_10_50_90_99": np.percentile(lab_np[ev], [10, 50, 90, 99]).tolist(),
+        "roofline": {
+            "bound": "hbm", "achieved": ach_l, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach_l / HBM_PEAK_GBS,
+            "traffic": None, "kernel": "bucket_scan_kernel (learned probe)", "avg_ms": res["learned"]["scan_ms"],
+            "algorithmic_bytes": bytes_l,
+            "note": f"bytes = vectors of the opened buckets x (G+4) + LUT rows; {vec_learned} vectors learned, "
+                    f"{vec_fixed} fixed (fixed: {ach_f:.0f} GB/s)",
+        },
+        "build": {"seconds": build_s, "vectors_per_s": n / build_s, "judge_seconds": judge_s,
+                  "judge_qps": q_all.shape[0] / judge_s, "train_seconds": train_s, "loss_first": loss_first,
+                  "loss_last": loss_last},
+        "recall_at_10_vs_fp64": {"learned": rec64, "flat_scan": rec64_flat, "queries": ns},
+        "latency": lat,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def bucketed_cpu_baseline(st, q_np, judge_ids, bits, k, n_full, budget_s=12.0):
+    """The reference's own search_flat (oracle/_ref) over all the index's codes
+    (PERM0 blocks of the exported flat codes) on a bounded query sample: the
+    CPU's exhaustive answer to the same question, with every host core."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from oracle import Oracle, available
+
+    kind = "reference" if available("reference") else "port"
+    o = Oracle(kind)
+    port = Oracle("port")
+    cores = os.cpu_count() or 1
+    o.set_threads(cores)
+    flat = st["flat"]
+    G = flat.shape[1]
+    nb = (flat.shape[0] + 191) // 192
+    perm0 = np.zeros(nb * G * 192, np.uint8)
+    port._port("pack_perm0")(flat.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(flat.shape[0]), ctypes.c_size_t(G),
+                             perm0.ctypes.data_as(ctypes.c_void_p))
+    alpha = st["alpha"]
+    t0 = time.perf_counter()
+    s, i = o.search_flat(q_np[:2], k, bits, perm0, alpha)
+    per_q = (time.perf_counter() - t0) / 2
+    m = int(max(2, min(q_np.shape[0], budget_s / max(per_q, 1e-6))))
+    t0 = time.perf_counter()
+    s, i = o.search_flat(q_np[:m], k, bits, perm0, alpha)
+    dt = time.perf_counter() - t0
+    ids = st["pos_ids"][i.astype(np.int64)]
+    return {"value": m / dt, "unit": "queries/s", "cores": cores, "cpu": cpu_model(), "kind": kind,
+            "sample": f"{m} of the held-out queries, exhaustive search_flat over all {n_full} codes "
+                      "(the reference has no bucketed index)",
+            "parity_ids_vs_gpu_flat_scan": bool(np.array_equal(ids, judge_ids[:m].astype(np.uint64)))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="flat1m", choices=sorted(WORKLOADS) + sorted(LANDMARK_WORKLOADS))
+    ap.add_argument("--workload", default="bucketed10m",
+                    choices=sorted(WORKLOADS) + sorted(LANDMARK_WORKLOADS) + sorted(BUCKETED_WORKLOADS))
     ap.add_argument("--tile", type=int, default=0, help="cap the scan query tile (1, 2, 4, 8)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-latency", action="store_true")
@@ -182,6 +481,8 @@ def main():
 
     if args.workload in LANDMARK_WORKLOADS:
         return run_landmark(args)
+    if args.workload in BUCKETED_WORKLOADS:
+        return run_bucketed(args)
 
     import torch
 

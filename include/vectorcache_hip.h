@@ -245,6 +245,13 @@ vc_status vc_bucketed_search(vc_bucketed* index, const float* queries, size_t nq
 vc_status vc_bucketed_search_device(vc_bucketed* index, const float* queries_dev, size_t nq, size_t k, size_t n_probe,
                                     const int32_t* probe_counts_dev, float* scores_dev, uint64_t* ids_dev,
                                     uint64_t* counts_dev, size_t* k_out, void* stream);
+/* residual = 0 only: the plain flat scan (the flat index's byte-pair LUT
+ * scan and exact top-k) over every code of the index, ids by insertion order:
+ * the flat index's search of the normalized rows, bit for bit, and what a
+ * bucketed search with every bucket open returns. The judge of the probe
+ * budget's recall. */
+vc_status vc_bucketed_search_exhaustive_device(vc_bucketed* index, const float* queries_dev, size_t nq, size_t k,
+                                               float* scores_dev, uint64_t* ids_dev, size_t* k_out, void* stream);
 /* Each query's P best buckets by routing score (bucket ids int32, scores f32,
  * nq x P, sorted by (score desc, bucket asc)); 1 <= P <= min(512, n_buckets). */
 vc_status vc_bucketed_route_device(vc_bucketed* index, const float* queries_dev, size_t nq, size_t P,
@@ -404,7 +411,8 @@ vc_status vc_set_scan_tile(size_t tile);
 size_t vc_scan_tile_for(size_t dim, size_t bits, size_t nq);
 
 /* Instrumentation (no reference counterpart): when profiling is on, every
- * search brackets its scan kernel with HIP events on the launch stream.
+ * search brackets its scan kernel (the bucketed index: its bucket scan, tile
+ * 1, ranges 0) with HIP events on the launch stream.
  * vc_last_scan_stats waits for the last such kernel and reports its duration
  * in ms, its query tile and the number of vector ranges (work split) it used. */
 vc_status vc_set_profiling(int on);

@@ -135,6 +135,7 @@ _SIGS = {
     "vc_bucketed_search": ([P, P, S, S, P, P, ctypes.POINTER(S)], I),
     "vc_bucketed_search_device": ([P, P, S, S, S, P, P, P, P, ctypes.POINTER(S), P], I),
     "vc_bucketed_route_device": ([P, P, S, S, P, P, P], I),
+    "vc_bucketed_search_exhaustive_device": ([P, P, S, S, P, P, ctypes.POINTER(S), P], I),
     "vc_bucketed_dim": ([P], S),
     "vc_bucketed_size": ([P], S),
     "vc_bucketed_bits": ([P], S),

@@ -150,6 +150,22 @@ class BucketedTurboQuantIndex:
                                             ctypes.byref(kout), st))
         return scores, ids, counts
 
+    def search_exhaustive_device(self, queries, k: int, stream=None):
+        """residual = False only: the plain flat scan over every code (the
+        judge of the probe budget's recall). Returns (scores f32, ids int64)."""
+        import torch
+
+        q = _cuda_rows(queries, self.dim(), "queries")
+        nq = q.numel() // self.dim()
+        kk = min(int(k), self.size())
+        scores = torch.empty((nq, kk), dtype=torch.float32, device=q.device)
+        ids = torch.empty((nq, kk), dtype=torch.int64, device=q.device)
+        kout = S()
+        check(lib.vc_bucketed_search_exhaustive_device(self._h, P(q.data_ptr()), S(nq), S(k), P(scores.data_ptr()),
+                                                       P(ids.data_ptr()), ctypes.byref(kout),
+                                                       _launch_stream(stream, q)))
+        return scores, ids
+
     def route_device(self, queries, P_: int, stream=None):
         """Each query's P_ best buckets: (bucket ids int32, routing scores f32)."""
         import torch

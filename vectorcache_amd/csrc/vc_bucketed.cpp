@@ -5,7 +5,7 @@
 // ceil(n / energy_soft) lists is trained by spherical k-means on a strided
 // sample and every row is assigned to its nearest list; a list above
 // max_bucket_size is split by one iteration of spherical 2-means until none
-// is. Each bucket's encode centroid is frozen at that point and its rows are
+// is; lists left empty are dropped. Each bucket's encode centroid is frozen at that point and its rows are
 // encoded as residuals x_hat - c (residual mode) or as themselves, with the
 // flat index's encode, into one bucket-ordered code set.
 // Search: the query's routing scores against every bucket (matrix cores)
@@ -346,6 +346,27 @@ void build_buckets(vc_bucketed* ix) {
               "apply split");
     L += nl;
   }
+  // lists the coarse quantizer left empty are dropped (buckets renumbered in order)
+  if (std::find(h.begin(), h.end(), 0) != h.end()) {
+    std::vector<int> map(L, 0), keep;
+    for (int b = 0; b < L; ++b) {
+      map[b] = int(keep.size());
+      if (h[b] > 0) keep.push_back(b);
+    }
+    DevBuf d_map, d_keep, c2;
+    upload(d_map, map, s);
+    upload(d_keep, keep, s);
+    hip_check(vchip::launch_remap(assign.as<int>(), N, d_map.as<int>(), s), "remap buckets");
+    c2.reserve(keep.size() * D * 4, s);
+    hip_check(vchip::launch_gather_rows_f32(ix->cent.as<float>(), int(D), d_keep.as<int>(), keep.size(), c2.as<float>(),
+                                            s),
+              "compact centroids");
+    std::swap(ix->cent, c2);
+    hip_check(hipStreamSynchronize(s), "sync");
+    c2.release();
+    L = int(keep.size());
+    h = group_by_bucket(ix, assign.as<int>(), N, L, perm.as<uint32_t>(), bsize, boff, tmp);
+  }
   ix->L = L;
   ix->h_bsize = h;
   std::swap(ix->bsize, bsize);
@@ -545,7 +566,18 @@ size_t search_probed(vc_bucketed* ix, const float* q_dev, size_t nq, size_t k, i
   p.part_s = b.part_s;
   p.part_i = b.part_i;
   p.part_cnt = b.part_cnt;
+  std::unique_lock probe_lock(g_probe.mu);
+  const bool probe = g_probe.on;
+  if (!probe) probe_lock.unlock();
+  if (probe) hip_check(hipEventRecord(g_probe.start, s), "hipEventRecord");
   hip_check(vchip::launch_bucket_scan(p, b.luts, ix->t.groups, b.src, s), "bucket scan");
+  if (probe) {
+    hip_check(hipEventRecord(g_probe.stop, s), "hipEventRecord");
+    g_probe.recorded = true;
+    g_probe.tile = 1;
+    g_probe.ranges = 0;
+    probe_lock.unlock();
+  }
   hip_check(vchip::launch_merge_items(b.part_s, b.part_i, b.part_cnt, b.item_off, nq, ke, ke, ix->ids.as<uint64_t>(),
                                       out_s, out_i, counts, s),
             "merge items");
@@ -622,6 +654,20 @@ vc_status vc_bucketed_search_device(vc_bucketed* ix, const float* queries_dev, s
     const size_t ke = bucketed_search_dev(ix, queries_dev, nq, k, n_probe, probe_counts_dev, scores_dev, ids_dev,
                                           counts_dev, static_cast<hipStream_t>(stream));
     if (k_out) *k_out = ke;
+  });
+}
+
+vc_status vc_bucketed_search_exhaustive_device(vc_bucketed* ix, const float* queries_dev, size_t nq, size_t k,
+                                               float* scores_dev, uint64_t* ids_dev, size_t* k_out, void* stream) {
+  return guarded([&] {
+    if (ix == nullptr || (queries_dev == nullptr && nq > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    if (!ix->prepared) fail(VC_ERR_STATE, "search() before prepare()");
+    if (ix->prm.residual) fail(VC_ERR_STATE, "the exhaustive flat scan needs codes of the rows (residual = 0)");
+    const auto r = search_tiled(ix->t, ix->tiled.as<uint8_t>(), ix->alpha.as<float>(), ix->n, queries_dev, nq, k,
+                                ix->ids.as<uint64_t>(), scores_dev, ids_dev, ix->scratch,
+                                static_cast<hipStream_t>(stream));
+    if (k_out) *k_out = r.k_eff;
   });
 }
 

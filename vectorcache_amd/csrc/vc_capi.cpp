@@ -30,14 +30,7 @@ thread_local std::string g_last_error;
 std::atomic<int> g_forced_tile{0};
 std::atomic<int> g_debug_flags{0};
 
-// scan-kernel instrumentation (vc_set_profiling / vc_last_scan_stats)
-struct ScanProbe {
-  std::mutex mu;
-  bool on = false;
-  hipEvent_t start = nullptr, stop = nullptr;
-  bool recorded = false;
-  int tile = 0, ranges = 0;
-} g_probe;
+ScanProbe g_probe;
 
 // Device copies of the rotation tables, per (device, dim).
 struct DevRotation {

@@ -9,6 +9,7 @@
 #include <atomic>
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -17,6 +18,19 @@
 #include "vectorcache_hip.h"
 
 namespace vcapi {
+
+// scan-kernel instrumentation (vc_set_profiling / vc_last_scan_stats): when
+// on, the dominant scan kernel of a search is bracketed by HIP events on its
+// launch stream (tile = query tile, ranges = work split; for the bucketed
+// scan, tile 1 and ranges = 0).
+struct ScanProbe {
+  std::mutex mu;
+  bool on = false;
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool recorded = false;
+  int tile = 0, ranges = 0;
+};
+extern ScanProbe g_probe;
 
 extern thread_local std::string g_last_error;
 extern std::atomic<int> g_forced_tile;

@@ -493,11 +493,11 @@ __device__ __forceinline__ float wave_dot(const float* __restrict__ xr, float in
 }
 
 // One split of every listed oversized bucket by one iteration of spherical
-// 2-means: seeds = the first member (lowest row) and the member least
-// similar to it; members go to the nearer seed, each side's normalized mean
-// becomes a child centroid, members go to the nearer child. side[i] (over
-// the bucket's sorted members) = 1 for the second child; a split that leaves
-// a side empty cuts the member list in half instead. One workgroup per
+// 2-means: seeds = the first and the middle member (members in row order);
+// members go to the nearer seed, each side's normalized mean becomes a child
+// centroid, members go to the nearer child. side[i] (over the bucket's sorted
+// members) = 1 for the second child; a split that leaves a side with fewer
+// than n / 8 members cuts the member list in half instead. One workgroup per
 // bucket; cent_a / cent_b receive the children.
 __global__ __launch_bounds__(256) void split_buckets_kernel(const float* __restrict__ x, int D,
                                                             const float* __restrict__ inv,
@@ -512,8 +512,6 @@ __global__ __launch_bounds__(256) void split_buckets_kernel(const float* __restr
   float* s2 = sm + D;
   float* ca = sm + 2 * D;
   float* cb = sm + 3 * D;
-  __shared__ float wbest[4];
-  __shared__ int wbi[4];
   __shared__ int cnt[2];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int li = blockIdx.x; li < n_list; li += gridDim.x) {
@@ -525,30 +523,10 @@ __global__ __launch_bounds__(256) void split_buckets_kernel(const float* __restr
     const uint32_t r1 = rb[0];
     for (int d = threadIdx.x; d < D; d += blockDim.x) s1[d] = x[(long)r1 * D + d] * inv[r1];
     __syncthreads();
-    // second seed: the member least similar to the first (ties: lowest index)
-    float best = INFINITY;
-    int bi = 0;
-    for (int i = w; i < n; i += 4) {
-      const uint32_t r = rb[i];
-      const float s = wave_dot(x + (long)r * D, inv[r], s1, D, lane);
-      if (s < best) {
-        best = s;
-        bi = i;
-      }
-    }
-    if (lane == 0) {
-      wbest[w] = best;
-      wbi[w] = bi;
-    }
-    __syncthreads();
-    int i2 = wbi[0];
-    float b2 = wbest[0];
-    for (int u = 1; u < 4; ++u)
-      if (wbest[u] < b2 || (wbest[u] == b2 && wbi[u] < i2)) {
-        b2 = wbest[u];
-        i2 = wbi[u];
-      }
-    const uint32_t r2 = rb[i2];
+    // second seed: the middle member (members are in row order, so two
+    // members far apart in it; the member least similar to the first would
+    // make an outlier a seed and split off a handful of rows per round)
+    const uint32_t r2 = rb[n / 2];
     for (int d = threadIdx.x; d < D; d += blockDim.x) s2[d] = x[(long)r2 * D + d] * inv[r2];
     __syncthreads();
     for (int i = w; i < n; i += 4) {
@@ -602,7 +580,7 @@ __global__ __launch_bounds__(256) void split_buckets_kernel(const float* __restr
         }
       }
       __syncthreads();
-      if (cnt[0] == 0 || cnt[1] == 0) {
+      if (8 * min(cnt[0], cnt[1]) < n) {  // a side below n / 8: halves instead, so every round halves
         for (int i = threadIdx.x; i < n; i += blockDim.x) sd[i] = i >= n / 2 ? 1 : 0;
         __syncthreads();
       }
@@ -662,6 +640,21 @@ __global__ void scatter_children_kernel(float* __restrict__ cent, int D, const i
 __global__ void widen_ids_kernel(const uint32_t* __restrict__ rows, long n, uint64_t base, uint64_t* __restrict__ ids) {
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
     ids[t] = base + rows[t];
+}
+
+
+// dst[i] = src[rows[i]] (float rows of D).
+__global__ void gather_rows_f32_kernel(const float* __restrict__ src, int D, const int* __restrict__ rows, long n,
+                                       float* __restrict__ dst) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n * D; t += (long)gridDim.x * blockDim.x) {
+    const long i = t / D;
+    dst[t] = src[(long)rows[i] * D + (t - i * D)];
+  }
+}
+
+// v[i] = map[v[i]].
+__global__ void remap_kernel(int* __restrict__ v, long n, const int* __restrict__ map) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) v[t] = map[v[t]];
 }
 
 int grid_of(long work, int threads, int cap = 65536) {
@@ -815,6 +808,20 @@ hipError_t launch_widen_ids(const std::uint32_t* rows, std::size_t n, std::uint6
                             hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(widen_ids_kernel, dim3(grid_of((long)n, 256, 65536)), dim3(256), 0, s, rows, (long)n, base, ids);
+  return hipGetLastError();
+}
+
+
+hipError_t launch_gather_rows_f32(const float* src, int D, const int* rows, std::size_t n, float* dst, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_rows_f32_kernel, dim3(grid_of((long)n * D, 256, 65536)), dim3(256), 0, s, src, D, rows,
+                     (long)n, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_remap(int* v, std::size_t n, const int* map, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(remap_kernel, dim3(grid_of((long)n, 256, 65536)), dim3(256), 0, s, v, (long)n, map);
   return hipGetLastError();
 }
 

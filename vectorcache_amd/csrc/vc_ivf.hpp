@@ -53,6 +53,9 @@ hipError_t launch_scatter_children(float* cent, int D, const int* list, int n_li
 hipError_t launch_widen_ids(const std::uint32_t* rows, std::size_t n, std::uint64_t base, std::uint64_t* ids,
                             hipStream_t s);
 
+hipError_t launch_gather_rows_f32(const float* src, int D, const int* rows, std::size_t n, float* dst, hipStream_t s);
+hipError_t launch_remap(int* v, std::size_t n, const int* map, hipStream_t s);
+
 // ---- vc_kernels.hip ------------------------------------------------------------
 // Row norms as the encode computes them (8 fused chains, tree) -> inv[r] =
 // norm > 1e-10 ? 1 / norm : 0; when qhat is set, qhat = x * inv (n x D).
