@@ -212,11 +212,50 @@ def recall_at(ranks, n_probe):
     return float((ranks < n.view(-1, 1)).float().mean())
 
 
-def learned_counts(y, shift, valid):
+PROBE_GRID = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 160, 192, 224, 256)
+
+
+def fit_probe_table(y, ranks, csz, target, y0=0.0, y_step=0.25, n_bins=40, grid=PROBE_GRID):
+    """Probe counts per bin of the model's prediction y (bin = floor((y - y0) /
+    y_step), clamped) that reach a mean recall of `target` with the fewest
+    scanned vectors on these queries: each step moves the bin whose next
+    larger count (any later grid point) gains the most recall per scanned
+    vector. ranks: routing rank of each judge row's bucket; csz: cumulative
+    vectors of each query's routing list."""
     import torch
 
-    n = torch.clamp(torch.ceil(torch.exp2(torch.clamp(y + shift, max=30.0))), min=1)
-    return torch.minimum(n, valid.to(n.dtype)).to(torch.int32)
+    g = torch.tensor(grid, device=ranks.device)
+    g = g[g <= csz.shape[1]]
+    rec = (ranks.unsqueeze(2) < g.view(1, 1, -1)).float().mean(1)  # [n, G]
+    cost = csz[:, g - 1].double()                                   # [n, G]
+    b = torch.clamp(torch.floor((y - y0) / y_step), 0, n_bins - 1).long()
+    R = torch.zeros((n_bins, g.numel()), dtype=torch.float64, device=ranks.device).index_add_(0, b, rec.double())
+    C = torch.zeros((n_bins, g.numel()), dtype=torch.float64, device=ranks.device).index_add_(0, b, cost)
+    R, C, gl = R.cpu().numpy(), C.cpu().numpy(), g.cpu().numpy()
+    n = ranks.shape[0]
+    at = np.zeros(n_bins, np.int64)
+    total = R[np.arange(n_bins), at].sum()
+    while total < target * n:
+        best, pick = 0.0, None
+        for bi in range(n_bins):
+            for j in range(at[bi] + 1, len(gl)):
+                dr = R[bi, j] - R[bi, at[bi]]
+                dc = C[bi, j] - C[bi, at[bi]]
+                if dr > 0 and dc > 0 and dr / dc > best:
+                    best, pick = dr / dc, (bi, j)
+        if pick is None:
+            break
+        total += R[pick[0], pick[1]] - R[pick[0], at[pick[0]]]
+        at[pick[0]] = pick[1]
+    return gl[at].astype(np.int32)
+
+
+def table_counts(y, table, valid, y0=0.0, y_step=0.25):
+    import torch
+
+    t = torch.from_numpy(table).to(y.device)
+    b = torch.clamp(torch.floor((y - y0) / y_step), 0, len(table) - 1).long()
+    return torch.minimum(t[b], valid.to(torch.int32)).to(torch.int32)
 
 
 def run_bucketed(args):
@@ -294,27 +333,24 @@ def run_bucketed(args):
     train_s = time.perf_counter() - t0
     log(f"probe model: {epochs} epochs of {n_tr} queries in {train_s:.1f} s, loss {loss_first:.3f} -> {loss_last:.3f}")
 
-    # ---- recall target, fixed n_probe and the learned shift, chosen on the validation split ----
+    # ---- recall target, fixed n_probe and the learned policy, chosen on the validation split ----
     valid = torch.clamp(torch.full((q_all.shape[0],), min(256, L), device=dev), max=P)
     y = ix.probe_predict(feats)
     fixed_curve = {p: recall_at(ranks[va], p) for p in range(1, min(256, L) + 1)}
     target = next((t for t in RECALL_TARGETS if max(fixed_curve.values()) >= t), RECALL_TARGETS[-1])
     n_fixed = next(p for p, r in fixed_curve.items() if r >= target) if max(fixed_curve.values()) >= target else min(
         256, L)
-    shift = None
-    for s10 in range(-40, 121):
-        sh = s10 / 20.0
-        if recall_at(ranks[va], learned_counts(y[va], sh, valid[va])) >= target:
-            shift = sh
-            break
-    if shift is None:
-        shift = 6.0
-    ix.probe_enable(True, shift)
-    cnt_ev = learned_counts(y[ev], shift, valid[ev])
+    # the learned policy: the model's prediction picks a bin, each bin's probe count is fitted on validation
+    csz_all = torch.cat([torch.cumsum(sizes[ix.route_device(q_all[s:e], min(256, L))[0].long()], dim=1)
+                         for s, e in chunked(q_all.shape[0], 10_000)])
+    table = fit_probe_table(y[va], ranks[va], csz_all[va], target)
+    ix.probe_set_table(table, 0.0, 0.25)
+    ix.probe_enable(True, 0.0)
+    cnt_ev = table_counts(y[ev], table, valid[ev])
     rec_fixed_ev = recall_at(ranks[ev], n_fixed)
     rec_learned_ev = recall_at(ranks[ev], cnt_ev)
     reach = {p: recall_at(ranks[ev], p) for p in (1, 2, 4, 8, 16, 32, 64, 128, 256) if p <= L}
-    log(f"recall target {target}: fixed n_probe {n_fixed} (eval recall {rec_fixed_ev:.4f}), learned shift {shift} "
+    log(f"recall target {target}: fixed n_probe {n_fixed} (eval recall {rec_fixed_ev:.4f}), learned table {table.tolist()} "
         f"mean n_probe {float(cnt_ev.float().mean()):.1f} (eval recall {rec_learned_ev:.4f}); curve {reach}")
 
     # ---- timed: the eval queries, fixed vs learned ----
@@ -344,8 +380,7 @@ def run_bucketed(args):
             f"{res[mode]['scan_ms']:.2f} ms, recall@10 vs flat scan {rec:.4f}")
 
     # vectors the bucket scan reads per pass (algorithmic bytes: codes + alpha of the opened buckets, LUT rows)
-    pb_ev = torch.cat([ix.route_device(qe[s:e], min(256, L))[0] for s, e in chunked(n_ev, 10_000)])
-    csz = torch.cumsum(sizes[pb_ev.long()], dim=1)
+    csz = csz_all[ev]
     vec_fixed = int(csz[:, n_fixed - 1].sum())
     vec_learned = int(torch.gather(csz, 1, (cnt_ev.long() - 1).view(-1, 1)).sum())
     lut_bytes = n_ev * G * 32
@@ -404,7 +439,8 @@ def run_bucketed(args):
                         f"{n_ev} held-out queries, k={k}",
             "n": n, "dim": dim, "bits": bits, "k": k, "n_buckets": L, "nq": n_ev,
             "recall_target": target, "recall_judge": "the plain flat scan over every code (exact LUT top-10)",
-            "n_probe_fixed": n_fixed, "n_probe_learned_mean": float(cnt_ev.float().mean()), "learned_shift": shift,
+            "n_probe_fixed": n_fixed, "n_probe_learned_mean": float(cnt_ev.float().mean()),
+            "learned_probe_table": {"y0": 0.0, "y_step": 0.25, "counts": table.tolist()},
             "seeds": {"centers": 4001, "rows": 4002, "train_val_queries": 4003, "eval_queries": 4004},
         },
         "fixed": {k2: v for k2, v in res["fixed"].items() if k2 != "ids"},

@@ -293,6 +293,12 @@ vc_status vc_bucketed_probe_train(vc_bucketed* index, const float* feats_dev, co
 vc_status vc_bucketed_probe_grads(vc_bucketed* index, const float* feats_dev, const float* targets_dev, size_t n,
                                   const int32_t* order_dev, float* grads_host);
 vc_status vc_bucketed_probe_predict(vc_bucketed* index, const float* feats_dev, size_t n, float* y_dev, void* stream);
+/* Probe table: a prediction y opens counts[b] buckets, b = clamp(floor((y -
+ * y0) / y_step), 0, n_bins - 1) (n_bins <= 64, counts >= 1; n_bins = 0 goes
+ * back to ceil(2^(y + shift))). The counts are chosen by the caller, e.g. to
+ * meet a mean recall at the least scanned vectors on validation queries. */
+vc_status vc_bucketed_probe_set_table(vc_bucketed* index, const int32_t* counts, size_t n_bins, float y0,
+                                      float y_step);
 /* on = 1: searches with n_probe = 0 use the model with this log2 shift. */
 vc_status vc_bucketed_probe_enable(vc_bucketed* index, int on, float shift);
 

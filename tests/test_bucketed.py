@@ -272,3 +272,30 @@ def test_probe_training_lowers_loss_and_predicts(vc):
     assert last < 0.1 * first, (first, last)
     y = ix.probe_predict(Xd).cpu().numpy()
     assert np.mean((y - t) ** 2) < 0.1
+
+
+def test_probe_table_counts_match_explicit_counts(vc):
+    """A learned search with a probe table opens, per query, the table's count
+    for the bin of the model's prediction: its results equal a search with
+    those counts given explicitly."""
+    import torch
+
+    from vectorcache_amd import synthetic
+
+    x = synthetic.zipf_mixture(8000, 64, 30, 1.1, 0.3, 61, 62)
+    q = synthetic.zipf_mixture(48, 64, 30, 1.1, 0.3, 61, 63)
+    ix = _bucketed(x, 64, 4, energy_soft=100, max_bucket_size=200)
+    ix.probe_init(9)
+    ix.set_probe_weights(ix.probe_weights())  # given weights count as a trained model
+    qd = torch.from_numpy(q).cuda()
+    y = ix.probe_predict(ix.probe_features(qd)).cpu().numpy()
+    table = np.array([1, 2, 3, 5, 8, 13, 21, 34], np.int32)
+    y0, step = float(np.percentile(y, 10)), float((y.max() - y.min()) / 6 + 1e-3)
+    ix.probe_set_table(table, y0, step)
+    ix.probe_enable(True)
+    L = ix.n_buckets()
+    want = np.minimum(table[np.clip(np.floor((y - y0) / step), 0, len(table) - 1).astype(int)], min(256, L))
+    s, i, _ = ix.search_device(qd, 10)
+    s2, i2, _ = ix.search_device(qd, 10, n_probe=256, probe_counts=torch.from_numpy(want.astype(np.int32)).cuda())
+    assert torch.equal(i, i2) and torch.equal(s, s2)
+    assert len(set(want.tolist())) > 2

@@ -152,6 +152,7 @@ _SIGS = {
     "vc_bucketed_probe_grads": ([P, P, P, S, P, P], I),
     "vc_bucketed_probe_predict": ([P, P, S, P, P], I),
     "vc_bucketed_probe_enable": ([P, I, F], I),
+    "vc_bucketed_probe_set_table": ([P, P, S, F, F], I),
     "vc_set_scan_tile": ([S], I),
     "vc_scan_tile_for": ([S, S, S], S),
     "vc_set_profiling": ([I], I),

@@ -252,5 +252,12 @@ class BucketedTurboQuantIndex:
         check(lib.vc_bucketed_probe_predict(self._h, P(f.data_ptr()), S(n), P(y.data_ptr()), _launch_stream(stream, f)))
         return y
 
+    def probe_set_table(self, counts, y0: float, y_step: float):
+        """Prediction y opens counts[clamp(floor((y - y0) / y_step), 0, len - 1)]
+        buckets (empty counts: back to ceil(2^(y + shift)))."""
+        c = np.ascontiguousarray(counts, np.int32)
+        check(lib.vc_bucketed_probe_set_table(self._h, _ptr(c) if c.size else None, S(c.size), ctypes.c_float(y0),
+                                              ctypes.c_float(y_step)))
+
     def probe_enable(self, on: bool = True, shift: float = 0.0):
         check(lib.vc_bucketed_probe_enable(self._h, 1 if on else 0, ctypes.c_float(shift)))

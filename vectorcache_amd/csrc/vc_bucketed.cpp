@@ -831,6 +831,19 @@ vc_status vc_bucketed_probe_predict(vc_bucketed* ix, const float* feats_dev, siz
   });
 }
 
+vc_status vc_bucketed_probe_set_table(vc_bucketed* ix, const int32_t* counts, size_t n_bins, float y0, float y_step) {
+  return guarded([&] {
+    if (ix == nullptr || (counts == nullptr && n_bins > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_bins > size_t(vchip::kProbeMaxBins)) fail(VC_ERR_INVALID_ARGUMENT, "at most 64 bins");
+    if (!(y_step > 0.f)) fail(VC_ERR_INVALID_ARGUMENT, "y_step must be positive");
+    for (size_t b = 0; b < n_bins; ++b)
+      if (counts[b] < 1) fail(VC_ERR_INVALID_ARGUMENT, "probe counts must be at least 1");
+    std::lock_guard lock(ix->mu);
+    require_model(ix);
+    hip_check(vchip::probe_model_set_table(ix->probe, counts, int(n_bins), y0, y_step, ix->stream), "probe table");
+  });
+}
+
 vc_status vc_bucketed_probe_enable(vc_bucketed* ix, int on, float shift) {
   return guarded([&] {
     if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");

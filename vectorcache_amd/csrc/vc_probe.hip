@@ -310,6 +310,7 @@ __global__ __launch_bounds__(256) void probe_predict_kernel(ProbeDev p, const fl
                                                             const float* __restrict__ probe_s,
                                                             const int* __restrict__ probe_b,
                                                             const int* __restrict__ bsize, int P, float shift,
+                                                            int n_bins, float y0, float y_step,
                                                             int* __restrict__ probe_n) {
   __shared__ __align__(16) __bf16 X[kB * kI];
   __shared__ __align__(16) __bf16 H1[kB * kH];
@@ -342,7 +343,13 @@ __global__ __launch_bounds__(256) void probe_predict_kernel(ProbeDev p, const fl
         const int* pb = probe_b + r * P;
         int nv = 0;
         while (nv < P && pb[nv] >= 0) ++nv;
-        const float want = ceilf(exp2f(fminf(out + shift, 30.f)));
+        float want;
+        if (n_bins > 0) {
+          const float fb = floorf((out - y0) / y_step);
+          want = (float)p.table[(int)fminf(fmaxf(fb, 0.f), (float)(n_bins - 1))];
+        } else {
+          want = ceilf(exp2f(fminf(out + shift, 30.f)));
+        }
         probe_n[r] = nv == 0 ? 0 : (int)fminf(fmaxf(want, 1.f), (float)nv);
       }
     }
@@ -390,7 +397,7 @@ uint64_t mix64(uint64_t& s) {
 
 hipError_t probe_model_init(ProbeModel& pm, std::uint64_t seed, hipStream_t s) {
   if (pm.mem == nullptr) {
-    const size_t bytes = (size_t)kProbeParams * 4 * 4 + (size_t)(kH * kI + kH * kH) * 2;
+    const size_t bytes = (size_t)kProbeParams * 4 * 4 + (size_t)(kH * kI + kH * kH) * 2 + kProbeMaxBins * 4;
     hipError_t e = hipMalloc(&pm.mem, bytes);
     if (e != hipSuccess) return e;
     char* b = static_cast<char*>(pm.mem);
@@ -400,7 +407,9 @@ hipError_t probe_model_init(ProbeModel& pm, std::uint64_t seed, hipStream_t s) {
     pm.d.grad = pm.d.v + kProbeParams;
     pm.d.w1b = reinterpret_cast<__bf16*>(pm.d.grad + kProbeParams);
     pm.d.w2b = pm.d.w1b + kH * kI;
+    pm.d.table = reinterpret_cast<int*>(pm.d.w2b + kH * kH);
   }
+  pm.n_bins = 0;
   std::vector<float> w(kProbeParams, 0.f);
   uint64_t st = seed;
   auto uni = [&](float lim) { return (float)(((double)(mix64(st) >> 11) * 0x1.0p-53) * 2.0 - 1.0) * lim; };
@@ -421,6 +430,21 @@ hipError_t probe_model_set(ProbeModel& pm, const float* host_w, hipStream_t s) {
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   return hipStreamSynchronize(s);
+}
+
+hipError_t probe_model_set_table(ProbeModel& pm, const std::int32_t* host_counts, int n_bins, float y0, float y_step,
+                                 hipStream_t s) {
+  if (n_bins < 0 || n_bins > kProbeMaxBins || !(y_step > 0.f)) return hipErrorInvalidValue;
+  if (n_bins > 0) {
+    hipError_t e = hipMemcpyAsync(pm.d.table, host_counts, (size_t)n_bins * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+  }
+  pm.n_bins = n_bins;
+  pm.y0 = y0;
+  pm.y_step = y_step;
+  return hipSuccess;
 }
 
 hipError_t probe_model_get(const ProbeModel& pm, float* host_w, hipStream_t s) {
@@ -453,7 +477,7 @@ hipError_t launch_probe_train(ProbeModel& pm, const float* feats, const float* t
 hipError_t launch_probe_predict(const ProbeModel& pm, const float* feats, std::size_t n, float* y, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(probe_predict_kernel, dim3(grid_n((long)n, kB, 4096)), dim3(256), 0, s, pm.d, feats, (long)n, y,
-                     nullptr, nullptr, nullptr, 0, 0.f, nullptr);
+                     nullptr, nullptr, nullptr, 0, 0.f, 0, 0.f, 1.f, nullptr);
   return hipGetLastError();
 }
 
@@ -461,7 +485,7 @@ hipError_t probe_counts_from_model(const ProbeModel& pm, std::size_t nq, int P, 
                                    const int* probe_b, const int* bsize, int* probe_n, hipStream_t s) {
   if (nq == 0) return hipSuccess;
   hipLaunchKernelGGL(probe_predict_kernel, dim3(grid_n((long)nq, kB, 4096)), dim3(256), 0, s, pm.d, nullptr, (long)nq,
-                     nullptr, probe_s, probe_b, bsize, P, pm.shift, probe_n);
+                     nullptr, probe_s, probe_b, bsize, P, pm.shift, pm.n_bins, pm.y0, pm.y_step, probe_n);
   return hipGetLastError();
 }
 

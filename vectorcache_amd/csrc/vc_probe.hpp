@@ -14,6 +14,7 @@ namespace vchip {
 
 constexpr int kProbeIn = 96, kProbeHid = 256, kProbeBatch = 64;
 constexpr int kProbeMaxP = 256;  // probe counts the model chooses from
+constexpr int kProbeMaxBins = 64;
 // fp32 parameter layout
 constexpr int kPoW1 = 0;                               // [256][96]
 constexpr int kPob1 = kPoW1 + kProbeHid * kProbeIn;    // [256]
@@ -30,6 +31,7 @@ struct ProbeDev {
   __bf16* w1b;    // bf16 copy of W1 [256][96]
   __bf16* w2b;    // bf16 copy of W2 [256][256]
   float* grad;    // kProbeParams, written by the gradient-only step
+  int* table;     // kProbeMaxBins probe counts
 };
 
 struct ProbeModel {
@@ -37,6 +39,11 @@ struct ProbeModel {
   ProbeDev d{};
   bool trained = false;
   float shift = 0.f;  // log2 offset applied to the prediction
+  // probe table (when n_bins > 0): prediction y falls in bin
+  // clamp(floor((y - y0) / y_step), 0, n_bins - 1), which opens table[bin]
+  // buckets (table in device memory, kProbeMaxBins entries at mem's end)
+  int n_bins = 0;
+  float y0 = 0.f, y_step = 1.f;
   long step = 0;      // Adam steps taken
   void release() {
     if (mem != nullptr) (void)hipFree(mem);
@@ -52,6 +59,8 @@ hipError_t probe_model_init(ProbeModel& pm, std::uint64_t seed, hipStream_t s);
 // Uploads / downloads the fp32 master weights (bf16 copies refreshed).
 hipError_t probe_model_set(ProbeModel& pm, const float* host_w, hipStream_t s);
 hipError_t probe_model_get(const ProbeModel& pm, float* host_w, hipStream_t s);
+hipError_t probe_model_set_table(ProbeModel& pm, const std::int32_t* host_counts, int n_bins, float y0, float y_step,
+                                 hipStream_t s);
 
 // features [nq][96] of routing lists (probe_s sorted desc, P >= 1 entries,
 // -inf past the valid ones) and bucket sizes.
@@ -67,8 +76,9 @@ hipError_t launch_probe_train(ProbeModel& pm, const float* feats, const float* t
                               hipStream_t s);
 // y[i] = model(feats[i]).
 hipError_t launch_probe_predict(const ProbeModel& pm, const float* feats, std::size_t n, float* y, hipStream_t s);
-// Routing lists -> features -> model -> probe_n[q] = clamp(ceil(2^(y + shift)),
-// 1, valid entries of q's list), in one kernel.
+// Routing lists -> features -> model -> probe_n[q] = clamp(table[bin(y)] or,
+// without a table, ceil(2^(y + shift)), 1, valid entries of q's list), in one
+// kernel.
 hipError_t probe_counts_from_model(const ProbeModel& pm, std::size_t nq, int P, const float* probe_s,
                                    const int* probe_b, const int* bsize, int* probe_n, hipStream_t s);
 // probe_n[q] = clamp(counts[q], 1, valid entries of q's list), at most P.
