@@ -411,6 +411,15 @@ def run_bucketed(args):
             lat[f"batch{b}"] = {"p50_ms": float(np.percentile(ts, 50)), "p99_ms": float(np.percentile(ts, 99))}
         log(f"latency (learned) {lat}")
 
+    probe = copy_probe(dev)
+    log(f"copy probe {probe}")
+    traffic, traffic_src = None, None
+    tp = ROOT / "profiles" / f"bucket_scan_kernel_{args.workload}_traffic.json"
+    if tp.exists():
+        td = json.loads(tp.read_text())
+        if td.get("nq") == n_ev and td.get("n_buckets") == L:
+            traffic, traffic_src = td["learned_traffic_bytes_per_launch"], str(tp.relative_to(ROOT))
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -451,7 +460,8 @@ def run_bucketed(args):
         "label_quantiles_10_50_90_99": np.percentile(lab_np[ev], [10, 50, 90, 99]).tolist(),
         "roofline": {
             "bound": "hbm", "achieved": ach_l, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach_l / HBM_PEAK_GBS,
-            "traffic": None, "kernel": "bucket_scan_kernel (learned probe)", "avg_ms": res["learned"]["scan_ms"],
+            "traffic": traffic, "traffic_source": traffic_src, "kernel": "bucket_scan_kernel (learned probe)",
+            "avg_ms": res["learned"]["scan_ms"],
             "algorithmic_bytes": bytes_l,
             "note": f"bytes = vectors of the opened buckets x (G+4) + LUT rows; {vec_learned} vectors learned, "
                     f"{vec_fixed} fixed (fixed: {ach_f:.0f} GB/s)",
@@ -461,6 +471,7 @@ def run_bucketed(args):
                   "loss_last": loss_last},
         "recall_at_10_vs_fp64": {"learned": rec64, "flat_scan": rec64_flat, "queries": ns},
         "latency": lat,
+        "copy_probe": probe,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
