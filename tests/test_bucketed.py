@@ -299,3 +299,36 @@ def test_probe_table_counts_match_explicit_counts(vc):
     s2, i2, _ = ix.search_device(qd, 10, n_probe=256, probe_counts=torch.from_numpy(want.astype(np.int32)).cuda())
     assert torch.equal(i, i2) and torch.equal(s, s2)
     assert len(set(want.tolist())) > 2
+
+
+def test_routing_top_p_is_the_best_buckets(vc, oracle):
+    """Routing lists on an index with more buckets than the selection's
+    candidate buffer (the sampled-threshold path): sorted, and their scores
+    agree with fp64 scores of the same bf16-rounded inputs; no bucket left out
+    scores above the last one kept (up to the fp32 accumulation's rounding)."""
+    import probe_oracle as po
+    import torch
+
+    from vectorcache_amd import synthetic
+
+    x = synthetic.zipf_mixture(70000, 64, 500, 1.1, 0.3, 71, 72)
+    q = synthetic.zipf_mixture(40, 64, 500, 1.1, 0.3, 71, 73)
+    q[0] = 0.0  # a zero query: every score ties (the exact fallback)
+    ix = _bucketed(x, 64, 4, energy_soft=25, max_bucket_size=50, kmeans_iters=2)
+    L = ix.n_buckets()
+    assert L > 2048
+    cent = ix.export()["centroids"]
+    qn = np.stack([r * (np.float32(1) / oracle.row_norm(r) if oracle.row_norm(r) > 1e-10 else np.float32(0))
+                   for r in q]).astype(np.float32)
+    ref = po.bf16(qn).astype(np.float64) @ po.bf16(cent).astype(np.float64).T
+    for P in (1, 64, 256, 512):
+        pb, ps = ix.route_device(torch.from_numpy(q).cuda(), P)
+        pb, ps = pb.cpu().numpy(), ps.cpu().numpy()
+        for r in range(40):
+            assert np.all(np.diff(ps[r]) <= 0), (P, r)
+            assert len(set(pb[r].tolist())) == P
+            assert np.allclose(ps[r], ref[r, pb[r]], atol=2e-5), (P, r)
+            rest = np.setdiff1d(np.arange(L), pb[r])
+            assert ref[r, rest].max() <= ps[r, -1] + 2e-5, (P, r)
+        # the zero query: all scores 0, the lowest bucket ids first
+        assert np.array_equal(pb[0], np.arange(P)) and np.all(ps[0] == 0)

@@ -161,12 +161,39 @@ __global__ __launch_bounds__(256) void gemm_argmax_kernel(const __bf16* __restri
 
 // ---------------------------------------------------------------------------
 // Routing top-P per row: the P best columns of a score row by (score desc,
-// column asc), sorted. One workgroup per row: radix select of the P-th key
-// over the row (staged in LDS when it fits), then the rows above it and the
-// lowest-column ties, then a bitonic sort.
+// column asc), sorted. One workgroup per row. A threshold is read off 1024
+// evenly spaced samples of the row (sorted), so that about 1.25 P + 4 N /
+// 1024 entries lie at or above it; those entries are compacted into LDS (if
+// fewer than P or more than the buffer, the threshold moves and the pass
+// repeats) and bitonic-sorted. Every entry above the threshold is a
+// candidate, so the result is the exact top P.
 // ---------------------------------------------------------------------------
 constexpr int kTopThreads = 256;
 constexpr int kTopMaxP = 512;
+constexpr int kTopCand = 2048;   // candidate buffer
+constexpr int kTopSamples = 1024;
+
+__device__ __forceinline__ void bitonic_desc(float* s, int* c, int size, int tid) {
+  for (int kk = 2; kk <= size; kk <<= 1)
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      for (int i = tid; i < size; i += kTopThreads) {
+        const int p = i ^ j;
+        if (p > i) {
+          const bool desc = (i & kk) == 0;
+          const bool pb = better_ci(s[p], c[p], s[i], c[i]);
+          if (desc ? pb : !pb) {
+            const float ts = s[i];
+            const int tc = c[i];
+            s[i] = s[p];
+            c[i] = c[p];
+            s[p] = ts;
+            c[p] = tc;
+          }
+        }
+      }
+      __syncthreads();
+    }
+}
 
 __device__ __forceinline__ uint32_t desc_key(float f) {
   // larger float -> larger key; NaN never wins (mapped to 0)
@@ -176,114 +203,143 @@ __device__ __forceinline__ uint32_t desc_key(float f) {
 }
 
 __global__ __launch_bounds__(kTopThreads) void topk_rows_kernel(const float* __restrict__ scores, long M, int N,
-                                                                long lds_, int P, int stage, int* __restrict__ out_c,
+                                                                long lds_, int P, int* __restrict__ out_c,
                                                                 float* __restrict__ out_s) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int hist[256];
-  __shared__ int sh_digit, sh_need;
-  __shared__ float sel_s[kTopMaxP];
-  __shared__ int sel_c[kTopMaxP];
-  __shared__ int n_sel, scan_buf[kTopThreads];
-  uint32_t* keys = reinterpret_cast<uint32_t*>(smem);
+  __shared__ float cs[kTopCand];
+  __shared__ int cc[kTopCand];
+  __shared__ float smp[kTopSamples];
+  __shared__ int smp_c[kTopSamples];
+  __shared__ int n_cand, hist[256], sh_digit, sh_need, scan_buf[kTopThreads];
   const int tid = threadIdx.x;
+  const int total = min(P, N);
   for (long row = blockIdx.x; row < M; row += gridDim.x) {
     const float* sr = scores + row * lds_;
-    __syncthreads();
-    if (stage)
-      for (int c = tid; c < N; c += kTopThreads) keys[c] = desc_key(sr[c]);
-    auto key_at = [&](int c) -> uint32_t { return stage ? keys[c] : desc_key(sr[c]); };
-    uint32_t prefix = 0u, mask = 0u;
-    int need = min(P, N);
-    for (int shift = 24; shift >= 0; shift -= 8) {
-      for (int d = tid; d < 256; d += kTopThreads) hist[d] = 0;
+    // ---- sampled threshold: at most 12 moves of the sample rank ----
+    bool done = false;
+    if (N <= kTopCand) {
+      __syncthreads();
+      if (tid == 0) n_cand = 0;
       __syncthreads();
       for (int c = tid; c < N; c += kTopThreads) {
-        const uint32_t k = key_at(c);
-        if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255], 1);
-      }
-      __syncthreads();
-      if (tid == 0) {
-        int cum = 0, d = 255;
-        for (; d > 0; --d) {
-          if (cum + hist[d] >= need) break;
-          cum += hist[d];
+        const float v = sr[c];
+        if (v == v) {
+          const int at = atomicAdd(&n_cand, 1);
+          cs[at] = v;
+          cc[at] = c;
         }
-        sh_digit = d;
-        sh_need = need - cum;
       }
       __syncthreads();
-      prefix |= (uint32_t)sh_digit << shift;
-      mask |= 255u << shift;
-      need = sh_need;
+      done = true;  // every finite entry is a candidate
+    } else {
       __syncthreads();
-    }
-    const uint32_t thr = prefix;  // the P-th key; `need` of the keys equal to it are taken
-    if (tid == 0) n_sel = 0;
-    __syncthreads();
-    // keys above the threshold, in any order (sorted below)
-    for (int c = tid; c < N; c += kTopThreads) {
-      const uint32_t k = key_at(c);
-      if (k > thr) {
-        const int at = atomicAdd(&n_sel, 1);
-        sel_s[at] = sr[c];
-        sel_c[at] = c;
+      for (int i = tid; i < kTopSamples; i += kTopThreads) {
+        const float v = sr[(long)i * N / kTopSamples];
+        smp[i] = v == v ? v : -INFINITY;
+        smp_c[i] = i;
       }
-    }
-    // ties at the threshold: the `need` lowest columns (contiguous column
-    // ranges per thread, ranked by a prefix count)
-    const int per = (N + kTopThreads - 1) / kTopThreads;
-    const int c0 = tid * per, c1 = min(N, c0 + per);
-    int mine = 0;
-    for (int c = c0; c < c1; ++c) mine += key_at(c) == thr ? 1 : 0;
-    scan_buf[tid] = mine;
-    __syncthreads();
-    for (int o = 1; o < kTopThreads; o <<= 1) {
-      const int v = tid >= o ? scan_buf[tid - o] : 0;
       __syncthreads();
-      scan_buf[tid] += v;
-      __syncthreads();
-    }
-    int rank = scan_buf[tid] - mine;
-    const int base = n_sel;  // every thread read the count after the barrier above
-    __syncthreads();
-    for (int c = c0; c < c1 && rank < need; ++c)
-      if (key_at(c) == thr) {
-        sel_s[base + rank] = sr[c];
-        sel_c[base + rank] = c;
-        ++rank;
-      }
-    const int total = min(P, N);
-    __syncthreads();
-    // bitonic sort of the selection, padded to a power of two
-    int size = 1;
-    while (size < total) size <<= 1;
-    for (int i = total + tid; i < size; i += kTopThreads) {
-      sel_s[i] = -INFINITY;
-      sel_c[i] = 0x7fffffff;
-    }
-    __syncthreads();
-    for (int kk = 2; kk <= size; kk <<= 1)
-      for (int j = kk >> 1; j > 0; j >>= 1) {
-        for (int i = tid; i < size; i += kTopThreads) {
-          const int p = i ^ j;
-          if (p > i) {
-            const bool desc = (i & kk) == 0;
-            const bool pb = better_ci(sel_s[p], sel_c[p], sel_s[i], sel_c[i]);
-            if (desc ? pb : !pb) {
-              const float ts = sel_s[i];
-              const int tc = sel_c[i];
-              sel_s[i] = sel_s[p];
-              sel_c[i] = sel_c[p];
-              sel_s[p] = ts;
-              sel_c[p] = tc;
+      bitonic_desc(smp, smp_c, kTopSamples, tid);
+      int lo = -1, hi = kTopSamples;  // ranks known too high (too few above) / too low (too many)
+      int r = (int)min((long)kTopSamples - 1, ((long)5 * total * kTopSamples) / (4L * N) + 4);
+      for (int it = 0; it < 12 && r > lo && r < hi; ++it) {
+        const float thr = smp[r];
+        __syncthreads();
+        if (tid == 0) n_cand = 0;
+        __syncthreads();
+        for (int c = tid; c < N; c += kTopThreads) {
+          const float v = sr[c];
+          if (v >= thr) {  // false for NaN
+            const int at = atomicAdd(&n_cand, 1);
+            if (at < kTopCand) {
+              cs[at] = v;
+              cc[at] = c;
             }
           }
         }
         __syncthreads();
+        const int nc = n_cand;
+        if (nc >= total && nc <= kTopCand) {
+          done = true;
+          break;
+        }
+        if (nc < total) lo = r; else hi = r;
+        r = hi == kTopSamples ? min(kTopSamples - 1, 2 * r + 8) : (lo + hi) / 2;
       }
+    }
+    if (!done) {
+      // ---- exact fallback (degenerate rows: runs of equal scores): radix
+      // select of the P-th key, the keys above it, the lowest-column ties ----
+      uint32_t prefix = 0u, mask = 0u;
+      int need = total;
+      for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int d = tid; d < 256; d += kTopThreads) hist[d] = 0;
+        __syncthreads();
+        for (int c = tid; c < N; c += kTopThreads) {
+          const uint32_t k = desc_key(sr[c]);
+          if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255], 1);
+        }
+        __syncthreads();
+        if (tid == 0) {
+          int cum = 0, d = 255;
+          for (; d > 0; --d) {
+            if (cum + hist[d] >= need) break;
+            cum += hist[d];
+          }
+          sh_digit = d;
+          sh_need = need - cum;
+        }
+        __syncthreads();
+        prefix |= (uint32_t)sh_digit << shift;
+        mask |= 255u << shift;
+        need = sh_need;
+        __syncthreads();
+      }
+      if (tid == 0) n_cand = 0;
+      __syncthreads();
+      for (int c = tid; c < N; c += kTopThreads)
+        if (desc_key(sr[c]) > prefix) {
+          const int at = atomicAdd(&n_cand, 1);  // fewer than total <= kTopMaxP
+          cs[at] = sr[c];
+          cc[at] = c;
+        }
+      const int per = (N + kTopThreads - 1) / kTopThreads;
+      const int c0 = tid * per, c1 = min(N, c0 + per);
+      int mine = 0;
+      for (int c = c0; c < c1; ++c) mine += desc_key(sr[c]) == prefix ? 1 : 0;
+      scan_buf[tid] = mine;
+      __syncthreads();
+      for (int o = 1; o < kTopThreads; o <<= 1) {
+        const int v = tid >= o ? scan_buf[tid - o] : 0;
+        __syncthreads();
+        scan_buf[tid] += v;
+        __syncthreads();
+      }
+      int rank = scan_buf[tid] - mine;
+      const int base = n_cand;
+      __syncthreads();
+      for (int c = c0; c < c1 && rank < need; ++c)
+        if (desc_key(sr[c]) == prefix) {
+          cs[base + rank] = sr[c];
+          cc[base + rank] = c;
+          ++rank;
+        }
+      __syncthreads();
+      if (tid == 0) n_cand = min(total, base + need);
+      __syncthreads();
+    }
+    const int nc = min(n_cand, kTopCand);
+    int size = 1;
+    while (size < nc) size <<= 1;
+    for (int i = nc + tid; i < size; i += kTopThreads) {
+      cs[i] = -INFINITY;
+      cc[i] = 0x7fffffff;
+    }
+    __syncthreads();
+    bitonic_desc(cs, cc, size, tid);
     for (int i = tid; i < P; i += kTopThreads) {
-      out_c[row * P + i] = i < total ? sel_c[i] : -1;
-      out_s[row * P + i] = i < total ? sel_s[i] : -INFINITY;
+      const bool ok = i < total && i < nc;
+      out_c[row * P + i] = ok ? cc[i] : -1;
+      out_s[row * P + i] = ok ? cs[i] : -INFINITY;
     }
   }
 }
@@ -687,12 +743,8 @@ hipError_t launch_topk_rows(const float* scores, std::size_t M, int N, std::size
                             float* out_s, hipStream_t s) {
   if (M == 0) return hipSuccess;
   if (P < 1 || P > kTopMaxP) return hipErrorInvalidValue;
-  const size_t lds = (size_t)N * 4;
-  const int stage = lds <= 120 * 1024 ? 1 : 0;
-  if (stage)
-    (void)hipFuncSetAttribute((const void*)topk_rows_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(topk_rows_kernel, dim3(grid_of((long)M, 1, 65536)), dim3(kTopThreads), stage ? lds : 0, s,
-                     scores, (long)M, N, (long)ld, P, stage, out_c, out_s);
+  hipLaunchKernelGGL(topk_rows_kernel, dim3(grid_of((long)M, 1, 65536)), dim3(kTopThreads), 0, s, scores, (long)M, N,
+                     (long)ld, P, out_c, out_s);
   return hipGetLastError();
 }
 
