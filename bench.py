@@ -1,19 +1,20 @@
-"""Benchmark of the plain flat TurboQuant encode + search path on one MI355X.
+"""Benchmarks of the MI355X TurboQuant path on one GPU.
 
-Default workload (the embedding-scale flat index): 1,000,000 x 1536 float32
-vectors, anisotropic Gaussian (variance (d+1)^-0.8, L2-normalized), 4-bit codes
-with TQ+ calibration on a separate 20,000-row sample, 10,000 queries, k = 10,
-all generated on the device from fixed seeds (2001 / 2002 / 2003).
+Default workload `bucketed10m` (the config the headline metric is quoted on):
+the bucketed index over 10,000,000 x 768 float32 rows from a seeded mixture
+of 20,000 centers on the unit sphere with Zipf(1.1) cluster sizes and noise
+of RMS norm 0.2, 4-bit codes, max_bucket_size 800, energy_soft 400; the
+learned probe model trained on 180k queries (labels from the plain flat
+scan); 10k held-out queries, k = 10. `value` is the learned-probe search's
+queries/s at the recall@10 target (against the plain flat scan) that fixed
+n_probe <= 256 reaches on validation queries; beside it the fixed-n_probe
+QPS at the same target, the bucket scan's HBM roofline, latency and the
+reference's CPU search_flat on a sample.
 
-A step is one search of the whole 10,000-query batch against the
-device-resident index (queries already in HBM). `value` is queries/s.
-Beside it: encode vectors/s, the scan kernel's HBM roofline fraction (HIP
-events on the launch stream), recall@10 against an FP64 exact inner product,
-small-batch p50/p99 latency, and the reference's own CPU search (oracle/_ref,
-built from the reference sources) on a bounded sample of the same queries,
-which also checks the GPU's ids/scores on that sample.
+Other workloads: flat1m (1M x 1536 flat index, TQ+, 10k queries), flat1m_k64,
+flat1m_b2, flat100k, landmark1m, bucketed1m.
 
-    python bench.py [--steps K] [--warmup W] [--workload flat1m|flat1m_k64|flat1m_b2|flat100k]
+    python bench.py [--steps K] [--warmup W] [--workload NAME]
 """
 from __future__ import annotations
 
