@@ -8,7 +8,7 @@ from vectorcache_amd import synthetic
 # (bits, dim, n, with TQ+ calibration)
 CASES = [(4, 128, 300, False), (3, 128, 300, False), (2, 128, 300, False), (4, 200, 250, False),
          (4, 768, 200, True), (2, 768, 200, True), (3, 1536, 150, True), (4, 1536, 150, True),
-         (2, 1536, 150, False), (4, 8, 64, False)]
+         (2, 1536, 150, False), (4, 8, 64, False), (2, 1536, 150, True)]
 KS = [1, 10, 64, 1000]
 
 
@@ -79,6 +79,8 @@ LANDMARK_CASES = (
     ("d1536", 1500, 8, 1536, 1024, 4, 4, 0, 10, None, False, False),
     ("d768b3", 1200, 5, 768, 256, 3, 2, 4, 64, None, False, False),
     ("d384b2", 900, 4, 384, 200, 2, 4, 6, 100, (300, 600), False, False),
+    ("m47", 400, 5, 96, 47, 4, 0, 13, 10, None, False, False),
+    ("m17b3", 300, 4, 64, 17, 3, 4, 2, 12, None, False, False),
 )
 
 

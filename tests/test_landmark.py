@@ -97,7 +97,10 @@ def test_oracle_q4_projection_closed_form(oracle):
 def test_oracle_matches_reference_build(oracle, ref):
     """Extra seeds and shapes against the reference build itself."""
     r = np.random.default_rng(77)
-    for n, nq, dim, m, bits, rb, k in ((350, 4, 72, 29, 4, 3, 9), (260, 3, 40, 40, 2, 4, 33)):
+    # m % 8 in {5, 0, 1, 3, 6, 7}: every tail of the residual norm's 8-wide blocks
+    for n, nq, dim, m, bits, rb, k in ((350, 4, 72, 29, 4, 3, 9), (260, 3, 40, 40, 2, 4, 33),
+                                       (300, 3, 64, 17, 4, 4, 10), (280, 3, 48, 11, 3, 2, 7),
+                                       (320, 4, 96, 30, 4, 3, 12), (300, 3, 56, 47, 2, 4, 15)):
         x = r.standard_normal((n, dim)).astype(np.float32)
         q = r.standard_normal((nq, dim)).astype(np.float32)
         x[5] = 0.5
