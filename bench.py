@@ -91,34 +91,216 @@ def exact_topk(x, q, k, chunk=200_000):
     return best_s, best_i
 
 
-def cpu_baseline(ix, q_np, gpu_ids, gpu_scores, bits, shift, scale, budget_s=12.0):
-    """The reference's own search_flat (oracle/_ref) on a bounded query sample,
-    over the GPU index's exported PERM0 codes and alpha (bit-identical to the
-    reference's own encode). Also checks the GPU results on that sample."""
+# ---- CPU baseline protocol ---------------------------------------------------
+# The reference's OpenMP CPU path (oracle/_ref when built, else the C
+# restatement) timed on this host: every physical core the process may use
+# (OMP_PROC_BIND=close, OMP_PLACES=cores) and an 8-thread cell (the
+# reference's bench convention, Makefile:31-34), each cell 2 warmup runs and
+# the mean of 21 timed runs (tools/query-bench.cpp:526-607), repeated in
+# fresh processes. A run is one search_flat over a query sample, or
+# encode + codes_flat + repack_perm0 over a row sample; the sample is sized so
+# that a run takes about CPU_RUN_S, which bounds the whole leg to ~1-2 min.
+CPU_WARMUP, CPU_RUNS, CPU_REPEATS, CPU_RUN_S = 2, 21, 3, 0.25
+
+
+def host_cpu_info():
+    """lscpu's model and topology, the physical cores among the CPUs this
+    process may run on, and the cgroup CPU quota when one is set."""
+    info = {"model": cpu_model()}
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            if key.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info["lscpu " + key.strip()] = val.strip()
+        if info.get("lscpu Model name"):
+            info["model"] = info["lscpu Model name"]
+    except (OSError, ValueError):
+        pass
+    cpus = sorted(os.sched_getaffinity(0))
+    phys = set()
+    for c in cpus:
+        topo = Path(f"/sys/devices/system/cpu/cpu{c}/topology")
+        try:
+            phys.add((topo.joinpath("physical_package_id").read_text().strip(),
+                      topo.joinpath("core_id").read_text().strip()))
+        except OSError:
+            phys.add(("cpu", c))
+    info["logical_cpus_allowed"] = len(cpus)
+    info["physical_cores_allowed"] = len(phys)
+    try:
+        quota, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if quota != "max":
+            info["cgroup_cpu_quota"] = int(quota) / int(period)
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def cpu_thread_cells(info):
+    cores = max(1, info["physical_cores_allowed"])
+    return [cores] if cores == 8 else [cores, 8]
+
+
+def cpu_cell_main(spec_path):
+    """One cell of the CPU baseline, in a child process started by
+    run_cpu_cells: loads the shared arrays (memory-mapped), sizes the sample
+    to about CPU_RUN_S per run unless the spec fixes it, then CPU_WARMUP
+    untimed and CPU_RUNS timed runs; prints one JSON line and saves the last
+    run's outputs beside the inputs."""
+    spec = json.loads(Path(spec_path).read_text())
+    d = Path(spec["dir"])
     sys.path.insert(0, str(ROOT / "oracle"))
-    from oracle import Oracle, available
+    from oracle import Oracle
+
+    o = Oracle(spec["kind"])
+    o.set_threads(spec["threads"])
+    bits, dim = spec["bits"], spec["dim"]
+
+    def ld(name):
+        p = d / f"{name}.npy"
+        return np.load(p, mmap_mode="r") if p.exists() else None
+
+    sh, sc = ld("shift"), ld("scale")
+    if spec["op"] == "search":
+        q, perm0, alpha = ld("q"), ld("perm0"), ld("alpha")
+        avail, step = q.shape[0], 1
+
+        def run(m):
+            return o.search_flat(q[:m], spec["k"], bits, perm0, alpha, sh, sc)
+    else:
+        x = ld("x")
+        avail, step = x.shape[0] // 192 * 192, 192
+
+        def run(m):
+            packed, a = o.encode(x[:m], bits, sh, sc)
+            return o.codes_flat(packed, bits, dim), o.repack_perm0(packed, bits, dim), a
+
+    m = spec.get("m")
+    if not m:
+        m0 = min(avail, step * (1 if spec["op"] == "search" else 4))
+        run(m0)  # pages the mapped inputs in
+        t0 = time.perf_counter()
+        run(m0)
+        per = (time.perf_counter() - t0) / m0
+        m = int(min(avail, max(step, CPU_RUN_S / max(per, 1e-9) // step * step)))
+    for _ in range(CPU_WARMUP):
+        out = run(m)
+    ts = []
+    for _ in range(CPU_RUNS):
+        t0 = time.perf_counter()
+        out = run(m)
+        ts.append(time.perf_counter() - t0)
+    names = ("scores", "ids") if spec["op"] == "search" else ("flat", "perm0", "alpha")
+    for name, a in zip(names, out):
+        np.save(d / f"out_{name}.npy", a)
+    print(json.dumps({"m": m, "mean_s": float(np.mean(ts)), "min_s": float(np.min(ts)), "max_s": float(np.max(ts))}))
+
+
+def run_cpu_cells(op, arrays, params, info):
+    """Runs `op` ("search" or "encode") for every thread cell, CPU_REPEATS
+    fresh processes each, over arrays written once to a temporary folder.
+    Returns the cells (rate = sample / mean run time, mean over the repeats)
+    and the last run's outputs."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from oracle import available
 
     kind = "reference" if available("reference") else "port"
-    o = Oracle(kind)
-    cores = min(16, os.cpu_count() or 1)
-    o.set_threads(cores)
-    perm0 = ix.export_perm0()
-    alpha = ix.export_alpha()
+    tmp = Path(tempfile.mkdtemp(prefix="vc_cpu_"))
+    try:
+        for name, a in arrays.items():
+            if a is not None:
+                np.save(tmp / f"{name}.npy", np.ascontiguousarray(a))
+        cells = []
+        for threads in cpu_thread_cells(info):
+            m, reps = None, []
+            for _ in range(CPU_REPEATS):
+                spec = dict(params, op=op, kind=kind, threads=threads, dir=str(tmp), m=m)
+                (tmp / "spec.json").write_text(json.dumps(spec))
+                env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores", OMP_NUM_THREADS=str(threads))
+                p = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-cell", str(tmp / "spec.json")],
+                                   env=env, capture_output=True, text=True, timeout=600)
+                if p.returncode != 0:
+                    raise RuntimeError(f"cpu cell ({op}, {threads} threads) failed: {p.stderr[-1500:]}")
+                r = json.loads(p.stdout.strip().splitlines()[-1])
+                m = r["m"]
+                reps.append(r["mean_s"])
+            cells.append({"threads": threads, "sample": m, "mean_s_per_run": reps,
+                          "rate": m / float(np.mean(reps))})
+        names = ("scores", "ids") if op == "search" else ("flat", "perm0", "alpha")
+        outs = {nm: np.load(tmp / f"out_{nm}.npy") for nm in names}
+        return kind, cells, outs
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def cpu_record(kind, info, search_cells, search_unit, search_sample, encode_cells=None, encode_sample=None, **extra):
+    """The bench's cpu_baseline object: value / cores from the all-cores
+    search cell, every cell beside it, the host description."""
+    top = search_cells[0]
+    rec = {"value": top["rate"], "unit": search_unit, "cores": top["threads"], "kind": kind,
+           "cpu": info["model"], "host": info,
+           "protocol": f"{CPU_WARMUP} warmups + mean of {CPU_RUNS} runs, {CPU_REPEATS} fresh processes per cell, "
+                       "OMP_PROC_BIND=close OMP_PLACES=cores",
+           "sample": search_sample, "search_cells": search_cells}
+    if encode_cells is not None:
+        rec["encode"] = {"unit": "vectors/s", "value": encode_cells[0]["rate"], "cores": encode_cells[0]["threads"],
+                         "sample": encode_sample, "cells": encode_cells}
+    rec.update(extra)
+    return rec
+
+
+def score_error_fp64(x, q, scores, ids):
+    """Error of the quantized scores against the FP64 exact inner product they
+    estimate, <q / |q|, x_id> (queries are normalized, rows are not), over the
+    returned (query, id) pairs."""
+    import torch
+
+    qd = q.double()
+    qd = qd / qd.norm(dim=1, keepdim=True).clamp_min(1e-30)
+    idx = ids.long()
+    ok = (idx >= 0) & (idx < x.shape[0])
+    rows = x[idx.clamp(0, x.shape[0] - 1).view(-1)].double().view(idx.shape[0], idx.shape[1], -1)
+    exact = torch.einsum("qd,qkd->qk", qd, rows)
+    err = (scores.double() - exact)[ok]
+    ex = exact[ok]
+    return {"pairs": int(ok.sum()), "mean_abs": float(err.abs().mean()), "rms": float(err.pow(2).mean().sqrt()),
+            "max_abs": float(err.abs().max()), "mean_signed": float(err.mean()),
+            "mean_abs_exact": float(ex.abs().mean()),
+            "relative_mean_abs": float(err.abs().mean() / ex.abs().mean().clamp_min(1e-30))}
+
+
+def cpu_baseline(ix, x_rows, q_np, gpu_ids, gpu_scores, bits, dim, shift, scale):
+    """The reference's own search_flat on a bounded query sample, over the GPU
+    index's exported PERM0 codes and alpha (bit-identical to the reference's
+    own encode), and its encode + codes_flat + repack_perm0 on a row sample;
+    both checked against the GPU's results on the same sample."""
+    info = host_cpu_info()
     k = gpu_ids.shape[1]
-    t0 = time.perf_counter()
-    s, i = o.search_flat(q_np[:4], k, bits, perm0, alpha, shift, scale)
-    per_q = (time.perf_counter() - t0) / 4
-    m = int(max(4, min(q_np.shape[0], budget_s / max(per_q, 1e-6))))
-    t0 = time.perf_counter()
-    s, i = o.search_flat(q_np[:m], k, bits, perm0, alpha, shift, scale)
-    dt = time.perf_counter() - t0
-    ids_ok = bool(np.array_equal(i, gpu_ids[:m]))
-    sc_ok = bool(np.array_equal(s, gpu_scores[:m]))
-    return {
-        "value": m / dt, "unit": "queries/s", "cores": cores, "kind": kind,
-        "sample": f"{m} of the {q_np.shape[0]} queries, same index (search_flat over the exported PERM0 codes)",
-        "parity_ids": ids_ok, "parity_scores": sc_ok,
-    }
+    perm0 = ix.export_perm0()
+    prm = {"bits": bits, "dim": dim, "k": k}
+    kind, s_cells, s_out = run_cpu_cells("search", {"q": q_np, "perm0": perm0, "alpha": ix.export_alpha(),
+                                                    "shift": shift, "scale": scale}, prm, info)
+    m = s_cells[0]["sample"]
+    ms = s_cells[-1]["sample"]  # the saved outputs are the last cell's
+    _, e_cells, e_out = run_cpu_cells("encode", {"x": x_rows, "shift": shift, "scale": scale}, prm, info)
+    me = e_cells[-1]["sample"]
+    G = dim // (8 // bits)
+    return cpu_record(
+        kind, info, s_cells, "queries/s",
+        f"first {m} of the {q_np.shape[0]} queries per run (8-thread cell: {s_cells[-1]['sample']}), same index "
+        "(search_flat over the exported PERM0 codes)",
+        e_cells, f"first rows of the database per run ({', '.join(str(c['sample']) for c in e_cells)} rows), "
+                 "encode + codes_flat + repack_perm0",
+        parity_ids=bool(np.array_equal(s_out["ids"], gpu_ids[:ms])),
+        parity_scores=bool(np.array_equal(s_out["scores"], gpu_scores[:ms])),
+        encode_parity_perm0=bool(np.array_equal(e_out["perm0"], perm0[: me * G])))
 
 
 def host_buffer_qps(ix, q, k, reps=2):
@@ -376,7 +558,7 @@ def run_bucketed(args):
         jd = judge[ev].cpu().numpy()
         rec = float(np.mean([len(set(got[i]) & set(jd[i])) / k for i in range(n_ev)]))
         res[mode] = {"qps": n_ev * args.steps / el, "ms_per_step": el / args.steps * 1e3,
-                     "scan_ms": float(np.mean(scan_ms)), "recall_at_10_vs_flat_scan": rec, "ids": ids}
+                     "scan_ms": float(np.mean(scan_ms)), "recall_at_10_vs_flat_scan": rec, "ids": ids, "scores": sc}
         log(f"{mode}: {res[mode]['qps']:.0f} QPS, {res[mode]['ms_per_step']:.2f} ms/step, bucket scan "
             f"{res[mode]['scan_ms']:.2f} ms, recall@10 vs flat scan {rec:.4f}")
 
@@ -397,6 +579,8 @@ def run_bucketed(args):
     rec64 = float(np.mean([len(set(got[i]) & set(ei.cpu().numpy()[i])) / k for i in range(ns)]))
     got_j = judge[ev][:ns].cpu().numpy()
     rec64_flat = float(np.mean([len(set(got_j[i]) & set(ei.cpu().numpy()[i])) / k for i in range(ns)]))
+    score_err = score_error_fp64(x, qe[:ns], res["learned"]["scores"][:ns], res["learned"]["ids"][:ns])
+    log(f"score error vs fp64 (learned, {ns} queries): {score_err}")
     lat = {}
     if not args.no_latency:
         for b in (1, 64):
@@ -424,7 +608,8 @@ def run_bucketed(args):
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = bucketed_cpu_baseline(st, q_ev[: 64].cpu().numpy(), judge[ev][:64].cpu().numpy(), bits, k, n)
+            cpu = bucketed_cpu_baseline(st, x[:65_472].cpu().numpy(), q_ev[:1024].cpu().numpy(),
+                                        judge[ev][:1024].cpu().numpy(), bits, dim, k, n)
             log(f"cpu baseline {cpu}")
         except Exception as e:  # reported, never fatal for the GPU figure
             cpu = {"error": str(e)}
@@ -453,8 +638,8 @@ def run_bucketed(args):
             "learned_probe_table": {"y0": 0.0, "y_step": 0.25, "counts": table.tolist()},
             "seeds": {"centers": 4001, "rows": 4002, "train_val_queries": 4003, "eval_queries": 4004},
         },
-        "fixed": {k2: v for k2, v in res["fixed"].items() if k2 != "ids"},
-        "learned": {k2: v for k2, v in res["learned"].items() if k2 != "ids"},
+        "fixed": {k2: v for k2, v in res["fixed"].items() if k2 not in ("ids", "scores")},
+        "learned": {k2: v for k2, v in res["learned"].items() if k2 not in ("ids", "scores")},
         "learned_over_fixed_qps": res["learned"]["qps"] / res["fixed"]["qps"],
         "eval_recall_from_ranks": {"fixed": rec_fixed_ev, "learned": rec_learned_ev},
         "recall_curve_fixed_n_probe": reach,
@@ -471,6 +656,7 @@ def run_bucketed(args):
                   "judge_qps": q_all.shape[0] / judge_s, "train_seconds": train_s, "loss_first": loss_first,
                   "loss_last": loss_last},
         "recall_at_10_vs_fp64": {"learned": rec64, "flat_scan": rec64_flat, "queries": ns},
+        "score_error_vs_fp64": score_err,
         "latency": lat,
         "copy_probe": probe,
         "cpu_baseline": cpu,
@@ -478,37 +664,40 @@ def run_bucketed(args):
     print(json.dumps(out))
 
 
-def bucketed_cpu_baseline(st, q_np, judge_ids, bits, k, n_full, budget_s=12.0):
-    """The reference's own search_flat (oracle/_ref) over all the index's codes
-    (PERM0 blocks of the exported flat codes) on a bounded query sample: the
-    CPU's exhaustive answer to the same question, with every host core."""
+def bucketed_cpu_baseline(st, x_rows, q_np, judge_ids, bits, dim, k, n_full):
+    """The reference's own search_flat over all the index's codes (PERM0
+    blocks of the exported flat codes) on a bounded query sample: the CPU's
+    exhaustive answer to the same question (the reference has no GPU bucketed
+    path to time), checked against the GPU flat scan; and its encode +
+    codes_flat + repack_perm0 on a row sample, checked against the index's
+    codes of those rows."""
     sys.path.insert(0, str(ROOT / "oracle"))
-    from oracle import Oracle, available
+    from oracle import Oracle
 
-    kind = "reference" if available("reference") else "port"
-    o = Oracle(kind)
-    port = Oracle("port")
-    cores = os.cpu_count() or 1
-    o.set_threads(cores)
+    info = host_cpu_info()
     flat = st["flat"]
     G = flat.shape[1]
     nb = (flat.shape[0] + 191) // 192
     perm0 = np.zeros(nb * G * 192, np.uint8)
-    port._port("pack_perm0")(flat.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(flat.shape[0]), ctypes.c_size_t(G),
-                             perm0.ctypes.data_as(ctypes.c_void_p))
-    alpha = st["alpha"]
-    t0 = time.perf_counter()
-    s, i = o.search_flat(q_np[:2], k, bits, perm0, alpha)
-    per_q = (time.perf_counter() - t0) / 2
-    m = int(max(2, min(q_np.shape[0], budget_s / max(per_q, 1e-6))))
-    t0 = time.perf_counter()
-    s, i = o.search_flat(q_np[:m], k, bits, perm0, alpha)
-    dt = time.perf_counter() - t0
-    ids = st["pos_ids"][i.astype(np.int64)]
-    return {"value": m / dt, "unit": "queries/s", "cores": cores, "cpu": cpu_model(), "kind": kind,
-            "sample": f"{m} of the held-out queries, exhaustive search_flat over all {n_full} codes "
-                      "(the reference has no bucketed index)",
-            "parity_ids_vs_gpu_flat_scan": bool(np.array_equal(ids, judge_ids[:m].astype(np.uint64)))}
+    Oracle("port")._port("pack_perm0")(flat.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(flat.shape[0]),
+                                       ctypes.c_size_t(G), perm0.ctypes.data_as(ctypes.c_void_p))
+    prm = {"bits": bits, "dim": dim, "k": k}
+    kind, s_cells, s_out = run_cpu_cells("search", {"q": q_np, "perm0": perm0, "alpha": st["alpha"]}, prm, info)
+    del perm0
+    ms = s_cells[-1]["sample"]  # the saved outputs are the last cell's
+    ids = st["pos_ids"][s_out["ids"].astype(np.int64)]
+    _, e_cells, e_out = run_cpu_cells("encode", {"x": x_rows}, prm, info)
+    me = e_cells[-1]["sample"]
+    pos_of_row = np.empty(flat.shape[0], np.int64)
+    pos_of_row[st["pos_ids"].astype(np.int64)] = np.arange(flat.shape[0])
+    return cpu_record(
+        kind, info, s_cells, "queries/s",
+        f"first {s_cells[0]['sample']} held-out queries per run (8-thread cell: {ms}), exhaustive search_flat "
+        f"over all {n_full} codes (the reference has no bucketed index)",
+        e_cells, f"first rows of the database per run ({', '.join(str(c['sample']) for c in e_cells)} rows), "
+                 "encode + codes_flat + repack_perm0",
+        parity_ids_vs_gpu_flat_scan=bool(np.array_equal(ids, judge_ids[:ms].astype(np.uint64))),
+        encode_parity_codes=bool(np.array_equal(e_out["flat"], flat[pos_of_row[:me]])))
 
 
 def main():
@@ -521,7 +710,10 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="cap the scan query tile (1, 2, 4, 8)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-latency", action="store_true")
+    ap.add_argument("--cpu-cell", help=argparse.SUPPRESS)  # child process of the CPU baseline
     args = ap.parse_args()
+    if args.cpu_cell:
+        return cpu_cell_main(args.cpu_cell)
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; the flat scan would shard by rows "
                                    "with an all-gather of per-GPU top-k lists", "n_gpus": args.gpus}))
@@ -609,13 +801,14 @@ def main():
                    "achieved_gbs": bytes_b / t_b / 1e9, "frac": bytes_b / t_b / 1e9 / HBM_PEAK_GBS})
     log(f"bandwidth regime {bw}")
 
-    # ---- quality: recall@10 vs FP64 exact on a query sample ----
+    # ---- quality: recall@10 vs FP64 exact and the scores' error on a query sample ----
     ns = min(200, nq)
     es, ei = exact_topk(x, q[:ns], 10)
     got = ids[:ns, :10].cpu().numpy()
     exact = ei.cpu().numpy()
     recall = float(np.mean([len(set(got[i]) & set(exact[i])) / 10 for i in range(ns)]))
-    log(f"recall@10 vs fp64 exact ({ns} queries): {recall:.4f}")
+    score_err = score_error_fp64(x, q[:ns], scores[:ns], ids[:ns])
+    log(f"recall@10 vs fp64 exact ({ns} queries): {recall:.4f}; score error vs fp64 {score_err}")
 
     # ---- latency of small request batches ----
     lat = {}
@@ -644,8 +837,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(ix, q.cpu().numpy(), ids.cpu().numpy().astype(np.uint64), scores.cpu().numpy(),
-                               bits, shift, scale)
+            x_rows = x[: min(n, 65_472)].cpu().numpy()
+            cpu = cpu_baseline(ix, x_rows, q.cpu().numpy(), ids.cpu().numpy().astype(np.uint64),
+                               scores.cpu().numpy(), bits, dim, shift, scale)
             log(f"cpu baseline {cpu}")
         except Exception as e:  # reported, never fatal for the GPU figure
             cpu = {"error": str(e)}
@@ -682,6 +876,7 @@ def main():
         "host_buffer_qps": host_qps,
         "copy_probe": probe,
         "recall_at_10_vs_fp64": recall,
+        "score_error_vs_fp64": score_err,
         "latency": lat,
         "cpu_baseline": cpu,
     }
@@ -700,7 +895,8 @@ def landmark_cpu_baseline(x_np, q_np, bits, m, k, n_full, budget_queries=1000, n
 
     kind = "reference" if available("reference") else "port"
     o = Oracle(kind)
-    cores = min(16, os.cpu_count() or 1)
+    host = host_cpu_info()
+    cores = host["physical_cores_allowed"]
     o.set_threads(cores)
     xs = np.ascontiguousarray(x_np[:n_sample])
     qs = np.ascontiguousarray(q_np[:budget_queries])
@@ -712,7 +908,7 @@ def landmark_cpu_baseline(x_np, q_np, bits, m, k, n_full, budget_queries=1000, n
     r = g.search(qs, k)
     g.close()
     out = {
-        "unit": "queries/s", "cores": cores, "kind": kind,
+        "unit": "queries/s", "cores": cores, "kind": kind, "cpu": host["model"], "host": host,
         "sample": f"LandmarkIndex on the first {n_sample} of the {n_full} vectors, search() of {len(qs)} queries; "
                   f"rate x {n_sample}/{n_full} (scan linear in n)",
         "parity_ids": bool(np.array_equal(r.ids, i)), "parity_scores": bool(np.array_equal(r.scores, s)),
