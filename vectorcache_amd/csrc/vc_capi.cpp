@@ -133,17 +133,16 @@ void calibrate_device(const vchip::CodeTables& t, const float* sample_dev, size_
             "calibrate");
 }
 
-SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const float* alpha, size_t n,
-                       const float* q_dev, size_t nq, size_t k, const uint64_t* id_map_dev, float* out_s,
-                       uint64_t* out_i, DevBuf& scratch, hipStream_t s, uint8_t** luts_out, size_t* luts_off,
-                       ScratchArena* arena) {
-  SearchOut r{std::min(k, n)};
-  if (nq == 0 || r.k_eff == 0) return r;
-  if (r.k_eff > size_t(vchip::kMaxTopK))
+SearchLayout search_layout(const vchip::CodeTables& t, size_t n, size_t nq, size_t k) {
+  SearchLayout L;
+  L.k_eff = std::min(k, n);
+  if (nq == 0 || L.k_eff == 0) return L;
+  if (L.k_eff > size_t(vchip::kMaxTopK))
     fail(VC_ERR_INVALID_ARGUMENT, "k exceeds the GPU top-k capacity of 2048");
   if (n > 0xFFFFFFFFull) fail(VC_ERR_INVALID_ARGUMENT, "index exceeds 2^32 vectors");
-  const int ke = int(r.k_eff);
-  vchip::ScanPlan plan = vchip::plan_scan(t.groups16, n, nq, ke, g_forced_tile.load(), (g_debug_flags.load() & 2) != 0);
+  const int ke = int(L.k_eff);
+  vchip::ScanPlan& plan = L.plan;
+  plan = vchip::plan_scan(t.groups16, n, nq, ke, g_forced_tile.load(), (g_debug_flags.load() & 2) != 0);
   if (plan.tile == 0) fail(VC_ERR_INVALID_ARGUMENT, "no scan configuration fits LDS for this k / tile");
   if ((g_debug_flags.load() & 8) != 0 && plan.pair) plan.vec_per_thread = 3 - plan.vec_per_thread;  // test hook
   if (const size_t fr = std::min<size_t>(unsigned(g_debug_flags.load()) >> 16, 512); fr != 0 && n <= (size_t(1) << 30)) {
@@ -155,11 +154,33 @@ SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const f
   }
   const size_t G = size_t(t.groups);
   Carve c;
-  const size_t o_luts = c.take(nq * G * 32);
-  const size_t o_ls = c.take(nq * 4), o_lb = c.take(nq * 4), o_bc = c.take(nq * 4), o_bt = c.take(nq * 4);
+  L.o_luts = c.take(nq * G * 32);
+  L.o_ls = c.take(nq * 4);
+  L.o_lb = c.take(nq * 4);
+  L.o_bc = c.take(nq * 4);
+  L.o_bt = c.take(nq * 4);
   const size_t parts = nq * size_t(plan.n_ranges);
-  const size_t o_ps = c.take(parts * ke * 4), o_pi = c.take(parts * ke * 4), o_pc = c.take(parts * 4);
-  const size_t o_lt = c.take(vchip::scan_lut_bytes(plan, t.groups16, nq));
+  L.o_ps = c.take(parts * ke * 4);
+  L.o_pi = c.take(parts * ke * 4);
+  L.o_pc = c.take(parts * 4);
+  L.o_lt = c.take(vchip::scan_lut_bytes(plan, t.groups16, nq));
+  L.bytes = c.off;
+  return L;
+}
+
+SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const float* alpha, size_t n,
+                       const float* q_dev, size_t nq, size_t k, const uint64_t* id_map_dev, float* out_s,
+                       uint64_t* out_i, DevBuf& scratch, hipStream_t s, uint8_t** luts_out, size_t* luts_off,
+                       ScratchArena* arena) {
+  const SearchLayout L = search_layout(t, n, nq, k);
+  SearchOut r{L.k_eff};
+  if (L.bytes == 0) return r;
+  const int ke = int(L.k_eff);
+  const vchip::ScanPlan& plan = L.plan;
+  const size_t o_luts = L.o_luts, o_ls = L.o_ls, o_lb = L.o_lb, o_bc = L.o_bc, o_bt = L.o_bt;
+  const size_t o_ps = L.o_ps, o_pi = L.o_pi, o_pc = L.o_pc, o_lt = L.o_lt;
+  Carve c;
+  c.off = L.bytes;
   ScratchArena own{&scratch};
   uint8_t* base = (arena != nullptr ? arena : &own)->take(c.off, s);
   auto at = [base](size_t off) { return base + off; };
