@@ -68,7 +68,11 @@ vc_status vc_index_calibrate_device(vc_index* index, const float* sample_dev, si
  * (src/index.cpp:151-213). Vectors are staged on the device; prepare() encodes
  * them. add() after prepare() is allowed and marks the index unprepared. */
 vc_status vc_index_add(vc_index* index, const float* vectors, size_t n);
-vc_status vc_index_add_device(vc_index* index, const float* vectors_dev, size_t n);
+/* Same, rows in device memory: they are read after the work enqueued so far
+ * on `stream` (NULL: the null stream), and work enqueued on `stream` after
+ * the call runs after the copy, so the caller may reuse the rows' memory at
+ * once. No device-wide synchronization. */
+vc_status vc_index_add_device(vc_index* index, const float* vectors_dev, size_t n, void* stream);
 
 /* Encode every staged vector on the device (replaces prepare(),
  * src/index.cpp:215-265). */
@@ -87,6 +91,15 @@ vc_status vc_index_search_device(vc_index* index, const float* queries_dev, size
                                  float* scores_dev, uint64_t* ids_dev, size_t* k_out,
                                  void* stream);
 
+/* Search of the first n_prefix vectors only (n_prefix <= size()): the result
+ * a search issued while the index held n_prefix vectors returned, since a
+ * vector's code depends only on its own row and the calibration. For
+ * checking a request served during concurrent adds against the same request
+ * alone. No counterpart in the reference. */
+vc_status vc_index_search_prefix_device(vc_index* index, const float* queries_dev, size_t nq, size_t k,
+                                        size_t n_prefix, float* scores_dev, uint64_t* ids_dev, size_t* k_out,
+                                        void* stream);
+
 /* vc_index_search_device without allocation. No counterpart in the reference:
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc). Same
@@ -102,6 +115,11 @@ vc_status vc_index_search_device(vc_index* index, const float* queries_dev, size
 vc_status vc_index_search_device_noalloc(vc_index* index, const float* queries_dev, size_t nq, size_t k,
                                          float* scores_dev, uint64_t* ids_dev, size_t* k_out, void* stream);
 size_t vc_noalloc_scratch_bytes(void);
+
+/* Streams: the vc_index_* calls of one index share its scratch and staging
+ * buffers, so one index takes work from one stream at a time (calls are
+ * serialized by a mutex, but the enqueued work of two streams may overlap on
+ * the device). Concurrent requests go through a vc_server. */
 
 /* Accessors (index.hpp:23-40). */
 size_t vc_index_dim(const vc_index* index);
@@ -160,7 +178,9 @@ vc_status vc_lm_index_create(size_t dim, size_t scan_bits, size_t n_landmarks, u
                              size_t rerank_bits, vc_lm_index** out);
 vc_status vc_lm_index_destroy(vc_lm_index* index);
 vc_status vc_lm_index_add(vc_lm_index* index, const float* vectors, size_t n);
-vc_status vc_lm_index_add_device(vc_lm_index* index, const float* vectors_dev, size_t n);
+/* Rows in device memory, read after the work enqueued so far on `stream`
+ * (NULL: the null stream); the call returns once they are encoded. */
+vc_status vc_lm_index_add_device(vc_lm_index* index, const float* vectors_dev, size_t n, void* stream);
 vc_status vc_lm_index_prepare(vc_lm_index* index);
 /* search(queries, k, float_corpus): k_eff = min(k, size()) <= 2048; with a
  * float corpus (size() x dim rows, may be NULL) the top k is re-scored by exact
@@ -483,6 +503,83 @@ vc_status vc_vcex_save(const char* path, const vc_vcex_key* key, const uint64_t*
 vc_status vc_vcex_load(const char* path, const vc_vcex_key* key, uint64_t* ids, int* loaded);
 vc_status vc_vcex_source_label(const char* npy_path, const char* dataset, char* out, size_t cap, size_t* len);
 vc_status vc_vcex_cache_path(const vc_vcex_key* key, char* out, size_t cap, size_t* len);
+
+/* ---- request server: continuous batching over one flat index --------------
+ * No counterpart in the reference (its GPU search mutates shared device
+ * buffers and is not safe to call concurrently, src/gpu/kernels.cu:602-627).
+ * For serving loops: many callers submit search requests (host query rows,
+ * any k <= max_k) and add batches (host rows) from any thread; each submit
+ * copies its rows and returns a ticket at once.
+ *
+ * Batching. `slots` workers, each with its own HIP stream, take the waiting
+ * search requests in arrival order into one batch whenever they are free (at
+ * most max_batch_queries queries), so requests that arrive while others are
+ * on the device join the next batch a slot starts. A batch runs at the
+ * largest k it holds; a request keeps the first min(k, n) entries of its
+ * rows, which are its own top-k (rows are sorted by score, ties by lower id,
+ * exactly as search() orders them, src/index.cpp:267-289).
+ * Admission. A batch holds device memory for its query rows, LUTs, per-range
+ * lists and results, carved from a pool of hbm_budget_bytes allocated at
+ * create; it starts only when that memory is free, taking fewer requests if
+ * need be, else it waits for a running batch to finish. A request that
+ * cannot fit even alone fails with VC_ERR_OOM.
+ * Adds. Add batches are encoded on a stream of their own, add_chunk rows per
+ * launch, into the index's capacity, which is reserved at create for
+ * reserve_vectors vectors (growing it later waits for the running batches to
+ * finish). An add beyond max_vectors (when not 0) fails with VC_ERR_OOM.
+ * A search batch scans the vectors whose encode had finished when it started
+ * (vc_request_info.n_snapshot); rows being encoded are never read.
+ *
+ * The index must be prepared; while the server exists the index's own calls
+ * that add, prepare, calibrate, search, export or destroy fail with
+ * VC_ERR_STATE. vc_server_destroy finishes every submitted request first and
+ * must not run concurrently with other calls on the server. */
+typedef struct vc_server vc_server;
+typedef struct vc_server_params {
+  size_t slots;             /* batches in flight at once, one HIP stream each (default 4, <= 64) */
+  size_t max_batch_queries; /* queries per batch and per request (default 512) */
+  size_t max_k;             /* largest k of a request (default 100, <= 2048) */
+  size_t hbm_budget_bytes;  /* device pool for the batches in flight (default 1 GiB) */
+  size_t reserve_vectors;   /* index capacity reserved at create (0: size() + 1,000,000) */
+  size_t max_vectors;       /* adds past this size fail (0: no limit but device memory) */
+  size_t add_chunk;         /* rows encoded per add launch (default 65,536) */
+} vc_server_params;
+
+typedef struct vc_request_info {
+  double t_submit, t_start, t_done; /* seconds on CLOCK_MONOTONIC */
+  size_t n_snapshot;    /* search: vectors the batch scanned; add: size() after it */
+  size_t k_eff;         /* search: min(k, n_snapshot); add: rows added */
+  size_t batch_queries; /* queries in the request's batch */
+  uint64_t batch;       /* batch sequence number */
+  int slot;             /* worker slot that ran it (-1: add) */
+} vc_request_info;
+
+typedef struct vc_server_stats {
+  uint64_t requests, queries, batches, adds, vectors_added, admission_waits;
+  size_t in_flight_peak;  /* most batches on the device at once */
+  size_t pool_peak_bytes; /* most pool bytes held at once */
+  size_t pool_bytes;      /* the pool's size */
+  size_t size;            /* vectors visible to the next batch */
+} vc_server_stats;
+
+vc_status vc_server_params_default(vc_server_params* params);
+/* Pool bytes one batch of nq queries at top-k over n vectors of `index` holds
+ * (search scratch, query rows, results); 0 for nq = 0 or an invalid k. For
+ * sizing hbm_budget_bytes. */
+size_t vc_server_batch_bytes(const vc_index* index, size_t n, size_t nq, size_t k);
+/* params may be NULL (defaults). The index's device must be current. */
+vc_status vc_server_create(vc_index* index, const vc_server_params* params, vc_server** out);
+vc_status vc_server_destroy(vc_server* server);
+/* queries: nq x dim host rows, 1 <= nq <= max_batch_queries, 1 <= k <= max_k. */
+vc_status vc_server_submit_search(vc_server* server, const float* queries, size_t nq, size_t k, uint64_t* ticket);
+vc_status vc_server_submit_add(vc_server* server, const float* vectors, size_t n, uint64_t* ticket);
+/* Blocks until the request is done and releases its ticket. Search results go
+ * to scores / ids (nq x info->k_eff, room for nq * k); add requests take NULL
+ * outputs. Returns the request's own status (its message in vc_last_error). */
+vc_status vc_server_wait(vc_server* server, uint64_t ticket, float* scores, uint64_t* ids, vc_request_info* info);
+/* Blocks until every submitted request is done. */
+vc_status vc_server_flush(vc_server* server);
+vc_status vc_server_get_stats(vc_server* server, vc_server_stats* out);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,31 @@ class BucketParams(ctypes.Structure):
                 ("seed", ctypes.c_uint64), ("residual", ctypes.c_int)]
 
 
+class ServerParams(ctypes.Structure):
+    """vc_server_params."""
+
+    _fields_ = [("slots", ctypes.c_size_t), ("max_batch_queries", ctypes.c_size_t), ("max_k", ctypes.c_size_t),
+                ("hbm_budget_bytes", ctypes.c_size_t), ("reserve_vectors", ctypes.c_size_t),
+                ("max_vectors", ctypes.c_size_t), ("add_chunk", ctypes.c_size_t)]
+
+
+class RequestInfo(ctypes.Structure):
+    """vc_request_info."""
+
+    _fields_ = [("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("n_snapshot", ctypes.c_size_t), ("k_eff", ctypes.c_size_t), ("batch_queries", ctypes.c_size_t),
+                ("batch", ctypes.c_uint64), ("slot", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    """vc_server_stats."""
+
+    _fields_ = [("requests", ctypes.c_uint64), ("queries", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("adds", ctypes.c_uint64), ("vectors_added", ctypes.c_uint64), ("admission_waits", ctypes.c_uint64),
+                ("in_flight_peak", ctypes.c_size_t), ("pool_peak_bytes", ctypes.c_size_t),
+                ("pool_bytes", ctypes.c_size_t), ("size", ctypes.c_size_t)]
+
+
 class VcexKey(ctypes.Structure):
     """vc_vcex_key: the key of a VCEX exact top-k cache file."""
 
@@ -69,11 +94,21 @@ _SIGS = {
     "vc_index_calibrate": ([P, P, S], I),
     "vc_index_calibrate_device": ([P, P, S], I),
     "vc_index_add": ([P, P, S], I),
-    "vc_index_add_device": ([P, P, S], I),
+    "vc_index_add_device": ([P, P, S, P], I),
     "vc_index_prepare": ([P], I),
     "vc_index_search": ([P, P, S, S, P, P, ctypes.POINTER(S)], I),
     "vc_index_search_device": ([P, P, S, S, P, P, ctypes.POINTER(S), P], I),
     "vc_index_search_device_noalloc": ([P, P, S, S, P, P, ctypes.POINTER(S), P], I),
+    "vc_index_search_prefix_device": ([P, P, S, S, S, P, P, ctypes.POINTER(S), P], I),
+    "vc_server_params_default": ([ctypes.POINTER(ServerParams)], I),
+    "vc_server_batch_bytes": ([P, S, S, S], S),
+    "vc_server_create": ([P, ctypes.POINTER(ServerParams), ctypes.POINTER(P)], I),
+    "vc_server_destroy": ([P], I),
+    "vc_server_submit_search": ([P, P, S, S, ctypes.POINTER(ctypes.c_uint64)], I),
+    "vc_server_submit_add": ([P, P, S, ctypes.POINTER(ctypes.c_uint64)], I),
+    "vc_server_wait": ([P, ctypes.c_uint64, P, P, ctypes.POINTER(RequestInfo)], I),
+    "vc_server_flush": ([P], I),
+    "vc_server_get_stats": ([P, ctypes.POINTER(ServerStats)], I),
     "vc_noalloc_scratch_bytes": ([], S),
     "vc_index_dim": ([P], S),
     "vc_index_size": ([P], S),
@@ -109,7 +144,7 @@ _SIGS = {
     "vc_lm_index_create": ([S, S, S, ctypes.c_uint64, S, ctypes.POINTER(P)], I),
     "vc_lm_index_destroy": ([P], I),
     "vc_lm_index_add": ([P, P, S], I),
-    "vc_lm_index_add_device": ([P, P, S], I),
+    "vc_lm_index_add_device": ([P, P, S, P], I),
     "vc_lm_index_prepare": ([P], I),
     "vc_lm_index_search": ([P, P, S, S, P, P, P, ctypes.POINTER(S)], I),
     "vc_lm_index_search_device": ([P, P, S, S, P, P, P, ctypes.POINTER(S), P], I),

@@ -23,6 +23,7 @@
 #include "vc_internal.hpp"
 #include "vectorcache_hip.h"
 #include "vc_common.hpp"
+#include "vc_index.hpp"
 
 namespace vcapi {
 
@@ -218,26 +219,6 @@ SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const f
 using namespace vcapi;
 
 // ---------------------------------------------------------------------------
-struct vc_index {
-  size_t dim = 0, bits = 0;
-  vchip::CodeTables t{};
-  int device = 0;
-  hipStream_t stream = nullptr;
-  DevBuf calib;      // shift | scale | inv_scale (3 x dim floats)
-  bool calibrated = false;
-  std::vector<float> h_shift, h_scale;
-  DevBuf tiled;      // codes, tiled layout
-  DevBuf alpha;
-  size_t cap = 0;    // vector capacity of tiled/alpha (multiple of 64)
-  size_t n_encoded = 0;
-  DevBuf pending;    // staged raw vectors
-  size_t n_pending = 0;
-  bool prepared = false;
-  DevBuf scratch;
-  DevBuf io;         // host-call staging
-  std::mutex mu;
-};
-
 extern "C" {
 
 const char* vc_last_error(void) { return g_last_error.c_str(); }
@@ -269,6 +250,10 @@ vc_status vc_index_create(size_t dim, size_t bits, vc_index** out) {
 vc_status vc_index_destroy(vc_index* ix) {
   return guarded([&] {
     if (ix == nullptr) return;
+    {
+      std::lock_guard lock(ix->mu);
+      refuse_if_served(ix);
+    }
     (void)hipStreamSynchronize(ix->stream);
     ix->calib.release();
     ix->tiled.release();
@@ -282,6 +267,7 @@ vc_status vc_index_destroy(vc_index* ix) {
 }
 
 static void index_calibrate(vc_index* ix, const float* sample_dev, size_t n) {
+  refuse_if_served(ix);
   if (ix->n_encoded > 0) fail(VC_ERR_STATE, "calibrate() must precede prepare() of any vector");
   const size_t D = ix->dim;
   ix->calib.reserve(3 * D * 4, ix->stream);
@@ -318,12 +304,18 @@ vc_status vc_index_calibrate_device(vc_index* ix, const float* sample_dev, size_
   });
 }
 
-static void index_add(vc_index* ix, const float* v, size_t n, hipMemcpyKind kind) {
+// Stages n rows for the next prepare(). Device rows are read in order after
+// the work enqueued so far on `producer`, and later work on `producer` is
+// ordered after the copy (the caller may reuse the rows' memory at once).
+static void index_add(vc_index* ix, const float* v, size_t n, hipMemcpyKind kind, hipStream_t producer = nullptr) {
+  refuse_if_served(ix);
   if (n == 0) return;
   const size_t row = ix->dim * 4;
   ix->pending.reserve((ix->n_pending + n) * row, ix->stream, /*keep=*/true);
+  if (kind == hipMemcpyDeviceToDevice) stream_after(ix->stream, producer);
   hip_check(hipMemcpyAsync(ix->pending.as<char>(ix->n_pending * row), v, n * row, kind, ix->stream), "copy");
   if (kind == hipMemcpyHostToDevice) hip_check(hipStreamSynchronize(ix->stream), "sync");
+  else stream_after(producer, ix->stream);
   ix->n_pending += n;
   ix->prepared = false;
 }
@@ -336,12 +328,11 @@ vc_status vc_index_add(vc_index* ix, const float* vectors, size_t n) {
   });
 }
 
-vc_status vc_index_add_device(vc_index* ix, const float* vectors_dev, size_t n) {
+vc_status vc_index_add_device(vc_index* ix, const float* vectors_dev, size_t n, void* stream) {
   return guarded([&] {
     if (ix == nullptr || (vectors_dev == nullptr && n > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
     std::lock_guard lock(ix->mu);
-    hip_check(hipDeviceSynchronize(), "sync caller work");  // caller's producer stream may differ
-    index_add(ix, vectors_dev, n, hipMemcpyDeviceToDevice);
+    index_add(ix, vectors_dev, n, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream));  // NULL: null stream
   });
 }
 
@@ -349,16 +340,11 @@ vc_status vc_index_prepare(vc_index* ix) {
   return guarded([&] {
     if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");
     std::lock_guard lock(ix->mu);
+    refuse_if_served(ix);
     if (ix->n_pending > 0) {
       const size_t total = ix->n_encoded + ix->n_pending;
       if (total > 0xFFFFFFFFull) fail(VC_ERR_INVALID_ARGUMENT, "index exceeds 2^32 vectors");
-      if (total > ix->cap) {
-        size_t ncap = std::max(total, ix->cap + ix->cap / 2);
-        ncap = (ncap + 63) / 64 * 64;
-        ix->tiled.reserve(vchip::tiled_bytes(ncap, ix->t.groups16), ix->stream, /*keep=*/true);
-        ix->alpha.reserve(ncap * 4, ix->stream, /*keep=*/true);
-        ix->cap = ncap;
-      }
+      index_reserve(ix, total, ix->stream);
       ix->t.debug_flags = g_debug_flags.load();
       hip_check(vchip::launch_encode(ix->t, ix->pending.as<float>(), ix->n_pending, ix->n_encoded,
                                      ix->tiled.as<uint8_t>(), nullptr, nullptr, ix->alpha.as<float>(), ix->stream),
@@ -372,10 +358,12 @@ vc_status vc_index_prepare(vc_index* ix) {
 }
 
 static size_t index_search_dev(vc_index* ix, const float* q_dev, size_t nq, size_t k, float* s_dev,
-                               uint64_t* i_dev, hipStream_t s) {
+                               uint64_t* i_dev, hipStream_t s, size_t n_prefix = SIZE_MAX) {
+  refuse_if_served(ix);
   if (!ix->prepared) fail(VC_ERR_STATE, "search() before prepare()");
-  const auto r = search_tiled(ix->t, ix->tiled.as<uint8_t>(), ix->alpha.as<float>(), ix->n_encoded, q_dev, nq, k,
-                              nullptr, s_dev, i_dev, ix->scratch, s);
+  const size_t n = std::min(n_prefix, ix->n_encoded);
+  const auto r = search_tiled(ix->t, ix->tiled.as<uint8_t>(), ix->alpha.as<float>(), n, q_dev, nq, k, nullptr, s_dev,
+                              i_dev, ix->scratch, s);
   return r.k_eff;
 }
 
@@ -408,6 +396,18 @@ vc_status vc_index_search_device(vc_index* ix, const float* queries_dev, size_t 
     std::lock_guard lock(ix->mu);
     hipStream_t s = static_cast<hipStream_t>(stream);  // NULL: the null stream
     const size_t ke = index_search_dev(ix, queries_dev, nq, k, scores_dev, ids_dev, s);
+    if (k_out) *k_out = ke;
+  });
+}
+
+vc_status vc_index_search_prefix_device(vc_index* ix, const float* queries_dev, size_t nq, size_t k, size_t n_prefix,
+                                        float* scores_dev, uint64_t* ids_dev, size_t* k_out, void* stream) {
+  return guarded([&] {
+    if (ix == nullptr || (queries_dev == nullptr && nq > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard lock(ix->mu);
+    if (n_prefix > ix->n_encoded) fail(VC_ERR_OUT_OF_RANGE, "n_prefix exceeds the encoded vectors");
+    const size_t ke = index_search_dev(ix, queries_dev, nq, k, scores_dev, ids_dev, static_cast<hipStream_t>(stream),
+                                       n_prefix);
     if (k_out) *k_out = ke;
   });
 }
@@ -448,6 +448,7 @@ vc_status vc_index_search_device_noalloc(vc_index* ix, const float* queries_dev,
   return guarded([&] {
     if (ix == nullptr || (queries_dev == nullptr && nq > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
     std::lock_guard lock(ix->mu);
+    refuse_if_served(ix);
     if (!ix->prepared) fail(VC_ERR_STATE, "search() before prepare()");
     int dev = -1;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
@@ -482,6 +483,7 @@ static void export_flat_dev(vc_index* ix, uint8_t* flat_dev) {
 }
 
 static void require_prepared(vc_index* ix) {
+  refuse_if_served(ix);
   if (ix->n_pending > 0 || !ix->prepared) fail(VC_ERR_STATE, "export before prepare()");
 }
 
@@ -1432,12 +1434,12 @@ vc_status vc_lm_index_add(vc_lm_index* ix, const float* vectors, size_t n) {
   });
 }
 
-vc_status vc_lm_index_add_device(vc_lm_index* ix, const float* vectors_dev, size_t n) {
+vc_status vc_lm_index_add_device(vc_lm_index* ix, const float* vectors_dev, size_t n, void* stream) {
   return guarded([&] {
     if (ix == nullptr || (vectors_dev == nullptr && n > 0)) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
     std::lock_guard lock(ix->mu);
-    hip_check(hipDeviceSynchronize(), "sync caller work");  // the producer's stream may differ
-    lm_add(ix, vectors_dev, n, false);
+    stream_after(ix->stream, static_cast<hipStream_t>(stream));  // rows are read after the producer's work
+    lm_add(ix, vectors_dev, n, false);  // ends with the index stream drained
   });
 }
 

@@ -106,6 +106,18 @@ struct DevBuf {
   }
 };
 
+// Orders the work enqueued next on `consumer` after the work enqueued so far
+// on `producer` (NULL: the null stream), with no host wait.
+inline void stream_after(hipStream_t consumer, hipStream_t producer) {
+  if (consumer == producer) return;
+  hipEvent_t ev;
+  hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+  const hipError_t e1 = hipEventRecord(ev, producer);
+  const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(consumer, ev, 0) : e1;
+  (void)hipEventDestroy(ev);
+  hip_check(e2, "stream ordering");
+}
+
 // Carves one scratch allocation into aligned sub-buffers.
 struct Carve {
   size_t off = 0;
@@ -145,6 +157,16 @@ struct ScratchArena {
     return grow->as<uint8_t>();
   }
 };
+
+// Scan plan and scratch carve of one search of nq queries over n vectors:
+// `bytes` of scratch (0 when there is nothing to scan), offsets into it.
+struct SearchLayout {
+  vchip::ScanPlan plan{};
+  size_t k_eff = 0;
+  size_t o_luts = 0, o_ls = 0, o_lb = 0, o_bc = 0, o_bt = 0, o_ps = 0, o_pi = 0, o_pc = 0, o_lt = 0;
+  size_t bytes = 0;
+};
+SearchLayout search_layout(const vchip::CodeTables& t, size_t n, size_t nq, size_t k);
 
 SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const float* alpha, size_t n,
                        const float* q_dev, size_t nq, size_t k, const uint64_t* id_map_dev, float* out_s,

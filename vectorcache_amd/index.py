@@ -146,12 +146,13 @@ class TurboQuantIndex:
         a = _f32(sample, self.dim())
         check(lib.vc_index_calibrate(self._h, _ptr(a), S(a.shape[0])))
 
-    def add(self, vectors):
+    def add(self, vectors, stream=None):
+        """Append rows (numpy / host, or a CUDA tensor read in order on
+        `stream`, default torch's current stream; no device-wide sync)."""
         if _is_torch(vectors) and vectors.is_cuda:
-            t = vectors.contiguous().float()
-            if t.numel() % self.dim() != 0:
-                raise ValueError("input size is not a multiple of dim")
-            check(lib.vc_index_add_device(self._h, P(t.data_ptr()), S(t.numel() // self.dim())))
+            t = _cuda_rows(vectors, self.dim(), "vectors")
+            check(lib.vc_index_add_device(self._h, P(t.data_ptr()), S(t.numel() // self.dim()),
+                                          _launch_stream(stream, t)))
             return
         a = _f32(vectors, self.dim())
         check(lib.vc_index_add(self._h, _ptr(a), S(a.shape[0])))
@@ -185,6 +186,22 @@ class TurboQuantIndex:
         st = _launch_stream(stream, q)
         check(lib.vc_index_search_device(self._h, P(q.data_ptr()), S(nq), S(k), P(scores.data_ptr()),
                                          P(ids.data_ptr()), ctypes.byref(kout), st))
+        return scores, ids
+
+    def search_prefix_device(self, queries, k: int, n_prefix: int, stream=None):
+        """search_device over the first n_prefix vectors only: what a search
+        issued while the index held n_prefix vectors returned."""
+        import torch
+
+        q = _cuda_rows(queries, self.dim(), "queries")
+        nq = q.numel() // self.dim()
+        kk = min(int(k), int(n_prefix))
+        scores = torch.empty((nq, kk), dtype=torch.float32, device=q.device)
+        ids = torch.empty((nq, kk), dtype=torch.int64, device=q.device)
+        kout = S()
+        check(lib.vc_index_search_prefix_device(self._h, P(q.data_ptr()), S(nq), S(k), S(n_prefix),
+                                                P(scores.data_ptr()), P(ids.data_ptr()), ctypes.byref(kout),
+                                                _launch_stream(stream, q)))
         return scores, ids
 
     # ---- exports in the reference formats ---------------------------------------

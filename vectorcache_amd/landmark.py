@@ -72,12 +72,13 @@ class LandmarkIndex:
         return bool(lib.vc_lm_index_is_prepared(self._h))
 
     # ---- build ----------------------------------------------------------------
-    def add(self, vectors):
+    def add(self, vectors, stream=None):
         if _is_torch(vectors) and vectors.is_cuda:
             t = vectors.contiguous().float()
             if t.numel() % self.dim() != 0:
                 raise ValueError("vectors length must be multiple of dim")
-            check(lib.vc_lm_index_add_device(self._h, P(t.data_ptr()), S(t.numel() // self.dim())))
+            check(lib.vc_lm_index_add_device(self._h, P(t.data_ptr()), S(t.numel() // self.dim()),
+                                             _launch_stream(stream, t)))
             return
         a = np.ascontiguousarray(vectors, dtype=np.float32)
         if a.size % self.dim() != 0:
