@@ -511,13 +511,15 @@ vc_status vc_vcex_cache_path(const vc_vcex_key* key, char* out, size_t cap, size
  * any k <= max_k) and add batches (host rows) from any thread; each submit
  * copies its rows and returns a ticket at once.
  *
- * Batching. `slots` workers, each with its own HIP stream, take the waiting
- * search requests in arrival order into one batch whenever they are free (at
- * most max_batch_queries queries), so requests that arrive while others are
- * on the device join the next batch a slot starts. A batch runs at the
- * largest k it holds; a request keeps the first min(k, n) entries of its
- * rows, which are its own top-k (rows are sorted by score, ties by lower id,
- * exactly as search() orders them, src/index.cpp:267-289).
+ * Batching. `slots` workers, each with its own HIP stream, take waiting
+ * search requests into one batch whenever they are free (at most
+ * max_batch_queries queries), so requests that arrive while others are on
+ * the device join the next batch a slot starts. A batch holds the requests
+ * of the oldest waiting request's k class (k in (2^(c-1), 2^c]), in arrival
+ * order, and runs at the largest k it holds; a request keeps the first
+ * min(k, n) entries of its rows, which are its own top-k (rows are sorted by
+ * score, ties by lower id, exactly as search() orders them,
+ * src/index.cpp:267-289).
  * Admission. A batch holds device memory for its query rows, LUTs, per-range
  * lists and results, carved from a pool of hbm_budget_bytes allocated at
  * create; it starts only when that memory is free, taking fewer requests if

@@ -43,6 +43,13 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// k class of a request: the smallest c with k <= 2^c
+int k_class(size_t k) {
+  int c = 0;
+  while ((size_t(1) << c) < k) ++c;
+  return c;
+}
+
 constexpr size_t kGranule = 256;
 size_t granules(size_t bytes) { return (bytes + kGranule - 1) / kGranule * kGranule; }
 
@@ -217,10 +224,15 @@ void vc_server::worker(int slot) {
   for (;;) {
     cv_work.wait(lk, [&] { return (stop && search_q.empty()) || (!search_q.empty() && !growing); });
     if (search_q.empty()) break;  // stopping, nothing left
-    // the batch: waiting requests in arrival order, up to max_batch_queries
+    // the batch: the waiting requests of the oldest request's k class (k in
+    // (2^(c-1), 2^c]: a batch runs at its largest k, so requests of a much
+    // smaller k go to batches of their own), in arrival order, up to
+    // max_batch_queries
     std::vector<Request*> batch;
     size_t nq = 0;
+    const int cls = k_class(search_q.front()->k);
     for (Request* r : search_q) {
+      if (k_class(r->k) != cls) continue;
       if (nq + r->n > p.max_batch_queries) break;
       batch.push_back(r);
       nq += r->n;
@@ -268,8 +280,19 @@ void vc_server::worker(int slot) {
     (void)placed;
     const uint64_t bid = next_batch++;
     const double t0 = now_s();
+    {
+      size_t taken = 0;  // the batch's requests leave the queue, the others keep their order
+      search_q.erase(std::remove_if(search_q.begin(), search_q.end(),
+                                    [&](Request* r) {
+                                      if (taken < batch.size() && r == batch[taken]) {
+                                        ++taken;
+                                        return true;
+                                      }
+                                      return false;
+                                    }),
+                     search_q.end());
+    }
     for (Request* r : batch) {
-      search_q.pop_front();
       r->info.t_start = t0;
       r->info.n_snapshot = n;
       r->info.batch = bid;
