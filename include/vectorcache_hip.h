@@ -504,6 +504,79 @@ vc_status vc_vcex_load(const char* path, const vc_vcex_key* key, uint64_t* ids, 
 vc_status vc_vcex_source_label(const char* npy_path, const char* dataset, char* out, size_t cap, size_t* len);
 vc_status vc_vcex_cache_path(const vc_vcex_key* key, char* out, size_t cap, size_t* len);
 
+/* ---- many collections in one launch -----------------------------------------
+ * No counterpart in the reference (one index per search call,
+ * src/index.cpp:267-289). A set holds prepared flat indexes of any dim, bits
+ * and size (each at most once), with a query batch size nq[j] and k[j] per
+ * collection, fixed at create. Its chain: per collection the query
+ * preparation and LUTs (prepare_queries), then ONE persistent scan launch over
+ * work items (collection, query tile, vector range) of every collection, then
+ * per collection the exact merge into its top-k rows; every collection's rows
+ * are bit-identical to its own vc_index_search_device of the same queries.
+ * Queries are read from, and results written to, device rows the set owns
+ * (vc_multi_rows): nq[j] x dim queries; nq[j] x k_eff scores / ids with
+ * k_eff = min(k[j], size()). While a set exists its indexes refuse to add,
+ * prepare, calibrate, be served or be destroyed. The set's work on a stream
+ * must finish before vc_multi_destroy (which synchronizes the device). */
+typedef struct vc_multi vc_multi;
+typedef struct vc_multi_plan_info {
+  size_t tile;           /* queries per work item */
+  size_t vec_per_thread; /* vectors per thread per scan step */
+  float item_scale;      /* work-item size relative to the default (about 4 items per workgroup slot) */
+  size_t items, grid;    /* work items; persistent workgroups */
+  size_t lds_bytes;      /* LDS per workgroup */
+  double measured_ms;    /* the chain's time when the plan was tuned (0: not measured) */
+} vc_multi_plan_info;
+
+vc_status vc_multi_create(vc_index* const* indexes, size_t n_sets, const size_t* nq, const size_t* k,
+                          vc_multi** out);
+vc_status vc_multi_destroy(vc_multi* set);
+vc_status vc_multi_rows(vc_multi* set, size_t j, float** queries_dev, float** scores_dev, uint64_t** ids_dev,
+                        size_t* k_eff);
+/* Stream-ordered copies into collection j's query rows (nq[j] x dim from
+ * device memory) and out of its result rows (nq[j] x k_eff each). */
+vc_status vc_multi_set_queries(vc_multi* set, size_t j, const float* queries_dev, void* stream);
+vc_status vc_multi_get_results(vc_multi* set, size_t j, float* scores_dev, uint64_t* ids_dev, void* stream);
+/* Enqueue the chain on `stream` (NULL: the null stream). */
+vc_status vc_multi_search(vc_multi* set, void* stream);
+/* Capture the chain on `stream` (not NULL) into a HIP graph, replacing a
+ * previous one; vc_multi_replay launches it on any stream. The graph holds no
+ * memset nodes. A new plan (set_plan, tune) drops the graph. */
+vc_status vc_multi_capture(vc_multi* set, void* stream);
+vc_status vc_multi_replay(vc_multi* set, void* stream);
+/* Plan: tile 1/2/4/8, vec_per_thread 1/2, item_scale > 0. The set starts with
+ * the measured plan of its signature when one is loaded (vc_tune_load), else
+ * tile 8, 2 vectors per thread, item_scale 1. */
+vc_status vc_multi_set_plan(vc_multi* set, size_t tile, size_t vec_per_thread, float item_scale);
+vc_status vc_multi_get_plan(vc_multi* set, vc_multi_plan_info* out);
+/* Times the chain for every plan (tiles 8/4/2, 2 or 1 vectors per thread,
+ * item scales 0.5/1/2; median of reps runs each) on the set's current
+ * queries, keeps the fastest and records it under the set's signature. */
+vc_status vc_multi_tune(vc_multi* set, size_t reps, vc_multi_plan_info* best);
+
+/* ---- scan-plan autotuner ------------------------------------------------------
+ * No counterpart in the reference. The single-index scan's plan (query tile,
+ * vectors per thread, vector ranges) comes from a fixed rule unless a measured
+ * entry exists for its (groups, log2 n, log2 nq, log2 k) bucket. vc_tune_scan
+ * measures every candidate (tiles 1/2/4/8, 1 or 2 vectors per thread, ranges
+ * x0.5/1/2) against the rule on pseudo-random codes of the shape and records
+ * the fastest; vc_tune_save / vc_tune_load write / read all measured entries
+ * (scan plans and multi-collection plans) as a text file, one per line. */
+typedef struct vc_scan_choice {
+  size_t tile, vec_per_thread;
+  float range_scale;
+  double ms;        /* the chosen plan's median search time */
+  double rule_ms;   /* the fixed rule's */
+  size_t rule_tile;
+} vc_scan_choice;
+vc_status vc_tune_scan(size_t dim, size_t bits, size_t n, size_t nq, size_t k, size_t reps, vc_scan_choice* out);
+vc_status vc_tune_save(const char* path);
+/* Adds the file's entries to the table; VC_ERR_STATE for a missing file or a
+ * malformed line (then nothing is added). */
+vc_status vc_tune_load(const char* path, size_t* n_entries);
+vc_status vc_tune_clear(void);
+size_t vc_tune_count(void);
+
 /* ---- request server: continuous batching over one flat index --------------
  * No counterpart in the reference (its GPU search mutates shared device
  * buffers and is not safe to call concurrently, src/gpu/kernels.cu:602-627).

@@ -78,6 +78,21 @@ class ServerStats(ctypes.Structure):
                 ("pool_bytes", ctypes.c_size_t), ("size", ctypes.c_size_t)]
 
 
+class MultiPlanInfo(ctypes.Structure):
+    """vc_multi_plan_info."""
+
+    _fields_ = [("tile", ctypes.c_size_t), ("vec_per_thread", ctypes.c_size_t), ("item_scale", ctypes.c_float),
+                ("items", ctypes.c_size_t), ("grid", ctypes.c_size_t), ("lds_bytes", ctypes.c_size_t),
+                ("measured_ms", ctypes.c_double)]
+
+
+class ScanChoice(ctypes.Structure):
+    """vc_scan_choice."""
+
+    _fields_ = [("tile", ctypes.c_size_t), ("vec_per_thread", ctypes.c_size_t), ("range_scale", ctypes.c_float),
+                ("ms", ctypes.c_double), ("rule_ms", ctypes.c_double), ("rule_tile", ctypes.c_size_t)]
+
+
 class VcexKey(ctypes.Structure):
     """vc_vcex_key: the key of a VCEX exact top-k cache file."""
 
@@ -100,6 +115,22 @@ _SIGS = {
     "vc_index_search_device": ([P, P, S, S, P, P, ctypes.POINTER(S), P], I),
     "vc_index_search_device_noalloc": ([P, P, S, S, P, P, ctypes.POINTER(S), P], I),
     "vc_index_search_prefix_device": ([P, P, S, S, S, P, P, ctypes.POINTER(S), P], I),
+    "vc_multi_create": ([P, S, P, P, ctypes.POINTER(P)], I),
+    "vc_multi_destroy": ([P], I),
+    "vc_multi_rows": ([P, S, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(S)], I),
+    "vc_multi_set_queries": ([P, S, P, P], I),
+    "vc_multi_get_results": ([P, S, P, P, P], I),
+    "vc_multi_search": ([P, P], I),
+    "vc_multi_capture": ([P, P], I),
+    "vc_multi_replay": ([P, P], I),
+    "vc_multi_set_plan": ([P, S, S, F], I),
+    "vc_multi_get_plan": ([P, ctypes.POINTER(MultiPlanInfo)], I),
+    "vc_multi_tune": ([P, S, ctypes.POINTER(MultiPlanInfo)], I),
+    "vc_tune_scan": ([S, S, S, S, S, S, ctypes.POINTER(ScanChoice)], I),
+    "vc_tune_save": ([ctypes.c_char_p], I),
+    "vc_tune_load": ([ctypes.c_char_p, ctypes.POINTER(S)], I),
+    "vc_tune_clear": ([], I),
+    "vc_tune_count": ([], S),
     "vc_server_params_default": ([ctypes.POINTER(ServerParams)], I),
     "vc_server_batch_bytes": ([P, S, S, S], S),
     "vc_server_create": ([P, ctypes.POINTER(ServerParams), ctypes.POINTER(P)], I),

@@ -28,6 +28,7 @@
 namespace vcapi {
 
 thread_local std::string g_last_error;
+thread_local const vchip::ScanPlan* g_plan_override = nullptr;
 std::atomic<int> g_forced_tile{0};
 std::atomic<int> g_debug_flags{0};
 
@@ -143,7 +144,9 @@ SearchLayout search_layout(const vchip::CodeTables& t, size_t n, size_t nq, size
   if (n > 0xFFFFFFFFull) fail(VC_ERR_INVALID_ARGUMENT, "index exceeds 2^32 vectors");
   const int ke = int(L.k_eff);
   vchip::ScanPlan& plan = L.plan;
-  plan = vchip::plan_scan(t.groups16, n, nq, ke, g_forced_tile.load(), (g_debug_flags.load() & 2) != 0);
+  plan = g_plan_override != nullptr
+             ? *g_plan_override
+             : vchip::plan_scan(t.groups16, n, nq, ke, g_forced_tile.load(), (g_debug_flags.load() & 2) != 0);
   if (plan.tile == 0) fail(VC_ERR_INVALID_ARGUMENT, "no scan configuration fits LDS for this k / tile");
   if ((g_debug_flags.load() & 8) != 0 && plan.pair) plan.vec_per_thread = 3 - plan.vec_per_thread;  // test hook
   if (const size_t fr = std::min<size_t>(unsigned(g_debug_flags.load()) >> 16, 512); fr != 0 && n <= (size_t(1) << 30)) {
@@ -253,6 +256,7 @@ vc_status vc_index_destroy(vc_index* ix) {
     {
       std::lock_guard lock(ix->mu);
       refuse_if_served(ix);
+      refuse_if_in_set(ix);
     }
     (void)hipStreamSynchronize(ix->stream);
     ix->calib.release();
@@ -268,6 +272,7 @@ vc_status vc_index_destroy(vc_index* ix) {
 
 static void index_calibrate(vc_index* ix, const float* sample_dev, size_t n) {
   refuse_if_served(ix);
+  refuse_if_in_set(ix);
   if (ix->n_encoded > 0) fail(VC_ERR_STATE, "calibrate() must precede prepare() of any vector");
   const size_t D = ix->dim;
   ix->calib.reserve(3 * D * 4, ix->stream);
@@ -309,6 +314,7 @@ vc_status vc_index_calibrate_device(vc_index* ix, const float* sample_dev, size_
 // ordered after the copy (the caller may reuse the rows' memory at once).
 static void index_add(vc_index* ix, const float* v, size_t n, hipMemcpyKind kind, hipStream_t producer = nullptr) {
   refuse_if_served(ix);
+  refuse_if_in_set(ix);
   if (n == 0) return;
   const size_t row = ix->dim * 4;
   ix->pending.reserve((ix->n_pending + n) * row, ix->stream, /*keep=*/true);
@@ -341,6 +347,7 @@ vc_status vc_index_prepare(vc_index* ix) {
     if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");
     std::lock_guard lock(ix->mu);
     refuse_if_served(ix);
+    if (ix->n_pending > 0) refuse_if_in_set(ix);
     if (ix->n_pending > 0) {
       const size_t total = ix->n_encoded + ix->n_pending;
       if (total > 0xFFFFFFFFull) fail(VC_ERR_INVALID_ARGUMENT, "index exceeds 2^32 vectors");

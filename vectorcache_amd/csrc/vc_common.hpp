@@ -33,6 +33,9 @@ struct ScanProbe {
 extern ScanProbe g_probe;
 
 extern thread_local std::string g_last_error;
+// When set, the searches of this thread use this scan plan instead of
+// plan_scan's (the autotuner's candidates).
+extern thread_local const vchip::ScanPlan* g_plan_override;
 extern std::atomic<int> g_forced_tile;
 extern std::atomic<int> g_debug_flags;
 

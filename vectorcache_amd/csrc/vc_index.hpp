@@ -32,6 +32,9 @@ struct vc_index {
   // set while a vc_server serves the index: the server owns the codes, and
   // the index's own entry points refuse to add, prepare, search or destroy
   vc_server* server = nullptr;
+  // vc_multi sets that scan the index's codes: while any exists the index
+  // refuses to add, prepare, calibrate or be destroyed
+  int sets = 0;
 };
 
 namespace vcapi {
@@ -51,6 +54,10 @@ inline void index_reserve(vc_index* ix, size_t total, hipStream_t s) {
 inline void refuse_if_served(const vc_index* ix) {
   if (ix->server != nullptr)
     fail(VC_ERR_STATE, "the index is being served: submit through its vc_server, or destroy the server first");
+}
+
+inline void refuse_if_in_set(const vc_index* ix) {
+  if (ix->sets > 0) fail(VC_ERR_STATE, "the index belongs to a vc_multi set: destroy the set first");
 }
 
 }  // namespace vcapi

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <string>
 #include <cstdint>
 
 namespace vchip {
@@ -79,6 +80,77 @@ struct ScanPlan {
 inline constexpr int kMaxTopK = 2048;
 // nibble = true selects the nibble-table kernel (two lookups per code byte).
 ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int forced_tile, bool nibble);
+// The fixed rule alone (no lookup of measured plans).
+ScanPlan plan_scan_rule(int groups16, std::size_t n, std::size_t nq, int k, int forced_tile, bool nibble);
+
+// ---- measured plans ----------------------------------------------------------
+// plan_scan's byte-pair branch first looks up a table of measured choices,
+// keyed by (groups16, floor(log2 n), ceil(log2 nq), ceil(log2 k)); a hit
+// replaces the fixed rule's tile, vectors per thread and range count (scaled
+// from plan_ranges' choice). The table starts empty; the autotuner fills it
+// and it round-trips through a text file, one entry per line:
+//   scan g16=<G16> nlog=<a> nqlog=<b> kcls=<c> tile=<T> vpt=<V> rscale=<x> ms=<t>
+struct TunedScan {
+  int tile = 0;
+  int vec_per_thread = 0;
+  float range_scale = 1.f;
+  double ms = 0.0;
+};
+struct TuneKey {
+  int g16, nlog, nqlog, kcls;
+  bool operator<(const TuneKey& o) const {
+    return g16 != o.g16 ? g16 < o.g16 : nlog != o.nlog ? nlog < o.nlog : nqlog != o.nqlog ? nqlog < o.nqlog : kcls < o.kcls;
+  }
+};
+TuneKey tune_key(int groups16, std::size_t n, std::size_t nq, int k);
+void tune_put(const TuneKey& key, const TunedScan& v);
+bool tune_get(const TuneKey& key, TunedScan* v);
+void tune_clear();
+std::size_t tune_count();
+std::string tune_dump();                      // the table as text lines
+int tune_parse(const std::string& text);      // "scan" entries read (other lines skipped), -1 if malformed
+// The plan of one explicit choice (tile, vectors per thread, range scale),
+// LDS permitting; tile 0 when it does not fit.
+ScanPlan plan_scan_with(int groups16, std::size_t n, std::size_t nq, int k, int tile, int vec_per_thread,
+                        float range_scale);
+
+// ---- many code sets in one launch ------------------------------------------
+// One code set of a multi-set scan, as the device reads it.
+struct MultiColl {
+  const std::uint8_t* tiled;
+  const float* alpha;
+  const std::uint8_t* src;  // pair-source rows of its query tiles (pair_source_bytes)
+  const float* lut_scale;
+  const float* bias_total;
+  float* part_s;            // [nq][n_lists][k]
+  std::uint32_t* part_i;
+  int* part_cnt;            // [nq][n_lists]
+  long n, nq;
+  int G16, k, cap, n_lists;
+};
+// One work item: query tile qt of code set `coll` over vectors [v_begin, v_end), writing list `list`.
+struct MultiItem {
+  int coll;
+  int list;
+  long qt;
+  long v_begin, v_end;
+};
+// Bytes of the pair-source rows of nq queries for a tile-T scan.
+std::size_t pair_source_bytes(int tile, int groups16, std::size_t nq);
+hipError_t launch_pair_source(int tile, const std::uint8_t* luts, std::size_t nq, int groups, int groups16,
+                              std::uint8_t* out, hipStream_t s);
+// LDS the scan holds besides its top-k state (table + stage), and the top-k
+// state of a T-query tile at top-k with a cap-candidate buffer.
+std::size_t pair_fixed_lds();
+std::size_t pair_topk_lds(int tile, int k, int cap);
+inline constexpr int kLdsPerCu = 160 * 1024;
+inline constexpr int kMaxCandCap = 512;
+// Sets the kernel's dynamic-LDS limit (outside graph capture), then launches.
+hipError_t prepare_multi_scan(int tile, int vec_per_thread, std::size_t dyn_lds);
+hipError_t launch_multi_scan(int tile, int vec_per_thread, const MultiColl* colls, const MultiItem* items,
+                             long n_items, std::size_t dyn_lds, int grid, hipStream_t s);
+// Deterministic pseudo-random rows (autotuning inputs): n x dim floats.
+hipError_t launch_fill_rows(float* x, std::size_t n, int dim, std::uint64_t seed, hipStream_t s);
 
 // Bytes of the per-query-tile LUT rows (interleaved / pair source) the scan needs.
 std::size_t scan_lut_bytes(const ScanPlan& p, int groups16, std::size_t nq);

@@ -10,7 +10,12 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
 #include <utility>
+#include <vector>
 
 #include "vc_internal.hpp"
 #include "vc_ivf.hpp"
@@ -1796,12 +1801,21 @@ __device__ __forceinline__ void pair_chunk(const uint8_t* __restrict__ table, ui
 #ifndef VC_PAIR_MIN_WAVES
 #define VC_PAIR_MIN_WAVES 8  // waves per SIMD the register budget must allow
 #endif
+// One work item of the byte-pair scan: the T queries of one tile (first
+// query q0; their source rows at tile_src) over vectors [r_begin, r_end) of
+// a code set, ending in the item's sorted lists (list `list` of each query's
+// n_lists). table / stage: the caller's static LDS; smem: dynamic LDS of at
+// least topk_lds(T, k, cap) bytes.
 template <int T, int V>
-__global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void scan_pair_kernel(
-    const uint8_t* __restrict__ tiled, const float* __restrict__ alpha, long n, int G16,
-    const uint8_t* __restrict__ src, const float* __restrict__ lut_scale,
-    const float* __restrict__ bias_total, long nq, int k, int cap, int n_ranges, long range_len,
-    float* __restrict__ part_s, uint32_t* __restrict__ part_i, int* __restrict__ part_cnt) {
+__device__ __forceinline__ void scan_pair_body(uint8_t* __restrict__ table, uint8_t* __restrict__ stage,
+                                               unsigned char* __restrict__ smem, const uint8_t* __restrict__ tiled,
+                                               const float* __restrict__ alpha, int G16,
+                                               const uint8_t* __restrict__ tile_src,
+                                               const float* __restrict__ lut_scale,
+                                               const float* __restrict__ bias_total, long q0, long nq, int k, int cap,
+                                               long r_begin, long r_end, int n_lists, int list,
+                                               float* __restrict__ part_s, uint32_t* __restrict__ part_i,
+                                               int* __restrict__ part_cnt) {
   constexpr int EB = pair_entry_bytes<T>();
   constexpr int CG = kPairTableBytes / (256 * EB);  // groups per table chunk
   constexpr int CB = CG / 16;                       // 16-group code blocks per chunk
@@ -1809,19 +1823,7 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void scan_pair_ker
   constexpr int NR = (T == 1 && V == 1) ? 4 : 2;  // code blocks in flight per vector
   static_assert(CB % NR == 0, "a chunk holds whole rounds of the code-block ring");
   constexpr int NA = pair_acc_words<T>();
-
-  // the table at a fixed LDS address, so entry offsets fold into the reads
-  __shared__ __align__(16) uint8_t table[kPairTableBytes];
-  __shared__ __align__(16) uint8_t stage[kPairStageBytes];
-  extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x;
-  const long n_qt = (nq + T - 1) / T;
-  const long L = blockIdx.x;
-  const int r = (int)(L / n_qt);   // range-major: concurrent workgroups share codes
-  const long qt = L % n_qt;
-  const long q0 = qt * T;
-  const long r_begin = (long)r * range_len;
-  const long r_end = min(n, r_begin + range_len);
 
   TopkShared sh;
   init_topk(sh, smem, T, k, cap);
@@ -1838,7 +1840,6 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void scan_pair_ker
   // the source rows are padded to whole chunks, so every DMA is in bounds;
   // chunk 0's rows are staged by plain loads before the first build
   const int n_chunks = (G16 + CB - 1) / CB;
-  const uint8_t* tile_src = src + qt * ((long)n_chunks * kPairStageBytes);
   reinterpret_cast<uint64_t*>(stage)[tid] = reinterpret_cast<const uint64_t*>(tile_src)[tid];
   __syncthreads();
   bool built = false;
@@ -1886,7 +1887,52 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void scan_pair_ker
     }
     admit_scores<T, V>(sh, fa, vv, valid, alpha, q0, nq, k, cap);
   }
-  write_partials<T>(sh, q0, nq, n_ranges, r, k, cap, part_s, part_i, part_cnt);
+  write_partials<T>(sh, q0, nq, n_lists, list, k, cap, part_s, part_i, part_cnt);
+}
+
+template <int T, int V>
+__global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void scan_pair_kernel(
+    const uint8_t* __restrict__ tiled, const float* __restrict__ alpha, long n, int G16,
+    const uint8_t* __restrict__ src, const float* __restrict__ lut_scale,
+    const float* __restrict__ bias_total, long nq, int k, int cap, int n_ranges, long range_len,
+    float* __restrict__ part_s, uint32_t* __restrict__ part_i, int* __restrict__ part_cnt) {
+  constexpr int CB = kPairTableBytes / (256 * pair_entry_bytes<T>()) / 16;
+  // the table at a fixed LDS address, so entry offsets fold into the reads
+  __shared__ __align__(16) uint8_t table[kPairTableBytes];
+  __shared__ __align__(16) uint8_t stage[kPairStageBytes];
+  extern __shared__ __align__(16) unsigned char smem[];
+  const long n_qt = (nq + T - 1) / T;
+  const long L = blockIdx.x;
+  const int r = (int)(L / n_qt);   // range-major: concurrent workgroups share codes
+  const long qt = L % n_qt;
+  const long r_begin = (long)r * range_len;
+  const long r_end = min(n, r_begin + range_len);
+  const int n_chunks = (G16 + CB - 1) / CB;
+  scan_pair_body<T, V>(table, stage, smem, tiled, alpha, G16, src + qt * ((long)n_chunks * kPairStageBytes),
+                       lut_scale, bias_total, qt * T, nq, k, cap, r_begin, r_end, n_ranges, r, part_s, part_i,
+                       part_cnt);
+}
+
+// Many code sets in one persistent launch: each workgroup walks the work
+// items blockIdx.x, blockIdx.x + gridDim.x, ... and runs each as the
+// single-set scan runs its (tile, range) workgroup, so every item's lists are
+// bit-identical to what a search of its set alone would merge.
+template <int T, int V>
+__global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void multi_scan_kernel(
+    const MultiColl* __restrict__ colls, const MultiItem* __restrict__ items, long n_items) {
+  constexpr int CB = kPairTableBytes / (256 * pair_entry_bytes<T>()) / 16;
+  __shared__ __align__(16) uint8_t table[kPairTableBytes];
+  __shared__ __align__(16) uint8_t stage[kPairStageBytes];
+  extern __shared__ __align__(16) unsigned char smem[];
+  for (long it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const MultiItem w = items[it];
+    const MultiColl c = colls[w.coll];
+    const int n_chunks = (c.G16 + CB - 1) / CB;
+    scan_pair_body<T, V>(table, stage, smem, c.tiled, c.alpha, c.G16,
+                         c.src + w.qt * ((long)n_chunks * kPairStageBytes), c.lut_scale, c.bias_total, w.qt * T,
+                         c.nq, c.k, c.cap, w.v_begin, w.v_end, c.n_lists, w.list, c.part_s, c.part_i, c.part_cnt);
+    __syncthreads();  // the next item reuses the LDS
+  }
 }
 
 // Merge the per-range sorted lists of one query into its final row (one
@@ -2551,6 +2597,20 @@ __global__ __launch_bounds__(256) void merge_results_kernel(long nq, int kd, flo
 // ---------------------------------------------------------------------------
 // Format conversions (src/pack/pack.cpp:98-186,403-420)
 // ---------------------------------------------------------------------------
+// Pseudo-random rows for autotuning inputs: element i = (sum of two
+// splitmix64 uniforms in [0, 1)) - 1, a triangular value in (-1, 1).
+__global__ __launch_bounds__(256) void fill_rows_kernel(float* __restrict__ x, long total, uint64_t seed) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const float u0 = (float)(uint32_t)(z >> 40) * (1.f / 16777216.f);
+    const float u1 = (float)(uint32_t)(z & 0xFFFFFF) * (1.f / 16777216.f);
+    x[i] = u0 + u1 - 1.f;
+  }
+}
+
 __device__ __forceinline__ long tiled_offset(long v, int g, int G16) {
   return ((v >> 6) * G16 + (g >> 4)) * 1024 + (v & 63) * 16 + (g & 15);
 }
@@ -3124,34 +3184,75 @@ static void plan_ranges(ScanPlan& p, std::size_t n, std::size_t nq) {
   p.range_len = (int)len;
 }
 
+// Byte-pair plan of tile T at top-k: the largest candidate buffer (at least
+// 32) that still lets two workgroups share a CU, else 512 with one workgroup
+// per CU (measured: k = 64 at T = 8 scans faster with 32 candidates and two
+// workgroups). False when even one workgroup does not fit.
+static bool pair_fit(int T, int k, ScanPlan& p) {
+  const std::size_t fixed = (std::size_t)kPairTableBytes + kPairStageBytes;
+  int cap = kCandCap;
+  std::size_t lds = fixed + topk_lds(T, k, cap);
+  if (lds > (std::size_t)kLdsBudget / 2) {
+    const long room = (long)kLdsBudget / 2 - (long)fixed - (long)topk_lds(T, k, 0);
+    const long c2 = room / (8L * T) / 32 * 32;
+    if (c2 >= 32) {
+      cap = (int)std::min<long>(c2, kCandCap);
+      lds = fixed + topk_lds(T, k, cap);
+    }
+  }
+  if (lds > (std::size_t)kLdsBudget) return false;
+  p.tile = T;
+  p.pair = 1;
+  p.cand_cap = cap;
+  p.lds_bytes = lds;
+  p.lut_chunk16 = 0;
+  return true;
+}
+
+// plan_ranges' split scaled by `scale` (at least 1 range, at most 512 and
+// one step of vectors per range).
+static void scale_ranges(ScanPlan& p, std::size_t n, float scale) {
+  if (scale == 1.f) return;
+  const long step = (long)kScanThreads * p.vec_per_thread;
+  const long max_by_n = std::max(1L, (long)((n + step - 1) / step));
+  long R = std::lround((double)p.n_ranges * scale);
+  R = std::max(1L, std::min(R, std::min(max_by_n, 512L)));
+  long len = (long)((n + R - 1) / R);
+  len = (len + 63) / 64 * 64;
+  p.n_ranges = (int)std::max(1L, (long)((n + len - 1) / len));
+  p.range_len = (int)len;
+}
+
+ScanPlan plan_scan_with(int groups16, std::size_t n, std::size_t nq, int k, int tile, int vec_per_thread,
+                        float range_scale) {
+  (void)groups16;
+  ScanPlan p{};
+  if (tile != 1 && tile != 2 && tile != 4 && tile != 8) return p;
+  if (!pair_fit(tile, k, p)) return ScanPlan{};
+  p.vec_per_thread = vec_per_thread == 1 ? 1 : 2;
+  plan_ranges(p, n, nq);
+  scale_ranges(p, n, range_scale);
+  return p;
+}
+
 ScanPlan plan_scan(int groups16, std::size_t n, std::size_t nq, int k, int forced_tile, bool nibble) {
+  TunedScan tuned;
+  if (!nibble && forced_tile == 0 && tune_get(tune_key(groups16, n, nq, k), &tuned)) {
+    const ScanPlan p = plan_scan_with(groups16, n, nq, k, tuned.tile, tuned.vec_per_thread, tuned.range_scale);
+    if (p.tile != 0) return p;
+  }
+  return plan_scan_rule(groups16, n, nq, k, forced_tile, nibble);
+}
+
+ScanPlan plan_scan_rule(int groups16, std::size_t n, std::size_t nq, int k, int forced_tile, bool nibble) {
   ScanPlan p{};
   if (!nibble) {
     // Byte-pair kernel. Measured on 1M x 1536 4-bit: one query per tile is
     // fastest up to two queries, a 4-query tile for three or four, 8 above.
     const int want = nq >= 5 ? 8 : nq >= 3 ? 4 : 1;
-    const std::size_t fixed = (std::size_t)kPairTableBytes + kPairStageBytes;
     for (int T : {8, 4, 2, 1}) {
       if (T > (forced_tile != 0 ? forced_tile : want)) continue;  // a forced tile is an upper bound
-      // largest candidate buffer (at least 32) that still lets two
-      // workgroups share a CU, else 512 with one workgroup per CU (measured:
-      // k = 64 at T = 8 scans faster with 32 candidates and two workgroups)
-      int cap = kCandCap;
-      std::size_t lds = fixed + topk_lds(T, k, cap);
-      if (lds > (std::size_t)kLdsBudget / 2) {
-        const long room = (long)kLdsBudget / 2 - (long)fixed - (long)topk_lds(T, k, 0);
-        const long c2 = room / (8L * T) / 32 * 32;
-        if (c2 >= 32) {
-          cap = (int)std::min<long>(c2, kCandCap);
-          lds = fixed + topk_lds(T, k, cap);
-        }
-      }
-      if (lds > (std::size_t)kLdsBudget) continue;
-      p.tile = T;
-      p.pair = 1;
-      p.cand_cap = cap;
-      p.lds_bytes = lds;
-      p.lut_chunk16 = 0;
+      if (!pair_fit(T, k, p)) continue;
       p.vec_per_thread = T == 1 ? 1 : 2;  // measured; T = 1 keeps a deeper code prefetch instead
       break;
     }
@@ -3269,6 +3370,163 @@ hipError_t launch_merge_ranges(const float* part_s, const std::uint32_t* part_i,
     hipLaunchKernelGGL(merge_ranges_kernel, dim3(grid), dim3(64), 0, s, part_s, part_i, part_cnt, (long)nq,
                        n_ranges, k, k_out, id_map, out_s, out_i);
   }
+  return hipGetLastError();
+}
+
+// ---- measured plans ----------------------------------------------------------
+namespace {
+std::mutex g_tune_mu;
+std::map<TuneKey, TunedScan> g_tuned;
+
+int floor_log2(std::size_t v) {
+  int r = 0;
+  while (v > 1) {
+    v >>= 1;
+    ++r;
+  }
+  return r;
+}
+int ceil_log2(std::size_t v) {
+  int r = 0;
+  while ((std::size_t(1) << r) < v) ++r;
+  return r;
+}
+}  // namespace
+
+TuneKey tune_key(int groups16, std::size_t n, std::size_t nq, int k) {
+  return TuneKey{groups16, floor_log2(std::max<std::size_t>(n, 1)), ceil_log2(std::max<std::size_t>(nq, 1)),
+                 ceil_log2((std::size_t)std::max(k, 1))};
+}
+
+void tune_put(const TuneKey& key, const TunedScan& v) {
+  std::lock_guard lock(g_tune_mu);
+  g_tuned[key] = v;
+}
+
+bool tune_get(const TuneKey& key, TunedScan* v) {
+  std::lock_guard lock(g_tune_mu);
+  if (g_tuned.empty()) return false;
+  auto it = g_tuned.find(key);
+  if (it == g_tuned.end()) return false;
+  *v = it->second;
+  return true;
+}
+
+void tune_clear() {
+  std::lock_guard lock(g_tune_mu);
+  g_tuned.clear();
+}
+
+std::size_t tune_count() {
+  std::lock_guard lock(g_tune_mu);
+  return g_tuned.size();
+}
+
+std::string tune_dump() {
+  std::lock_guard lock(g_tune_mu);
+  std::string out = "# vectorcache-hip measured scan plans (gfx950)\n";
+  char line[256];
+  for (const auto& [k, v] : g_tuned) {
+    std::snprintf(line, sizeof line, "scan g16=%d nlog=%d nqlog=%d kcls=%d tile=%d vpt=%d rscale=%.4g ms=%.6g\n",
+                  k.g16, k.nlog, k.nqlog, k.kcls, v.tile, v.vec_per_thread, (double)v.range_scale, v.ms);
+    out += line;
+  }
+  return out;
+}
+
+int tune_parse(const std::string& text) {
+  std::vector<std::pair<TuneKey, TunedScan>> got;
+  std::size_t pos = 0;
+  while (pos < text.size()) {
+    std::size_t end = text.find('\n', pos);
+    if (end == std::string::npos) end = text.size();
+    const std::string line = text.substr(pos, end - pos);
+    pos = end + 1;
+    if (line.rfind("scan ", 0) != 0) continue;  // comments and other kinds of entries
+    TuneKey k{};
+    TunedScan v{};
+    double rs = 1.0;
+    if (std::sscanf(line.c_str(), "scan g16=%d nlog=%d nqlog=%d kcls=%d tile=%d vpt=%d rscale=%lf ms=%lf", &k.g16,
+                    &k.nlog, &k.nqlog, &k.kcls, &v.tile, &v.vec_per_thread, &rs, &v.ms) != 8)
+      return -1;
+    if ((v.tile != 1 && v.tile != 2 && v.tile != 4 && v.tile != 8) || (v.vec_per_thread != 1 && v.vec_per_thread != 2) ||
+        !(rs > 0.0 && rs <= 64.0) || k.g16 < 1)
+      return -1;
+    v.range_scale = (float)rs;
+    got.emplace_back(k, v);
+  }
+  std::lock_guard lock(g_tune_mu);
+  for (const auto& [k, v] : got) g_tuned[k] = v;
+  return (int)got.size();
+}
+
+// ---- many code sets in one launch ------------------------------------------
+std::size_t pair_source_bytes(int tile, int groups16, std::size_t nq) {
+  const std::size_t n_qt = (nq + tile - 1) / tile;
+  const std::size_t per_elem = (tile == 2 || tile == 4) ? 2 : 1;
+  return n_qt * (std::size_t)pair_src_blocks(tile, groups16) * 16 * 32 * tile * per_elem;
+}
+
+hipError_t launch_pair_source(int tile, const std::uint8_t* luts, std::size_t nq, int groups, int groups16,
+                              std::uint8_t* out, hipStream_t s) {
+  if (nq == 0) return hipSuccess;
+  const long n_qt = (long)((nq + tile - 1) / tile);
+  const long elems = n_qt * pair_src_blocks(tile, groups16) * 16 * 32 * tile;
+  const dim3 grid(grid_for(elems, 256));
+  switch (tile) {
+    case 8: hipLaunchKernelGGL(pair_source_kernel<8>, grid, dim3(256), 0, s, luts, (long)nq, groups, groups16, out); break;
+    case 4: hipLaunchKernelGGL(pair_source_kernel<4>, grid, dim3(256), 0, s, luts, (long)nq, groups, groups16, out); break;
+    case 2: hipLaunchKernelGGL(pair_source_kernel<2>, grid, dim3(256), 0, s, luts, (long)nq, groups, groups16, out); break;
+    case 1: hipLaunchKernelGGL(pair_source_kernel<1>, grid, dim3(256), 0, s, luts, (long)nq, groups, groups16, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+std::size_t pair_fixed_lds() { return (std::size_t)kPairTableBytes + kPairStageBytes; }
+std::size_t pair_topk_lds(int tile, int k, int cap) { return topk_lds(tile, k, cap); }
+
+namespace {
+template <int T, int V>
+hipError_t multi_attr(std::size_t dyn) {
+  return hipFuncSetAttribute((const void*)multi_scan_kernel<T, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)dyn);
+}
+}  // namespace
+
+hipError_t prepare_multi_scan(int tile, int vec_per_thread, std::size_t dyn_lds) {
+  const bool v2 = vec_per_thread == 2;
+  switch (tile) {
+    case 8: return v2 ? multi_attr<8, 2>(dyn_lds) : multi_attr<8, 1>(dyn_lds);
+    case 4: return v2 ? multi_attr<4, 2>(dyn_lds) : multi_attr<4, 1>(dyn_lds);
+    case 2: return v2 ? multi_attr<2, 2>(dyn_lds) : multi_attr<2, 1>(dyn_lds);
+    case 1: return v2 ? multi_attr<1, 2>(dyn_lds) : multi_attr<1, 1>(dyn_lds);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_multi_scan(int tile, int vec_per_thread, const MultiColl* colls, const MultiItem* items,
+                             long n_items, std::size_t dyn_lds, int grid, hipStream_t s) {
+  if (n_items == 0) return hipSuccess;
+  const dim3 g((unsigned)std::max(1, grid)), b(kScanThreads);
+  const bool v2 = vec_per_thread == 2;
+#define VC_MULTI(TT, VV) \
+  hipLaunchKernelGGL((multi_scan_kernel<TT, VV>), g, b, dyn_lds, s, colls, items, n_items)
+  switch (tile) {
+    case 8: if (v2) VC_MULTI(8, 2); else VC_MULTI(8, 1); break;
+    case 4: if (v2) VC_MULTI(4, 2); else VC_MULTI(4, 1); break;
+    case 2: if (v2) VC_MULTI(2, 2); else VC_MULTI(2, 1); break;
+    case 1: if (v2) VC_MULTI(1, 2); else VC_MULTI(1, 1); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef VC_MULTI
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_rows(float* x, std::size_t n, int dim, std::uint64_t seed, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_rows_kernel, dim3(grid_for((long)(n * dim), 256, 65536)), dim3(256), 0, s, x,
+                     (long)(n * dim), seed);
   return hipGetLastError();
 }
 

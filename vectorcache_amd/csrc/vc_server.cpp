@@ -440,6 +440,7 @@ vc_status vc_server_create(vc_index* ix, const vc_server_params* params, vc_serv
     if (p.add_chunk < 1) fail(VC_ERR_INVALID_ARGUMENT, "add_chunk must be >= 1");
     std::lock_guard ilk(ix->mu);
     refuse_if_served(ix);
+    refuse_if_in_set(ix);
     if (!ix->prepared || ix->n_pending > 0) fail(VC_ERR_STATE, "the server needs a prepared index");
     int dev = -1;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
