@@ -520,9 +520,11 @@ vc_status vc_vcex_cache_path(const vc_vcex_key* key, char* out, size_t cap, size
  * must finish before vc_multi_destroy (which synchronizes the device). */
 typedef struct vc_multi vc_multi;
 typedef struct vc_multi_plan_info {
-  size_t tile;           /* queries per work item */
-  size_t vec_per_thread; /* vectors per thread per scan step */
-  float item_scale;      /* work-item size relative to the default (about 4 items per workgroup slot) */
+  size_t tile;              /* largest query tile: a collection takes the smallest tile (1, 2, 4, 8) that
+                               holds its batch, at most this, halved while its top-k state does not fit */
+  size_t vec_per_thread;    /* vectors per thread per scan step (tiles >= 2) */
+  float item_scale;         /* work-item size relative to the default (about 4 items per workgroup slot) */
+  size_t workgroups_per_cu; /* 2: every collection's LDS fits half a CU; 1: one workgroup per CU */
   size_t items, grid;    /* work items; persistent workgroups */
   size_t lds_bytes;      /* LDS per workgroup */
   double measured_ms;    /* the chain's time when the plan was tuned (0: not measured) */
@@ -544,14 +546,19 @@ vc_status vc_multi_search(vc_multi* set, void* stream);
  * memset nodes. A new plan (set_plan, tune) drops the graph. */
 vc_status vc_multi_capture(vc_multi* set, void* stream);
 vc_status vc_multi_replay(vc_multi* set, void* stream);
-/* Plan: tile 1/2/4/8, vec_per_thread 1/2, item_scale > 0. The set starts with
- * the measured plan of its signature when one is loaded (vc_tune_load), else
- * tile 8, 2 vectors per thread, item_scale 1. */
-vc_status vc_multi_set_plan(vc_multi* set, size_t tile, size_t vec_per_thread, float item_scale);
+/* Plan: tile 1/2/4/8, vec_per_thread 1/2, item_scale > 0, workgroups per CU
+ * 1/2. The set starts with the measured plan of its signature when one is
+ * loaded (vc_tune_load), else tile 8, 2 vectors per thread, item_scale 1, 2
+ * workgroups per CU. */
+vc_status vc_multi_set_plan(vc_multi* set, size_t tile, size_t vec_per_thread, float item_scale,
+                            size_t workgroups_per_cu);
 vc_status vc_multi_get_plan(vc_multi* set, vc_multi_plan_info* out);
-/* Times the chain for every plan (tiles 8/4/2, 2 or 1 vectors per thread,
- * item scales 0.5/1/2; median of reps runs each) on the set's current
- * queries, keeps the fastest and records it under the set's signature. */
+/* Collection j's query tile and vector ranges (lists per query) under the plan. */
+vc_status vc_multi_collection_plan(vc_multi* set, size_t j, size_t* tile, size_t* lists);
+/* Times the chain for every plan (2 or 1 workgroups per CU, largest tile 8
+ * or 4, 2 or 1 vectors per thread, item scales 0.5/1/2; median of reps runs
+ * each) on the set's current queries, keeps the fastest and records it under
+ * the set's signature. */
 vc_status vc_multi_tune(vc_multi* set, size_t reps, vc_multi_plan_info* best);
 
 /* ---- scan-plan autotuner ------------------------------------------------------

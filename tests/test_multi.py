@@ -76,10 +76,11 @@ def test_eager_equals_standalone_under_every_plan(collections):
         ms.search()
         _check(ms, want, "default")
         print(f"[multi] default plan {ms.plan()}", flush=True)
-        for tile, vpt, scale in ((8, 1, 0.5), (4, 2, 2.0), (2, 1, 1.0), (1, 2, 4.0), (8, 2, 0.25)):
-            ms.set_plan(tile, vpt, scale)
+        for tile, vpt, scale, wg in ((8, 1, 0.5, 2), (4, 2, 2.0, 1), (2, 1, 1.0, 2), (1, 2, 4.0, 1), (8, 2, 0.25, 1),
+                                     (8, 2, 1.0, 2)):
+            ms.set_plan(tile, vpt, scale, wg)
             ms.search()
-            _check(ms, want, (tile, vpt, scale))
+            _check(ms, want, (tile, vpt, scale, wg))
     finally:
         ms.close()
 
@@ -137,7 +138,8 @@ def test_tuned_plan_and_tuning_file(collections, tmp_path):
     ms2 = multi.MultiSearch([c[0] for c in collections], [s[3] for s in SPECS], [s[4] for s in SPECS])
     try:
         p2 = ms2.plan()
-        assert (p2.tile, p2.vec_per_thread, p2.item_scale) == (best.tile, best.vec_per_thread, best.item_scale)
+        assert (p2.tile, p2.vec_per_thread, p2.item_scale, p2.workgroups_per_cu) == (
+            best.tile, best.vec_per_thread, best.item_scale, best.workgroups_per_cu)
     finally:
         ms2.close()
     bad = tmp_path / "bad.txt"

@@ -82,8 +82,8 @@ class MultiPlanInfo(ctypes.Structure):
     """vc_multi_plan_info."""
 
     _fields_ = [("tile", ctypes.c_size_t), ("vec_per_thread", ctypes.c_size_t), ("item_scale", ctypes.c_float),
-                ("items", ctypes.c_size_t), ("grid", ctypes.c_size_t), ("lds_bytes", ctypes.c_size_t),
-                ("measured_ms", ctypes.c_double)]
+                ("workgroups_per_cu", ctypes.c_size_t), ("items", ctypes.c_size_t), ("grid", ctypes.c_size_t),
+                ("lds_bytes", ctypes.c_size_t), ("measured_ms", ctypes.c_double)]
 
 
 class ScanChoice(ctypes.Structure):
@@ -123,8 +123,9 @@ _SIGS = {
     "vc_multi_search": ([P, P], I),
     "vc_multi_capture": ([P, P], I),
     "vc_multi_replay": ([P, P], I),
-    "vc_multi_set_plan": ([P, S, S, F], I),
+    "vc_multi_set_plan": ([P, S, S, F, S], I),
     "vc_multi_get_plan": ([P, ctypes.POINTER(MultiPlanInfo)], I),
+    "vc_multi_collection_plan": ([P, S, ctypes.POINTER(S), ctypes.POINTER(S)], I),
     "vc_multi_tune": ([P, S, ctypes.POINTER(MultiPlanInfo)], I),
     "vc_tune_scan": ([S, S, S, S, S, S, ctypes.POINTER(ScanChoice)], I),
     "vc_tune_save": ([ctypes.c_char_p], I),

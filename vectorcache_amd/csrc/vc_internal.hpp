@@ -127,6 +127,7 @@ struct MultiColl {
   int* part_cnt;            // [nq][n_lists]
   long n, nq;
   int G16, k, cap, n_lists;
+  int tile;                 // queries per work item: 1, 2, 4 or 8
 };
 // One work item: query tile qt of code set `coll` over vectors [v_begin, v_end), writing list `list`.
 struct MultiItem {
@@ -145,10 +146,11 @@ std::size_t pair_fixed_lds();
 std::size_t pair_topk_lds(int tile, int k, int cap);
 inline constexpr int kLdsPerCu = 160 * 1024;
 inline constexpr int kMaxCandCap = 512;
-// Sets the kernel's dynamic-LDS limit (outside graph capture), then launches.
-hipError_t prepare_multi_scan(int tile, int vec_per_thread, std::size_t dyn_lds);
-hipError_t launch_multi_scan(int tile, int vec_per_thread, const MultiColl* colls, const MultiItem* items,
-                             long n_items, std::size_t dyn_lds, int grid, hipStream_t s);
+// Sets the kernel's dynamic-LDS limit (outside graph capture); the launch.
+// vec_per_thread applies to the code sets of tile >= 2 (tile 1 runs 1).
+hipError_t prepare_multi_scan(int vec_per_thread, std::size_t dyn_lds);
+hipError_t launch_multi_scan(int vec_per_thread, const MultiColl* colls, const MultiItem* items, long n_items,
+                             std::size_t dyn_lds, int grid, hipStream_t s);
 // Deterministic pseudo-random rows (autotuning inputs): n x dim floats.
 hipError_t launch_fill_rows(float* x, std::size_t n, int dim, std::uint64_t seed, hipStream_t s);
 

@@ -1915,22 +1915,34 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void scan_pair_ker
 
 // Many code sets in one persistent launch: each workgroup walks the work
 // items blockIdx.x, blockIdx.x + gridDim.x, ... and runs each as the
-// single-set scan runs its (tile, range) workgroup, so every item's lists are
-// bit-identical to what a search of its set alone would merge.
+// single-set scan runs its (tile, range) workgroup, at its code set's query
+// tile (1, 2, 4 or 8; tile 1 always with one vector per thread), so every
+// item's lists are bit-identical to what a search of its set alone merges.
 template <int T, int V>
+__device__ __forceinline__ void multi_item(uint8_t* table, uint8_t* stage, unsigned char* smem, const MultiColl& c,
+                                           const MultiItem& w) {
+  constexpr int CB = kPairTableBytes / (256 * pair_entry_bytes<T>()) / 16;
+  const int n_chunks = (c.G16 + CB - 1) / CB;
+  scan_pair_body<T, V>(table, stage, smem, c.tiled, c.alpha, c.G16, c.src + w.qt * ((long)n_chunks * kPairStageBytes),
+                       c.lut_scale, c.bias_total, w.qt * T, c.nq, c.k, c.cap, w.v_begin, w.v_end, c.n_lists, w.list,
+                       c.part_s, c.part_i, c.part_cnt);
+}
+
+template <int V>
 __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void multi_scan_kernel(
     const MultiColl* __restrict__ colls, const MultiItem* __restrict__ items, long n_items) {
-  constexpr int CB = kPairTableBytes / (256 * pair_entry_bytes<T>()) / 16;
   __shared__ __align__(16) uint8_t table[kPairTableBytes];
   __shared__ __align__(16) uint8_t stage[kPairStageBytes];
   extern __shared__ __align__(16) unsigned char smem[];
   for (long it = blockIdx.x; it < n_items; it += gridDim.x) {
     const MultiItem w = items[it];
     const MultiColl c = colls[w.coll];
-    const int n_chunks = (c.G16 + CB - 1) / CB;
-    scan_pair_body<T, V>(table, stage, smem, c.tiled, c.alpha, c.G16,
-                         c.src + w.qt * ((long)n_chunks * kPairStageBytes), c.lut_scale, c.bias_total, w.qt * T,
-                         c.nq, c.k, c.cap, w.v_begin, w.v_end, c.n_lists, w.list, c.part_s, c.part_i, c.part_cnt);
+    switch (c.tile) {
+      case 8: multi_item<8, V>(table, stage, smem, c, w); break;
+      case 4: multi_item<4, V>(table, stage, smem, c, w); break;
+      case 2: multi_item<2, V>(table, stage, smem, c, w); break;
+      default: multi_item<1, 1>(table, stage, smem, c, w); break;
+    }
     __syncthreads();  // the next item reuses the LDS
   }
 }
@@ -3486,40 +3498,19 @@ hipError_t launch_pair_source(int tile, const std::uint8_t* luts, std::size_t nq
 std::size_t pair_fixed_lds() { return (std::size_t)kPairTableBytes + kPairStageBytes; }
 std::size_t pair_topk_lds(int tile, int k, int cap) { return topk_lds(tile, k, cap); }
 
-namespace {
-template <int T, int V>
-hipError_t multi_attr(std::size_t dyn) {
-  return hipFuncSetAttribute((const void*)multi_scan_kernel<T, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)dyn);
-}
-}  // namespace
-
-hipError_t prepare_multi_scan(int tile, int vec_per_thread, std::size_t dyn_lds) {
-  const bool v2 = vec_per_thread == 2;
-  switch (tile) {
-    case 8: return v2 ? multi_attr<8, 2>(dyn_lds) : multi_attr<8, 1>(dyn_lds);
-    case 4: return v2 ? multi_attr<4, 2>(dyn_lds) : multi_attr<4, 1>(dyn_lds);
-    case 2: return v2 ? multi_attr<2, 2>(dyn_lds) : multi_attr<2, 1>(dyn_lds);
-    case 1: return v2 ? multi_attr<1, 2>(dyn_lds) : multi_attr<1, 1>(dyn_lds);
-    default: return hipErrorInvalidValue;
-  }
+hipError_t prepare_multi_scan(int vec_per_thread, std::size_t dyn_lds) {
+  const void* f = vec_per_thread == 2 ? (const void*)multi_scan_kernel<2> : (const void*)multi_scan_kernel<1>;
+  return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds);
 }
 
-hipError_t launch_multi_scan(int tile, int vec_per_thread, const MultiColl* colls, const MultiItem* items,
-                             long n_items, std::size_t dyn_lds, int grid, hipStream_t s) {
+hipError_t launch_multi_scan(int vec_per_thread, const MultiColl* colls, const MultiItem* items, long n_items,
+                             std::size_t dyn_lds, int grid, hipStream_t s) {
   if (n_items == 0) return hipSuccess;
   const dim3 g((unsigned)std::max(1, grid)), b(kScanThreads);
-  const bool v2 = vec_per_thread == 2;
-#define VC_MULTI(TT, VV) \
-  hipLaunchKernelGGL((multi_scan_kernel<TT, VV>), g, b, dyn_lds, s, colls, items, n_items)
-  switch (tile) {
-    case 8: if (v2) VC_MULTI(8, 2); else VC_MULTI(8, 1); break;
-    case 4: if (v2) VC_MULTI(4, 2); else VC_MULTI(4, 1); break;
-    case 2: if (v2) VC_MULTI(2, 2); else VC_MULTI(2, 1); break;
-    case 1: if (v2) VC_MULTI(1, 2); else VC_MULTI(1, 1); break;
-    default: return hipErrorInvalidValue;
-  }
-#undef VC_MULTI
+  if (vec_per_thread == 2)
+    hipLaunchKernelGGL(multi_scan_kernel<2>, g, b, dyn_lds, s, colls, items, n_items);
+  else
+    hipLaunchKernelGGL(multi_scan_kernel<1>, g, b, dyn_lds, s, colls, items, n_items);
   return hipGetLastError();
 }
 

@@ -46,13 +46,14 @@ struct Coll {
   size_t dim = 0, G = 0, G16 = 0, n = 0, nq = 0, k = 0, k_eff = 0;
   size_t o_q = 0, o_s = 0, o_i = 0;  // rows / out (stable)
   size_t o_luts = 0, o_ls = 0, o_lb = 0, o_bc = 0, o_bt = 0, o_src = 0, o_ps = 0, o_pi = 0, o_pc = 0;  // work
-  int cap = 0, n_lists = 0;
+  int cap = 0, n_lists = 0, tile = 0;
   size_t len = 0;
 };
 
 struct Plan {
-  int tile = 8, vpt = 2;
+  int tile = 8, vpt = 2;  // largest query tile; vectors per thread (tiles >= 2)
   float item_scale = 1.f;
+  int wg_per_cu = 2;      // 2: tiles shrink until two workgroups share a CU; 1: one per CU
   size_t dyn = 0;
   int grid = 0;
   size_t n_items = 0;
@@ -91,7 +92,7 @@ struct vc_multi {
     graph = nullptr;
   }
   void layout(const Plan& want, hipStream_t s);
-  void enqueue(hipStream_t s);
+  void enqueue(hipStream_t s, bool capturing = false);
 };
 
 // Signature of a set for the measured-plan table: per collection its
@@ -114,33 +115,38 @@ static uint64_t set_signature(const std::vector<Coll>& cs) {
   return h;
 }
 
-// Work buffers, candidate caps, LDS and work items of a plan.
+// Work buffers, per-collection tiles and candidate caps, LDS and work items
+// of a plan.
 void vc_multi::layout(const Plan& want, hipStream_t s) {
-  const int T = want.tile, V = want.vpt;
-  if ((T != 1 && T != 2 && T != 4 && T != 8) || (V != 1 && V != 2) || !(want.item_scale > 0.f))
-    fail(VC_ERR_INVALID_ARGUMENT, "tile must be 1, 2, 4 or 8, vec_per_thread 1 or 2, item_scale > 0");
-  const size_t fixed = vchip::pair_fixed_lds(), half = size_t(vchip::kLdsPerCu) / 2;
-  // two workgroups per CU when every collection keeps >= 32 candidates in half the LDS
-  bool two = true;
-  for (const Coll& x : c) {
-    if (x.nq == 0 || x.k_eff == 0) continue;
-    const long room = long(half) - long(fixed) - long(vchip::pair_topk_lds(T, int(x.k_eff), 0));
-    if (room / (8L * T) / 32 * 32 < 32) two = false;
-  }
+  const int V = want.vpt;
+  if ((want.tile != 1 && want.tile != 2 && want.tile != 4 && want.tile != 8) || (V != 1 && V != 2) ||
+      !(want.item_scale > 0.f) || (want.wg_per_cu != 1 && want.wg_per_cu != 2))
+    fail(VC_ERR_INVALID_ARGUMENT, "tile must be 1, 2, 4 or 8, vec_per_thread 1 or 2, item_scale > 0, "
+                                  "workgroups per CU 1 or 2");
+  const size_t fixed = vchip::pair_fixed_lds();
+  const size_t lim = size_t(vchip::kLdsPerCu) / size_t(want.wg_per_cu) - fixed;  // dynamic LDS per workgroup
+  auto cap_for = [&](int T, size_t k) {
+    const long room = long(lim) - long(vchip::pair_topk_lds(T, int(k), 0));
+    return std::min<long>(vchip::kMaxCandCap, room / (8L * T) / 32 * 32);
+  };
   Plan p = want;
   p.dyn = 0;
   double total = 0.0;
   for (Coll& x : c) {
     if (x.nq == 0 || x.k_eff == 0) continue;
-    const size_t lim = (two ? half : size_t(vchip::kLdsPerCu)) - fixed;
-    const long room = long(lim) - long(vchip::pair_topk_lds(T, int(x.k_eff), 0));
-    const long cap = std::min<long>(vchip::kMaxCandCap, room / (8L * T) / 32 * 32);
-    if (cap < 32) fail(VC_ERR_INVALID_ARGUMENT, "k too large for the LDS of a tile of this size");
+    // the smallest tile that holds the batch, at most the plan's, shrunk
+    // until its top-k state keeps >= 32 candidates in the LDS share
+    int T = 1;
+    while (T < want.tile && size_t(T) < x.nq) T *= 2;
+    while (T > 1 && cap_for(T, x.k_eff) < 32) T /= 2;
+    const long cap = cap_for(T, x.k_eff);
+    if (cap < 32) fail(VC_ERR_INVALID_ARGUMENT, "k too large for the LDS share of this plan");
+    x.tile = T;
     x.cap = int(cap);
     p.dyn = std::max(p.dyn, vchip::pair_topk_lds(T, int(x.k_eff), x.cap));
     total += double((x.nq + T - 1) / T) * double(x.n) * double(x.G16);
   }
-  const int slots = kCus * ((fixed + p.dyn <= half) ? 2 : 1);
+  const int slots = kCus * want.wg_per_cu;
   // about 4 items per workgroup slot, scaled; each collection's ranges hold
   // whole steps of the scan and at most 512 lists per query
   const double target = std::max(1.0, total / (slots * 4.0) * p.item_scale);
@@ -162,7 +168,7 @@ void vc_multi::layout(const Plan& want, hipStream_t s) {
   std::vector<vchip::MultiItem> items;
   for (const auto& [cost, j] : order) {
     const Coll& x = c[j];
-    const long n_qt = long((x.nq + T - 1) / T);
+    const long n_qt = long((x.nq + x.tile - 1) / x.tile);
     for (int l = 0; l < x.n_lists; ++l)
       for (long qt = 0; qt < n_qt; ++qt) {
         vchip::MultiItem it{};
@@ -185,7 +191,7 @@ void vc_multi::layout(const Plan& want, hipStream_t s) {
     x.o_lb = cv.take(x.nq * 4);
     x.o_bc = cv.take(x.nq * 4);
     x.o_bt = cv.take(x.nq * 4);
-    x.o_src = cv.take(vchip::pair_source_bytes(T, int(x.G16), x.nq));
+    x.o_src = cv.take(x.k_eff > 0 ? vchip::pair_source_bytes(x.tile, int(x.G16), x.nq) : 0);
     const size_t lists = x.nq * size_t(std::max(x.n_lists, 0));
     x.o_ps = cv.take(lists * x.k_eff * 4);
     x.o_pi = cv.take(lists * x.k_eff * 4);
@@ -213,6 +219,7 @@ void vc_multi::layout(const Plan& want, hipStream_t s) {
     m.k = int(x.k_eff);
     m.cap = x.cap;
     m.n_lists = x.n_lists;
+    m.tile = x.tile;
   }
   Carve mv;
   const size_t o_c = mv.take(mc.size() * sizeof(vchip::MultiColl));
@@ -226,29 +233,41 @@ void vc_multi::layout(const Plan& want, hipStream_t s) {
                              hipMemcpyHostToDevice, s),
               "copy items");
   hip_check(hipStreamSynchronize(s), "sync");
-  hip_check(vchip::prepare_multi_scan(T, V, p.dyn), "multi scan LDS limit");
+  hip_check(vchip::prepare_multi_scan(V, p.dyn), "multi scan LDS limit");
   plan = p;
   drop_graph();  // captured buffers and items may have moved
 }
 
-void vc_multi::enqueue(hipStream_t s) {
-  const int T = plan.tile;
+void vc_multi::enqueue(hipStream_t s, bool capturing) {
   for (Coll& x : c) {
     if (x.nq == 0 || x.k_eff == 0) continue;
     hip_check(vchip::launch_prepare_queries(x.t, rows.as<float>(x.o_q), x.nq, work.as<uint8_t>(x.o_luts),
                                             work.as<float>(x.o_ls), work.as<float>(x.o_lb), work.as<float>(x.o_bc),
                                             work.as<float>(x.o_bt), s),
               "prepare_queries");
-    hip_check(vchip::launch_pair_source(T, work.as<uint8_t>(x.o_luts), x.nq, int(x.G), int(x.G16),
+    hip_check(vchip::launch_pair_source(x.tile, work.as<uint8_t>(x.o_luts), x.nq, int(x.G), int(x.G16),
                                         work.as<uint8_t>(x.o_src), s),
               "pair_source");
   }
   Carve mv;
   const size_t o_c = mv.take(c.size() * sizeof(vchip::MultiColl));
   const size_t o_it = mv.take(std::max<size_t>(plan.n_items, 1) * sizeof(vchip::MultiItem));
-  hip_check(vchip::launch_multi_scan(T, plan.vpt, meta.as<vchip::MultiColl>(o_c), meta.as<vchip::MultiItem>(o_it),
+  // profiling (vc_set_profiling): the scan launch between HIP events on its
+  // stream, outside graph capture; vc_last_scan_stats reports tile and items
+  std::unique_lock probe_lock(g_probe.mu);
+  const bool probe = g_probe.on && !capturing;
+  if (!probe) probe_lock.unlock();
+  if (probe) hip_check(hipEventRecord(g_probe.start, s), "hipEventRecord");
+  hip_check(vchip::launch_multi_scan(plan.vpt, meta.as<vchip::MultiColl>(o_c), meta.as<vchip::MultiItem>(o_it),
                                      long(plan.n_items), plan.dyn, plan.grid, s),
             "multi scan");
+  if (probe) {
+    hip_check(hipEventRecord(g_probe.stop, s), "hipEventRecord");
+    g_probe.recorded = true;
+    g_probe.tile = plan.tile;
+    g_probe.ranges = int(plan.n_items);
+    probe_lock.unlock();
+  }
   for (Coll& x : c) {
     if (x.nq == 0 || x.k_eff == 0) continue;
     hip_check(vchip::launch_merge_ranges(work.as<float>(x.o_ps), work.as<uint32_t>(x.o_pi), work.as<int>(x.o_pc),
@@ -392,6 +411,16 @@ vc_status vc_multi_rows(vc_multi* m, size_t set, float** queries_dev, float** sc
   });
 }
 
+vc_status vc_multi_collection_plan(vc_multi* m, size_t set, size_t* tile, size_t* lists) {
+  return guarded([&] {
+    if (m == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null set");
+    std::lock_guard lock(m->mu);
+    if (set >= m->c.size()) fail(VC_ERR_OUT_OF_RANGE, "collection index out of range");
+    if (tile) *tile = size_t(m->c[set].tile);
+    if (lists) *lists = size_t(std::max(m->c[set].n_lists, 0));
+  });
+}
+
 vc_status vc_multi_set_queries(vc_multi* m, size_t set, const float* queries_dev, void* stream) {
   return guarded([&] {
     if (m == nullptr || queries_dev == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
@@ -435,7 +464,7 @@ vc_status vc_multi_capture(vc_multi* m, void* stream) {
     m->drop_graph();
     hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
-      m->enqueue(s);
+      m->enqueue(s, /*capturing=*/true);
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(s, &g);
@@ -456,7 +485,19 @@ vc_status vc_multi_replay(vc_multi* m, void* stream) {
   });
 }
 
-vc_status vc_multi_set_plan(vc_multi* m, size_t tile, size_t vec_per_thread, float item_scale) {
+static void plan_info(const Plan& p, vc_multi_plan_info* out) {
+  out->tile = size_t(p.tile);
+  out->vec_per_thread = size_t(p.vpt);
+  out->item_scale = p.item_scale;
+  out->workgroups_per_cu = size_t(p.wg_per_cu);
+  out->items = p.n_items;
+  out->grid = size_t(p.grid);
+  out->lds_bytes = vchip::pair_fixed_lds() + p.dyn;
+  out->measured_ms = p.ms;
+}
+
+vc_status vc_multi_set_plan(vc_multi* m, size_t tile, size_t vec_per_thread, float item_scale,
+                            size_t workgroups_per_cu) {
   return guarded([&] {
     if (m == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null set");
     std::lock_guard lock(m->mu);
@@ -464,6 +505,7 @@ vc_status vc_multi_set_plan(vc_multi* m, size_t tile, size_t vec_per_thread, flo
     want.tile = int(tile);
     want.vpt = int(vec_per_thread);
     want.item_scale = item_scale;
+    want.wg_per_cu = int(workgroups_per_cu);
     StreamGuard sg;
     m->layout(want, sg.s);
   });
@@ -473,13 +515,7 @@ vc_status vc_multi_get_plan(vc_multi* m, vc_multi_plan_info* out) {
   return guarded([&] {
     if (m == nullptr || out == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null argument");
     std::lock_guard lock(m->mu);
-    out->tile = size_t(m->plan.tile);
-    out->vec_per_thread = size_t(m->plan.vpt);
-    out->item_scale = m->plan.item_scale;
-    out->items = m->plan.n_items;
-    out->grid = size_t(m->plan.grid);
-    out->lds_bytes = vchip::pair_fixed_lds() + m->plan.dyn;
-    out->measured_ms = m->plan.ms;
+    plan_info(m->plan, out);
   });
 }
 
@@ -491,13 +527,15 @@ vc_status vc_multi_tune(vc_multi* m, size_t reps, vc_multi_plan_info* best) {
     const int r = int(std::max<size_t>(reps, 1));
     Plan best_p;
     double best_ms = 1e30;
-    for (int T : {8, 4, 2})
-      for (int V : {2, 1})
-        for (float sc : {0.5f, 1.f, 2.f}) {
+    for (int wg : {2, 1})
+      for (int T : {8, 4})
+        for (int V : {2, 1})
+          for (float sc : {0.5f, 1.f, 2.f}) {
           Plan want;
           want.tile = T;
           want.vpt = V;
           want.item_scale = sc;
+          want.wg_per_cu = wg;
           try {
             m->layout(want, sg.s);
           } catch (const VcError&) {
@@ -517,15 +555,7 @@ vc_status vc_multi_tune(vc_multi* m, size_t reps, vc_multi_plan_info* best) {
       std::lock_guard g(g_multi_mu);
       g_multi_tuned[m->sig] = m->plan;
     }
-    if (best != nullptr) {
-      best->tile = size_t(m->plan.tile);
-      best->vec_per_thread = size_t(m->plan.vpt);
-      best->item_scale = m->plan.item_scale;
-      best->items = m->plan.n_items;
-      best->grid = size_t(m->plan.grid);
-      best->lds_bytes = vchip::pair_fixed_lds() + m->plan.dyn;
-      best->measured_ms = best_ms;
-    }
+    if (best != nullptr) plan_info(m->plan, best);
   });
 }
 
@@ -609,8 +639,8 @@ vc_status vc_tune_save(const char* path) {
       std::lock_guard lock(g_multi_mu);
       char line[256];
       for (const auto& [sig, p] : g_multi_tuned) {
-        std::snprintf(line, sizeof line, "multi sig=%016llx tile=%d vpt=%d iscale=%.4g ms=%.6g\n",
-                      (unsigned long long)sig, p.tile, p.vpt, (double)p.item_scale, p.ms);
+        std::snprintf(line, sizeof line, "multi sig=%016llx tile=%d vpt=%d iscale=%.4g wg=%d ms=%.6g\n",
+                      (unsigned long long)sig, p.tile, p.vpt, (double)p.item_scale, p.wg_per_cu, p.ms);
         text += line;
       }
     }
@@ -638,9 +668,10 @@ vc_status vc_tune_load(const char* path, size_t* n_entries) {
       unsigned long long sig = 0;
       Plan p;
       double sc = 1.0;
-      if (std::sscanf(line.c_str(), "multi sig=%llx tile=%d vpt=%d iscale=%lf ms=%lf", &sig, &p.tile, &p.vpt, &sc,
-                      &p.ms) != 5 ||
+      if (std::sscanf(line.c_str(), "multi sig=%llx tile=%d vpt=%d iscale=%lf wg=%d ms=%lf", &sig, &p.tile, &p.vpt,
+                      &sc, &p.wg_per_cu, &p.ms) != 6 ||
           (p.tile != 1 && p.tile != 2 && p.tile != 4 && p.tile != 8) || (p.vpt != 1 && p.vpt != 2) ||
+          (p.wg_per_cu != 1 && p.wg_per_cu != 2) ||
           !(sc > 0.0 && sc <= 64.0))
         fail(VC_ERR_STATE, std::string("malformed tuning entry: ") + line);
       p.item_scale = float(sc);

@@ -23,6 +23,7 @@ class MultiPlan:
     tile: int
     vec_per_thread: int
     item_scale: float
+    workgroups_per_cu: int
     items: int
     grid: int
     lds_bytes: int
@@ -30,8 +31,8 @@ class MultiPlan:
 
 
 def _plan(info: MultiPlanInfo) -> MultiPlan:
-    return MultiPlan(int(info.tile), int(info.vec_per_thread), float(info.item_scale), int(info.items),
-                     int(info.grid), int(info.lds_bytes), float(info.measured_ms))
+    return MultiPlan(int(info.tile), int(info.vec_per_thread), float(info.item_scale), int(info.workgroups_per_cu),
+                     int(info.items), int(info.grid), int(info.lds_bytes), float(info.measured_ms))
 
 
 class MultiSearch:
@@ -97,13 +98,20 @@ class MultiSearch:
     def replay(self, stream=None):
         check(lib.vc_multi_replay(self._h, _launch_stream(stream)))
 
-    def set_plan(self, tile: int, vec_per_thread: int, item_scale: float = 1.0):
-        check(lib.vc_multi_set_plan(self._h, S(tile), S(vec_per_thread), ctypes.c_float(item_scale)))
+    def set_plan(self, tile: int, vec_per_thread: int, item_scale: float = 1.0, workgroups_per_cu: int = 2):
+        check(lib.vc_multi_set_plan(self._h, S(tile), S(vec_per_thread), ctypes.c_float(item_scale),
+                                    S(workgroups_per_cu)))
 
     def plan(self) -> MultiPlan:
         info = MultiPlanInfo()
         check(lib.vc_multi_get_plan(self._h, ctypes.byref(info)))
         return _plan(info)
+
+    def collection_plan(self, j: int) -> tuple[int, int]:
+        """(query tile, vector ranges) of collection j under the plan."""
+        t, l = S(), S()
+        check(lib.vc_multi_collection_plan(self._h, S(j), ctypes.byref(t), ctypes.byref(l)))
+        return int(t.value), int(l.value)
 
     def tune(self, reps: int = 3) -> MultiPlan:
         info = MultiPlanInfo()
