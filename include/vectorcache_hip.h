@@ -445,7 +445,9 @@ vc_status vc_set_profiling(int on);
 /* Test hook (no reference counterpart): flags bit 0 makes every later encode
  * take the exact sequential alpha path instead of the certified fast one;
  * bit 2 makes indexes created later encode with the 8-rows-per-wavefront tile
- * kernel instead of the two-pass streaming row kernels;
+ * kernel instead of the streaming kernels; bit 6 makes them encode with the
+ * two-pass streaming kernels (a norm pass, then a row pass) instead of the
+ * one-pass kernel of the common shapes;
  * bit 1 makes searches use the nibble-table scan kernel (two 16-entry lookups
  * per code byte) instead of the byte-pair table kernel; bit 3 makes the
  * byte-pair kernel take the other of its two vectors-per-thread variants

@@ -140,11 +140,13 @@ def test_sequential_bias_corr_matches_golden(vc, golden_cases):
         vc.set_debug(0)
 
 
-@pytest.mark.parametrize("flags", [1, 4, 5])
+@pytest.mark.parametrize("flags", [1, 4, 5, 64, 65])
 def test_exact_alpha_path_matches_golden(vc, golden_cases, flags):
     """Bit 0: every vector through the reference's sequential alpha chains
     (the path the certified fast alpha falls back to); bit 2: the 8-rows-per-
-    wavefront tile encode instead of the streaming row encode. Same bits."""
+    wavefront tile encode instead of the streaming encode; bit 6: the two-pass
+    streaming encode (norm pass, row pass) instead of the one-pass kernel.
+    Same bits."""
     vc.set_debug(flags)
     try:
         for c, g, x, q, sample in golden_cases:

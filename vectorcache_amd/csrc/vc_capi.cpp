@@ -109,8 +109,8 @@ vchip::CodeTables make_tables(size_t dim, size_t bits) {
   const auto& ct = vchip::cell_table(bits, dim);
   t.cells = device_cells(bits, dim);
   t.use_cells = t.cells != nullptr ? 1 : 0;
-  t.cell_lo = ct.lo;
   t.cell_inv_w = ct.inv_w;
+  t.cell_off = ct.off;
   t.debug_flags = g_debug_flags.load();
   return t;
 }

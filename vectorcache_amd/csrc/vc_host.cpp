@@ -205,9 +205,10 @@ std::unique_ptr<Codebook> lloyd_max(std::size_t bits, std::size_t dim) {
   return cb;
 }
 
-// Device cell index of a value (the same float operations as the kernel).
-int cell_of(float v, float lo, float inv_w) {
-  const float f = (v - lo) * inv_w;
+// Device cell index of a value (the same float operation as the kernels: one
+// fused multiply-add, monotone in v since inv_w > 0).
+int cell_of(float v, float inv_w, float off) {
+  const float f = std::fmaf(v, inv_w, off);
   if (f >= float(kCells - 1)) return kCells - 1;
   if (f >= 0.f) return int(f);
   return 0;
@@ -220,10 +221,11 @@ std::unique_ptr<CellTable> make_cell_table(const Codebook& cb) {
   const float w = (b[nb - 1] - b[0]) / float(kCells - 2);
   t->lo = b[0] - w;
   t->inv_w = 1.f / w;
+  t->off = -t->lo * t->inv_w;
   std::vector<int> cell(nb);
   bool ok = w > 0.f && std::isfinite(t->inv_w);
   for (std::size_t i = 0; ok && i < nb; ++i) {
-    cell[i] = cell_of(b[i], t->lo, t->inv_w);
+    cell[i] = cell_of(b[i], t->inv_w, t->off);
     if (i > 0 && cell[i] <= cell[i - 1]) ok = false;  // two boundaries share a cell
   }
   t->ok = ok;

@@ -48,7 +48,7 @@ struct CellEntry {
 };
 inline constexpr int kCells = 256;
 struct CellTable {
-  float lo = 0.f, inv_w = 0.f;
+  float lo = 0.f, inv_w = 0.f, off = 0.f;  // cell of v: floor(fmaf(v, inv_w, off)) clamped to [0, kCells)
   bool ok = false;  // false: the device counts boundaries one by one
   CellEntry cells[kCells];
 };

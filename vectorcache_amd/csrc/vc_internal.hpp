@@ -24,7 +24,7 @@ struct CodeTables {
   const std::uint32_t* perm_sign;  // device, dim: inv_perm[j] | (signs[inv_perm[j]] < 0) << 31
   const std::uint32_t* perm_gather;  // device, dim: perm[i] | (signs[i] < 0) << 31
   const void* cells;              // device, 256 x {u32 base, f32 bound, f32 c_lo, f32 c_hi}
-  float cell_lo, cell_inv_w;
+  float cell_inv_w, cell_off;      // cell index of v: fmaf(v, cell_inv_w, cell_off), clamped to [0, 255]
   int use_cells;
   int debug_flags;                // bit 0: take the sequential alpha path for every vector
   // Landmark index modes (zero for the plain flat path):

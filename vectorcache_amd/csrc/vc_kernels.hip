@@ -276,7 +276,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 __device__ __forceinline__ unsigned quant_code(const CodeTables& t, const CellDev* __restrict__ cells, float v,
                                                float& cen) {
   if (t.use_cells) {
-    const float f = (v - t.cell_lo) * t.cell_inv_w;
+    const float f = __fmaf_rn(v, t.cell_inv_w, t.cell_off);
     int ci = 0;
     if (f >= 255.f) ci = 255;
     else if (f >= 0.f) ci = (int)f;
@@ -571,6 +571,97 @@ __device__ __forceinline__ void rotate_block(const CodeTables& t, const uint32_t
   for (int r = 0; r < R; ++r) v[r] = v[r] * t.inv_sqrt_block;
 }
 
+// rotate_block over a lane-major gather table: pgl[b * B + r * 64 + lane] =
+// perm_gather[b * B + lane * R + r], so the 64 lanes of one (block, r) read 64
+// consecutive words (no bank conflicts; the by-destination table read with a
+// lane stride of R words costs an R-way conflict per read). The sign is
+// applied by flipping the product's sign bit, which equals the product with
+// -1 exactly.
+template <int R>
+__device__ __forceinline__ void rotate_block_lm(const CodeTables& t, const uint32_t* __restrict__ pgl,
+                                                const float* __restrict__ X, float inv, int b, int lane,
+                                                float (&v)[R]) {
+  const int B = t.block;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t w = pgl[b * B + r * 64 + lane];
+    v[r] = __uint_as_float(__float_as_uint(X[w & 0x7fffffffu] * inv) ^ (w & 0x80000000u));
+  }
+#pragma unroll
+  for (int len = 1; len < R; len <<= 1) {
+#pragma unroll
+    for (int i = 0; i < R; i += 2 * len) {
+#pragma unroll
+      for (int j = i; j < i + len; ++j) {
+        const float a = v[j], c = v[j + len];
+        v[j] = a + c;
+        v[j + len] = a - c;
+      }
+    }
+  }
+  lane_stage<R, 1>(v, lane);
+  lane_stage<R, 2>(v, lane);
+  lane_stage<R, 4>(v, lane);
+  lane_stage<R, 8>(v, lane);
+  lane_stage<R, 16>(v, lane);
+  lane_stage<R, 32>(v, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = v[r] * t.inv_sqrt_block;
+}
+
+// The butterfly stage across lanes 16 apart (X32: 32 apart) for pairs of
+// registers with the gfx950 row swaps: v_permlane16_swap exchanges the odd
+// 16-lane rows of its first operand with the even rows of its second
+// (v_permlane32_swap: the upper 32 lanes of the first with the lower 32 of the
+// second). After one swap each lane holds both members of a butterfly pair of
+// one register (register r in the lower row, r + 1 in the upper); the sum and
+// difference are formed in the lane, and a second swap puts every result back
+// in its own lane and register. Same operations and operands as lane_stage
+// (a + c, a - c), with no LDS traffic.
+template <int R, bool X32>
+__device__ __forceinline__ void swap_stage(float (&v)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; r += 2) {
+    const auto p = X32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(v[r]), __float_as_uint(v[r + 1]), false, false)
+                       : __builtin_amdgcn_permlane16_swap(__float_as_uint(v[r]), __float_as_uint(v[r + 1]), false, false);
+    const float a = __uint_as_float(p[0]), c = __uint_as_float(p[1]);
+    const float sum = a + c, dif = a - c;
+    const auto q = X32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(sum), __float_as_uint(dif), false, false)
+                       : __builtin_amdgcn_permlane16_swap(__float_as_uint(sum), __float_as_uint(dif), false, false);
+    v[r] = __uint_as_float(q[0]);
+    v[r + 1] = __uint_as_float(q[1]);
+  }
+}
+
+// The stage across lanes 8 apart by a DPP row rotation by 8 (lane ^ 8 inside
+// each 16-lane row).
+template <int R>
+__device__ __forceinline__ void dpp8_stage(float (&v)[R], int lane) {
+  const float sg = (lane & 8) ? -1.f : 1.f;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const float pv = __builtin_bit_cast(
+        float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v[r]), 0x128, 0xF, 0xF, false));
+    v[r] = __fmaf_rn(sg, v[r], pv);
+  }
+}
+
+// Lane-major slot of rotated position j (block B = 64 R): the position
+// b * B + lane * R + r is kept at b * B + r * 64 + lane.
+__device__ __forceinline__ int lane_major_slot(int j, int B, int R) {
+  const int b = j / B, w = j - b * B;
+  return b * B + (w % R) * 64 + w / R;
+}
+
+// The NB code bytes of one lane's block (little-endian in cw) with one store.
+template <int NB>
+__device__ __forceinline__ void store_code_bytes(uint8_t* dst, uint64_t cw) {
+  if constexpr (NB == 8) *reinterpret_cast<uint64_t*>(dst) = cw;
+  else if constexpr (NB == 4) *reinterpret_cast<uint32_t*>(dst) = (uint32_t)cw;
+  else if constexpr (NB == 2) *reinterpret_cast<uint16_t*>(dst) = (uint16_t)cw;
+  else *dst = (uint8_t)cw;
+}
+
 // R = block / 64 values per lane per rotation block; NXM >= D / 64 values per
 // lane per row. alpha_io[out_offset + row] holds the row norm on entry and
 // alpha on exit.
@@ -595,15 +686,17 @@ __global__ __launch_bounds__(64 * kRowMaxWaves) void encode_row_kernel(CodeTable
   uint8_t* codes = stage;
   float* ys = reinterpret_cast<float*>(stage);
 
-  for (int j = threadIdx.x; j < D; j += blockDim.x) pg[j] = t.perm_gather[j];
+  // gather and TQ+ tables in lane-major order (lane_major_slot)
+  for (int j = threadIdx.x; j < D; j += blockDim.x) pg[lane_major_slot(j, B, R)] = t.perm_gather[j];
   if (t.use_cells)
     for (int j = threadIdx.x; j < 256 * 4; j += blockDim.x)
       reinterpret_cast<uint32_t*>(cells)[j] = reinterpret_cast<const uint32_t*>(t.cells)[j];
   if (tq)
     for (int j = threadIdx.x; j < D; j += blockDim.x) {
-      tsh[j] = t.shift[j];
-      tsc[j] = t.scale[j];
-      tis[j] = t.inv_scale[j];
+      const int sl = lane_major_slot(j, B, R);
+      tsh[sl] = t.shift[j];
+      tsc[sl] = t.scale[j];
+      tis[sl] = t.inv_scale[j];
     }
   __syncthreads();
 
@@ -636,19 +729,19 @@ __global__ __launch_bounds__(64 * kRowMaxWaves) void encode_row_kernel(CodeTable
 #pragma unroll 1
     for (int b = 0; b < nblk; ++b) {
       float v[R];
-      rotate_block<R>(t, pg, X, inv, b, lane, v);
+      rotate_block_lm<R>(t, pg, X, inv, b, lane, v);
       uint64_t cw = 0;  // this lane's codes of the block, packed as flat bytes
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const float yv = v[r];
-        const int j = b * B + lane * R + r;
+        const int jl = b * B + r * 64 + lane;  // lane-major slot of position b * B + lane * R + r
         float cen;
-        const unsigned k = quant_code(t, cells, tq ? (yv + tsh[j]) * tsc[j] : yv, cen);
+        const unsigned k = quant_code(t, cells, tq ? (yv + tsh[jl]) * tsc[jl] : yv, cen);
         double term;
         if (!tq)
           term = (double)yv * (double)cen;
         else
-          term = (double)yv * fma((double)cen, (double)tis[j], -(double)tsh[j]);
+          term = (double)yv * fma((double)cen, (double)tis[jl], -(double)tsh[jl]);
         sum += term;
         asum += fabs(term);
         const int slot = r % cpb;  // position inside its byte, first code highest
@@ -656,8 +749,8 @@ __global__ __launch_bounds__(64 * kRowMaxWaves) void encode_row_kernel(CodeTable
         cw |= (uint64_t)k << (8 * (r / cpb) + shift);
       }
       uint8_t* dst = codes + (b * B + lane * R) / cpb;
-#pragma unroll
-      for (int i = 0; i < R / cpb; ++i) dst[i] = (uint8_t)(cw >> (8 * i));
+      if (cpb == 2) store_code_bytes<R / 2>(dst, cw);
+      else store_code_bytes<(R >= 4 ? R / 4 : 1)>(dst, cw);
     }
     for (int o = 32; o > 0; o >>= 1) {
       sum += __shfl_xor(sum, o);
@@ -689,20 +782,20 @@ __global__ __launch_bounds__(64 * kRowMaxWaves) void encode_row_kernel(CodeTable
 #pragma unroll 1
       for (int b = 0; b < nblk; ++b) {
         float yb[R];
-        rotate_block<R>(t, pg, X, inv, b, lane, yb);
+        rotate_block_lm<R>(t, pg, X, inv, b, lane, yb);
 #pragma unroll
         for (int r = 0; r < R; ++r) ys[lane * R + r] = yb[r];
         wave_lds_sync();
         if (lane < 4) {
           for (int jj = lane; jj < B; jj += 4) {
-            const int j = b * B + jj;
+            const int jl = b * B + (jj % R) * 64 + jj / R;  // lane-major slot of position b * B + jj
             const float yv = ys[jj];
             float cen;
-            (void)quant_code(t, cells, tq ? (yv + tsh[j]) * tsc[j] : yv, cen);
+            (void)quant_code(t, cells, tq ? (yv + tsh[jl]) * tsc[jl] : yv, cen);
             if (!tq)
               d = fma((double)yv, (double)cen, d);
             else
-              d = fma((double)yv, fma((double)cen, (double)tis[j], -(double)tsh[j]), d);
+              d = fma((double)yv, fma((double)cen, (double)tis[jl], -(double)tsh[jl]), d);
           }
         }
         wave_lds_sync();
@@ -713,6 +806,267 @@ __global__ __launch_bounds__(64 * kRowMaxWaves) void encode_row_kernel(CodeTable
     }
     if (lane == 0) alpha_io[v] = a;
     wave_lds_sync();  // X and the staging area reused by the next row
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One-pass encode for the common shapes: D = 64 NX with NX in
+// {2, 4, 6, 8, 12, 24} (rotation block B = 64 R, R = NX & -NX), cell
+// quantization. A wavefront takes eight consecutive rows of a 64-row output
+// tile (a workgroup owns whole tiles, so a tile's code lines are written
+// through one L2):
+//   1. their norms: the reference's 8 fused chains c[j % 8] and the tree
+//      ((c0+c1)+(c2+c3))+((c4+c5)+(c6+c7)) (src/encode/encode.cpp:16-26), lane
+//      = (row, chain), loads straight from global memory;
+//   2. each row in turn: a second read of the row (from L2 / MALL: step 1 has
+//      just read it; the next row's loads are in flight while a row is
+//      encoded), staged in the wavefront's LDS slot, gathered through the
+//      lane-major table (address (w << 2) + slot: the sign bit shifts out;
+//      the sign rides on the multiplier, x * (-inv) = (x * inv) * -1 exactly),
+//      butterflies, cell quantization, alpha and 16-byte code pieces.
+// The separate norm pass of the two-pass kernels (a whole extra read of the
+// rows from HBM) is gone.
+// ---------------------------------------------------------------------------
+constexpr int kFusedWaves = 16;
+
+template <int NX, int CPB>
+struct FusedShape {
+  static constexpr int D = 64 * NX;
+  static constexpr int R = NX & -NX;
+  static constexpr int B = 64 * R;
+  static constexpr int NBLK = NX / R;
+  static constexpr int G = D / CPB;
+  static constexpr int G16 = (G + 15) / 16;
+  static constexpr int STAGE = (G16 * 16 > B * 4) ? G16 * 16 : B * 4;  // codes, or one rotated block
+  static constexpr int SLOT = ((D * 4 + 15) & ~15) + ((STAGE + 15) & ~15);
+  static constexpr int NT = D / 8;  // steps of each norm chain
+  static constexpr int NA = NT <= 48 ? NT : (NT % 48 == 0 ? 48 : 32);
+  static constexpr int VW = NX % 4 == 0 ? 4 : 2;  // floats per lane per row load
+  static constexpr int NV = NX / VW;
+  static constexpr size_t shared_bytes(bool tq) {
+    return (size_t)D * 4 + 256 * 16 + (tq ? 3 * (size_t)D * 4 : 0);
+  }
+};
+
+// Row pieces as native vectors (HIP's float4 is a struct, and an array of
+// them crossing the row loop lands in scratch memory).
+typedef float rowv4 __attribute__((ext_vector_type(4)));
+typedef float rowv2 __attribute__((ext_vector_type(2)));
+template <int VW>
+struct RowVec;
+template <>
+struct RowVec<4> {
+  using T = rowv4;
+};
+template <>
+struct RowVec<2> {
+  using T = rowv2;
+};
+
+template <int NX, int CPB, bool TQ>
+__global__ __launch_bounds__(64 * kFusedWaves) void encode_fused_kernel(CodeTables t, const float* __restrict__ x,
+                                                                       long n, long out_offset,
+                                                                       uint8_t* __restrict__ tiled,
+                                                                       float* __restrict__ alpha) {
+  using S = FusedShape<NX, CPB>;
+  constexpr int D = S::D, R = S::R, B = S::B;
+  using V = typename RowVec<S::VW>::T;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: row loops stay scalar
+  uint32_t* pgl = reinterpret_cast<uint32_t*>(smem);
+  CellDev* cells = reinterpret_cast<CellDev*>(smem + D * 4);
+  float* tsh = reinterpret_cast<float*>(smem + D * 4 + 256 * 16);
+  float* tsc = tsh + D;
+  float* tis = tsc + D;
+  unsigned char* slot = smem + S::shared_bytes(TQ) + (size_t)wave * S::SLOT;
+  float* X = reinterpret_cast<float*>(slot);
+  unsigned char* stage = slot + ((D * 4 + 15) & ~15);
+  uint8_t* codes = stage;
+  float* ys = reinterpret_cast<float*>(stage);
+
+  for (int j = threadIdx.x; j < D; j += blockDim.x) pgl[lane_major_slot(j, B, R)] = t.perm_gather[j];
+  for (int j = threadIdx.x; j < 256 * 4; j += blockDim.x)
+    reinterpret_cast<uint32_t*>(cells)[j] = reinterpret_cast<const uint32_t*>(t.cells)[j];
+  if constexpr (TQ)
+    for (int j = threadIdx.x; j < D; j += blockDim.x) {
+      const int sl = lane_major_slot(j, B, R);
+      tsh[sl] = t.shift[j];
+      tsc[sl] = t.scale[j];
+      tis[sl] = t.inv_scale[j];
+    }
+  __syncthreads();
+
+  const float cell_off = t.cell_off, cell_inv_w = t.cell_inv_w, isb = t.inv_sqrt_block;
+  const bool certify = (t.debug_flags & 1) == 0;
+  // octets (eight rows) of whole 64-row tiles; with a multiple of 8 wavefronts
+  // per workgroup, a workgroup's octets of one round form whole tiles
+  const long tile0 = out_offset >> 6;
+  const long oct_total = (((out_offset + n + 63) >> 6) - tile0) * 8;
+  for (long oi = (long)blockIdx.x * waves + wave; oi < oct_total; oi += (long)gridDim.x * waves) {
+    const long v0 = (tile0 << 6) + oi * 8;  // output vector of the octet's first row
+    const long r0 = v0 - out_offset;        // its input row (may be < 0 or past n at the edges)
+
+    // 1. norms of the eight rows
+    float nrm, invn;
+    {
+      const int q = lane >> 3, c = lane & 7;
+      const long row = r0 + q;
+      const long rc = (row >= 0 && row < n) ? row : 0;  // rows outside [0, n): any valid row, result unused
+      const float* xr = x + rc * (long)D + c;
+      float acc = 0.f;
+#pragma unroll 1
+      for (int t0 = 0; t0 < S::NT; t0 += S::NA) {
+        float vv[S::NA];
+#pragma unroll
+        for (int u = 0; u < S::NA; ++u) vv[u] = xr[8 * (t0 + u)];
+#pragma unroll
+        for (int u = 0; u < S::NA; ++u) acc = __fmaf_rn(vv[u], vv[u], acc);
+      }
+      acc = acc + __shfl_xor(acc, 1);
+      acc = acc + __shfl_xor(acc, 2);
+      acc = acc + __shfl_xor(acc, 4);
+      nrm = sqrt_rn(acc);
+      invn = nrm > kMinNorm ? div_rn(1.f, nrm) : 0.f;
+    }
+
+    // 2. the rows, one at a time; the next row's loads fly while a row is encoded
+    int q = 0;
+    while (q < 8 && (r0 + q < 0 || r0 + q >= n)) ++q;  // first row of the octet inside [0, n)
+    if (q == 8) continue;
+    V cur[S::NV];
+    {
+      const V* xr = reinterpret_cast<const V*>(x + (r0 + q) * (long)D) + lane;
+#pragma unroll
+      for (int i = 0; i < S::NV; ++i) cur[i] = xr[64 * i];
+    }
+    for (; q < 8; ++q) {
+      const long row = r0 + q;
+      if (row >= n) break;  // rows past the end: none after this one either
+      const float nv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nrm), 8 * q));
+      const float inv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(invn), 8 * q));
+      wave_lds_sync();  // the previous row is done with X and the stage
+#pragma unroll
+      for (int i = 0; i < S::NV; ++i) reinterpret_cast<V*>(X)[lane + 64 * i] = cur[i];
+      if (q + 1 < 8 && row + 1 < n) {
+        const V* xr = reinterpret_cast<const V*>(x + (row + 1) * (long)D) + lane;
+#pragma unroll
+        for (int i = 0; i < S::NV; ++i) cur[i] = xr[64 * i];
+      }
+      wave_lds_sync();  // the row is in X
+
+      const uint32_t inv_bits = __float_as_uint(inv);
+      double sum = 0.0, asum = 0.0;
+#pragma unroll
+      for (int b = 0; b < S::NBLK; ++b) {
+        float v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const uint32_t w = pgl[b * B + r * 64 + lane];
+          const float xv = *reinterpret_cast<const float*>(reinterpret_cast<const unsigned char*>(X) + (w << 2));
+          v[r] = xv * __uint_as_float((w & 0x80000000u) | (inv_bits & 0x7fffffffu));
+        }
+#pragma unroll
+        for (int len = 1; len < R; len <<= 1) {
+#pragma unroll
+          for (int i = 0; i < R; i += 2 * len) {
+#pragma unroll
+            for (int j = i; j < i + len; ++j) {
+              const float a = v[j], c = v[j + len];
+              v[j] = a + c;
+              v[j + len] = a - c;
+            }
+          }
+        }
+        lane_stage<R, 1>(v, lane);
+        lane_stage<R, 2>(v, lane);
+        lane_stage<R, 4>(v, lane);
+        dpp8_stage<R>(v, lane);
+        swap_stage<R, false>(v);
+        swap_stage<R, true>(v);
+        uint64_t cw = 0;  // this lane's codes of the block, packed as flat bytes
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const float yv = v[r] * isb;
+          const int jl = b * B + r * 64 + lane;
+          float z = yv;
+          if constexpr (TQ) z = (yv + tsh[jl]) * tsc[jl];
+          const float f = __fmaf_rn(z, cell_inv_w, cell_off);
+          const int ci = (int)__builtin_fminf(__builtin_fmaxf(f, 0.f), 255.f);
+          const CellDev e = cells[ci];
+          const bool up = z > e.bound;
+          const float cen = up ? e.c_hi : e.c_lo;
+          const unsigned k = e.base + (up ? 1u : 0u);
+          // the term's rounding folds into the running sum's (fused multiply-add):
+          // the error bound below covers one rounding per term either way
+          const double rc = TQ ? fma((double)cen, (double)tis[jl], -(double)tsh[jl]) : (double)cen;
+          sum = fma((double)yv, rc, sum);
+          asum = fma(fabs((double)yv), fabs(rc), asum);
+          constexpr int bits_per = 8 / CPB;
+          const int shift = (CPB - 1 - r % CPB) * bits_per;  // first code highest in its byte
+          cw |= (uint64_t)k << (8 * (r / CPB) + shift);
+        }
+        uint8_t* dst = codes + (b * B + lane * R) / CPB;
+        store_code_bytes<(R / CPB > 0 ? R / CPB : 1)>(dst, cw);
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        sum += __shfl_xor(sum, o);
+        asum += __shfl_xor(asum, o);
+      }
+      wave_lds_sync();  // the row's codes are complete
+      const long vo = out_offset + row;
+      for (int c16 = lane; c16 < S::G16; c16 += 64)
+        *reinterpret_cast<uint4*>(tiled + ((vo >> 6) * S::G16 + c16) * 1024 + (vo & 63) * 16) =
+            *reinterpret_cast<const uint4*>(codes + c16 * 16);
+
+      // certified alpha (the bound of encode_tile_kernel), else the reference's
+      // sequential chains over the rotated values, one block at a time
+      const double u = 1.1102230246251565e-16;  // 2^-53
+      double bound = (double)(D / 4 + D / 64 + 16) * u * asum * 1.01;
+      bound += 2.0 * u * (fabs(sum) + bound);
+      const double lo = sum - bound, hi = sum + bound;
+      float a = 0.f;
+      bool exact = certify;
+      if (exact && lo > 1e-12 && __double2float_rn(lo) == __double2float_rn(hi))
+        a = div_rn(nv, __double2float_rn(hi));
+      else if (exact && hi <= 1e-12)
+        a = 0.f;
+      else
+        exact = false;
+      if (!exact) {
+        wave_lds_sync();  // codes written out: the stage takes rotated blocks
+        double d = 0.0;   // lane m < 4: chain m
+#pragma unroll 1
+        for (int b = 0; b < S::NBLK; ++b) {
+          float yb[R];
+          rotate_block_lm<R>(t, pgl, X, inv, b, lane, yb);
+#pragma unroll
+          for (int r = 0; r < R; ++r) ys[lane * R + r] = yb[r];
+          wave_lds_sync();
+          if (lane < 4) {
+            for (int jj = lane; jj < B; jj += 4) {
+              const int jl = b * B + (jj % R) * 64 + jj / R;
+              const float yv = ys[jj];
+              float z = yv;
+              if constexpr (TQ) z = (yv + tsh[jl]) * tsc[jl];
+              const float f = __fmaf_rn(z, cell_inv_w, cell_off);
+              const int ci = (int)__builtin_fminf(__builtin_fmaxf(f, 0.f), 255.f);
+              const CellDev e = cells[ci];
+              const float cen = z > e.bound ? e.c_hi : e.c_lo;
+              if constexpr (TQ)
+                d = fma((double)yv, fma((double)cen, (double)tis[jl], -(double)tsh[jl]), d);
+              else
+                d = fma((double)yv, (double)cen, d);
+            }
+          }
+          wave_lds_sync();
+        }
+        const double pr = d + __shfl_xor(d, 1);
+        const double inner = pr + __shfl_xor(pr, 2);
+        a = inner > 1e-12 ? div_rn(nv, (float)inner) : 0.f;
+      }
+      if (lane == 0) alpha[vo] = a;
+    }
   }
 }
 
@@ -3062,6 +3416,66 @@ int grid_for(long work, int threads, int cap = 65536) {
   return (int)g;
 }
 
+// Resident workgroups of a persistent launch: workgroups per CU (from the
+// occupancy calculator at this block size and LDS) times the CU count.
+int resident_grid(const void* fn, int threads, size_t lds) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  return cus * per_cu;
+}
+
+// The one-pass encode for its shapes; false when the shape is not one of them.
+template <int NX, int CPB, bool TQ>
+bool launch_fused_shape(const CodeTables& t, const float* x, long n, long out_offset, uint8_t* tiled, float* alpha,
+                        hipStream_t s) {
+  using S = FusedShape<NX, CPB>;
+  const size_t shared = S::shared_bytes(TQ);
+  if (shared + 8 * (size_t)S::SLOT > (size_t)kLdsBudget) return false;
+  const int waves = ((size_t)kLdsBudget - shared) / S::SLOT >= 16 ? 16 : 8;
+  const size_t lds = shared + (size_t)waves * S::SLOT;
+  const void* fn = (const void*)encode_fused_kernel<NX, CPB, TQ>;
+  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const long octets = (((out_offset + n + 63) >> 6) - (out_offset >> 6)) * 8;
+  const long need = (octets + waves - 1) / waves;
+  static const int resident = resident_grid(fn, 64 * waves, lds);  // one GPU per process
+  const int grid = (int)std::min<long>(need, resident);
+  hipLaunchKernelGGL((encode_fused_kernel<NX, CPB, TQ>), dim3(grid), dim3(64 * waves), lds, s, t, x, n, out_offset,
+                     tiled, alpha);
+  return true;
+}
+
+template <int CPB, bool TQ>
+bool launch_fused_nx(const CodeTables& t, const float* x, long n, long out_offset, uint8_t* tiled, float* alpha,
+                     hipStream_t s) {
+  switch (t.dim / 64) {
+    case 2: if constexpr (CPB == 2) return launch_fused_shape<2, CPB, TQ>(t, x, n, out_offset, tiled, alpha, s);
+            return false;
+    case 4: return launch_fused_shape<4, CPB, TQ>(t, x, n, out_offset, tiled, alpha, s);
+    case 6: if constexpr (CPB == 2) return launch_fused_shape<6, CPB, TQ>(t, x, n, out_offset, tiled, alpha, s);
+            return false;
+    case 8: return launch_fused_shape<8, CPB, TQ>(t, x, n, out_offset, tiled, alpha, s);
+    case 12: return launch_fused_shape<12, CPB, TQ>(t, x, n, out_offset, tiled, alpha, s);
+    case 24: return launch_fused_shape<24, CPB, TQ>(t, x, n, out_offset, tiled, alpha, s);
+    default: return false;
+  }
+}
+
+bool launch_fused(const CodeTables& t, const float* x, long n, long out_offset, uint8_t* tiled, float* alpha,
+                  hipStream_t s) {
+  if (t.dim % 64 != 0 || t.block != 64 * ((t.dim / 64) & -(t.dim / 64))) return false;
+  if ((reinterpret_cast<uintptr_t>(x) & 15) != 0) return false;  // rows are read as 16-byte pieces
+  const bool tq = t.shift != nullptr;
+  if (t.bits == 2)
+    return tq ? launch_fused_nx<4, true>(t, x, n, out_offset, tiled, alpha, s)
+              : launch_fused_nx<4, false>(t, x, n, out_offset, tiled, alpha, s);
+  return tq ? launch_fused_nx<2, true>(t, x, n, out_offset, tiled, alpha, s)
+            : launch_fused_nx<2, false>(t, x, n, out_offset, tiled, alpha, s);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -3074,6 +3488,10 @@ hipError_t launch_encode(const CodeTables& t, const float* x, std::size_t n, std
   const int R = t.block / 64;
   const int NX = t.dim / 64;
   const bool plain = t.prerotated == 0 && t.norm_seq == 0;
+  if (plain && tiled != nullptr && alpha != nullptr && packed == nullptr && rotated == nullptr &&
+      t.perm_gather != nullptr && t.use_cells && (t.debug_flags & (4 | 64)) == 0 &&
+      launch_fused(t, x, (long)n, (long)out_offset, tiled, alpha, s))
+    return hipGetLastError();
   if (plain && tiled != nullptr && alpha != nullptr && packed == nullptr && rotated == nullptr && t.dim % 64 == 0 &&
       t.perm_gather != nullptr && (R == 2 || R == 4 || R == 8 || R == 16) && (t.bits != 2 || R >= 4) &&
       NX <= 64 && (t.debug_flags & 4) == 0 &&
