@@ -403,6 +403,50 @@ vc_status vc_score_perm0_into(const float* queries, size_t nq, size_t dim, size_
                               size_t n_indices, float* heap_s, uint64_t* heap_i, size_t* heap_sz,
                               float* heap_min, size_t* heap_mi, const float* score_offsets);
 
+/* ---- prepared queries (PreparedQueries) ------------------------------------
+ * prepare_queries (src/search/search.cpp:762-860) run once on the GPU and kept
+ * in device memory (LUT rows, LUT scales and biases, bias_corr), so that the
+ * score calls of a multi-bucket search reuse them as the reference reuses its
+ * PreparedQueries (include/vectorcache/search/search.hpp:95-112, 143-177).
+ * vc_prepare_queries takes host query rows (nq x dim) and TQ+ shift / scale
+ * (both NULL: none); vc_prepare_queries_device takes device rows and device
+ * shift / scale and is ordered on `stream` (NULL: the null stream); later
+ * calls on the handle wait for it. nq may be 0. */
+typedef struct vc_prepared vc_prepared;
+vc_status vc_prepare_queries(const float* queries, size_t nq, size_t dim, size_t bits, const float* shift,
+                             const float* scale, vc_prepared** out);
+vc_status vc_prepare_queries_device(const float* queries_dev, size_t nq, size_t dim, size_t bits,
+                                    const float* shift_dev, const float* scale_dev, void* stream, vc_prepared** out);
+vc_status vc_prepared_destroy(vc_prepared* prepared);
+vc_status vc_prepared_shape(const vc_prepared* prepared, size_t* nq, size_t* dim, size_t* bits,
+                            size_t* n_byte_groups);
+/* The state as host arrays (any may be NULL): luts nq x G*32 u8, lut_scale,
+ * lut_bias (without bias_corr), bias_corr; G = n_byte_groups. */
+vc_status vc_prepared_export(const vc_prepared* prepared, uint8_t* luts, float* lut_scale, float* lut_bias,
+                             float* bias_corr);
+/* prepared_query_at (src/search/search.cpp:1045-1068): a new handle holding
+ * query qi alone; qi >= nq -> VC_ERR_OUT_OF_RANGE. */
+vc_status vc_prepared_query_at(const vc_prepared* prepared, size_t qi, vc_prepared** out);
+/* score_prepared / score_prepared_into over one range of PERM0 blocks + alpha
+ * with the handle's queries: the arguments, results and errors of
+ * vc_score_perm0 / vc_score_perm0_into, without rebuilding the query LUTs. */
+vc_status vc_score_prepared(const vc_prepared* prepared, size_t k, const uint8_t* perm0, const float* alpha, size_t n,
+                            const uint64_t* id_map, const size_t* query_indices, size_t n_indices,
+                            const float* residual_norms, const float* residual_scales, float* scores, uint64_t* ids,
+                            size_t* k_out);
+vc_status vc_score_prepared_into(const vc_prepared* prepared, size_t k, const uint8_t* perm0, const float* alpha,
+                                 size_t n, const uint64_t* id_map, const size_t* query_indices, size_t n_indices,
+                                 float* heap_s, uint64_t* heap_i, size_t* heap_sz, float* heap_min, size_t* heap_mi,
+                                 const float* score_offsets);
+
+/* Repack free functions over host buffers, on the GPU (src/pack/pack.cpp:
+ * 98-131, 157-186, 352-401): bit-plane codes (n x bits*dim/8) -> flat rows
+ * (n x G, codes_flat_from_packed), PERM0 blocks (ceil(n/192) x G x 192,
+ * repack_perm0) or sequential blocks (same size, repack_seq). */
+vc_status vc_codes_flat_from_packed(const uint8_t* packed, size_t n, size_t dim, size_t bits, uint8_t* flat_out);
+vc_status vc_repack_perm0(const uint8_t* packed, size_t n, size_t dim, size_t bits, uint8_t* perm0_out);
+vc_status vc_repack_seq(const uint8_t* packed, size_t n, size_t dim, size_t bits, uint8_t* seq_out);
+
 /* heaps_to_search_results (src/search/search.cpp:1269-1291): each of the nq
  * heaps' heap_sz entries by (score desc, id asc) into scores / ids [nq][k],
  * the rest of each row (0, 0). */

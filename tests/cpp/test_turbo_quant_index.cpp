@@ -7,6 +7,7 @@
 // tests for vectorcache::LandmarkIndex, which is what they exercise in the
 // reference (its TurboQuantIndex aliases LandmarkIndex).
 // Prints "OK <checks>" and exits 0 on success.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -18,8 +19,14 @@
 #include <string>
 #include <vector>
 
+#include "vectorcache/cluster/kmeans_buckets.hpp"
 #include "vectorcache/datasets/npy.hpp"
+#include "vectorcache/encode/encode.hpp"
 #include "vectorcache/index.hpp"
+#include "vectorcache/pack/pack.hpp"
+#include "vectorcache/quantize/codebook.hpp"
+#include "vectorcache/search/search.hpp"
+#include "vectorcache/transform/rotation.hpp"
 #include "vectorcache/turbo_quant_index.hpp"
 
 namespace {
@@ -47,6 +54,12 @@ void expect_throw(F&& f, const std::string& what) {
   std::exit(1);
 }
 
+bool same_rows(const vectorcache::SearchResults& a, const vectorcache::SearchResults& b) {
+  return a.nq == b.nq && a.k == b.k && a.ids.size() == b.ids.size() && a.scores.size() == b.scores.size() &&
+         std::equal(a.ids.begin(), a.ids.end(), b.ids.begin()) &&
+         std::equal(a.scores.begin(), a.scores.end(), b.scores.begin());
+}
+
 std::vector<float> gaussian(std::size_t n, std::size_t dim, unsigned seed) {
   std::mt19937 rng(seed);
   std::normal_distribution<float> nd(0.f, 1.f);
@@ -54,6 +67,116 @@ std::vector<float> gaussian(std::size_t n, std::size_t dim, unsigned seed) {
   for (auto& x : v) x = nd(rng);
   return v;
 }
+// The reference's free-function path (encode -> repack -> prepare_queries ->
+// score_prepared / search_flat, score_prepared_into chains over buckets) against
+// the index built from the same rows, and the README's bucketed index.
+void free_function_checks() {
+  using namespace vectorcache;
+  const std::size_t dim = 128, n = 2000, nq = 6, k = 10, bits = 4;
+  auto data = gaussian(n, dim, 21);
+  std::vector<float> queries(data.begin(), data.begin() + nq * dim);
+  TurboQuantIndex index(dim, bits);
+  index.add(data);
+  index.prepare();
+  const SearchResults want = index.search(queries, k);
+
+  const Rotation rot(dim);
+  expect(rot.dim() == dim && rot.block() == 128 && rot.perm(0).size() == dim, "rotation tables");
+  const auto cb = codebook(bits, dim);
+  expect(cb.first.size() == 15 && cb.second.size() == 16, "codebook sizes");
+  for (std::size_t i = 0; i + 1 < cb.second.size(); ++i)
+    expect(cb.first[i] == (cb.second[i] + cb.second[i + 1]) * 0.5f, "boundaries are f32 midpoints");
+
+  std::vector<float> scratch, alpha;
+  std::vector<std::uint8_t> packed;
+  encode(data, n, dim, rot, cb.first, cb.second, bits, nullptr, scratch, packed, alpha);
+  expect(packed.size() == n * bits * dim / 8 && alpha.size() == n, "encode output sizes");
+  std::vector<float> alpha_ix(n);
+  expect(vc_index_export_alpha(index.handle(), alpha_ix.data()) == VC_OK, "export alpha");
+  expect(std::equal(alpha.begin(), alpha.end(), alpha_ix.begin()), "encode alpha == index alpha");
+  encode(std::span<const float>(data.data(), dim), 1, dim, rot, cb.first, cb.second, bits, nullptr, scratch, packed,
+         alpha);
+  expect(packed.size() == (n + 1) * bits * dim / 8 && alpha.size() == n + 1 && alpha[n] == alpha[0],
+         "encode appends");
+  packed.resize(n * bits * dim / 8);
+  alpha.resize(n);
+
+  const auto geo = blocked_geometry(n, bits, dim);
+  expect(geo.n_blocks == 11 && geo.n_byte_groups == 64 && geo.blocked_len == 11 * 64 * 192, "blocked geometry");
+  const auto flat = codes_flat_from_packed(packed, n, bits, dim);
+  auto [perm0, nb] = repack_perm0(packed, n, bits, dim);
+  expect(nb == geo.n_blocks && perm0.size() == geo.blocked_len, "repack_perm0 shape");
+  std::vector<std::uint8_t> perm0_ix(perm0.size());
+  expect(vc_index_export_perm0(index.handle(), perm0_ix.data()) == VC_OK, "export perm0");
+  expect(perm0 == perm0_ix, "repack_perm0 == index PERM0 export");
+  bool codes_ok = true;
+  for (std::size_t v = 0; v < n; v += 97)
+    for (std::size_t g = 0; g < geo.n_byte_groups; g += 7)
+      codes_ok = codes_ok && read_code(perm0, bits, geo.n_byte_groups, v / kBlock, g, v % kBlock) ==
+                                 flat[v * geo.n_byte_groups + g];
+  expect(codes_ok, "read_code(PERM0) == codes_flat_from_packed");
+  expect(repack_seq(packed, n, bits, dim).size() == geo.blocked_len, "repack_seq shape");
+
+  SearchResults sf = search_flat(queries, nq, dim, k, rot, cb.second, bits, perm0, nb, alpha, {}, {});
+  expect(same_rows(sf, want), "search_flat == TurboQuantIndex::search");
+
+  PreparedQueries prep = prepare_queries(queries, nq, dim, rot, cb.second, bits, {}, {});
+  expect(prep.nq == nq && prep.n_byte_groups == 64, "prepared shape");
+  // two buckets (halves of the rows), each scored into persistent heaps with
+  // its global ids: the union equals the flat search
+  const std::size_t h = 1000;
+  auto [p0, nb0] = repack_perm0(std::span<const std::uint8_t>(packed.data(), h * bits * dim / 8), h, bits, dim);
+  auto [p1, nb1] = repack_perm0(std::span<const std::uint8_t>(packed.data() + h * bits * dim / 8,
+                                                              (n - h) * bits * dim / 8), n - h, bits, dim);
+  std::vector<std::uint64_t> ids0(h), ids1(n - h);
+  for (std::size_t i = 0; i < h; ++i) ids0[i] = i;
+  for (std::size_t i = 0; i < n - h; ++i) ids1[i] = h + i;
+  std::vector<float> hs(nq * k), hmin(nq, -std::numeric_limits<float>::infinity());
+  std::vector<std::uint64_t> hi(nq * k);
+  std::vector<std::size_t> hsz(nq, 0), hmi(nq, 0), all(nq);
+  for (std::size_t q = 0; q < nq; ++q) all[q] = q;
+  score_prepared_into(prep, k, p0, nb0, std::span<const float>(alpha.data(), h), ids0, all, hs.data(), hi.data(),
+                      hsz.data(), hmin.data(), hmi.data());
+  score_prepared_into(prep, k, p1, nb1, std::span<const float>(alpha.data() + h, n - h), ids1, all, hs.data(),
+                      hi.data(), hsz.data(), hmin.data(), hmi.data());
+  SearchResults hr;
+  heaps_to_search_results(hr, nq, k, hs.data(), hi.data(), hsz.data());
+  expect(same_rows(hr, want), "two-bucket score_prepared_into chain == flat search");
+  SearchResults one = score_prepared(prepared_query_at(prep, 2), k, perm0, nb, alpha);
+  expect(one.nq == 1 && std::equal(one.ids.begin(), one.ids.end(), want.ids.begin() + 2 * k),
+         "prepared_query_at(2) == row 2");
+  std::vector<std::size_t> sub{5, 0, 5};
+  SearchResults rs = score_prepared(prep, k, perm0, nb, alpha, {}, sub);
+  expect(rs.nq == 3 && std::equal(rs.ids.begin(), rs.ids.begin() + k, want.ids.begin() + 5 * k),
+         "score_prepared query subset");
+  expect_throw<std::out_of_range>([&] { (void)prepared_query_at(prep, nq); }, "prepared_query_at out of range");
+  std::vector<float> wrong(cb.second);
+  wrong[0] += 1.f;
+  expect_throw<std::invalid_argument>([&] { (void)prepare_queries(queries, nq, dim, rot, wrong, bits, {}, {}); },
+                                      "prepare_queries: foreign centroids");
+  Calibration cal = fit_calibration(data, 500, dim, rot, cb.second, scratch);
+  expect(cal.shift.size() == dim && cal.scale_tq.size() == dim, "fit_calibration sizes");
+
+  // the README's bucketed index: with every bucket open the residual scores
+  // rank the rows themselves first (a stored vector queried by itself)
+  BucketParams bp;
+  bp.max_bucket_size = 200;
+  bp.energy_soft = 100;
+  bp.n_probe = 512;
+  BucketedTurboQuantIndex bix(dim, bits, bp);
+  bix.add(data);
+  expect_throw<std::runtime_error>([&] { (void)bix.search(queries, k); }, "bucketed search before prepare");
+  bix.prepare();
+  expect(bix.size() == n && bix.n_buckets() >= n / 200, "bucketed build");
+  SearchResults br = bix.search(queries, k);
+  expect(br.nq == nq && br.k == k, "bucketed result shape");
+  for (std::size_t q = 0; q < nq; ++q) {
+    expect(br.ids[q * k] == q, "bucketed self-query");
+    for (std::size_t j = 1; j < k; ++j) expect(br.scores[q * k + j - 1] >= br.scores[q * k + j], "bucketed rows sorted");
+  }
+  expect_throw<std::invalid_argument>([&] { BucketedTurboQuantIndex bad(12, bits, bp); }, "bucketed bad dim");
+}
+
 // The reference's LandmarkIndex tests (tests/test_index.cpp:11-113).
 void landmark_checks() {
   using vectorcache::LandmarkIndex;
@@ -191,7 +314,7 @@ int main() {
       SearchResults hr;
       vectorcache::heaps_to_search_results(hr, nq, kk, hs.data(), hi.data(), hsz.data());
       SearchResults sr = index.search(queries, kk);
-      expect(hr.ids == sr.ids && hr.scores == sr.scores, "score_prepared_into into empty heaps == search");
+      expect(same_rows(hr, sr), "score_prepared_into into empty heaps == search");
       std::vector<std::size_t> none;
       expect_throw<std::invalid_argument>([&] {
         vectorcache::score_perm0_into(queries, dim, 4, kk, perm0, alpha, {}, none, hs.data(), hi.data(), hsz.data(),
@@ -237,6 +360,7 @@ int main() {
                                      "npy missing file");
     std::filesystem::remove_all(p.parent_path());
   }
+  free_function_checks();
   landmark_checks();
   std::printf("OK %d\n", g_checks);
   return 0;

@@ -411,6 +411,96 @@ def test_score_into_matches_oracle(vc, oracle, bits, dim, k, n):
         assert heaps_equal(hg, ho), (bits, dim, k, ci)
 
 
+# ------------------------------------------------------ prepared queries
+def test_prepared_queries_match_golden(vc, golden_cases):
+    """prepare_queries kept on the device: its state equals the reference's
+    (LUT bytes, scales, biases, bias_corr); prepared_query_at(i) holds row i;
+    search through the handle (score_prepared over the index's PERM0 export)
+    equals the reference's top-k; repacking the golden bit-planes equals the
+    golden flat bytes, PERM0 and sequential blocks."""
+    for c, g, x, q, sample in golden_cases:
+        sh, sc = (g["shift"], g["scale"]) if c["tq"] else (None, None)
+        prep = vc.prepare_queries(q, c["bits"], sh, sc)
+        st = prep.export()
+        assert np.array_equal(st["luts"], g["prep_luts"]), c
+        assert bits_equal(st["lut_scale"], g["prep_lut_scale"]), c
+        assert bits_equal(st["lut_bias"], g["prep_lut_bias"]), c
+        assert bits_equal(st["bias_corr"], g["prep_bias_corr"]), c
+        one = prep.query_at(3).export()
+        assert np.array_equal(one["luts"][0], g["prep_luts"][3]) and bits_equal(one["bias_corr"],
+                                                                                g["prep_bias_corr"][3:4]), c
+        n, dim, bits = c["n"], c["dim"], c["bits"]
+        assert np.array_equal(vc.codes_flat_from_packed(g["packed"], n, dim, bits), g["flat"]), c
+        perm0 = vc.repack_perm0(g["packed"], n, dim, bits)
+        assert sha(perm0) == bytes(g["perm0_sha"]).hex(), c
+        assert sha(vc.repack_seq(g["packed"], n, dim, bits)) == bytes(g["seq_sha"]).hex(), c
+        for k in (1, 10, 64):
+            r = vc.score_prepared(prep, k, perm0, g["alpha"])
+            assert np.array_equal(r.ids, g[f"ids_k{k}"]), (c, k)
+            assert scores_equal(r.scores, g[f"scores_k{k}"]), (c, k)
+        prep.close()
+
+
+def test_prepared_score_arguments_match_golden(vc, oracle):
+    """score_prepared through one prepared handle per case, every argument
+    (id_map, query subsets with a repeat, residual correction), against the
+    reference build's golden outputs."""
+    from cases import SCORE_PREPARED_CASES, SCORE_PREPARED_VARIANTS, score_prepared_inputs
+    from conftest import load_golden
+
+    g = load_golden("score_prepared.npz")
+    for ci, (bits, dim, n, nq, k) in enumerate(SCORE_PREPARED_CASES):
+        x, q, *extra = score_prepared_inputs(bits, dim, n, nq)
+        packed, alpha = oracle.encode(x, bits)
+        perm0 = oracle.repack_perm0(packed, bits, dim)
+        prep = vc.prepare_queries(q, bits)
+        for tag, kw in SCORE_PREPARED_VARIANTS(*extra):
+            r = vc.score_prepared(prep, k, perm0, alpha, **kw)
+            assert np.array_equal(r.ids, g[f"c{ci}_{tag}_ids"]), (ci, tag)
+            assert bits_equal(r.scores, g[f"c{ci}_{tag}_scores"]), (ci, tag)
+        with pytest.raises(IndexError):
+            vc.score_prepared(prep, k, perm0, alpha, query_indices=np.array([nq], np.uint64))
+        with pytest.raises(IndexError):
+            prep.query_at(nq)
+        prep.close()
+
+
+def test_prepared_into_chain_matches_golden(vc):
+    """The score_prepared_into golden chains with the queries prepared once
+    and the handle reused by every call: every heap array after every call
+    equals the reference's."""
+    from conftest import load_golden
+    from util import empty_heaps, heaps_equal, score_into_scenarios
+
+    g = load_golden("score_into.npz")
+    checked = 0
+    for name, bits, dim, k, q, shift, scale, calls, want in score_into_scenarios(g):
+        prep = vc.prepare_queries(q, bits, shift, scale)
+        heaps = empty_heaps(q.shape[0], k, {4: 0.05})
+        for ci, (perm0, alpha, id_map, qidx, off) in enumerate(calls):
+            vc.score_prepared_into(prep, k, perm0, alpha, qidx, heaps, id_map=id_map, score_offsets=off)
+            assert heaps_equal(heaps, want[ci]), (name, k, ci)
+            checked += 1
+        prep.close()
+    assert checked >= 10
+
+
+def test_repack_matches_oracle(vc, oracle):
+    """The repack free functions on the GPU equal the oracle's on ragged sizes
+    (a partial last block of 192) for every bit width."""
+    from vectorcache_amd import synthetic
+
+    for bits, dim, n in ((4, 128, 1000), (3, 200, 385), (2, 768, 577), (4, 1536, 193)):
+        x = synthetic.mixture(n, dim, 16, 0.3, 900 + dim, 901 + n)
+        packed, _ = oracle.encode(x, bits)
+        assert np.array_equal(vc.codes_flat_from_packed(packed, n, dim, bits),
+                              oracle.codes_flat(packed, bits, dim).reshape(n, -1)), (bits, dim)
+        assert np.array_equal(vc.repack_perm0(packed, n, dim, bits), oracle.repack_perm0(packed, bits, dim).ravel()), \
+            (bits, dim)
+        assert np.array_equal(vc.repack_seq(packed, n, dim, bits), oracle.repack_seq(packed, bits, dim).ravel()), \
+            (bits, dim)
+
+
 # ------------------------------------------------------------ float re-rank
 def test_float_rescore_matches_golden(vc):
     """float_rescore_topk on the GPU, host buffers: every dim's rows equal

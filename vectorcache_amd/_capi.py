@@ -169,6 +169,17 @@ _SIGS = {
     "vc_merge_results": ([S, S, P, P, S, P, P], I),
     "vc_score_perm0_into": ([P, S, S, S, S, P, P, S, P, P, P, P, S, P, P, P, P, P, P], I),
     "vc_heaps_to_results": ([S, S, P, P, P, P, P], I),
+    "vc_prepare_queries": ([P, S, S, S, P, P, ctypes.POINTER(P)], I),
+    "vc_prepare_queries_device": ([P, S, S, S, P, P, P, ctypes.POINTER(P)], I),
+    "vc_prepared_destroy": ([P], I),
+    "vc_prepared_shape": ([P, ctypes.POINTER(S), ctypes.POINTER(S), ctypes.POINTER(S), ctypes.POINTER(S)], I),
+    "vc_prepared_export": ([P, P, P, P, P], I),
+    "vc_prepared_query_at": ([P, S, ctypes.POINTER(P)], I),
+    "vc_score_prepared": ([P, S, P, P, S, P, P, S, P, P, P, P, ctypes.POINTER(S)], I),
+    "vc_score_prepared_into": ([P, S, P, P, S, P, P, S, P, P, P, P, P, P], I),
+    "vc_codes_flat_from_packed": ([P, S, S, S, P], I),
+    "vc_repack_perm0": ([P, S, S, S, P], I),
+    "vc_repack_seq": ([P, S, S, S, P], I),
     "vc_float_rescore": ([S, S, P, P, P, P, P, S, S], I),
     "vc_float_rescore_device": ([S, S, P, P, P, P, P, S, S, P], I),
     "vc_landmarks_make": ([S, S, ctypes.c_uint64, P], I),

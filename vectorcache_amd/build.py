@@ -2,7 +2,8 @@
 
     python -m vectorcache_amd.build          # or vectorcache_amd.build.build()
 
-* vc_kernels.hip, vc_ivf.hip, vc_probe.hip, vc_capi.cpp, vc_bucketed.cpp, vc_server.cpp, vc_multi.cpp -> hipcc --offload-arch=gfx950
+* vc_kernels.hip, vc_ivf.hip, vc_probe.hip, vc_capi.cpp, vc_bucketed.cpp, vc_server.cpp, vc_multi.cpp,
+  vc_prepared.cpp -> hipcc --offload-arch=gfx950
 * vc_host.cpp, vc_landmark_host.cpp, vc_io.cpp -> g++ (host precompute, contraction off; file formats)
 
 Floating-point flags are part of the numerics contract: -ffp-contract=off
@@ -42,7 +43,8 @@ def _run(cmd):
 def build(verbose: bool = True) -> Path:
     BUILD.mkdir(exist_ok=True)
     objs = []
-    for src in ("vc_kernels.hip", "vc_ivf.hip", "vc_probe.hip", "vc_capi.cpp", "vc_bucketed.cpp", "vc_server.cpp", "vc_multi.cpp"):
+    for src in ("vc_kernels.hip", "vc_ivf.hip", "vc_probe.hip", "vc_capi.cpp", "vc_bucketed.cpp", "vc_server.cpp",
+                "vc_multi.cpp", "vc_prepared.cpp"):
         obj = BUILD / (src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         _run([HIPCC, *HIP_FLAGS, *lang, "-c", CSRC / src, "-o", obj])

@@ -65,15 +65,14 @@ class LandmarkIndex {
     SearchResults r;
     r.nq = queries.size() / dim();
     const std::size_t kk = std::min(k, size());
-    r.scores.resize(r.nq * kk);
-    r.ids.resize(r.nq * kk);
+    r.scores.resize_uninitialized(r.nq * kk);
+    r.ids.resize_uninitialized(r.nq * kk);
     std::size_t k_out = 0;
     detail::throw_on(vc_lm_index_search(h_, queries.data(), r.nq, k,
                                         float_corpus.empty() ? nullptr : float_corpus.data(), r.scores.data(),
                                         r.ids.data(), &k_out));
     r.k = k_out;
-    r.scores.resize(r.nq * k_out);
-    r.ids.resize(r.nq * k_out);
+    detail::narrow_rows(r, kk, k_out);
     return r;
   }
 

@@ -8,9 +8,10 @@
 //    (README.md:105-111; fit_calibration semantics, encode.hpp:33-35);
 //  * SearchResults: row-major nq x k scores/ids, rows sorted by
 //    (score desc, id asc) (reference include/vectorcache/search/search.hpp:77-84);
-//  * merge_search_results (search.hpp:191-192);
-//  * heaps_to_search_results (search.hpp:186-189) and score_perm0_into, the
-//    score_prepared_into of one bucket given as PERM0 blocks (search.hpp:163-177).
+//  * score_perm0_into, the score_prepared_into of one bucket given as PERM0
+//    blocks with the query rows (search.hpp:163-177); SearchResults, the
+//    prepared-query free functions, heaps_to_search_results and
+//    merge_search_results come from vectorcache/search/search.hpp.
 // Exceptions follow the reference: std::invalid_argument for bad dim/bits or
 // a span that is not a multiple of dim, std::runtime_error for search before
 // prepare and for device failures, std::out_of_range for index errors.
@@ -24,29 +25,10 @@
 #include <utility>
 #include <vector>
 
+#include "vectorcache/search/search.hpp"
 #include "vectorcache_hip.h"
 
 namespace vectorcache {
-
-struct SearchResults {
-  std::vector<float> scores;         // nq * k
-  std::vector<std::uint64_t> ids;    // nq * k
-  std::size_t k = 0;
-  std::size_t nq = 0;
-  std::vector<std::size_t> counts;   // empty: every row has k hits
-};
-
-namespace detail {
-inline void throw_on(vc_status st) {
-  if (st == VC_OK) return;
-  const std::string msg = vc_last_error();
-  switch (st) {
-    case VC_ERR_INVALID_ARGUMENT: throw std::invalid_argument(msg);
-    case VC_ERR_OUT_OF_RANGE: throw std::out_of_range(msg);
-    default: throw std::runtime_error(msg);
-  }
-}
-}  // namespace detail
 
 class TurboQuantIndex {
  public:
@@ -87,13 +69,12 @@ class TurboQuantIndex {
     SearchResults r;
     r.nq = queries.size() / dim();
     const std::size_t kk = std::min(k, size());
-    r.scores.resize(r.nq * kk);
-    r.ids.resize(r.nq * kk);
+    r.scores.resize_uninitialized(r.nq * kk);
+    r.ids.resize_uninitialized(r.nq * kk);
     std::size_t k_out = 0;
     detail::throw_on(vc_index_search(h_, queries.data(), r.nq, k, r.scores.data(), r.ids.data(), &k_out));
     r.k = k_out;
-    r.scores.resize(r.nq * k_out);
-    r.ids.resize(r.nq * k_out);
+    detail::narrow_rows(r, kk, k_out);
     return r;
   }
 
@@ -105,33 +86,6 @@ class TurboQuantIndex {
   }
   vc_index* h_ = nullptr;
 };
-
-// Keep the top dst.k of dst U src per query (higher score wins, tie -> lower
-// id, a duplicated id keeps its higher score).
-inline void merge_search_results(SearchResults& dst, const SearchResults& src) {
-  if (src.nq == 0 || src.k == 0) return;
-  if (dst.nq == 0 || dst.k == 0) {
-    dst = src;
-    return;
-  }
-  if (dst.nq != src.nq) throw std::invalid_argument("merge_search_results: nq mismatch");
-  detail::throw_on(vc_merge_results(dst.nq, dst.k, dst.scores.data(), dst.ids.data(), src.k, src.scores.data(),
-                                    src.ids.data()));
-}
-
-// heaps_to_search_results (reference search.hpp:186-189): each heap's
-// heap_sz[q] entries sorted by (score desc, id asc) into row q of out; the rest
-// of the row is (0, 0).
-inline void heaps_to_search_results(SearchResults& out, std::size_t nq, std::size_t k, const float* heap_s,
-                                    const std::uint64_t* heap_i, const std::size_t* heap_sz) {
-  out.nq = nq;
-  out.k = k;
-  out.scores.assign(nq * k, 0.f);
-  out.ids.assign(nq * k, 0);
-  out.counts.clear();
-  if (nq == 0 || k == 0) return;
-  detail::throw_on(vc_heaps_to_results(nq, k, heap_s, heap_i, heap_sz, out.scores.data(), out.ids.data()));
-}
 
 // score_prepared_into (reference search.hpp:163-177) for one bucket given as
 // PERM0 blocks + per-vector scales: `queries` (nq x dim rows; TQ+ shift/scale

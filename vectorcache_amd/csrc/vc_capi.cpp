@@ -175,7 +175,7 @@ SearchLayout search_layout(const vchip::CodeTables& t, size_t n, size_t nq, size
 SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const float* alpha, size_t n,
                        const float* q_dev, size_t nq, size_t k, const uint64_t* id_map_dev, float* out_s,
                        uint64_t* out_i, DevBuf& scratch, hipStream_t s, uint8_t** luts_out, size_t* luts_off,
-                       ScratchArena* arena) {
+                       ScratchArena* arena, const PreLuts* pre) {
   const SearchLayout L = search_layout(t, n, nq, k);
   SearchOut r{L.k_eff};
   if (L.bytes == 0) return r;
@@ -189,14 +189,22 @@ SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const f
   uint8_t* base = (arena != nullptr ? arena : &own)->take(c.off, s);
   auto at = [base](size_t off) { return base + off; };
   auto atf = [base](size_t off) { return reinterpret_cast<float*>(base + off); };
-  hip_check(vchip::launch_prepare_queries(t, q_dev, nq, at(o_luts), atf(o_ls), atf(o_lb), atf(o_bc), atf(o_bt), s),
-            "prepare_queries");
+  const uint8_t* luts = at(o_luts);
+  const float *ls = atf(o_ls), *bt = atf(o_bt);
+  if (pre != nullptr) {
+    luts = pre->luts;
+    ls = pre->scale;
+    bt = pre->bias_total;
+  } else {
+    hip_check(vchip::launch_prepare_queries(t, q_dev, nq, at(o_luts), atf(o_ls), atf(o_lb), atf(o_bc), atf(o_bt), s),
+              "prepare_queries");
+  }
   std::unique_lock probe_lock(g_probe.mu);
   const bool probe = g_probe.on;
   if (!probe) probe_lock.unlock();
   if (probe) hip_check(hipEventRecord(g_probe.start, s), "hipEventRecord");
-  hip_check(vchip::launch_scan(plan, tiled, alpha, n, t.groups, t.groups16, at(o_luts), at(o_lt), atf(o_ls),
-                               atf(o_bt), nq, ke, atf(o_ps), reinterpret_cast<uint32_t*>(at(o_pi)),
+  hip_check(vchip::launch_scan(plan, tiled, alpha, n, t.groups, t.groups16, luts, at(o_lt), ls,
+                               bt, nq, ke, atf(o_ps), reinterpret_cast<uint32_t*>(at(o_pi)),
                                reinterpret_cast<int*>(at(o_pc)), s),
             "scan");
   if (probe) {
@@ -663,20 +671,6 @@ vc_status vc_landmarks_quantize(const float* L, size_t dim, size_t m, uint8_t* c
     std::memcpy(scales, q.scales.data(), m * 4);
   });
 }
-
-namespace {
-struct Scoped {
-  hipStream_t s = nullptr;
-  DevBuf a, scratch;
-  Scoped() { hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"); }
-  ~Scoped() {
-    (void)hipStreamSynchronize(s);
-    a.release();
-    scratch.release();
-    (void)hipStreamDestroy(s);
-  }
-};
-}  // namespace
 
 vc_status vc_encode(const float* vectors, size_t n, size_t dim, size_t bits, const float* shift, const float* scale,
                     uint8_t* packed_out, float* alpha_out) {

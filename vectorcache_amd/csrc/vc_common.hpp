@@ -171,9 +171,34 @@ struct SearchLayout {
 };
 SearchLayout search_layout(const vchip::CodeTables& t, size_t n, size_t nq, size_t k);
 
+// Query state already on the device (a prepared-queries handle): nq rows of
+// G x 32 LUT bytes, their scales and total biases (lut_bias + bias_corr).
+struct PreLuts {
+  const uint8_t* luts = nullptr;
+  const float* scale = nullptr;
+  const float* bias_total = nullptr;
+};
+
+// `pre` (not NULL): scan with those LUTs; q_dev is not read.
 SearchOut search_tiled(const vchip::CodeTables& t, const uint8_t* tiled, const float* alpha, size_t n,
                        const float* q_dev, size_t nq, size_t k, const uint64_t* id_map_dev, float* out_s,
                        uint64_t* out_i, DevBuf& scratch, hipStream_t s, uint8_t** luts_out = nullptr,
-                       size_t* luts_off = nullptr, ScratchArena* arena = nullptr);
+                       size_t* luts_off = nullptr, ScratchArena* arena = nullptr, const PreLuts* pre = nullptr);
+
+// A private non-blocking stream with its own device buffers, synchronized and
+// released at scope exit (the stateless host-buffer entry points).
+struct Scoped {
+  hipStream_t s = nullptr;
+  DevBuf a, scratch;
+  Scoped() { hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"); }
+  ~Scoped() {
+    (void)hipStreamSynchronize(s);
+    a.release();
+    scratch.release();
+    (void)hipStreamDestroy(s);
+  }
+  Scoped(const Scoped&) = delete;
+  Scoped& operator=(const Scoped&) = delete;
+};
 
 }  // namespace vcapi

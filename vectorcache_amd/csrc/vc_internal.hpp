@@ -236,5 +236,11 @@ hipError_t launch_perm0_to_flat(const std::uint8_t* perm0, std::size_t n, int gr
                                 std::uint8_t* flat, hipStream_t s);
 hipError_t launch_flat_to_packed(const std::uint8_t* flat, std::size_t n, int dim, int bits,
                                  std::uint8_t* packed, hipStream_t s);
+// bit-planes -> flat bytes (codes_flat_from_packed)
+hipError_t launch_packed_to_flat(const std::uint8_t* packed, std::size_t n, int dim, int bits, std::uint8_t* flat,
+                                 hipStream_t s);
+// dst row i = src row idx[i]; row_bytes a multiple of 4
+hipError_t launch_gather_rows(const void* src, const std::uint64_t* idx, std::size_t m, std::size_t row_bytes,
+                              void* dst, hipStream_t s);
 
 }  // namespace vchip

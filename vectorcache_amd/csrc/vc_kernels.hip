@@ -3123,6 +3123,39 @@ __global__ void flat_to_packed_kernel(const uint8_t* __restrict__ flat, long n, 
   }
 }
 
+// Bit-planes -> flat bytes (codes_flat_from_packed, src/pack/pack.cpp:98-131,
+// 352-356), the inverse of flat_to_packed_kernel: one thread per 8 codes.
+__global__ void packed_to_flat_kernel(const uint8_t* __restrict__ packed, long n, int D, int bits,
+                                      uint8_t* __restrict__ flat) {
+  const int G = D / (8 / bits), bpp = D / 8;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n * bpp; i += (long)gridDim.x * blockDim.x) {
+    const long v = i / bpp;
+    const int c8 = (int)(i - v * bpp);
+    unsigned codes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int p = 0; p < bits; ++p) {
+      const uint32_t byte = packed[v * (long)(bits * bpp) + p * bpp + c8];
+      for (int kk = 0; kk < 8; ++kk) codes[kk] |= ((byte >> (7 - kk)) & 1u) << p;
+    }
+    uint8_t* out = flat + v * G;
+    if (bits == 2) {
+      out[c8 * 2] = (uint8_t)(codes[0] << 6 | codes[1] << 4 | codes[2] << 2 | codes[3]);
+      out[c8 * 2 + 1] = (uint8_t)(codes[4] << 6 | codes[5] << 4 | codes[6] << 2 | codes[7]);
+    } else {
+      for (int h = 0; h < 4; ++h) out[c8 * 4 + h] = (uint8_t)(codes[2 * h] << 4 | codes[2 * h + 1]);
+    }
+  }
+}
+
+// dst row i = src row idx[i] (rows of row_words 32-bit words).
+__global__ void gather_rows_kernel(const uint32_t* __restrict__ src, const uint64_t* __restrict__ idx, long m,
+                                   int row_words, uint32_t* __restrict__ dst) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < m * row_words; i += (long)gridDim.x * blockDim.x) {
+    const long r = i / row_words;
+    const int w = (int)(i - r * row_words);
+    dst[i] = src[(long)idx[r] * row_words + w];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Landmark index (reference src/index.cpp:151-265 and the AVX2 path of
 // src/search/search.cpp:1304-1546, where sign planes are never built).
@@ -4102,6 +4135,24 @@ hipError_t launch_flat_to_packed(const std::uint8_t* flat, std::size_t n, int di
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(flat_to_packed_kernel, dim3(grid_for((long)n * (dim / 8), 256)), dim3(256), 0, s, flat,
                      (long)n, dim, bits, packed);
+  return hipGetLastError();
+}
+
+hipError_t launch_packed_to_flat(const std::uint8_t* packed, std::size_t n, int dim, int bits, std::uint8_t* flat,
+                                 hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(packed_to_flat_kernel, dim3(grid_for((long)n * (dim / 8), 256)), dim3(256), 0, s, packed,
+                     (long)n, dim, bits, flat);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(const void* src, const std::uint64_t* idx, std::size_t m, std::size_t row_bytes,
+                              void* dst, hipStream_t s) {
+  if (m == 0 || row_bytes == 0) return hipSuccess;
+  if (row_bytes % 4 != 0) return hipErrorInvalidValue;
+  const int words = (int)(row_bytes / 4);
+  hipLaunchKernelGGL(gather_rows_kernel, dim3(grid_for((long)m * words, 256)), dim3(256), 0, s,
+                     static_cast<const uint32_t*>(src), idx, (long)m, words, static_cast<uint32_t*>(dst));
   return hipGetLastError();
 }
 

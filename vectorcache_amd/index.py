@@ -407,6 +407,127 @@ def score_perm0_into(queries, k: int, bits: int, perm0, alpha, query_indices, he
     return heaps
 
 
+class PreparedQueries:
+    """prepare_queries (reference search.cpp:762-860) run once on the GPU and
+    kept in device memory, reused by score_prepared / score_prepared_into over
+    many code ranges as the reference reuses its PreparedQueries
+    (include/vectorcache/search/search.hpp:95-112)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        nq, dim, bits, g = S(), S(), S(), S()
+        check(lib.vc_prepared_shape(self._h, ctypes.byref(nq), ctypes.byref(dim), ctypes.byref(bits), ctypes.byref(g)))
+        self.nq, self.dim, self.bits, self.n_byte_groups = nq.value, dim.value, bits.value, g.value
+
+    def close(self):
+        if self._h:
+            lib.vc_prepared_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def export(self) -> dict:
+        """luts [nq, G*32] u8, lut_scale, lut_bias (without bias_corr), bias_corr."""
+        g = self.n_byte_groups
+        out = {"luts": np.zeros((self.nq, g * 32), np.uint8), "lut_scale": np.zeros(self.nq, np.float32),
+               "lut_bias": np.zeros(self.nq, np.float32), "bias_corr": np.zeros(self.nq, np.float32)}
+        check(lib.vc_prepared_export(self._h, _ptr(out["luts"]), _ptr(out["lut_scale"]), _ptr(out["lut_bias"]),
+                                     _ptr(out["bias_corr"])))
+        return out
+
+    def query_at(self, qi: int) -> "PreparedQueries":
+        """prepared_query_at (reference search.cpp:1045-1068)."""
+        h = P()
+        check(lib.vc_prepared_query_at(self._h, S(qi), ctypes.byref(h)))
+        return PreparedQueries(h)
+
+
+def prepare_queries(queries, bits: int, shift=None, scale=None) -> PreparedQueries:
+    """Host query rows (nq x dim) -> PreparedQueries on the GPU."""
+    q = _f32(queries)
+    nq, dim = q.shape
+    sh = None if shift is None else _f32(shift)
+    sc = None if scale is None else _f32(scale)
+    h = P()
+    check(lib.vc_prepare_queries(_ptr(q), S(nq), S(dim), S(bits), _ptr(sh), _ptr(sc), ctypes.byref(h)))
+    return PreparedQueries(h)
+
+
+def score_prepared(prep: PreparedQueries, k: int, perm0, alpha, id_map=None, query_indices=None, residual_norms=None,
+                   residual_scales=None) -> SearchResults:
+    """score_prepared (reference search.cpp:1072-1154) with prepared queries:
+    vc_score_perm0's results without rebuilding the LUTs."""
+    alpha = _f32(alpha)
+    n = alpha.size
+    perm0 = np.ascontiguousarray(perm0, np.uint8)
+    im = None if id_map is None else np.ascontiguousarray(id_map, np.uint64)
+    qi = None if query_indices is None else np.ascontiguousarray(query_indices, np.uint64)
+    rn = None if residual_norms is None else _f32(residual_norms)
+    rs = None if residual_scales is None else _f32(residual_scales)
+    nq_out = prep.nq if qi is None else qi.size
+    ke = min(k, n)
+    scores = np.zeros((nq_out, max(ke, 0)), np.float32)
+    ids = np.zeros((nq_out, max(ke, 0)), np.uint64)
+    ko = S()
+    check(lib.vc_score_prepared(prep._h, S(k), _ptr(perm0), _ptr(alpha), S(n), _ptr(im), _ptr(qi),
+                                S(0 if qi is None else qi.size), _ptr(rn), _ptr(rs), _ptr(scores), _ptr(ids),
+                                ctypes.byref(ko)))
+    return SearchResults(scores[:, :ko.value], ids[:, :ko.value])
+
+
+def score_prepared_into(prep: PreparedQueries, k: int, perm0, alpha, query_indices, heaps, id_map=None,
+                        score_offsets=None):
+    """score_prepared_into (reference search.cpp:1156-1267) with prepared
+    queries; the listed queries' heaps (see empty_heaps) are updated in place."""
+    nq = prep.nq
+    alpha = _f32(alpha)
+    n = alpha.size
+    perm0 = np.ascontiguousarray(perm0, np.uint8)
+    qi = np.ascontiguousarray(query_indices, np.uint64)
+    hs, hi, hsz, hmin, hmi = heaps
+    for a, dt in ((hs, np.float32), (hi, np.uint64), (hsz, np.uint64), (hmin, np.float32), (hmi, np.uint64)):
+        if a.dtype != dt or not a.flags.c_contiguous:
+            raise ValueError("heaps must be C-contiguous arrays of the empty_heaps() types")
+    if hs.shape != (nq, k) or hi.shape != (nq, k) or hsz.shape != (nq,) or hmin.shape != (nq,) or hmi.shape != (nq,):
+        raise ValueError("heap shapes must be [nq, k] and [nq]")
+    im = None if id_map is None else np.ascontiguousarray(id_map, np.uint64)
+    off = None if score_offsets is None else _f32(score_offsets)
+    check(lib.vc_score_prepared_into(prep._h, S(k), _ptr(perm0), _ptr(alpha), S(n), _ptr(im), _ptr(qi), S(qi.size),
+                                     _ptr(hs), _ptr(hi), _ptr(hsz), _ptr(hmin), _ptr(hmi), _ptr(off)))
+    return heaps
+
+
+def _repack(fn, packed, n: int, dim: int, bits: int, out_bytes: int) -> np.ndarray:
+    packed = np.ascontiguousarray(packed, np.uint8)
+    if packed.size != n * bits * dim // 8:
+        raise ValueError("packed must hold n x bits*dim/8 bytes")
+    out = np.zeros(out_bytes, np.uint8)
+    check(fn(_ptr(packed), S(n), S(dim), S(bits), _ptr(out)))
+    return out
+
+
+def codes_flat_from_packed(packed, n: int, dim: int, bits: int) -> np.ndarray:
+    """codes_flat_from_packed (reference pack.cpp:98-131,352-356) on the GPU: [n, G] bytes."""
+    g = dim // (8 // bits)
+    return _repack(lib.vc_codes_flat_from_packed, packed, n, dim, bits, n * g).reshape(n, g)
+
+
+def repack_perm0(packed, n: int, dim: int, bits: int) -> np.ndarray:
+    """repack_perm0 (reference pack.cpp:157-186,388-401) on the GPU."""
+    g = dim // (8 // bits)
+    return _repack(lib.vc_repack_perm0, packed, n, dim, bits, (n + 191) // 192 * g * 192)
+
+
+def repack_seq(packed, n: int, dim: int, bits: int) -> np.ndarray:
+    """repack_seq (reference pack.cpp:133-154,358-386) on the GPU."""
+    g = dim // (8 // bits)
+    return _repack(lib.vc_repack_seq, packed, n, dim, bits, (n + 191) // 192 * g * 192)
+
+
 def heaps_to_results(heaps) -> SearchResults:
     """heaps_to_search_results (reference search.cpp:1269-1291) on the GPU."""
     hs, hi, hsz = heaps[0], heaps[1], heaps[2]
