@@ -12,7 +12,8 @@ QPS at the same target, the bucket scan's HBM roofline, latency and the
 reference's CPU search_flat on a sample.
 
 Other workloads: flat1m (1M x 1536 flat index, TQ+, 10k queries), flat1m_k64,
-flat1m_b2, flat100k, landmark1m, bucketed1m.
+flat1m_b2, flat100k, landmark1m, landmark1m_planes (the same index searched
+by the staged sign-plane branch), bucketed1m.
 
     python bench.py [--steps K] [--warmup W] [--workload NAME]
 """
@@ -48,6 +49,9 @@ LANDMARK_WORKLOADS = {
     # (TurboQuantIndex) with its default 1024 landmarks
     "landmark1m": (1_000_000, 1536, 4, 1024, 10_000, 10),
 }
+# The same index with sign planes kept: the staged search the reference runs
+# on an AVX-512 VNNI host (sign-plane shortlist, then the permute-dot rescore).
+LANDMARK_PLANES = {"landmark1m_planes": "landmark1m"}
 
 
 SERVE_WORKLOADS = {
@@ -717,7 +721,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="bucketed10m",
-                    choices=sorted(WORKLOADS) + sorted(LANDMARK_WORKLOADS) + sorted(BUCKETED_WORKLOADS)
+                    choices=sorted(WORKLOADS) + sorted(LANDMARK_WORKLOADS) + sorted(LANDMARK_PLANES)
+                    + sorted(BUCKETED_WORKLOADS)
                     + sorted(SERVE_WORKLOADS) + sorted(MULTI_WORKLOADS))
     ap.add_argument("--tile", type=int, default=0, help="cap the scan query tile (1, 2, 4, 8)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -734,7 +739,7 @@ def main():
                                    "with an all-gather of per-GPU top-k lists", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    if args.workload in LANDMARK_WORKLOADS:
+    if args.workload in LANDMARK_WORKLOADS or args.workload in LANDMARK_PLANES:
         return run_landmark(args)
     if args.workload in SERVE_WORKLOADS:
         return run_serve(args)
@@ -902,26 +907,37 @@ def main():
     print(json.dumps(out))
 
 
-def landmark_cpu_baseline(x_np, q_np, bits, m, k, n_full, budget_queries=1000, n_sample=100_000):
+def _host_has_vnni():
+    try:
+        flags = open("/proc/cpuinfo").read()
+    except OSError:
+        return False
+    return all(f in flags for f in ("avx512_vnni", "avx512vbmi", "avx512bw"))
+
+
+def landmark_cpu_baseline(x_np, q_np, bits, m, k, n_full, budget_queries=1000, n_sample=100_000, planes=False):
     """The reference's own LandmarkIndex (oracle/_ref) built on the first
     n_sample vectors, its search() of budget_queries queries timed; the rate is
     scaled to the full index (the scan is linear in n). Also checks a GPU
-    index of the same sample against it."""
+    index of the same sample against it. planes: the reference's VNNI build
+    (its staged search), which needs an AVX-512 VNNI host."""
     sys.path.insert(0, str(ROOT / "oracle"))
     from oracle import Oracle, available
 
     from vectorcache_amd.landmark import LandmarkIndex
 
     kind = "reference" if available("reference") else "port"
+    if planes and (kind != "reference" or not _host_has_vnni()):
+        return {"error": "the staged search's CPU figure needs the reference build on an AVX-512 VNNI host"}
     o = Oracle(kind)
     host = host_cpu_info()
     cores = host["physical_cores_allowed"]
     o.set_threads(cores)
     xs = np.ascontiguousarray(x_np[:n_sample])
     qs = np.ascontiguousarray(q_np[:budget_queries])
-    s, i, info = o.lm_search(xs, qs, k, scan_bits=bits, n_landmarks=m, seed=0)
+    s, i, info = o.lm_search(xs, qs, k, scan_bits=bits, n_landmarks=m, seed=0, planes=planes)
     dt = info["search_s"]
-    g = LandmarkIndex(xs.shape[1], bits, m)
+    g = LandmarkIndex(xs.shape[1], bits, m, planes=planes)
     g.add(xs)
     g.prepare()
     r = g.search(qs, k)
@@ -1312,14 +1328,15 @@ def run_landmark(args):
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    n, dim, bits, m, nq, k = LANDMARK_WORKLOADS[args.workload]
+    planes = args.workload in LANDMARK_PLANES
+    n, dim, bits, m, nq, k = LANDMARK_WORKLOADS[LANDMARK_PLANES.get(args.workload, args.workload)]
     S = (m + 7) // 8 * 8
     G = S // (8 // bits)
     log(f"workload {args.workload}: n={n} dim={dim} scan_bits={bits} landmarks={m} nq={nq} k={k}")
     x = synthetic.anisotropic_torch(n, dim, 2001, dev)
     q = synthetic.anisotropic_torch(nq, dim, 2003, dev)
     LandmarkIndex(dim, bits, m).close()  # the basis (host Gram-Schmidt) is memoized per process
-    ix = LandmarkIndex(dim, bits, m)
+    ix = LandmarkIndex(dim, bits, m, planes=planes)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ix.add(x)
@@ -1328,6 +1345,10 @@ def run_landmark(args):
     ix.prepare()
     t_prep = time.perf_counter() - t0
     log(f"add {n / t_add / 1e6:.2f} M vectors/s, prepare {n / t_prep / 1e6:.2f} M vectors/s")
+    if planes and not ix.planes_active():
+        raise RuntimeError("sign planes were not kept for the staged workload")
+    # the scanned bytes per vector: the sign plane (S / 8) in the staged search
+    G_scan = S // 8 if planes else G
 
     if args.tile:
         vc.set_scan_tile(args.tile)
@@ -1348,7 +1369,7 @@ def run_landmark(args):
     elapsed = time.perf_counter() - t0
     qps = nq * args.steps / elapsed
     scan_avg_s = float(np.mean(scan_ms)) / 1e3
-    scan_bytes = math.ceil(nq / tile) * n * (G + 4) + nq * G * 32
+    scan_bytes = math.ceil(nq / tile) * n * (G_scan + 4) + nq * G_scan * 32
     achieved = scan_bytes / scan_avg_s / 1e9
     log(f"search {qps:.0f} QPS, {elapsed / args.steps * 1e3:.1f} ms/step, scan kernel {scan_avg_s * 1e3:.1f} ms "
         f"(tile {tile}, ranges {ranges})")
@@ -1385,7 +1406,7 @@ def run_landmark(args):
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = landmark_cpu_baseline(x.cpu().numpy(), q.cpu().numpy(), bits, m, k, n)
+            cpu = landmark_cpu_baseline(x.cpu().numpy(), q.cpu().numpy(), bits, m, k, n, planes=planes)
             log(f"cpu baseline {cpu}")
         except Exception as e:  # reported, never fatal for the GPU figure
             cpu = {"error": str(e)}
@@ -1406,8 +1427,10 @@ def run_landmark(args):
         "data": "synthetic (seeded, generated on device)",
         "config": {
             "workload": f"{args.workload}: landmark index (LandmarkIndex), {n} x {dim} float32, {m} landmarks, "
-                        f"{bits}-bit scan codes, {nq} queries, k={k}",
+                        f"{bits}-bit scan codes, {nq} queries, k={k}"
+                        + (", sign planes (staged search of an AVX-512 VNNI host)" if planes else ""),
             "n": n, "dim": dim, "scan_bits": bits, "n_landmarks": m, "scan_dim": S, "nq": nq, "k": k,
+            "planes": planes,
             "query_tile": tile, "vector_ranges": ranges,
         },
         "roofline": {
@@ -1415,7 +1438,9 @@ def run_landmark(args):
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
             "kernel": f"scan_pair_kernel (query tile {tile})", "avg_ms": scan_avg_s * 1e3,
             "algorithmic_bytes": scan_bytes,
-            "note": f"bytes = ceil(nq/T) passes x N x (G+4) + LUTs; G={G} groups of the landmark rows; T={tile}",
+            "note": f"bytes = ceil(nq/T) passes x N x (G+4) + LUTs; G={G_scan} "
+                    + ("sign bytes of the landmark rows (the staged search's first step)" if planes
+                       else "groups of the landmark rows") + f"; T={tile}",
         },
         "build": {"add_vectors_per_s": n / t_add, "prepare_vectors_per_s": n / t_prep,
                   "add_ms": t_add * 1e3, "prepare_ms": t_prep * 1e3},

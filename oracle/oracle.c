@@ -1415,10 +1415,241 @@ static float lm_norm_sq(const float* x, size_t m) {
  * a prepare() find the earlier rows zeroed (landmark_rows_ was released and is
  * resized), and prepare() encodes every row. Arguments as vcref_lm_search
  * (search_seconds is left 0: the restatement is not timed). */
+/* ---- staged sign-plane search (the reference's VNNI-host branch) ----------
+ * search_landmark_rerank with sign planes (src/search/search.cpp:1379-1510):
+ * on an AVX-512 VNNI host the index keeps the sign plane (the top bit of each
+ * 4-bit landmark code; src/pack/pack.cpp:447-449,490-541) once n >= 32768,
+ * and a search
+ *   1. scans the sign bytes with build_sign_lut's uint8 LUTs (search.cpp:
+ *      862-914): per vector A = the integer sum of the LUT entries of its
+ *      nibbles, score = fma((float) A, lut_scale, lut_bias) (search_vnni.cpp:
+ *      368-389; alpha on sign scores is opt-in and off by default, :1478-1487),
+ *      and keeps the E best (E =
+ *      planes_exact_len(k, planes_shortlist_len(k, 4)), pack.cpp:451-463) by
+ *      (score desc, id asc): the buffered collector with its sample seeds
+ *      returns that set whenever no score ties the E-th;
+ *   2. rescores those E in that order with build_permute_dot's int8 query
+ *      weights and centroid levels (search_vnni.cpp:1318-1360): acc = sum
+ *      over byte groups of lev[lo] * w_lo + lev[hi] * w_hi (exact int32),
+ *      f = (float) acc * (cs * qs), score = the flush's
+ *      f * alpha + q_perp * rn (search_vnni.cpp:1194-1217, one product fused),
+ *      pushed into a k-heap in candidate order (topk_heap.hpp:101-157), then
+ *      sorted by (score desc, id asc).
+ * The fusion sites of steps 1 and 2 are the compiled reference's; the
+ * variant flags below exist so that tests can show the other choices fail. */
+static int g_staged_variant = 0; /* bit 0: step 1 unfused; bit 1: step 2 fuses the other product */
+void oc_set_staged_variant(int v) { g_staged_variant = v; }
+
+size_t oc_planes_exact_len(size_t k) {
+  const size_t s_len = 256 > ((k >= 64) ? k * 16 : k * 20) ? 256 : ((k >= 64) ? k * 16 : k * 20);
+  const size_t want = 256 > 6 * k ? 256 : 6 * k;
+  return s_len < want ? s_len : want;
+}
+
+static size_t planes_shortlist_len4(size_t k) {
+  const size_t a = (k >= 64) ? k * 16 : k * 20;
+  return a > 256 ? a : 256;
+}
+
+/* build_sign_lut (search.cpp:862-914) for one query row of S dims */
+static void build_sign_lut(const float* q, size_t S, uint8_t* lut, float* scale_out, float* bias_out,
+                           float* fv) {
+  const size_t nsg = S / 8;
+  float* mins = fv + nsg * 32;
+  float bias = 0.f, max_span = 0.f;
+  for (size_t g = 0; g < nsg; ++g)
+    for (size_t half = 0; half < 2; ++half) {
+      const size_t d = g * 8 + half * 4;
+      const float p0 = q[d], p1 = q[d + 1], p2 = q[d + 2], p3 = q[d + 3];
+      const float a[4] = {-p0 - p1, p1 - p0, p0 - p1, p0 + p1};
+      const float b[4] = {-p2 - p3, p3 - p2, p2 - p3, p2 + p3};
+      for (size_t pat = 0; pat < 16; ++pat) fv[g * 32 + half * 16 + pat] = a[pat >> 2] + b[pat & 3];
+      const float m01 = fabsf(p0) + fabsf(p1), m23 = fabsf(p2) + fabsf(p3);
+      const float mn = -m01 - m23, mx = m01 + m23;
+      mins[g * 2 + half] = mn;
+      bias = bias + mn;
+      if (mx - mn > max_span) max_span = mx - mn;
+    }
+  float scale = max_span > 0.f ? max_span / 127.f : 1.f;
+  float inv = 1.f;
+  if (scale >= FLT_MIN)
+    inv = 1.f / scale;
+  else
+    scale = 1.f;
+  for (size_t g = 0; g < nsg; ++g)
+    for (size_t half = 0; half < 2; ++half) {
+      const float mn = mins[g * 2 + half];
+      for (size_t i = 0; i < 16; ++i) {
+        const size_t j = g * 32 + half * 16 + i;
+        float v = roundf((fv[j] - mn) * inv);
+        v = v < 0.f ? 0.f : (v > 127.f ? 127.f : v);
+        lut[j] = (uint8_t)v;
+      }
+    }
+  *scale_out = scale;
+  *bias_out = bias;
+}
+
+typedef struct {
+  int8_t lev[16];
+  int8_t* w; /* [2 * G]: w[2g] = weight of the low nibble (dim 2g+1), w[2g+1] of the high (dim 2g) */
+  float scale;
+} oc_pdot;
+
+/* build_permute_dot (search_vnni.cpp:1318-1360) for one query row of S dims */
+static void build_permute_dot(const float* q, size_t S, const float* cen, oc_pdot* pd) {
+  float cmax = 0.f;
+  for (int i = 0; i < 16; ++i) cmax = fabsf(cen[i]) > cmax ? fabsf(cen[i]) : cmax;
+  const float cs = cmax > 0.f ? cmax / 127.f : 1.f;
+  for (int i = 0; i < 16; ++i) {
+    float v = roundf(cen[i] / cs);
+    v = v < -127.f ? -127.f : (v > 127.f ? 127.f : v);
+    pd->lev[i] = (int8_t)v;
+  }
+  float qmax = 0.f;
+  for (size_t d = 0; d < S; ++d) qmax = fabsf(q[d]) > qmax ? fabsf(q[d]) : qmax;
+  float qs = qmax / 127.f, inv = 1.f;
+  if (qs >= FLT_MIN)
+    inv = 1.f / qs;
+  else
+    qs = 1.f;
+  for (size_t g = 0; g < S / 2; ++g) {
+    float lo = roundf(q[2 * g + 1] * inv), hi = roundf(q[2 * g] * inv);
+    lo = lo < -127.f ? -127.f : (lo > 127.f ? 127.f : lo);
+    hi = hi < -127.f ? -127.f : (hi > 127.f ? 127.f : hi);
+    pd->w[2 * g] = (int8_t)lo;
+    pd->w[2 * g + 1] = (int8_t)hi;
+  }
+  pd->scale = cs * qs;
+}
+
+/* the staged search of one query (steps 1 and 2 above); rows of ke entries */
+static void lm_staged_query(const float* q_land, size_t S, const float* lcen, const uint8_t* flat, size_t n,
+                            const float* alpha, const float* rn, float q_perp, size_t ke, float* fv,
+                            uint8_t* slut, oc_cand* cand, int8_t* wbuf, float* scores_out, uint64_t* ids_out) {
+  const size_t G = S / 2, nsg = S / 8;
+  float ls, lb;
+  build_sign_lut(q_land, S, slut, &ls, &lb, fv);
+  for (size_t v = 0; v < n; ++v) {
+    const uint8_t* row = flat + v * G;
+    int32_t A = 0;
+    for (size_t g = 0; g < nsg; ++g) {
+      unsigned sb = 0;
+      for (int h = 0; h < 4; ++h) {
+        const unsigned b = row[4 * g + h];
+        sb = (sb << 2) | ((b >> 7) & 1u) << 1 | ((b >> 3) & 1u);
+      }
+      A += slut[g * 32 + (sb >> 4)] + slut[g * 32 + 16 + (sb & 15u)];
+    }
+    cand[v].s = (g_staged_variant & 1) ? ((float)A * ls + lb) : fmaf((float)A, ls, lb);
+    cand[v].id = v;
+  }
+  const size_t E = oc_planes_exact_len(ke) < n ? oc_planes_exact_len(ke) : n;
+  qsort(cand, n, sizeof(oc_cand), cand_cmp);
+  oc_pdot pd;
+  pd.w = wbuf;
+  build_permute_dot(q_land, S, lcen, &pd);
+  float* hs = (float*)malloc(ke * sizeof(float));
+  uint64_t* hi = (uint64_t*)malloc(ke * sizeof(uint64_t));
+  size_t hsz = 0, hmi = 0;
+  float hmin = -INFINITY;
+  for (size_t c = 0; c < E; ++c) {
+    const uint64_t id = cand[c].id;
+    const uint8_t* row = flat + id * G;
+    int32_t acc = 0;
+    for (size_t g = 0; g < G; ++g)
+      acc += (int32_t)pd.lev[row[g] & 15u] * pd.w[2 * g] + (int32_t)pd.lev[row[g] >> 4] * pd.w[2 * g + 1];
+    const float f = (float)acc * pd.scale;
+    const float s = (g_staged_variant & 2) ? fmaf(q_perp, rn[id], f * alpha[id]) : fmaf(f, alpha[id], q_perp * rn[id]);
+    oc_heap_push(hs, hi, &hsz, &hmin, &hmi, ke, s, id);
+  }
+  oc_cand* out = (oc_cand*)malloc((hsz + 1) * sizeof(oc_cand));
+  for (size_t i = 0; i < hsz; ++i) {
+    out[i].s = hs[i];
+    out[i].id = hi[i];
+  }
+  qsort(out, hsz, sizeof(oc_cand), cand_cmp);
+  for (size_t j = 0; j < ke; ++j) {
+    scores_out[j] = j < hsz ? out[j].s : 0.f;
+    ids_out[j] = j < hsz ? out[j].id : 0;
+  }
+  free(out);
+  free(hs);
+  free(hi);
+}
+
+/* The sign-plane scan alone (step 1 above) over bit-plane codes (n x 4 S/8):
+ * top k of every vector's sign score by (score desc, id asc). */
+int oc_sign_topk(const float* rows, size_t nq, size_t S, const uint8_t* packed, size_t n, size_t k,
+                 float* scores_out, uint64_t* ids_out, size_t* k_out) {
+  if (S == 0 || S % 8 != 0) return -1;
+  const size_t ke = k < n ? k : n, G = S / 2;
+  *k_out = ke;
+  uint8_t* flat = (uint8_t*)malloc(n * G + 1);
+  oc_codes_flat(packed, n, 4, S, flat);
+  float* fv = (float*)malloc((S / 8) * 34 * sizeof(float));
+  uint8_t* slut = (uint8_t*)malloc((S / 8) * 32);
+  oc_cand* cand = (oc_cand*)malloc((n + 1) * sizeof(oc_cand));
+  for (size_t i = 0; i < nq; ++i) {
+    float ls, lb;
+    build_sign_lut(rows + i * S, S, slut, &ls, &lb, fv);
+    for (size_t v = 0; v < n; ++v) {
+      const uint8_t* row = flat + v * G;
+      int32_t A = 0;
+      for (size_t g = 0; g < S / 8; ++g) {
+        unsigned sb = 0;
+        for (int h = 0; h < 4; ++h) {
+          const unsigned b = row[4 * g + h];
+          sb = (sb << 2) | ((b >> 7) & 1u) << 1 | ((b >> 3) & 1u);
+        }
+        A += slut[g * 32 + (sb >> 4)] + slut[g * 32 + 16 + (sb & 15u)];
+      }
+      cand[v].s = (g_staged_variant & 1) ? ((float)A * ls + lb) : fmaf((float)A, ls, lb);
+      cand[v].id = v;
+    }
+    qsort(cand, n, sizeof(oc_cand), cand_cmp);
+    for (size_t j = 0; j < ke; ++j) {
+      scores_out[i * ke + j] = cand[j].s;
+      ids_out[i * ke + j] = cand[j].id;
+    }
+  }
+  free(flat);
+  free(fv);
+  free(slut);
+  free(cand);
+  return 0;
+}
+
+static int lm_search_impl(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
+                          const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
+                          size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
+                          uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds, int planes);
+
 int oc_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
                  const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
                  size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
                  uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds) {
+  return lm_search_impl(x, batch_rows, n_batches, prepare_between, q, nq, dim, scan_bits, m, seed, rerank_bits, k,
+                        float_corpus, scores_out, ids_out, k_out, info, search_seconds, 0);
+}
+
+/* oc_lm_search as the reference runs on an AVX-512 VNNI host (planes = 1):
+ * the staged branch when scan_bits == 4, at least 32768 vectors are prepared
+ * and the shortlist is shorter than the index (search.cpp:1379-1384,
+ * pack.cpp:447-449). Below that the reference's VNNI host scans with its
+ * permute-dot kernels, which this restatement does not cover (-2). */
+int oc_lm_search_planes(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
+                        const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
+                        size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
+                        uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds) {
+  return lm_search_impl(x, batch_rows, n_batches, prepare_between, q, nq, dim, scan_bits, m, seed, rerank_bits, k,
+                        float_corpus, scores_out, ids_out, k_out, info, search_seconds, 1);
+}
+
+static int lm_search_impl(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
+                          const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
+                          size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
+                          uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds, int planes) {
   if (search_seconds != NULL) *search_seconds = 0.0;
   if (rerank_bits == 0) rerank_bits = scan_bits;
   if (dim == 0 || dim % 8 != 0 || scan_bits < 2 || scan_bits > 4 || rerank_bits < 2 || rerank_bits > 4 ||
@@ -1519,12 +1750,30 @@ int oc_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, int
       build_lut(q_land + i * S, S, scan_bits, lcen, luts + i * G * 32, &ls[i], &bias, fv);
       lb[i] = bias + 0.f;
     }
+    /* S % 32 != 0: the reference's sign scan reads past its 32-wide sign
+     * blocks with the 192-wide kernel (search_vnni.cpp:1466-1469, 803-843), an
+     * undefined result; the exact scan below stands in for it. */
+    const int sign_ok = S % 32 == 0;
+    const int staged = planes && sign_ok && scan_bits == 4 && n_enc >= 32768 && planes_shortlist_len4(ke) < n_enc;
+    if (planes && sign_ok && !staged) rc = -2;
+    if (!sign_ok) planes = 0;
+    if (staged) {
+      uint8_t* slut = (uint8_t*)malloc((S / 8) * 32);
+      oc_cand* cand = (oc_cand*)malloc(n_enc * sizeof(oc_cand));
+      int8_t* wbuf = (int8_t*)malloc(S);
+      for (size_t i = 0; i < nq; ++i)
+        lm_staged_query(q_land + i * S, S, lcen, flat, n_enc, alpha, rn, q_perp[i], ke, fv, slut, cand, wbuf,
+                        scores_out + i * ke, ids_out + i * ke);
+      free(slut);
+      free(cand);
+      free(wbuf);
+    }
     free(fv);
     size_t kk = 0;
-    rc = oc_scan_topk(luts, ls, lb, nq, flat, n_enc, G, alpha, NULL, ke, scores_out, ids_out, &kk);
+    if (!planes) rc = oc_scan_topk(luts, ls, lb, nq, flat, n_enc, G, alpha, NULL, ke, scores_out, ids_out, &kk);
     /* residual post-correction and re-rank (search.cpp:1123-1146) */
     oc_cand* rowc = (oc_cand*)malloc(ke * sizeof(oc_cand));
-    for (size_t i = 0; rc == 0 && i < nq; ++i) {
+    for (size_t i = 0; rc == 0 && !planes && i < nq; ++i) {
       for (size_t j = 0; j < ke; ++j) {
         const uint64_t vi = ids_out[i * ke + j];
         float s = scores_out[i * ke + j];

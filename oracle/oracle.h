@@ -71,6 +71,16 @@ int oc_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, int
                  const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
                  size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
                  uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds);
+/* as oc_lm_search on an AVX-512 VNNI host: the staged sign-plane branch
+ * (n >= 32768 prepared vectors, scan_bits 4); -2 when that branch is not taken */
+int oc_lm_search_planes(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
+                        const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
+                        size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
+                        uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds);
+size_t oc_planes_exact_len(size_t k);
+int oc_sign_topk(const float* rows, size_t nq, size_t S, const uint8_t* packed, size_t n, size_t k,
+                 float* scores_out, uint64_t* ids_out, size_t* k_out);
+void oc_set_staged_variant(int v);
 /* bucketed index (README.md:56: rows L2-normalized, residuals x_hat - c,
  * scores <q_hat, c> + alpha <q_hat, r_hat>) */
 int oc_bucket_residuals(const float* x, size_t dim, const uint64_t* pos_rows, const uint32_t* pos_bucket,

@@ -442,11 +442,12 @@ class Oracle:
         return out
 
     def lm_search(self, x, q, k, scan_bits=4, n_landmarks=1024, seed=0, rerank_bits=0, batches=None,
-                  prepare_between=False, float_corpus=None):
+                  prepare_between=False, float_corpus=None, planes=False):
         """LandmarkIndex(dim, scan_bits, n_landmarks, seed, rerank_bits): add the
         rows of x (in `batches` row counts), prepare, search. Returns scores,
         ids, info {scan_dim, codes_bytes, index_bytes, search_s (reference: wall time
-        of search())}."""
+        of search())}. planes=True: as on an AVX-512 VNNI host (the staged
+        sign-plane branch; the reference build runs with its VNNI kernels)."""
         x = np.ascontiguousarray(x, np.float32)
         q = np.ascontiguousarray(q, np.float32)
         n, dim = x.shape
@@ -458,14 +459,41 @@ class Oracle:
         info = np.zeros(3, np.uint64)
         secs = ctypes.c_double()
         fc = None if float_corpus is None else np.ascontiguousarray(float_corpus, np.float32)
-        self._check(self._f("lm_search")(_ptr(x), _ptr(br), _S(len(br)), ctypes.c_int(int(prepare_between)),
+        fn = "lm_search_planes" if planes else "lm_search"
+        self._check(self._f(fn)(_ptr(x), _ptr(br), _S(len(br)), ctypes.c_int(int(prepare_between)),
                                          _ptr(q), _S(nq), _S(dim), _S(scan_bits), _S(n_landmarks),
                                          ctypes.c_uint64(seed), _S(rerank_bits), _S(k), _ptr(fc), _ptr(scores),
-                                         _ptr(ids), ctypes.byref(ko), _ptr(info), ctypes.byref(secs)), "lm_search")
+                                         _ptr(ids), ctypes.byref(ko), _ptr(info), ctypes.byref(secs)), fn)
         ke = ko.value
         return (scores[: nq * ke].reshape(nq, ke), ids[: nq * ke].reshape(nq, ke),
                 {"scan_dim": int(info[0]), "codes_bytes": int(info[1]), "index_bytes": int(info[2]),
                  "search_s": secs.value})
+
+    def sign_topk(self, rows, packed, k):
+        """The sign-plane scan alone: rows (nq x S), bit-plane codes (n x 4 S/8)
+        -> top k (scores, ids) by (score desc, id asc)."""
+        rows = np.ascontiguousarray(rows, np.float32)
+        packed = np.ascontiguousarray(packed, np.uint8)
+        nq, S = rows.shape
+        n = packed.shape[0]
+        scores = np.zeros(max(1, nq * k), np.float32)
+        ids = np.zeros(max(1, nq * k), np.uint64)
+        ko = ctypes.c_size_t()
+        self._check(self._f("sign_topk")(_ptr(rows), _S(nq), _S(S), _ptr(packed), _S(n), _S(k), _ptr(scores),
+                                         _ptr(ids), ctypes.byref(ko)), "sign_topk")
+        ke = ko.value
+        return scores[: nq * ke].reshape(nq, ke), ids[: nq * ke].reshape(nq, ke)
+
+    def set_staged_variant(self, v: int):
+        """The oracle's fusion choices in the staged branch (test hook: 0 is the
+        reference's)."""
+        self.lib.oc_set_staged_variant(ctypes.c_int(v))
+
+    def planes_exact_len(self, k: int) -> int:
+        f = self.lib.oc_planes_exact_len
+        f.restype = ctypes.c_size_t
+        return int(f(_S(k)))
+
 
 def available(kind: str) -> bool:
     if kind == "port":

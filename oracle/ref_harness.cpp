@@ -38,9 +38,9 @@ thread_local std::string g_err;
 void force_avx2_path() { setenv("TURBOVEC_NO_VNNI", "1", 1); }
 
 template <typename F>
-int guarded(F&& f) {
+int guarded(F&& f, bool avx2 = true) {
   try {
-    force_avx2_path();
+    if (avx2) force_avx2_path();
     f();
     return 0;
   } catch (const std::exception& e) {
@@ -421,10 +421,10 @@ int vcref_project_q4(const float* q_rot, size_t nq, size_t dim, const float* L, 
 // float corpus.
 // info[0..2] = scan_dim, codes_bytes(), index_bytes(); search_seconds (may
 // be null) = wall time of the search() call.
-int vcref_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
-                    const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
-                    size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
-                    uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds) {
+static int lm_search_run(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
+                         const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
+                         size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
+                         uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds, bool avx2) {
   return guarded([&] {
     vc::LandmarkIndex ix(dim, scan_bits, m, seed, rerank_bits);
     size_t off = 0;
@@ -450,7 +450,53 @@ int vcref_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, 
       info[1] = ix.codes_bytes();
       info[2] = ix.index_bytes();
     }
-  });
+  }, avx2);
+}
+
+int vcref_lm_search(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
+                    const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
+                    size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
+                    uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds) {
+  return lm_search_run(x, batch_rows, n_batches, prepare_between, q, nq, dim, scan_bits, m, seed, rerank_bits, k,
+                       float_corpus, scores_out, ids_out, k_out, info, search_seconds, true);
+}
+
+// The sign-plane scan alone on an AVX-512 VNNI host: prepare_sign_from_rows
+// of the rows (nq x S) and score_prepared over planes_repack_sign of the
+// bit-plane codes (n x 4 S / 8), top k (src/search/search.cpp:916-971,
+// src/pack/pack.cpp:490-541, the VNNI sign kernels).
+int vcref_sign_topk(const float* rows, size_t nq, size_t S, const uint8_t* packed, size_t n, size_t k,
+                    float* scores_out, uint64_t* ids_out, size_t* k_out) {
+  unsetenv("TURBOVEC_NO_VNNI");
+  const int rc = guarded([&] {
+    auto sign = vc::planes_repack_sign(std::span<const uint8_t>(packed, n * 4 * S / 8), n, 4, S);
+    auto prep = vc::prepare_sign_from_rows(std::span<const float>(rows, nq * S), nq, S);
+    std::vector<float> ones(n, 1.f);
+    auto res = vc::score_prepared(prep, k, sign.first, sign.second, std::span<const float>(ones));
+    *k_out = res.k;
+    if (res.k > 0 && nq > 0) {
+      std::memcpy(scores_out, res.scores.data(), res.nq * res.k * sizeof(float));
+      std::memcpy(ids_out, res.ids.data(), res.nq * res.k * sizeof(uint64_t));
+    }
+  }, false);
+  force_avx2_path();
+  return rc;
+}
+
+// vcref_lm_search as the reference runs on an AVX-512 VNNI host: the call
+// sees the environment without TURBOVEC_NO_VNNI, so prepare() keeps the sign
+// planes (n >= 32768) and search() takes the staged branch
+// (src/search/search.cpp:1379-1510); every other entry point keeps forcing
+// the AVX2 path.
+int vcref_lm_search_planes(const float* x, const size_t* batch_rows, size_t n_batches, int prepare_between,
+                           const float* q, size_t nq, size_t dim, size_t scan_bits, size_t m, uint64_t seed,
+                           size_t rerank_bits, size_t k, const float* float_corpus, float* scores_out,
+                           uint64_t* ids_out, size_t* k_out, size_t* info, double* search_seconds) {
+  unsetenv("TURBOVEC_NO_VNNI");
+  const int rc = lm_search_run(x, batch_rows, n_batches, prepare_between, q, nq, dim, scan_bits, m, seed,
+                               rerank_bits, k, float_corpus, scores_out, ids_out, k_out, info, search_seconds, false);
+  force_avx2_path();
+  return rc;
 }
 
 }  // extern "C"

@@ -84,6 +84,22 @@ LANDMARK_CASES = (
 )
 
 
+# The staged sign-plane branch (the reference on an AVX-512 VNNI host):
+# (tag, n, nq, dim, n_landmarks, seed, k); n >= 32768 so the index keeps sign
+# planes, and n_landmarks % 32 == 0, the scan widths the reference's sign
+# kernels cover.
+STAGED_CASES = (
+    ("s128k10", 40000, 6, 128, 64, 3, 10),
+    ("s128k50", 40000, 6, 128, 64, 3, 50),
+    ("s192k100", 36000, 5, 256, 192, 8, 100),
+    ("s64k300", 33000, 4, 64, 64, 1, 300),
+)
+
+# Scan widths that are not a multiple of 32 (the reference's sign scan reads
+# past its sign codes there): the planes index must give the exact scan's result.
+UNSTAGED_PLANES_CASE = ("s200k100", 36000, 5, 256, 200, 8, 100)
+
+
 def landmark_inputs(n, nq, dim):
     """Seeded vectors (row scales varied) and queries with the structured rows
     the reference's quantizers tie on: a zero row and rows whose rotation has

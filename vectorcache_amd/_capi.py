@@ -201,6 +201,8 @@ _SIGS = {
     "vc_lm_index_scan_dim": ([P], S),
     "vc_lm_index_is_prepared": ([P], I),
     "vc_lm_index_codes_bytes": ([P], S),
+    "vc_lm_index_set_planes": ([P, I], I),
+    "vc_lm_index_planes_active": ([P], I),
     "vc_lm_index_index_bytes": ([P], S),
     "vc_bucket_params_default": ([ctypes.POINTER(BucketParams)], None),
     "vc_bucketed_create": ([S, S, ctypes.POINTER(BucketParams), ctypes.POINTER(P)], I),

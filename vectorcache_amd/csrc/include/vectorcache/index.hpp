@@ -56,6 +56,10 @@ class LandmarkIndex {
     detail::throw_on(vc_lm_index_add(h_, vectors.data(), vectors.size() / dim()));
   }
   void prepare() { detail::throw_on(vc_lm_index_prepare(h_)); }
+  /// The reference's behaviour on an AVX-512 VNNI host: sign planes from
+  /// 32768 prepared vectors and the staged sign-plane search.
+  void set_planes(bool on) { detail::throw_on(vc_lm_index_set_planes(h_, on ? 1 : 0)); }
+  bool planes_active() const { return vc_lm_index_planes_active(h_) != 0; }
 
   SearchResults search(std::span<const float> queries, std::size_t k,
                        std::span<const float> float_corpus = {}) const {

@@ -1268,6 +1268,12 @@ struct vc_lm_index {
   size_t n = 0, n_encoded = 0;
   bool prepared = false;
   DevBuf work, qwork, scratch, io;
+  // staged sign-plane search (the reference's AVX-512 VNNI host behaviour):
+  // sign bytes of every encoded vector in the tiled layout, and 1.0 per vector
+  // (the sign scan's scores carry no alpha)
+  int planes = 0;
+  bool has_sign = false;
+  DevBuf sign_tiled, ones, stage;
   std::mutex mu;
 };
 
@@ -1337,6 +1343,69 @@ void lm_query_rows(vc_lm_index* ix, const float* q_dev, size_t nq, float** q_lan
             "lm_query");
 }
 
+// Sign planes kept at prepare on a VNNI host (src/pack/pack.cpp:428-449): 4-bit
+// scan codes and at least 32768 prepared vectors (VECTORCACHE_PLANES_MIN's
+// default). The staged branch also needs scan_dim % 32 == 0: below that the
+// reference's sign scan has no kernel for its 32-wide sign blocks and falls
+// through to the 192-wide 4-bit kernel (src/search/search_vnni.cpp:1466-1469,
+// 803-843), which reads past the sign codes, so no defined result exists; such
+// an index keeps the exact scan, the AVX2 host's result.
+constexpr size_t kPlanesMinVectors = 32768;
+
+void lm_build_sign(vc_lm_index* ix) {
+  ix->has_sign = false;
+  const size_t n = ix->n_encoded, S = ix->scan_dim, Gs = S / 8;
+  if (ix->planes == 0 || ix->scan_bits != 4 || n < kPlanesMinVectors || S % 32 != 0) {
+    ix->sign_tiled.release();
+    ix->ones.release();
+    return;
+  }
+  const int g16s = int((Gs + 15) / 16);
+  const size_t bytes = vchip::tiled_bytes(n, g16s);
+  ix->sign_tiled.reserve(bytes, ix->stream);
+  ix->ones.reserve(n * 4, ix->stream);
+  hip_check(hipMemsetAsync(ix->sign_tiled.p, 0, bytes, ix->stream), "memset");
+  hip_check(vchip::launch_lm_sign_codes(ix->tiled.as<uint8_t>(), n, ix->t_scan.groups16, int(Gs), g16s,
+                                        ix->sign_tiled.as<uint8_t>(), ix->stream),
+            "sign codes");
+  hip_check(vchip::launch_fill_value(ix->ones.as<float>(), n, 1.f, ix->stream), "fill");
+  hip_check(hipStreamSynchronize(ix->stream), "sync");
+  ix->has_sign = true;
+}
+
+// The reference's shortlist lengths (src/pack/pack.cpp:451-463) for 4-bit scans.
+size_t planes_shortlist_len4(size_t k) { return std::max<size_t>(256, k >= 64 ? k * 16 : k * 20); }
+size_t planes_exact_len4(size_t k) { return std::min(planes_shortlist_len4(k), std::max<size_t>(256, 6 * k)); }
+
+// The staged branch of search_landmark_rerank (src/search/search.cpp:
+// 1379-1510): the E best sign-plane scores (exact, ties to the lower
+// position), then their 4-bit permute-dot rescore with the residual term into
+// a k-heap in shortlist order.
+void lm_search_staged(vc_lm_index* ix, const float* q_land, const float* q_perp, size_t nq, size_t ke, float* s_dev,
+                      uint64_t* i_dev, hipStream_t s) {
+  const size_t S = ix->scan_dim, Gs = S / 8, n = ix->n_encoded, E = planes_exact_len4(ke);
+  if (Gs > 256) fail(VC_ERR_INVALID_ARGUMENT, "staged sign search supports at most 2048 landmarks");
+  if (E > size_t(vchip::kMaxTopK) || vchip::staged_lds_bytes(int(S), int(E), int(ke)) > size_t(160 * 1024))
+    fail(VC_ERR_INVALID_ARGUMENT, "k too large for the staged sign search (shortlist above 2048)");
+  vchip::CodeTables t_sign = ix->t_scan;
+  t_sign.groups = int(Gs);
+  t_sign.groups16 = int((Gs + 15) / 16);
+  Carve c;
+  const size_t ol = c.take(nq * Gs * 32), ols = c.take(nq * 4), obt = c.take(nq * 4), oss = c.take(nq * E * 4),
+               osi = c.take(nq * E * 8);
+  ix->stage.reserve(c.off, s);
+  hip_check(vchip::launch_sign_luts(q_land, nq, int(S), ix->stage.as<uint8_t>(ol), ix->stage.as<float>(ols),
+                                    ix->stage.as<float>(obt), s),
+            "sign luts");
+  const PreLuts pre{ix->stage.as<uint8_t>(ol), ix->stage.as<float>(ols), ix->stage.as<float>(obt)};
+  search_tiled(t_sign, ix->sign_tiled.as<uint8_t>(), ix->ones.as<float>(), n, nullptr, nq, E, nullptr,
+               ix->stage.as<float>(oss), ix->stage.as<uint64_t>(osi), ix->scratch, s, nullptr, nullptr, nullptr, &pre);
+  hip_check(vchip::launch_lm_staged(q_land, nq, int(S), q_perp, ix->t_scan.centroids, ix->tiled.as<uint8_t>(),
+                                    ix->t_scan.groups16, ix->alpha.as<float>(), ix->rn.as<float>(),
+                                    ix->stage.as<uint64_t>(osi), int(E), int(ke), s_dev, i_dev, s),
+            "staged rescore");
+}
+
 size_t lm_search_dev(vc_lm_index* ix, const float* q_dev, size_t nq, size_t k, const float* corpus_dev,
                      float* s_dev, uint64_t* i_dev, hipStream_t s) {
   if (!ix->prepared) fail(VC_ERR_STATE, "call prepare() before search");
@@ -1345,11 +1414,15 @@ size_t lm_search_dev(vc_lm_index* ix, const float* q_dev, size_t nq, size_t k, c
   if (ke > size_t(vchip::kMaxTopK)) fail(VC_ERR_INVALID_ARGUMENT, "k exceeds the GPU top-k capacity of 2048");
   float *q_land = nullptr, *q_perp = nullptr;
   lm_query_rows(ix, q_dev, nq, &q_land, &q_perp, s);
-  search_tiled(ix->t_scan, ix->tiled.as<uint8_t>(), ix->alpha.as<float>(), ix->n_encoded, q_land, nq, ke, nullptr,
-               s_dev, i_dev, ix->scratch, s);
-  hip_check(vchip::launch_residual_post(nq, int(ke), s_dev, i_dev, ix->rn.as<float>(), q_perp, ix->n_encoded,
-                                        nullptr, s),
-            "residual_post");
+  if (ix->has_sign && planes_shortlist_len4(ke) < ix->n_encoded) {
+    lm_search_staged(ix, q_land, q_perp, nq, ke, s_dev, i_dev, s);
+  } else {
+    search_tiled(ix->t_scan, ix->tiled.as<uint8_t>(), ix->alpha.as<float>(), ix->n_encoded, q_land, nq, ke, nullptr,
+                 s_dev, i_dev, ix->scratch, s);
+    hip_check(vchip::launch_residual_post(nq, int(ke), s_dev, i_dev, ix->rn.as<float>(), q_perp, ix->n_encoded,
+                                          nullptr, s),
+              "residual_post");
+  }
   if (corpus_dev != nullptr)
     hip_check(vchip::launch_float_rescore(nq, int(ke), s_dev, i_dev, nullptr, q_dev, corpus_dev, ix->n, int(ix->dim), s),
               "float_rescore");
@@ -1467,6 +1540,7 @@ vc_status vc_lm_index_prepare(vc_lm_index* ix) {
       ix->rows.release();
       ix->rows_live = false;
       ix->n_encoded = n;
+      lm_build_sign(ix);
     }
     ix->prepared = true;
   });
@@ -1555,6 +1629,15 @@ size_t vc_lm_index_scan_bits(const vc_lm_index* ix) { return ix ? ix->scan_bits 
 size_t vc_lm_index_rerank_bits(const vc_lm_index* ix) { return ix ? ix->rerank_bits : 0; }
 size_t vc_lm_index_n_landmarks(const vc_lm_index* ix) { return ix ? ix->m : 0; }
 size_t vc_lm_index_scan_dim(const vc_lm_index* ix) { return ix ? ix->scan_dim : 0; }
+vc_status vc_lm_index_set_planes(vc_lm_index* ix, int on) {
+  return guarded([&] {
+    if (ix == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null index");
+    std::lock_guard lock(ix->mu);
+    ix->planes = on ? 1 : 0;
+    if (ix->prepared) lm_build_sign(ix);
+  });
+}
+int vc_lm_index_planes_active(const vc_lm_index* ix) { return ix && ix->has_sign ? 1 : 0; }
 int vc_lm_index_is_prepared(const vc_lm_index* ix) { return ix && ix->prepared ? 1 : 0; }
 size_t vc_lm_index_codes_bytes(const vc_lm_index* ix) {
   return ix ? (ix->n_encoded + 191) / 192 * 192 * size_t(ix->t_scan.groups) : 0;

@@ -153,6 +153,8 @@ hipError_t launch_multi_scan(int vec_per_thread, const MultiColl* colls, const M
                              std::size_t dyn_lds, int grid, hipStream_t s);
 // Deterministic pseudo-random rows (autotuning inputs): n x dim floats.
 hipError_t launch_fill_rows(float* x, std::size_t n, int dim, std::uint64_t seed, hipStream_t s);
+// x[0 .. n) = v
+hipError_t launch_fill_value(float* x, std::size_t n, float v, hipStream_t s);
 
 // Bytes of the per-query-tile LUT rows (interleaved / pair source) the scan needs.
 std::size_t scan_lut_bytes(const ScanPlan& p, int groups16, std::size_t nq);
@@ -215,6 +217,19 @@ hipError_t launch_lm_yhat(const CodeTables& t, float* rows, std::size_t n, float
 hipError_t launch_lm_project(const float* Y, std::size_t n, int dim, const float* L, int m, float* out,
                              int stride, hipStream_t s);
 // rn[r] = sqrt(max(0, vnorm2[r] - |rows_r[0:m)|^2)), rows with the given stride.
+// Staged sign-plane search of the landmark index: sign bytes (top bit of 8
+// codes) from the 4-bit tiled codes into a tiled layout of groups_s bytes;
+// build_sign_lut LUTs of the query rows (S dims, S / 8 groups); the shortlist
+// rescore (E positions per query in shortlist order) into sorted k rows.
+hipError_t launch_lm_sign_codes(const std::uint8_t* tiled4, std::size_t n, int groups16_4, int groups_s,
+                                int groups16_s, std::uint8_t* tiled_s, hipStream_t s);
+hipError_t launch_sign_luts(const float* q_land, std::size_t nq, int S, std::uint8_t* luts, float* lut_scale,
+                            float* bias_total, hipStream_t s);
+std::size_t staged_lds_bytes(int S, int E, int k);
+hipError_t launch_lm_staged(const float* q_land, std::size_t nq, int S, const float* q_perp, const float* centroids16,
+                            const std::uint8_t* tiled4, int groups16_4, const float* alpha, const float* rn,
+                            const std::uint64_t* cand, int E, int k, float* out_s, std::uint64_t* out_i,
+                            hipStream_t s);
 hipError_t launch_lm_resid(const float* rows, std::size_t n, int m, int stride, const float* vnorm2, float* rn,
                            hipStream_t s);
 // Query projection onto the 4-bit basis: q_land nq x stride (zero past m) and

@@ -2965,6 +2965,10 @@ __global__ __launch_bounds__(256) void merge_results_kernel(long nq, int kd, flo
 // ---------------------------------------------------------------------------
 // Pseudo-random rows for autotuning inputs: element i = (sum of two
 // splitmix64 uniforms in [0, 1)) - 1, a triangular value in (-1, 1).
+__global__ __launch_bounds__(256) void fill_value_kernel(float* __restrict__ x, long n, float v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = v;
+}
+
 __global__ __launch_bounds__(256) void fill_rows_kernel(float* __restrict__ x, long total, uint64_t seed) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
     uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
@@ -3282,6 +3286,260 @@ __global__ __launch_bounds__(256) void lm_project_kernel(const float* __restrict
 // summed in order; of the tail, a block of 4 rounded squares added in order
 // when at least 4 remain, then fused multiply-adds), rn = sqrtf(max(0,
 // vnorm2 - that)).
+// ---------------------------------------------------------------------------
+// Staged sign-plane search of the landmark index (the reference's branch on an
+// AVX-512 VNNI host, src/search/search.cpp:1379-1510).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long tiled_at(long v, int g, int G16) {
+  return ((v >> 6) * G16 + (g >> 4)) * 1024 + (v & 63) * 16 + (g & 15);
+}
+
+// Sign bytes from the 4-bit landmark codes (both in the tiled layout): byte gs
+// of vector v holds the top bit of codes 8 gs .. 8 gs + 7, first code in bit 7
+// (the packed codes' top bit-plane, src/pack/pack.cpp:490-507). One thread
+// per (vector, sign byte).
+__global__ void lm_sign_codes_kernel(const uint8_t* __restrict__ tiled4, long n, int G16_4, int Gs, int G16_s,
+                                     uint8_t* __restrict__ tiled_s) {
+  const long total = n * (long)Gs;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long v = i / Gs;
+    const int gs = (int)(i - v * Gs);
+    unsigned sb = 0;
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const unsigned b = tiled4[tiled_at(v, 4 * gs + h, G16_4)];
+      sb = (sb << 2) | (((b >> 7) & 1u) << 1) | ((b >> 3) & 1u);
+    }
+    tiled_s[tiled_at(v, gs, G16_s)] = (uint8_t)sb;
+  }
+}
+
+// build_sign_lut (src/search/search.cpp:862-914) of one query row of S dims
+// per workgroup: per byte group and half, the 16 sums of +-q over 4 dims,
+// their minimum, the bias (the minima summed in order), scale = largest span
+// / 127, then u8 = clamp(round((v - min) / scale ... ), 0, 127).
+constexpr int kSignLutThreads = 256;
+__global__ __launch_bounds__(kSignLutThreads) void sign_lut_kernel(const float* __restrict__ q_land, int S,
+                                                                  uint8_t* __restrict__ luts,
+                                                                  float* __restrict__ lut_scale,
+                                                                  float* __restrict__ bias_total) {
+  extern __shared__ float sl[];  // fv [Gs * 32] | mins [Gs * 2] | span partials [threads]
+  const int Gs = S / 8, tid = threadIdx.x;
+  const float* q = q_land + (long)blockIdx.x * S;
+  float* fv = sl;
+  float* mins = fv + Gs * 32;
+  float* red = mins + Gs * 2;
+  float span = 0.f;
+  for (int h = tid; h < Gs * 2; h += blockDim.x) {
+    const int d = h * 4;  // (group, half) = (h / 2, h % 2): dims 8 g + 4 half ..
+    const float p0 = q[d], p1 = q[d + 1], p2 = q[d + 2], p3 = q[d + 3];
+    const float a[4] = {-p0 - p1, p1 - p0, p0 - p1, p0 + p1};
+    const float b[4] = {-p2 - p3, p3 - p2, p2 - p3, p2 + p3};
+#pragma unroll
+    for (int pat = 0; pat < 16; ++pat) fv[h * 16 + pat] = a[pat >> 2] + b[pat & 3];
+    const float m01 = fabsf(p0) + fabsf(p1), m23 = fabsf(p2) + fabsf(p3);
+    const float mn = -m01 - m23, mx = m01 + m23;
+    mins[h] = mn;
+    span = fmaxf(span, mx - mn);
+  }
+  red[tid] = span;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (tid < o) red[tid] = fmaxf(red[tid], red[tid + o]);
+    __syncthreads();
+  }
+  float scale = red[0] > 0.f ? div_rn(red[0], 127.f) : 1.f;
+  float inv = 1.f;
+  if (scale >= FLT_MIN)
+    inv = div_rn(1.f, scale);
+  else
+    scale = 1.f;
+  if (tid == 0) {
+    float bias = 0.f;
+    for (int h = 0; h < Gs * 2; ++h) bias = bias + mins[h];
+    lut_scale[blockIdx.x] = scale;
+    bias_total[blockIdx.x] = bias;
+  }
+  uint8_t* out = luts + (long)blockIdx.x * Gs * 32;
+  for (int j = tid; j < Gs * 32; j += blockDim.x) {
+    float v = roundf((fv[j] - mins[j >> 4]) * inv);
+    v = fminf(fmaxf(v, 0.f), 127.f);
+    out[j] = (uint8_t)v;
+  }
+}
+
+// Stage 2: the E shortlisted vectors of one query per workgroup, rescored in
+// shortlist order with build_permute_dot's int8 weights and levels
+// (src/search/search_vnni.cpp:1218-1360): acc = sum over byte groups of
+// lev[lo] * w_lo + lev[hi] * w_hi (exact), f = (float) acc * (cs * qs),
+// score = fmaf(f, alpha, q_perp * rn) (the compiled flush, :1194-1217); each
+// score goes into a k-heap in shortlist order (topk_heap.hpp:101-157: strict
+// admission, rescan below 128, binary heap from 128), then the heap is
+// written sorted by (score desc, id asc).
+constexpr int kStagedThreads = 256;
+struct StagedArgs {
+  const float* q_land;
+  int S;
+  const float* q_perp;
+  float cen[16];
+  const uint8_t* tiled4;
+  int G16;
+  const float* alpha;
+  const float* rn;
+  const uint64_t* cand;  // [nq][E] positions, shortlist order
+  int E, k;
+  float* out_s;
+  uint64_t* out_i;
+};
+
+__device__ __forceinline__ bool staged_worse(float sa, uint64_t ia, float sb, uint64_t ib) {
+  return sa != sb ? sa < sb : ia > ib;
+}
+
+__global__ __launch_bounds__(kStagedThreads) void lm_staged_kernel(StagedArgs a) {
+  extern __shared__ __align__(16) unsigned char sm[];
+  const int S = a.S, G = S / 2, E = a.E, k = a.k, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long qi = blockIdx.x;
+  float* sc = reinterpret_cast<float*>(sm);                     // [E] rescored
+  float* hs = sc + ((E + 1) & ~1);                               // [k] heap scores
+  uint64_t* hid = reinterpret_cast<uint64_t*>(hs + ((k + 1) & ~1));  // [k] heap ids
+  int8_t* wlo = reinterpret_cast<int8_t*>(hid + k);              // [G]
+  int8_t* whi = wlo + G;                                         // [G]
+  __shared__ float red[kStagedThreads];
+  __shared__ int8_t lev[16];
+  __shared__ int hsz_sh;
+  const float* q = a.q_land + qi * S;
+  // query weights: qs = max |q| / 127
+  float qm = 0.f;
+  for (int d = tid; d < S; d += blockDim.x) qm = fmaxf(qm, fabsf(q[d]));
+  red[tid] = qm;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (tid < o) red[tid] = fmaxf(red[tid], red[tid + o]);
+    __syncthreads();
+  }
+  float qs = div_rn(red[0], 127.f), inv = 1.f;
+  if (qs >= FLT_MIN)
+    inv = div_rn(1.f, qs);
+  else
+    qs = 1.f;
+  float cmax = 0.f;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) cmax = fmaxf(cmax, fabsf(a.cen[i]));
+  const float cs = cmax > 0.f ? div_rn(cmax, 127.f) : 1.f;
+  if (tid < 16) lev[tid] = (int8_t)fminf(fmaxf(roundf(div_rn(a.cen[tid], cs)), -127.f), 127.f);
+  for (int g = tid; g < G; g += blockDim.x) {
+    wlo[g] = (int8_t)fminf(fmaxf(roundf(q[2 * g + 1] * inv), -127.f), 127.f);
+    whi[g] = (int8_t)fminf(fmaxf(roundf(q[2 * g] * inv), -127.f), 127.f);
+  }
+  __syncthreads();
+  const float pscale = cs * qs, qp = a.q_perp[qi];
+  const uint64_t* cand = a.cand + qi * E;
+  const int waves = blockDim.x >> 6;
+  for (int c = wave; c < E; c += waves) {
+    const uint64_t id = cand[c];
+    int acc = 0;
+    for (int g = lane; g < G; g += 64) {
+      const unsigned b = a.tiled4[tiled_at((long)id, g, a.G16)];
+      acc += (int)lev[b & 15u] * (int)wlo[g] + (int)lev[b >> 4] * (int)whi[g];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) {
+      const float f = (float)acc * pscale;
+      sc[c] = __fmaf_rn(f, a.alpha[id], qp * a.rn[id]);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int hsz = 0, hmi = 0;
+    float hmin = -INFINITY;
+    for (int c = 0; c < E; ++c) {
+      const float s = sc[c];
+      if (!(s > hmin)) continue;  // strict admission; -inf while the heap fills
+      const uint64_t id = cand[c];
+      if (k < 128) {
+        if (hsz < k) {
+          hs[hsz] = s;
+          hid[hsz] = id;
+          ++hsz;
+          if (hsz < k) continue;
+        } else {
+          hs[hmi] = s;
+          hid[hmi] = id;
+        }
+        int mi = 0;  // rescan for the worst entry
+        for (int h = 1; h < k; ++h)
+          if (staged_worse(hs[h], hid[h], hs[mi], hid[mi])) mi = h;
+        hmin = hs[mi];
+        hmi = mi;
+      } else {
+        int i;
+        if (hsz < k) {
+          hs[hsz] = s;
+          hid[hsz] = id;
+          ++hsz;
+          if (hsz < k) continue;
+          i = k / 2;  // heapify, then the root is the worst
+          while (i-- > 0) {
+            int j = i;
+            for (;;) {
+              const int l = 2 * j + 1, r = 2 * j + 2;
+              int w = j;
+              if (l < k && staged_worse(hs[l], hid[l], hs[w], hid[w])) w = l;
+              if (r < k && staged_worse(hs[r], hid[r], hs[w], hid[w])) w = r;
+              if (w == j) break;
+              const float ts = hs[j];
+              hs[j] = hs[w];
+              hs[w] = ts;
+              const uint64_t ti = hid[j];
+              hid[j] = hid[w];
+              hid[w] = ti;
+              j = w;
+            }
+          }
+        } else {
+          hs[0] = s;
+          hid[0] = id;
+          int j = 0;
+          for (;;) {
+            const int l = 2 * j + 1, r = 2 * j + 2;
+            int w = j;
+            if (l < k && staged_worse(hs[l], hid[l], hs[w], hid[w])) w = l;
+            if (r < k && staged_worse(hs[r], hid[r], hs[w], hid[w])) w = r;
+            if (w == j) break;
+            const float ts = hs[j];
+            hs[j] = hs[w];
+            hs[w] = ts;
+            const uint64_t ti = hid[j];
+            hid[j] = hid[w];
+            hid[w] = ti;
+            j = w;
+          }
+        }
+        hmin = hs[0];
+        hmi = 0;
+      }
+    }
+    hsz_sh = hsz;
+  }
+  __syncthreads();
+  const int hsz = hsz_sh;
+  float* os = a.out_s + qi * k;
+  uint64_t* oi = a.out_i + qi * k;
+  for (int i = tid; i < k; i += blockDim.x) {
+    if (i >= hsz) {
+      os[i] = 0.f;
+      oi[i] = 0;
+      continue;
+    }
+    int rank = 0;  // entries better than entry i (ids are distinct)
+    for (int j = 0; j < hsz; ++j) rank += staged_worse(hs[i], hid[i], hs[j], hid[j]) ? 1 : 0;
+    os[rank] = hs[i];
+    oi[rank] = hid[i];
+  }
+}
+
 __global__ __launch_bounds__(256) void lm_resid_kernel(const float* __restrict__ rows, long n, int m,
                                                        int stride, const float* __restrict__ vnorm2,
                                                        float* __restrict__ rn) {
@@ -3965,6 +4223,12 @@ hipError_t launch_multi_scan(int vec_per_thread, const MultiColl* colls, const M
   return hipGetLastError();
 }
 
+hipError_t launch_fill_value(float* x, std::size_t n, float v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_value_kernel, dim3(grid_for((long)n, 256, 65536)), dim3(256), 0, s, x, (long)n, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_rows(float* x, std::size_t n, int dim, std::uint64_t seed, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(fill_rows_kernel, dim3(grid_for((long)(n * dim), 256, 65536)), dim3(256), 0, s, x,
@@ -4170,6 +4434,56 @@ hipError_t launch_lm_project(const float* Y, std::size_t n, int dim, const float
   if (dim % 8 != 0 || stride < m) return hipErrorInvalidValue;
   hipLaunchKernelGGL(lm_project_kernel, dim3((unsigned)((n + 31) / 32), (unsigned)((stride + 31) / 32)), dim3(256),
                      0, s, Y, (long)n, dim, L, m, out, stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_lm_sign_codes(const std::uint8_t* tiled4, std::size_t n, int groups16_4, int groups_s,
+                                int groups16_s, std::uint8_t* tiled_s, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(lm_sign_codes_kernel, dim3(grid_for((long)n * groups_s, 256)), dim3(256), 0, s, tiled4, (long)n,
+                     groups16_4, groups_s, groups16_s, tiled_s);
+  return hipGetLastError();
+}
+
+hipError_t launch_sign_luts(const float* q_land, std::size_t nq, int S, std::uint8_t* luts, float* lut_scale,
+                            float* bias_total, hipStream_t s) {
+  if (nq == 0) return hipSuccess;
+  const int Gs = S / 8;
+  const size_t lds = ((size_t)Gs * 34 + kSignLutThreads) * 4;
+  if (lds > (size_t)kLdsBudget) return hipErrorInvalidValue;
+  (void)hipFuncSetAttribute((const void*)sign_lut_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(sign_lut_kernel, dim3((unsigned)nq), dim3(kSignLutThreads), lds, s, q_land, S, luts, lut_scale,
+                     bias_total);
+  return hipGetLastError();
+}
+
+std::size_t staged_lds_bytes(int S, int E, int k) {
+  return ((size_t)((E + 1) & ~1) + (size_t)((k + 1) & ~1)) * 4 + (size_t)k * 8 + (size_t)S;
+}
+
+hipError_t launch_lm_staged(const float* q_land, std::size_t nq, int S, const float* q_perp, const float* centroids16,
+                            const std::uint8_t* tiled4, int groups16_4, const float* alpha, const float* rn,
+                            const std::uint64_t* cand, int E, int k, float* out_s, std::uint64_t* out_i,
+                            hipStream_t s) {
+  if (nq == 0 || k == 0) return hipSuccess;
+  const size_t lds = staged_lds_bytes(S, E, k);
+  if (lds > (size_t)kLdsBudget || E < k) return hipErrorInvalidValue;
+  StagedArgs a{};
+  a.q_land = q_land;
+  a.S = S;
+  a.q_perp = q_perp;
+  for (int i = 0; i < 16; ++i) a.cen[i] = centroids16[i];
+  a.tiled4 = tiled4;
+  a.G16 = groups16_4;
+  a.alpha = alpha;
+  a.rn = rn;
+  a.cand = cand;
+  a.E = E;
+  a.k = k;
+  a.out_s = out_s;
+  a.out_i = out_i;
+  (void)hipFuncSetAttribute((const void*)lm_staged_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(lm_staged_kernel, dim3((unsigned)nq), dim3(kStagedThreads), lds, s, a);
   return hipGetLastError();
 }
 

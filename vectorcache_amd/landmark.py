@@ -22,11 +22,22 @@ DEFAULT_LANDMARKS = 1024  # reference kDefaultLandmarks (include/vectorcache/con
 
 class LandmarkIndex:
     def __init__(self, dim: int, scan_bits: int, n_landmarks: int = DEFAULT_LANDMARKS, landmark_seed: int = 0,
-                 rerank_bits: int = 0):
+                 rerank_bits: int = 0, planes: bool = False):
+        """planes=True: the reference's behaviour on an AVX-512 VNNI host (sign
+        planes from 32768 prepared vectors, the staged sign-plane search)."""
         h = P()
         check(lib.vc_lm_index_create(S(dim), S(scan_bits), S(n_landmarks), ctypes.c_uint64(landmark_seed),
                                      S(rerank_bits), ctypes.byref(h)))
         self._h = h
+        if planes:
+            self.set_planes(True)
+
+    def set_planes(self, on: bool):
+        check(lib.vc_lm_index_set_planes(self._h, ctypes.c_int(int(on))))
+
+    def planes_active(self) -> bool:
+        """Whether the index holds sign planes (searches may take the staged branch)."""
+        return bool(lib.vc_lm_index_planes_active(self._h))
 
     def close(self):
         if getattr(self, "_h", None):
