@@ -1556,20 +1556,50 @@ vc_status vc_lm_index_search(vc_lm_index* ix, const float* queries, size_t nq, s
     if (k_out) *k_out = ke;
     if (nq == 0 || ke == 0) return;
     if (scores == nullptr || ids == nullptr) fail(VC_ERR_INVALID_ARGUMENT, "null output");
+    const size_t D = ix->dim, m = nq * ke;
     Carve c;
-    const size_t oq = c.take(nq * ix->dim * 4), os = c.take(nq * ke * 4), oi = c.take(nq * ke * 8),
-                 ox = c.take(float_corpus ? ix->n * ix->dim * 4 : 0);
+    const size_t oq = c.take(nq * D * 4), os = c.take(m * 4), oi = c.take(m * 8),
+                 ox = c.take(float_corpus ? m * D * 4 : 0);
     ix->io.reserve(c.off, ix->stream);
-    hip_check(hipMemcpyAsync(ix->io.as<float>(oq), queries, nq * ix->dim * 4, hipMemcpyHostToDevice, ix->stream),
-              "copy");
-    if (float_corpus)
-      hip_check(hipMemcpyAsync(ix->io.as<float>(ox), float_corpus, ix->n * ix->dim * 4, hipMemcpyHostToDevice,
-                               ix->stream),
+    hip_check(hipMemcpyAsync(ix->io.as<float>(oq), queries, nq * D * 4, hipMemcpyHostToDevice, ix->stream), "copy");
+    lm_search_dev(ix, ix->io.as<float>(oq), nq, k, nullptr, ix->io.as<float>(os), ix->io.as<uint64_t>(oi),
+                  ix->stream);
+    if (float_corpus) {
+      // Re-rank through the host corpus: only the nq x k candidate rows go up.
+      // Row r of the upload holds the candidate of local id r; within a query
+      // the local ids follow the global ids, so the rescore's tie order (id
+      // asc) is the global one, and the ids come back through `global`.
+      std::vector<uint64_t> got(m), global(m), local(m);
+      hip_check(hipMemcpyAsync(got.data(), ix->io.as<uint64_t>(oi), m * 8, hipMemcpyDeviceToHost, ix->stream), "copy");
+      hip_check(hipStreamSynchronize(ix->stream), "sync");
+      std::vector<size_t> order(ke);
+      for (size_t i = 0; i < nq; ++i) {
+        const uint64_t* g = got.data() + i * ke;
+        for (size_t j = 0; j < ke; ++j) order[j] = j;
+        std::sort(order.begin(), order.end(), [g](size_t a, size_t b) { return g[a] < g[b]; });
+        for (size_t r = 0; r < ke; ++r) {
+          global[i * ke + r] = g[order[r]];
+          local[i * ke + order[r]] = i * ke + r;
+        }
+      }
+      std::vector<float> rows(m * D, 0.f);
+      for (size_t r = 0; r < m; ++r)
+        if (global[r] < ix->n) std::memcpy(rows.data() + r * D, float_corpus + global[r] * D, D * 4);
+      hip_check(hipMemcpyAsync(ix->io.as<float>(ox), rows.data(), m * D * 4, hipMemcpyHostToDevice, ix->stream),
                 "copy");
-    lm_search_dev(ix, ix->io.as<float>(oq), nq, k, float_corpus ? ix->io.as<float>(ox) : nullptr,
-                  ix->io.as<float>(os), ix->io.as<uint64_t>(oi), ix->stream);
-    hip_check(hipMemcpyAsync(scores, ix->io.as<float>(os), nq * ke * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
-    hip_check(hipMemcpyAsync(ids, ix->io.as<uint64_t>(oi), nq * ke * 8, hipMemcpyDeviceToHost, ix->stream), "copy");
+      hip_check(hipMemcpyAsync(ix->io.as<uint64_t>(oi), local.data(), m * 8, hipMemcpyHostToDevice, ix->stream),
+                "copy");
+      hip_check(vchip::launch_float_rescore(nq, int(ke), ix->io.as<float>(os), ix->io.as<uint64_t>(oi), nullptr,
+                                            ix->io.as<float>(oq), ix->io.as<float>(ox), m, int(D), ix->stream),
+                "float_rescore");
+      hip_check(hipMemcpyAsync(scores, ix->io.as<float>(os), m * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+      hip_check(hipMemcpyAsync(ids, ix->io.as<uint64_t>(oi), m * 8, hipMemcpyDeviceToHost, ix->stream), "copy");
+      hip_check(hipStreamSynchronize(ix->stream), "sync");
+      for (size_t r = 0; r < m; ++r) ids[r] = ids[r] < m ? global[ids[r]] : ids[r];
+      return;
+    }
+    hip_check(hipMemcpyAsync(scores, ix->io.as<float>(os), m * 4, hipMemcpyDeviceToHost, ix->stream), "copy");
+    hip_check(hipMemcpyAsync(ids, ix->io.as<uint64_t>(oi), m * 8, hipMemcpyDeviceToHost, ix->stream), "copy");
     hip_check(hipStreamSynchronize(ix->stream), "sync");
   });
 }

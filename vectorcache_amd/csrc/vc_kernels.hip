@@ -1599,6 +1599,19 @@ __device__ __forceinline__ void wave_sort_desc(float& s, uint32_t& id, int lane)
   }
 }
 
+// The kw-th largest u among the lanes with `cand` set (at least kw of them),
+// bit by bit from the top with ballots. Not inlined: the scan kernels reach it
+// only on a workgroup's first steps, and inlined into the admission of a
+// two-vector tile it cost the scan half its occupancy.
+__device__ __attribute__((noinline)) uint32_t wave_kth_largest(uint32_t u, bool cand, int kw) {
+  uint32_t kth = 0u;
+  for (int b = 31; b >= 0; --b) {
+    const uint32_t trial = kth | (1u << b);
+    if (__popcll(__ballot(cand && u >= trial)) >= kw) kth = trial;
+  }
+  return kth;
+}
+
 // Scores of V vectors x T queries (fa * alpha, -0.f reported as +0.f, as the
 // reference's flush_batch_to_fa epilogue) and their admission into the
 // workgroup's top-k lists: strict "better than the current k-th" test,
@@ -1610,16 +1623,17 @@ __device__ __forceinline__ void wave_sort_desc(float& s, uint32_t& id, int lane)
 template <int T, int V>
 __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], const long (&vv)[V],
                              const bool (&valid)[V], const float* __restrict__ alpha, long q0, long nq,
-                             int k, int cap, const float (*voff)[V] = nullptr) {
+                             int k, int cap, const float (*voff)[V] = nullptr,
+                             const float (*av_pre)[V] = nullptr) {
   const int lane = threadIdx.x & 63;
   const int kw = min(k, 64);
 #pragma unroll
   for (int j = 0; j < V; ++j) {
     float s[T];
     uint32_t sid[T];
-    const float av = valid[j] ? alpha[vv[j]] : 0.f;
+    const float av = valid[j] ? (av_pre != nullptr ? (*av_pre)[j] : alpha[vv[j]]) : 0.f;
     const uint32_t vid = (uint32_t)vv[j];
-    uint32_t pend = 0u;
+    uint32_t pend = 0u, candm = 0u;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
       s[t] = valid[j] ? fa[j][t] * av + (voff != nullptr ? (*voff)[j] : 0.f) : -INFINITY;
@@ -1627,22 +1641,38 @@ __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], cons
       // below the bound: worse than k vectors already offered (see below)
       const bool cand = valid[j] && q0 + t < nq && better(s[t], vid, sh.thr_s[t], sh.thr_i[t]) &&
                         !(s[t] < order_float(sh.bound[t]));
-      // (tiles of one or two queries only: the bandwidth regime, where a
-      // workgroup scans few steps and the first one admits every vector; the
-      // sort would cost larger tiles their register budget)
-      const int nc = T <= 2 ? __popcll(__ballot(cand)) : 0;
-      if (T <= 2 && nc > kw) {
-        // more candidates in this wavefront than can reach the list: keep the
-        // wavefront's best kw (sorted into lanes 0 .. kw-1)
-        if (!cand) { s[t] = -INFINITY; sid[t] = 0xffffffffu; }
-        wave_sort_desc(s[t], sid[t], lane);
-        if (lane < kw) pend |= 1u << t;
-        // The wavefront offers its k best; every later vector scoring below
-        // the k-th of them is worse than k offered vectors (each of which
-        // stays in the lists or is displaced by a better one), so it can
-        // never enter the top k: that score bounds the admission from below
-        // (an equal score may still win on its id).
-        if (kw == k && lane == k - 1) atomicMax(&sh.bound[t], float_order(s[t]));
+      candm |= (cand ? 1u : 0u) << t;
+    }
+    // More candidates in this wavefront than can reach a query's list (a
+    // workgroup's first steps admit almost every vector): offer only the
+    // wavefront's best kw of that query. The wavefront then offers its k best;
+    // every later vector scoring below the k-th of them is worse than k
+    // offered vectors (each of which stays in the lists or is displaced by a
+    // better one), so it can never enter the top k: that score bounds the
+    // admission from below (an equal score may still win on its id). Without
+    // this a tile of 8 queries pushed its whole first step through the
+    // candidate buffer. (nc <= 64, so this happens for k < 64 only.)
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const bool cand = ((candm >> t) & 1u) != 0u;
+      const int nc = __popcll(__ballot(cand));
+      if (nc > kw) {
+        if constexpr (T <= 2) {
+          // tiles of one or two queries: sort the best kw into lanes 0 .. kw-1
+          if (!cand) { s[t] = -INFINITY; sid[t] = 0xffffffffu; }
+          wave_sort_desc(s[t], sid[t], lane);
+          if (lane < kw) pend |= 1u << t;
+          if (kw == k && lane == k - 1) atomicMax(&sh.bound[t], float_order(s[t]));
+        } else {
+          // larger tiles: the kw-th largest score by a radix select over the
+          // candidates' order-preserving bits (ballots, no lane exchange: an
+          // unrolled sort per query cost these tiles half their occupancy);
+          // the lanes at or above it are offered (ties may add a few)
+          const uint32_t u = (uint32_t)float_order(s[t]) ^ 0x80000000u;  // unsigned, same order
+          const uint32_t kth = wave_kth_largest(u, cand, kw);
+          if (cand && u >= kth) pend |= 1u << t;
+          if (kw == k && lane == 0) atomicMax(&sh.bound[t], (int)(kth ^ 0x80000000u));
+        }
       } else if (cand) {
         pend |= 1u << t;
       }
@@ -1653,6 +1683,9 @@ __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], cons
       // barriers, only the uniform exit below
 #pragma unroll
       for (int t = 0; t < T; ++t) {
+        // the bound may have risen since the candidates were chosen (other
+        // wavefronts' k-th best): what falls below it can be dropped now
+        if (s[t] < order_float(sh.bound[t])) pend &= ~(1u << t);
         const bool want = ((pend >> t) & 1u) != 0u;
         // one LDS atomic per wavefront: base + rank among the wanting lanes
         const uint64_t wm = __ballot(want);
@@ -2203,7 +2236,7 @@ __device__ __forceinline__ void scan_pair_body(uint8_t* __restrict__ table, uint
     long vv[V];
     bool valid[V];
     const uint4* cp[V];
-    float fa[V][T];
+    float fa[V][T], avp[V];
     uint32_t acc[V][NA], acc2[V][NA];
 #pragma unroll
     for (int j = 0; j < V; ++j) {
@@ -2211,6 +2244,10 @@ __device__ __forceinline__ void scan_pair_body(uint8_t* __restrict__ table, uint
       valid[j] = vv[j] < r_end;
       const long vs = valid[j] ? vv[j] : 0;
       cp[j] = reinterpret_cast<const uint4*>(tiled) + (vs >> 6) * (long)G16 * 64 + (vs & 63);
+      // tiles of one or two queries: alpha loaded now, so its latency is not
+      // exposed at the admission (a workgroup of the bandwidth regime scans
+      // only a few steps)
+      if constexpr (T <= 2) avp[j] = alpha[vs];
 #pragma unroll
       for (int t = 0; t < T; ++t) fa[j][t] = bias[t];
 #pragma unroll
@@ -2239,7 +2276,7 @@ __device__ __forceinline__ void scan_pair_body(uint8_t* __restrict__ table, uint
       }
       pair_chunk<T, V, NR>(table, stage, dma, scl, cp, c, G16, ring, acc, acc2, fa);
     }
-    admit_scores<T, V>(sh, fa, vv, valid, alpha, q0, nq, k, cap);
+    admit_scores<T, V>(sh, fa, vv, valid, alpha, q0, nq, k, cap, nullptr, T <= 2 ? &avp : nullptr);
   }
   write_partials<T>(sh, q0, nq, n_lists, list, k, cap, part_s, part_i, part_cnt);
 }
