@@ -1683,9 +1683,6 @@ __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], cons
       // barriers, only the uniform exit below
 #pragma unroll
       for (int t = 0; t < T; ++t) {
-        // the bound may have risen since the candidates were chosen (other
-        // wavefronts' k-th best): what falls below it can be dropped now
-        if (s[t] < order_float(sh.bound[t])) pend &= ~(1u << t);
         const bool want = ((pend >> t) & 1u) != 0u;
         // one LDS atomic per wavefront: base + rank among the wanting lanes
         const uint64_t wm = __ballot(want);
