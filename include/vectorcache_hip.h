@@ -184,7 +184,9 @@ vc_status vc_lm_index_add_device(vc_lm_index* index, const float* vectors_dev, s
 vc_status vc_lm_index_prepare(vc_lm_index* index);
 /* search(queries, k, float_corpus): k_eff = min(k, size()) <= 2048; with a
  * float corpus (size() x dim rows, may be NULL) the top k is re-scored by exact
- * inner products (float_rescore_topk). VC_ERR_STATE before prepare(). */
+ * inner products (float_rescore_topk). The host call reads the corpus on the
+ * host and uploads only the nq x k_eff candidate rows (one synchronization
+ * between the search and the re-score). VC_ERR_STATE before prepare(). */
 vc_status vc_lm_index_search(vc_lm_index* index, const float* queries, size_t nq, size_t k,
                              const float* float_corpus, float* scores, uint64_t* ids, size_t* k_out);
 vc_status vc_lm_index_search_device(vc_lm_index* index, const float* queries_dev, size_t nq, size_t k,
