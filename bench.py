@@ -451,6 +451,52 @@ def fit_probe_table(y, ranks, csz, target, y0=0.0, y_step=0.25, n_bins=40, grid=
     return gl[at].astype(np.int32)
 
 
+def oracle_policy(ranks, csz, target):
+    """The fewest scanned vectors any per-query probe count reaches `target`
+    mean recall with, on these queries: an oracle that knows each query's
+    judge rows opens, per query, just enough of its routing list (at most
+    csz.shape[1] buckets) to take in some of them. Chosen by a Lagrange
+    multiplier on recall against vectors (each query takes the count that
+    maximizes recall - lambda * vectors), bisected until the mean recall meets
+    the target: the lower convex hull of the trade-off, so a bound the learned
+    policy can be held against. ranks: routing rank of each judge row's
+    bucket; csz: cumulative vectors of each query's routing list."""
+    import torch
+
+    cap = csz.shape[1]
+    n, k = ranks.shape
+    r, _ = torch.sort(ranks.long(), dim=1)
+    reach = r < cap
+    p = torch.clamp(r + 1, max=cap)                                   # probes to take in rows 0 .. j
+    rec = (ranks.long().unsqueeze(1) < p.unsqueeze(2)).double().mean(2)  # [n, k]
+    cost = torch.gather(csz, 1, p - 1).double()
+    cost0 = csz[:, 0].double()
+    rows = torch.arange(n, device=ranks.device)
+
+    def pick(lam):
+        val = torch.where(reach, rec - lam * cost, torch.full_like(cost, -np.inf))
+        best, j = val.max(1)
+        take = best > -lam * cost0
+        recall = torch.where(take, rec[rows, j], torch.zeros_like(best))
+        vecs = torch.where(take, cost[rows, j], cost0)
+        probes = torch.where(take, p[rows, j], torch.ones_like(j))
+        return float(recall.mean()), float(vecs.sum()), float(probes.double().mean())
+
+    lo, hi = 0.0, 1.0
+    while pick(hi)[0] >= target and hi < 1e6:
+        hi *= 2.0
+    if pick(lo)[0] < target:
+        return None
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if pick(mid)[0] >= target:
+            lo = mid
+        else:
+            hi = mid
+    recall, vecs, probes = pick(lo)
+    return {"recall": recall, "vectors": vecs, "mean_n_probe": probes}
+
+
 def table_counts(y, table, valid, y0=0.0, y_step=0.25):
     import torch
 
@@ -517,8 +563,18 @@ def run_bucketed(args):
     log("labels (buckets for recall 1.0) quantiles 10/50/90/99: "
         f"{np.percentile(lab_np[tr], [10, 50, 90, 99]).tolist()}")
 
+    # ---- the recall target: the highest of RECALL_TARGETS that fixed n_probe <= 256 reaches on validation ----
+    fixed_curve = {p: recall_at(ranks[va], p) for p in range(1, min(256, L) + 1)}
+    target = next((t for t in RECALL_TARGETS if max(fixed_curve.values()) >= t), RECALL_TARGETS[-1])
+    # labels at that target: per query the smallest n_probe whose result reaches recall@k >= target
+    # against the judge (for 0.95 and k = 10: every judge row, the largest rank + 1)
+    q_lab = min(k, int(math.ceil(target * k - 1e-9))) - 1
+    labels_t = torch.sort(ranks, dim=1).values[:, q_lab] + 1
+    log(f"labels at recall {target} (rank of judge row {q_lab + 1} of {k}) quantiles 10/50/90/99: "
+        f"{np.percentile(labels_t[tr].cpu().numpy(), [10, 50, 90, 99]).tolist()}")
+
     # ---- train the probe model on the GPU ----
-    targets = torch.log2(torch.clamp(labels, 1, 512).float())
+    targets = torch.log2(torch.clamp(labels_t, 1, 512).float())
     ix.probe_init(4005)
     rng = np.random.default_rng(4006)
     epochs = 8
@@ -537,8 +593,6 @@ def run_bucketed(args):
     # ---- recall target, fixed n_probe and the learned policy, chosen on the validation split ----
     valid = torch.clamp(torch.full((q_all.shape[0],), min(256, L), device=dev), max=P)
     y = ix.probe_predict(feats)
-    fixed_curve = {p: recall_at(ranks[va], p) for p in range(1, min(256, L) + 1)}
-    target = next((t for t in RECALL_TARGETS if max(fixed_curve.values()) >= t), RECALL_TARGETS[-1])
     n_fixed = next(p for p, r in fixed_curve.items() if r >= target) if max(fixed_curve.values()) >= target else min(
         256, L)
     # the learned policy: the model's prediction picks a bin, each bin's probe count is fitted on validation
@@ -584,6 +638,11 @@ def run_bucketed(args):
     csz = csz_all[ev]
     vec_fixed = int(csz[:, n_fixed - 1].sum())
     vec_learned = int(torch.gather(csz, 1, (cnt_ev.long() - 1).view(-1, 1)).sum())
+    oracle = oracle_policy(ranks[ev], csz, rec_fixed_ev)
+    if oracle is not None:
+        oracle["vectors_fixed_over_oracle"] = vec_fixed / oracle["vectors"]
+        oracle["vectors_fixed_over_learned"] = vec_fixed / max(vec_learned, 1)
+        log(f"oracle probe policy at recall {rec_fixed_ev:.4f}: {oracle}")
     lut_bytes = n_ev * G * 32
     bytes_l = vec_learned * (G + 4) + lut_bytes
     bytes_f = vec_fixed * (G + 4) + lut_bytes
@@ -660,8 +719,12 @@ def run_bucketed(args):
         "learned": {k2: v for k2, v in res["learned"].items() if k2 not in ("ids", "scores")},
         "learned_over_fixed_qps": res["learned"]["qps"] / res["fixed"]["qps"],
         "eval_recall_from_ranks": {"fixed": rec_fixed_ev, "learned": rec_learned_ev},
+        "oracle_probe_policy": oracle,
         "recall_curve_fixed_n_probe": reach,
         "label_quantiles_10_50_90_99": np.percentile(lab_np[ev], [10, 50, 90, 99]).tolist(),
+        "training_labels": {"recall": target, "judge_row": q_lab + 1,
+                            "quantiles_10_50_90_99": np.percentile(labels_t[ev].cpu().numpy(),
+                                                                   [10, 50, 90, 99]).tolist()},
         "roofline": {
             "bound": "hbm", "achieved": ach_l, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach_l / HBM_PEAK_GBS,
             "traffic": traffic, "traffic_source": traffic_src, "kernel": "bucket_scan_kernel (learned probe)",
