@@ -458,9 +458,10 @@ def oracle_policy(ranks, csz, target):
     csz.shape[1] buckets) to take in some of them. Chosen by a Lagrange
     multiplier on recall against vectors (each query takes the count that
     maximizes recall - lambda * vectors), bisected until the mean recall meets
-    the target: the lower convex hull of the trade-off, so a bound the learned
-    policy can be held against. ranks: routing rank of each judge row's
-    bucket; csz: cumulative vectors of each query's routing list."""
+    the target: a point of the trade-off's lower convex hull, within one
+    query's vectors of the cheapest combination of per-query counts, so a
+    bound the learned policy can be held against. ranks: routing rank of each
+    judge row's bucket; csz: cumulative vectors of each query's routing list."""
     import torch
 
     cap = csz.shape[1]

@@ -39,3 +39,32 @@ def test_cpu_cells_match_direct_calls(oracle, monkeypatch):
     assert np.array_equal(out["perm0"], perm0[: me * dim // 2])
     assert np.array_equal(out["alpha"], alpha[:me])
     assert all(c["rate"] > 0 for c in cells)
+
+
+def test_oracle_probe_policy_against_brute_force():
+    """On tiny instances every combination of per-query probe counts is
+    tried: the oracle meets the target, and scans at most one query's
+    vectors more than the cheapest combination that meets it."""
+    import itertools
+
+    import torch
+
+    rng = np.random.default_rng(3)
+    for trial in range(30):
+        n, k, cap = 4, 3, 6
+        ranks = torch.from_numpy(rng.integers(0, cap + 2, (n, k)).astype(np.int32))
+        csz = torch.from_numpy(np.cumsum(rng.integers(1, 50, (n, cap)), axis=1))
+        counts = list(range(1, cap + 1))
+        rec = [[float((ranks[i] < p).float().mean()) for p in counts] for i in range(n)]
+        cost = [[float(csz[i, p - 1]) for p in counts] for i in range(n)]
+        best_rec = float(np.mean([max(r) for r in rec]))
+        target = float(rng.uniform(0.1, 1.0)) * best_rec
+        if target <= 0:
+            continue
+        best = min(sum(cost[i][c[i]] for i in range(n))
+                   for c in itertools.product(range(cap), repeat=n)
+                   if np.mean([rec[i][c[i]] for i in range(n)]) >= target - 1e-12)
+        o = bench.oracle_policy(ranks, csz, target)
+        assert o is not None and o["recall"] >= target - 1e-12, trial
+        slack = max(max(c) - min(c) for c in cost)
+        assert best - 1e-9 <= o["vectors"] <= best + slack, (trial, o, best)
