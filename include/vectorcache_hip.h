@@ -216,14 +216,17 @@ int vc_lm_index_is_prepared(const vc_lm_index* index);
  * (src/pack/pack.cpp:428-449,490-541), and search() then takes the staged
  * branch of search_landmark_rerank (src/search/search.cpp:1379-1510) whenever
  * its shortlist is shorter than the index: the E = max(256, 6k) best sign
- * scores (build_sign_lut LUTs, score = fma(A, lut_scale, lut_bias), no alpha;
- * the reference's collector returns that set unless a score ties the E-th),
- * then those E rescored in shortlist order with build_permute_dot's int8
- * weights and levels plus the residual term, into a k-heap. Results equal the
- * reference's VNNI host's staged search; k up to 341 (E <= 2048) and at most
- * 2048 landmarks. Below 32768 vectors a VNNI host scans with its permute-dot
- * kernels, which are not rebuilt: the index keeps the AVX2 scan there.
- * Takes effect at the next prepare() (or at once on a prepared index). */
+ * scores by (score desc, id asc) (build_sign_lut LUTs, score = fma(A,
+ * lut_scale, lut_bias), no alpha; the reference's buffered collector returns
+ * exactly that set, ties included), then those E rescored with
+ * build_permute_dot's int8 weights and levels plus the residual term, into a
+ * k-heap. Results equal the reference's VNNI host's staged search; k up to 341
+ * (E <= 2048, else VC_ERR_INVALID_ARGUMENT) and at most 2048 landmarks.
+ * Planes are kept only when n_landmarks rounds to a multiple of 32 (the
+ * reference has no sign kernel for other widths and reads past its sign
+ * codes there); below 32768 vectors a VNNI host scans with its permute-dot
+ * kernels, which are not rebuilt. In both cases the index keeps the AVX2
+ * scan. Takes effect at the next prepare() (or at once on a prepared index). */
 vc_status vc_lm_index_set_planes(vc_lm_index* index, int on);
 int vc_lm_index_planes_active(const vc_lm_index* index);
 size_t vc_lm_index_codes_bytes(const vc_lm_index* index);
