@@ -1427,7 +1427,8 @@ static float lm_norm_sq(const float* x, size_t m) {
  *      and keeps the E best (E =
  *      planes_exact_len(k, planes_shortlist_len(k, 4)), pack.cpp:451-463) by
  *      (score desc, id asc): the buffered collector with its sample seeds
- *      returns that set whenever no score ties the E-th;
+ *      returns exactly that set, ties included (one query's vectors reach it
+ *      in id order, and it drops only what E kept entries beat);
  *   2. rescores those E in that order with build_permute_dot's int8 query
  *      weights and centroid levels (search_vnni.cpp:1318-1360): acc = sum
  *      over byte groups of lev[lo] * w_lo + lev[hi] * w_hi (exact int32),
