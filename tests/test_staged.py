@@ -221,3 +221,61 @@ def test_gpu_planes_unstaged_width_keeps_the_exact_scan(oracle):
     s_a, i_a, _ = oracle.lm_search(x, q, k, scan_bits=4, n_landmarks=m, seed=seed)
     r = ix.search(q, k)
     assert np.array_equal(r.ids, i_a) and scores_equal(r.scores, s_a)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", [1, 341])
+def test_gpu_staged_k_limits_match_oracle(oracle, k):
+    """k = 1 (E = 256) and k = 341, the largest k whose shortlist fits the
+    GPU top-k (E = 2046): against the oracle."""
+    from vectorcache_amd.landmark import LandmarkIndex
+
+    case = ("klim", 34000, 3, 128, 64, 2, k)
+    tag, n, nq, dim, m, seed, kk = case
+    x, q = landmark_inputs(n, nq, dim)
+    s_o, i_o, _ = _run(oracle, case)
+    ix = LandmarkIndex(dim, 4, m, seed, planes=True)
+    ix.add(x)
+    ix.prepare()
+    r = ix.search(q, kk)
+    assert np.array_equal(r.ids, i_o) and scores_equal(r.scores, s_o)
+
+
+@pytest.mark.gpu
+def test_gpu_staged_rejects_a_shortlist_past_the_gpu_top_k():
+    """k = 342 needs a 2052-row shortlist: a clear error, and the index keeps
+    working."""
+    from vectorcache_amd.landmark import LandmarkIndex
+
+    x, q = landmark_inputs(34000, 2, 128)
+    ix = LandmarkIndex(128, 4, 64, 2, planes=True)
+    ix.add(x)
+    ix.prepare()
+    with pytest.raises(Exception, match="shortlist"):
+        ix.search(q, 342)
+    assert ix.search(q, 10).ids.shape == (2, 10)
+
+
+@pytest.mark.gpu
+def test_gpu_staged_batches_and_empty_queries(oracle):
+    """Adds in three batches with a prepare after each (the sign planes are
+    rebuilt every time; the first prepare is below 32768 vectors, so no planes
+    yet), then no queries at all."""
+    from vectorcache_amd.landmark import LandmarkIndex
+
+    n, nq, dim, m, seed, k = 36000, 4, 128, 64, 5, 20
+    x, q = landmark_inputs(n, nq, dim)
+    batches = [20000, 10000, 6000]
+    ix = LandmarkIndex(dim, 4, m, seed, planes=True)
+    start = 0
+    for b in batches:
+        ix.add(x[start:start + b])
+        ix.prepare()
+        start += b
+        assert ix.planes_active() == (start >= 32768)
+    s_o, i_o, _ = oracle.lm_search(x, q, k, scan_bits=4, n_landmarks=m, seed=seed, batches=batches,
+                                   prepare_between=True, planes=True)
+    r = ix.search(q, k)
+    assert np.array_equal(r.ids, i_o) and scores_equal(r.scores, s_o)
+    r0 = ix.search(q[:0], k)
+    assert r0.ids.shape[0] == 0
