@@ -161,12 +161,14 @@ std::size_t scan_lut_bytes(const ScanPlan& p, int groups16, std::size_t nq);
 
 // Scan + per-range exact top-k. luts: nq x groups*32 (prepare_queries);
 // lut_t: scratch of scan_lut_bytes. Partial lists: part_s/part_i
-// [nq][n_ranges][k], part_cnt [nq][n_ranges].
+// [nq][n_ranges][k], part_cnt [nq][n_ranges]. phase 0 launches both steps;
+// 1 only the table sources (pair sources or interleaved LUTs) into lut_t, 2
+// only the scan kernel (for timing the scan alone).
 hipError_t launch_scan(const ScanPlan& p, const std::uint8_t* tiled, const float* alpha,
                        std::size_t n, int groups, int groups16, const std::uint8_t* luts,
                        std::uint8_t* lut_t, const float* lut_scale, const float* bias_total,
                        std::size_t nq, int k, float* part_s, std::uint32_t* part_i, int* part_cnt,
-                       hipStream_t s);
+                       hipStream_t s, int phase = 0);
 
 // Merge per-range lists into final rows of k_out entries (padded with 0/0).
 // id_map (device, may be null) maps vector positions to reported ids.

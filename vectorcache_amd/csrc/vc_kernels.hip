@@ -4061,7 +4061,8 @@ std::size_t scan_lut_bytes(const ScanPlan& p, int groups16, std::size_t nq) {
 hipError_t launch_scan(const ScanPlan& p, const std::uint8_t* tiled, const float* alpha, std::size_t n,
                        int groups, int groups16, const std::uint8_t* luts, std::uint8_t* lut_t,
                        const float* lut_scale, const float* bias_total, std::size_t nq, int k, float* part_s,
-                       std::uint32_t* part_i, int* part_cnt, hipStream_t s) {
+                       std::uint32_t* part_i, int* part_cnt, hipStream_t s, int phase) {
+  const bool sources = phase != 2, scan = phase != 1;
   const long n_qt = (long)((nq + p.tile - 1) / p.tile);
   const dim3 grid((unsigned)(n_qt * p.n_ranges));
   const dim3 block(kScanThreads);
@@ -4070,13 +4071,16 @@ hipError_t launch_scan(const ScanPlan& p, const std::uint8_t* tiled, const float
     const long pair_elems = n_qt * pair_src_blocks(p.tile, groups16) * 16 * 32 * p.tile;
 #define VC_LAUNCH_PAIR(TT, VV)                                                                            \
   do {                                                                                                    \
-    hipLaunchKernelGGL(pair_source_kernel<TT>, dim3(grid_for(pair_elems, 256)), dim3(256), 0, s, luts,    \
-                       (long)nq, groups, groups16, lut_t);                                                \
-    (void)hipFuncSetAttribute((const void*)scan_pair_kernel<TT, VV>,                                       \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)pair_dyn);                 \
-    hipLaunchKernelGGL((scan_pair_kernel<TT, VV>), grid, block, pair_dyn, s, tiled, alpha, (long)n,         \
-                       groups16, lut_t, lut_scale, bias_total, (long)nq, k, p.cand_cap, p.n_ranges,       \
-                       (long)p.range_len, part_s, part_i, part_cnt);                                      \
+    if (sources)                                                                                          \
+      hipLaunchKernelGGL(pair_source_kernel<TT>, dim3(grid_for(pair_elems, 256)), dim3(256), 0, s, luts,  \
+                         (long)nq, groups, groups16, lut_t);                                              \
+    if (scan) {                                                                                           \
+      (void)hipFuncSetAttribute((const void*)scan_pair_kernel<TT, VV>,                                     \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)pair_dyn);               \
+      hipLaunchKernelGGL((scan_pair_kernel<TT, VV>), grid, block, pair_dyn, s, tiled, alpha, (long)n,       \
+                         groups16, lut_t, lut_scale, bias_total, (long)nq, k, p.cand_cap, p.n_ranges,     \
+                         (long)p.range_len, part_s, part_i, part_cnt);                                    \
+    }                                                                                                     \
   } while (0)
     const size_t pair_dyn = p.lds_bytes - kPairTableBytes - kPairStageBytes;  // table and stage are static LDS
     const bool v2 = p.vec_per_thread == 2;
@@ -4089,8 +4093,10 @@ hipError_t launch_scan(const ScanPlan& p, const std::uint8_t* tiled, const float
 #undef VC_LAUNCH_PAIR
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(interleave_luts_kernel, dim3(grid_for(src_elems, 256)), dim3(256),
-                     0, s, luts, (long)nq, groups, groups16, p.tile, lut_t);
+  if (sources)
+    hipLaunchKernelGGL(interleave_luts_kernel, dim3(grid_for(src_elems, 256)), dim3(256),
+                       0, s, luts, (long)nq, groups, groups16, p.tile, lut_t);
+  if (!scan) return hipGetLastError();
 #define VC_LAUNCH_SCAN(TT, VV)                                                                          \
   do {                                                                                                  \
     (void)hipFuncSetAttribute((const void*)scan_topk_kernel<TT, VV>,                                     \
