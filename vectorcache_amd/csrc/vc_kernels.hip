@@ -2658,11 +2658,13 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void bucket_scan_k
     reinterpret_cast<uint64_t*>(stage)[tid] = reinterpret_cast<const uint64_t*>(q_src)[tid];
     __syncthreads();
     bool built = false;
+    const bool wave_lists = a.k <= 64;  // a top-k list per wavefront, merged once per item
+    WaveList wl;
     const int step = kScanThreads * V;
     for (int base = i0; base < i1; base += step) {
       long vv[V];
       bool valid[V];
-      float voff[V];
+      float voff[V], avp[V];
       const uint4* cp[V];
       float fa[V][T];
       uint32_t acc[V][1], acc2[V][1];
@@ -2678,6 +2680,7 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void bucket_scan_k
         }
         vv[j] = valid[j] ? l_pos[e0] + (i - l_pre[e0]) : 0;
         voff[j] = valid[j] ? l_off[e0] : 0.f;
+        avp[j] = a.alpha[vv[j]];  // loaded now, used at the admission
         cp[j] = reinterpret_cast<const uint4*>(a.tiled) + (vv[j] >> 6) * (long)G16 * 64 + (vv[j] & 63);
         fa[j][0] = bias[0];
         acc[j][0] = 0u;
@@ -2700,7 +2703,19 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void bucket_scan_k
         }
         pair_chunk<T, V, NR>(table, stage, dma, scl, cp, c, G16, ring, acc, acc2, fa);
       }
-      admit_scores<T, V>(sh, fa, vv, valid, a.alpha, 0, 1, a.k, a.cap, &voff);
+      if (wave_lists) {
+        // the score as admit_scores forms it: fa * alpha, then + the offset
+#pragma unroll
+        for (int j = 0; j < V; ++j)
+          wave_list_admit(wl, valid[j] ? fa[j][0] * avp[j] + voff[j] : -INFINITY, (uint32_t)vv[j], valid[j], a.k,
+                          tid & 63);
+      } else {
+        admit_scores<T, V>(sh, fa, vv, valid, a.alpha, 0, 1, a.k, a.cap, &voff, &avp);
+      }
+    }
+    if (wave_lists) {
+      wave_lists_write(wl, table, it, 1, 0, a.k, a.part_s, a.part_i, a.part_cnt);
+      continue;
     }
     __syncthreads();
     if (__builtin_amdgcn_readfirstlane(sh.cnt[0]) > 0) topk_merge(sh, 0, a.k, a.cap);
