@@ -246,13 +246,25 @@ __global__ __launch_bounds__(kTopThreads) void topk_rows_kernel(const float* __r
         __syncthreads();
         if (tid == 0) n_cand = 0;
         __syncthreads();
-        for (int c = tid; c < N; c += kTopThreads) {
-          const float v = sr[c];
-          if (v >= thr) {  // false for NaN
-            const int at = atomicAdd(&n_cand, 1);
-            if (at < kTopCand) {
-              cs[at] = v;
-              cc[at] = c;
+        // eight loads in flight per thread (one wavefront load covers 256
+        // consecutive scores), then their compares and appends
+        constexpr int U = 8;
+        for (int c0 = tid; c0 < N; c0 += kTopThreads * U) {
+          float v[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int c = c0 + u * kTopThreads;
+            v[u] = c < N ? sr[c] : 0.f;
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int c = c0 + u * kTopThreads;
+            if (c < N && v[u] >= thr) {  // false for NaN
+              const int at = atomicAdd(&n_cand, 1);
+              if (at < kTopCand) {
+                cs[at] = v[u];
+                cc[at] = c;
+              }
             }
           }
         }
