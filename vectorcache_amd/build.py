@@ -54,7 +54,7 @@ def build(verbose: bool = True) -> Path:
         _run(["g++", *HOST_FLAGS, "-c", CSRC / src, "-o", obj])
         objs.append(obj)
     tmp = OUT.with_suffix(".so.tmp")
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs, "-lrocblas"])
     tmp.replace(OUT)
     build_cpp_check()
     return OUT
