@@ -680,7 +680,7 @@ def run_bucketed(args):
     tp = ROOT / "profiles" / f"bucket_scan_kernel_{args.workload}_traffic.json"
     if tp.exists():
         td = json.loads(tp.read_text())
-        if td.get("nq") == n_ev and td.get("n_buckets") == L:
+        if td.get("nq") == n_ev and td.get("n_buckets") == L and td.get("learned_algorithmic_bytes") == bytes_l:
             traffic, traffic_src = td["learned_traffic_bytes_per_launch"], str(tp.relative_to(ROOT))
 
     cpu = None
