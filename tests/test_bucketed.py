@@ -76,8 +76,20 @@ def test_bucket_codes_match_oracle(vc, oracle, dim, bits, n, es, mb, tq):
     assert np.mean(own >= best - 0.05) > 0.95
 
 
+# debug flags: bit 7 forces the bucket-major tile scan, bit 8 the query-major one
+SCAN_PATHS = {"query": 256, "tile": 128}
+
+
+@pytest.fixture
+def scan_path(vc, request):
+    vc.set_debug(SCAN_PATHS[request.param])
+    yield request.param
+    vc.set_debug(0)
+
+
+@pytest.mark.parametrize("scan_path", list(SCAN_PATHS), indirect=True)
 @pytest.mark.parametrize("dim,bits,n,es,mb,tq", CASES)
-def test_bucketed_search_matches_oracle_chain(vc, oracle, dim, bits, n, es, mb, tq):
+def test_bucketed_search_matches_oracle_chain(vc, oracle, dim, bits, n, es, mb, tq, scan_path):
     import torch
 
     from vectorcache_amd import synthetic
@@ -103,6 +115,61 @@ def test_bucketed_search_matches_oracle_chain(vc, oracle, dim, bits, n, es, mb, 
             found = np.array([min(k, int(st["bucket_sizes"][pb[r]].sum())) for r in range(24)])
             assert np.array_equal(cnt, found), (n_probe, k)
             _same_topk(s, i, ws, wi)
+
+
+def _tile_vs_query(vc, ix, qd, k, **kw):
+    import torch
+
+    nq = qd.shape[0]
+    out = {}
+    for name, flag in SCAN_PATHS.items():
+        vc.set_debug(flag)
+        try:
+            s, i, c = ix.search_device(qd, k, counts=torch.empty((nq, 1), dtype=torch.int64, device="cuda"), **kw)
+        finally:
+            vc.set_debug(0)
+        out[name] = (s.cpu().numpy(), i.cpu().numpy(), c.cpu().numpy().ravel())
+    (qs, qi, qc), (ts, ti, tc) = out["query"], out["tile"]
+    # both keep (score desc, position asc): ids equal even on ties
+    assert scores_equal(ts, qs)
+    assert np.array_equal(ti, qi)
+    assert np.array_equal(tc, qc)
+    return out["tile"]
+
+
+@pytest.mark.parametrize("bits,tq", [(4, True), (2, False)])
+def test_tile_scan_equals_query_scan(vc, oracle, bits, tq):
+    """The bucket-major tile scan (eight queries per bucket read) returns the
+    query-major scan's lists bit for bit: many queries per bucket, fixed and
+    per-query probe counts, k from 1 to the tile limit of 64, duplicated rows
+    for heavy score ties; 48 queries against the oracle's chain."""
+    import torch
+
+    from vectorcache_amd import synthetic
+
+    dim, n = 128, 12000
+    x = synthetic.zipf_mixture(n, dim, 30, 1.1, 0.3, 41, 42)
+    x[6000:9000] = x[:3000]  # equal rows: equal scores, broken by position
+    q = synthetic.zipf_mixture(700, dim, 30, 1.1, 0.3, 41, 43)
+    sample = synthetic.zipf_mixture(600, dim, 30, 1.1, 0.3, 41, 44) if tq else None
+    ix = _bucketed(x, dim, bits, sample=sample, energy_soft=100, max_bucket_size=200)
+    L = ix.n_buckets()
+    qd = torch.from_numpy(q).cuda()
+    rng = np.random.default_rng(45)
+    for n_probe in (3, 17, L):
+        for k in (1, 10, 64):
+            _tile_vs_query(vc, ix, qd, k, n_probe=n_probe)
+    counts = torch.from_numpy(rng.integers(0, 40, 700).astype(np.int32)).cuda()
+    ts, ti, _ = _tile_vs_query(vc, ix, qd, 10, n_probe=40, probe_counts=counts)
+    # the oracle's chain over each query's first counts[q] buckets
+    st = ix.export()
+    pb = ix.route_device(qd[:48], 40)[0].cpu().numpy()
+    sh, sc = (st["shift"], st["scale"]) if tq else (None, None)
+    cnt = np.clip(counts[:48].cpu().numpy(), 1, 40).astype(np.int32)  # counts are clamped to [1, n_probe]
+    ws, wi = oracle.bucketed_search(q[:48], 10, bits, st["bucket_sizes"], st["flat"], st["alpha"], st["pos_ids"],
+                                    st["centroids"], pb, cnt, sh, sc)
+    # rows with fewer than k vectors opened hold (0, 0) past their count in both
+    _same_topk(ts[:48], ti[:48].astype(np.uint64), ws, wi)
 
 
 def test_bucketed_direct_codes_all_buckets_equal_flat_search(vc, oracle):

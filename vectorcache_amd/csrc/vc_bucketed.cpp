@@ -471,13 +471,31 @@ struct SearchBufs {
   uint32_t* part_i;
   uint8_t *luts, *src;
   size_t route_chunk;
+  // the bucket-major tile scan (tiles set): its planning buffers
+  bool tiles;
+  int *bcnt, *bstart, *tstart, *n_tiles, *prank, *pair_q, *pair_j, *tile_b, *tile_p0, *tile_n, *gbound;
 };
 
+constexpr size_t kTilePartBytes = size_t(1) << 30;  // slot lists of the tile scan, at most
+
+// The bucket-major tile scan serves batches that put several queries on a
+// bucket (each tile then reads its bucket's codes once for up to eight
+// queries); small batches keep the query-major scan. Debug flags bit 7 /
+// bit 8 force one or the other (test hooks).
+bool use_tile_scan(const vc_bucketed* ix, size_t nq, int P, size_t k) {
+  if (k > size_t(vchip::kTileMaxK) || nq * size_t(P) * k * 8 > kTilePartBytes || nq > size_t(INT32_MAX) / size_t(P))
+    return false;
+  const int f = g_debug_flags.load();
+  if ((f & 128) != 0) return true;
+  if ((f & 256) != 0) return false;
+  return nq * size_t(P) >= 8 * size_t(ix->L);
+}
+
 SearchBufs carve_search(vc_bucketed* ix, size_t nq, int P, int k, uint8_t* (*take)(void*, size_t), void* ctx,
-                        size_t* need) {
+                        size_t* need, bool tiles = false) {
   const size_t D = ix->dim, G = size_t(ix->t.groups);
   const int L = ix->L, Kp = ix->Kp;
-  const size_t items = nq * vchip::kBucketMaxItems;
+  const size_t items = tiles ? nq * size_t(P) : nq * vchip::kBucketMaxItems;
   const size_t chunk = std::max<size_t>(1, std::min(nq, kRouteScoreBytes / (size_t(L) * 4)));
   Carve c;
   const size_t o_qinv = c.take(nq * 4), o_qhat = c.take(nq * D * 4), o_qb = c.take(nq * size_t(Kp) * 2),
@@ -487,6 +505,12 @@ SearchBufs carve_search(vc_bucketed* ix, size_t nq, int P, int k, uint8_t* (*tak
                o_ls = c.take(nq * 4), o_lb = c.take(nq * 4), o_bc = c.take(nq * 4), o_bt = c.take(nq * 4),
                o_src = c.take(vchip::bucket_src_bytes(ix->t.groups16, nq)), o_ps2 = c.take(items * k * 4),
                o_pi = c.take(items * k * 4), o_pc = c.take(items * 4);
+  size_t o_tile[11] = {};
+  if (tiles) {
+    const size_t sizes[11] = {size_t(L), size_t(L) + 1, size_t(L) + 1, 1, nq * P, nq * P, nq * P,
+                              nq * P,    nq * P,        nq * P,        nq};
+    for (int i = 0; i < 11; ++i) o_tile[i] = c.take(sizes[i] * 4);
+  }
   if (need != nullptr) *need = c.off;
   uint8_t* b = take(ctx, c.off);
   SearchBufs r;
@@ -512,6 +536,10 @@ SearchBufs carve_search(vc_bucketed* ix, size_t nq, int P, int k, uint8_t* (*tak
   r.part_i = reinterpret_cast<uint32_t*>(b + o_pi);
   r.part_cnt = reinterpret_cast<int*>(b + o_pc);
   r.route_chunk = chunk;
+  r.tiles = tiles;
+  int** tp[11] = {&r.bcnt, &r.bstart, &r.tstart, &r.n_tiles, &r.prank, &r.pair_q,
+                  &r.pair_j, &r.tile_b, &r.tile_p0, &r.tile_n, &r.gbound};
+  for (int i = 0; i < 11; ++i) *tp[i] = tiles ? reinterpret_cast<int*>(b + o_tile[i]) : nullptr;
   return r;
 }
 
@@ -540,6 +568,55 @@ void route(vc_bucketed* ix, const float* q_dev, size_t nq, int P, const SearchBu
 size_t search_probed(vc_bucketed* ix, const float* q_dev, size_t nq, size_t k, int P, const SearchBufs& b,
                      float* out_s, uint64_t* out_i, uint64_t* counts, hipStream_t s) {
   const int ke = int(std::min(k, ix->n));
+  if (b.tiles) {
+    hip_check(vchip::launch_probe_offsets(b.qhat, nq, int(ix->dim), ix->prm.residual ? ix->cent.as<float>() : nullptr,
+                                          P, b.probe_b, b.probe_n, b.probe_off, s),
+              "probe offsets");
+    hip_check(vchip::launch_prepare_queries(ix->t, q_dev, nq, b.luts, b.ls, b.lb, b.bc, b.bt, s), "prepare_queries");
+    vchip::BucketTileParams p{};
+    p.tiled = ix->tiled.as<uint8_t>();
+    p.alpha = ix->alpha.as<float>();
+    p.G = ix->t.groups;
+    p.G16 = ix->t.groups16;
+    p.luts = b.luts;
+    p.lut_scale = b.ls;
+    p.bias_total = b.bt;
+    p.nq = long(nq);
+    p.P = P;
+    p.probe_n = b.probe_n;
+    p.probe_b = b.probe_b;
+    p.probe_off = b.probe_off;
+    p.boff = ix->boff.as<long>();
+    p.bsize = ix->bsize.as<int>();
+    p.L = ix->L;
+    p.k = ke;
+    p.bcnt = b.bcnt;
+    p.bstart = b.bstart;
+    p.tstart = b.tstart;
+    p.n_tiles = b.n_tiles;
+    p.prank = b.prank;
+    p.pair_q = b.pair_q;
+    p.pair_j = b.pair_j;
+    p.tile_b = b.tile_b;
+    p.tile_p0 = b.tile_p0;
+    p.tile_n = b.tile_n;
+    p.gbound = b.gbound;
+    p.part_s = b.part_s;
+    p.part_i = b.part_i;
+    p.part_cnt = b.part_cnt;
+    std::unique_lock probe_lock(g_probe.mu);
+    const bool probe = g_probe.on;
+    if (!probe) probe_lock.unlock();
+    hip_check(vchip::launch_bucket_tiles(p, ke, ix->ids.as<uint64_t>(), out_s, out_i, counts, s,
+                                         probe ? g_probe.start : nullptr, probe ? g_probe.stop : nullptr),
+              "bucket tile scan");
+    if (probe) {
+      g_probe.recorded = true;
+      g_probe.tile = vchip::kTileQueries;
+      g_probe.ranges = 0;
+    }
+    return size_t(ke);
+  }
   hip_check(vchip::launch_plan_probes(nq, P, b.probe_b, b.probe_n, ix->bsize.as<int>(), vchip::kBucketItem,
                                       vchip::kBucketMaxItems, b.pre, b.item_len, b.n_items, b.item_off, s),
             "plan probes");
@@ -603,7 +680,7 @@ size_t bucketed_search_dev(vc_bucketed* ix, const float* q_dev, size_t nq, size_
   if (probe_counts != nullptr && n_probe == 0)
     fail(VC_ERR_INVALID_ARGUMENT, "per-query probe counts need n_probe as their upper bound");
   const int P = probe_width(ix, learned ? vchip::kProbeMaxP : np, learned);
-  const SearchBufs b = carve_search(ix, nq, P, int(ke), take_growable, ix, nullptr);
+  const SearchBufs b = carve_search(ix, nq, P, int(ke), take_growable, ix, nullptr, use_tile_scan(ix, nq, P, ke));
   route(ix, q_dev, nq, P, b, s);
   if (learned) {
     hip_check(vchip::probe_counts_from_model(ix->probe, nq, P, b.probe_s, b.probe_b, ix->bsize.as<int>(), b.probe_n,

@@ -94,4 +94,48 @@ hipError_t launch_merge_items(const float* part_s, const std::uint32_t* part_i, 
                               const int* item_off, std::size_t nq, int k, int k_out, const std::uint64_t* id_map,
                               float* out_s, std::uint64_t* out_i, std::uint64_t* counts, hipStream_t s);
 
+// Bucket-major scan of a large query batch: the (query, probe) pairs grouped
+// by bucket, cut into tiles of up to kTileQueries queries of one bucket; each
+// tile reads its bucket's codes once for all its queries. Every pair (q, j)
+// gets its own exact top-k list in slot q * P + j.
+constexpr int kTileQueries = 8;
+constexpr int kTileMaxK = 64;  // list length the tile scan keeps
+struct BucketTileParams {
+  const std::uint8_t* tiled;
+  const float* alpha;
+  int G, G16;
+  const std::uint8_t* luts;  // nq x G x 32
+  const float* lut_scale;
+  const float* bias_total;
+  long nq;
+  int P;
+  const int* probe_n;
+  const int* probe_b;
+  const float* probe_off;
+  const long* boff;
+  const int* bsize;
+  int L;
+  int k, cap;
+  // planning buffers, filled by launch_bucket_tiles
+  int* bcnt;    // [L] pairs per bucket
+  int* bstart;  // [L + 1] first pair of each bucket
+  int* tstart;  // [L + 1] first tile of each bucket
+  int* n_tiles; // [1]
+  int* prank;   // [nq * P] rank of pair (q, j) within its bucket
+  int* pair_q;  // [nq * P] pairs grouped by bucket
+  int* pair_j;
+  int* tile_b;  // [nq * P] tiles: bucket, first pair, pair count
+  int* tile_p0;
+  int* tile_n;
+  int* gbound;  // [nq] per-query score bound (float_order image)
+  float* part_s;  // [nq * P * k]
+  std::uint32_t* part_i;
+  int* part_cnt;  // [nq * P]
+};
+// Plan (pairs by bucket, tiles), scan, and merge each query's probe_n[q]
+// slots into out_s / out_i (ids through id_map; counts as launch_merge_items).
+hipError_t launch_bucket_tiles(BucketTileParams p, int k_out, const std::uint64_t* id_map, float* out_s,
+                               std::uint64_t* out_i, std::uint64_t* counts, hipStream_t s, hipEvent_t scan_start,
+                               hipEvent_t scan_stop);
+
 }  // namespace vchip

@@ -1619,12 +1619,13 @@ __device__ __attribute__((noinline)) uint32_t wave_kth_largest(uint32_t u, bool 
 // the workgroup calls it (it synchronizes).
 // voff (when not null): a per-vector offset added after alpha, the reference's
 // score_offsets of score_prepared_into (src/search/search_avx2.cpp:160-214:
-// fa * alpha, then + offset, each rounded).
+// fa * alpha, then + offset, each rounded); toff (when voff is null): the
+// same offset per query instead (a tile of queries over one bucket).
 template <int T, int V>
 __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], const long (&vv)[V],
                              const bool (&valid)[V], const float* __restrict__ alpha, long q0, long nq,
                              int k, int cap, const float (*voff)[V] = nullptr,
-                             const float (*av_pre)[V] = nullptr) {
+                             const float (*av_pre)[V] = nullptr, const float* toff = nullptr) {
   const int lane = threadIdx.x & 63;
   const int kw = min(k, 64);
 #pragma unroll
@@ -1636,7 +1637,8 @@ __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], cons
     uint32_t pend = 0u, candm = 0u;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-      s[t] = valid[j] ? fa[j][t] * av + (voff != nullptr ? (*voff)[j] : 0.f) : -INFINITY;
+      s[t] = valid[j] ? fa[j][t] * av + (voff != nullptr ? (*voff)[j] : toff != nullptr ? toff[t] : 0.f)
+                      : -INFINITY;
       sid[t] = vid;
       // below the bound: worse than k vectors already offered (see below)
       const bool cand = valid[j] && q0 + t < nq && better(s[t], vid, sh.thr_s[t], sh.thr_i[t]) &&
@@ -2741,6 +2743,324 @@ __global__ __launch_bounds__(64) void merge_items_kernel(const float* __restrict
   for (long q = blockIdx.x; q < nq; q += gridDim.x) {
     const int f = item_off[q], R = item_off[q + 1] - f;
     merge_query(part_s + (long)f * k, part_i + (long)f * k, part_cnt + f, R, k, k_out, id_map, out_s + q * k_out,
+                out_i + q * k_out, threadIdx.x);
+    if (counts != nullptr) {
+      int c = 0;
+      for (int r = threadIdx.x; r < R; r += 64) c += part_cnt[f + r];
+      for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+      if (threadIdx.x == 0) counts[q] = (uint64_t)min(c, k_out);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Bucket-major scan of a large query batch (the multi-bucket plumbing of
+// score_prepared_into, src/search/search.cpp:1156-1267, turned inside out).
+// The pairs (q, j), j < probe_n[q], are grouped by their bucket
+// probe_b[q][j] and cut into tiles of up to kTileQueries queries of one
+// bucket. A tile reads its bucket's codes once for all its queries: the
+// tile's LUTs sit in LDS interleaved by query (entry (g, e, t) at byte
+// g * kTileGroupStride + e * 8 + t), so one 64-bit read fetches a nibble's
+// entry for all eight queries (two reads per code byte, no pair table to
+// build per tile). The window sums, the float chain and the score
+// fa * alpha + offset are those of the query-major scan, and each pair keeps
+// its own exact top-k list in slot q * P + j.
+// A per-query bound in HBM (the float_order image of the k-th score of any
+// full list written so far) seeds each tile's admission bound: a vector
+// scoring below it is worse than the k vectors of that list.
+// ---------------------------------------------------------------------------
+constexpr int kTileChunkBlocks = 16;                      // 256 groups = one flush window per table chunk
+constexpr int kTileGroupStride = 264;                     // 256 bytes of entries + 8: spreads the fill over the banks
+constexpr int kTileTableBytes = kTileChunkBlocks * 16 * kTileGroupStride;
+#ifndef VC_TILE_FILL_BATCH
+#define VC_TILE_FILL_BATCH 2
+#endif
+#ifndef VC_TILE_RING
+#define VC_TILE_RING 2
+#endif
+constexpr int kTileFillBatch = VC_TILE_FILL_BATCH;  // fill tasks whose loads are in flight together
+
+// byte E of w shifted right by SH, one SDWA instruction
+template <int E, int SH>
+__device__ __forceinline__ uint32_t byte_shr(uint32_t w) {
+  static_assert(E >= 0 && E < 4 && SH > 0 && SH < 8, "byte and shift");
+  uint32_t r;
+  if constexpr (E == 0) {
+    asm("v_lshrrev_b32_sdwa %0, %2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0"
+        : "=v"(r) : "v"(w), "i"(SH));
+  } else if constexpr (E == 1) {
+    asm("v_lshrrev_b32_sdwa %0, %2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_1"
+        : "=v"(r) : "v"(w), "i"(SH));
+  } else if constexpr (E == 2) {
+    asm("v_lshrrev_b32_sdwa %0, %2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_2"
+        : "=v"(r) : "v"(w), "i"(SH));
+  } else {
+    asm("v_lshrrev_b32_sdwa %0, %2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_3"
+        : "=v"(r) : "v"(w), "i"(SH));
+  }
+  return r;
+}
+
+// bcnt[b] = 0 (b < L), gbound[q] = the image of -inf (q < nq)
+__global__ void tile_init_kernel(int L, int* __restrict__ bcnt, long nq, int* __restrict__ gbound) {
+  const long n = max((long)L, nq);
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    if (i < L) bcnt[i] = 0;
+    if (i < nq) gbound[i] = float_order(-INFINITY);
+  }
+}
+
+// pairs per bucket; prank[q * P + j] = the pair's rank among its bucket's
+__global__ void tile_count_kernel(long nq, int P, const int* __restrict__ probe_b, const int* __restrict__ probe_n,
+                                  int* __restrict__ bcnt, int* __restrict__ prank) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nq * P; i += (long)gridDim.x * blockDim.x) {
+    const long q = i / P;
+    if (i - q * P < probe_n[q]) prank[i] = atomicAdd(&bcnt[probe_b[i]], 1);
+  }
+}
+
+// Exclusive prefix sums of the pair counts (bstart) and tile counts (tstart)
+// of the L buckets; one workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void tile_scan_kernel(int L, const int* __restrict__ bcnt, int* __restrict__ bstart,
+                                                         int* __restrict__ tstart, int* __restrict__ n_tiles) {
+  __shared__ int wsum[2][16];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int run_p = 0, run_t = 0;
+  for (int b0 = 0; b0 < L; b0 += 1024) {
+    const int b = b0 + tid;
+    const int c = b < L ? bcnt[b] : 0;
+    int vp = c, vt = (c + kTileQueries - 1) / kTileQueries;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int up = __shfl_up(vp, o), ut = __shfl_up(vt, o);
+      if (lane >= o) { vp += up; vt += ut; }
+    }
+    if (lane == 63) { wsum[0][w] = vp; wsum[1][w] = vt; }
+    __syncthreads();
+    int bp = run_p, bt = run_t;
+    for (int i = 0; i < w; ++i) { bp += wsum[0][i]; bt += wsum[1][i]; }
+    if (b < L) {
+      bstart[b] = bp + vp - c;
+      tstart[b] = bt + vt - (c + kTileQueries - 1) / kTileQueries;
+    }
+    for (int i = 0; i < 16; ++i) { run_p += wsum[0][i]; run_t += wsum[1][i]; }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    bstart[L] = run_p;
+    tstart[L] = run_t;
+    *n_tiles = run_t;
+  }
+}
+
+// pair lists grouped by bucket, and each bucket's tiles
+__global__ void tile_place_kernel(long nq, int P, const int* __restrict__ probe_b, const int* __restrict__ probe_n,
+                                  const int* __restrict__ prank, const int* __restrict__ bstart,
+                                  int* __restrict__ pair_q, int* __restrict__ pair_j) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nq * P; i += (long)gridDim.x * blockDim.x) {
+    const long q = i / P;
+    const int j = (int)(i - q * P);
+    if (j < probe_n[q]) {
+      const int at = bstart[probe_b[i]] + prank[i];
+      pair_q[at] = (int)q;
+      pair_j[at] = j;
+    }
+  }
+}
+
+__global__ void tile_list_kernel(int L, const int* __restrict__ bcnt, const int* __restrict__ bstart,
+                                 const int* __restrict__ tstart, int* __restrict__ tile_b, int* __restrict__ tile_p0,
+                                 int* __restrict__ tile_n) {
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < L; b += gridDim.x * blockDim.x) {
+    const int c = bcnt[b], t0 = tstart[b], p0 = bstart[b];
+    for (int i = 0; i * kTileQueries < c; ++i) {
+      tile_b[t0 + i] = b;
+      tile_p0[t0 + i] = p0 + i * kTileQueries;
+      tile_n[t0 + i] = min(kTileQueries, c - i * kTileQueries);
+    }
+  }
+}
+
+// Fill the table with groups [16 * 16 c, 16 * 16 (c + 1)) of the tile's
+// eight LUT rows: 4096 tasks of (group, query quad, entry quad), each four
+// 32-bit loads (one per query) and a 4 x 4 byte transpose. Groups past G
+// (the padding of the last code block) get zero entries.
+__device__ __forceinline__ void tile_fill(const BucketTileParams& a, const int* lq, int c, uint8_t* table) {
+  const int tid = threadIdx.x;
+  const int g0 = c * kTileChunkBlocks * 16;
+#pragma unroll
+  for (int r0 = 0; r0 < 4; r0 += kTileFillBatch) {
+  uint32_t w[kTileFillBatch][4];
+#pragma unroll
+  for (int r = 0; r < kTileFillBatch; ++r) {
+    const int task = tid + (r0 + r) * kScanThreads;
+    const int e4 = task & 7, t4 = (task >> 3) & 1, g = g0 + (task >> 4);
+    const int gc = min(g, a.G - 1);  // clamped: rows past G are loaded but not stored
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      w[r][i] = *reinterpret_cast<const uint32_t*>(a.luts + (long)lq[4 * t4 + i] * a.G * 32 + (long)gc * 32 + e4 * 4);
+  }
+#pragma unroll
+  for (int r = 0; r < kTileFillBatch; ++r) {
+    const int task = tid + (r0 + r) * kScanThreads;
+    const int e4 = task & 7, t4 = (task >> 3) & 1, gl = task >> 4;
+    if (g0 + gl < a.G16 * 16) {
+      if (g0 + gl >= a.G) w[r][0] = w[r][1] = w[r][2] = w[r][3] = 0u;
+      // w[i] = entries 4 e4 .. 4 e4 + 3 of query 4 t4 + i; out[j] = entry 4 e4 + j of the four queries
+      const uint32_t t01l = __builtin_amdgcn_perm(w[r][1], w[r][0], 0x05010400u);
+      const uint32_t t01h = __builtin_amdgcn_perm(w[r][1], w[r][0], 0x07030602u);
+      const uint32_t t23l = __builtin_amdgcn_perm(w[r][3], w[r][2], 0x05010400u);
+      const uint32_t t23h = __builtin_amdgcn_perm(w[r][3], w[r][2], 0x07030602u);
+      uint32_t* dst = reinterpret_cast<uint32_t*>(table + gl * kTileGroupStride + e4 * 32 + t4 * 4);
+      dst[0] = __builtin_amdgcn_perm(t23l, t01l, 0x05040100u);
+      dst[2] = __builtin_amdgcn_perm(t23l, t01l, 0x07060302u);
+      dst[4] = __builtin_amdgcn_perm(t23h, t01h, 0x05040100u);
+      dst[6] = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u);
+    }
+  }
+  }
+}
+
+__device__ __forceinline__ uint2 add64(uint2 x, uint2 y) {
+  uint64_t r;
+  const uint64_t a = ((uint64_t)x.y << 32) | x.x, b = ((uint64_t)y.y << 32) | y.x;
+  asm("v_lshl_add_u64 %0, %1, 0, %2" : "=v"(r) : "v"(a), "v"(b));
+  return make_uint2((uint32_t)r, (uint32_t)(r >> 32));
+}
+
+__global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void bucket_tile_kernel(BucketTileParams a) {
+  constexpr int T = kTileQueries;
+  constexpr int NR = VC_TILE_RING;  // code blocks in flight per thread
+  static_assert(T == 8, "eight queries per 64-bit entry");
+  __shared__ __align__(16) uint8_t table[kTileTableBytes];
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x;
+  const int G16 = a.G16;
+  const int n_chunks = (G16 + kTileChunkBlocks - 1) / kTileChunkBlocks;
+  const size_t tk = ((size_t)T * (2 * (size_t)a.k + 2 * (size_t)a.cap) * 4 + 20 * T + 15) & ~size_t(15);
+  int* lq = reinterpret_cast<int*>(smem + tk);  // [T] query of each tile slot
+  int* lslot = lq + T;                          // [T] its pair's list slot
+  float* lscl = reinterpret_cast<float*>(lslot + T);
+  float* lbias = lscl + T;
+  float* loff = lbias + T;
+  const int n_tiles = __builtin_amdgcn_readfirstlane(*a.n_tiles);
+  for (int it = blockIdx.x; it < n_tiles; it += gridDim.x) {
+    const int b = __builtin_amdgcn_readfirstlane(a.tile_b[it]);
+    const int p0 = __builtin_amdgcn_readfirstlane(a.tile_p0[it]);
+    const int nt = __builtin_amdgcn_readfirstlane(a.tile_n[it]);
+    const int nv = __builtin_amdgcn_readfirstlane(a.bsize[b]);
+    const long pos0 = a.boff[b];
+    __syncthreads();  // the previous tile's lists are written out
+    TopkShared sh;
+    init_topk(sh, smem, T, a.k, a.cap);
+    if (tid < T) {
+      // slots past nt repeat the last query: scanned, never admitted
+      const int pi = p0 + min(tid, nt - 1);
+      const int q = a.pair_q[pi], j = a.pair_j[pi];
+      lq[tid] = q;
+      lslot[tid] = q * a.P + j;
+      lscl[tid] = a.lut_scale[q];
+      lbias[tid] = a.bias_total[q];
+      loff[tid] = a.probe_off[(long)q * a.P + j];
+      if (tid < nt) sh.bound[tid] = a.gbound[q];  // after init_topk's reset by this same thread
+    }
+    __syncthreads();
+    for (int vp0 = 0; vp0 < nv; vp0 += kScanThreads) {
+      const int i = vp0 + tid;
+      const bool valid = i < nv;
+      const bool live = vp0 + (tid & ~63) < nv;  // uniform: the wavefront has a vector
+      const long v = pos0 + (valid ? i : 0);
+      const uint4* cp = reinterpret_cast<const uint4*>(a.tiled) + (v >> 6) * (long)G16 * 64 + (v & 63);
+      const float av = a.alpha[v];
+      float fa[1][T];
+#pragma unroll
+      for (int t = 0; t < T; ++t) fa[0][t] = lbias[t];
+      uint32_t accA[2] = {0u, 0u}, accB[2] = {0u, 0u};
+      uint4 ring[NR];
+#pragma unroll
+      for (int u = 0; u < NR; ++u) ring[u] = cp[(long)min(u, G16 - 1) * 64];
+      for (int c = 0; c < n_chunks; ++c) {
+        __syncthreads();  // the previous chunk's lookups are done
+        tile_fill(a, lq, c, table);
+        __syncthreads();
+        if (!live) continue;
+#pragma unroll
+        for (int u = 0; u < kTileChunkBlocks; ++u) {
+          const int bb = c * kTileChunkBlocks + u;
+          if (bb < G16) {  // uniform
+            const uint4 cw = ring[u % NR];
+            const uint32_t wl[4] = {cw.x & 0x0f0f0f0fu, cw.y & 0x0f0f0f0fu, cw.z & 0x0f0f0f0fu, cw.w & 0x0f0f0f0fu};
+            const uint32_t wh[4] = {cw.x & 0xf0f0f0f0u, cw.y & 0xf0f0f0f0u, cw.z & 0xf0f0f0f0u, cw.w & 0xf0f0f0f0u};
+            const uint8_t* tb = table + u * 16 * kTileGroupStride;
+            static_for<16>([&](auto ec) {
+              constexpr int e = decltype(ec)::value;
+              const uint8_t* gp = tb + e * kTileGroupStride;
+              // entries 0-15: the high nibble (first code); 16-31: the low one
+              const uint2 rh = *reinterpret_cast<const uint2*>(gp + byte_shr<e & 3, 1>(wh[e >> 2]));
+              const uint2 rl = *reinterpret_cast<const uint2*>(gp + 128 + byte_scaled<e & 3, 3>(wl[e >> 2]));
+              // bytes of each entry are <= 127: the byte sums have no carries
+              const uint2 w2 = add64(rh, rl);
+              // accA: sum over q of 2^(8q) S_q (mod 2^64); accB: the odd
+              // queries' bytes alone as u16 lanes (two v_perm, 0x0c = zero)
+              acc64_add(accA, w2);
+              acc64_add<true>(accB, make_uint2(__builtin_amdgcn_perm(0u, w2.x, 0x0c030c01u),
+                                               __builtin_amdgcn_perm(0u, w2.y, 0x0c030c01u)));
+            });
+          }
+          ring[u % NR] = cp[(long)min(bb + NR, G16 - 1) * 64];  // clamped, in bounds
+        }
+        // window flush (a chunk is one 256-group window): every window sum is
+        // < 2^16, so accA - 2^8 accB = sum over i of 2^(16i) S_2i exactly
+        const uint64_t a64 = ((uint64_t)accA[1] << 32) | accA[0];
+        const uint64_t b64 = ((uint64_t)accB[1] << 32) | accB[0];
+        const uint64_t e64 = a64 - (b64 << 8);
+        const uint32_t evw[2] = {(uint32_t)e64, (uint32_t)(e64 >> 32)};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const u16x2 od = __builtin_bit_cast(u16x2, accB[h]);  // queries 4h + 1, 4h + 3
+          const u16x2 ev = __builtin_bit_cast(u16x2, evw[h]);   // queries 4h, 4h + 2
+          fa[0][4 * h + 0] = __fmaf_rn(lscl[4 * h + 0], (float)ev.x, fa[0][4 * h + 0]);
+          fa[0][4 * h + 1] = __fmaf_rn(lscl[4 * h + 1], (float)od.x, fa[0][4 * h + 1]);
+          fa[0][4 * h + 2] = __fmaf_rn(lscl[4 * h + 2], (float)ev.y, fa[0][4 * h + 2]);
+          fa[0][4 * h + 3] = __fmaf_rn(lscl[4 * h + 3], (float)od.y, fa[0][4 * h + 3]);
+        }
+        accA[0] = accA[1] = accB[0] = accB[1] = 0u;
+      }
+      const long vv[1] = {v};
+      const bool vl[1] = {valid};
+      const float avp[1] = {av};
+      admit_scores<T, 1>(sh, fa, vv, vl, a.alpha, 0, nt, a.k, a.cap, nullptr, &avp, loff);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+      if (__builtin_amdgcn_readfirstlane(sh.cnt[t]) > 0) topk_merge(sh, t, a.k, a.cap);
+    for (int t = 0; t < nt; ++t) {
+      const long slot = lslot[t];
+      const int m = sh.m[t];
+      for (int j = tid; j < m; j += blockDim.x) {
+        a.part_s[slot * a.k + j] = sh.top_s[t * a.k + j];
+        a.part_i[slot * a.k + j] = sh.top_i[t * a.k + j];
+      }
+      if (tid == 0) {
+        a.part_cnt[slot] = m;
+        if (m == a.k) atomicMax(&a.gbound[lq[t]], float_order(sh.top_s[t * a.k + a.k - 1]));
+      }
+    }
+  }
+}
+
+// One wavefront per query: merge its probe_n[q] slot lists (slots q * P ..).
+__global__ __launch_bounds__(64) void merge_slots_kernel(const float* __restrict__ part_s,
+                                                         const uint32_t* __restrict__ part_i,
+                                                         const int* __restrict__ part_cnt,
+                                                         const int* __restrict__ probe_n, int P, long nq, int k,
+                                                         int k_out, const uint64_t* __restrict__ id_map,
+                                                         float* __restrict__ out_s, uint64_t* __restrict__ out_i,
+                                                         uint64_t* __restrict__ counts) {
+  for (long q = blockIdx.x; q < nq; q += gridDim.x) {
+    const long f = q * P;
+    const int R = probe_n[q];
+    merge_query(part_s + f * k, part_i + f * k, part_cnt + f, R, k, k_out, id_map, out_s + q * k_out,
                 out_i + q * k_out, threadIdx.x);
     if (counts != nullptr) {
       int c = 0;
@@ -4703,6 +5023,38 @@ hipError_t launch_merge_items(const float* part_s, const std::uint32_t* part_i, 
   if (nq == 0 || k_out == 0) return hipSuccess;
   hipLaunchKernelGGL(merge_items_kernel, dim3(grid_for((long)nq, 1, 65536)), dim3(64), 0, s, part_s, part_i,
                      part_cnt, item_off, (long)nq, k, k_out, id_map, out_s, out_i, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_bucket_tiles(BucketTileParams p, int k_out, const std::uint64_t* id_map, float* out_s,
+                               std::uint64_t* out_i, std::uint64_t* counts, hipStream_t s, hipEvent_t scan_start,
+                               hipEvent_t scan_stop) {
+  if (p.nq == 0 || k_out == 0) return hipSuccess;
+  if (p.k < 1 || p.k > kTileMaxK || p.G > p.G16 * 16 || p.L < 1 || p.P < 1 || p.P > 512)
+    return hipErrorInvalidValue;
+  const long pairs = p.nq * p.P;
+  hipLaunchKernelGGL(tile_init_kernel, dim3(grid_for(std::max<long>(p.L, p.nq), 256)), dim3(256), 0, s, p.L, p.bcnt,
+                     p.nq, p.gbound);
+  hipLaunchKernelGGL(tile_count_kernel, dim3(grid_for(pairs, 256)), dim3(256), 0, s, p.nq, p.P, p.probe_b, p.probe_n,
+                     p.bcnt, p.prank);
+  hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, s, p.L, p.bcnt, p.bstart, p.tstart, p.n_tiles);
+  hipLaunchKernelGGL(tile_place_kernel, dim3(grid_for(pairs, 256)), dim3(256), 0, s, p.nq, p.P, p.probe_b, p.probe_n,
+                     p.prank, p.bstart, p.pair_q, p.pair_j);
+  hipLaunchKernelGGL(tile_list_kernel, dim3(grid_for(p.L, 256)), dim3(256), 0, s, p.L, p.bcnt, p.bstart, p.tstart,
+                     p.tile_b, p.tile_p0, p.tile_n);
+  // LDS: the table, the top-k state of eight queries and the slot data; the
+  // candidate buffer takes what is left of half a CU (two workgroups per CU)
+  const long room = (long)kLdsBudget / 2 - (long)kTileTableBytes - (long)topk_lds(kTileQueries, p.k, 0) -
+                    5 * 4 * kTileQueries - 16;
+  p.cap = (int)std::min<long>(kCandCap, room / (8 * kTileQueries) / 32 * 32);
+  if (p.cap < 64) return hipErrorInvalidValue;
+  const size_t dyn = ((topk_lds(kTileQueries, p.k, p.cap) + 15) & ~size_t(15)) + 5 * 4 * kTileQueries;
+  (void)hipFuncSetAttribute((const void*)bucket_tile_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  if (scan_start != nullptr) (void)hipEventRecord(scan_start, s);
+  hipLaunchKernelGGL(bucket_tile_kernel, dim3(kBucketScanGrid), dim3(kScanThreads), dyn, s, p);
+  if (scan_stop != nullptr) (void)hipEventRecord(scan_stop, s);
+  hipLaunchKernelGGL(merge_slots_kernel, dim3(grid_for(p.nq, 1, 65536)), dim3(64), 0, s, p.part_s, p.part_i,
+                     p.part_cnt, p.probe_n, p.P, p.nq, p.k, k_out, id_map, out_s, out_i, counts);
   return hipGetLastError();
 }
 
