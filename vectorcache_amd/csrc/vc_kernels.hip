@@ -1559,6 +1559,14 @@ __device__ void topk_merge(const TopkShared& sh, int t, int k, int cap) {
   __syncthreads();
 }
 
+// topk_merge for every query of a tile at once: wavefront t takes query t's
+// list and candidates into registers (at most 256 of them, four per lane)
+// and writes the merged list back by repeated wavefront argmax under
+// (score desc, id asc). One barrier in all; the queries merge in parallel.
+// Needs T <= the wavefronts of the workgroup and k + cap <= 256. Defined
+// after wave_best (below).
+__device__ void topk_merge_all(const TopkShared& sh, int T, int k, int cap);
+
 // Carve the top-k state out of LDS at `base` and reset it (threads < T write;
 // the caller synchronizes before first use).
 __device__ __forceinline__ void init_topk(TopkShared& sh, unsigned char* base, int T, int k, int cap) {
@@ -1632,14 +1640,20 @@ __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], cons
   for (int j = 0; j < V; ++j) {
     float s[T];
     uint32_t sid[T];
-    const float av = valid[j] ? (av_pre != nullptr ? (*av_pre)[j] : alpha[vv[j]]) : 0.f;
     const uint32_t vid = (uint32_t)vv[j];
     uint32_t pend = 0u, candm = 0u;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
+      s[t] = -INFINITY;
+      sid[t] = vid;
+    }
+    // a wavefront without vectors (the tail of a range) goes straight to the barriers
+    if (__ballot(valid[j]) != 0) {
+    const float av = valid[j] ? (av_pre != nullptr ? (*av_pre)[j] : alpha[vv[j]]) : 0.f;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
       s[t] = valid[j] ? fa[j][t] * av + (voff != nullptr ? (*voff)[j] : toff != nullptr ? toff[t] : 0.f)
                       : -INFINITY;
-      sid[t] = vid;
       // below the bound: worse than k vectors already offered (see below)
       const bool cand = valid[j] && q0 + t < nq && better(s[t], vid, sh.thr_s[t], sh.thr_i[t]) &&
                         !(s[t] < order_float(sh.bound[t]));
@@ -1679,10 +1693,11 @@ __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], cons
         pend |= 1u << t;
       }
     }
+    }
     while (true) {
-      // every lane takes part in the atomic (adding 0 when it has nothing to
-      // store): the loop then has no lane-dependent control flow around its
-      // barriers, only the uniform exit below
+      // the branches here are uniform (a ballot, or counts read by every
+      // thread): no lane-dependent control flow around the barriers
+      if (__ballot(pend != 0u) != 0)  // uniform: a wavefront with nothing to store skips the atomics
 #pragma unroll
       for (int t = 0; t < T; ++t) {
         const bool want = ((pend >> t) & 1u) != 0u;
@@ -2730,6 +2745,64 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void bucket_scan_k
   }
 }
 
+__device__ void topk_merge_all(const TopkShared& sh, int T, int k, int cap) {
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (w < T) {
+    const int t = w;
+    const int c = __builtin_amdgcn_readfirstlane(min(sh.cnt[t], cap));
+    const int m = __builtin_amdgcn_readfirstlane(sh.m[t]);
+    if (c > 0) {
+      float* ts = sh.top_s + t * k;
+      uint32_t* ti = sh.top_i + t * k;
+      float es[4];
+      uint32_t ei[4];
+      bool have[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int e = lane + 64 * r;
+        have[r] = e < m + c;
+        const int ec = min(max(e - m, 0), cap - 1);
+        es[r] = e < m ? ts[min(e, k - 1)] : sh.cand_s[t * cap + ec];
+        ei[r] = e < m ? ti[min(e, k - 1)] : sh.cand_i[t * cap + ec];
+      }
+      const int m2 = min(k, m + c);
+      for (int r = 0; r < m2; ++r) {
+        bool h = false;
+        float bs = 0.f;
+        uint32_t bi = 0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (have[j] && (!h || better(es[j], ei[j], bs, bi))) {
+            h = true;
+            bs = es[j];
+            bi = ei[j];
+          }
+        bool wh = h;
+        float ws = bs;
+        uint32_t wi = bi;
+        wave_best(wh, ws, wi);
+        // the ids of a query's entries are distinct: exactly one lane holds the winner
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (have[j] && es[j] == ws && ei[j] == wi) have[j] = false;
+        if (lane == 0) {
+          ts[r] = ws;
+          ti[r] = wi;
+        }
+      }
+      if (lane == 0) {
+        sh.m[t] = m2;
+        sh.cnt[t] = 0;
+        if (m2 == k) {
+          sh.thr_s[t] = ts[k - 1];
+          sh.thr_i[t] = ti[k - 1];
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // One wavefront per query: merge its items' lists (at most 512) into k_out
 // entries, positions mapped through id_map; counts[q] (when set) = entries
 // found, the rest of the row (0, 0).
@@ -2772,6 +2845,13 @@ __global__ __launch_bounds__(64) void merge_items_kernel(const float* __restrict
 constexpr int kTileChunkBlocks = 16;                      // 256 groups = one flush window per table chunk
 constexpr int kTileGroupStride = 264;                     // 256 bytes of entries + 8: spreads the fill over the banks
 constexpr int kTileTableBytes = kTileChunkBlocks * 16 * kTileGroupStride;
+#ifndef VC_TILE_THREADS
+#define VC_TILE_THREADS 1024
+#endif
+// threads per tile: a bucket of up to this many vectors in one pass (two
+// workgroups per CU)
+constexpr int kTileThreads = VC_TILE_THREADS;
+constexpr int kTileFillTasks = kTileChunkBlocks * 16 * 16 / kTileThreads;  // per thread and chunk
 #ifndef VC_TILE_FILL_BATCH
 #define VC_TILE_FILL_BATCH 2
 #endif
@@ -2810,12 +2890,14 @@ __global__ void tile_init_kernel(int L, int* __restrict__ bcnt, long nq, int* __
   }
 }
 
-// pairs per bucket; prank[q * P + j] = the pair's rank among its bucket's
-__global__ void tile_count_kernel(long nq, int P, const int* __restrict__ probe_b, const int* __restrict__ probe_n,
-                                  int* __restrict__ bcnt, int* __restrict__ prank) {
+// pairs (q, j), j_lo <= j < min(j_hi, probe_n[q]), per bucket; prank[q * P + j]
+// = the pair's rank among its bucket's
+__global__ void tile_count_kernel(long nq, int P, int j_lo, int j_hi, const int* __restrict__ probe_b,
+                                  const int* __restrict__ probe_n, int* __restrict__ bcnt, int* __restrict__ prank) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nq * P; i += (long)gridDim.x * blockDim.x) {
     const long q = i / P;
-    if (i - q * P < probe_n[q]) prank[i] = atomicAdd(&bcnt[probe_b[i]], 1);
+    const int j = (int)(i - q * P);
+    if (j >= j_lo && j < j_hi && j < probe_n[q]) prank[i] = atomicAdd(&bcnt[probe_b[i]], 1);
   }
 }
 
@@ -2853,13 +2935,13 @@ __global__ __launch_bounds__(1024) void tile_scan_kernel(int L, const int* __res
 }
 
 // pair lists grouped by bucket, and each bucket's tiles
-__global__ void tile_place_kernel(long nq, int P, const int* __restrict__ probe_b, const int* __restrict__ probe_n,
-                                  const int* __restrict__ prank, const int* __restrict__ bstart,
-                                  int* __restrict__ pair_q, int* __restrict__ pair_j) {
+__global__ void tile_place_kernel(long nq, int P, int j_lo, int j_hi, const int* __restrict__ probe_b,
+                                  const int* __restrict__ probe_n, const int* __restrict__ prank,
+                                  const int* __restrict__ bstart, int* __restrict__ pair_q, int* __restrict__ pair_j) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nq * P; i += (long)gridDim.x * blockDim.x) {
     const long q = i / P;
     const int j = (int)(i - q * P);
-    if (j < probe_n[q]) {
+    if (j >= j_lo && j < j_hi && j < probe_n[q]) {
       const int at = bstart[probe_b[i]] + prank[i];
       pair_q[at] = (int)q;
       pair_j[at] = j;
@@ -2881,18 +2963,18 @@ __global__ void tile_list_kernel(int L, const int* __restrict__ bcnt, const int*
 }
 
 // Fill the table with groups [16 * 16 c, 16 * 16 (c + 1)) of the tile's
-// eight LUT rows: 4096 tasks of (group, query quad, entry quad), each four
-// 32-bit loads (one per query) and a 4 x 4 byte transpose. Groups past G
+// eight LUT rows: 4096 tasks of (group, query quad, entry quad), eight per
+// thread, each four 32-bit loads (one per query) and a 4 x 4 byte transpose. Groups past G
 // (the padding of the last code block) get zero entries.
 __device__ __forceinline__ void tile_fill(const BucketTileParams& a, const int* lq, int c, uint8_t* table) {
   const int tid = threadIdx.x;
   const int g0 = c * kTileChunkBlocks * 16;
 #pragma unroll
-  for (int r0 = 0; r0 < 4; r0 += kTileFillBatch) {
+  for (int r0 = 0; r0 < kTileFillTasks; r0 += kTileFillBatch) {
   uint32_t w[kTileFillBatch][4];
 #pragma unroll
   for (int r = 0; r < kTileFillBatch; ++r) {
-    const int task = tid + (r0 + r) * kScanThreads;
+    const int task = tid + (r0 + r) * kTileThreads;
     const int e4 = task & 7, t4 = (task >> 3) & 1, g = g0 + (task >> 4);
     const int gc = min(g, a.G - 1);  // clamped: rows past G are loaded but not stored
 #pragma unroll
@@ -2901,7 +2983,7 @@ __device__ __forceinline__ void tile_fill(const BucketTileParams& a, const int* 
   }
 #pragma unroll
   for (int r = 0; r < kTileFillBatch; ++r) {
-    const int task = tid + (r0 + r) * kScanThreads;
+    const int task = tid + (r0 + r) * kTileThreads;
     const int e4 = task & 7, t4 = (task >> 3) & 1, gl = task >> 4;
     if (g0 + gl < a.G16 * 16) {
       if (g0 + gl >= a.G) w[r][0] = w[r][1] = w[r][2] = w[r][3] = 0u;
@@ -2927,7 +3009,8 @@ __device__ __forceinline__ uint2 add64(uint2 x, uint2 y) {
   return make_uint2((uint32_t)r, (uint32_t)(r >> 32));
 }
 
-__global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void bucket_tile_kernel(BucketTileParams a) {
+__global__ __launch_bounds__(kTileThreads, kTileThreads == 1024 ? VC_PAIR_MIN_WAVES : 4) void bucket_tile_kernel(
+    BucketTileParams a) {
   constexpr int T = kTileQueries;
   constexpr int NR = VC_TILE_RING;  // code blocks in flight per thread
   static_assert(T == 8, "eight queries per 64-bit entry");
@@ -2943,19 +3026,41 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void bucket_tile_k
   float* lbias = lscl + T;
   float* loff = lbias + T;
   const int n_tiles = __builtin_amdgcn_readfirstlane(*a.n_tiles);
-  for (int it = blockIdx.x; it < n_tiles; it += gridDim.x) {
-    const int b = __builtin_amdgcn_readfirstlane(a.tile_b[it]);
-    const int p0 = __builtin_amdgcn_readfirstlane(a.tile_p0[it]);
-    const int nt = __builtin_amdgcn_readfirstlane(a.tile_n[it]);
+  // Tile descriptors are loaded two tiles ahead and the pairs one tile ahead,
+  // so no load round trip stands between two tiles. (Slots past a tile's
+  // count repeat its last pair.)
+  const int G = gridDim.x;
+  auto desc = [&](int it, int& db, int& dp0, int& dn) {
+    db = 0; dp0 = 0; dn = 1;
+    if (it < n_tiles) { db = a.tile_b[it]; dp0 = a.tile_p0[it]; dn = a.tile_n[it]; }
+  };
+  auto pair = [&](int it, int dp0, int dn, int& pq, int& pj) {
+    pq = 0; pj = 0;
+    if (it < n_tiles && tid < T) {
+      const int pi = dp0 + min(tid, dn - 1);
+      pq = a.pair_q[pi];
+      pj = a.pair_j[pi];
+    }
+  };
+  int cur_b, cur_p0, cur_n, cur_q, cur_j, nx_b, nx_p0, nx_n;
+  desc(blockIdx.x, cur_b, cur_p0, cur_n);
+  pair(blockIdx.x, cur_p0, cur_n, cur_q, cur_j);
+  desc(blockIdx.x + G, nx_b, nx_p0, nx_n);
+  for (int it = blockIdx.x; it < n_tiles; it += G) {
+    const int b = __builtin_amdgcn_readfirstlane(cur_b);
+    const int nt = __builtin_amdgcn_readfirstlane(cur_n);
+    const int q_t = cur_q, j_t = cur_j;
     const int nv = __builtin_amdgcn_readfirstlane(a.bsize[b]);
     const long pos0 = a.boff[b];
+    int nn_b, nn_p0, nn_n, nx_q, nx_j;
+    pair(it + G, nx_p0, nx_n, nx_q, nx_j);
+    desc(it + 2 * G, nn_b, nn_p0, nn_n);
     __syncthreads();  // the previous tile's lists are written out
     TopkShared sh;
     init_topk(sh, smem, T, a.k, a.cap);
     if (tid < T) {
       // slots past nt repeat the last query: scanned, never admitted
-      const int pi = p0 + min(tid, nt - 1);
-      const int q = a.pair_q[pi], j = a.pair_j[pi];
+      const int q = q_t, j = j_t;
       lq[tid] = q;
       lslot[tid] = q * a.P + j;
       lscl[tid] = a.lut_scale[q];
@@ -2964,7 +3069,7 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void bucket_tile_k
       if (tid < nt) sh.bound[tid] = a.gbound[q];  // after init_topk's reset by this same thread
     }
     __syncthreads();
-    for (int vp0 = 0; vp0 < nv; vp0 += kScanThreads) {
+    for (int vp0 = 0; vp0 < nv; vp0 += kTileThreads) {
       const int i = vp0 + tid;
       const bool valid = i < nv;
       const bool live = vp0 + (tid & ~63) < nv;  // uniform: the wavefront has a vector
@@ -2991,19 +3096,29 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void bucket_tile_k
             const uint32_t wl[4] = {cw.x & 0x0f0f0f0fu, cw.y & 0x0f0f0f0fu, cw.z & 0x0f0f0f0fu, cw.w & 0x0f0f0f0fu};
             const uint32_t wh[4] = {cw.x & 0xf0f0f0f0u, cw.y & 0xf0f0f0f0u, cw.z & 0xf0f0f0f0u, cw.w & 0xf0f0f0f0u};
             const uint8_t* tb = table + u * 16 * kTileGroupStride;
-            static_for<16>([&](auto ec) {
-              constexpr int e = decltype(ec)::value;
-              const uint8_t* gp = tb + e * kTileGroupStride;
-              // entries 0-15: the high nibble (first code); 16-31: the low one
-              const uint2 rh = *reinterpret_cast<const uint2*>(gp + byte_shr<e & 3, 1>(wh[e >> 2]));
-              const uint2 rl = *reinterpret_cast<const uint2*>(gp + 128 + byte_scaled<e & 3, 3>(wl[e >> 2]));
-              // bytes of each entry are <= 127: the byte sums have no carries
-              const uint2 w2 = add64(rh, rl);
-              // accA: sum over q of 2^(8q) S_q (mod 2^64); accB: the odd
-              // queries' bytes alone as u16 lanes (two v_perm, 0x0c = zero)
-              acc64_add(accA, w2);
-              acc64_add<true>(accB, make_uint2(__builtin_amdgcn_perm(0u, w2.x, 0x0c030c01u),
-                                               __builtin_amdgcn_perm(0u, w2.y, 0x0c030c01u)));
+            // four bytes at a time: their eight reads are issued before the
+            // first sum waits on one (the address and add instructions are
+            // written out, and the compiler keeps their order)
+            static_for<4>([&](auto wc) {
+              constexpr int w = decltype(wc)::value;
+              uint2 rh[4], rl[4];
+              static_for<4>([&](auto ec) {
+                constexpr int e = decltype(ec)::value;
+                const uint8_t* gp = tb + (4 * w + e) * kTileGroupStride;
+                // entries 0-15: the high nibble (first code); 16-31: the low one
+                rh[e] = *reinterpret_cast<const uint2*>(gp + byte_shr<e, 1>(wh[w]));
+                rl[e] = *reinterpret_cast<const uint2*>(gp + 128 + byte_scaled<e, 3>(wl[w]));
+              });
+              static_for<4>([&](auto ec) {
+                constexpr int e = decltype(ec)::value;
+                // bytes of each entry are <= 127: the byte sums have no carries
+                const uint2 w2 = add64(rh[e], rl[e]);
+                // accA: sum over q of 2^(8q) S_q (mod 2^64); accB: the odd
+                // queries' bytes alone as u16 lanes (two v_perm, 0x0c = zero)
+                acc64_add<true>(accA, w2);
+                acc64_add<true>(accB, make_uint2(__builtin_amdgcn_perm(0u, w2.x, 0x0c030c01u),
+                                                 __builtin_amdgcn_perm(0u, w2.y, 0x0c030c01u)));
+              });
             });
           }
           ring[u % NR] = cp[(long)min(bb + NR, G16 - 1) * 64];  // clamped, in bounds
@@ -3014,9 +3129,10 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void bucket_tile_k
         const uint64_t b64 = ((uint64_t)accB[1] << 32) | accB[0];
         const uint64_t e64 = a64 - (b64 << 8);
         const uint32_t evw[2] = {(uint32_t)e64, (uint32_t)(e64 >> 32)};
+        const uint32_t odw[2] = {accB[0], accB[1]};
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          const u16x2 od = __builtin_bit_cast(u16x2, accB[h]);  // queries 4h + 1, 4h + 3
+          const u16x2 od = __builtin_bit_cast(u16x2, odw[h]);   // queries 4h + 1, 4h + 3
           const u16x2 ev = __builtin_bit_cast(u16x2, evw[h]);   // queries 4h, 4h + 2
           fa[0][4 * h + 0] = __fmaf_rn(lscl[4 * h + 0], (float)ev.x, fa[0][4 * h + 0]);
           fa[0][4 * h + 1] = __fmaf_rn(lscl[4 * h + 1], (float)od.x, fa[0][4 * h + 1]);
@@ -3031,9 +3147,7 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void bucket_tile_k
       admit_scores<T, 1>(sh, fa, vv, vl, a.alpha, 0, nt, a.k, a.cap, nullptr, &avp, loff);
     }
     __syncthreads();
-#pragma unroll
-    for (int t = 0; t < T; ++t)
-      if (__builtin_amdgcn_readfirstlane(sh.cnt[t]) > 0) topk_merge(sh, t, a.k, a.cap);
+    topk_merge_all(sh, T, a.k, a.cap);
     for (int t = 0; t < nt; ++t) {
       const long slot = lslot[t];
       const int m = sh.m[t];
@@ -3046,6 +3160,8 @@ __global__ __launch_bounds__(kScanThreads, VC_PAIR_MIN_WAVES) void bucket_tile_k
         if (m == a.k) atomicMax(&a.gbound[lq[t]], float_order(sh.top_s[t * a.k + a.k - 1]));
       }
     }
+    cur_b = nx_b; cur_n = nx_n; cur_q = nx_q; cur_j = nx_j;
+    nx_b = nn_b; nx_p0 = nn_p0; nx_n = nn_n;
   }
 }
 
@@ -5033,25 +5149,25 @@ hipError_t launch_bucket_tiles(BucketTileParams p, int k_out, const std::uint64_
   if (p.k < 1 || p.k > kTileMaxK || p.G > p.G16 * 16 || p.L < 1 || p.P < 1 || p.P > 512)
     return hipErrorInvalidValue;
   const long pairs = p.nq * p.P;
-  hipLaunchKernelGGL(tile_init_kernel, dim3(grid_for(std::max<long>(p.L, p.nq), 256)), dim3(256), 0, s, p.L, p.bcnt,
-                     p.nq, p.gbound);
-  hipLaunchKernelGGL(tile_count_kernel, dim3(grid_for(pairs, 256)), dim3(256), 0, s, p.nq, p.P, p.probe_b, p.probe_n,
-                     p.bcnt, p.prank);
-  hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, s, p.L, p.bcnt, p.bstart, p.tstart, p.n_tiles);
-  hipLaunchKernelGGL(tile_place_kernel, dim3(grid_for(pairs, 256)), dim3(256), 0, s, p.nq, p.P, p.probe_b, p.probe_n,
-                     p.prank, p.bstart, p.pair_q, p.pair_j);
-  hipLaunchKernelGGL(tile_list_kernel, dim3(grid_for(p.L, 256)), dim3(256), 0, s, p.L, p.bcnt, p.bstart, p.tstart,
-                     p.tile_b, p.tile_p0, p.tile_n);
   // LDS: the table, the top-k state of eight queries and the slot data; the
   // candidate buffer takes what is left of half a CU (two workgroups per CU)
   const long room = (long)kLdsBudget / 2 - (long)kTileTableBytes - (long)topk_lds(kTileQueries, p.k, 0) -
                     5 * 4 * kTileQueries - 16;
-  p.cap = (int)std::min<long>(kCandCap, room / (8 * kTileQueries) / 32 * 32);
+  p.cap = (int)std::min<long>(std::min<long>(kCandCap, 256 - p.k), room / (8 * kTileQueries)) / 32 * 32;
   if (p.cap < 64) return hipErrorInvalidValue;
   const size_t dyn = ((topk_lds(kTileQueries, p.k, p.cap) + 15) & ~size_t(15)) + 5 * 4 * kTileQueries;
   (void)hipFuncSetAttribute((const void*)bucket_tile_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+  hipLaunchKernelGGL(tile_init_kernel, dim3(grid_for(std::max<long>(p.L, p.nq), 256)), dim3(256), 0, s, p.L, p.bcnt,
+                     p.nq, p.gbound);
+  hipLaunchKernelGGL(tile_count_kernel, dim3(grid_for(pairs, 256)), dim3(256), 0, s, p.nq, p.P, 0, p.P, p.probe_b,
+                     p.probe_n, p.bcnt, p.prank);
+  hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, s, p.L, p.bcnt, p.bstart, p.tstart, p.n_tiles);
+  hipLaunchKernelGGL(tile_place_kernel, dim3(grid_for(pairs, 256)), dim3(256), 0, s, p.nq, p.P, 0, p.P, p.probe_b,
+                     p.probe_n, p.prank, p.bstart, p.pair_q, p.pair_j);
+  hipLaunchKernelGGL(tile_list_kernel, dim3(grid_for(p.L, 256)), dim3(256), 0, s, p.L, p.bcnt, p.bstart, p.tstart,
+                     p.tile_b, p.tile_p0, p.tile_n);
   if (scan_start != nullptr) (void)hipEventRecord(scan_start, s);
-  hipLaunchKernelGGL(bucket_tile_kernel, dim3(kBucketScanGrid), dim3(kScanThreads), dyn, s, p);
+  hipLaunchKernelGGL(bucket_tile_kernel, dim3(kBucketScanGrid), dim3(kTileThreads), dyn, s, p);
   if (scan_stop != nullptr) (void)hipEventRecord(scan_stop, s);
   hipLaunchKernelGGL(merge_slots_kernel, dim3(grid_for(p.nq, 1, 65536)), dim3(64), 0, s, p.part_s, p.part_i,
                      p.part_cnt, p.probe_n, p.P, p.nq, p.k, k_out, id_map, out_s, out_i, counts);
