@@ -1560,9 +1560,10 @@ __device__ void topk_merge(const TopkShared& sh, int t, int k, int cap) {
 }
 
 // topk_merge for every query of a tile at once: wavefront t takes query t's
-// list and candidates into registers (at most 256 of them, four per lane)
-// and writes the merged list back by repeated wavefront argmax under
-// (score desc, id asc). One barrier in all; the queries merge in parallel.
+// list and candidates into registers and writes the merged list back: up to
+// 64 entries by one bitonic sort across the lanes, more (at most 256, four
+// per lane) by repeated wavefront argmax, both under (score desc, id asc).
+// One barrier in all; the queries merge in parallel.
 // Needs T <= the wavefronts of the workgroup and k + cap <= 256. Defined
 // after wave_best (below).
 __device__ void topk_merge_all(const TopkShared& sh, int T, int k, int cap);
@@ -2751,7 +2752,34 @@ __device__ void topk_merge_all(const TopkShared& sh, int T, int k, int cap) {
     const int t = w;
     const int c = __builtin_amdgcn_readfirstlane(min(sh.cnt[t], cap));
     const int m = __builtin_amdgcn_readfirstlane(sh.m[t]);
-    if (c > 0) {
+    if (c > 0 && m + c <= 64) {
+      // the usual case (a tile's list starts empty): one bitonic sort across the lanes
+      float* ts = sh.top_s + t * k;
+      uint32_t* ti = sh.top_i + t * k;
+      float s = -INFINITY;
+      uint32_t id = 0xffffffffu;
+      if (lane < m) {
+        s = ts[lane];
+        id = ti[lane];
+      } else if (lane < m + c) {
+        s = sh.cand_s[t * cap + lane - m];
+        id = sh.cand_i[t * cap + lane - m];
+      }
+      wave_sort_desc(s, id, lane);
+      const int m2 = min(k, m + c);
+      if (lane < m2) {
+        ts[lane] = s;
+        ti[lane] = id;
+      }
+      if (lane == m2 - 1 && m2 == k) {
+        sh.thr_s[t] = s;
+        sh.thr_i[t] = id;
+      }
+      if (lane == 0) {
+        sh.m[t] = m2;
+        sh.cnt[t] = 0;
+      }
+    } else if (c > 0) {
       float* ts = sh.top_s + t * k;
       uint32_t* ti = sh.top_i + t * k;
       float es[4];
