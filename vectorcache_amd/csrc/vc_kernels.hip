@@ -3071,10 +3071,14 @@ __global__ __launch_bounds__(kTileThreads, kTileThreads == 1024 ? VC_PAIR_MIN_WA
     }
   };
   int cur_b, cur_p0, cur_n, cur_q, cur_j, nx_b, nx_p0, nx_n;
-  desc(blockIdx.x, cur_b, cur_p0, cur_n);
-  pair(blockIdx.x, cur_p0, cur_n, cur_q, cur_j);
-  desc(blockIdx.x + G, nx_b, nx_p0, nx_n);
-  for (int it = blockIdx.x; it < n_tiles; it += G) {
+  // Workgroups go round-robin to the eight XCDs; each round gives XCD x the
+  // consecutive tiles [x G / 8, (x + 1) G / 8), so the tiles of one bucket
+  // (consecutive) share its L2.
+  const int slot0 = G % 8 == 0 ? (int)(blockIdx.x % 8) * (G / 8) + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+  desc(slot0, cur_b, cur_p0, cur_n);
+  pair(slot0, cur_p0, cur_n, cur_q, cur_j);
+  desc(slot0 + G, nx_b, nx_p0, nx_n);
+  for (int it = slot0; it < n_tiles; it += G) {
     const int b = __builtin_amdgcn_readfirstlane(cur_b);
     const int nt = __builtin_amdgcn_readfirstlane(cur_n);
     const int q_t = cur_q, j_t = cur_j;
