@@ -745,7 +745,8 @@ def run_bucketed(args):
             "learned_probe_table": {"y0": 0.0, "y_step": 0.25, "counts": table.tolist()},
             "seeds": {"centers": 4001, "rows": 4002, "train_val_queries": 4003, "eval_queries": 4004},
         },
-        "fixed": {k2: v for k2, v in res["fixed"].items() if k2 not in ("ids", "scores")},
+        "fixed": {**{k2: v for k2, v in res["fixed"].items() if k2 not in ("ids", "scores")},
+                  "tiles": tile_info.get("fixed"), "algorithmic_bytes": bytes_f},
         "learned": {k2: v for k2, v in res["learned"].items() if k2 not in ("ids", "scores")},
         "learned_over_fixed_qps": res["learned"]["qps"] / res["fixed"]["qps"],
         "eval_recall_from_ranks": {"fixed": rec_fixed_ev, "learned": rec_learned_ev},
