@@ -25,6 +25,15 @@ namespace {
 
 constexpr float kMinNorm = 1e-10f;
 
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 // Correctly rounded sqrt / division (IEEE, as the reference's sqrtf and '/').
 // Note: HIP's __fsqrt_rn maps to the approximate native sqrt unless
 // OCML_BASIC_ROUNDED_OPERATIONS is defined, so it is not used here.
@@ -1593,19 +1602,50 @@ __device__ __forceinline__ void init_topk(TopkShared& sh, unsigned char* base, i
 
 // Bitonic sort of one key per lane across the wavefront into descending
 // (score, then ascending id) order: lane i ends with the i-th best key.
-__device__ __forceinline__ void wave_sort_desc(float& s, uint32_t& id, int lane) {
-#pragma unroll
-  for (int size = 2; size <= 64; size <<= 1) {
-#pragma unroll
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      const float os = __shfl_xor(s, stride);
-      const uint32_t oi = (uint32_t)__shfl_xor((int)id, stride);
-      const bool desc = (lane & size) == 0 || size == 64;
-      const bool lower = (lane & stride) == 0;
-      const bool o_better = better(os, oi, s, id);
-      if ((lower == desc) ? o_better : !o_better) { s = os; id = oi; }
-    }
+// Value of lane ^ M of a wavefront, without LDS: DPP quad permutes for 1 and
+// 2, two DPP moves for 4 (quad reversal = ^3, then the half-row mirror = ^7),
+// a DPP row rotation by 8 (^8 inside each 16-lane row), and for 16 and 32 the
+// gfx950 row swaps of the value with itself (v_permlane16_swap: the first
+// result holds the even rows twice, the second the odd rows; the same for the
+// halves with v_permlane32_swap), the partner's copy picked per lane.
+template <int M>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v, int lane) {
+  const int b = (int)v;
+  if constexpr (M == 1) return (uint32_t)__builtin_amdgcn_mov_dpp(b, 0xB1, 0xF, 0xF, false);
+  else if constexpr (M == 2) return (uint32_t)__builtin_amdgcn_mov_dpp(b, 0x4E, 0xF, 0xF, false);
+  else if constexpr (M == 4)
+    return (uint32_t)__builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(b, 0x1B, 0xF, 0xF, false), 0x141, 0xF, 0xF,
+                                              false);
+  else if constexpr (M == 8) return (uint32_t)__builtin_amdgcn_mov_dpp(b, 0x128, 0xF, 0xF, false);
+  else if constexpr (M == 16) {
+    const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane & 16) ? p[0] : p[1];
+  } else {
+    static_assert(M == 32, "xor distance");
+    const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane & 32) ? p[0] : p[1];
   }
+}
+
+// One compare-exchange stage of a bitonic sort across the wavefront.
+template <int SIZE, int STRIDE>
+__device__ __forceinline__ void bitonic_stage(float& s, uint32_t& id, int lane) {
+  const float os = __uint_as_float(xor_lane<STRIDE>(__float_as_uint(s), lane));
+  const uint32_t oi = xor_lane<STRIDE>(id, lane);
+  const bool desc = (lane & SIZE) == 0 || SIZE == 64;
+  const bool lower = (lane & STRIDE) == 0;
+  const bool o_better = better(os, oi, s, id);
+  if ((lower == desc) ? o_better : !o_better) { s = os; id = oi; }
+}
+
+__device__ __forceinline__ void wave_sort_desc(float& s, uint32_t& id, int lane) {
+  static_for<6>([&](auto lg) {
+    constexpr int L = decltype(lg)::value;
+    static_for<L + 1>([&](auto st) {
+      constexpr int STRIDE = (1 << L) >> decltype(st)::value;
+      bitonic_stage<2 << L, STRIDE>(s, id, lane);
+    });
+  });
 }
 
 // The kw-th largest u among the lanes with `cand` set (at least kw of them),
@@ -2041,14 +2081,6 @@ __device__ __forceinline__ void build_pair_chunk(const uint8_t* __restrict__ sta
 }
 
 // f(integral_constant<int, I>) for I = 0 .. N-1, unrolled at compile time.
-template <typename F, int... I>
-__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
-  (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-  static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
 
 // Byte e of w times 2^SH in one VALU op: the SDWA operand select picks the
 // byte, so the LDS address of a table entry costs one instruction.
@@ -2220,17 +2252,21 @@ struct WaveList {
 };
 
 __device__ __forceinline__ void wave_list_merge_sorted(WaveList& w, float x, uint32_t xi, int lane) {
-  // x, xi: 64 entries sorted descending across the lanes
-  const float ys = __shfl(x, 63 - lane);
-  const uint32_t yi = (uint32_t)__shfl((int)xi, 63 - lane);
+  // x, xi: 64 entries sorted descending across the lanes; reversed (lane ^ 63
+  // = the row mirror ^15, then ^16 and ^32) they form a bitonic sequence with the list
+  auto rev = [&](uint32_t v) {
+    return xor_lane<32>(xor_lane<16>((uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false), lane), lane);
+  };
+  const float ys = __uint_as_float(rev(__float_as_uint(x)));
+  const uint32_t yi = rev(xi);
   if (better(ys, yi, w.s, w.i)) { w.s = ys; w.i = yi; }
-#pragma unroll
-  for (int st = 32; st > 0; st >>= 1) {
-    const float os = __shfl_xor(w.s, st);
-    const uint32_t oi = (uint32_t)__shfl_xor((int)w.i, st);
+  static_for<6>([&](auto si) {
+    constexpr int st = 32 >> decltype(si)::value;
+    const float os = __uint_as_float(xor_lane<st>(__float_as_uint(w.s), lane));
+    const uint32_t oi = xor_lane<st>(w.i, lane);
     const bool ob = better(os, oi, w.s, w.i);
     if (((lane & st) == 0) ? ob : !ob) { w.s = os; w.i = oi; }
-  }
+  });
 }
 
 __device__ __forceinline__ void wave_list_admit(WaveList& w, float sc, uint32_t vid, bool valid, int k, int lane) {
