@@ -3216,16 +3216,20 @@ __global__ __launch_bounds__(kTileThreads, kTileThreads == 1024 ? VC_PAIR_MIN_WA
     }
     __syncthreads();
     topk_merge_all(sh, T, a.k, a.cap);
-    for (int t = 0; t < nt; ++t) {
-      const long slot = lslot[t];
-      const int m = sh.m[t];
-      for (int j = tid; j < m; j += blockDim.x) {
-        a.part_s[slot * a.k + j] = sh.top_s[t * a.k + j];
-        a.part_i[slot * a.k + j] = sh.top_i[t * a.k + j];
-      }
-      if (tid == 0) {
-        a.part_cnt[slot] = m;
-        if (m == a.k) atomicMax(&a.gbound[lq[t]], float_order(sh.top_s[t * a.k + a.k - 1]));
+    {
+      // wavefront t writes query t's list (k <= 64: one entry per lane)
+      const int t = tid >> 6, j = tid & 63;
+      if (t < nt) {
+        const long slot = lslot[t];
+        const int m = sh.m[t];
+        if (j < m) {
+          a.part_s[slot * a.k + j] = sh.top_s[t * a.k + j];
+          a.part_i[slot * a.k + j] = sh.top_i[t * a.k + j];
+        }
+        if (j == 0) {
+          a.part_cnt[slot] = m;
+          if (m == a.k) atomicMax(&a.gbound[lq[t]], float_order(sh.top_s[t * a.k + a.k - 1]));
+        }
       }
     }
     cur_b = nx_b; cur_n = nx_n; cur_q = nx_q; cur_j = nx_j;
