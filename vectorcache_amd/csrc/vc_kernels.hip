@@ -1746,7 +1746,7 @@ __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], cons
         const uint64_t wm = __ballot(want);
         int base = 0;
         if ((threadIdx.x & 63) == 0) base = atomicAdd(&sh.cnt[t], __popcll(wm));
-        base = __shfl(base, 0);
+        base = __builtin_amdgcn_readfirstlane(base);  // lane 0's (the whole wavefront is active here)
         const int slot = base + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(wm >> 32),
                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)wm, 0u));
         const bool store = want && slot < cap;
