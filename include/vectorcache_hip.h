@@ -282,7 +282,11 @@ vc_status vc_bucketed_search(vc_bucketed* index, const float* queries, size_t nq
  * n_probe > 0: every query opens min(n_probe, 512, n_buckets) buckets, or with
  * probe_counts_dev (int32 [nq]) query q opens clamp(counts[q], 1, that bound);
  * n_probe = 0: the learned model when enabled, else params.n_probe.
- * counts_dev (u64 [nq], may be NULL) receives the entries found per row. */
+ * counts_dev (u64 [nq], may be NULL) receives the entries found per row.
+ * Batches that put several queries on a bucket (nq * probe width >= 8 *
+ * n_buckets, k <= 64) are scanned bucket-major (each bucket's codes read once
+ * per tile of up to 8 of its queries), smaller ones query-major; the results
+ * are the same. */
 vc_status vc_bucketed_search_device(vc_bucketed* index, const float* queries_dev, size_t nq, size_t k, size_t n_probe,
                                     const int32_t* probe_counts_dev, float* scores_dev, uint64_t* ids_dev,
                                     uint64_t* counts_dev, size_t* k_out, void* stream);
@@ -520,7 +524,9 @@ vc_status vc_set_profiling(int on);
  * of indexes created later always take the reference's sequential bias_corr
  * chain instead of the certified parallel sum; bit 5 makes landmark searches
  * rotate their queries in a separate kernel instead of inside the query
- * projection kernel; bits 16-31, when not
+ * projection kernel; bit 7 / bit 8 make bucketed searches take the
+ * bucket-major tile scan (tiles of up to 8 queries of one bucket; k <= 64) /
+ * the query-major scan whatever the batch size; bits 16-31, when not
  * zero, set the number of vector ranges each query tile is split into (at
  * most 512) in place of the automatic split. */
 vc_status vc_set_debug(unsigned flags);

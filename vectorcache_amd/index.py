@@ -600,6 +600,8 @@ def set_debug(flags: int):
     bit 3 swaps the byte-pair kernel's vectors-per-thread variant (1 <-> 2);
     bit 4 forces the sequential bias_corr chain in query preparation;
     bit 5 makes landmark searches rotate queries in a separate kernel;
+    bit 7 / bit 8 make bucketed searches take the bucket-major tile scan /
+    the query-major scan whatever the batch (k <= 64 for the tile scan);
     bits 16-31 (when not zero) force the number of vector ranges per query
     tile (at most 512)."""
     check(lib.vc_set_debug(ctypes.c_uint(flags)))
