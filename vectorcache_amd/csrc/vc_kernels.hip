@@ -1669,12 +1669,15 @@ __device__ __attribute__((noinline)) uint32_t wave_kth_largest(uint32_t u, bool 
 // voff (when not null): a per-vector offset added after alpha, the reference's
 // score_offsets of score_prepared_into (src/search/search_avx2.cpp:160-214:
 // fa * alpha, then + offset, each rounded); toff (when voff is null): the
-// same offset per query instead (a tile of queries over one bucket).
+// same offset per query instead (a tile of queries over one bucket). fresh:
+// the lists are empty (the first step of a work item), so the test against
+// their k-th entries is skipped.
 template <int T, int V>
 __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], const long (&vv)[V],
                              const bool (&valid)[V], const float* __restrict__ alpha, long q0, long nq,
                              int k, int cap, const float (*voff)[V] = nullptr,
-                             const float (*av_pre)[V] = nullptr, const float* toff = nullptr) {
+                             const float (*av_pre)[V] = nullptr, const float* toff = nullptr,
+                             bool fresh = false) {
   const int lane = threadIdx.x & 63;
   const int kw = min(k, 64);
 #pragma unroll
@@ -1696,7 +1699,8 @@ __device__ void admit_scores(const TopkShared& sh, const float (&fa)[V][T], cons
       s[t] = valid[j] ? fa[j][t] * av + (voff != nullptr ? (*voff)[j] : toff != nullptr ? toff[t] : 0.f)
                       : -INFINITY;
       // below the bound: worse than k vectors already offered (see below)
-      const bool cand = valid[j] && q0 + t < nq && better(s[t], vid, sh.thr_s[t], sh.thr_i[t]) &&
+      // fresh: every list is still empty (its k-th is -inf), no need to read it
+      const bool cand = valid[j] && q0 + t < nq && (fresh || better(s[t], vid, sh.thr_s[t], sh.thr_i[t])) &&
                         !(s[t] < order_float(sh.bound[t]));
       candm |= (cand ? 1u : 0u) << t;
     }
@@ -3212,7 +3216,7 @@ __global__ __launch_bounds__(kTileThreads, kTileThreads == 1024 ? VC_PAIR_MIN_WA
       const long vv[1] = {v};
       const bool vl[1] = {valid};
       const float avp[1] = {av};
-      admit_scores<T, 1>(sh, fa, vv, vl, a.alpha, 0, nt, a.k, a.cap, nullptr, &avp, loff);
+      admit_scores<T, 1>(sh, fa, vv, vl, a.alpha, 0, nt, a.k, a.cap, nullptr, &avp, loff, vp0 == 0);
     }
     __syncthreads();
     topk_merge_all(sh, T, a.k, a.cap);
