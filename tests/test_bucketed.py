@@ -172,6 +172,30 @@ def test_tile_scan_equals_query_scan(vc, oracle, bits, tq):
     _same_topk(ts[:48], ti[:48].astype(np.uint64), ws, wi)
 
 
+def test_tile_scan_buckets_above_one_pass(vc, oracle):
+    """Buckets of more than 1024 vectors take several passes of the tile scan
+    over the bucket (the lists carried across the passes): the same lists as
+    the query-major scan, and as the oracle's chain for 16 queries."""
+    import torch
+
+    from vectorcache_amd import synthetic
+
+    dim, bits, n = 64, 4, 9000
+    x = synthetic.zipf_mixture(n, dim, 6, 1.1, 0.3, 51, 52)
+    q = synthetic.zipf_mixture(300, dim, 6, 1.1, 0.3, 51, 53)
+    ix = _bucketed(x, dim, bits, energy_soft=1500, max_bucket_size=2600)
+    st = ix.export()
+    assert st["bucket_sizes"].max() > 2048  # three passes of 1024
+    L = ix.n_buckets()
+    qd = torch.from_numpy(q).cuda()
+    for k in (1, 10, 64):
+        ts, ti, _ = _tile_vs_query(vc, ix, qd, k, n_probe=min(3, L))
+    pb = ix.route_device(qd[:16], min(3, L))[0].cpu().numpy()
+    ws, wi = oracle.bucketed_search(q[:16], 64, bits, st["bucket_sizes"], st["flat"], st["alpha"], st["pos_ids"],
+                                    st["centroids"], pb, np.full(16, min(3, L), np.int32))
+    _same_topk(ts[:16], ti[:16].astype(np.uint64), ws, wi)
+
+
 def test_bucketed_direct_codes_all_buckets_equal_flat_search(vc, oracle):
     """residual = 0: codes of the normalized rows; opening every bucket gives
     the flat index's search of the same rows, ties included."""
