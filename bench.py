@@ -417,7 +417,7 @@ def recall_at(ranks, n_probe):
     return float((ranks < n.view(-1, 1)).float().mean())
 
 
-PROBE_GRID = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 160, 192, 224, 256)
+PROBE_GRID = tuple(range(1, 17)) + tuple(range(20, 65, 4)) + tuple(range(72, 257, 8))
 
 
 def fit_probe_table(y, ranks, csz, target, y0=0.0, y_step=0.25, n_bins=40, grid=PROBE_GRID):
@@ -425,7 +425,8 @@ def fit_probe_table(y, ranks, csz, target, y0=0.0, y_step=0.25, n_bins=40, grid=
     y_step), clamped) that reach a mean recall of `target` with the fewest
     scanned vectors on these queries: each step moves the bin whose next
     larger count (any later grid point) gains the most recall per scanned
-    vector. ranks: routing rank of each judge row's bucket; csz: cumulative
+    vector; the last move is the cheapest one that reaches the target.
+    ranks: routing rank of each judge row's bucket; csz: cumulative
     vectors of each query's routing list."""
     import torch
 
@@ -440,16 +441,21 @@ def fit_probe_table(y, ranks, csz, target, y0=0.0, y_step=0.25, n_bins=40, grid=
     n = ranks.shape[0]
     at = np.zeros(n_bins, np.int64)
     total = R[np.arange(n_bins), at].sum()
+    cols = np.arange(len(gl))
     while total < target * n:
-        best, pick = 0.0, None
-        for bi in range(n_bins):
-            for j in range(at[bi] + 1, len(gl)):
-                dr = R[bi, j] - R[bi, at[bi]]
-                dc = C[bi, j] - C[bi, at[bi]]
-                if dr > 0 and dc > 0 and dr / dc > best:
-                    best, pick = dr / dc, (bi, j)
-        if pick is None:
+        dr = R - R[np.arange(n_bins), at][:, None]
+        dc = C - C[np.arange(n_bins), at][:, None]
+        ok = (cols[None, :] > at[:, None]) & (dr > 0) & (dc > 0)
+        if not ok.any():
             break
+        reach = ok & (total + dr >= target * n)
+        if reach.any():
+            # the last move: the cheapest one that reaches the target (not the best ratio, which can overshoot)
+            flat = int(np.argmin(np.where(reach, dc, np.inf)))
+        else:
+            ratio = np.where(ok, dr / np.where(dc > 0, dc, 1.0), -1.0)
+            flat = int(np.argmax(ratio))  # first of the best (bin, count), as the scan in bin, count order
+        pick = divmod(flat, len(gl))
         total += R[pick[0], pick[1]] - R[pick[0], at[pick[0]]]
         at[pick[0]] = pick[1]
     return gl[at].astype(np.int32)
@@ -603,7 +609,9 @@ def run_bucketed(args):
     # the learned policy: the model's prediction picks a bin, each bin's probe count is fitted on validation
     csz_all = torch.cat([torch.cumsum(sizes[ix.route_device(q_all[s:e], min(256, L))[0].long()], dim=1)
                          for s, e in chunked(q_all.shape[0], 10_000)])
-    table = fit_probe_table(y[va], ranks[va], csz_all[va], target)
+    # the learned table is fitted to the fixed policy's validation recall (at least the target), so that the
+    # two are compared at the same recall
+    table = fit_probe_table(y[va], ranks[va], csz_all[va], max(target, fixed_curve[n_fixed]))
     ix.probe_set_table(table, 0.0, 0.25)
     ix.probe_enable(True, 0.0)
     cnt_ev = table_counts(y[ev], table, valid[ev])
