@@ -34,7 +34,7 @@ namespace {
 
 constexpr int kRouteMaxP = 512;
 constexpr size_t kEncodeChunk = size_t(1) << 20;  // residual rows per encode pass
-constexpr size_t kRouteScoreBytes = size_t(1) << 30;  // routing score matrix per query chunk
+constexpr size_t kRouteScoreBytes = size_t(2) << 30;  // routing score matrix per query chunk
 
 int pad32(size_t d) { return int((d + 31) / 32 * 32); }
 
