@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void gemm_argmax_kernel(const __bf16* __restri
 constexpr int kTopThreads = 256;
 constexpr int kTopMaxP = 512;
 constexpr int kTopCand = 2048;   // candidate buffer
-constexpr int kTopSamples = 1024;
+constexpr int kTopSamples = 512;
 
 __device__ __forceinline__ void bitonic_desc(float* s, int* c, int size, int tid) {
   for (int kk = 2; kk <= size; kk <<= 1)
